@@ -1,0 +1,145 @@
+/*
+ * t3ns_amd: MI355X-native effective-Hamiltonian matvec, Davidson and
+ * operator renormalisation for block-sparse three-legged tree tensor
+ * networks.  C ABI: plain pointers and sizes, no C++ or torch types.
+ *
+ * Drop-in boundary.  The reference's Heff path is
+ *   init_Heffdata(struct Heffdata*, const struct rOperators*, const struct siteTensor*)
+ *                                                  include/Heff.h:166-167
+ *   matvecT3NS(const double* vec, double* result, void* vdata)   include/Heff.h:177
+ *   make_diagonal(const struct Heffdata*)                        include/Heff.h:185
+ *   destroy_Heffdata(struct Heffdata*)                           include/Heff.h:192
+ *   davidson(...)                                                include/davidson.h:56-60
+ * and reads the process globals bookie / netw / ham.  A shim in the
+ * reference flattens its structs and globals into the descriptors below and
+ * calls t3ns_heff_init / t3ns_heff_matvec / t3ns_heff_diagonal /
+ * t3ns_heff_destroy / t3ns_davidson.
+ *
+ * All functions return 0 on success and a negative code on error; the
+ * message of the last error of the calling thread is t3ns_last_error().
+ */
+#ifndef T3NS_AMD_H
+#define T3NS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define T3NS_MAX_SYMMETRIES 5
+
+enum t3ns_status {
+    T3NS_OK = 0,
+    T3NS_ERR_INVALID = -1,   /* bad argument or inconsistent input */
+    T3NS_ERR_DEVICE = -2,    /* HIP runtime error or no device */
+    T3NS_ERR_NOMEM = -3,     /* device allocation failed */
+    T3NS_ERR_SOLVER = -4     /* eigensolver failure (dsyev in the reference) */
+};
+
+enum t3ns_precision {
+    T3NS_FP64 = 0, /* plain path: FP64 storage and arithmetic */
+    T3NS_FP32 = 1  /* FP32 storage and arithmetic for the bulk matvec */
+};
+
+/* Symmetry sectors of one bond: struct symsecs, include/symsecs.h:32-54. */
+typedef struct {
+    int bond;             /* reference bond number (-1 for the MPO bond) */
+    int nrSecs;
+    const int* irreps;    /* [nrSecs][T3NS_MAX_SYMMETRIES] */
+    const int* dims;      /* [nrSecs] */
+} t3ns_symsecs_desc;
+
+/* Renormalised operators: struct rOperators, include/rOperators.h:91-123.
+ * Offsets are 64-bit element offsets (the reference uses int). */
+typedef struct {
+    int bond;
+    int is_left;
+    int P_operator;
+    int nrhss;
+    const int* begin_blocks_of_hss;   /* [nrhss+1] */
+    const int64_t* qnumbers;          /* [nblocks * (P_operator ? 3 : 1)] */
+    int nrops;
+    const int* hss_of_ops;            /* [nrops] */
+    const int64_t* const* beginblock; /* [nrops] arrays of nblocks(op)+1 offsets */
+    const double* const* tel;         /* [nrops] host element arrays */
+} t3ns_rops_desc;
+
+/* The multisite tensor being optimised: struct siteTensor,
+ * include/siteTensor.h:73-95 (elements are passed to the matvec). */
+typedef struct {
+    int nrsites;
+    int sites[4];
+    int site_is_phys[4];
+    int nrblocks;
+    const int64_t* qnumbers;   /* [nrblocks * nrsites] */
+    const int64_t* beginblock; /* [nrblocks+1] */
+} t3ns_site_desc;
+
+/* Everything init_Heffdata reads from the globals bookie / netw / ham and
+ * from fetch_merge (src/Heff.c:1167-1221, src/instructions.c:188-222). */
+typedef struct {
+    int nrSyms;
+    int sgs[T3NS_MAX_SYMMETRIES];        /* enum symmetrygroup values */
+    t3ns_symsecs_desc mpo;               /* MPO bond sectors */
+    const int* hermitian;                /* [mpo.nrSecs]: hermitian_symsec */
+    const unsigned char* singlet;        /* [H^3]: MPO_couples_to_singlet({a,b,c}) at a+b*H+c*H*H */
+    t3ns_symsecs_desc symarr[4][3];      /* sectors of the three bonds of each site */
+    int rops_on_site[3];                 /* site index each operator attaches to */
+    int merge_step;                      /* 2 (DMRG) or 3 (T3NS) */
+    int merge_nr;                        /* raw merge instructions as the model emits them */
+    const int* merge_instr;              /* [merge_nr * 3] */
+    const double* merge_pref;            /* [merge_nr] */
+} t3ns_heff_desc;
+
+typedef struct t3ns_heff t3ns_heff;
+
+const char* t3ns_last_error(void);
+
+/* Number of visible devices (0 if none). */
+int t3ns_device_count(void);
+
+/* init_Heffdata (include/Heff.h:166): builds the dispatch plan on the host,
+ * uploads plan tables and operator blocks to HBM.  ops[2] is ignored for a
+ * two-site (DMRG) object.  precision: enum t3ns_precision. */
+int t3ns_heff_init(t3ns_heff** h, const t3ns_heff_desc* desc, const t3ns_rops_desc* ops,
+                   const t3ns_site_desc* site, int precision);
+
+/* matvecT3NS (include/Heff.h:177): result = Heff vec, host FP64 arrays of
+ * t3ns_heff_size() elements; result is overwritten. */
+int t3ns_heff_matvec(t3ns_heff* h, const double* vec, double* result);
+
+/* Same on device-resident FP64 arrays, enqueued on `stream` (hipStream_t,
+ * may be NULL for the default stream). */
+int t3ns_heff_matvec_device(t3ns_heff* h, const double* dvec, double* dresult, void* stream);
+
+/* make_diagonal (include/Heff.h:185): diag(Heff) into a host array. */
+int t3ns_heff_diagonal(t3ns_heff* h, double* diag);
+
+/* destroy_Heffdata (include/Heff.h:192). */
+int t3ns_heff_destroy(t3ns_heff* h);
+
+/* Plan statistics: vector length, work-list sizes and the algorithmic flops
+ * of one matvec (sum of 2*M*N*K*L over the GEMMs the reference executes). */
+int64_t t3ns_heff_size(const t3ns_heff* h);
+int t3ns_heff_stats(const t3ns_heff* h, int64_t* nblocks, int64_t* npairs, int64_t* nterms,
+                    double* flops);
+
+/* The dispatch table the reference builds in its first matvec
+ * (struct secondrun, include/Heff.h:36-51; src/Heff.c:815-914), ordered by
+ * output block, then pair, then MPO entry.  Per entry 7 ints (newsb, oldsb,
+ * bestorder, mpo_group, sbops[0..2]; sbops[2] = -1 for DMRG) and the symmetry
+ * prefactor.  Returns the number of entries; arrays may be NULL to count. */
+int64_t t3ns_heff_plan_table(const t3ns_heff* h, int* ints, double* prefs);
+
+/* The same table built on the host only (no device needed), plus the
+ * algorithmic flops of one matvec.  Returns the entry count or a negative
+ * status. */
+int64_t t3ns_plan_table(const t3ns_heff_desc* desc, const t3ns_rops_desc* ops,
+                        const t3ns_site_desc* site, int* ints, double* prefs, double* flops);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* T3NS_AMD_H */

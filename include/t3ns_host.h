@@ -1,0 +1,54 @@
+/*
+ * t3ns_amd host layer: solve contexts (the reference's globals made
+ * explicit) and step objects that produce the inputs of the Heff boundary in
+ * include/t3ns_amd.h.  These replace the calling pattern of the reference's
+ * tests (tests/test1.c.in:31-54: make_network, readinteraction,
+ * preparebookkeeper) and the per-step glue of optimize_siteTensor
+ * (src/optimize_network.c:188-226).
+ */
+#ifndef T3NS_HOST_H
+#define T3NS_HOST_H
+
+#include <stdint.h>
+
+#include "t3ns_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct t3ns_ctx t3ns_ctx;
+typedef struct t3ns_step t3ns_step;
+
+/* Solve context.  netw_text: contents of a .netw network file
+ * (src/network.c:207-354).  model: "NN_HUBBARD" with params {t, U}
+ * (src/hamiltonian_nn_hubbard.c:75-84).  sgs/ts: symmetry groups and target
+ * irreps (enum symmetrygroup order, include/symmetries.h:34); Z2 is added
+ * when absent (src/bookkeeper.c:495-513).  max_dim and minocc as in
+ * preparebookkeeper(NULL, max_dim, 1, minocc, NULL) (src/bookkeeper.c:725). */
+int t3ns_ctx_create(t3ns_ctx** c, const char* netw_text, const char* model, const double* params,
+                    int nparams, const int* sgs, const int* ts, int nsym, int max_dim, int minocc);
+int t3ns_ctx_destroy(t3ns_ctx* c);
+/* nr_bonds, sites, psites of the network; nrSyms and sgs after Z2 insertion. */
+int t3ns_ctx_info(const t3ns_ctx* c, int* nr_bonds, int* sites, int* psites, int* nrsyms,
+                  int* sgs);
+/* Sector count and dims of bond `bond` (reference numbering). */
+int t3ns_ctx_bond(const t3ns_ctx* c, int bond, int* nrsecs, int* totaldims, int* dims_out,
+                  int dims_cap);
+
+/* A multisite optimisation object over `sites` (2 physical sites, or a
+ * branching site with up to three physical neighbours) whose renormalised
+ * operators hold deterministic pseudo-random blocks (seed): the Heff inputs
+ * of one step without a preceding sweep.  Used by parity tests. */
+int t3ns_step_random(t3ns_step** s, t3ns_ctx* c, const int* sites, int nsites, uint64_t seed);
+int t3ns_step_destroy(t3ns_step* s);
+int64_t t3ns_step_size(const t3ns_step* s);
+/* Fills the Heff boundary descriptors; pointers stay valid while s lives. */
+int t3ns_step_heff_desc(t3ns_step* s, t3ns_heff_desc* desc, t3ns_rops_desc* ops,
+                        t3ns_site_desc* site);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* T3NS_HOST_H */

@@ -1,0 +1,113 @@
+"""ORACLE -- test infrastructure, not product code.
+
+ctypes binding of oracle/liboracle.so (the plain-C restatement of the
+reference's Heff path, see oracle.h) and of oracle/_ref/libref_sym.so (the
+reference's own symmetry sources compiled unchanged).  Only tests/, the smoke
+check and bench.py's CPU-baseline leg import this package, as the checker.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "liboracle.so")
+REFPATH = os.path.join(_HERE, "_ref", "libref_sym.so")
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError("oracle not built: run oracle/build.py")
+        from t3ns_amd._lib import HeffDesc, RopsDesc, SiteDesc
+        L = C.CDLL(LIBPATH)
+        P = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        i64 = C.POINTER(C.c_int64)
+        sig = {
+            "oracle_wigner6j": (C.c_double, [C.c_int] * 6),
+            "oracle_wigner9j": (C.c_double, [C.c_int] * 9),
+            "oracle_prefactor_add_P_operator": (C.c_double, [ip, C.c_int, ip, C.c_int]),
+            "oracle_prefactor_combine_MPOs": (C.c_double, [ip, ip, ip, C.c_int, C.c_int, C.c_int]),
+            "oracle_prefactor_pAppend": (C.c_double, [ip, C.c_int, ip, C.c_int]),
+            "oracle_prefactor_adjoint": (C.c_double, [ip, C.c_char, ip, C.c_int]),
+            "oracle_prefactor_pUpdate": (C.c_double, [ip, C.c_int, ip, C.c_int]),
+            "oracle_prefactor_mirror_coupling": (C.c_double, [ip, ip, C.c_int]),
+            "oracle_prefactor_bUpdate": (C.c_double, [ip, C.c_int, ip, C.c_int]),
+            "oracle_dgemm_batched": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                            C.c_double, dp, C.c_int, C.c_int64, dp, C.c_int,
+                                            C.c_int64, C.c_double, dp, C.c_int, C.c_int64]),
+            "oracle_heff_init": (C.c_int, [C.POINTER(P), C.POINTER(HeffDesc), C.POINTER(RopsDesc),
+                                           C.POINTER(SiteDesc)]),
+            "oracle_heff_size": (C.c_int64, [P]),
+            "oracle_heff_matvec": (None, [P, dp, dp]),
+            "oracle_heff_diagonal": (None, [P, dp]),
+            "oracle_heff_plan_stats": (None, [P, i64, i64, i64, dp]),
+            "oracle_heff_plan_table": (C.c_int64, [P, ip, dp]),
+            "oracle_heff_destroy": (None, [P]),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(L, n)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def ref():
+    """The reference's compiled symmetry library, or None when not built."""
+    global _ref
+    if _ref is None and os.path.exists(REFPATH):
+        _ref = C.CDLL(REFPATH)
+    return _ref
+
+
+class OracleHeff:
+    def __init__(self, desc, ops, site):
+        h = C.c_void_p()
+        rc = lib().oracle_heff_init(C.byref(h), C.byref(desc), ops, C.byref(site))
+        if rc != 0:
+            raise RuntimeError("oracle_heff_init failed")
+        self._h = h
+        self.size = int(lib().oracle_heff_size(h))
+
+    def matvec(self, vec):
+        vec = np.ascontiguousarray(vec, dtype=np.float64)
+        out = np.empty(self.size)
+        lib().oracle_heff_matvec(self._h, vec.ctypes.data_as(C.POINTER(C.c_double)),
+                                 out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def diagonal(self):
+        out = np.empty(self.size)
+        lib().oracle_heff_diagonal(self._h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def plan_stats(self):
+        a, b, c, d = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double()
+        lib().oracle_heff_plan_stats(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        return {"npairs": a.value, "nentries": b.value, "nterms": c.value, "flops": d.value}
+
+    def plan_table(self):
+        n = lib().oracle_heff_plan_table(self._h, None, None)
+        ints = np.zeros((max(n, 1), 7), dtype=np.int32)
+        prefs = np.zeros(max(n, 1))
+        lib().oracle_heff_plan_table(self._h, ints.ctypes.data_as(C.POINTER(C.c_int)),
+                                     prefs.ctypes.data_as(C.POINTER(C.c_double)))
+        return ints[:n], prefs[:n]
+
+    def close(self):
+        if self._h:
+            lib().oracle_heff_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

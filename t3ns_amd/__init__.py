@@ -1,0 +1,7 @@
+"""t3ns_amd: MI355X-native effective-Hamiltonian matvec, Davidson and
+operator renormalisation for block-sparse three-legged tree tensor networks
+(the hot path of the T3NS code), behind the C ABI in include/t3ns_amd.h."""
+from ._lib import FP32, FP64, lib  # noqa: F401
+from .heff import Context, Heff, Step, plan_table  # noqa: F401
+
+__all__ = ["Context", "Step", "Heff", "plan_table", "FP64", "FP32", "lib"]

@@ -1,0 +1,111 @@
+"""ctypes binding of libt3ns_amd.so (include/t3ns_amd.h, include/t3ns_host.h).
+
+The shared library is built in-tree by t3ns_amd/build.py.  Loading fails
+loudly when it is missing: there is no Python fallback for any kernel.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libt3ns_amd.so")
+
+MAX_SYMMETRIES = 5
+FP64, FP32 = 0, 1
+ERRORS = {0: "ok", -1: "invalid argument", -2: "device error", -3: "out of memory", -4: "solver failure"}
+
+# symmetry group enum (include/symmetries.h:34)
+Z2, U1, SU2, C1, Ci, C2, Cs, D2, C2v, C2h, D2h = range(11)
+SYMGROUPS = {"Z2": Z2, "U1": U1, "SU2": SU2, "C1": C1, "Ci": Ci, "C2": C2, "Cs": Cs,
+             "D2": D2, "C2v": C2v, "C2h": C2h, "D2h": D2h}
+
+c_int_p = C.POINTER(C.c_int)
+c_i64_p = C.POINTER(C.c_int64)
+c_dbl_p = C.POINTER(C.c_double)
+
+
+class SymsecsDesc(C.Structure):
+    _fields_ = [("bond", C.c_int), ("nrSecs", C.c_int), ("irreps", c_int_p), ("dims", c_int_p)]
+
+
+class RopsDesc(C.Structure):
+    _fields_ = [("bond", C.c_int), ("is_left", C.c_int), ("P_operator", C.c_int),
+                ("nrhss", C.c_int), ("begin_blocks_of_hss", c_int_p), ("qnumbers", c_i64_p),
+                ("nrops", C.c_int), ("hss_of_ops", c_int_p),
+                ("beginblock", C.POINTER(c_i64_p)), ("tel", C.POINTER(c_dbl_p))]
+
+
+class SiteDesc(C.Structure):
+    _fields_ = [("nrsites", C.c_int), ("sites", C.c_int * 4), ("site_is_phys", C.c_int * 4),
+                ("nrblocks", C.c_int), ("qnumbers", c_i64_p), ("beginblock", c_i64_p)]
+
+
+class HeffDesc(C.Structure):
+    _fields_ = [("nrSyms", C.c_int), ("sgs", C.c_int * MAX_SYMMETRIES), ("mpo", SymsecsDesc),
+                ("hermitian", c_int_p), ("singlet", C.POINTER(C.c_ubyte)),
+                ("symarr", (SymsecsDesc * 3) * 4), ("rops_on_site", C.c_int * 3),
+                ("merge_step", C.c_int), ("merge_nr", C.c_int), ("merge_instr", c_int_p),
+                ("merge_pref", c_dbl_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads the in-tree library once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError("libt3ns_amd.so is not built (run t3ns_amd/build.py): " + LIBPATH)
+    L = C.CDLL(LIBPATH)
+    P = C.c_void_p
+    sig = {
+        "t3ns_last_error": (C.c_char_p, []),
+        "t3ns_device_count": (C.c_int, []),
+        "t3ns_heff_init": (C.c_int, [C.POINTER(P), C.POINTER(HeffDesc), C.POINTER(RopsDesc),
+                                     C.POINTER(SiteDesc), C.c_int]),
+        "t3ns_heff_matvec": (C.c_int, [P, c_dbl_p, c_dbl_p]),
+        "t3ns_heff_matvec_device": (C.c_int, [P, P, P, P]),
+        "t3ns_heff_diagonal": (C.c_int, [P, c_dbl_p]),
+        "t3ns_heff_destroy": (C.c_int, [P]),
+        "t3ns_heff_size": (C.c_int64, [P]),
+        "t3ns_heff_stats": (C.c_int, [P, c_i64_p, c_i64_p, c_i64_p, c_dbl_p]),
+        "t3ns_heff_plan_table": (C.c_int64, [P, c_int_p, c_dbl_p]),
+        "t3ns_plan_table": (C.c_int64, [C.POINTER(HeffDesc), C.POINTER(RopsDesc),
+                                        C.POINTER(SiteDesc), c_int_p, c_dbl_p, c_dbl_p]),
+        "t3ns_ctx_create": (C.c_int, [C.POINTER(P), C.c_char_p, C.c_char_p, c_dbl_p, C.c_int,
+                                      c_int_p, c_int_p, C.c_int, C.c_int, C.c_int]),
+        "t3ns_ctx_destroy": (C.c_int, [P]),
+        "t3ns_ctx_info": (C.c_int, [P, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p]),
+        "t3ns_ctx_bond": (C.c_int, [P, C.c_int, c_int_p, c_int_p, c_int_p, C.c_int]),
+        "t3ns_step_random": (C.c_int, [C.POINTER(P), P, c_int_p, C.c_int, C.c_uint64]),
+        "t3ns_step_destroy": (C.c_int, [P]),
+        "t3ns_step_size": (C.c_int64, [P]),
+        "t3ns_step_heff_desc": (C.c_int, [P, C.POINTER(HeffDesc), C.POINTER(RopsDesc),
+                                          C.POINTER(SiteDesc)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().t3ns_last_error().decode(errors="replace")
+        raise RuntimeError("%s failed (%s): %s" % (what, ERRORS.get(rc, rc), msg))
+    return rc
+
+
+def exported_symbols():
+    """Names of the C-ABI entry points declared in include/*.h."""
+    import re
+    names = []
+    inc = os.path.join(os.path.dirname(_HERE), "include")
+    for f in sorted(os.listdir(inc)):
+        if f.endswith(".h"):
+            txt = open(os.path.join(inc, f)).read()
+            names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*(t3ns_[a-z0-9_]+)\s*\(", txt, re.M)
+    return sorted(set(names))

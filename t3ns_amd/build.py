@@ -1,0 +1,70 @@
+"""Builds libt3ns_amd.so in-tree: host C++ with g++, HIP sources with hipcc
+for gfx950, linked by hipcc.  Always recompiles (parallel over files)."""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libt3ns_amd.so")
+ARCH = os.environ.get("T3NS_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+CXXFLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function"]
+# Packed FP32 ops disabled for all device code.
+HIPFLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def sources():
+    host, dev = [], []
+    for root, _, files in os.walk(CSRC):
+        for f in sorted(files):
+            p = os.path.join(root, f)
+            if f.endswith(".hip"):
+                dev.append(p)
+            elif f.endswith(".cpp"):
+                # files that include HIP runtime headers go through hipcc
+                with open(p) as fh:
+                    txt = fh.read()
+                (dev if "hip/hip_runtime" in txt or "dev/" in txt else host).append(p)
+    return host, dev
+
+
+def compile_one(src, is_hip):
+    rel = os.path.relpath(src, CSRC).replace(os.sep, "_")
+    obj = os.path.join(OBJ, rel + ".o")
+    if is_hip:
+        cmd = [HIPCC] + CXXFLAGS + HIPFLAGS + ["-x", "hip" if src.endswith(".hip") else "c++", "-c", src, "-o", obj]
+        if not src.endswith(".hip"):
+            cmd = [HIPCC] + CXXFLAGS + ["-c", src, "-o", obj]
+    else:
+        cmd = ["g++"] + CXXFLAGS + ["-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    if r.stderr.strip():
+        sys.stderr.write(r.stderr)
+    return obj
+
+
+def build(verbose=True):
+    os.makedirs(OBJ, exist_ok=True)
+    host, dev = sources()
+    jobs = [(s, False) for s in host] + [(s, True) for s in dev]
+    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
+        objs = list(ex.map(lambda a: compile_one(*a), jobs))
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", LIB + ".tmp"] + objs + [
+        "-Wl,-rpath,/opt/rocm/lib", "-lamdhip64"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
+    os.replace(LIB + ".tmp", LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()

@@ -1,0 +1,284 @@
+// C ABI of t3ns_amd (include/t3ns_amd.h, include/t3ns_host.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/t3ns_amd.h"
+#include "../../include/t3ns_host.h"
+#include "dev/heff_dev.hpp"
+#include "host/context.hpp"
+#include "host/heff_plan.hpp"
+#include "host/tensors.hpp"
+
+using namespace t3;
+
+namespace {
+thread_local std::string g_last_error;
+}
+namespace t3 {
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}
+namespace {
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return T3NS_OK;
+    } catch (const std::bad_alloc& e) {
+        return fail(T3NS_ERR_NOMEM, e.what());
+    } catch (const std::invalid_argument& e) {
+        return fail(T3NS_ERR_INVALID, e.what());
+    } catch (const std::runtime_error& e) {
+        const std::string m = e.what();
+        return fail(m.rfind("HIP error", 0) == 0 ? T3NS_ERR_DEVICE : T3NS_ERR_INVALID, m);
+    } catch (...) {
+        return fail(T3NS_ERR_INVALID, "unknown error");
+    }
+}
+
+Symsecs symsecs_from(const t3ns_symsecs_desc& d) {
+    Symsecs s;
+    s.bond = d.bond;
+    s.nrSecs = d.nrSecs;
+    s.irreps.resize(d.nrSecs);
+    s.dims.resize(d.nrSecs);
+    s.fcidims.assign(d.nrSecs, 1.0);
+    s.totaldims = 0;
+    for (int i = 0; i < d.nrSecs; ++i) {
+        for (int k = 0; k < MAX_SYMMETRIES; ++k) s.irreps[i][k] = d.irreps[i * T3NS_MAX_SYMMETRIES + k];
+        s.dims[i] = d.dims[i];
+        s.totaldims += d.dims[i];
+    }
+    return s;
+}
+
+ROperators rops_from(const t3ns_rops_desc& d) {
+    ROperators r;
+    r.bond = d.bond;
+    r.is_left = d.is_left;
+    r.P_operator = d.P_operator;
+    r.nrhss = d.nrhss;
+    if (d.nrhss <= 0 || !d.begin_blocks_of_hss) throw std::invalid_argument("rops: bad nrhss");
+    r.begin_blocks_of_hss.assign(d.begin_blocks_of_hss, d.begin_blocks_of_hss + d.nrhss + 1);
+    const int64_t nq = (int64_t)r.begin_blocks_of_hss[d.nrhss] * (d.P_operator ? 3 : 1);
+    r.qnumbers.assign(d.qnumbers, d.qnumbers + nq);
+    r.nrops = d.nrops;
+    r.hss_of_ops.assign(d.hss_of_ops, d.hss_of_ops + d.nrops);
+    r.beginblock.resize(d.nrops);
+    for (int o = 0; o < d.nrops; ++o) {
+        const int h = r.hss_of_ops[o];
+        if (h < 0 || h >= r.nrhss) throw std::invalid_argument("rops: bad hss_of_ops");
+        const int nb = r.nblocks_hss(h);
+        r.beginblock[o].assign(d.beginblock[o], d.beginblock[o] + nb + 1);
+        const int64_t b0 = r.beginblock[o][0];
+        for (auto& v : r.beginblock[o]) v -= b0;
+    }
+    r.finalize_offsets();
+    r.tel.resize(r.total_size());
+    for (int o = 0; o < d.nrops; ++o) {
+        const int64_t n = r.beginblock[o].back();
+        if (n) std::memcpy(r.tel.data() + r.op_offset[o], d.tel[o] + d.beginblock[o][0], n * sizeof(double));
+    }
+    return r;
+}
+}  // namespace
+
+// Host copy of the boundary descriptors in the internal representation.
+struct HeffHost {
+    Symsecs symarr[4][3];
+    Symsecs mpo;
+    ROperators ops[3];
+    SiteTensor site;
+    InstructionSet iset;
+    HeffInput in;
+};
+
+namespace {
+void convert_heff_input(HeffHost* h, const t3ns_heff_desc* d, const t3ns_rops_desc* ops,
+                        const t3ns_site_desc* site) {
+    HeffInput& in = h->in;
+    in.sym.nrSyms = d->nrSyms;
+    for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) in.sym.sgs[i] = d->sgs[i];
+    h->mpo = symsecs_from(d->mpo);
+    in.mpo = &h->mpo;
+    const int H = h->mpo.nrSecs;
+    in.hermitian.assign(d->hermitian, d->hermitian + H);
+    in.singlet.assign(d->singlet, d->singlet + (size_t)H * H * H);
+    h->site.nrsites = site->nrsites;
+    if (site->nrsites < 1 || site->nrsites > 4) throw std::invalid_argument("bad nrsites");
+    for (int i = 0; i < 4; ++i) h->site.sites[i] = site->sites[i];
+    h->site.nrblocks = site->nrblocks;
+    h->site.qnumbers.assign(site->qnumbers, site->qnumbers + (size_t)site->nrblocks * site->nrsites);
+    h->site.beginblock.assign(site->beginblock, site->beginblock + site->nrblocks + 1);
+    in.site = &h->site;
+    for (int i = 0; i < site->nrsites; ++i) {
+        in.site_is_phys[i] = site->site_is_phys[i] != 0;
+        for (int k = 0; k < 3; ++k) {
+            h->symarr[i][k] = symsecs_from(d->symarr[i][k]);
+            in.symarr[i][k] = &h->symarr[i][k];
+        }
+    }
+    const bool isdmrg = (site->nrsites == 1 && site->site_is_phys[0]) ||
+                        (site->nrsites == 2 && site->site_is_phys[0] && site->site_is_phys[1]);
+    for (int k = 0; k < (isdmrg ? 2 : 3); ++k) {
+        h->ops[k] = rops_from(ops[k]);
+        in.ops[k] = &h->ops[k];
+        in.rops_on_site[k] = d->rops_on_site[k];
+    }
+    if (isdmrg) {
+        h->ops[2] = ROperators();
+        h->ops[2].P_operator = 0;
+        in.ops[2] = &h->ops[2];
+    }
+    // merge instruction set, sorted and grouped as fetch_merge does
+    h->iset.step = d->merge_step;
+    for (int i = 0; i < d->merge_nr; ++i) {
+        Instruction ins;
+        for (int k = 0; k < 3; ++k) ins.instr[k] = d->merge_instr[3 * i + k];
+        ins.pref = d->merge_pref[i];
+        if (ins.instr[0] < 0 || ins.instr[1] < 0 || ins.instr[2] < 0) continue;
+        h->iset.instr.push_back(ins);
+    }
+    const int* hss[3] = {h->ops[0].hss_of_ops.data(), h->ops[1].hss_of_ops.data(),
+                         isdmrg ? nullptr : h->ops[2].hss_of_ops.data()};
+    sort_instructions_merge(h->iset, hss, H);
+    in.iset = &h->iset;
+}
+
+int64_t write_plan_table(const HeffPlan& P, int* ints, double* prefs) {
+    int64_t c = 0;
+    for (const PlanPair& pp : P.pairs)
+        for (const PlanEntry& e : pp.entries) {
+            if (ints) {
+                int* r = ints + 7 * c;
+                r[0] = pp.newsb;
+                r[1] = pp.oldsb;
+                r[2] = pp.bestorder;
+                r[3] = e.mpo_group;
+                r[4] = e.sbops[0];
+                r[5] = e.sbops[1];
+                r[6] = P.isdmrg ? -1 : e.sbops[2];
+            }
+            if (prefs) prefs[c] = e.prefactor;
+            ++c;
+        }
+    return c;
+}
+}  // namespace
+
+struct t3ns_heff {
+    HeffHost host;
+    std::unique_ptr<HeffDevice> dev;
+    DevBuf<double> dvec, dres;
+    hipStream_t stream = nullptr;
+};
+
+extern "C" {
+
+const char* t3ns_last_error(void) { return g_last_error.c_str(); }
+
+int t3ns_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int t3ns_heff_init(t3ns_heff** out, const t3ns_heff_desc* d, const t3ns_rops_desc* ops,
+                   const t3ns_site_desc* site, int precision) {
+    if (!out || !d || !ops || !site) return fail(T3NS_ERR_INVALID, "null argument");
+    if (precision != T3NS_FP64 && precision != T3NS_FP32)
+        return fail(T3NS_ERR_INVALID, "unknown precision");
+    return guarded([&] {
+        if (t3ns_device_count() < 1) throw std::runtime_error("HIP error: no device visible");
+        auto h = std::make_unique<t3ns_heff>();
+        convert_heff_input(&h->host, d, ops, site);
+        HeffInput& in = h->host.in;
+        T3_HIP_CHECK(hipStreamCreate(&h->stream));
+        h->dev = std::make_unique<HeffDevice>(in, precision, h->stream);
+        h->dvec.alloc(h->dev->size());
+        h->dres.alloc(h->dev->size());
+        *out = h.release();
+    });
+}
+
+int t3ns_heff_matvec(t3ns_heff* h, const double* vec, double* result) {
+    if (!h || !vec || !result) return fail(T3NS_ERR_INVALID, "null argument");
+    return guarded([&] {
+        const size_t n = h->dev->size();
+        T3_HIP_CHECK(hipMemcpyAsync(h->dvec.get(), vec, n * sizeof(double), hipMemcpyHostToDevice,
+                                    h->stream));
+        h->dev->matvec(h->dvec.get(), h->dres.get(), h->stream);
+        T3_HIP_CHECK(hipMemcpyAsync(result, h->dres.get(), n * sizeof(double), hipMemcpyDeviceToHost,
+                                    h->stream));
+        T3_HIP_CHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int t3ns_heff_matvec_device(t3ns_heff* h, const double* dvec, double* dres, void* stream) {
+    if (!h || !dvec || !dres) return fail(T3NS_ERR_INVALID, "null argument");
+    return guarded([&] { h->dev->matvec(dvec, dres, (hipStream_t)stream); });
+}
+
+int t3ns_heff_diagonal(t3ns_heff* h, double* diag) {
+    if (!h || !diag) return fail(T3NS_ERR_INVALID, "null argument");
+    return guarded([&] {
+        const size_t n = h->dev->size();
+        h->dev->diagonal(h->dres.get(), h->stream);
+        T3_HIP_CHECK(hipMemcpyAsync(diag, h->dres.get(), n * sizeof(double), hipMemcpyDeviceToHost,
+                                    h->stream));
+        T3_HIP_CHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int t3ns_heff_destroy(t3ns_heff* h) {
+    if (!h) return T3NS_OK;
+    return guarded([&] {
+        hipStream_t s = h->stream;
+        delete h;
+        if (s) (void)hipStreamDestroy(s);
+    });
+}
+
+int64_t t3ns_heff_size(const t3ns_heff* h) { return h ? h->dev->size() : -1; }
+
+int t3ns_heff_stats(const t3ns_heff* h, int64_t* nblocks, int64_t* npairs, int64_t* nterms,
+                    double* flops) {
+    if (!h) return fail(T3NS_ERR_INVALID, "null argument");
+    const HeffStats& s = h->dev->stats();
+    if (nblocks) *nblocks = s.nblocks;
+    if (npairs) *npairs = s.npairs;
+    if (nterms) *nterms = s.nterms;
+    if (flops) *flops = s.flops;
+    return T3NS_OK;
+}
+
+int64_t t3ns_heff_plan_table(const t3ns_heff* h, int* ints, double* prefs) {
+    if (!h) return -1;
+    return write_plan_table(h->dev->plan(), ints, prefs);
+}
+
+int64_t t3ns_plan_table(const t3ns_heff_desc* d, const t3ns_rops_desc* ops,
+                        const t3ns_site_desc* site, int* ints, double* prefs, double* flops) {
+    if (!d || !ops || !site) return fail(T3NS_ERR_INVALID, "null argument");
+    int64_t n = -1;
+    const int rc = guarded([&] {
+        HeffHost host;
+        convert_heff_input(&host, d, ops, site);
+        HeffPlan P = build_heff_plan(host.in);
+        n = write_plan_table(P, ints, prefs);
+        if (flops) *flops = P.flops;
+    });
+    return rc == T3NS_OK ? n : rc;
+}
+
+}  // extern "C"

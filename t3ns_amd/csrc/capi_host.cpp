@@ -1,0 +1,319 @@
+// C ABI of the host layer (include/t3ns_host.h): contexts and steps.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/t3ns_host.h"
+#include "host/context.hpp"
+#include "host/tensors.hpp"
+
+using namespace t3;
+
+namespace t3 {
+void set_last_error(const std::string& msg);
+}
+
+namespace {
+template <typename F>
+int guarded_host(F&& f) {
+    try {
+        f();
+        return T3NS_OK;
+    } catch (const std::bad_alloc& e) {
+        t3::set_last_error(e.what());
+        return T3NS_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        t3::set_last_error(e.what());
+        return T3NS_ERR_INVALID;
+    }
+}
+
+// SplitMix64 + Box-Muller: a small deterministic normal generator.
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed * 0x9E3779B97F4A7C15ull + 0x1234567ull) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+    double normal() {
+        const double u = uniform(), v = uniform();
+        return std::sqrt(-2.0 * std::log(u)) * std::cos(6.283185307179586 * v);
+    }
+};
+
+// Flattened copies of symsecs for descriptors.
+struct SymsecsFlat {
+    std::vector<int> irreps, dims;
+    t3ns_symsecs_desc desc{};
+    void set(const Symsecs& s) {
+        irreps.assign((size_t)s.nrSecs * T3NS_MAX_SYMMETRIES, 0);
+        for (int i = 0; i < s.nrSecs; ++i)
+            for (int k = 0; k < T3NS_MAX_SYMMETRIES; ++k) irreps[i * T3NS_MAX_SYMMETRIES + k] = s.irreps[i][k];
+        dims = s.dims;
+        desc.bond = s.bond;
+        desc.nrSecs = s.nrSecs;
+        desc.irreps = irreps.data();
+        desc.dims = dims.data();
+    }
+};
+
+// nrops and MPO sector of every operator at `bond` as produced by the
+// renormalisation that creates it (vacuum, physical or branching update).
+void operator_types(Context& c, int bond, int is_left, int* nrops, std::vector<int>* hss) {
+    const int far = c.netw.bonds[bond][!is_left];
+    if (far == -1) {
+        *nrops = 1;
+        *hss = {c.ham->trivial_hss()};
+        return;
+    }
+    const InstructionSet* s;
+    if (c.netw.is_psite(far)) {
+        int b[3];
+        c.netw.bonds_of_site(far, b);
+        s = &c.fetch_pUpdate(b[is_left ? 0 : 2], is_left);
+    } else {
+        s = &c.fetch_bUpdate(bond, is_left);
+    }
+    int n = 0;
+    for (const auto& i : s->instr) n = std::max(n, i.instr[2] + 1);
+    *nrops = n;
+    hss->assign(s->hss_of_new.begin(), s->hss_of_new.begin() + n);
+}
+}  // namespace
+
+struct t3ns_ctx {
+    std::unique_ptr<Context> c;
+};
+
+struct t3ns_step {
+    t3ns_ctx* ctx = nullptr;
+    SiteTensor site;
+    ROperators ops[3];
+    InstructionSet merge_raw;
+    std::vector<int> merge_instr;
+    std::vector<double> merge_pref;
+    std::vector<int> hermitian;
+    std::vector<unsigned char> singlet;
+    SymsecsFlat mpo;
+    SymsecsFlat symarr[4][3];
+    int rops_on_site[3] = {-1, -1, -1};
+    int site_is_phys[4] = {0, 0, 0, 0};
+    std::vector<const int64_t*> bb_ptr[3];
+    std::vector<const double*> tel_ptr[3];
+};
+
+extern "C" {
+
+int t3ns_ctx_create(t3ns_ctx** out, const char* netw_text, const char* model, const double* params,
+                    int nparams, const int* sgs, const int* ts, int nsym, int max_dim, int minocc) {
+    if (!out || !netw_text || !model) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        Network n = Network::from_text(netw_text);
+        std::unique_ptr<Model> m;
+        const std::string mname = model;
+        std::vector<int> vs(sgs, sgs + nsym), vt(ts, ts + nsym);
+        bool su2 = std::find(vs.begin(), vs.end(), (int)SU2) != vs.end();
+        if (mname == "NN_HUBBARD") {
+            if (nparams < 2) throw std::invalid_argument("NN_HUBBARD needs t and U");
+            m = make_hubbard_model(params[0], params[1], su2, nparams > 2 && params[2] != 0);
+        } else {
+            throw std::invalid_argument("unknown model " + mname);
+        }
+        auto ctx = std::make_unique<t3ns_ctx>();
+        ctx->c = make_context(std::move(n), std::move(m), vs, vt);
+        ctx->c->prepare_bookkeeper(max_dim, true, minocc);
+        *out = ctx.release();
+    });
+}
+
+int t3ns_ctx_destroy(t3ns_ctx* c) {
+    delete c;
+    return T3NS_OK;
+}
+
+int t3ns_ctx_info(const t3ns_ctx* c, int* nr_bonds, int* sites, int* psites, int* nrsyms, int* sgs) {
+    if (!c) return T3NS_ERR_INVALID;
+    const Context& x = *c->c;
+    if (nr_bonds) *nr_bonds = x.netw.nr_bonds;
+    if (sites) *sites = x.netw.sites;
+    if (psites) *psites = x.netw.psites;
+    if (nrsyms) *nrsyms = x.sym.nrSyms;
+    if (sgs)
+        for (int i = 0; i < x.sym.nrSyms; ++i) sgs[i] = x.sym.sgs[i];
+    return T3NS_OK;
+}
+
+int t3ns_ctx_bond(const t3ns_ctx* c, int bond, int* nrsecs, int* totaldims, int* dims_out,
+                  int dims_cap) {
+    if (!c) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        const Symsecs& s = c->c->symsecs(bond);
+        if (nrsecs) *nrsecs = s.nrSecs;
+        int tot = 0;
+        for (int i = 0; i < s.nrSecs; ++i) {
+            tot += s.dims.empty() ? 0 : s.dims[i];
+            if (dims_out && i < dims_cap) dims_out[i] = s.dims.empty() ? 0 : s.dims[i];
+        }
+        if (totaldims) *totaldims = tot;
+    });
+}
+
+int t3ns_step_random(t3ns_step** out, t3ns_ctx* cx, const int* sites, int nsites, uint64_t seed) {
+    if (!out || !cx || !sites) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        Context& c = *cx->c;
+        auto st = std::make_unique<t3ns_step>();
+        st->ctx = cx;
+        MultisiteInfo md;
+        st->site = make_multisite_structure(c, sites, nsites, &md);
+        SiteTensor& T = st->site;
+        Rng rng(seed);
+        T.tel.resize(T.size());
+        // External bonds of the object, sorted (src/network.c:614-649).
+        std::vector<int> ext;
+        for (int i = 0; i < T.nrsites; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const int b = md.bonds[i][j];
+                if (c.netw.is_pbond(b)) continue;
+                int cnt = 0;
+                for (int k = 0; k < T.nrsites; ++k)
+                    for (int l = 0; l < 3; ++l) cnt += md.bonds[k][l] == b;
+                if (cnt == 1) ext.push_back(b);
+            }
+        std::sort(ext.begin(), ext.end());
+        const bool isdmrg = ext.size() == 2;
+        auto in_obj = [&](int s) { return std::find(T.sites, T.sites + T.nrsites, s) != T.sites + T.nrsites; };
+        for (size_t i = 0; i < ext.size(); ++i) {
+            const int b = ext[i];
+            const int is_left = in_obj(c.netw.bonds[b][1]) ? 1 : 0;
+            const int adj = c.netw.bonds[b][is_left];
+            int nrops;
+            std::vector<int> hss;
+            ROperators r;
+            std::vector<std::vector<int64_t>> tmpbb;
+            if (c.netw.is_psite(adj)) {
+                // Physical operator: the append of site `adj` to rops[b].
+                int ab[3];
+                c.netw.bonds_of_site(adj, ab);
+                const int pbond = ab[is_left ? 2 : 0];
+                const InstructionSet& ps = c.fetch_pUpdate(b, is_left);
+                nrops = 0;
+                for (const auto& ins : ps.instr) nrops = std::max(nrops, ins.instr[2] + 1);
+                hss.assign(ps.hss_of_new.begin(), ps.hss_of_new.begin() + nrops);
+                r = init_rOperators_structure(c, pbond, is_left, true, &tmpbb);
+            } else if (c.netw.bonds[b][!is_left] == -1) {
+                r = vacuum_rOperators(c, b, is_left);
+                st->ops[i] = std::move(r);
+                continue;
+            } else {
+                operator_types(c, b, is_left, &nrops, &hss);
+                r = init_rOperators_structure(c, b, is_left, false, &tmpbb);
+            }
+            r.nrops = nrops;
+            r.hss_of_ops = hss;
+            r.beginblock.resize(nrops);
+            for (int o = 0; o < nrops; ++o) r.beginblock[o] = tmpbb[hss[o]];
+            r.finalize_offsets();
+            r.tel.resize(r.total_size());
+            for (auto& v : r.tel) v = rng.normal();
+            st->ops[i] = std::move(r);
+        }
+        // Heff descriptor pieces
+        for (int i = 0; i < T.nrsites; ++i) {
+            st->site_is_phys[i] = c.netw.is_psite(T.sites[i]);
+            int b[3];
+            c.netw.bonds_of_site(T.sites[i], b);
+            for (int k = 0; k < 3; ++k) st->symarr[i][k].set(c.symsecs(b[k]));
+        }
+        for (int i = 0; i < (isdmrg ? 2 : 3); ++i) {
+            const int s = rOperators_site_to_attach(c, st->ops[i]);
+            for (int j = 0; j < T.nrsites; ++j)
+                if (T.sites[j] == s) st->rops_on_site[i] = j;
+        }
+        st->mpo.set(c.symsecs(-1));
+        const int H = c.ham->nr_hss();
+        for (int h = 0; h < H; ++h) st->hermitian.push_back(c.ham->hermitian_hss(h));
+        st->singlet.assign((size_t)H * H * H, 0);
+        for (int a = 0; a < H; ++a)
+            for (int b2 = 0; b2 < H; ++b2)
+                for (int d = 0; d < H; ++d) {
+                    const int m[3] = {a, b2, d};
+                    st->singlet[a + (size_t)b2 * H + (size_t)d * H * H] = c.ham->mpo_couples_to_singlet(m);
+                }
+        st->merge_raw = c.ham->raw_merge(c.netw, st->ops[0].bond, isdmrg);
+        for (const auto& ins : st->merge_raw.instr) {
+            for (int k = 0; k < 3; ++k) st->merge_instr.push_back(ins.instr[k]);
+            st->merge_pref.push_back(ins.pref);
+        }
+        for (auto& v : T.tel) v = rng.normal();
+        *out = st.release();
+    });
+}
+
+int t3ns_step_destroy(t3ns_step* s) {
+    delete s;
+    return T3NS_OK;
+}
+
+int64_t t3ns_step_size(const t3ns_step* s) { return s ? s->site.size() : -1; }
+
+int t3ns_step_heff_desc(t3ns_step* s, t3ns_heff_desc* d, t3ns_rops_desc* ops, t3ns_site_desc* site) {
+    if (!s || !d || !ops || !site) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        const Context& c = *s->ctx->c;
+        std::memset(d, 0, sizeof(*d));
+        d->nrSyms = c.sym.nrSyms;
+        for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) d->sgs[i] = c.sym.sgs[i];
+        d->mpo = s->mpo.desc;
+        d->hermitian = s->hermitian.data();
+        d->singlet = s->singlet.data();
+        const SiteTensor& T = s->site;
+        for (int i = 0; i < T.nrsites; ++i)
+            for (int k = 0; k < 3; ++k) d->symarr[i][k] = s->symarr[i][k].desc;
+        for (int k = 0; k < 3; ++k) d->rops_on_site[k] = s->rops_on_site[k];
+        d->merge_step = s->merge_raw.step;
+        d->merge_nr = s->merge_raw.nr_instr();
+        d->merge_instr = s->merge_instr.data();
+        d->merge_pref = s->merge_pref.data();
+        for (int k = 0; k < 3; ++k) {
+            const ROperators& r = s->ops[k];
+            t3ns_rops_desc& o = ops[k];
+            std::memset(&o, 0, sizeof(o));
+            if (r.nrhss <= 0) continue;
+            o.bond = r.bond;
+            o.is_left = r.is_left;
+            o.P_operator = r.P_operator;
+            o.nrhss = r.nrhss;
+            o.begin_blocks_of_hss = r.begin_blocks_of_hss.data();
+            o.qnumbers = r.qnumbers.data();
+            o.nrops = r.nrops;
+            o.hss_of_ops = r.hss_of_ops.data();
+            s->bb_ptr[k].resize(r.nrops);
+            s->tel_ptr[k].resize(r.nrops);
+            for (int op = 0; op < r.nrops; ++op) {
+                s->bb_ptr[k][op] = r.beginblock[op].data();
+                s->tel_ptr[k][op] = r.tel.data() + r.op_offset[op];
+            }
+            o.beginblock = s->bb_ptr[k].data();
+            o.tel = s->tel_ptr[k].data();
+        }
+        std::memset(site, 0, sizeof(*site));
+        site->nrsites = T.nrsites;
+        for (int i = 0; i < 4; ++i) {
+            site->sites[i] = T.sites[i];
+            site->site_is_phys[i] = s->site_is_phys[i];
+        }
+        site->nrblocks = T.nrblocks;
+        site->qnumbers = T.qnumbers.data();
+        site->beginblock = T.beginblock.data();
+    });
+}
+
+}  // extern "C"

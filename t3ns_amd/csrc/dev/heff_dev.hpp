@@ -1,0 +1,90 @@
+// Device-resident effective Hamiltonian: the plan flattened into work lists
+// in HBM and the kernels that execute Psi' = Heff Psi and diag(Heff).
+// Replaces the OpenMP replay loop of the reference (src/Heff.c:916-1000) and
+// make_diagonal (src/Heff.c:1268-1312).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../host/heff_plan.hpp"
+#include "hip_util.hpp"
+
+namespace t3 {
+
+enum Precision : int { PREC_FP64 = 0, PREC_FP32 = 1 };
+
+struct DevBlock {      // one output block
+    int64_t y_off;     // element offset of the block in the vector
+    int d[3];          // leg dims in operator order
+    int pair_begin;
+    int pair_end;
+    int pad;
+};
+struct DevPair {       // one (output block, input block) pair
+    int64_t x_off;
+    int d[3];          // input leg dims in operator order
+    int order;         // contraction order (index into the 6 permutations)
+    int term_begin;
+    int term_end;
+};
+struct DevTerm {       // one instruction of one MPO group of a pair
+    int64_t op[3];     // element offsets into the three operator arenas
+    double alpha;      // instruction prefactor x symmetry prefactor
+};
+struct DevDiagBlock {
+    int64_t off;
+    int d[3];
+    int term_begin;
+    int term_end;
+    int pad;
+};
+
+struct HeffStats {
+    int64_t nblocks = 0, npairs = 0, nterms = 0, size = 0;
+    double flops = 0;            // reference-order GEMM flops of one matvec
+    int64_t op_elems[3] = {0, 0, 0};
+    int64_t scratch_per_wg = 0;
+};
+
+class HeffDevice {
+public:
+    // Builds the plan tables and uploads the operator arenas (host data in
+    // in.ops[k]->tel).  `stream` may be null (default stream).
+    HeffDevice(const HeffInput& in, int precision, hipStream_t stream);
+    ~HeffDevice();
+    HeffDevice(const HeffDevice&) = delete;
+    HeffDevice& operator=(const HeffDevice&) = delete;
+
+    int64_t size() const { return size_; }
+    const HeffStats& stats() const { return stats_; }
+    const HeffPlan& plan() const { return plan_; }
+    int precision() const { return prec_; }
+
+    // Device pointers, FP64 in and out.
+    void matvec(const double* dvec, double* dres, hipStream_t s);
+    void diagonal(double* ddiag, hipStream_t s);
+
+private:
+    HeffPlan plan_;
+    HeffStats stats_;
+    int prec_;
+    int isdmrg_;
+    int map_[3];
+    int64_t size_;
+    int nwg_;
+    DevBuf<DevBlock> blocks_;
+    DevBuf<DevPair> pairs_;
+    DevBuf<DevTerm> terms_;
+    DevBuf<DevDiagBlock> dblocks_;
+    DevBuf<DevTerm> dterms_;
+    DevBuf<double> ops64_[3];
+    DevBuf<float> ops32_[3];
+    DevBuf<double> scratch64_;
+    DevBuf<float> scratch32_;
+    DevBuf<float> x32_, y32_;
+    int ndblocks_ = 0;
+};
+
+}  // namespace t3

@@ -1,0 +1,69 @@
+// Symmetry sectors, instruction sets and the Hamiltonian (MPO) model
+// interface.  The reference dispatches on a process-global `ham` enum
+// (src/hamiltonian.c:31); here a Model object carries the same operations
+// (include/hamiltonian.h:30-120, include/instructions.h:39-86).
+#pragma once
+#include <array>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "network.hpp"
+#include "sym.hpp"
+
+namespace t3 {
+
+// Symmetry sectors of one bond (include/symsecs.h:32-54).
+struct Symsecs {
+    int bond = 0;
+    int nrSecs = 0;
+    std::vector<Irrep> irreps;
+    std::vector<double> fcidims;
+    std::vector<int> dims;
+    int totaldims = 0;
+};
+
+struct Instruction {  // include/instructions.h:24-33
+    int instr[3];
+    double pref;
+};
+
+struct InstructionSet {  // include/instructions.h:36-66
+    int step = 0;
+    std::vector<Instruction> instr;
+    std::vector<int> hss_of_new;
+    // Merge sets only: MPO combination per group and group starts.
+    std::vector<int> MPOc;
+    std::vector<int> MPOc_beg;
+    int nr_instr() const { return (int)instr.size(); }
+};
+
+class Model {
+public:
+    virtual ~Model() = default;
+    virtual std::string name() const = 0;
+    virtual Symsecs phys_symsecs(int orbital) const = 0;
+    virtual const Symsecs& mpo_symsecs() const = 0;
+    int nr_hss() const { return mpo_symsecs().nrSecs; }
+    virtual int trivial_hss() const = 0;
+    virtual int hermitian_hss(int hss) const = 0;
+    virtual bool mpo_couples_to_singlet(const int mpo[3]) const = 0;
+    virtual int symsec_siteop(int siteop, int orbital) const = 0;
+    virtual double el_siteop(int siteop, int braid, int ketid) const = 0;
+    // All (hss_i, hss_j) whose product can give `resulting` at a site.
+    virtual std::vector<std::array<int, 2>> tprods_ham(int resulting, int orbital) const = 0;
+    virtual bool consistent_state(const int* ts, const SymInfo& s) const = 0;
+    // Raw (unsorted) instruction lists as the model generates them.
+    virtual InstructionSet raw_pUpdate(const Network& n, int bond, int is_left) = 0;
+    virtual InstructionSet raw_bUpdate(const Network& n, int bond, int is_left) = 0;
+    virtual InstructionSet raw_merge(const Network& n, int bond, int isdmrg) = 0;
+};
+
+// NN_HUBBARD (t, U) model on any network; U1xU1 = (N_up, N_down) or
+// U1xSU2 = (N, 2S) (src/hamiltonian_nn_hubbard.c, src/instructions_nn_hubbard.c).
+// With reference_merge_quirk the merge set is chosen as the reference does
+// (by `bond != 0`, src/instructions.c:204), otherwise by isdmrg.
+std::unique_ptr<Model> make_hubbard_model(double t, double U, bool su2,
+                                          bool reference_merge_quirk = false);
+
+}  // namespace t3

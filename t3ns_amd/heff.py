@@ -1,0 +1,191 @@
+"""Host-side mirror of the reference's Heff interface on top of the C ABI.
+
+Reference API (include/Heff.h:166-192)   ->  here
+    init_Heffdata(data, Operators, site)  ->  Heff(step_or_descs, precision)
+    matvecT3NS(vec, result, vdata)        ->  matvecT3NS(vec, result, heff)
+    make_diagonal(data)                   ->  make_diagonal(heff)
+    destroy_Heffdata(data)                ->  destroy_Heffdata(heff) / Heff.close()
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import FP32, FP64, HeffDesc, RopsDesc, SiteDesc, check, lib
+
+
+class Context:
+    """Solve context: network, model, symmetry sectors of every bond.
+
+    Replaces make_network + readinteraction + preparebookkeeper of the
+    reference's tests (tests/test1.c.in:31-54)."""
+
+    def __init__(self, netw_text, model, params, symmetries, target, max_dim, minocc=2):
+        sgs = [_lib.SYMGROUPS[s] if isinstance(s, str) else int(s) for s in symmetries]
+        ts = [int(t) for t in target]
+        par = (C.c_double * len(params))(*params)
+        h = C.c_void_p()
+        check(lib().t3ns_ctx_create(C.byref(h), netw_text.encode(), model.encode(), par,
+                                    len(params), (C.c_int * len(sgs))(*sgs),
+                                    (C.c_int * len(ts))(*ts), len(sgs), int(max_dim), int(minocc)),
+              "t3ns_ctx_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        nb, ns, nps, nsym = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        sgs = (C.c_int * 5)()
+        check(lib().t3ns_ctx_info(self._h, C.byref(nb), C.byref(ns), C.byref(nps), C.byref(nsym), sgs))
+        return {"nr_bonds": nb.value, "sites": ns.value, "psites": nps.value,
+                "sgs": list(sgs)[:nsym.value]}
+
+    def bond_dims(self, bond):
+        n, tot = C.c_int(), C.c_int()
+        check(lib().t3ns_ctx_bond(self._h, bond, C.byref(n), C.byref(tot), None, 0))
+        buf = (C.c_int * max(n.value, 1))()
+        check(lib().t3ns_ctx_bond(self._h, bond, C.byref(n), C.byref(tot), buf, n.value))
+        return list(buf)[:n.value]
+
+    def close(self):
+        if self._h:
+            lib().t3ns_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Step:
+    """Inputs of one Heff (multisite tensor + renormalised operators)."""
+
+    def __init__(self, ctx, sites, seed=0):
+        h = C.c_void_p()
+        arr = (C.c_int * len(sites))(*sites)
+        check(lib().t3ns_step_random(C.byref(h), ctx.handle, arr, len(sites), int(seed)),
+              "t3ns_step_random")
+        self._h = h
+        self.ctx = ctx  # keeps the context alive
+
+    @property
+    def size(self):
+        return int(lib().t3ns_step_size(self._h))
+
+    def descriptors(self):
+        d, o, s = HeffDesc(), (RopsDesc * 3)(), SiteDesc()
+        check(lib().t3ns_step_heff_desc(self._h, C.byref(d), o, C.byref(s)), "t3ns_step_heff_desc")
+        return d, o, s
+
+    def close(self):
+        if self._h:
+            lib().t3ns_step_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plan_table(desc, ops, site):
+    """Host-built dispatch table: (ints[n,7], prefactors[n], flops)."""
+    fl = C.c_double()
+    n = lib().t3ns_plan_table(C.byref(desc), ops, C.byref(site), None, None, C.byref(fl))
+    if n < 0:
+        check(int(n), "t3ns_plan_table")
+    ints = np.zeros((max(n, 1), 7), dtype=np.int32)
+    prefs = np.zeros(max(n, 1), dtype=np.float64)
+    lib().t3ns_plan_table(C.byref(desc), ops, C.byref(site),
+                          ints.ctypes.data_as(_lib.c_int_p), prefs.ctypes.data_as(_lib.c_dbl_p), None)
+    return ints[:n], prefs[:n], fl.value
+
+
+class Heff:
+    """init_Heffdata (include/Heff.h:166) on the device."""
+
+    def __init__(self, desc, ops, site, precision=FP64, keepalive=None):
+        h = C.c_void_p()
+        check(lib().t3ns_heff_init(C.byref(h), C.byref(desc), ops, C.byref(site), int(precision)),
+              "t3ns_heff_init")
+        self._h = h
+        self._keep = keepalive
+        self.size = int(lib().t3ns_heff_size(h))
+
+    @classmethod
+    def from_step(cls, step, precision=FP64):
+        d, o, s = step.descriptors()
+        return cls(d, o, s, precision, keepalive=step)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def matvec(self, vec, out=None):
+        vec = np.ascontiguousarray(vec, dtype=np.float64)
+        if vec.shape != (self.size,):
+            raise ValueError("vector has %s elements, Heff needs %d" % (vec.shape, self.size))
+        if out is None:
+            out = np.empty(self.size, dtype=np.float64)
+        check(lib().t3ns_heff_matvec(self._h, vec.ctypes.data_as(_lib.c_dbl_p),
+                                     out.ctypes.data_as(_lib.c_dbl_p)), "t3ns_heff_matvec")
+        return out
+
+    def matvec_device(self, dvec_ptr, dres_ptr, stream_ptr=None):
+        check(lib().t3ns_heff_matvec_device(self._h, C.c_void_p(dvec_ptr), C.c_void_p(dres_ptr),
+                                            C.c_void_p(stream_ptr or 0)), "t3ns_heff_matvec_device")
+
+    def diagonal(self):
+        out = np.empty(self.size, dtype=np.float64)
+        check(lib().t3ns_heff_diagonal(self._h, out.ctypes.data_as(_lib.c_dbl_p)), "t3ns_heff_diagonal")
+        return out
+
+    def stats(self):
+        nb, npair, nt, fl = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double()
+        check(lib().t3ns_heff_stats(self._h, C.byref(nb), C.byref(npair), C.byref(nt), C.byref(fl)))
+        return {"nblocks": nb.value, "npairs": npair.value, "nterms": nt.value, "flops": fl.value,
+                "size": self.size}
+
+    def plan_table(self):
+        n = lib().t3ns_heff_plan_table(self._h, None, None)
+        ints = np.zeros((max(n, 1), 7), dtype=np.int32)
+        prefs = np.zeros(max(n, 1), dtype=np.float64)
+        lib().t3ns_heff_plan_table(self._h, ints.ctypes.data_as(_lib.c_int_p),
+                                   prefs.ctypes.data_as(_lib.c_dbl_p))
+        return ints[:n], prefs[:n]
+
+    def close(self):
+        if self._h:
+            lib().t3ns_heff_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# Reference-shaped entry points ------------------------------------------------
+
+def init_Heffdata(Operators, siteObject, desc, precision=FP64):
+    """init_Heffdata(data, Operators, siteObject) with the globals passed as `desc`."""
+    return Heff(desc, Operators, siteObject, precision)
+
+
+def matvecT3NS(vec, result, vdata):
+    """matvecT3NS(vec, result, vdata): result is overwritten with Heff vec."""
+    vdata.matvec(vec, out=result)
+
+
+def make_diagonal(data):
+    return data.diagonal()
+
+
+def destroy_Heffdata(data):
+    data.close()

@@ -1,0 +1,34 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+NETW = os.path.join(ROOT, "tests", "golden", "networks")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+def netw_text(name):
+    with open(os.path.join(NETW, name + ".netw")) as f:
+        return f.read()
+
+
+def have_gpu():
+    try:
+        from t3ns_amd import lib
+        return lib().t3ns_device_count() > 0
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    if not have_gpu():
+        pytest.fail("GPU test run without a visible HIP device or without libt3ns_amd.so")
+    return True
