@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Heff matvec benchmark on one MI355X.
+
+Workload: synthetic quantum-chemistry Hamiltonian with 24 orbitals and 24
+electrons (FCIDUMP from t3ns_amd.synthetic, seed 1, ERI rank 48), symmetry
+U1 x SU2 (N = 24, 2S = 0), T3NS tree network with 24 physical sites,
+bond dimension D = 512 (bookkeeper sector dims), three-site optimisation
+step {physical, branching, physical} around branching site 31 (sites 18, 31,
+17).  One step of the benchmark = one effective-Hamiltonian matvec on a
+device-resident N(0,1) vector (seed 1); the operators hold N(0,1) entries on
+the exact block structure of the renormalised operators of that step.
+
+Prints one JSON line: matvec/s of the plain FP64 path (value) and of the
+FP32 path, the roofline of the matvec kernel (HIP events on its stream), and
+the CPU baseline (the FP64 restatement in oracle/, OpenMP over output blocks,
+timed on a bounded sample of output blocks on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+NETWORK = os.path.join(ROOT, "tests", "golden", "networks", "24_T3NS.netw")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--sites", default="18,31,17")
+    ap.add_argument("--bond-dim", type=int, default=512)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target CPU time of the baseline sample (0 disables it)")
+    ap.add_argument("--precisions", default="f64,f32")
+    return ap.parse_args()
+
+
+def build_step(sites, D):
+    import t3ns_amd as T
+    from t3ns_amd import synthetic
+    fci = synthetic.qc_workload(24, 24, seed=1, rank=48)
+    with open(NETWORK) as f:
+        netw = f.read()
+    ctx = T.Context(netw, "QC", [], ["U1", "SU2"], [24, 0], D, fcidump=fci)
+    step = T.Step(ctx, sites, seed=1)
+    return ctx, step
+
+
+def cpu_baseline(step, x, target_seconds, flops_total):
+    """FP64 oracle matvec over a strided sample of output blocks."""
+    import oracle as O
+    d, o, s = step.descriptors()
+    oh = O.OracleHeff(d, o, s)
+    bf = oh.block_flops()
+    out = np.zeros(oh.size)
+    nb = oh.nrblocks
+    # calibrate on a small sample, then size the sample to the target time
+    stride = max(1, nb // 64)
+    t0 = time.perf_counter()
+    oh.matvec_blocks(x, out, 0, nb, stride)
+    t_cal = time.perf_counter() - t0
+    f_cal = bf[0:nb:stride].sum()
+    rate = f_cal / max(t_cal, 1e-9)
+    want = rate * target_seconds
+    frac = min(1.0, want / max(flops_total, 1.0))
+    stride = max(1, int(round(1.0 / max(frac, 1e-9))))
+    t0 = time.perf_counter()
+    oh.matvec_blocks(x, out, 0, nb, stride)
+    t = time.perf_counter() - t0
+    f = bf[0:nb:stride].sum()
+    oh.close()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": (f / t) / flops_total, "unit": "matvec/s", "cores": threads,
+            "kind": "port",
+            "sample": "FP64 oracle restatement of src/Heff.c matvec, OpenMP over output blocks, "
+                      "every %d-th output block (%d of %d blocks, %.1f%% of the matvec flops) in "
+                      "%.1f s; matvec/s = sample flop rate / full-matvec flops"
+                      % (stride, len(range(0, nb, stride)), nb, 100.0 * f / flops_total, t)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import t3ns_amd as T
+
+    sites = [int(v) for v in args.sites.split(",")]
+    t0 = time.perf_counter()
+    ctx, step = build_step(sites, args.bond_dim)
+    t_setup = time.perf_counter() - t0
+    print("# setup %.1f s, vector size %d" % (t_setup, step.size), file=sys.stderr, flush=True)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(step.size)
+    x /= np.linalg.norm(x)
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    dx = torch.from_numpy(x).to(dev)
+    results = {}
+    ys = {}
+    heffs = {}
+    for prec in args.precisions.split(","):
+        t0 = time.perf_counter()
+        h = T.Heff.from_step(step, T.FP64 if prec == "f64" else T.FP32)
+        t_init = time.perf_counter() - t0
+        heffs[prec] = h
+        print("# %s: Heff init %.1f s" % (prec, t_init), file=sys.stderr, flush=True)
+        dy = torch.empty_like(dx)
+        torch.cuda.synchronize()
+        for _ in range(args.warmup):
+            w0 = time.perf_counter()
+            h.matvec_device(dx.data_ptr(), dy.data_ptr(), stream.cuda_stream)
+            stream.synchronize()
+            print("# %s: warmup matvec %.3f s" % (prec, time.perf_counter() - w0),
+                  file=sys.stderr, flush=True)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            h.matvec_device(dx.data_ptr(), dy.data_ptr(), stream.cuda_stream)
+        ev1.record(stream)
+        stream.synchronize()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        ms = ev0.elapsed_time(ev1) / args.steps
+        st = h.stats()
+        ys[prec] = dy.cpu().numpy()
+        results[prec] = {"ms_per_matvec": ms, "matvec_per_s": 1000.0 / ms,
+                         "wall_s": wall, "init_s": t_init, "flops": st["flops"],
+                         "tflops": st["flops"] / (ms * 1e-3) / 1e12}
+        print("# %s: %.3f ms/matvec, %.2f TFLOP/s (init %.1f s)" %
+              (prec, ms, results[prec]["tflops"], t_init), file=sys.stderr, flush=True)
+
+    main_prec = "f64" if "f64" in results else list(results)[0]
+    r = results[main_prec]
+    flops = r["flops"]
+    out = {
+        "metric": "Heff matvec/s (QC-24 T3NS, D=512, 3-site branching step)",
+        "value": r["matvec_per_s"],
+        "unit": "matvec/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_matvec"],
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": main_prec,
+        "data": "synthetic (FCIDUMP seed 1; N(0,1) operator blocks and input vector)",
+        "config": {"workload": "qc24_t3ns_D%d_sites%s" % (args.bond_dim, "-".join(map(str, sites))),
+                   "norb": 24, "nelec": 24, "symmetry": "U1xSU2", "bond_dim": args.bond_dim,
+                   "sites": sites, "vector_size": step.size,
+                   "flops_per_matvec": flops, "setup_s": t_setup},
+        "roofline": {"bound": "mfma", "achieved": r["tflops"], "peak": PEAK_TFLOPS[main_prec],
+                     "unit": "TFLOP/s", "frac": r["tflops"] / PEAK_TFLOPS[main_prec],
+                     "traffic": None},
+    }
+    if "f32" in results and "f64" in results:
+        err = float(np.linalg.norm(ys["f32"] - ys["f64"]) / np.linalg.norm(ys["f64"]))
+        out["mixed"] = {"dtype": "f32", "matvec_per_s": results["f32"]["matvec_per_s"],
+                        "ms_per_matvec": results["f32"]["ms_per_matvec"],
+                        "speedup_vs_f64": results["f32"]["matvec_per_s"] / r["matvec_per_s"],
+                        "rel_err_vs_f64": err,
+                        "roofline_frac": results["f32"]["tflops"] / PEAK_TFLOPS["f32"]}
+    if args.cpu_seconds > 0:
+        out["cpu_baseline"] = cpu_baseline(step, x, args.cpu_seconds, flops)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -28,6 +28,17 @@ typedef struct t3ns_step t3ns_step;
  * preparebookkeeper(NULL, max_dim, 1, minocc, NULL) (src/bookkeeper.c:725). */
 int t3ns_ctx_create(t3ns_ctx** c, const char* netw_text, const char* model, const double* params,
                     int nparams, const int* sgs, const int* ts, int nsym, int max_dim, int minocc);
+/* Quantum-chemistry Hamiltonian from FCIDUMP text (src/hamiltonian_qc.c:152,
+ * QC_make_hamiltonian; header and integrals as read at src/hamiltonian_qc.c:783-900).
+ * sgs must hold U1 and U1 or SU2, optionally one point group whose ORBSYM
+ * labels the FCIDUMP carries. */
+int t3ns_ctx_create_qc(t3ns_ctx** c, const char* netw_text, const char* fcidump_text,
+                       const int* sgs, const int* ts, int nsym, int max_dim, int minocc);
+/* The optimisation steps of one sweep as next_opt_step yields them
+ * (src/network.c:529-727) with at most maxsites sites per step: for step i,
+ * sites_out[4*i .. 4*i+3] (unused slots -1) and nsites_out[i].  Returns the
+ * number of steps (entries beyond cap are counted, not written). */
+int t3ns_ctx_sweep_steps(t3ns_ctx* c, int maxsites, int* sites_out, int* nsites_out, int cap);
 int t3ns_ctx_destroy(t3ns_ctx* c);
 /* nr_bonds, sites, psites of the network; nrSyms and sgs after Z2 insertion. */
 int t3ns_ctx_info(const t3ns_ctx* c, int* nr_bonds, int* sites, int* psites, int* nrsyms,

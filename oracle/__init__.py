@@ -46,6 +46,8 @@ def lib():
                                            C.POINTER(SiteDesc)]),
             "oracle_heff_size": (C.c_int64, [P]),
             "oracle_heff_matvec": (None, [P, dp, dp]),
+            "oracle_heff_matvec_blocks": (None, [P, dp, dp, C.c_int, C.c_int, C.c_int]),
+            "oracle_heff_block_stats": (None, [P, i64, i64, i64, dp, dp]),
             "oracle_heff_diagonal": (None, [P, dp]),
             "oracle_heff_plan_stats": (None, [P, i64, i64, i64, dp]),
             "oracle_heff_plan_table": (C.c_int64, [P, ip, dp]),
@@ -75,12 +77,26 @@ class OracleHeff:
             raise RuntimeError("oracle_heff_init failed")
         self._h = h
         self.size = int(lib().oracle_heff_size(h))
+        self.nrblocks = int(site.nrblocks)
 
     def matvec(self, vec):
         vec = np.ascontiguousarray(vec, dtype=np.float64)
         out = np.empty(self.size)
         lib().oracle_heff_matvec(self._h, vec.ctypes.data_as(C.POINTER(C.c_double)),
                                  out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def matvec_blocks(self, vec, out, first, last, stride=1):
+        """Fills only output blocks first, first+stride, ... < last of out."""
+        lib().oracle_heff_matvec_blocks(self._h, vec.ctypes.data_as(C.POINTER(C.c_double)),
+                                        out.ctypes.data_as(C.POINTER(C.c_double)),
+                                        int(first), int(last), int(stride))
+
+    def block_flops(self):
+        n = self.nrblocks
+        out = np.zeros(n)
+        lib().oracle_heff_block_stats(self._h, None, None, None, None,
+                                      out.ctypes.data_as(C.POINTER(C.c_double)))
         return out
 
     def diagonal(self):

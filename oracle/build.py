@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "liboracle.so")
 REFLIB = os.path.join(HERE, "_ref", "libref_sym.so")
 REF_SOURCES = ["symmetries.c", "symmetry_z2.c", "symmetry_u1.c", "symmetry_su2.c",
                "symmetry_pg.c", "Wigner.c", "macros.c", "sort.c"]
-CFLAGS = ["-O3", "-fPIC", "-shared", "-std=gnu11", "-fopenmp", "-march=x86-64-v2"]
+CFLAGS = ["-O3", "-fPIC", "-shared", "-std=gnu11", "-fopenmp", "-march=x86-64-v3"]
 
 
 def build_oracle():

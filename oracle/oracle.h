@@ -55,6 +55,10 @@ int oracle_heff_init(oracle_heff** h, const t3ns_heff_desc* desc, const t3ns_rop
 int64_t oracle_heff_size(const oracle_heff* h);
 /* result = Heff vec (result overwritten), OpenMP over output blocks. */
 void oracle_heff_matvec(oracle_heff* h, const double* vec, double* result);
+void oracle_heff_matvec_blocks(oracle_heff* h, const double* vec, double* result, int first,
+                               int last, int stride);
+void oracle_heff_block_stats(const oracle_heff* h, int64_t* npairs, int64_t* nentries,
+                             int64_t* nterms, double* flops, double* block_flops);
 void oracle_heff_diagonal(const oracle_heff* h, double* diag);
 /* Plan table summary: number of (new,old) pairs, MPO entries and executed
  * GEMM chains, and the algorithmic flops of one matvec. */

@@ -694,8 +694,18 @@ int64_t oracle_heff_size(const oracle_heff* h) { return h->bb[h->nrblocks]; }
 /* exec_secondrun / execute_heffcontr, src/Heff.c:916-1000 */
 void oracle_heff_matvec(oracle_heff* h, const double* vec, double* result)
 {
-        const int64_t size = h->bb[h->nrblocks];
-        for (int64_t i = 0; i < size; ++i) result[i] = 0;
+        oracle_heff_matvec_blocks(h, vec, result, 0, h->nrblocks, 1);
+}
+
+/* The same over output blocks first, first+stride, ... < last only (the
+ * other entries of result are left untouched); used to time a bounded
+ * sample of a matvec. */
+void oracle_heff_matvec_blocks(oracle_heff* h, const double* vec, double* result, int first,
+                               int last, int stride)
+{
+        for (int i = first; i < last; i += stride)
+                for (int64_t e = h->bb[i]; e < h->bb[i + 1]; ++e) result[e] = 0;
+        const int nsel = stride > 0 && last > first ? (last - first + stride - 1) / stride : 0;
 #pragma omp parallel
         {
                 double* tels[7];
@@ -704,7 +714,8 @@ void oracle_heff_matvec(oracle_heff* h, const double* vec, double* result)
                 tels[T_WORK1] = w1;
                 tels[T_WORK2] = w2;
 #pragma omp for schedule(dynamic)
-                for (int i = 0; i < h->nrblocks; ++i) {
+                for (int ii = 0; ii < nsel; ++ii) {
+                        const int i = first + ii * stride;
                         int dims[2][3];
                         memcpy(dims[0], h->dims[i], sizeof(dims[0]));
                         tels[T_NEW] = result + h->bb[i];
@@ -793,9 +804,18 @@ void oracle_heff_diagonal(const oracle_heff* h, double* diag)
 void oracle_heff_plan_stats(const oracle_heff* h, int64_t* npairs, int64_t* nentries,
                             int64_t* nterms, double* flops)
 {
+        oracle_heff_block_stats(h, npairs, nentries, nterms, flops, NULL);
+}
+
+/* As oracle_heff_plan_stats, and the flops of every output block in
+ * block_flops[nrblocks] when it is not NULL. */
+void oracle_heff_block_stats(const oracle_heff* h, int64_t* npairs, int64_t* nentries,
+                             int64_t* nterms, double* flops, double* block_flops)
+{
         int64_t np = 0, ne = 0, nt = 0;
         double fl = 0;
         for (int i = 0; i < h->nrblocks; ++i) {
+                const double fl_before = fl;
                 for (int j = 0; j < h->nr_oldsb[i]; ++j) {
                         const struct ontom* t = &h->ntom[i][j];
                         ++np;
@@ -819,6 +839,7 @@ void oracle_heff_plan_stats(const oracle_heff* h, int64_t* npairs, int64_t* nent
                                 }
                         }
                 }
+                if (block_flops) block_flops[i] = fl - fl_before;
         }
         if (npairs) *npairs = np;
         if (nentries) *nentries = ne;

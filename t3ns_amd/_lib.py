@@ -75,6 +75,9 @@ def lib():
                                         C.POINTER(SiteDesc), c_int_p, c_dbl_p, c_dbl_p]),
         "t3ns_ctx_create": (C.c_int, [C.POINTER(P), C.c_char_p, C.c_char_p, c_dbl_p, C.c_int,
                                       c_int_p, c_int_p, C.c_int, C.c_int, C.c_int]),
+        "t3ns_ctx_create_qc": (C.c_int, [C.POINTER(P), C.c_char_p, C.c_char_p, c_int_p, c_int_p,
+                                         C.c_int, C.c_int, C.c_int]),
+        "t3ns_ctx_sweep_steps": (C.c_int, [P, C.c_int, c_int_p, c_int_p, C.c_int]),
         "t3ns_ctx_destroy": (C.c_int, [P]),
         "t3ns_ctx_info": (C.c_int, [P, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p]),
         "t3ns_ctx_bond": (C.c_int, [P, C.c_int, c_int_p, c_int_p, c_int_p, C.c_int]),

@@ -133,6 +133,42 @@ int t3ns_ctx_create(t3ns_ctx** out, const char* netw_text, const char* model, co
     });
 }
 
+int t3ns_ctx_create_qc(t3ns_ctx** out, const char* netw_text, const char* fcidump_text,
+                       const int* sgs, const int* ts, int nsym, int max_dim, int minocc) {
+    if (!out || !netw_text || !fcidump_text) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        Network n = Network::from_text(netw_text);
+        std::vector<int> vs(sgs, sgs + nsym), vt(ts, ts + nsym);
+        const bool su2 = std::find(vs.begin(), vs.end(), (int)SU2) != vs.end();
+        if (std::find(vs.begin(), vs.end(), (int)SENIORITY) != vs.end())
+            throw std::invalid_argument("QC: seniority symmetry is not supported");
+        auto ctx = std::make_unique<t3ns_ctx>();
+        ctx->c = make_context(std::move(n), make_qc_model(fcidump_text, su2), vs, vt);
+        ctx->c->prepare_bookkeeper(max_dim, true, minocc);
+        *out = ctx.release();
+    });
+}
+
+int t3ns_ctx_sweep_steps(t3ns_ctx* c, int maxsites, int* sites_out, int* nsites_out, int cap) {
+    if (!c || maxsites < 1) return T3NS_ERR_INVALID;
+    int n = 0;
+    const int rc = guarded_host([&] {
+        Network& nw = c->c->netw;
+        nw.reset_sweep_state();
+        StepSpecs sp;
+        while (nw.next_opt_step(maxsites, &sp)) {
+            if (n < cap) {
+                for (int i = 0; i < STEPSPECS_MSITES; ++i)
+                    sites_out[4 * n + i] = i < sp.nr_sites_opt ? sp.sites_opt[i] : -1;
+                nsites_out[n] = sp.nr_sites_opt;
+            }
+            ++n;
+        }
+        nw.reset_sweep_state();
+    });
+    return rc != T3NS_OK ? rc : n;
+}
+
 int t3ns_ctx_destroy(t3ns_ctx* c) {
     delete c;
     return T3NS_OK;

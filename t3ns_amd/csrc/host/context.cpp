@@ -531,6 +531,7 @@ std::unique_ptr<Context> make_context(Network netw, std::unique_ptr<Model> ham,
     }
     ctx->netw = std::move(netw);
     ctx->ham = std::move(ham);
+    ctx->ham->attach(ctx->netw, ctx->sym);
     return ctx;
 }
 

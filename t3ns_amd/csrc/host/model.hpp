@@ -48,11 +48,16 @@ public:
     virtual int trivial_hss() const = 0;
     virtual int hermitian_hss(int hss) const = 0;
     virtual bool mpo_couples_to_singlet(const int mpo[3]) const = 0;
-    virtual int symsec_siteop(int siteop, int orbital) const = 0;
+    // Site operator `siteop` at network site `site`: its MPO sector and its
+    // matrix elements.
+    virtual int symsec_siteop(int siteop, int site) const = 0;
     virtual double el_siteop(int siteop, int braid, int ketid) const = 0;
-    // All (hss_i, hss_j) whose product can give `resulting` at a site.
-    virtual std::vector<std::array<int, 2>> tprods_ham(int resulting, int orbital) const = 0;
+    // All (hss_i, hss_j) whose product can give `resulting` at network site
+    // `site` (src/hamiltonian.c:227-251).
+    virtual std::vector<std::array<int, 2>> tprods_ham(int resulting, int site) const = 0;
     virtual bool consistent_state(const int* ts, const SymInfo& s) const = 0;
+    // Called once the network and symmetries are known.
+    virtual void attach(const Network& n, const SymInfo& s) { (void)n; (void)s; }
     // Raw (unsorted) instruction lists as the model generates them.
     virtual InstructionSet raw_pUpdate(const Network& n, int bond, int is_left) = 0;
     virtual InstructionSet raw_bUpdate(const Network& n, int bond, int is_left) = 0;
@@ -65,5 +70,10 @@ public:
 // (by `bond != 0`, src/instructions.c:204), otherwise by isdmrg.
 std::unique_ptr<Model> make_hubbard_model(double t, double U, bool su2,
                                           bool reference_merge_quirk = false);
+
+// Quantum chemistry model from FCIDUMP text (src/hamiltonian_qc.c,
+// src/opType.c, src/opType_qc.c, src/instructions_qc.c).  su2 selects U1xSU2,
+// otherwise U1xU1; a point group is taken from the symmetry list at attach().
+std::unique_ptr<Model> make_qc_model(const std::string& fcidump_text, bool su2);
 
 }  // namespace t3

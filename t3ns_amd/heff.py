@@ -20,15 +20,26 @@ class Context:
     Replaces make_network + readinteraction + preparebookkeeper of the
     reference's tests (tests/test1.c.in:31-54)."""
 
-    def __init__(self, netw_text, model, params, symmetries, target, max_dim, minocc=2):
+    def __init__(self, netw_text, model, params, symmetries, target, max_dim, minocc=2,
+                 fcidump=None):
+        """model "NN_HUBBARD" takes params (t, U); model "QC" takes the FCIDUMP
+        text in `fcidump` (params unused)."""
         sgs = [_lib.SYMGROUPS[s] if isinstance(s, str) else int(s) for s in symmetries]
         ts = [int(t) for t in target]
-        par = (C.c_double * len(params))(*params)
+        csgs, cts = (C.c_int * len(sgs))(*sgs), (C.c_int * len(ts))(*ts)
         h = C.c_void_p()
-        check(lib().t3ns_ctx_create(C.byref(h), netw_text.encode(), model.encode(), par,
-                                    len(params), (C.c_int * len(sgs))(*sgs),
-                                    (C.c_int * len(ts))(*ts), len(sgs), int(max_dim), int(minocc)),
-              "t3ns_ctx_create")
+        if model == "QC":
+            if fcidump is None:
+                raise ValueError("model QC needs the FCIDUMP text")
+            check(lib().t3ns_ctx_create_qc(C.byref(h), netw_text.encode(), fcidump.encode(),
+                                           csgs, cts, len(sgs), int(max_dim), int(minocc)),
+                  "t3ns_ctx_create_qc")
+        else:
+            par = (C.c_double * max(len(params), 1))(*params)
+            check(lib().t3ns_ctx_create(C.byref(h), netw_text.encode(), model.encode(), par,
+                                        len(params), csgs, cts, len(sgs), int(max_dim),
+                                        int(minocc)),
+                  "t3ns_ctx_create")
         self._h = h
 
     @property
@@ -48,6 +59,16 @@ class Context:
         buf = (C.c_int * max(n.value, 1))()
         check(lib().t3ns_ctx_bond(self._h, bond, C.byref(n), C.byref(tot), buf, n.value))
         return list(buf)[:n.value]
+
+    def sweep_steps(self, maxsites):
+        """Site lists of the optimisation steps of one sweep."""
+        n = lib().t3ns_ctx_sweep_steps(self._h, int(maxsites), None, None, 0)
+        if n < 0:
+            check(n, "t3ns_ctx_sweep_steps")
+        sites = (C.c_int * (4 * max(n, 1)))()
+        ns = (C.c_int * max(n, 1))()
+        lib().t3ns_ctx_sweep_steps(self._h, int(maxsites), sites, ns, n)
+        return [list(sites[4 * i:4 * i + ns[i]]) for i in range(n)]
 
     def close(self):
         if self._h:

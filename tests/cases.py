@@ -14,3 +14,18 @@ HEFF_CASES = [
 
 def case_id(c):
     return "%s-%s-D%d-%s" % (c[0], "x".join(c[1]), c[3], "_".join(map(str, c[4])))
+
+
+# Quantum-chemistry cases on the reference's own test inputs
+# (tests/fcidumps/N2.STO3G.FCIDUMP with tests/networks/10_T3NS.netw and
+# 10_DMRG.netw, symmetry sets of tests/test1.c.in:34-41).
+QC_FCIDUMP = "fcidumps/N2.STO3G.FCIDUMP"
+QC_CASES = [
+    ("ref_networks/10_T3NS", ["U1", "U1"], [7, 7], 64, [0, 1, 2, 3]),
+    ("ref_networks/10_T3NS", ["U1", "SU2"], [14, 0], 64, [0, 1, 2, 3]),
+    ("ref_networks/10_T3NS", ["U1", "SU2", "D2h"], [14, 0, 0], 64, [2, 3]),
+    ("ref_networks/10_T3NS", ["U1", "U1", "D2h"], [7, 7, 0], 64, [1, 2]),
+    ("ref_networks/10_T3NS", ["U1", "SU2", "D2h"], [14, 0, 0], 64, [5, 6, 7]),
+    ("ref_networks/10_DMRG", ["U1", "SU2", "D2h"], [14, 0, 0], 64, [4, 5]),
+    ("ref_networks/10_DMRG", ["U1", "U1"], [7, 7], 48, [2, 3]),
+]

@@ -8,6 +8,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 NETW = os.path.join(ROOT, "tests", "golden", "networks")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
 def pytest_configure(config):
@@ -15,7 +16,10 @@ def pytest_configure(config):
 
 
 def netw_text(name):
-    with open(os.path.join(NETW, name + ".netw")) as f:
+    """Network fixture: a bare name is a makenetwork.py output under
+    tests/golden/networks, a path is relative to tests/golden."""
+    path = os.path.join(GOLDEN, name) if "/" in name else os.path.join(NETW, name)
+    with open(path + ".netw") as f:
         return f.read()
 
 
