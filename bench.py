@@ -137,6 +137,9 @@ def main():
         wall = time.perf_counter() - w0
         ms = ev0.elapsed_time(ev1) / args.steps
         st = h.stats()
+        ks = h.kernel_stats()
+        print("# %s: %s" % (prec, " ".join("%s=%.4g" % kv for kv in {**st, **ks}.items())),
+              file=sys.stderr, flush=True)
         ys[prec] = dy.cpu().numpy()
         results[prec] = {"ms_per_matvec": ms, "matvec_per_s": 1000.0 / ms,
                          "wall_s": wall, "init_s": t_init, "flops": st["flops"],

@@ -125,6 +125,14 @@ int64_t t3ns_heff_size(const t3ns_heff* h);
 int t3ns_heff_stats(const t3ns_heff* h, int64_t* nblocks, int64_t* npairs, int64_t* nterms,
                     double* flops);
 
+/* Work-list statistics of the device kernels, out[0..n): chunks, runs,
+ * pairs whose input block is read from global memory, pairs whose stage
+ * outputs use global scratch, matrix-core steps per matvec (total and per
+ * stage A/B/C), global scratch elements per workgroup, work units, output
+ * blocks split over several units.  Returns the number of statistics
+ * available. */
+int t3ns_heff_kernel_stats(const t3ns_heff* h, double* out, int n);
+
 /* The dispatch table the reference builds in its first matvec
  * (struct secondrun, include/Heff.h:36-51; src/Heff.c:815-914), ordered by
  * output block, then pair, then MPO entry.  Per entry 7 ints (newsb, oldsb,
@@ -137,6 +145,13 @@ int64_t t3ns_heff_plan_table(const t3ns_heff* h, int* ints, double* prefs);
  * status. */
 int64_t t3ns_plan_table(const t3ns_heff_desc* desc, const t3ns_rops_desc* ops,
                         const t3ns_site_desc* site, int* ints, double* prefs, double* flops);
+
+/* Leg dimensions of every block of the multisite tensor in operator-leg
+ * order (dims[3*b + leg]; leg 2 is 1 for two-site DMRG objects), as
+ * fill_indexes derives them (src/Heff.c:695-739).  Returns the block count
+ * or a negative status. */
+int t3ns_plan_block_dims(const t3ns_heff_desc* desc, const t3ns_rops_desc* ops,
+                         const t3ns_site_desc* site, int* dims);
 
 #ifdef __cplusplus
 }

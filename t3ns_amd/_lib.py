@@ -71,8 +71,11 @@ def lib():
         "t3ns_heff_size": (C.c_int64, [P]),
         "t3ns_heff_stats": (C.c_int, [P, c_i64_p, c_i64_p, c_i64_p, c_dbl_p]),
         "t3ns_heff_plan_table": (C.c_int64, [P, c_int_p, c_dbl_p]),
+        "t3ns_heff_kernel_stats": (C.c_int, [P, c_dbl_p, C.c_int]),
         "t3ns_plan_table": (C.c_int64, [C.POINTER(HeffDesc), C.POINTER(RopsDesc),
                                         C.POINTER(SiteDesc), c_int_p, c_dbl_p, c_dbl_p]),
+        "t3ns_plan_block_dims": (C.c_int, [C.POINTER(HeffDesc), C.POINTER(RopsDesc),
+                                           C.POINTER(SiteDesc), c_int_p]),
         "t3ns_ctx_create": (C.c_int, [C.POINTER(P), C.c_char_p, C.c_char_p, c_dbl_p, C.c_int,
                                       c_int_p, c_int_p, C.c_int, C.c_int, C.c_int]),
         "t3ns_ctx_create_qc": (C.c_int, [C.POINTER(P), C.c_char_p, C.c_char_p, c_int_p, c_int_p,

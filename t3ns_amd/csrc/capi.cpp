@@ -262,9 +262,36 @@ int t3ns_heff_stats(const t3ns_heff* h, int64_t* nblocks, int64_t* npairs, int64
     return T3NS_OK;
 }
 
+int t3ns_heff_kernel_stats(const t3ns_heff* h, double* out, int n) {
+    if (!h || !out) return fail(T3NS_ERR_INVALID, "null argument");
+    const HeffStats& s = h->dev->stats();
+    const double v[] = {(double)s.nchunks, (double)s.nruns, (double)s.pairs_global_x,
+                        (double)s.pairs_global_w, s.mfma_steps, s.stage_steps[0],
+                        s.stage_steps[1], s.stage_steps[2], (double)s.scratch_per_wg,
+                        (double)s.nunits, (double)s.nsplit_blocks};
+    const int m = (int)(sizeof(v) / sizeof(v[0]));
+    for (int i = 0; i < n && i < m; ++i) out[i] = v[i];
+    return m;
+}
+
 int64_t t3ns_heff_plan_table(const t3ns_heff* h, int* ints, double* prefs) {
     if (!h) return -1;
     return write_plan_table(h->dev->plan(), ints, prefs);
+}
+
+int t3ns_plan_block_dims(const t3ns_heff_desc* d, const t3ns_rops_desc* ops,
+                         const t3ns_site_desc* site, int* dims) {
+    if (!d || !ops || !site || !dims) return fail(T3NS_ERR_INVALID, "null argument");
+    int n = -1;
+    const int rc = guarded([&] {
+        HeffHost host;
+        convert_heff_input(&host, d, ops, site);
+        HeffPlan P = build_heff_plan(host.in);
+        for (int b = 0; b < P.nrblocks; ++b)
+            for (int k = 0; k < 3; ++k) dims[3 * b + k] = P.dims[b][k];
+        n = P.nrblocks;
+    });
+    return rc != T3NS_OK ? rc : n;
 }
 
 int64_t t3ns_plan_table(const t3ns_heff_desc* d, const t3ns_rops_desc* ops,

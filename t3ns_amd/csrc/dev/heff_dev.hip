@@ -1,5 +1,6 @@
 // Device effective Hamiltonian: work-list construction and kernels.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 #include <stdexcept>
@@ -9,107 +10,385 @@
 namespace t3 {
 
 namespace {
-constexpr int kThreads = 256;
-__constant__ int kOrd[6][3] = {{0, 1, 2}, {1, 0, 2}, {0, 2, 1}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+constexpr int kThreads = 512;  // 8 waves
+constexpr int kWaves = kThreads / 64;
+constexpr int kDiagThreads = 256;
+// LDS elements of the input block, the first-stage outputs, the
+// second-stage outputs of one chunk and the stage-C partial tiles.
+constexpr int kCapX64 = 4096, kCapW164 = 6144, kCapW264 = 6144;
+constexpr int kCapX32 = 8192, kCapW132 = 12288, kCapW232 = 12288;
+constexpr int kPartial = kWaves * 256;
+constexpr int kMaxChunkTerms = 64;
+// Global scratch per workgroup for pairs whose stage outputs exceed LDS.
+constexpr int64_t kGlobalChunkElems = 1 << 18;
 const int hOrd[6][3] = {{0, 1, 2}, {1, 0, 2}, {0, 2, 1}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 
-// out = O x_p in  (or out += alpha * O x_p in): mode-p product of a
-// three-leg block (memory dims a0,a1,a2, first fastest) with a
-// column-major (dn x a_p) operator block.
+// 16x16x4 matrix-core step, D = A B + C, one wave: lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[row(l, v)][l % 16],
+// v = 0..3, with row = 4 (l / 16) + v for FP32 and l / 16 + 4 v for FP64.
 template <typename T>
-__device__ __forceinline__ void mode_product(const T* __restrict__ in, T* __restrict__ out,
-                                             const T* __restrict__ O, int a0, int a1, int a2,
-                                             int p, int dn, T alpha, bool acc) {
-    int o0 = a0, o1 = a1, o2 = a2;
-    if (p == 0) o0 = dn;
-    else if (p == 1) o1 = dn;
-    else o2 = dn;
-    const int total = o0 * o1 * o2;
-    const int s1 = a0, s2 = a0 * a1;
-    const int K = p == 0 ? a0 : (p == 1 ? a1 : a2);
-    const int st = p == 0 ? 1 : (p == 1 ? s1 : s2);
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int i0 = idx % o0;
-        const int r = idx / o0;
-        const int i1 = r % o1;
-        const int i2 = r / o1;
-        int ip;
-        const T* src;
-        if (p == 0) {
-            ip = i0;
-            src = in + i1 * s1 + i2 * s2;
-        } else if (p == 1) {
-            ip = i1;
-            src = in + i0 + i2 * s2;
-        } else {
-            ip = i2;
-            src = in + i0 + i1 * s1;
-        }
-        T s = 0;
-        for (int k = 0; k < K; ++k) s += O[ip + k * dn] * src[k * st];
-        if (acc) out[idx] += alpha * s;
-        else out[idx] = s;
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef double V __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ V run(double a, double b, V c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <>
+struct Mfma<float> {
+    typedef float V __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ V run(float a, float b, V c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+// Geometry of a mode-p product of a three-leg block S (memory dims s0, s1,
+// s2; first fastest) with a column-major (dn x s_p) operator block O:
+//     Z[f, k'] = sum_k S[f, k] O[k', k],   f = (i_lo, i_hi) a fiber,
+// with S[f, k] at i_lo + k lo + i_hi lo K and Z[f, k'] at i_lo + k' lo +
+// i_hi lo dn.  Tiles are 16 fibers x 16 columns.
+struct Geo {
+    int lo, K, F, dn, tf, tiles, ksteps;
+};
+__device__ __forceinline__ Geo make_geo(int s0, int s1, int s2, int p, int dn) {
+    Geo g;
+    g.lo = p == 0 ? 1 : (p == 1 ? s0 : s0 * s1);
+    g.K = p == 0 ? s0 : (p == 1 ? s1 : s2);
+    const int hi = p == 0 ? s1 * s2 : (p == 1 ? s2 : 1);
+    g.F = g.lo * hi;
+    g.dn = dn;
+    g.tf = (g.F + 15) >> 4;
+    g.tiles = g.tf * ((dn + 15) >> 4);
+    g.ksteps = (g.K + 3) >> 2;
+    return g;
+}
+
+// Per-lane offsets of one tile, computed once and reused for every operator
+// and term that goes through the tile.
+struct TileLane {
+    int soff, ooff;  // A and B operand offsets at this lane's first k
+    int zoff[4];     // output offsets of the lane's four D values (-1: none)
+    bool fv, nv;
+};
+template <typename T>
+__device__ __forceinline__ TileLane tile_lane(const Geo& g, int tile) {
+    TileLane L;
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 15, kk = lane >> 4;
+    const int tf = tile % g.tf, tn = tile / g.tf;
+    const int f = tf * 16 + i;
+    L.fv = f < g.F;
+    const int fc = L.fv ? f : 0;
+    const int ilo = fc % g.lo, ihi = fc / g.lo;
+    L.soff = ilo + ihi * g.lo * g.K + kk * g.lo;
+    const int n = tn * 16 + i;
+    L.nv = n < g.dn;
+    L.ooff = (L.nv ? n : 0) + kk * g.dn;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int fr = tf * 16 + Mfma<T>::row(lane, v);
+        L.zoff[v] = (L.nv && fr < g.F) ? (fr % g.lo) + n * g.lo + (fr / g.lo) * g.lo * g.dn : -1;
+    }
+    return L;
+}
+
+// acc += (scale S)[tile rows, :] O[tile cols, :]^T over the whole K.
+template <typename T>
+__device__ __forceinline__ void tile_mac(const T* __restrict__ S, const T* __restrict__ O,
+                                         const Geo& g, const TileLane& L, T scale,
+                                         typename Mfma<T>::V& acc) {
+    const T* sp = S + L.soff;
+    const T* op = O + L.ooff;
+    const int sst = 4 * g.lo, ost = 4 * g.dn;
+    int k = (threadIdx.x & 63) >> 4;
+#pragma unroll 4
+    for (int s = 0; s < g.ksteps; ++s) {
+        const bool kv = k < g.K;
+        const T av = (L.fv && kv) ? *sp : (T)0;
+        const T bv = (L.nv && kv) ? *op : (T)0;
+        acc = Mfma<T>::run(scale * av, bv, acc);
+        sp += sst;
+        op += ost;
+        k += 4;
     }
 }
 
-// One workgroup owns one output block at a time and accumulates every
-// contributing term into it: no atomics, deterministic summation order.
+// Operands of up to kFragSteps k-steps of one tile, loaded ahead of their
+// matrix-core steps so that the loads of the next operator overlap the
+// arithmetic of the current one.
+constexpr int kFragSteps = 6;
 template <typename T>
-__global__ void __launch_bounds__(kThreads)
+struct Frag {
+    T a[kFragSteps], b[kFragSteps];
+};
+template <typename T>
+__device__ __forceinline__ void frag_load(const T* __restrict__ S, const T* __restrict__ O,
+                                          const Geo& g, const TileLane& L, Frag<T>& fr) {
+    const int kk = (threadIdx.x & 63) >> 4;
+    const T* sp = S + L.soff;
+    const T* op = O + L.ooff;
+#pragma unroll
+    for (int u = 0; u < kFragSteps; ++u) {
+        const bool kv = 4 * u + kk < g.K;
+        fr.a[u] = (L.fv && kv) ? sp[4 * u * g.lo] : (T)0;
+        fr.b[u] = (L.nv && kv) ? op[4 * u * g.dn] : (T)0;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void frag_mma(const Frag<T>& fr, const Geo& g, T scale,
+                                         typename Mfma<T>::V& acc) {
+#pragma unroll
+    for (int u = 0; u < kFragSteps; ++u)
+        if (u < g.ksteps) acc = Mfma<T>::run(scale * fr.a[u], fr.b[u], acc);
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_put(T* __restrict__ Z, const TileLane& L,
+                                         const typename Mfma<T>::V& acc, bool add) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        if (L.zoff[v] >= 0) {
+            if (add) Z[L.zoff[v]] += acc[v];
+            else Z[L.zoff[v]] = acc[v];
+        }
+    }
+}
+
+// All chunks of one pair.  Stage A: first-stage outputs of the chunk's
+// distinct first operators; stage B: second-stage output of every term;
+// stage C: Y += sum over the chunk's terms of alpha_t O_c x_c W2_t, summed
+// in matrix-core accumulators per output tile.  Waves take (tile, group)
+// tasks: a wave keeps its tile's lane offsets and walks the operators or
+// terms of its group; when a geometry has fewer tiles than waves, items
+// are split over groups (stage C partial tiles are then reduced in a fixed
+// order).
+// Operator offsets, prefactors and first-stage slots of one chunk, staged
+// in LDS at the start of the chunk.
+struct ChunkRec {
+    int64_t opa[kMaxChunkTerms], opb[kMaxChunkTerms], opc[kMaxChunkTerms];
+    double alpha[kMaxChunkTerms];
+    int slot[kMaxChunkTerms];
+};
+
+template <typename T, int NT>
+__device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
+                                            const DevChunk* __restrict__ chunks,
+                                            const int* __restrict__ runs,
+                                            const DevTerm* __restrict__ terms,
+                                            const T* const* opb, const T* X, T* W1, T* W2, T* Y,
+                                            T* part, ChunkRec* rec, int isdmrg, const int* map,
+                                            const int* pos) {
+    typedef typename Mfma<T>::V V;
+    constexpr int NW = NT / 64;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int a = P.ord[0], b = P.ord[1], c = isdmrg ? P.ord[1] : P.ord[2];
+    int cur[3] = {P.d[map[0]], P.d[map[1]], P.d[map[2]]};
+    const Geo gA = make_geo(cur[0], cur[1], cur[2], pos[a], bd[a]);
+    cur[pos[a]] = bd[a];
+    Geo gB = gA;
+    if (!isdmrg) {
+        gB = make_geo(cur[0], cur[1], cur[2], pos[b], bd[b]);
+        cur[pos[b]] = bd[b];
+    }
+    const Geo gC = make_geo(cur[0], cur[1], cur[2], pos[c], bd[c]);
+    const int GA = gA.tiles >= NW ? 1 : NW / gA.tiles;
+    const int GB = gB.tiles >= NW ? 1 : NW / gB.tiles;
+    const int GC = gC.tiles >= NW ? 1 : NW / gC.tiles;
+    const T* opa = opb[a];
+    const T* opbb = opb[b];
+    const T* opc = opb[c];
+    for (int ci = P.chunk_begin; ci < P.chunk_end; ++ci) {
+        const DevChunk C = chunks[ci];
+        const int nr = C.run_end - C.run_begin;
+        const int nt = C.term_end - C.term_begin;
+        // chunk records into LDS
+        if (threadIdx.x < nt) {
+            const DevTerm& tm = terms[C.term_begin + threadIdx.x];
+            rec->opb[threadIdx.x] = tm.op[b];
+            rec->opc[threadIdx.x] = tm.op[c];
+            rec->alpha[threadIdx.x] = tm.alpha;
+            rec->slot[threadIdx.x] = tm.slot;
+        }
+        if (threadIdx.x < nr) rec->opa[threadIdx.x] = terms[runs[C.run_begin + threadIdx.x]].op[a];
+        __syncthreads();
+        {  // stage A
+            for (int task = wave; task < gA.tiles * GA; task += NW) {
+                const int tile = task % gA.tiles, grp = task / gA.tiles;
+                const TileLane L = tile_lane<T>(gA, tile);
+                if (gA.ksteps <= kFragSteps) {
+                    Frag<T> cur, nxt;
+                    if (grp < nr) frag_load<T>(X, opa + rec->opa[grp], gA, L, cur);
+                    for (int j = grp; j < nr; j += GA) {
+                        if (j + GA < nr) frag_load<T>(X, opa + rec->opa[j + GA], gA, L, nxt);
+                        V acc = {0, 0, 0, 0};
+                        frag_mma<T>(cur, gA, (T)1, acc);
+                        tile_put<T>(W1 + j * P.w1, L, acc, false);
+                        cur = nxt;
+                    }
+                } else {
+                    for (int j = grp; j < nr; j += GA) {
+                        V acc = {0, 0, 0, 0};
+                        tile_mac<T>(X, opa + rec->opa[j], gA, L, (T)1, acc);
+                        tile_put<T>(W1 + j * P.w1, L, acc, false);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (!isdmrg) {  // stage B
+            for (int task = wave; task < gB.tiles * GB; task += NW) {
+                const int tile = task % gB.tiles, grp = task / gB.tiles;
+                const TileLane L = tile_lane<T>(gB, tile);
+                if (gB.ksteps <= kFragSteps) {
+                    Frag<T> cur, nxt;
+                    if (grp < nt)
+                        frag_load<T>(W1 + rec->slot[grp] * P.w1, opbb + rec->opb[grp], gB, L, cur);
+                    for (int t = grp; t < nt; t += GB) {
+                        if (t + GB < nt)
+                            frag_load<T>(W1 + rec->slot[t + GB] * P.w1, opbb + rec->opb[t + GB], gB,
+                                         L, nxt);
+                        V acc = {0, 0, 0, 0};
+                        frag_mma<T>(cur, gB, (T)1, acc);
+                        tile_put<T>(W2 + t * P.w2, L, acc, false);
+                        cur = nxt;
+                    }
+                } else {
+                    for (int t = grp; t < nt; t += GB) {
+                        V acc = {0, 0, 0, 0};
+                        tile_mac<T>(W1 + rec->slot[t] * P.w1, opbb + rec->opb[t], gB, L, (T)1, acc);
+                        tile_put<T>(W2 + t * P.w2, L, acc, false);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // stage C
+        for (int task = wave; task < gC.tiles * GC; task += NW) {
+            const int tile = task % gC.tiles, grp = task / gC.tiles;
+            const TileLane L = tile_lane<T>(gC, tile);
+            V acc = {0, 0, 0, 0};
+            if (gC.ksteps <= kFragSteps) {
+                Frag<T> cur, nxt;
+                if (grp < nt) {
+                    const T* S = isdmrg ? W1 + rec->slot[grp] * P.w1 : W2 + grp * P.w2;
+                    frag_load<T>(S, opc + rec->opc[grp], gC, L, cur);
+                }
+                for (int t = grp; t < nt; t += GC) {
+                    if (t + GC < nt) {
+                        const int u = t + GC;
+                        const T* S = isdmrg ? W1 + rec->slot[u] * P.w1 : W2 + u * P.w2;
+                        frag_load<T>(S, opc + rec->opc[u], gC, L, nxt);
+                    }
+                    frag_mma<T>(cur, gC, (T)rec->alpha[t], acc);
+                    cur = nxt;
+                }
+            } else {
+                for (int t = grp; t < nt; t += GC) {
+                    const T* S = isdmrg ? W1 + rec->slot[t] * P.w1 : W2 + t * P.w2;
+                    tile_mac<T>(S, opc + rec->opc[t], gC, L, (T)rec->alpha[t], acc);
+                }
+            }
+            if (GC == 1) {
+                tile_put<T>(Y, L, acc, true);
+            } else {
+                T* pp = part + ((grp * gC.tiles + tile) * 64 + lane) * 4;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) pp[v] = acc[v];
+            }
+        }
+        if (GC > 1) {
+            __syncthreads();
+            for (int tile = wave; tile < gC.tiles; tile += NW) {
+                const TileLane L = tile_lane<T>(gC, tile);
+                V acc = {0, 0, 0, 0};
+                for (int grp = 0; grp < GC; ++grp) {
+                    const T* pp = part + ((grp * gC.tiles + tile) * 64 + lane) * 4;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[v] += pp[v];
+                }
+                tile_put<T>(Y, L, acc, true);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Out-of-line instance for pairs whose buffers do not all fit in LDS (rare;
+// generic pointers), keeping the register budget of the common path.
+template <typename T, int NT>
+__device__ __noinline__ void pair_chunks_generic(const DevPair& P, const int* bd,
+                                                 const DevChunk* __restrict__ chunks,
+                                                 const int* __restrict__ runs,
+                                                 const DevTerm* __restrict__ terms,
+                                                 const T* const* opb, const T* X, T* W1, T* W2,
+                                                 T* Y, T* part, ChunkRec* rec, int isdmrg,
+                                                 const int* map, const int* pos) {
+    pair_chunks<T, NT>(P, bd, chunks, runs, terms, opb, X, W1, W2, Y, part, rec, isdmrg, map, pos);
+}
+
+// One workgroup owns one output block at a time (largest first) and
+// accumulates every contributing term into it: no atomics, a fixed
+// summation order.  Stage buffers live in LDS when the pair's chunks fit,
+// else in the workgroup's global scratch region.
+template <typename T, int CX, int C1, int C2, int NT>
+__global__ void __launch_bounds__(NT)
     heff_matvec_kernel(const DevBlock* __restrict__ blocks, int nblocks,
-                       const DevPair* __restrict__ pairs, const DevTerm* __restrict__ terms,
+                       const DevPair* __restrict__ pairs, const DevChunk* __restrict__ chunks,
+                       const int* __restrict__ runs, const DevTerm* __restrict__ terms,
                        const T* __restrict__ op0, const T* __restrict__ op1,
                        const T* __restrict__ op2, const T* __restrict__ x, T* __restrict__ y,
-                       T* __restrict__ scratch, int64_t scratch_stride, int isdmrg, int m0, int m1,
-                       int m2) {
-    T* w1base = scratch + (int64_t)blockIdx.x * scratch_stride;
+                       T* __restrict__ scratch, int64_t scratch_stride, int* __restrict__ next,
+                       T* __restrict__ partial, int isdmrg, int m0, int m1, int m2) {
+    __shared__ T lds[CX + C1 + C2 + kPartial];
+    __shared__ ChunkRec rec;
+    __shared__ int cur_block;
+    T* Xs = lds;
+    T* W1s = lds + CX;
+    T* W2s = lds + CX + C1;
+    T* part = lds + CX + C1 + C2;
+    T* gw1 = scratch + (int64_t)blockIdx.x * scratch_stride;
+    T* gw2 = gw1 + scratch_stride / 2;
     const T* opb[3] = {op0, op1, op2};
     const int map[3] = {m0, m1, m2};
     int pos[3];
     pos[m0] = 0;
     pos[m1] = 1;
     pos[m2] = 2;
-    for (int bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
+    // Blocks are taken from a shared counter in cost order (dynamic
+    // longest-first schedule); every workgroup leaves once it runs past the
+    // end of the list.
+    for (;;) {
+        if (threadIdx.x == 0) cur_block = atomicAdd(next, 1);
+        __syncthreads();
+        const int bi = cur_block;
+        __syncthreads();
+        if (bi >= nblocks) break;
         const DevBlock B = blocks[bi];
-        T* Y = y + B.y_off;
+        T* Y = (B.to_part ? partial : y) + B.y_off;
         for (int pi = B.pair_begin; pi < B.pair_end; ++pi) {
             const DevPair P = pairs[pi];
             const T* X = x + P.x_off;
-            const int nst = isdmrg ? 2 : 3;
-            // memory-order dims before each stage
-            int dm[3] = {P.d[map[0]], P.d[map[1]], P.d[map[2]]};
-            int dims_after1[3], sz1 = 0;
-            {
-                const int o = isdmrg ? (P.order == 0 ? 0 : 1) : kOrd[P.order][0];
-                for (int k = 0; k < 3; ++k) dims_after1[k] = dm[k];
-                dims_after1[pos[o]] = B.d[o];
-                sz1 = dims_after1[0] * dims_after1[1] * dims_after1[2];
+            if (P.xlds) {
+                const int xsz = P.d[0] * P.d[1] * P.d[2];
+                for (int i = threadIdx.x; i < xsz; i += NT) Xs[i] = X[i];
+                __syncthreads();
             }
-            T* w1 = w1base;
-            T* w2 = w1base + sz1;
-            for (int ti = P.term_begin; ti < P.term_end; ++ti) {
-                const DevTerm tm = terms[ti];
-                const T alpha = (T)tm.alpha;
-                int cur[3] = {dm[0], dm[1], dm[2]};
-                const T* src = X;
-                for (int s = 0; s < nst; ++s) {
-                    const int o = isdmrg ? (P.order == 0 ? s : 1 - s) : kOrd[P.order][s];
-                    const int p = pos[o];
-                    const T* O = opb[o] + tm.op[o];
-                    const bool last = s == nst - 1;
-                    T* dst = last ? Y : (s == 0 ? w1 : w2);
-                    mode_product<T>(src, dst, O, cur[0], cur[1], cur[2], p, B.d[o], alpha, last);
-                    __syncthreads();
-                    cur[p] = B.d[o];
-                    src = dst;
-                }
-            }
-        }
-    }
+            if (P.xlds && P.wlds)
+                pair_chunks<T, NT>(P, B.d, chunks, runs, terms, opb, Xs, W1s, W2s, Y, part, &rec,
+                                   isdmrg, map, pos);
+            else
+                pair_chunks_generic<T, NT>(P, B.d, chunks, runs, terms, opb, P.xlds ? Xs : X,
+                                           P.wlds ? W1s : gw1, P.wlds ? W2s : gw2, Y, part, &rec,
+                                           isdmrg, map, pos);
+        }    }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kDiagThreads)
     heff_diag_kernel(const DevDiagBlock* __restrict__ blocks, int nblocks,
                      const DevTerm* __restrict__ terms, const T* __restrict__ op0,
                      const T* __restrict__ op1, const T* __restrict__ op2, double* __restrict__ diag,
@@ -150,6 +429,21 @@ __global__ void convert_kernel(const A* __restrict__ src, B* __restrict__ dst, i
         dst[i] = (B)src[i];
 }
 
+// Output blocks split over several work units: sum their partial copies
+// in a fixed order.
+template <typename T>
+__global__ void split_reduce_kernel(const DevSplit* __restrict__ splits, int nsplits,
+                                    const T* __restrict__ partial, T* __restrict__ y) {
+    for (int si = blockIdx.x; si < nsplits; si += gridDim.x) {
+        const DevSplit S = splits[si];
+        for (int i = threadIdx.x; i < S.size; i += blockDim.x) {
+            T acc = 0;
+            for (int p = 0; p < S.nparts; ++p) acc += partial[S.part_off + (int64_t)p * S.size + i];
+            y[S.y_off + i] = acc;
+        }
+    }
+}
+
 inline int grid_for(int64_t n) {
     int64_t g = (n + 255) / 256;
     if (g > 8192) g = 8192;
@@ -168,11 +462,29 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     const InstructionSet& is = *in.iset;
 
     // Flatten the plan into work lists.
+    const bool f32 = prec_ == PREC_FP32;
+    const int capx = f32 ? kCapX32 : kCapX64;
+    const int cap1 = f32 ? kCapW132 : kCapW164;
+    const int cap2 = f32 ? kCapW232 : kCapW264;
+    // Global chunk capacity: at least one stage output of the largest pair.
+    int64_t gcap = kGlobalChunkElems;
+    for (int sb = 0; sb < plan_.nrblocks; ++sb)
+        for (int pi = plan_.pair_begin[sb]; pi < plan_.pair_begin[sb + 1]; ++pi) {
+            const PlanPair& pp = plan_.pairs[pi];
+            int w[3] = {plan_.dims[pp.oldsb][0], plan_.dims[pp.oldsb][1], plan_.dims[pp.oldsb][2]};
+            for (int s = 0; s < (isdmrg_ ? 1 : 2); ++s) {
+                const int o = isdmrg_ ? (pp.bestorder == 0 ? 0 : 1) : hOrd[pp.bestorder][s];
+                w[o] = plan_.dims[sb][o];
+                gcap = std::max<int64_t>(gcap, (int64_t)w[0] * w[1] * w[2]);
+            }
+        }
     std::vector<DevPair> hp;
+    std::vector<DevChunk> hc;
+    std::vector<int> hr;
     std::vector<DevTerm> ht;
     std::vector<DevBlock> hb;
     std::vector<double> cost(plan_.nrblocks, 0.0);
-    int64_t maxscratch = 1;
+    std::vector<double> pcost;
     for (int sb = 0; sb < plan_.nrblocks; ++sb) {
         DevBlock B{};
         B.y_off = plan_.beginblock[sb];
@@ -183,8 +495,14 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
             DevPair P{};
             P.x_off = plan_.beginblock[pp.oldsb];
             for (int k = 0; k < 3; ++k) P.d[k] = plan_.dims[pp.oldsb][k];
-            P.order = pp.bestorder;
-            P.term_begin = (int)ht.size();
+            if (isdmrg_) {
+                P.ord[0] = pp.bestorder == 0 ? 0 : 1;
+                P.ord[1] = 1 - P.ord[0];
+                P.ord[2] = 2;
+            } else {
+                for (int k = 0; k < 3; ++k) P.ord[k] = hOrd[pp.bestorder][k];
+            }
+            const size_t tb = ht.size();
             for (const PlanEntry& e : pp.entries) {
                 for (int i = is.MPOc_beg[e.mpo_group]; i < is.MPOc_beg[e.mpo_group + 1]; ++i) {
                     const Instruction& ins = is.instr[i];
@@ -204,31 +522,166 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
                     ht.push_back(t);
                 }
             }
-            P.term_end = (int)ht.size();
-            if (P.term_end == P.term_begin) continue;
-            // scratch: stage-1 and stage-2 outputs
+            if ((int64_t)ht.size() > INT32_MAX)
+                throw std::runtime_error("HeffDevice: more than 2^31 terms");
+            if (ht.size() == tb) continue;
+            // Terms sorted by their first-stage operator block: runs of equal
+            // first operators share one first-stage output.
+            const int first = P.ord[0];
+            std::stable_sort(ht.begin() + tb, ht.end(), [first](const DevTerm& a, const DevTerm& b) {
+                return a.op[first] < b.op[first];
+            });
+            // stage output sizes
             int w[3] = {P.d[0], P.d[1], P.d[2]};
-            int64_t need = 0;
             for (int s = 0; s < (isdmrg_ ? 1 : 2); ++s) {
-                const int o = isdmrg_ ? (P.order == 0 ? 0 : 1) : hOrd[P.order][s];
+                const int o = P.ord[s];
                 w[o] = B.d[o];
-                need += (int64_t)w[0] * w[1] * w[2];
+                const int64_t sz = (int64_t)w[0] * w[1] * w[2];
+                if (s == 0) P.w1 = (int)sz;
+                else P.w2 = (int)sz;
             }
-            maxscratch = std::max(maxscratch, need);
-            cost[sb] += (P.term_end - P.term_begin) *
-                        heff_term_flops(P.d, B.d, map_, P.order, isdmrg_);
+            const int64_t xsz = (int64_t)P.d[0] * P.d[1] * P.d[2];
+            P.xlds = xsz <= capx;
+            P.wlds = P.w1 <= cap1 && P.w2 <= cap2;
+            const int64_t c1 = P.wlds ? cap1 : gcap, c2 = P.wlds ? cap2 : gcap;
+            // chunks
+            P.chunk_begin = (int)hc.size();
+            DevChunk C{};
+            int nruns = 0, nterms = 0;
+            int64_t prev = -1;
+            auto close = [&](int end) {
+                C.term_end = end;
+                C.run_end = (int)hr.size();
+                hc.push_back(C);
+            };
+            for (size_t t = tb; t < ht.size(); ++t) {
+                const bool newrun = nterms == 0 || ht[t].op[first] != prev;
+                const bool fits = nterms < kMaxChunkTerms &&
+                                  (int64_t)(nruns + newrun) * P.w1 <= c1 &&
+                                  (isdmrg_ || (int64_t)(nterms + 1) * P.w2 <= c2);
+                if (nterms > 0 && !fits) {
+                    close((int)t);
+                    nruns = nterms = 0;
+                }
+                if (nterms == 0) {
+                    C = DevChunk{};
+                    C.term_begin = (int)t;
+                    C.run_begin = (int)hr.size();
+                }
+                if (nterms == 0 || ht[t].op[first] != prev) {
+                    hr.push_back((int)t);
+                    ++nruns;
+                }
+                ht[t].slot = nruns - 1;
+                prev = ht[t].op[first];
+                ++nterms;
+            }
+            close((int)ht.size());
+            P.chunk_end = (int)hc.size();
+            {
+                // matrix-core steps: tiles x ceil(K/4) per stage output
+                int cd[3] = {P.d[0], P.d[1], P.d[2]};
+                double st[3];
+                for (int s2 = 0; s2 < (isdmrg_ ? 2 : 3); ++s2) {
+                    const int o = P.ord[s2];
+                    double F = 1;
+                    for (int k = 0; k < 3; ++k)
+                        if (k != o) F *= cd[k];
+                    st[s2] = std::ceil(F / 16) * std::ceil(B.d[o] / 16.0) * std::ceil(cd[o] / 4.0);
+                    cd[o] = B.d[o];
+                }
+                double nr = 0, ntt = 0;
+                for (int ci = P.chunk_begin; ci < P.chunk_end; ++ci) {
+                    nr += hc[ci].run_end - hc[ci].run_begin;
+                    ntt += hc[ci].term_end - hc[ci].term_begin;
+                }
+                stats_.stage_steps[0] += nr * st[0];
+                stats_.stage_steps[1] += ntt * st[1];
+                if (!isdmrg_) stats_.stage_steps[2] += ntt * st[2];
+                stats_.nruns += (int64_t)nr;
+                stats_.pairs_global_x += !P.xlds;
+                stats_.pairs_global_w += !P.wlds;
+            }
+            {
+                double c = 0;
+                for (int ci = P.chunk_begin; ci < P.chunk_end; ++ci) {
+                    const double nr = hc[ci].run_end - hc[ci].run_begin;
+                    const double nt = hc[ci].term_end - hc[ci].term_begin;
+                    c += nr * xsz * B.d[P.ord[0]] + nt * (double)P.w1 * B.d[P.ord[1]] +
+                         (isdmrg_ ? 0.0 : nt * (double)P.w2 * B.d[P.ord[2]]);
+                }
+                cost[sb] += c;
+                pcost.push_back(c);
+            }
             hp.push_back(P);
         }
         B.pair_end = (int)hp.size();
         hb.push_back(B);
     }
-    // Largest blocks first so the persistent grid drains evenly.
-    std::vector<int> idx(hb.size());
+    const int64_t maxscratch = 2 * gcap;
+    // Work units: output blocks whose cost exceeds a share of the machine
+    // are split into slices of their pairs that accumulate into partial
+    // copies, summed afterwards in a fixed order.
+    int dev = 0, ncu = 256;
+    T3_HIP_CHECK(hipGetDevice(&dev));
+    T3_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    double total = 0;
+    for (double c : cost) total += c;
+    const double unit_cap = std::max(1.0, total / (4.0 * ncu));
+    std::vector<DevBlock> units;
+    std::vector<double> ucost;
+    std::vector<DevSplit> hs;
+    int64_t part_elems = 0;
+    for (const DevBlock& B : hb) {
+        if (B.pair_end == B.pair_begin) continue;
+        const int64_t ysz = (int64_t)B.d[0] * B.d[1] * B.d[2];
+        std::vector<std::pair<int, int>> slices;
+        std::vector<double> scost;
+        double acc = 0;
+        int beg = B.pair_begin;
+        for (int pi = B.pair_begin; pi < B.pair_end; ++pi) {
+            if (pi > beg && acc + pcost[pi] > unit_cap) {
+                slices.push_back({beg, pi});
+                scost.push_back(acc);
+                beg = pi;
+                acc = 0;
+            }
+            acc += pcost[pi];
+        }
+        slices.push_back({beg, B.pair_end});
+        scost.push_back(acc);
+        if (slices.size() == 1) {
+            units.push_back(B);
+            ucost.push_back(scost[0]);
+            continue;
+        }
+        DevSplit S{};
+        S.y_off = B.y_off;
+        S.part_off = part_elems;
+        S.nparts = (int)slices.size();
+        S.size = (int)ysz;
+        hs.push_back(S);
+        for (size_t k = 0; k < slices.size(); ++k) {
+            DevBlock U = B;
+            U.y_off = part_elems + (int64_t)k * ysz;
+            U.pair_begin = slices[k].first;
+            U.pair_end = slices[k].second;
+            U.to_part = 1;
+            units.push_back(U);
+            ucost.push_back(scost[k]);
+        }
+        part_elems += (int64_t)slices.size() * ysz;
+    }
+    // Costliest units first: the dynamic schedule then drains evenly.
+    std::vector<int> idx(units.size());
     std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ucost[a] > ucost[b]; });
     std::vector<DevBlock> hbs;
-    for (int i : idx)
-        if (hb[i].pair_end > hb[i].pair_begin) hbs.push_back(hb[i]);
+    for (int i : idx) hbs.push_back(units[i]);
+    stats_.nunits = (int64_t)hbs.size();
+    stats_.nsplit_blocks = (int64_t)hs.size();
+    nsplits_ = (int)hs.size();
+    part_elems_ = part_elems;
 
     // Diagonal work list.
     std::vector<DiagEntry> de = build_heff_diag_plan(in, plan_);
@@ -271,9 +724,13 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     stats_.size = size_;
     stats_.flops = plan_.flops;
     stats_.scratch_per_wg = maxscratch;
+    stats_.nchunks = (int64_t)hc.size();
+    stats_.mfma_steps = stats_.stage_steps[0] + stats_.stage_steps[1] + stats_.stage_steps[2];
 
     blocks_.upload(hbs, stream);
     pairs_.upload(hp, stream);
+    chunks_.upload(hc, stream);
+    runs_.upload(hr, stream);
     terms_.upload(ht, stream);
     dblocks_.upload(hdb, stream);
     dterms_.upload(hdt, stream);
@@ -295,9 +752,14 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
         x32_.alloc(size_);
         y32_.alloc(size_);
     }
-    nwg_ = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)hbs.size(), 4096));
+    // Persistent grid: one workgroup per CU (the LDS use allows one).
+    nwg_ = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)hbs.size(), ncu));
     if (prec_ == PREC_FP32) scratch32_.alloc((size_t)nwg_ * maxscratch);
     else scratch64_.alloc((size_t)nwg_ * maxscratch);
+    counter_.alloc(1);
+    splits_.upload(hs, stream);
+    if (f32) part32_.alloc(std::max<int64_t>(1, part_elems_));
+    else part64_.alloc(std::max<int64_t>(1, part_elems_));
     T3_HIP_CHECK(hipGetLastError());
     T3_HIP_CHECK(hipStreamSynchronize(stream));
 }
@@ -309,22 +771,39 @@ void HeffDevice::matvec(const double* dvec, double* dres, hipStream_t s) {
     if (prec_ == PREC_FP64) {
         hipLaunchKernelGGL((fill_kernel<double>), dim3(grid_for(size_)), dim3(256), 0, s, dres,
                            size_, 0.0);
+        hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s, counter_.get(), (int64_t)1, 0);
+        if (part_elems_)
+            hipLaunchKernelGGL((fill_kernel<double>), dim3(grid_for(part_elems_)), dim3(256), 0, s,
+                               part64_.get(), part_elems_, 0.0);
         if (nb)
-            hipLaunchKernelGGL((heff_matvec_kernel<double>), dim3(nwg_), dim3(kThreads), 0, s,
-                               blocks_.get(), nb, pairs_.get(), terms_.get(), ops64_[0].get(),
+            hipLaunchKernelGGL((heff_matvec_kernel<double, kCapX64, kCapW164, kCapW264, kThreads>),
+                               dim3(nwg_), dim3(kThreads), 0, s, blocks_.get(), nb, pairs_.get(),
+                               chunks_.get(), runs_.get(), terms_.get(), ops64_[0].get(),
                                ops64_[1].get(), ops64_[2].get(), dvec, dres, scratch64_.get(),
-                               stats_.scratch_per_wg, isdmrg_, map_[0], map_[1], map_[2]);
+                               stats_.scratch_per_wg, counter_.get(), part64_.get(), isdmrg_,
+                               map_[0], map_[1], map_[2]);
+        if (nsplits_)
+            hipLaunchKernelGGL((split_reduce_kernel<double>), dim3(std::min(nsplits_, 4096)),
+                               dim3(256), 0, s, splits_.get(), nsplits_, part64_.get(), dres);
     } else {
         hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(size_)), dim3(256), 0, s,
                            dvec, x32_.get(), size_);
         hipLaunchKernelGGL((fill_kernel<float>), dim3(grid_for(size_)), dim3(256), 0, s,
                            y32_.get(), size_, 0.0f);
+        hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s, counter_.get(), (int64_t)1, 0);
+        if (part_elems_)
+            hipLaunchKernelGGL((fill_kernel<float>), dim3(grid_for(part_elems_)), dim3(256), 0, s,
+                               part32_.get(), part_elems_, 0.0f);
         if (nb)
-            hipLaunchKernelGGL((heff_matvec_kernel<float>), dim3(nwg_), dim3(kThreads), 0, s,
-                               blocks_.get(), nb, pairs_.get(), terms_.get(), ops32_[0].get(),
+            hipLaunchKernelGGL((heff_matvec_kernel<float, kCapX32, kCapW132, kCapW232, kThreads>),
+                               dim3(nwg_), dim3(kThreads), 0, s, blocks_.get(), nb, pairs_.get(),
+                               chunks_.get(), runs_.get(), terms_.get(), ops32_[0].get(),
                                ops32_[1].get(), ops32_[2].get(), x32_.get(), y32_.get(),
-                               scratch32_.get(), stats_.scratch_per_wg, isdmrg_, map_[0], map_[1],
-                               map_[2]);
+                               scratch32_.get(), stats_.scratch_per_wg, counter_.get(),
+                               part32_.get(), isdmrg_, map_[0], map_[1], map_[2]);
+        if (nsplits_)
+            hipLaunchKernelGGL((split_reduce_kernel<float>), dim3(std::min(nsplits_, 4096)),
+                               dim3(256), 0, s, splits_.get(), nsplits_, part32_.get(), y32_.get());
         hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(size_)), dim3(256), 0, s,
                            y32_.get(), dres, size_);
     }
@@ -335,7 +814,8 @@ void HeffDevice::diagonal(double* ddiag, hipStream_t s) {
     hipLaunchKernelGGL((fill_kernel<double>), dim3(grid_for(size_)), dim3(256), 0, s, ddiag, size_,
                        0.0);
     if (ndblocks_)
-        hipLaunchKernelGGL((heff_diag_kernel<double>), dim3(std::min(ndblocks_, 8192)), dim3(256), 0,
+        hipLaunchKernelGGL((heff_diag_kernel<double>), dim3(std::min(ndblocks_, 8192)),
+                           dim3(kDiagThreads), 0,
                            s, dblocks_.get(), ndblocks_, dterms_.get(), ops64_[0].get(),
                            ops64_[1].get(), ops64_[2].get(), ddiag, isdmrg_, map_[0], map_[1],
                            map_[2]);

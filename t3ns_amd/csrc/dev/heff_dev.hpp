@@ -15,23 +15,41 @@ namespace t3 {
 
 enum Precision : int { PREC_FP64 = 0, PREC_FP32 = 1 };
 
-struct DevBlock {      // one output block
-    int64_t y_off;     // element offset of the block in the vector
+struct DevBlock {      // one work unit: an output block or a slice of its pairs
+    int64_t y_off;     // element offset of the destination (vector or partials)
     int d[3];          // leg dims in operator order
     int pair_begin;
     int pair_end;
-    int pad;
+    int to_part;       // 1: accumulate into the partial buffer
+};
+struct DevSplit {      // an output block computed by several units
+    int64_t y_off;     // block offset in the vector
+    int64_t part_off;  // first partial copy
+    int nparts;
+    int size;
 };
 struct DevPair {       // one (output block, input block) pair
     int64_t x_off;
     int d[3];          // input leg dims in operator order
-    int order;         // contraction order (index into the 6 permutations)
-    int term_begin;
-    int term_end;
+    int ord[3];        // operator legs in contraction order (ord[2] unused for DMRG)
+    int chunk_begin;   // term chunks of this pair
+    int chunk_end;
+    int w1;            // elements of one first-stage output
+    int w2;            // elements of one second-stage output (0 for DMRG)
+    int xlds;          // input block staged in LDS
+    int wlds;          // stage outputs in LDS (else workgroup global scratch)
+};
+// Consecutive terms of a pair processed between two barriers: their
+// distinct first-stage operators ("runs") and the terms themselves.
+struct DevChunk {
+    int term_begin, term_end;
+    int run_begin, run_end;  // into the run-lead list
 };
 struct DevTerm {       // one instruction of one MPO group of a pair
     int64_t op[3];     // element offsets into the three operator arenas
     double alpha;      // instruction prefactor x symmetry prefactor
+    int slot;          // first-stage output slot (run index within its chunk)
+    int pad;
 };
 struct DevDiagBlock {
     int64_t off;
@@ -46,6 +64,14 @@ struct HeffStats {
     double flops = 0;            // reference-order GEMM flops of one matvec
     int64_t op_elems[3] = {0, 0, 0};
     int64_t scratch_per_wg = 0;
+    int64_t nchunks = 0;
+    int64_t nruns = 0;
+    int64_t nunits = 0;
+    int64_t nsplit_blocks = 0;
+    int64_t pairs_global_x = 0;  // input block read from global memory
+    int64_t pairs_global_w = 0;  // stage outputs in global scratch
+    double mfma_steps = 0;       // 16x16x4 matrix-core steps of one matvec
+    double stage_steps[3] = {0, 0, 0};
 };
 
 class HeffDevice {
@@ -76,6 +102,8 @@ private:
     int nwg_;
     DevBuf<DevBlock> blocks_;
     DevBuf<DevPair> pairs_;
+    DevBuf<DevChunk> chunks_;
+    DevBuf<int> runs_;
     DevBuf<DevTerm> terms_;
     DevBuf<DevDiagBlock> dblocks_;
     DevBuf<DevTerm> dterms_;
@@ -84,6 +112,12 @@ private:
     DevBuf<double> scratch64_;
     DevBuf<float> scratch32_;
     DevBuf<float> x32_, y32_;
+    DevBuf<int> counter_;  // next output block of the dynamic schedule
+    DevBuf<DevSplit> splits_;
+    DevBuf<double> part64_;
+    DevBuf<float> part32_;
+    int64_t part_elems_ = 0;
+    int nsplits_ = 0;
     int ndblocks_ = 0;
 };
 

@@ -127,6 +127,16 @@ def plan_table(desc, ops, site):
     return ints[:n], prefs[:n], fl.value
 
 
+def plan_block_dims(desc, ops, site):
+    """Leg dimensions [nrblocks, 3] of the multisite tensor's blocks."""
+    dims = np.zeros((max(site.nrblocks, 1), 3), dtype=np.int32)
+    n = lib().t3ns_plan_block_dims(C.byref(desc), ops, C.byref(site),
+                                   dims.ctypes.data_as(_lib.c_int_p))
+    if n < 0:
+        check(n, "t3ns_plan_block_dims")
+    return dims[:n]
+
+
 class Heff:
     """init_Heffdata (include/Heff.h:166) on the device."""
 
@@ -171,6 +181,13 @@ class Heff:
         check(lib().t3ns_heff_stats(self._h, C.byref(nb), C.byref(npair), C.byref(nt), C.byref(fl)))
         return {"nblocks": nb.value, "npairs": npair.value, "nterms": nt.value, "flops": fl.value,
                 "size": self.size}
+
+    def kernel_stats(self):
+        names = ["chunks", "runs", "pairs_global_x", "pairs_global_w", "mfma_steps",
+                 "steps_A", "steps_B", "steps_C", "scratch_per_wg", "units", "split_blocks"]
+        buf = (C.c_double * len(names))()
+        check(lib().t3ns_heff_kernel_stats(self._h, buf, len(names)) < 0 and -1 or 0)
+        return dict(zip(names, list(buf)))
 
     def plan_table(self):
         n = lib().t3ns_heff_plan_table(self._h, None, None)
