@@ -27,6 +27,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+PROFILE = os.path.join(ROOT, "profiles", "heff_matvec_qc24_t3ns_D512.md")
+
+
+def profiled_traffic(sites, D):
+    """HBM bytes per FP64 matvec from the committed PMC summary of this
+    workload (FETCH_SIZE/WRITE_SIZE, copy-calibrated), or None."""
+    if sites != [18, 31, 17] or D != 512 or not os.path.exists(PROFILE):
+        return None
+    with open(PROFILE) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_matvec:"):
+                return float(line.split(":")[1])
+    return None
 NETWORK = os.path.join(ROOT, "tests", "golden", "networks", "24_T3NS.netw")
 
 
@@ -40,6 +53,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the baseline sample (0 disables it)")
     ap.add_argument("--precisions", default="f64,f32")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device copy (scale reference for HBM counters)")
     return ap.parse_args()
 
 
@@ -94,6 +109,16 @@ def main():
         sys.exit(2)
     import torch
     import t3ns_amd as T
+
+    if args.copy_calibration:
+        n = (256 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 8}))
+        return
 
     sites = [int(v) for v in args.sites.split(",")]
     t0 = time.perf_counter()
@@ -169,7 +194,8 @@ def main():
                    "flops_per_matvec": flops, "setup_s": t_setup},
         "roofline": {"bound": "mfma", "achieved": r["tflops"], "peak": PEAK_TFLOPS[main_prec],
                      "unit": "TFLOP/s", "frac": r["tflops"] / PEAK_TFLOPS[main_prec],
-                     "traffic": None},
+                     "traffic": profiled_traffic(sites, args.bond_dim) if main_prec == "f64"
+                     else None},
     }
     if "f32" in results and "f64" in results:
         err = float(np.linalg.norm(ys["f32"] - ys["f64"]) / np.linalg.norm(ys["f64"]))

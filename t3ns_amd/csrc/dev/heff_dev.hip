@@ -53,7 +53,16 @@ struct Mfma<float> {
 struct Geo {
     int lo, K, F, dn, tf, tiles, ksteps;
 };
+// Values every lane of a wave agrees on, kept in scalar registers so that
+// loop control stays scalar.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 __device__ __forceinline__ Geo make_geo(int s0, int s1, int s2, int p, int dn) {
+    s0 = uni(s0);
+    s1 = uni(s1);
+    s2 = uni(s2);
+    p = uni(p);
+    dn = uni(dn);
     Geo g;
     g.lo = p == 0 ? 1 : (p == 1 ? s0 : s0 * s1);
     g.K = p == 0 ? s0 : (p == 1 ? s1 : s2);
@@ -66,95 +75,145 @@ __device__ __forceinline__ Geo make_geo(int s0, int s1, int s2, int p, int dn) {
     return g;
 }
 
-// Per-lane offsets of one tile, computed once and reused for every operator
-// and term that goes through the tile.
-struct TileLane {
-    int soff, ooff;  // A and B operand offsets at this lane's first k
-    int zoff[4];     // output offsets of the lane's four D values (-1: none)
-    bool fv, nv;
+// Register blocking: a wave task covers one 16-fiber row tile and up to
+// kTN column tiles of the output, so one A-operand load feeds kTN
+// matrix-core steps.
+constexpr int kTN = 4;
+
+// Per-lane offsets of one row tile, computed once and reused for every
+// operator and term that goes through it.  Lanes past the last fiber or
+// column read a valid element (clamped index): their D rows and columns are
+// never stored, so only the K tail needs masking.
+struct RowLane {
+    int soff;       // A operand offset at this lane's first k (clamped fiber)
+    int ooff;       // B operand offset of column tile tn0 at this lane's first k
+    int rowoff[4];  // output offset of the lane's D rows without the column term (-1: none)
+    int tn0, ntn;   // column tiles [tn0, tn0 + ntn) of this task
 };
 template <typename T>
-__device__ __forceinline__ TileLane tile_lane(const Geo& g, int tile) {
-    TileLane L;
+__device__ __forceinline__ RowLane row_lane(const Geo& g, int tf, int tn0) {
+    RowLane L;
     const int lane = threadIdx.x & 63;
     const int i = lane & 15, kk = lane >> 4;
-    const int tf = tile % g.tf, tn = tile / g.tf;
-    const int f = tf * 16 + i;
-    L.fv = f < g.F;
-    const int fc = L.fv ? f : 0;
-    const int ilo = fc % g.lo, ihi = fc / g.lo;
-    L.soff = ilo + ihi * g.lo * g.K + kk * g.lo;
-    const int n = tn * 16 + i;
-    L.nv = n < g.dn;
-    L.ooff = (L.nv ? n : 0) + kk * g.dn;
+    const int f = min(tf * 16 + i, g.F - 1);
+    L.soff = (f % g.lo) + (f / g.lo) * g.lo * g.K + kk * g.lo;
+    L.tn0 = tn0;
+    const int ntn_all = (g.dn + 15) >> 4;
+    L.ntn = min(kTN, ntn_all - tn0);
+    const int n = min(tn0 * 16 + i, g.dn - 1);
+    L.ooff = n + kk * g.dn;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int fr = tf * 16 + Mfma<T>::row(lane, v);
-        L.zoff[v] = (L.nv && fr < g.F) ? (fr % g.lo) + n * g.lo + (fr / g.lo) * g.lo * g.dn : -1;
+        L.rowoff[v] = fr < g.F ? (fr % g.lo) + (fr / g.lo) * g.lo * g.dn : -1;
     }
     return L;
 }
 
-// acc += (scale S)[tile rows, :] O[tile cols, :]^T over the whole K.
-template <typename T>
-__device__ __forceinline__ void tile_mac(const T* __restrict__ S, const T* __restrict__ O,
-                                         const Geo& g, const TileLane& L, T scale,
-                                         typename Mfma<T>::V& acc) {
+// acc[j] += (scale S)[row tile, :] O[column tile tn0 + j, :]^T over the whole K,
+// j < NTN.  Loads are unconditional (clamped indices stay inside the
+// blocks) so the compiler can keep several in flight; only the K tail is
+// masked.
+template <typename T, int NTN, bool SCALE>
+__device__ __forceinline__ void row_mac_n(const T* __restrict__ S, const T* __restrict__ O,
+                                          const Geo& g, const RowLane& L, T scale,
+                                          typename Mfma<T>::V (&acc)[kTN]) {
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 15, kk = lane >> 4;
     const T* sp = S + L.soff;
     const T* op = O + L.ooff;
     const int sst = 4 * g.lo, ost = 4 * g.dn;
-    int k = (threadIdx.x & 63) >> 4;
-#pragma unroll 4
-    for (int s = 0; s < g.ksteps; ++s) {
-        const bool kv = k < g.K;
-        const T av = (L.fv && kv) ? *sp : (T)0;
-        const T bv = (L.nv && kv) ? *op : (T)0;
-        acc = Mfma<T>::run(scale * av, bv, acc);
-        sp += sst;
-        op += ost;
-        k += 4;
+    const int n0 = min(L.tn0 * 16 + i, g.dn - 1);
+    int coff[NTN];
+#pragma unroll
+    for (int j = 0; j < NTN; ++j) coff[j] = min((L.tn0 + j) * 16 + i, g.dn - 1) - n0;
+    const int kfull = uni(g.K >> 2);
+    // Full k-steps in slabs of KS: all loads of a slab are issued before its
+    // matrix-core steps so their latencies overlap.
+    constexpr int KS = 4;
+    for (int s0 = 0; s0 < kfull; s0 += KS) {
+        const int ns = uni(min(KS, kfull - s0));
+        T av[KS];
+        T bv[KS][NTN];
+#pragma unroll
+        for (int u = 0; u < KS; ++u) {
+            if (u < ns) {
+                av[u] = sp[u * sst];
+#pragma unroll
+                for (int j = 0; j < NTN; ++j) bv[u][j] = op[u * ost + coff[j]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < KS; ++u) {
+            if (u < ns) {
+                T a = av[u];
+                if (SCALE) a *= scale;
+#pragma unroll
+                for (int j = 0; j < NTN; ++j) acc[j] = Mfma<T>::run(a, bv[u][j], acc[j]);
+            }
+        }
+        sp += ns * sst;
+        op += ns * ost;
     }
-}
-
-// Operands of up to kFragSteps k-steps of one tile, loaded ahead of their
-// matrix-core steps so that the loads of the next operator overlap the
-// arithmetic of the current one.
-constexpr int kFragSteps = 6;
-template <typename T>
-struct Frag {
-    T a[kFragSteps], b[kFragSteps];
-};
-template <typename T>
-__device__ __forceinline__ void frag_load(const T* __restrict__ S, const T* __restrict__ O,
-                                          const Geo& g, const TileLane& L, Frag<T>& fr) {
-    const int kk = (threadIdx.x & 63) >> 4;
-    const T* sp = S + L.soff;
-    const T* op = O + L.ooff;
+    if (g.K & 3) {  // K tail: lanes past K contribute zero (their loads are skipped)
+        const bool kv = (kfull << 2) + kk < g.K;
+        const T* spc = kv ? sp : S + L.soff;
+        const T* opc = kv ? op : O + L.ooff;
+        T av = kv ? spc[0] : (T)0;
+        if (SCALE) av *= scale;
 #pragma unroll
-    for (int u = 0; u < kFragSteps; ++u) {
-        const bool kv = 4 * u + kk < g.K;
-        fr.a[u] = (L.fv && kv) ? sp[4 * u * g.lo] : (T)0;
-        fr.b[u] = (L.nv && kv) ? op[4 * u * g.dn] : (T)0;
-    }
-}
-template <typename T>
-__device__ __forceinline__ void frag_mma(const Frag<T>& fr, const Geo& g, T scale,
-                                         typename Mfma<T>::V& acc) {
-#pragma unroll
-    for (int u = 0; u < kFragSteps; ++u)
-        if (u < g.ksteps) acc = Mfma<T>::run(scale * fr.a[u], fr.b[u], acc);
-}
-
-template <typename T>
-__device__ __forceinline__ void tile_put(T* __restrict__ Z, const TileLane& L,
-                                         const typename Mfma<T>::V& acc, bool add) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        if (L.zoff[v] >= 0) {
-            if (add) Z[L.zoff[v]] += acc[v];
-            else Z[L.zoff[v]] = acc[v];
+        for (int j = 0; j < NTN; ++j) {
+            const T bv = opc[coff[j]];
+            acc[j] = Mfma<T>::run(av, bv, acc[j]);
         }
     }
+}
+
+template <typename T, bool SCALE>
+__device__ __forceinline__ void row_mac(const T* __restrict__ S, const T* __restrict__ O,
+                                        const Geo& g, const RowLane& L, T scale,
+                                        typename Mfma<T>::V (&acc)[kTN]) {
+    switch (__builtin_amdgcn_readfirstlane(L.ntn)) {
+        case 1: row_mac_n<T, 1, SCALE>(S, O, g, L, scale, acc); break;
+        case 2: row_mac_n<T, 2, SCALE>(S, O, g, L, scale, acc); break;
+        case 3: row_mac_n<T, 3, SCALE>(S, O, g, L, scale, acc); break;
+        default: row_mac_n<T, 4, SCALE>(S, O, g, L, scale, acc); break;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void row_put(T* __restrict__ Z, const Geo& g, const RowLane& L,
+                                        const typename Mfma<T>::V (&acc)[kTN], bool add) {
+    const int i = threadIdx.x & 15;
+#pragma unroll
+    for (int j = 0; j < kTN; ++j) {
+        const int n = (L.tn0 + j) * 16 + i;
+        if (j < L.ntn && n < g.dn) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (L.rowoff[v] >= 0) {
+                    T* d = Z + L.rowoff[v] + n * g.lo;
+                    if (add) *d += acc[j][v];
+                    else *d = acc[j][v];
+                }
+            }
+        }
+    }
+}
+
+// Tasks of one stage: (row tile, column-tile group, item group).
+struct StageTasks {
+    int tf, tng, G;  // row tiles, column-tile groups, item groups
+    int total;
+};
+__device__ __forceinline__ StageTasks stage_tasks(const Geo& g, int nw) {
+    StageTasks t;
+    t.tf = g.tf;
+    t.tng = (((g.dn + 15) >> 4) + kTN - 1) / kTN;
+    const int base = t.tf * t.tng;
+    t.G = base >= nw ? 1 : nw / base;
+    t.total = base * t.G;
+    return t;
 }
 
 // All chunks of one pair.  Stage A: first-stage outputs of the chunk's
@@ -178,15 +237,16 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                                             const DevChunk* __restrict__ chunks,
                                             const int* __restrict__ runs,
                                             const DevTerm* __restrict__ terms,
-                                            const T* const* opb, const T* X, T* W1, T* W2, T* Y,
+                                            const T* __restrict__ op0, const T* __restrict__ op1,
+                                            const T* __restrict__ op2, const T* X, T* W1, T* W2, T* Y,
                                             T* part, ChunkRec* rec, int isdmrg, const int* map,
                                             const int* pos) {
     typedef typename Mfma<T>::V V;
     constexpr int NW = NT / 64;
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int a = P.ord[0], b = P.ord[1], c = isdmrg ? P.ord[1] : P.ord[2];
-    int cur[3] = {P.d[map[0]], P.d[map[1]], P.d[map[2]]};
+    const int a = uni(P.ord[0]), b = uni(P.ord[1]), c = uni(isdmrg ? P.ord[1] : P.ord[2]);
+    int cur[3] = {uni(P.d[map[0]]), uni(P.d[map[1]]), uni(P.d[map[2]])};
     const Geo gA = make_geo(cur[0], cur[1], cur[2], pos[a], bd[a]);
     cur[pos[a]] = bd[a];
     Geo gB = gA;
@@ -195,146 +255,120 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
         cur[pos[b]] = bd[b];
     }
     const Geo gC = make_geo(cur[0], cur[1], cur[2], pos[c], bd[c]);
-    const int GA = gA.tiles >= NW ? 1 : NW / gA.tiles;
-    const int GB = gB.tiles >= NW ? 1 : NW / gB.tiles;
-    const int GC = gC.tiles >= NW ? 1 : NW / gC.tiles;
-    const T* opa = opb[a];
-    const T* opbb = opb[b];
-    const T* opc = opb[c];
-    for (int ci = P.chunk_begin; ci < P.chunk_end; ++ci) {
+    const StageTasks kA = stage_tasks(gA, NW);
+    const StageTasks kB = stage_tasks(gB, NW);
+    StageTasks kC = stage_tasks(gC, NW);
+    // stage C item groups are reduced through LDS: keep the partial tiles
+    // within NW x 256 values
+    const int ntnC = (gC.dn + 15) >> 4;
+    kC.G = gC.tf * ntnC >= NW || kC.tng > 1 ? 1 : NW / (gC.tf * ntnC);
+    kC.total = kC.tf * kC.tng * kC.G;
+    const T* opa = a == 0 ? op0 : (a == 1 ? op1 : op2);
+    const T* opbb = b == 0 ? op0 : (b == 1 ? op1 : op2);
+    const T* opc = c == 0 ? op0 : (c == 1 ? op1 : op2);
+    const int cb = uni(P.chunk_begin), ce = uni(P.chunk_end);
+    const int w1 = uni(P.w1), w2 = uni(P.w2);
+    for (int ci = cb; ci < ce; ++ci) {
         const DevChunk C = chunks[ci];
-        const int nr = C.run_end - C.run_begin;
-        const int nt = C.term_end - C.term_begin;
+        const int nr = uni(C.run_end - C.run_begin);
+        const int nt = uni(C.term_end - C.term_begin);
         // chunk records into LDS
         if (threadIdx.x < nt) {
-            const DevTerm& tm = terms[C.term_begin + threadIdx.x];
+            const DevTerm& tm = terms[uni(C.term_begin) + threadIdx.x];
             rec->opb[threadIdx.x] = tm.op[b];
             rec->opc[threadIdx.x] = tm.op[c];
             rec->alpha[threadIdx.x] = tm.alpha;
             rec->slot[threadIdx.x] = tm.slot;
         }
-        if (threadIdx.x < nr) rec->opa[threadIdx.x] = terms[runs[C.run_begin + threadIdx.x]].op[a];
+        if (threadIdx.x < nr) rec->opa[threadIdx.x] = terms[runs[uni(C.run_begin) + threadIdx.x]].op[a];
         __syncthreads();
-        {  // stage A
-            for (int task = wave; task < gA.tiles * GA; task += NW) {
-                const int tile = task % gA.tiles, grp = task / gA.tiles;
-                const TileLane L = tile_lane<T>(gA, tile);
-                if (gA.ksteps <= kFragSteps) {
-                    Frag<T> cur, nxt;
-                    if (grp < nr) frag_load<T>(X, opa + rec->opa[grp], gA, L, cur);
-                    for (int j = grp; j < nr; j += GA) {
-                        if (j + GA < nr) frag_load<T>(X, opa + rec->opa[j + GA], gA, L, nxt);
-                        V acc = {0, 0, 0, 0};
-                        frag_mma<T>(cur, gA, (T)1, acc);
-                        tile_put<T>(W1 + j * P.w1, L, acc, false);
-                        cur = nxt;
-                    }
-                } else {
-                    for (int j = grp; j < nr; j += GA) {
-                        V acc = {0, 0, 0, 0};
-                        tile_mac<T>(X, opa + rec->opa[j], gA, L, (T)1, acc);
-                        tile_put<T>(W1 + j * P.w1, L, acc, false);
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        if (!isdmrg) {  // stage B
-            for (int task = wave; task < gB.tiles * GB; task += NW) {
-                const int tile = task % gB.tiles, grp = task / gB.tiles;
-                const TileLane L = tile_lane<T>(gB, tile);
-                if (gB.ksteps <= kFragSteps) {
-                    Frag<T> cur, nxt;
-                    if (grp < nt)
-                        frag_load<T>(W1 + rec->slot[grp] * P.w1, opbb + rec->opb[grp], gB, L, cur);
-                    for (int t = grp; t < nt; t += GB) {
-                        if (t + GB < nt)
-                            frag_load<T>(W1 + rec->slot[t + GB] * P.w1, opbb + rec->opb[t + GB], gB,
-                                         L, nxt);
-                        V acc = {0, 0, 0, 0};
-                        frag_mma<T>(cur, gB, (T)1, acc);
-                        tile_put<T>(W2 + t * P.w2, L, acc, false);
-                        cur = nxt;
-                    }
-                } else {
-                    for (int t = grp; t < nt; t += GB) {
-                        V acc = {0, 0, 0, 0};
-                        tile_mac<T>(W1 + rec->slot[t] * P.w1, opbb + rec->opb[t], gB, L, (T)1, acc);
-                        tile_put<T>(W2 + t * P.w2, L, acc, false);
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        // stage C
-        for (int task = wave; task < gC.tiles * GC; task += NW) {
-            const int tile = task % gC.tiles, grp = task / gC.tiles;
-            const TileLane L = tile_lane<T>(gC, tile);
-            V acc = {0, 0, 0, 0};
-            if (gC.ksteps <= kFragSteps) {
-                Frag<T> cur, nxt;
-                if (grp < nt) {
-                    const T* S = isdmrg ? W1 + rec->slot[grp] * P.w1 : W2 + grp * P.w2;
-                    frag_load<T>(S, opc + rec->opc[grp], gC, L, cur);
-                }
-                for (int t = grp; t < nt; t += GC) {
-                    if (t + GC < nt) {
-                        const int u = t + GC;
-                        const T* S = isdmrg ? W1 + rec->slot[u] * P.w1 : W2 + u * P.w2;
-                        frag_load<T>(S, opc + rec->opc[u], gC, L, nxt);
-                    }
-                    frag_mma<T>(cur, gC, (T)rec->alpha[t], acc);
-                    cur = nxt;
-                }
-            } else {
-                for (int t = grp; t < nt; t += GC) {
-                    const T* S = isdmrg ? W1 + rec->slot[t] * P.w1 : W2 + t * P.w2;
-                    tile_mac<T>(S, opc + rec->opc[t], gC, L, (T)rec->alpha[t], acc);
-                }
-            }
-            if (GC == 1) {
-                tile_put<T>(Y, L, acc, true);
-            } else {
-                T* pp = part + ((grp * gC.tiles + tile) * 64 + lane) * 4;
+        {  // stage A: W1[j] = O_a[j] x_a X
+            const int base = kA.tf * kA.tng;
+            for (int task = wave; task < kA.total; task += NW) {
+                const int r = task % base, grp = task / base;
+                const RowLane L = row_lane<T>(gA, r % kA.tf, (r / kA.tf) * kTN);
+                for (int j = grp; j < nr; j += kA.G) {
+                    V acc[kTN];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) pp[v] = acc[v];
-            }
-        }
-        if (GC > 1) {
-            __syncthreads();
-            for (int tile = wave; tile < gC.tiles; tile += NW) {
-                const TileLane L = tile_lane<T>(gC, tile);
-                V acc = {0, 0, 0, 0};
-                for (int grp = 0; grp < GC; ++grp) {
-                    const T* pp = part + ((grp * gC.tiles + tile) * 64 + lane) * 4;
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) acc[v] += pp[v];
+                    for (int q = 0; q < kTN; ++q) acc[q] = V{0, 0, 0, 0};
+                    row_mac<T, false>(X, opa + rec->opa[j], gA, L, (T)1, acc);
+                    row_put<T>(W1 + j * w1, gA, L, acc, false);
                 }
-                tile_put<T>(Y, L, acc, true);
             }
+            __syncthreads();
         }
-        __syncthreads();
+        if (!isdmrg) {  // stage B: W2[t] = alpha_t O_b[t] x_b W1[slot t]
+            const int base = kB.tf * kB.tng;
+            for (int task = wave; task < kB.total; task += NW) {
+                const int r = task % base, grp = task / base;
+                const RowLane L = row_lane<T>(gB, r % kB.tf, (r / kB.tf) * kTN);
+                for (int t = grp; t < nt; t += kB.G) {
+                    V acc[kTN];
+#pragma unroll
+                    for (int q = 0; q < kTN; ++q) acc[q] = V{0, 0, 0, 0};
+                    row_mac<T, true>(W1 + rec->slot[t] * w1, opbb + rec->opb[t], gB, L,
+                                     (T)rec->alpha[t], acc);
+                    row_put<T>(W2 + t * w2, gB, L, acc, false);
+                }
+            }
+            __syncthreads();
+        }
+        {  // stage C: Y += sum_t O_c[t] x_c W2[t]  (DMRG: alpha_t O_c[t] x_c W1[slot t])
+            const int base = kC.tf * kC.tng;
+            for (int task = wave; task < kC.total; task += NW) {
+                const int r = task % base, grp = task / base;
+                const RowLane L = row_lane<T>(gC, r % kC.tf, (r / kC.tf) * kTN);
+                V acc[kTN];
+#pragma unroll
+                for (int q = 0; q < kTN; ++q) acc[q] = V{0, 0, 0, 0};
+                for (int t = grp; t < nt; t += kC.G) {
+                    if (isdmrg)
+                        row_mac<T, true>(W1 + rec->slot[t] * w1, opc + rec->opc[t], gC, L,
+                                         (T)rec->alpha[t], acc);
+                    else
+                        row_mac<T, false>(W2 + t * w2, opc + rec->opc[t], gC, L, (T)1, acc);
+                }
+                if (kC.G == 1) {
+                    row_put<T>(Y, gC, L, acc, true);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < kTN; ++q) {
+                        if (q < L.ntn) {
+                            T* pp = part + (((grp * kC.tf + r) * ntnC + q) * 64 + lane) * 4;
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) pp[v] = acc[q][v];
+                        }
+                    }
+                }
+            }
+            if (kC.G > 1) {
+                __syncthreads();
+                for (int r = wave; r < kC.tf; r += NW) {
+                    const RowLane L = row_lane<T>(gC, r, 0);
+                    V acc[kTN];
+#pragma unroll
+                    for (int q = 0; q < kTN; ++q) {
+                        acc[q] = V{0, 0, 0, 0};
+                        if (q < L.ntn)
+                            for (int grp = 0; grp < kC.G; ++grp) {
+                                const T* pp = part + (((grp * kC.tf + r) * ntnC + q) * 64 + lane) * 4;
+#pragma unroll
+                                for (int v = 0; v < 4; ++v) acc[q][v] += pp[v];
+                            }
+                    }
+                    row_put<T>(Y, gC, L, acc, true);
+                }
+            }
+            __syncthreads();
+        }
     }
-}
-
-// Out-of-line instance for pairs whose buffers do not all fit in LDS (rare;
-// generic pointers), keeping the register budget of the common path.
-template <typename T, int NT>
-__device__ __noinline__ void pair_chunks_generic(const DevPair& P, const int* bd,
-                                                 const DevChunk* __restrict__ chunks,
-                                                 const int* __restrict__ runs,
-                                                 const DevTerm* __restrict__ terms,
-                                                 const T* const* opb, const T* X, T* W1, T* W2,
-                                                 T* Y, T* part, ChunkRec* rec, int isdmrg,
-                                                 const int* map, const int* pos) {
-    pair_chunks<T, NT>(P, bd, chunks, runs, terms, opb, X, W1, W2, Y, part, rec, isdmrg, map, pos);
 }
 
 // One workgroup owns one output block at a time (largest first) and
 // accumulates every contributing term into it: no atomics, a fixed
 // summation order.  Stage buffers live in LDS when the pair's chunks fit,
 // else in the workgroup's global scratch region.
-template <typename T, int CX, int C1, int C2, int NT>
+template <typename T, int CX, int C1, int C2, int NT, bool GLOBAL>
 __global__ void __launch_bounds__(NT)
     heff_matvec_kernel(const DevBlock* __restrict__ blocks, int nblocks,
                        const DevPair* __restrict__ pairs, const DevChunk* __restrict__ chunks,
@@ -343,22 +377,19 @@ __global__ void __launch_bounds__(NT)
                        const T* __restrict__ op2, const T* __restrict__ x, T* __restrict__ y,
                        T* __restrict__ scratch, int64_t scratch_stride, int* __restrict__ next,
                        T* __restrict__ partial, int isdmrg, int m0, int m1, int m2) {
-    __shared__ T lds[CX + C1 + C2 + kPartial];
+    // GLOBAL = false: the input block and the stage outputs live in LDS (all
+    // pairs of the units of this launch fit).  GLOBAL = true: they live in
+    // the workgroup's region of global scratch.
+    __shared__ T lds[(GLOBAL ? 0 : CX + C1 + C2) + kPartial];
     __shared__ ChunkRec rec;
     __shared__ int cur_block;
-    T* Xs = lds;
-    T* W1s = lds + CX;
-    T* W2s = lds + CX + C1;
-    T* part = lds + CX + C1 + C2;
-    T* gw1 = scratch + (int64_t)blockIdx.x * scratch_stride;
-    T* gw2 = gw1 + scratch_stride / 2;
-    const T* opb[3] = {op0, op1, op2};
+    T* part = lds + (GLOBAL ? 0 : CX + C1 + C2);
     const int map[3] = {m0, m1, m2};
     int pos[3];
     pos[m0] = 0;
     pos[m1] = 1;
     pos[m2] = 2;
-    // Blocks are taken from a shared counter in cost order (dynamic
+    // Units are taken from a shared counter in cost order (dynamic
     // longest-first schedule); every workgroup leaves once it runs past the
     // end of the list.
     for (;;) {
@@ -371,20 +402,20 @@ __global__ void __launch_bounds__(NT)
         T* Y = (B.to_part ? partial : y) + B.y_off;
         for (int pi = B.pair_begin; pi < B.pair_end; ++pi) {
             const DevPair P = pairs[pi];
-            const T* X = x + P.x_off;
-            if (P.xlds) {
+            if (GLOBAL) {
+                T* gw1 = scratch + (int64_t)blockIdx.x * scratch_stride;
+                pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, gw1,
+                                   gw1 + scratch_stride / 2, Y, part, &rec, isdmrg, map, pos);
+            } else {
+                const T* X = x + P.x_off;
                 const int xsz = P.d[0] * P.d[1] * P.d[2];
-                for (int i = threadIdx.x; i < xsz; i += NT) Xs[i] = X[i];
+                for (int i = threadIdx.x; i < xsz; i += NT) lds[i] = X[i];
                 __syncthreads();
+                pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, lds, lds + CX,
+                                   lds + CX + C1, Y, part, &rec, isdmrg, map, pos);
             }
-            if (P.xlds && P.wlds)
-                pair_chunks<T, NT>(P, B.d, chunks, runs, terms, opb, Xs, W1s, W2s, Y, part, &rec,
-                                   isdmrg, map, pos);
-            else
-                pair_chunks_generic<T, NT>(P, B.d, chunks, runs, terms, opb, P.xlds ? Xs : X,
-                                           P.wlds ? W1s : gw1, P.wlds ? W2s : gw2, Y, part, &rec,
-                                           isdmrg, map, pos);
-        }    }
+        }
+    }
 }
 
 template <typename T>
@@ -630,19 +661,25 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     const double unit_cap = std::max(1.0, total / (4.0 * ncu));
     std::vector<DevBlock> units;
     std::vector<double> ucost;
+    std::vector<int> uglob;
     std::vector<DevSplit> hs;
     int64_t part_elems = 0;
     for (const DevBlock& B : hb) {
         if (B.pair_end == B.pair_begin) continue;
         const int64_t ysz = (int64_t)B.d[0] * B.d[1] * B.d[2];
+        // Pairs whose buffers fit in LDS and pairs that need global scratch
+        // go to different units (they run in different kernels).
         std::vector<std::pair<int, int>> slices;
         std::vector<double> scost;
+        std::vector<int> sglob;
         double acc = 0;
         int beg = B.pair_begin;
+        auto is_glob = [&](int pi) { return !(hp[pi].xlds && hp[pi].wlds); };
         for (int pi = B.pair_begin; pi < B.pair_end; ++pi) {
-            if (pi > beg && acc + pcost[pi] > unit_cap) {
+            if (pi > beg && (acc + pcost[pi] > unit_cap || is_glob(pi) != is_glob(beg))) {
                 slices.push_back({beg, pi});
                 scost.push_back(acc);
+                sglob.push_back(is_glob(beg));
                 beg = pi;
                 acc = 0;
             }
@@ -650,9 +687,11 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
         }
         slices.push_back({beg, B.pair_end});
         scost.push_back(acc);
+        sglob.push_back(is_glob(beg));
         if (slices.size() == 1) {
             units.push_back(B);
             ucost.push_back(scost[0]);
+            uglob.push_back(sglob[0]);
             continue;
         }
         DevSplit S{};
@@ -669,15 +708,25 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
             U.to_part = 1;
             units.push_back(U);
             ucost.push_back(scost[k]);
+            uglob.push_back(sglob[k]);
         }
         part_elems += (int64_t)slices.size() * ysz;
     }
-    // Costliest units first: the dynamic schedule then drains evenly.
+    // LDS units first, then global-scratch units; costliest first within
+    // each list so that the dynamic schedule drains evenly.
     std::vector<int> idx(units.size());
     std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ucost[a] > ucost[b]; });
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+        if (uglob[a] != uglob[b]) return uglob[a] < uglob[b];
+        return ucost[a] > ucost[b];
+    });
     std::vector<DevBlock> hbs;
-    for (int i : idx) hbs.push_back(units[i]);
+    nfast_ = 0;
+    for (int i : idx) {
+        hbs.push_back(units[i]);
+        nfast_ += !uglob[i];
+    }
+    nglob_ = (int)hbs.size() - nfast_;
     stats_.nunits = (int64_t)hbs.size();
     stats_.nsplit_blocks = (int64_t)hs.size();
     nsplits_ = (int)hs.size();
@@ -752,11 +801,13 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
         x32_.alloc(size_);
         y32_.alloc(size_);
     }
-    // Persistent grid: one workgroup per CU (the LDS use allows one).
-    nwg_ = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)hbs.size(), ncu));
-    if (prec_ == PREC_FP32) scratch32_.alloc((size_t)nwg_ * maxscratch);
-    else scratch64_.alloc((size_t)nwg_ * maxscratch);
-    counter_.alloc(1);
+    // Persistent grids: one workgroup per CU; global scratch only for the
+    // workgroups of the global-scratch kernel.
+    ncu_ = ncu;
+    nwg_glob_ = std::max(1, std::min(nglob_, ncu));
+    if (prec_ == PREC_FP32) scratch32_.alloc(nglob_ ? (size_t)nwg_glob_ * maxscratch : 1);
+    else scratch64_.alloc(nglob_ ? (size_t)nwg_glob_ * maxscratch : 1);
+    counter_.alloc(2);
     splits_.upload(hs, stream);
     if (f32) part32_.alloc(std::max<int64_t>(1, part_elems_));
     else part64_.alloc(std::max<int64_t>(1, part_elems_));
@@ -766,44 +817,42 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
 
 HeffDevice::~HeffDevice() = default;
 
+template <typename T, int CX, int C1, int C2>
+void HeffDevice::launch(const T* x, T* y, const T* const* ops, T* scratch, T* part,
+                        hipStream_t s) {
+    hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(size_)), dim3(256), 0, s, y, size_, (T)0);
+    hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s, counter_.get(), (int64_t)2, 0);
+    if (part_elems_)
+        hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(part_elems_)), dim3(256), 0, s, part,
+                           part_elems_, (T)0);
+    if (nfast_)
+        hipLaunchKernelGGL((heff_matvec_kernel<T, CX, C1, C2, kThreads, false>),
+                           dim3(std::min(nfast_, ncu_)), dim3(kThreads), 0, s, blocks_.get(),
+                           nfast_, pairs_.get(), chunks_.get(), runs_.get(), terms_.get(), ops[0],
+                           ops[1], ops[2], x, y, scratch, stats_.scratch_per_wg, counter_.get(),
+                           part, isdmrg_, map_[0], map_[1], map_[2]);
+    if (nglob_)
+        hipLaunchKernelGGL((heff_matvec_kernel<T, CX, C1, C2, kThreads, true>), dim3(nwg_glob_),
+                           dim3(kThreads), 0, s, blocks_.get() + nfast_, nglob_, pairs_.get(),
+                           chunks_.get(), runs_.get(), terms_.get(), ops[0], ops[1], ops[2], x, y,
+                           scratch, stats_.scratch_per_wg, counter_.get() + 1, part, isdmrg_,
+                           map_[0], map_[1], map_[2]);
+    if (nsplits_)
+        hipLaunchKernelGGL((split_reduce_kernel<T>), dim3(std::min(nsplits_, 4096)), dim3(256), 0,
+                           s, splits_.get(), nsplits_, part, y);
+}
+
 void HeffDevice::matvec(const double* dvec, double* dres, hipStream_t s) {
-    const int nb = (int)stats_.nblocks;
     if (prec_ == PREC_FP64) {
-        hipLaunchKernelGGL((fill_kernel<double>), dim3(grid_for(size_)), dim3(256), 0, s, dres,
-                           size_, 0.0);
-        hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s, counter_.get(), (int64_t)1, 0);
-        if (part_elems_)
-            hipLaunchKernelGGL((fill_kernel<double>), dim3(grid_for(part_elems_)), dim3(256), 0, s,
-                               part64_.get(), part_elems_, 0.0);
-        if (nb)
-            hipLaunchKernelGGL((heff_matvec_kernel<double, kCapX64, kCapW164, kCapW264, kThreads>),
-                               dim3(nwg_), dim3(kThreads), 0, s, blocks_.get(), nb, pairs_.get(),
-                               chunks_.get(), runs_.get(), terms_.get(), ops64_[0].get(),
-                               ops64_[1].get(), ops64_[2].get(), dvec, dres, scratch64_.get(),
-                               stats_.scratch_per_wg, counter_.get(), part64_.get(), isdmrg_,
-                               map_[0], map_[1], map_[2]);
-        if (nsplits_)
-            hipLaunchKernelGGL((split_reduce_kernel<double>), dim3(std::min(nsplits_, 4096)),
-                               dim3(256), 0, s, splits_.get(), nsplits_, part64_.get(), dres);
+        const double* ops[3] = {ops64_[0].get(), ops64_[1].get(), ops64_[2].get()};
+        launch<double, kCapX64, kCapW164, kCapW264>(dvec, dres, ops, scratch64_.get(),
+                                                    part64_.get(), s);
     } else {
         hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(size_)), dim3(256), 0, s,
                            dvec, x32_.get(), size_);
-        hipLaunchKernelGGL((fill_kernel<float>), dim3(grid_for(size_)), dim3(256), 0, s,
-                           y32_.get(), size_, 0.0f);
-        hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s, counter_.get(), (int64_t)1, 0);
-        if (part_elems_)
-            hipLaunchKernelGGL((fill_kernel<float>), dim3(grid_for(part_elems_)), dim3(256), 0, s,
-                               part32_.get(), part_elems_, 0.0f);
-        if (nb)
-            hipLaunchKernelGGL((heff_matvec_kernel<float, kCapX32, kCapW132, kCapW232, kThreads>),
-                               dim3(nwg_), dim3(kThreads), 0, s, blocks_.get(), nb, pairs_.get(),
-                               chunks_.get(), runs_.get(), terms_.get(), ops32_[0].get(),
-                               ops32_[1].get(), ops32_[2].get(), x32_.get(), y32_.get(),
-                               scratch32_.get(), stats_.scratch_per_wg, counter_.get(),
-                               part32_.get(), isdmrg_, map_[0], map_[1], map_[2]);
-        if (nsplits_)
-            hipLaunchKernelGGL((split_reduce_kernel<float>), dim3(std::min(nsplits_, 4096)),
-                               dim3(256), 0, s, splits_.get(), nsplits_, part32_.get(), y32_.get());
+        const float* ops[3] = {ops32_[0].get(), ops32_[1].get(), ops32_[2].get()};
+        launch<float, kCapX32, kCapW132, kCapW232>(x32_.get(), y32_.get(), ops, scratch32_.get(),
+                                                   part32_.get(), s);
         hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(size_)), dim3(256), 0, s,
                            y32_.get(), dres, size_);
     }

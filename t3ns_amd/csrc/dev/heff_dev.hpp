@@ -93,13 +93,19 @@ public:
     void diagonal(double* ddiag, hipStream_t s);
 
 private:
+    template <typename T, int CX, int C1, int C2>
+    void launch(const T* x, T* y, const T* const* ops, T* scratch, T* part, hipStream_t s);
+
     HeffPlan plan_;
     HeffStats stats_;
     int prec_;
     int isdmrg_;
     int map_[3];
     int64_t size_;
-    int nwg_;
+    int nwg_glob_ = 1;
+    int ncu_ = 256;
+    int nfast_ = 0;   // units whose pairs all fit in LDS (first in the unit list)
+    int nglob_ = 0;   // units that use global scratch
     DevBuf<DevBlock> blocks_;
     DevBuf<DevPair> pairs_;
     DevBuf<DevChunk> chunks_;
