@@ -146,6 +146,33 @@ int64_t t3ns_heff_plan_table(const t3ns_heff* h, int* ints, double* prefs);
 int64_t t3ns_plan_table(const t3ns_heff_desc* desc, const t3ns_rops_desc* ops,
                         const t3ns_site_desc* site, int* ints, double* prefs, double* flops);
 
+/* Lowest eigenpair of the effective Hamiltonian: davidson(result, energy,
+ * size, max_vecs, keep_deflate, davidson_tol, max_its, diagonal, matvecT3NS,
+ * vdata) of include/davidson.h:56-60 with the Heff handle as the operator,
+ * everything resident on the device.  result: host guess in, eigenvector
+ * out; diagonal: host diagonal or NULL (computed).  Returns 0 converged,
+ * 1 stopped at max_its (src/davidson.c:327), T3NS_ERR_SOLVER if the
+ * projected eigenproblem failed (-1 in the reference), other negatives on
+ * error. */
+int t3ns_davidson(t3ns_heff* h, double* result, double* energy, int max_vecs, int keep_deflate,
+                  double tol, int max_its, const double* diagonal, int* its);
+
+/* The same eigenpair with a reduced-precision operator for the bulk of the
+ * iterations: Davidson with h_low's matvec (e.g. T3NS_FP32) until the
+ * residual is below switch_tol, then Davidson with the FP64 operator h64
+ * from that vector down to tol.  Same return codes. */
+int t3ns_davidson_refined(t3ns_heff* h_low, t3ns_heff* h64, double* result, double* energy,
+                          int max_vecs, int keep_deflate, double tol, double switch_tol,
+                          int max_its, const double* diagonal, int* its_low, int* its64);
+
+/* Davidson on any symmetric operator given as a device matvec callback
+ * (y = A x on device pointers, on `stream`); dx: device guess in,
+ * eigenvector out; ddiag: device diagonal. */
+typedef void (*t3ns_dev_matvec)(const double* x, double* y, void* stream, void* vdata);
+int t3ns_davidson_dev(int64_t size, double* dx, const double* ddiag, t3ns_dev_matvec matvec,
+                      void* vdata, void* stream, double* energy, int max_vecs, int keep_deflate,
+                      double tol, int max_its, int* its);
+
 /* Leg dimensions of every block of the multisite tensor in operator-leg
  * order (dims[3*b + leg]; leg 2 is 1 for two-site DMRG objects), as
  * fill_indexes derives them (src/Heff.c:695-739).  Returns the block count

@@ -47,6 +47,9 @@ class HeffDesc(C.Structure):
                 ("merge_pref", c_dbl_p)]
 
 
+# void (*)(const double* x, double* y, void* stream, void* vdata)
+DEV_MATVEC = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)
+
 _lib = None
 
 
@@ -68,6 +71,12 @@ def lib():
         "t3ns_heff_matvec_device": (C.c_int, [P, P, P, P]),
         "t3ns_heff_diagonal": (C.c_int, [P, c_dbl_p]),
         "t3ns_heff_destroy": (C.c_int, [P]),
+        "t3ns_davidson": (C.c_int, [P, c_dbl_p, c_dbl_p, C.c_int, C.c_int, C.c_double, C.c_int,
+                                    c_dbl_p, c_int_p]),
+        "t3ns_davidson_refined": (C.c_int, [P, P, c_dbl_p, c_dbl_p, C.c_int, C.c_int, C.c_double,
+                                            C.c_double, C.c_int, c_dbl_p, c_int_p, c_int_p]),
+        "t3ns_davidson_dev": (C.c_int, [C.c_int64, P, P, DEV_MATVEC, P, P, c_dbl_p, C.c_int,
+                                        C.c_int, C.c_double, C.c_int, c_int_p]),
         "t3ns_heff_size": (C.c_int64, [P]),
         "t3ns_heff_stats": (C.c_int, [P, c_i64_p, c_i64_p, c_i64_p, c_dbl_p]),
         "t3ns_heff_plan_table": (C.c_int64, [P, c_int_p, c_dbl_p]),

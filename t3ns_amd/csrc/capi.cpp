@@ -9,6 +9,7 @@
 
 #include "../../include/t3ns_amd.h"
 #include "../../include/t3ns_host.h"
+#include "dev/davidson.hpp"
 #include "dev/heff_dev.hpp"
 #include "host/context.hpp"
 #include "host/heff_plan.hpp"
@@ -38,6 +39,8 @@ int guarded(F&& f) {
         return fail(T3NS_ERR_NOMEM, e.what());
     } catch (const std::invalid_argument& e) {
         return fail(T3NS_ERR_INVALID, e.what());
+    } catch (const std::domain_error& e) {
+        return fail(T3NS_ERR_SOLVER, e.what());
     } catch (const std::runtime_error& e) {
         const std::string m = e.what();
         return fail(m.rfind("HIP error", 0) == 0 ? T3NS_ERR_DEVICE : T3NS_ERR_INVALID, m);
@@ -238,6 +241,101 @@ int t3ns_heff_diagonal(t3ns_heff* h, double* diag) {
                                     h->stream));
         T3_HIP_CHECK(hipStreamSynchronize(h->stream));
     });
+}
+
+int t3ns_davidson(t3ns_heff* h, double* result, double* energy, int max_vecs, int keep_deflate,
+                  double tol, int max_its, const double* diagonal, int* its) {
+    if (!h || !result || !energy) return fail(T3NS_ERR_INVALID, "null argument");
+    int status = 0;
+    const int rc = guarded([&] {
+        const int64_t n = h->dev->size();
+        DevBuf<double> x(n), d(n);
+        T3_HIP_CHECK(hipMemcpyAsync(x.get(), result, n * sizeof(double), hipMemcpyHostToDevice,
+                                    h->stream));
+        if (diagonal)
+            T3_HIP_CHECK(hipMemcpyAsync(d.get(), diagonal, n * sizeof(double),
+                                        hipMemcpyHostToDevice, h->stream));
+        else
+            h->dev->diagonal(d.get(), h->stream);
+        DeviceDavidson dav(n, max_vecs, keep_deflate, h->stream);
+        HeffDevice* dev = h->dev.get();
+        const DavidsonResult R = dav.solve(
+            x.get(), d.get(), [dev](const double* a, double* b, hipStream_t s) { dev->matvec(a, b, s); },
+            tol, max_its);
+        if (R.status < 0) throw std::domain_error("davidson: projected eigenproblem failed");
+        T3_HIP_CHECK(hipMemcpyAsync(result, x.get(), n * sizeof(double), hipMemcpyDeviceToHost,
+                                    h->stream));
+        T3_HIP_CHECK(hipStreamSynchronize(h->stream));
+        *energy = R.energy;
+        if (its) *its = R.its;
+        status = R.status;
+    });
+    return rc != T3NS_OK ? rc : status;
+}
+
+int t3ns_davidson_refined(t3ns_heff* h_low, t3ns_heff* h64, double* result, double* energy,
+                          int max_vecs, int keep_deflate, double tol, double switch_tol,
+                          int max_its, const double* diagonal, int* its_low, int* its64) {
+    if (!h_low || !h64 || !result || !energy) return fail(T3NS_ERR_INVALID, "null argument");
+    if (h_low->dev->size() != h64->dev->size())
+        return fail(T3NS_ERR_INVALID, "davidson_refined: operators of different size");
+    int status = 0;
+    const int rc = guarded([&] {
+        const int64_t n = h64->dev->size();
+        DevBuf<double> x(n), d(n);
+        hipStream_t s = h64->stream;
+        T3_HIP_CHECK(hipMemcpyAsync(x.get(), result, n * sizeof(double), hipMemcpyHostToDevice, s));
+        if (diagonal)
+            T3_HIP_CHECK(hipMemcpyAsync(d.get(), diagonal, n * sizeof(double),
+                                        hipMemcpyHostToDevice, s));
+        else
+            h64->dev->diagonal(d.get(), s);
+        HeffDevice* lo = h_low->dev.get();
+        HeffDevice* hi = h64->dev.get();
+        // Phase 1: reduced-precision matvecs until the residual reaches
+        // switch_tol.  Phase 2: FP64 matvecs from that vector down to tol
+        // (the FP64 Rayleigh-Ritz and residual correct the reduced-precision
+        // operator error).
+        DeviceDavidson d1(n, max_vecs, keep_deflate, s);
+        const DavidsonResult R1 = d1.solve(
+            x.get(), d.get(), [lo](const double* a, double* b, hipStream_t st) { lo->matvec(a, b, st); },
+            std::max(tol, switch_tol), max_its);
+        if (R1.status < 0) throw std::domain_error("davidson: projected eigenproblem failed");
+        DeviceDavidson d2(n, max_vecs, keep_deflate, s);
+        const DavidsonResult R2 = d2.solve(
+            x.get(), d.get(), [hi](const double* a, double* b, hipStream_t st) { hi->matvec(a, b, st); },
+            tol, max_its);
+        if (R2.status < 0) throw std::domain_error("davidson: projected eigenproblem failed");
+        T3_HIP_CHECK(hipMemcpyAsync(result, x.get(), n * sizeof(double), hipMemcpyDeviceToHost, s));
+        T3_HIP_CHECK(hipStreamSynchronize(s));
+        *energy = R2.energy;
+        if (its_low) *its_low = R1.its;
+        if (its64) *its64 = R2.its;
+        status = R2.status;
+    });
+    return rc != T3NS_OK ? rc : status;
+}
+
+int t3ns_davidson_dev(int64_t size, double* dx, const double* ddiag, t3ns_dev_matvec matvec,
+                      void* vdata, void* stream, double* energy, int max_vecs, int keep_deflate,
+                      double tol, int max_its, int* its) {
+    if (size <= 0 || !dx || !ddiag || !matvec || !energy) return fail(T3NS_ERR_INVALID, "bad argument");
+    int status = 0;
+    const int rc = guarded([&] {
+        hipStream_t s = (hipStream_t)stream;
+        DeviceDavidson dav(size, max_vecs, keep_deflate, s);
+        const DavidsonResult R = dav.solve(
+            dx, ddiag,
+            [matvec, vdata](const double* a, double* b, hipStream_t st) {
+                matvec(a, b, (void*)st, vdata);
+            },
+            tol, max_its);
+        if (R.status < 0) throw std::domain_error("davidson: projected eigenproblem failed");
+        *energy = R.energy;
+        if (its) *its = R.its;
+        status = R.status;
+    });
+    return rc != T3NS_OK ? rc : status;
 }
 
 int t3ns_heff_destroy(t3ns_heff* h) {

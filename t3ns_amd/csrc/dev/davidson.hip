@@ -1,0 +1,321 @@
+// Device-resident Davidson (src/davidson.c restated for HBM-resident
+// vectors; see davidson.hpp).
+#include "davidson.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <stdexcept>
+
+namespace t3 {
+
+namespace {
+constexpr int kRedThreads = 256;
+constexpr int kRedBlocks = 512;
+constexpr double kDiagCutoff = 1e-12;  // src/davidson.c:33
+
+// partial[j * nb + b] = sum over block b's grid-stride share of
+// A[j*lda + i] * x[i], j < n.
+__global__ void __launch_bounds__(kRedThreads)
+    dots_kernel(const double* __restrict__ A, int64_t lda, int n, const double* __restrict__ x,
+                int64_t size, double* __restrict__ partial) {
+    __shared__ double red[kRedThreads];
+    for (int j = 0; j < n; ++j) {
+        const double* a = A + (int64_t)j * lda;
+        double s = 0;
+        for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < size;
+             i += (int64_t)gridDim.x * kRedThreads)
+            s += a[i] * x[i];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[(int64_t)j * gridDim.x + blockIdx.x] = red[0];
+        __syncthreads();
+    }
+}
+
+// out[j] = sum_b partial[j * nb + b], in block order.
+__global__ void __launch_bounds__(kRedThreads)
+    finish_kernel(const double* __restrict__ partial, int nb, int n, double* __restrict__ out) {
+    __shared__ double red[kRedThreads];
+    for (int j = blockIdx.x; j < n; j += gridDim.x) {
+        double s = 0;
+        for (int b = threadIdx.x; b < nb; b += kRedThreads) s += partial[(int64_t)j * nb + b];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[j] = red[0];
+        __syncthreads();
+    }
+}
+
+// out[i] = beta * out[i] + sum_j coef[j] A[j*size + i]
+__global__ void combine_kernel(const double* __restrict__ A, int64_t size, int n,
+                               const double* __restrict__ coef, double* __restrict__ out,
+                               double beta) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < size;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double s = beta == 0.0 ? 0.0 : beta * out[i];
+        for (int j = 0; j < n; ++j) s += coef[j] * A[(int64_t)j * size + i];
+        out[i] = s;
+    }
+}
+
+__global__ void scale_kernel(double* __restrict__ v, int64_t n, double a) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        v[i] *= a;
+}
+
+__global__ void copy_kernel(const double* __restrict__ a, double* __restrict__ b, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+__device__ __forceinline__ double precond_inv(double diff) {
+    // 1/diff with the cutoff of src/davidson.c:230-236
+    return fabs(diff) > kDiagCutoff ? 1.0 / diff : (diff < 0 ? -1.0 : 1.0) / kDiagCutoff;
+}
+
+// uK = r / (D - theta); partial sums of uK*t and uK*r (src/davidson.c:223-242)
+__global__ void __launch_bounds__(kRedThreads)
+    precond_dots_kernel(const double* __restrict__ r, const double* __restrict__ t,
+                        const double* __restrict__ diag, double theta, int64_t size,
+                        double* __restrict__ partial) {
+    __shared__ double red[2][kRedThreads];
+    double s0 = 0, s1 = 0;
+    for (int64_t i = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; i < size;
+         i += (int64_t)gridDim.x * kRedThreads) {
+        const double uK = r[i] * precond_inv(diag[i] - theta);
+        s0 += uK * t[i];
+        s1 += uK * r[i];
+    }
+    red[0][threadIdx.x] = s0;
+    red[1][threadIdx.x] = s1;
+    __syncthreads();
+    for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = red[0][0];
+        partial[gridDim.x + blockIdx.x] = red[1][0];
+    }
+}
+
+// t = -(t + alpha r) / (D - theta)  (src/davidson.c:243-256)
+__global__ void precond_apply_kernel(double* __restrict__ t, const double* __restrict__ r,
+                                     const double* __restrict__ diag, double theta, double alpha,
+                                     int64_t size) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < size;
+         i += (int64_t)gridDim.x * blockDim.x)
+        t[i] = -(t[i] + alpha * r[i]) * precond_inv(diag[i] - theta);
+}
+
+int grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(4096, (n + 255) / 256)); }
+}  // namespace
+
+int jacobi_eigh(int n, const double* A, int ld, double* w, double* V) {
+    std::vector<double> a((size_t)n * n), v((size_t)n * n, 0.0);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) a[i + (size_t)j * n] = A[std::min(i, j) + (size_t)std::max(i, j) * ld];
+    for (int i = 0; i < n; ++i) v[i + (size_t)i * n] = 1.0;
+    auto at = [&](int i, int j) -> double& { return a[i + (size_t)j * n]; };
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0, tot = 0;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                tot += at(i, j) * at(i, j);
+                if (i != j) off += at(i, j) * at(i, j);
+            }
+        if (off <= 1e-30 * std::max(tot, 1e-300)) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = at(p, q);
+                if (std::fabs(apq) < 1e-300) continue;
+                const double theta = (at(q, q) - at(p, p)) / (2 * apq);
+                const double tt = (theta >= 0 ? 1.0 : -1.0) /
+                                  (std::fabs(theta) + std::sqrt(theta * theta + 1));
+                const double c = 1 / std::sqrt(tt * tt + 1), s = tt * c;
+                for (int k = 0; k < n; ++k) {  // columns p, q
+                    const double akp = at(k, p), akq = at(k, q);
+                    at(k, p) = c * akp - s * akq;
+                    at(k, q) = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {  // rows p, q
+                    const double apk = at(p, k), aqk = at(q, k);
+                    at(p, k) = c * apk - s * aqk;
+                    at(q, k) = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = v[k + (size_t)p * n], vkq = v[k + (size_t)q * n];
+                    v[k + (size_t)p * n] = c * vkp - s * vkq;
+                    v[k + (size_t)q * n] = s * vkp + c * vkq;
+                }
+            }
+    }
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return at(x, x) < at(y, y); });
+    for (int j = 0; j < n; ++j) {
+        w[j] = at(idx[j], idx[j]);
+        for (int i = 0; i < n; ++i) V[i + (size_t)j * ld] = v[i + (size_t)idx[j] * n];
+    }
+    for (int j = 0; j < n; ++j)
+        if (!std::isfinite(w[j])) return 1;
+    return 0;
+}
+
+DeviceDavidson::DeviceDavidson(int64_t size, int max_vecs, int keep_deflate, hipStream_t s)
+    : size_(size), mv_(max_vecs), keep_(keep_deflate), s_(s) {
+    if (size <= 0 || max_vecs < 2 || keep_deflate < 1 || keep_deflate >= max_vecs)
+        throw std::invalid_argument("davidson: bad sizes");
+    if (max_vecs > size) mv_ = (int)std::max<int64_t>(2, size);
+    if (keep_ >= mv_) keep_ = mv_ - 1;
+    V_.alloc((size_t)size * mv_);
+    VA_.alloc((size_t)size * mv_);
+    t_.alloc(size);
+    tmp_.alloc((size_t)size * keep_);
+    nblk_ = (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (size + kRedThreads - 1) / kRedThreads));
+    partial_.alloc((size_t)nblk_ * (mv_ + 2));
+    coef_.alloc(mv_);
+    out_.alloc(mv_ + 2);
+    sub_.assign((size_t)mv_ * mv_, 0.0);
+    eigv_.assign((size_t)mv_ * mv_, 0.0);
+    eigval_.assign(mv_, 0.0);
+}
+
+void DeviceDavidson::dots(const double* A, int64_t lda, int n, const double* b, double* host_out) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(dots_kernel, dim3(nblk_), dim3(kRedThreads), 0, s_, A, lda, n, b, size_,
+                       partial_.get());
+    hipLaunchKernelGGL(finish_kernel, dim3(std::min(n, 64)), dim3(kRedThreads), 0, s_,
+                       partial_.get(), nblk_, n, out_.get());
+    T3_HIP_CHECK(hipMemcpyAsync(host_out, out_.get(), n * sizeof(double), hipMemcpyDeviceToHost, s_));
+    T3_HIP_CHECK(hipStreamSynchronize(s_));
+}
+
+void DeviceDavidson::combine(const double* A, int n, const double* coef_host, double* out,
+                             double beta) {
+    T3_HIP_CHECK(hipMemcpyAsync(coef_.get(), coef_host, n * sizeof(double), hipMemcpyHostToDevice, s_));
+    hipLaunchKernelGGL(combine_kernel, dim3(grid(size_)), dim3(256), 0, s_, A, size_, n,
+                       coef_.get(), out, beta);
+}
+
+double DeviceDavidson::norm(const double* v) {
+    double h;
+    dots(v, size_, 1, v, &h);
+    return std::sqrt(h);
+}
+
+// t <- orthonormal complement of t against V[:, :m] (two CGS passes).
+void DeviceDavidson::orthonormalise(double* t) {
+    std::vector<double> c(mv_);
+    for (int pass = 0; pass < 2 && m_ > 0; ++pass) {
+        dots(V_.get(), size_, m_, t, c.data());
+        for (int j = 0; j < m_; ++j) c[j] = -c[j];
+        combine(V_.get(), m_, c.data(), t, 1.0);
+    }
+    const double nrm = norm(t);
+    if (!(nrm > 0)) throw std::runtime_error("davidson: search vector vanished");
+    hipLaunchKernelGGL(scale_kernel, dim3(grid(size_)), dim3(256), 0, s_, t, size_, 1.0 / nrm);
+}
+
+// Column m of the projected matrix: V[:, i] . VA[:, m], i <= m
+// (src/davidson.c:132-145).
+void DeviceDavidson::expand(int m) {
+    std::vector<double> c(m + 1);
+    dots(V_.get(), size_, m + 1, VA_.get() + (int64_t)m * size_, c.data());
+    for (int i = 0; i <= m; ++i) sub_[i + (size_t)m * mv_] = c[i];
+}
+
+int DeviceDavidson::eigsolve() {
+    return jacobi_eigh(m_, sub_.data(), mv_, eigval_.data(), eigv_.data());
+}
+
+// Keep the keep_ lowest Ritz vectors (src/davidson.c:164-185).
+void DeviceDavidson::deflate() {
+    for (int which = 0; which < 2; ++which) {
+        double* B = which == 0 ? V_.get() : VA_.get();
+        for (int k = 0; k < keep_; ++k)
+            combine(B, m_, eigv_.data() + (size_t)k * mv_, tmp_.get() + (int64_t)k * size_, 0.0);
+        hipLaunchKernelGGL(copy_kernel, dim3(grid(size_ * keep_)), dim3(256), 0, s_, tmp_.get(), B,
+                           size_ * keep_);
+    }
+    std::fill(sub_.begin(), sub_.end(), 0.0);
+    m_ = 0;
+    while (m_ < keep_) {
+        expand(m_);
+        ++m_;
+    }
+}
+
+DavidsonResult DeviceDavidson::solve(double* x, const double* diag, const DevMatvec& mv,
+                                     double tol, int max_its) {
+    DavidsonResult R;
+    m_ = 0;
+    hipLaunchKernelGGL(copy_kernel, dim3(grid(size_)), dim3(256), 0, s_, x, t_.get(), size_);
+    double residual = tol * 10;
+    double energy = 0;
+    int its = 0;
+    std::vector<double> c(mv_);
+    while (residual > tol && its < max_its) {
+        orthonormalise(t_.get());
+        double* Vm = V_.get() + (int64_t)m_ * size_;
+        double* VAm = VA_.get() + (int64_t)m_ * size_;
+        hipLaunchKernelGGL(copy_kernel, dim3(grid(size_)), dim3(256), 0, s_, t_.get(), Vm, size_);
+        mv(Vm, VAm, s_);
+        expand(m_);
+        ++m_;
+        if (eigsolve() != 0) {
+            R.status = -1;
+            return R;
+        }
+        if (m_ == mv_) {
+            deflate();
+            if (eigsolve() != 0) {
+                R.status = -1;
+                return R;
+            }
+        }
+        // residual (src/davidson.c:187-202): x = V y, t = VA y - theta x
+        const double theta = eigval_[0];
+        combine(V_.get(), m_, eigv_.data(), x, 0.0);
+        combine(VA_.get(), m_, eigv_.data(), t_.get(), 0.0);
+        for (int j = 0; j < m_; ++j) c[j] = -theta * eigv_[j];
+        combine(V_.get(), m_, c.data(), t_.get(), 1.0);
+        residual = norm(t_.get());
+        energy = theta;
+        ++its;
+        // preconditioned correction of the residual t against the Ritz
+        // vector x (src/davidson.c:204-257)
+        hipLaunchKernelGGL(precond_dots_kernel, dim3(nblk_), dim3(kRedThreads), 0, s_, x, t_.get(),
+                           diag, theta, size_, partial_.get());
+        hipLaunchKernelGGL(finish_kernel, dim3(2), dim3(kRedThreads), 0, s_, partial_.get(), nblk_,
+                           2, out_.get());
+        double uk[2];
+        T3_HIP_CHECK(hipMemcpyAsync(uk, out_.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
+        T3_HIP_CHECK(hipStreamSynchronize(s_));
+        const double alpha = -uk[0] / uk[1];
+        hipLaunchKernelGGL(precond_apply_kernel, dim3(grid(size_)), dim3(256), 0, s_, t_.get(), x,
+                           diag, theta, alpha, size_);
+    }
+    T3_HIP_CHECK(hipStreamSynchronize(s_));
+    R.energy = energy;
+    R.residual = residual;
+    R.its = its;
+    R.status = residual > tol ? 1 : 0;
+    return R;
+}
+
+}  // namespace t3

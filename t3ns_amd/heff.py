@@ -182,6 +182,22 @@ class Heff:
         return {"nblocks": nb.value, "npairs": npair.value, "nterms": nt.value, "flops": fl.value,
                 "size": self.size}
 
+    def davidson(self, guess, max_vecs=30, keep_deflate=2, tol=1e-8, max_its=100,
+                 diagonal=None):
+        """Lowest eigenpair (davidson, include/davidson.h:56-60): returns
+        (energy, eigenvector, iterations, status) with status 0 converged,
+        1 stopped at max_its."""
+        x = np.array(guess, dtype=np.float64, copy=True)
+        e, its = C.c_double(), C.c_int()
+        dg = None if diagonal is None else np.ascontiguousarray(diagonal, dtype=np.float64)
+        rc = lib().t3ns_davidson(self._h, x.ctypes.data_as(_lib.c_dbl_p), C.byref(e),
+                                 int(max_vecs), int(keep_deflate), float(tol), int(max_its),
+                                 None if dg is None else dg.ctypes.data_as(_lib.c_dbl_p),
+                                 C.byref(its))
+        if rc < 0:
+            check(rc, "t3ns_davidson")
+        return e.value, x, its.value, rc
+
     def kernel_stats(self):
         names = ["chunks", "runs", "pairs_global_x", "pairs_global_w", "mfma_steps",
                  "steps_A", "steps_B", "steps_C", "scratch_per_wg", "units", "split_blocks"]
@@ -209,6 +225,38 @@ class Heff:
             pass
 
 
+def davidson_refined(h_low, h64, guess, max_vecs=30, keep_deflate=2, tol=1e-8, switch_tol=1e-4,
+                     max_its=100, diagonal=None):
+    """Davidson with h_low's (reduced-precision) matvec down to switch_tol,
+    then FP64 matvecs down to tol.  Returns (energy, vector, its_low,
+    its64, status)."""
+    x = np.array(guess, dtype=np.float64, copy=True)
+    e, i1, i2 = C.c_double(), C.c_int(), C.c_int()
+    dg = None if diagonal is None else np.ascontiguousarray(diagonal, dtype=np.float64)
+    rc = lib().t3ns_davidson_refined(h_low.handle, h64.handle, x.ctypes.data_as(_lib.c_dbl_p),
+                                     C.byref(e), int(max_vecs), int(keep_deflate), float(tol),
+                                     float(switch_tol), int(max_its),
+                                     None if dg is None else dg.ctypes.data_as(_lib.c_dbl_p),
+                                     C.byref(i1), C.byref(i2))
+    if rc < 0:
+        check(rc, "t3ns_davidson_refined")
+    return e.value, x, i1.value, i2.value, rc
+
+
+def davidson_device(size, dx_ptr, ddiag_ptr, matvec, stream_ptr=0, max_vecs=30, keep_deflate=2,
+                    tol=1e-8, max_its=100):
+    """Davidson on device data with a Python matvec callback
+    matvec(x_ptr, y_ptr, stream_ptr) working on device pointers."""
+    cb = _lib.DEV_MATVEC(lambda x, y, s, v: matvec(x, y, s))
+    e, its = C.c_double(), C.c_int()
+    rc = lib().t3ns_davidson_dev(int(size), C.c_void_p(dx_ptr), C.c_void_p(ddiag_ptr), cb, None,
+                                 C.c_void_p(stream_ptr), C.byref(e), int(max_vecs),
+                                 int(keep_deflate), float(tol), int(max_its), C.byref(its))
+    if rc < 0:
+        check(rc, "t3ns_davidson_dev")
+    return e.value, its.value, rc
+
+
 # Reference-shaped entry points ------------------------------------------------
 
 def init_Heffdata(Operators, siteObject, desc, precision=FP64):
@@ -227,3 +275,15 @@ def make_diagonal(data):
 
 def destroy_Heffdata(data):
     data.close()
+
+
+def davidson(result, size, max_vecs, keep_deflate, davidson_tol, max_its, diagonal, vdata):
+    """davidson(result, energy, size, max_vecs, keep_deflate, tol, max_its,
+    diagonal, matvecT3NS, vdata): result (host array) is overwritten with the
+    eigenvector; returns (status, energy)."""
+    if size != vdata.size:
+        raise ValueError("size does not match the operator")
+    e, x, its, rc = vdata.davidson(result, max_vecs, keep_deflate, davidson_tol, max_its,
+                                   diagonal)
+    result[:] = x
+    return rc, e

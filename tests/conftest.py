@@ -3,6 +3,13 @@ import sys
 
 import pytest
 
+# Load torch (and the HIP runtime it ships) before libt3ns_amd.so so that
+# one HIP runtime serves both in this process.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
