@@ -173,6 +173,53 @@ int t3ns_davidson_dev(int64_t size, double* dx, const double* ddiag, t3ns_dev_ma
                       void* vdata, void* stream, double* energy, int max_vecs, int keep_deflate,
                       double tol, int max_its, int* its);
 
+/* ------------------------------------------------------------------------
+ * Renormalisation: appending a physical site to renormalised operators.
+ * rOperators_append_phys(nr, or) (src/rOperators_pUpdate.c:665-673 with
+ * unique_rOperators_ap 642-663, pAppend_block 537-640, sum_unique_rOperators
+ * src/rOperators_init.c:228-265): the operators of the bond on the far side
+ * of a physical site become operators acting on (bond x site), P_operator = 1.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int nrSyms;
+    int sgs[T3NS_MAX_SYMMETRIES];
+    int is_left;                     /* of the old and the new operators */
+    t3ns_symsecs_desc site_ss[3];    /* sectors of the appended site's bonds (alpha, i, beta) */
+    t3ns_symsecs_desc new_ss[3];     /* bra, ket and MPO sectors of the new operators' bond */
+    t3ns_symsecs_desc old_ss[3];     /* bra, ket and MPO sectors of the old operators' bond */
+    t3ns_symsecs_desc mpo;           /* MPO sectors */
+    /* pUpdate instructions of the old bond as fetch_pUpdate sorts them
+     * (src/instructions.c:116-140): (old op, site op, new op), prefactor,
+     * MPO sector of every new operator */
+    int nr_instr;
+    const int* instr;                /* [nr_instr * 3] */
+    const double* pref;              /* [nr_instr] */
+    const int* hss_of_new;           /* [number of new operators] */
+    /* site operators: MPO sector and matrix elements el_siteop(op, bra, ket) */
+    int nr_siteops;
+    const int* siteop_hss;           /* [nr_siteops] */
+    int physdim;                     /* sectors of the physical bond */
+    const double* siteop_el;         /* [nr_siteops * physdim * physdim], (op*physdim + bra)*physdim + ket */
+    /* tprods_ham(new MPO sector, site): pairs (site-op sector, old-op sector) */
+    const int* tprod_begin;          /* [mpo.nrSecs + 1] */
+    const int* tprods;               /* [tprod_begin[H] * 2] */
+    /* structure of the new operators before empty blocks are dropped */
+    int nrhss;
+    const int* begin_blocks_of_hss;  /* [nrhss + 1] */
+    const int64_t* qnumbers;         /* [nblocks * 3] */
+    const int64_t* const* hss_beginblock; /* per MPO sector: block offsets [nblocks(hss) + 1] */
+} t3ns_append_desc;
+
+/* Renormalised operators owned by the library (host copy). */
+typedef struct t3ns_rops t3ns_rops;
+
+/* rOperators_append_phys on the device: *out receives the new operators
+ * (all-zero blocks dropped as kick_zero_blocks does, src/sparseblocks.c:73-107). */
+int t3ns_append_phys(const t3ns_append_desc* desc, const t3ns_rops_desc* old, t3ns_rops** out);
+/* Descriptor view of library-owned operators (valid until destroyed). */
+int t3ns_rops_view(const t3ns_rops* r, t3ns_rops_desc* d);
+int t3ns_rops_destroy(t3ns_rops* r);
+
 /* Leg dimensions of every block of the multisite tensor in operator-leg
  * order (dims[3*b + leg]; leg 2 is 1 for two-site DMRG objects), as
  * fill_indexes derives them (src/Heff.c:695-739).  Returns the block count

@@ -52,6 +52,12 @@ int t3ns_ctx_bond(const t3ns_ctx* c, int bond, int* nrsecs, int* totaldims, int*
  * operators hold deterministic pseudo-random blocks (seed): the Heff inputs
  * of one step without a preceding sweep.  Used by parity tests. */
 int t3ns_step_random(t3ns_step** s, t3ns_ctx* c, const int* sites, int nsites, uint64_t seed);
+/* The rOperators_append_phys problem behind physical leg `leg` of a step:
+ * the operators of the bond on the far side of the appended site (N(0,1)
+ * entries from `seed`, or the unit operator at a border) and the append
+ * descriptor.  Pointers stay valid until the step is destroyed. */
+int t3ns_step_append_desc(t3ns_step* s, int leg, uint64_t seed, t3ns_append_desc* d,
+                          t3ns_rops_desc* old);
 int t3ns_step_destroy(t3ns_step* s);
 int64_t t3ns_step_size(const t3ns_step* s);
 /* Fills the Heff boundary descriptors; pointers stay valid while s lives. */

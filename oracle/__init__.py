@@ -23,7 +23,7 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIBPATH):
             raise RuntimeError("oracle not built: run oracle/build.py")
-        from t3ns_amd._lib import HeffDesc, RopsDesc, SiteDesc
+        from t3ns_amd._lib import AppendDesc, HeffDesc, RopsDesc, SiteDesc
         L = C.CDLL(LIBPATH)
         P = C.c_void_p
         dp = C.POINTER(C.c_double)
@@ -52,6 +52,7 @@ def lib():
             "oracle_heff_plan_stats": (None, [P, i64, i64, i64, dp]),
             "oracle_heff_plan_table": (C.c_int64, [P, ip, dp]),
             "oracle_heff_destroy": (None, [P]),
+            "oracle_append_phys": (C.c_int64, [C.POINTER(AppendDesc), C.POINTER(RopsDesc), i64, dp]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -127,3 +128,30 @@ class OracleHeff:
             self.close()
         except Exception:
             pass
+
+
+def append_phys(desc, old):
+    """Oracle rOperators_append_phys: [(beginblock, elements)] per new operator."""
+    import ctypes as _C
+    nops = 0
+    for i in range(desc.nr_instr):
+        nops = max(nops, desc.instr[3 * i + 2] + 1)
+    nbb = 0
+    for k in range(nops):
+        h = desc.hss_of_new[k]
+        nbb += desc.begin_blocks_of_hss[h + 1] - desc.begin_blocks_of_hss[h] + 1
+    total = lib().oracle_append_phys(_C.byref(desc), _C.byref(old), None, None)
+    bb = np.zeros(max(nbb, 1), dtype=np.int64)
+    tel = np.zeros(max(total, 1))
+    lib().oracle_append_phys(_C.byref(desc), _C.byref(old),
+                             bb.ctypes.data_as(_C.POINTER(_C.c_int64)),
+                             tel.ctypes.data_as(_C.POINTER(_C.c_double)))
+    out, pb, pt = [], 0, 0
+    for k in range(nops):
+        h = desc.hss_of_new[k]
+        nb = desc.begin_blocks_of_hss[h + 1] - desc.begin_blocks_of_hss[h]
+        b = bb[pb:pb + nb + 1].copy()
+        out.append((b, tel[pt:pt + int(b[-1])].copy()))
+        pb += nb + 1
+        pt += int(b[-1])
+    return out

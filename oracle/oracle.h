@@ -70,6 +70,17 @@ void oracle_heff_plan_stats(const oracle_heff* h, int64_t* npairs, int64_t* nent
 int64_t oracle_heff_plan_table(const oracle_heff* h, int* ints, double* prefs);
 void oracle_heff_destroy(oracle_heff* h);
 
+/* rOperators_append_phys (src/rOperators_pUpdate.c:393-673 and
+ * sum_unique_rOperators, src/rOperators_init.c:185-265, kick_zero_blocks,
+ * src/sparseblocks.c:73-107) on the descriptor of include/t3ns_amd.h.
+ * Output: for new operator k (k < max instr[2] + 1) its block offsets
+ * (nblocks(hss_of_new[k]) + 1 entries, relative to the operator) appended
+ * to `beginblock`, and its elements appended to `tel`.  Returns the total
+ * element count; the arrays may be NULL to size them (the offsets then
+ * need sum_k (nblocks_k + 1) entries). */
+int64_t oracle_append_phys(const t3ns_append_desc* d, const t3ns_rops_desc* old,
+                           int64_t* beginblock, double* tel);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,17 @@ class HeffDesc(C.Structure):
                 ("merge_pref", c_dbl_p)]
 
 
+class AppendDesc(C.Structure):
+    _fields_ = [("nrSyms", C.c_int), ("sgs", C.c_int * MAX_SYMMETRIES), ("is_left", C.c_int),
+                ("site_ss", SymsecsDesc * 3), ("new_ss", SymsecsDesc * 3),
+                ("old_ss", SymsecsDesc * 3), ("mpo", SymsecsDesc),
+                ("nr_instr", C.c_int), ("instr", c_int_p), ("pref", c_dbl_p),
+                ("hss_of_new", c_int_p), ("nr_siteops", C.c_int), ("siteop_hss", c_int_p),
+                ("physdim", C.c_int), ("siteop_el", c_dbl_p), ("tprod_begin", c_int_p),
+                ("tprods", c_int_p), ("nrhss", C.c_int), ("begin_blocks_of_hss", c_int_p),
+                ("qnumbers", c_i64_p), ("hss_beginblock", C.POINTER(c_i64_p))]
+
+
 # void (*)(const double* x, double* y, void* stream, void* vdata)
 DEV_MATVEC = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)
 
@@ -71,6 +82,11 @@ def lib():
         "t3ns_heff_matvec_device": (C.c_int, [P, P, P, P]),
         "t3ns_heff_diagonal": (C.c_int, [P, c_dbl_p]),
         "t3ns_heff_destroy": (C.c_int, [P]),
+        "t3ns_append_phys": (C.c_int, [C.POINTER(AppendDesc), C.POINTER(RopsDesc), C.POINTER(P)]),
+        "t3ns_rops_view": (C.c_int, [P, C.POINTER(RopsDesc)]),
+        "t3ns_rops_destroy": (C.c_int, [P]),
+        "t3ns_step_append_desc": (C.c_int, [P, C.c_int, C.c_uint64, C.POINTER(AppendDesc),
+                                            C.POINTER(RopsDesc)]),
         "t3ns_davidson": (C.c_int, [P, c_dbl_p, c_dbl_p, C.c_int, C.c_int, C.c_double, C.c_int,
                                     c_dbl_p, c_int_p]),
         "t3ns_davidson_refined": (C.c_int, [P, P, c_dbl_p, c_dbl_p, C.c_int, C.c_int, C.c_double,

@@ -11,6 +11,7 @@
 #include "../../include/t3ns_host.h"
 #include "dev/davidson.hpp"
 #include "dev/heff_dev.hpp"
+#include "dev/renorm_dev.hpp"
 #include "host/context.hpp"
 #include "host/heff_plan.hpp"
 #include "host/tensors.hpp"
@@ -336,6 +337,106 @@ int t3ns_davidson_dev(int64_t size, double* dx, const double* ddiag, t3ns_dev_ma
         status = R.status;
     });
     return rc != T3NS_OK ? rc : status;
+}
+
+}  // extern "C"
+
+struct t3ns_rops {
+    ROperators r;
+    std::vector<int64_t> bb_flat;
+    std::vector<const int64_t*> bb_ptr;
+    std::vector<const double*> tel_ptr;
+    void views() {
+        bb_ptr.resize(r.nrops);
+        tel_ptr.resize(r.nrops);
+        for (int k = 0; k < r.nrops; ++k) {
+            bb_ptr[k] = r.beginblock[k].data();
+            tel_ptr[k] = r.tel.data() + r.op_offset[k];
+        }
+    }
+};
+
+namespace {
+AppendInput append_input_from(const t3ns_append_desc* d, const ROperators* old) {
+    AppendInput in;
+    in.sym.nrSyms = d->nrSyms;
+    for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) in.sym.sgs[i] = d->sgs[i];
+    in.is_left = d->is_left;
+    for (int k = 0; k < 3; ++k) {
+        in.site_ss[k] = symsecs_from(d->site_ss[k]);
+        in.new_ss[k] = symsecs_from(d->new_ss[k]);
+        in.old_ss[k] = symsecs_from(d->old_ss[k]);
+    }
+    in.mpo = symsecs_from(d->mpo);
+    const int H = in.mpo.nrSecs;
+    if (d->nr_instr < 0 || (d->nr_instr && (!d->instr || !d->pref || !d->hss_of_new)))
+        throw std::invalid_argument("append: bad instructions");
+    int nops = 0;
+    for (int i = 0; i < d->nr_instr; ++i) {
+        Instruction ins;
+        for (int k = 0; k < 3; ++k) ins.instr[k] = d->instr[3 * i + k];
+        ins.pref = d->pref[i];
+        if (ins.instr[0] < 0 || ins.instr[0] >= old->nrops || ins.instr[1] < 0 ||
+            ins.instr[1] >= d->nr_siteops || ins.instr[2] < 0)
+            throw std::invalid_argument("append: instruction out of range");
+        nops = std::max(nops, ins.instr[2] + 1);
+        in.instr.push_back(ins);
+    }
+    in.hss_of_new.assign(d->hss_of_new, d->hss_of_new + nops);
+    in.siteop_hss.assign(d->siteop_hss, d->siteop_hss + d->nr_siteops);
+    in.physdim = d->physdim;
+    in.siteop_el.assign(d->siteop_el, d->siteop_el + (size_t)d->nr_siteops * d->physdim * d->physdim);
+    in.tprod_begin.assign(d->tprod_begin, d->tprod_begin + H + 1);
+    in.tprods.assign(d->tprods, d->tprods + 2 * (size_t)in.tprod_begin[H]);
+    in.nrhss = d->nrhss;
+    if (d->nrhss != H) throw std::invalid_argument("append: nrhss differs from the MPO sectors");
+    in.begin_blocks_of_hss.assign(d->begin_blocks_of_hss, d->begin_blocks_of_hss + d->nrhss + 1);
+    const int nb = in.begin_blocks_of_hss[d->nrhss];
+    in.qnumbers.assign(d->qnumbers, d->qnumbers + 3 * (size_t)nb);
+    in.hss_bb.resize(d->nrhss);
+    for (int h = 0; h < d->nrhss; ++h) {
+        const int n = in.begin_blocks_of_hss[h + 1] - in.begin_blocks_of_hss[h];
+        in.hss_bb[h].assign(d->hss_beginblock[h], d->hss_beginblock[h] + n + 1);
+    }
+    in.old = old;
+    return in;
+}
+}  // namespace
+
+extern "C" {
+
+int t3ns_append_phys(const t3ns_append_desc* d, const t3ns_rops_desc* old, t3ns_rops** out) {
+    if (!d || !old || !out) return fail(T3NS_ERR_INVALID, "null argument");
+    return guarded([&] {
+        const ROperators o = rops_from(*old);
+        const AppendInput in = append_input_from(d, &o);
+        const AppendPlan plan = plan_append(in);
+        auto res = std::make_unique<t3ns_rops>();
+        hipStream_t s = nullptr;
+        res->r = run_append(plan, o, s);
+        res->views();
+        *out = res.release();
+    });
+}
+
+int t3ns_rops_view(const t3ns_rops* r, t3ns_rops_desc* d) {
+    if (!r || !d) return fail(T3NS_ERR_INVALID, "null argument");
+    d->bond = r->r.bond;
+    d->is_left = r->r.is_left;
+    d->P_operator = r->r.P_operator;
+    d->nrhss = r->r.nrhss;
+    d->begin_blocks_of_hss = r->r.begin_blocks_of_hss.data();
+    d->qnumbers = r->r.qnumbers.data();
+    d->nrops = r->r.nrops;
+    d->hss_of_ops = r->r.hss_of_ops.data();
+    d->beginblock = r->bb_ptr.data();
+    d->tel = r->tel_ptr.data();
+    return T3NS_OK;
+}
+
+int t3ns_rops_destroy(t3ns_rops* r) {
+    delete r;
+    return T3NS_OK;
 }
 
 int t3ns_heff_destroy(t3ns_heff* h) {

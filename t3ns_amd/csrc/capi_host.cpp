@@ -92,8 +92,26 @@ struct t3ns_ctx {
     std::unique_ptr<Context> c;
 };
 
+// An append problem of one physical leg of a step: the operators of the
+// far bond (old) and the flattened rOperators_append_phys inputs.
+struct AppendStore {
+    ROperators old;
+    std::vector<std::vector<int64_t>> old_bb;
+    std::vector<const int64_t*> old_bb_ptr;
+    std::vector<const double*> old_tel_ptr;
+    SymsecsFlat site_ss[3], new_ss[3], old_ss[3], mpo;
+    std::vector<int> instr, hss_of_new, siteop_hss, tprod_begin, tprods, bbh;
+    std::vector<double> pref, siteop_el;
+    std::vector<QN> qnumbers;
+    std::vector<std::vector<int64_t>> hss_bb;
+    std::vector<const int64_t*> hss_bb_ptr;
+    int physdim = 0, nr_siteops = 0, is_left = 0, nrhss = 0;
+    bool built = false;
+};
+
 struct t3ns_step {
     t3ns_ctx* ctx = nullptr;
+    AppendStore app[3];
     SiteTensor site;
     ROperators ops[3];
     InstructionSet merge_raw;
@@ -290,6 +308,142 @@ int t3ns_step_random(t3ns_step** out, t3ns_ctx* cx, const int* sites, int nsites
         }
         for (auto& v : T.tel) v = rng.normal();
         *out = st.release();
+    });
+}
+
+int t3ns_step_append_desc(t3ns_step* st, int leg, uint64_t seed, t3ns_append_desc* d,
+                          t3ns_rops_desc* old) {
+    if (!st || !d || !old || leg < 0 || leg > 2) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        Context& c = *st->ctx->c;
+        const ROperators& P = st->ops[leg];
+        if (P.nrhss <= 0 || !P.P_operator)
+            throw std::invalid_argument("step leg is not a physical operator");
+        AppendStore& a = st->app[leg];
+        if (!a.built) {
+            const int is_left = P.is_left;
+            const int pbond = P.bond;
+            // the appended site and the far bond of its operators
+            const int site = c.netw.bonds[pbond][!is_left];
+            int sb[3];
+            c.netw.bonds_of_site(site, sb);
+            const int b = sb[is_left ? 0 : 2];
+            int nrops;
+            std::vector<int> hss;
+            ROperators o;
+            if (c.netw.bonds[b][!is_left] == -1) {
+                o = vacuum_rOperators(c, b, is_left);
+            } else {
+                std::vector<std::vector<int64_t>> tmpbb;
+                operator_types(c, b, is_left, &nrops, &hss);
+                o = init_rOperators_structure(c, b, is_left, false, &tmpbb);
+                o.nrops = nrops;
+                o.hss_of_ops = hss;
+                o.beginblock.resize(nrops);
+                for (int k = 0; k < nrops; ++k) o.beginblock[k] = tmpbb[hss[k]];
+                o.finalize_offsets();
+                o.tel.resize(o.total_size());
+                Rng rng(seed);
+                for (auto& v : o.tel) v = rng.normal();
+            }
+            a.old = std::move(o);
+            a.is_left = is_left;
+            for (int k = 0; k < 3; ++k) a.site_ss[k].set(c.symsecs(sb[k]));
+            a.new_ss[0].set(c.symsecs(c.get_braT3NSbond(pbond)));
+            a.new_ss[1].set(c.symsecs(c.get_ketT3NSbond(pbond)));
+            a.new_ss[2].set(c.symsecs(-1));
+            a.old_ss[0].set(c.symsecs(c.get_braT3NSbond(b)));
+            a.old_ss[1].set(c.symsecs(c.get_ketT3NSbond(b)));
+            a.old_ss[2].set(c.symsecs(-1));
+            a.mpo.set(c.symsecs(-1));
+            const InstructionSet& ps = c.fetch_pUpdate(b, is_left);
+            int nops = 0, nso = 0;
+            for (const auto& i : ps.instr) {
+                for (int k = 0; k < 3; ++k) a.instr.push_back(i.instr[k]);
+                a.pref.push_back(i.pref);
+                nops = std::max(nops, i.instr[2] + 1);
+                nso = std::max(nso, i.instr[1] + 1);
+            }
+            a.hss_of_new.assign(ps.hss_of_new.begin(), ps.hss_of_new.begin() + nops);
+            a.nr_siteops = nso;
+            const Symsecs& phys = c.symsecs(sb[1]);
+            a.physdim = phys.nrSecs;
+            // site-operator ids may be sparse labels: tabulate the ones the
+            // instructions use
+            std::vector<char> used(nso, 0);
+            for (const auto& i : ps.instr) used[i.instr[1]] = 1;
+            a.siteop_hss.assign(nso, -1);
+            a.siteop_el.assign((size_t)nso * a.physdim * a.physdim, 0.0);
+            for (int so = 0; so < nso; ++so) {
+                if (!used[so]) continue;
+                a.siteop_hss[so] = c.ham->symsec_siteop(so, site);
+                for (int br = 0; br < a.physdim; ++br)
+                    for (int kt = 0; kt < a.physdim; ++kt)
+                        a.siteop_el[((size_t)so * a.physdim + br) * a.physdim + kt] =
+                            c.ham->el_siteop(so, br, kt);
+            }
+            const int H = c.ham->nr_hss();
+            a.tprod_begin.assign(1, 0);
+            for (int h = 0; h < H; ++h) {
+                for (const auto& pr : c.ham->tprods_ham(h, site)) {
+                    a.tprods.push_back(pr[0]);
+                    a.tprods.push_back(pr[1]);
+                }
+                a.tprod_begin.push_back((int)a.tprods.size() / 2);
+            }
+            std::vector<std::vector<int64_t>> tmpbb;
+            ROperators ns = init_rOperators_structure(c, pbond, is_left, true, &tmpbb);
+            a.nrhss = ns.nrhss;
+            a.bbh = ns.begin_blocks_of_hss;
+            a.qnumbers = ns.qnumbers;
+            a.hss_bb = tmpbb;
+            for (auto& v : a.hss_bb)
+                if (v.empty()) v.assign(1, 0);
+            a.hss_bb_ptr.resize(a.hss_bb.size());
+            for (size_t h = 0; h < a.hss_bb.size(); ++h) a.hss_bb_ptr[h] = a.hss_bb[h].data();
+            a.old_bb_ptr.resize(a.old.nrops);
+            a.old_tel_ptr.resize(a.old.nrops);
+            for (int k = 0; k < a.old.nrops; ++k) {
+                a.old_bb_ptr[k] = a.old.beginblock[k].data();
+                a.old_tel_ptr[k] = a.old.tel.data() + a.old.op_offset[k];
+            }
+            a.built = true;
+        }
+        std::memset(d, 0, sizeof(*d));
+        d->nrSyms = c.sym.nrSyms;
+        for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) d->sgs[i] = c.sym.sgs[i];
+        d->is_left = a.is_left;
+        for (int k = 0; k < 3; ++k) {
+            d->site_ss[k] = a.site_ss[k].desc;
+            d->new_ss[k] = a.new_ss[k].desc;
+            d->old_ss[k] = a.old_ss[k].desc;
+        }
+        d->mpo = a.mpo.desc;
+        d->nr_instr = (int)a.pref.size();
+        d->instr = a.instr.data();
+        d->pref = a.pref.data();
+        d->hss_of_new = a.hss_of_new.data();
+        d->nr_siteops = a.nr_siteops;
+        d->siteop_hss = a.siteop_hss.data();
+        d->physdim = a.physdim;
+        d->siteop_el = a.siteop_el.data();
+        d->tprod_begin = a.tprod_begin.data();
+        d->tprods = a.tprods.data();
+        d->nrhss = a.nrhss;
+        d->begin_blocks_of_hss = a.bbh.data();
+        d->qnumbers = a.qnumbers.data();
+        d->hss_beginblock = a.hss_bb_ptr.data();
+        std::memset(old, 0, sizeof(*old));
+        old->bond = a.old.bond;
+        old->is_left = a.old.is_left;
+        old->P_operator = a.old.P_operator;
+        old->nrhss = a.old.nrhss;
+        old->begin_blocks_of_hss = a.old.begin_blocks_of_hss.data();
+        old->qnumbers = a.old.qnumbers.data();
+        old->nrops = a.old.nrops;
+        old->hss_of_ops = a.old.hss_of_ops.data();
+        old->beginblock = a.old_bb_ptr.data();
+        old->tel = a.old_tel_ptr.data();
     });
 }
 

@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import FP32, FP64, HeffDesc, RopsDesc, SiteDesc, check, lib
+from ._lib import FP32, FP64, AppendDesc, HeffDesc, RopsDesc, SiteDesc, check, lib
 
 
 class Context:
@@ -102,6 +102,13 @@ class Step:
         check(lib().t3ns_step_heff_desc(self._h, C.byref(d), o, C.byref(s)), "t3ns_step_heff_desc")
         return d, o, s
 
+    def append_problem(self, leg, seed=0):
+        """(append descriptor, old operators) behind physical leg `leg`."""
+        d, o = AppendDesc(), RopsDesc()
+        check(lib().t3ns_step_append_desc(self._h, int(leg), int(seed), C.byref(d), C.byref(o)),
+              "t3ns_step_append_desc")
+        return d, o
+
     def close(self):
         if self._h:
             lib().t3ns_step_destroy(self._h)
@@ -125,6 +132,46 @@ def plan_table(desc, ops, site):
     lib().t3ns_plan_table(C.byref(desc), ops, C.byref(site),
                           ints.ctypes.data_as(_lib.c_int_p), prefs.ctypes.data_as(_lib.c_dbl_p), None)
     return ints[:n], prefs[:n], fl.value
+
+
+class ROps:
+    """Renormalised operators owned by the library (host copy)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self.desc = RopsDesc()
+        check(lib().t3ns_rops_view(self._h, C.byref(self.desc)), "t3ns_rops_view")
+
+    def blocks(self):
+        """[(beginblock array, elements array)] per operator."""
+        d = self.desc
+        out = []
+        for k in range(d.nrops):
+            h = d.hss_of_ops[k]
+            nb = d.begin_blocks_of_hss[h + 1] - d.begin_blocks_of_hss[h]
+            bb = np.ctypeslib.as_array(d.beginblock[k], shape=(nb + 1,)).copy()
+            n = int(bb[-1])
+            tel = np.ctypeslib.as_array(d.tel[k], shape=(max(n, 1),))[:n].copy() if n else np.zeros(0)
+            out.append((bb, tel))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().t3ns_rops_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def append_phys(desc, old):
+    """rOperators_append_phys (src/rOperators_pUpdate.c:665-673) on the device."""
+    h = C.c_void_p()
+    check(lib().t3ns_append_phys(C.byref(desc), C.byref(old), C.byref(h)), "t3ns_append_phys")
+    return ROps(h)
 
 
 def plan_block_dims(desc, ops, site):
