@@ -220,6 +220,43 @@ int t3ns_append_phys(const t3ns_append_desc* desc, const t3ns_rops_desc* old, t3
 int t3ns_rops_view(const t3ns_rops* r, t3ns_rops_desc* d);
 int t3ns_rops_destroy(t3ns_rops* r);
 
+/* ------------------------------------------------------------------------
+ * Renormalisation: update_rOperators_physical (src/rOperators_pUpdate.c:
+ * 364-387 with make_usb_to_osb 114-138, select_site_blocks 210-271,
+ * get_upd_aide 273-321, pUpdate_block 323-362): physical operators on an
+ * internal bond become operators on the new (truncated) bond,
+ *   left:  O' = T_herm^T O T,   right: O' = T_herm O T^T,
+ * with T the optimised one-site tensor and the adjoint/pUpdate prefactors.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int nrSyms;
+    int sgs[T3NS_MAX_SYMMETRIES];
+    int is_left;
+    t3ns_symsecs_desc site_ss[3];    /* sectors of the tensor's bonds (the updated bond truncated) */
+    t3ns_symsecs_desc internal_ss;   /* sectors the physical operators were built on */
+    t3ns_symsecs_desc mpo;
+    /* the one-site tensor */
+    int T_nrblocks;
+    const int64_t* T_qnumbers;       /* [T_nrblocks] over site_ss */
+    const int64_t* T_beginblock;     /* [T_nrblocks + 1] */
+    const double* T_tel;
+    /* structure of the updated operators (one quantum number per block over
+     * (bra, ket, MPO) of the updated bond), before empty blocks are dropped */
+    int nrhss;
+    const int* begin_blocks_of_hss;
+    const int64_t* qnumbers;
+    const int64_t* const* hss_beginblock;
+} t3ns_pupdate_desc;
+
+int t3ns_update_physical(const t3ns_pupdate_desc* desc, const t3ns_rops_desc* pops,
+                         t3ns_rops** out);
+
+/* Work of the device execution that produced `r`: algorithmic flops, bytes
+ * (inputs read once, output written once) and the kernels' elapsed time.
+ * No counterpart in the reference (it reports wall-clock timers only,
+ * src/optimize_network.c:255-279). */
+int t3ns_rops_work(const t3ns_rops* r, double* flops, double* bytes, double* kernel_ms);
+
 /* Leg dimensions of every block of the multisite tensor in operator-leg
  * order (dims[3*b + leg]; leg 2 is 1 for two-site DMRG objects), as
  * fill_indexes derives them (src/Heff.c:695-739).  Returns the block count

@@ -58,6 +58,16 @@ int t3ns_step_random(t3ns_step** s, t3ns_ctx* c, const int* sites, int nsites, u
  * descriptor.  Pointers stay valid until the step is destroyed. */
 int t3ns_step_append_desc(t3ns_step* s, int leg, uint64_t seed, t3ns_append_desc* d,
                           t3ns_rops_desc* old);
+/* An update_rOperators_physical problem for physical leg `leg` of a step:
+ * the bond's sectors are replaced by a truncated set (a seeded random
+ * subset of the internal sectors with dims 0..maxdim, as a decomposition
+ * leaves them; maxdim 0: all sectors at the full dimension of the product
+ * space, no truncation), the one-site tensor of the physical site gets
+ * N(0,1) entries, and the step's physical operators of that leg are the
+ * input.  The context is left as it was.  The arrays live in the step
+ * until the next call for the same leg with another seed or maxdim. */
+int t3ns_step_pupdate_desc(t3ns_step* s, int leg, uint64_t seed, int maxdim, t3ns_pupdate_desc* d,
+                           t3ns_rops_desc* pops);
 int t3ns_step_destroy(t3ns_step* s);
 int64_t t3ns_step_size(const t3ns_step* s);
 /* Fills the Heff boundary descriptors; pointers stay valid while s lives. */

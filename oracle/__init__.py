@@ -23,7 +23,7 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIBPATH):
             raise RuntimeError("oracle not built: run oracle/build.py")
-        from t3ns_amd._lib import AppendDesc, HeffDesc, RopsDesc, SiteDesc
+        from t3ns_amd._lib import AppendDesc, HeffDesc, PUpdateDesc, RopsDesc, SiteDesc
         L = C.CDLL(LIBPATH)
         P = C.c_void_p
         dp = C.POINTER(C.c_double)
@@ -53,6 +53,8 @@ def lib():
             "oracle_heff_plan_table": (C.c_int64, [P, ip, dp]),
             "oracle_heff_destroy": (None, [P]),
             "oracle_append_phys": (C.c_int64, [C.POINTER(AppendDesc), C.POINTER(RopsDesc), i64, dp]),
+            "oracle_update_physical": (C.c_int64, [C.POINTER(PUpdateDesc), C.POINTER(RopsDesc),
+                                                   i64, dp]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -149,6 +151,30 @@ def append_phys(desc, old):
     out, pb, pt = [], 0, 0
     for k in range(nops):
         h = desc.hss_of_new[k]
+        nb = desc.begin_blocks_of_hss[h + 1] - desc.begin_blocks_of_hss[h]
+        b = bb[pb:pb + nb + 1].copy()
+        out.append((b, tel[pt:pt + int(b[-1])].copy()))
+        pb += nb + 1
+        pt += int(b[-1])
+    return out
+
+
+def update_physical(desc, pops):
+    """Oracle update_rOperators_physical: [(beginblock, elements)] per operator."""
+    import ctypes as _C
+    nbb = 0
+    for k in range(pops.nrops):
+        h = pops.hss_of_ops[k]
+        nbb += desc.begin_blocks_of_hss[h + 1] - desc.begin_blocks_of_hss[h] + 1
+    total = lib().oracle_update_physical(_C.byref(desc), _C.byref(pops), None, None)
+    bb = np.zeros(max(nbb, 1), dtype=np.int64)
+    tel = np.zeros(max(total, 1))
+    lib().oracle_update_physical(_C.byref(desc), _C.byref(pops),
+                                 bb.ctypes.data_as(_C.POINTER(_C.c_int64)),
+                                 tel.ctypes.data_as(_C.POINTER(_C.c_double)))
+    out, pb, pt = [], 0, 0
+    for k in range(pops.nrops):
+        h = pops.hss_of_ops[k]
         nb = desc.begin_blocks_of_hss[h + 1] - desc.begin_blocks_of_hss[h]
         b = bb[pb:pb + nb + 1].copy()
         out.append((b, tel[pt:pt + int(b[-1])].copy()))

@@ -81,6 +81,12 @@ void oracle_heff_destroy(oracle_heff* h);
 int64_t oracle_append_phys(const t3ns_append_desc* d, const t3ns_rops_desc* old,
                            int64_t* beginblock, double* tel);
 
+/* update_rOperators_physical (src/rOperators_pUpdate.c:17-387) on the
+ * descriptor of include/t3ns_amd.h; output as oracle_append_phys, one
+ * operator per input operator. */
+int64_t oracle_update_physical(const t3ns_pupdate_desc* d, const t3ns_rops_desc* pops,
+                               int64_t* beginblock, double* tel);
+
 #ifdef __cplusplus
 }
 #endif

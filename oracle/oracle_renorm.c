@@ -181,3 +181,184 @@ int64_t oracle_append_phys(const t3ns_append_desc* d, const t3ns_rops_desc* old,
         free(u_hss);
         return total;
 }
+
+/* ------------------------------------------------------------------------
+ * update_rOperators_physical (src/rOperators_pUpdate.c:364-387):
+ * make_usb_to_osb (114-138), select_site_blocks (210-271), the dgemm order
+ * of get_upd_aide (273-321), pUpdate_block (323-362), cleanup_update's
+ * kick_zero_blocks (179-188).
+ * ---------------------------------------------------------------------- */
+
+static int search_irrep(const int* irrep, const t3ns_symsecs_desc* ss, int nrsy)
+{
+        for (int i = 0; i < ss->nrSecs; ++i) {
+                int k = 0;
+                for (; k < nrsy; ++k)
+                        if (irrep_of(ss, i)[k] != irrep[k]) break;
+                if (k == nrsy) return i;
+        }
+        return -1;
+}
+
+/* Th^T (left) or Th (right) as an N0 x M0 matrix */
+static double adj_el(const double* th, int il, int N0, int M0, int n, int m)
+{
+        return il ? th[m + (int64_t) n * M0] : th[n + (int64_t) m * N0];
+}
+
+/* T (left) or T^T (right) as an M1 x N1 matrix */
+static double site_el(const double* t, int il, int M1, int N1, int m, int n)
+{
+        return il ? t[m + (int64_t) n * M1] : t[n + (int64_t) m * N1];
+}
+
+int64_t oracle_update_physical(const t3ns_pupdate_desc* d, const t3ns_rops_desc* po,
+                               int64_t* out_bb, double* out_tel)
+{
+        const int il = d->is_left;
+        const int ns = d->nrSyms;
+        const int ub = il ? 2 : 0;
+        t3ns_symsecs_desc uss[3] = {d->site_ss[ub], d->site_ss[ub], d->mpo};
+        t3ns_symsecs_desc oss0[3] = {d->site_ss[0], d->site_ss[1], d->site_ss[2]};
+        oss0[ub] = d->internal_ss;
+        t3ns_symsecs_desc oss2[3] = {d->internal_ss, d->internal_ss, d->mpo};
+
+        const int N = d->begin_blocks_of_hss[d->nrhss];
+        const int NN = po->begin_blocks_of_hss[po->nrhss];
+        /* make_usb_to_osb; entries the reference leaves unset (no old block
+         * above the new one) are taken as NN */
+        int* usb_to_osb = malloc(sizeof(int) * (N + 1));
+        for (int i = 0; i <= N; ++i) usb_to_osb[i] = NN;
+        usb_to_osb[0] = 0;
+        int osb = 0;
+        for (int usb = 0; usb < N; ++usb) {
+                int id[3], nid[3];
+                indexize3(id, d->qnumbers[usb], uss);
+                for (int k = 0; k < 2; ++k) nid[k] = search_irrep(irrep_of(&uss[k], id[k]), &oss2[k], ns);
+                nid[2] = id[2];
+                const int64_t qn = qntypize3(nid, oss2);
+                for (; osb < NN; ++osb)
+                        if (qn < po->qnumbers[3 * osb + 2]) {
+                                usb_to_osb[usb + 1] = osb;
+                                break;
+                        }
+        }
+
+        double** ntel = malloc(sizeof(double*) * (po->nrops + 1));
+        for (int k = 0; k < po->nrops; ++k) {
+                const int h = po->hss_of_ops[k];
+                const int nb = d->begin_blocks_of_hss[h + 1] - d->begin_blocks_of_hss[h];
+                ntel[k] = calloc((size_t) d->hss_beginblock[h][nb] + 1, sizeof(double));
+        }
+
+        for (int usb = 0; usb < N; ++usb) {
+                int newhss = 0;
+                while (usb >= d->begin_blocks_of_hss[newhss + 1]) ++newhss;
+                const int usb2 = usb - d->begin_blocks_of_hss[newhss];
+                for (int o = usb_to_osb[usb]; o < usb_to_osb[usb + 1]; ++o) {
+                        const int osb2 = o - po->begin_blocks_of_hss[newhss];
+                        /* select_site_blocks */
+                        const int64_t* cqn = po->qnumbers + 3 * o;
+                        int ids[3][3];
+                        indexize3(ids[0], cqn[0], oss0);
+                        indexize3(ids[1], cqn[1], oss0);
+                        indexize3(ids[2], cqn[2], oss2);
+                        int Tids[2][3], bad = 0;
+                        for (int k = 0; k < 3; ++k) {
+                                Tids[0][k] = search_irrep(irrep_of(&oss0[k], ids[1][k]), &d->site_ss[k], ns);
+                                Tids[1][k] = search_irrep(irrep_of(&oss0[k], ids[0][k]), &d->site_ss[k], ns);
+                                bad |= Tids[0][k] < 0 || Tids[1][k] < 0;
+                        }
+                        if (bad) continue;
+                        const int tsb = bsearch_qn(qntypize3(Tids[0], d->site_ss), d->T_qnumbers, d->T_nrblocks);
+                        const int thsb = bsearch_qn(qntypize3(Tids[1], d->site_ss), d->T_qnumbers, d->T_nrblocks);
+                        if (tsb < 0 || thsb < 0) continue;
+                        if (d->T_beginblock[tsb + 1] == d->T_beginblock[tsb] ||
+                            d->T_beginblock[thsb + 1] == d->T_beginblock[thsb])
+                                continue;
+                        const double* T = d->T_tel + d->T_beginblock[tsb];
+                        const double* Th = d->T_tel + d->T_beginblock[thsb];
+                        const int N0 = d->site_ss[2 * il].dims[Tids[1][2 * il]];
+                        const int N1 = d->site_ss[2 * il].dims[Tids[0][2 * il]];
+                        const int M0 = d->site_ss[2 * !il].dims[Tids[1][2 * !il]] * d->site_ss[1].dims[Tids[1][1]];
+                        const int M1 = d->site_ss[2 * !il].dims[Tids[0][2 * !il]] * d->site_ss[1].dims[Tids[0][1]];
+                        int irr[3][3][T3NS_MAX_SYMMETRIES];
+                        for (int j = 0; j < 3; ++j) {
+                                memcpy(irr[0][j], irrep_of(&oss0[j], ids[0][j]), sizeof(irr[0][j]));
+                                memcpy(irr[1][j], irrep_of(&oss0[j], ids[1][j]), sizeof(irr[1][j]));
+                                memcpy(irr[2][j], irrep_of(&oss2[j], ids[2][j]), sizeof(irr[2][j]));
+                        }
+                        const double pref = oracle_prefactor_adjoint(&irr[0][0][0], il ? '3' : '1', d->sgs, ns) *
+                                            oracle_prefactor_pUpdate(&irr[0][0][0], il, d->sgs, ns);
+                        /* dgemm order of get_upd_aide */
+                        const int order = (int64_t) N1 * M1 * (M0 + N1) > (int64_t) M0 * N1 * (M1 + N0);
+                        double* work = malloc(sizeof(double) * ((order ? (size_t) M0 * N1 : (size_t) N0 * M1) + 1));
+                        for (int k = 0; k < po->nrops; ++k) {
+                                if (po->hss_of_ops[k] != newhss) continue;
+                                const int64_t* obb = po->beginblock[k];
+                                const int64_t* nbb = d->hss_beginblock[newhss];
+                                if (obb[osb2 + 1] == obb[osb2] || nbb[usb2 + 1] == nbb[usb2]) continue;
+                                const double* O = po->tel[k] + obb[osb2];
+                                double* U = ntel[k] + nbb[usb2];
+                                if (order) {
+                                        /* work = O T (M0 x N1); U += pref Th^T work */
+                                        for (int j = 0; j < N1; ++j)
+                                                for (int i = 0; i < M0; ++i) {
+                                                        double s = 0;
+                                                        for (int m = 0; m < M1; ++m)
+                                                                s += O[i + (int64_t) m * M0] * site_el(T, il, M1, N1, m, j);
+                                                        work[i + (int64_t) j * M0] = s;
+                                                }
+                                        for (int j = 0; j < N1; ++j)
+                                                for (int i = 0; i < N0; ++i) {
+                                                        double s = 0;
+                                                        for (int m = 0; m < M0; ++m)
+                                                                s += adj_el(Th, il, N0, M0, i, m) * work[m + (int64_t) j * M0];
+                                                        U[i + (int64_t) j * N0] += pref * s;
+                                                }
+                                } else {
+                                        /* work = Th^T O (N0 x M1); U += pref work T */
+                                        for (int j = 0; j < M1; ++j)
+                                                for (int i = 0; i < N0; ++i) {
+                                                        double s = 0;
+                                                        for (int m = 0; m < M0; ++m)
+                                                                s += adj_el(Th, il, N0, M0, i, m) * O[m + (int64_t) j * M0];
+                                                        work[i + (int64_t) j * N0] = s;
+                                                }
+                                        for (int j = 0; j < N1; ++j)
+                                                for (int i = 0; i < N0; ++i) {
+                                                        double s = 0;
+                                                        for (int m = 0; m < M1; ++m)
+                                                                s += work[i + (int64_t) m * N0] * site_el(T, il, M1, N1, m, j);
+                                                        U[i + (int64_t) j * N0] += pref * s;
+                                                }
+                                }
+                        }
+                        free(work);
+                }
+        }
+        /* kick_zero_blocks and output */
+        int64_t total = 0, nbbo = 0;
+        for (int k = 0; k < po->nrops; ++k) {
+                const int h = po->hss_of_ops[k];
+                const int nb = d->begin_blocks_of_hss[h + 1] - d->begin_blocks_of_hss[h];
+                const int64_t* bb = d->hss_beginblock[h];
+                int64_t pos = 0;
+                if (out_bb) out_bb[nbbo] = 0;
+                for (int b = 0; b < nb; ++b) {
+                        int nonzero = 0;
+                        for (int64_t j = bb[b]; j < bb[b + 1]; ++j)
+                                if (!(fabs(ntel[k][j]) < 1e-20)) { nonzero = 1; break; }
+                        const int64_t n = nonzero ? bb[b + 1] - bb[b] : 0;
+                        if (out_tel && n) memcpy(out_tel + total + pos, ntel[k] + bb[b], sizeof(double) * n);
+                        pos += n;
+                        if (out_bb) out_bb[nbbo + b + 1] = pos;
+                }
+                nbbo += nb + 1;
+                total += pos;
+        }
+        for (int k = 0; k < po->nrops; ++k) free(ntel[k]);
+        free(ntel);
+        free(usb_to_osb);
+        return total;
+}

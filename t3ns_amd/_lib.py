@@ -58,6 +58,14 @@ class AppendDesc(C.Structure):
                 ("qnumbers", c_i64_p), ("hss_beginblock", C.POINTER(c_i64_p))]
 
 
+class PUpdateDesc(C.Structure):
+    _fields_ = [("nrSyms", C.c_int), ("sgs", C.c_int * MAX_SYMMETRIES), ("is_left", C.c_int),
+                ("site_ss", SymsecsDesc * 3), ("internal_ss", SymsecsDesc), ("mpo", SymsecsDesc),
+                ("T_nrblocks", C.c_int), ("T_qnumbers", c_i64_p), ("T_beginblock", c_i64_p),
+                ("T_tel", c_dbl_p), ("nrhss", C.c_int), ("begin_blocks_of_hss", c_int_p),
+                ("qnumbers", c_i64_p), ("hss_beginblock", C.POINTER(c_i64_p))]
+
+
 # void (*)(const double* x, double* y, void* stream, void* vdata)
 DEV_MATVEC = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)
 
@@ -83,6 +91,11 @@ def lib():
         "t3ns_heff_diagonal": (C.c_int, [P, c_dbl_p]),
         "t3ns_heff_destroy": (C.c_int, [P]),
         "t3ns_append_phys": (C.c_int, [C.POINTER(AppendDesc), C.POINTER(RopsDesc), C.POINTER(P)]),
+        "t3ns_update_physical": (C.c_int, [C.POINTER(PUpdateDesc), C.POINTER(RopsDesc),
+                                           C.POINTER(P)]),
+        "t3ns_rops_work": (C.c_int, [P, c_dbl_p, c_dbl_p, c_dbl_p]),
+        "t3ns_step_pupdate_desc": (C.c_int, [P, C.c_int, C.c_uint64, C.c_int,
+                                             C.POINTER(PUpdateDesc), C.POINTER(RopsDesc)]),
         "t3ns_rops_view": (C.c_int, [P, C.POINTER(RopsDesc)]),
         "t3ns_rops_destroy": (C.c_int, [P]),
         "t3ns_step_append_desc": (C.c_int, [P, C.c_int, C.c_uint64, C.POINTER(AppendDesc),

@@ -343,6 +343,7 @@ int t3ns_davidson_dev(int64_t size, double* dx, const double* ddiag, t3ns_dev_ma
 
 struct t3ns_rops {
     ROperators r;
+    RenormWork work;
     std::vector<int64_t> bb_flat;
     std::vector<const int64_t*> bb_ptr;
     std::vector<const double*> tel_ptr;
@@ -417,6 +418,57 @@ int t3ns_append_phys(const t3ns_append_desc* d, const t3ns_rops_desc* old, t3ns_
         res->views();
         *out = res.release();
     });
+}
+
+int t3ns_update_physical(const t3ns_pupdate_desc* d, const t3ns_rops_desc* pops, t3ns_rops** out) {
+    if (!d || !pops || !out) return fail(T3NS_ERR_INVALID, "null argument");
+    return guarded([&] {
+        const ROperators po = rops_from(*pops);
+        PUpdateInput in;
+        in.sym.nrSyms = d->nrSyms;
+        for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) in.sym.sgs[i] = d->sgs[i];
+        in.is_left = d->is_left;
+        for (int k = 0; k < 3; ++k) in.site_ss[k] = symsecs_from(d->site_ss[k]);
+        in.iss = symsecs_from(d->internal_ss);
+        in.mpo = symsecs_from(d->mpo);
+        if (d->T_nrblocks < 0 || (d->T_nrblocks && (!d->T_qnumbers || !d->T_beginblock || !d->T_tel)))
+            throw std::invalid_argument("pUpdate: bad site tensor");
+        SiteTensor T;
+        T.nrsites = 1;
+        T.nrblocks = d->T_nrblocks;
+        T.qnumbers.assign(d->T_qnumbers, d->T_qnumbers + T.nrblocks);
+        T.beginblock.assign(d->T_beginblock, d->T_beginblock + T.nrblocks + 1);
+        T.tel.assign(d->T_tel, d->T_tel + T.size());
+        const int H = in.mpo.nrSecs;
+        if (d->nrhss != H) throw std::invalid_argument("pUpdate: nrhss differs from the MPO sectors");
+        in.nrhss = d->nrhss;
+        in.begin_blocks_of_hss.assign(d->begin_blocks_of_hss, d->begin_blocks_of_hss + H + 1);
+        const int nb = in.begin_blocks_of_hss[H];
+        in.qnumbers.assign(d->qnumbers, d->qnumbers + nb);
+        in.hss_bb.resize(H);
+        for (int h = 0; h < H; ++h) {
+            const int n = in.begin_blocks_of_hss[h + 1] - in.begin_blocks_of_hss[h];
+            in.hss_bb[h].assign(d->hss_beginblock[h], d->hss_beginblock[h] + n + 1);
+        }
+        if (po.nrhss != H || po.is_left != d->is_left)
+            throw std::invalid_argument("pUpdate: operators do not match the descriptor");
+        in.T = &T;
+        in.pops = &po;
+        const PUpdatePlan plan = plan_pupdate(in);
+        auto res = std::make_unique<t3ns_rops>();
+        hipStream_t s = nullptr;
+        res->r = run_pupdate(plan, po, T, s, &res->work);
+        res->views();
+        *out = res.release();
+    });
+}
+
+int t3ns_rops_work(const t3ns_rops* r, double* flops, double* bytes, double* kernel_ms) {
+    if (!r) return fail(T3NS_ERR_INVALID, "null argument");
+    if (flops) *flops = r->work.flops;
+    if (bytes) *bytes = r->work.bytes;
+    if (kernel_ms) *kernel_ms = r->work.kernel_ms;
+    return T3NS_OK;
 }
 
 int t3ns_rops_view(const t3ns_rops* r, t3ns_rops_desc* d) {

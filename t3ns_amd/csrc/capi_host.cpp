@@ -109,9 +109,24 @@ struct AppendStore {
     bool built = false;
 };
 
+struct PUpdateStore {
+    SymsecsFlat site_ss[3], iss, mpo;
+    SiteTensor T;
+    std::vector<int> bbh;
+    std::vector<QN> qnumbers;
+    std::vector<std::vector<int64_t>> hss_bb;
+    std::vector<const int64_t*> hss_bb_ptr;
+    std::vector<const int64_t*> bb_ptr;
+    std::vector<const double*> tel_ptr;
+    int is_left = 0, nrhss = 0, maxdim = -1;
+    uint64_t seed = 0;
+    bool built = false;
+};
+
 struct t3ns_step {
     t3ns_ctx* ctx = nullptr;
     AppendStore app[3];
+    PUpdateStore pup[3];
     SiteTensor site;
     ROperators ops[3];
     InstructionSet merge_raw;
@@ -444,6 +459,127 @@ int t3ns_step_append_desc(t3ns_step* st, int leg, uint64_t seed, t3ns_append_des
         old->hss_of_ops = a.old.hss_of_ops.data();
         old->beginblock = a.old_bb_ptr.data();
         old->tel = a.old_tel_ptr.data();
+    });
+}
+
+int t3ns_step_pupdate_desc(t3ns_step* st, int leg, uint64_t seed, int maxdim,
+                           t3ns_pupdate_desc* d, t3ns_rops_desc* pops) {
+    if (!st || !d || !pops || leg < 0 || leg > 2 || maxdim < 0) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        Context& c = *st->ctx->c;
+        const ROperators& P = st->ops[leg];
+        if (P.nrhss <= 0 || !P.P_operator)
+            throw std::invalid_argument("step leg is not a physical operator");
+        PUpdateStore& u = st->pup[leg];
+        if (u.built && (u.seed != seed || u.maxdim != maxdim)) u = PUpdateStore{};
+        if (!u.built) {
+            u.seed = seed;
+            u.maxdim = maxdim;
+            const int is_left = P.is_left;
+            const int pbond = P.bond;
+            const int site = c.netw.bonds[pbond][!is_left];
+            int sb[3];
+            c.netw.bonds_of_site(site, sb);
+            // The internal sectors the physical operators were built on and
+            // a truncated set as a decomposition of the optimised tensor
+            // leaves it: sectors dropped, kept with dim 0, or kept with
+            // dims 1..maxdim.
+            const Symsecs iss = c.symsecs(pbond);
+            Symsecs tr;
+            tr.bond = iss.bond;
+            Rng rng(seed);
+            Symsecs& slot = c.symsecs_mut(pbond);
+            if (maxdim == 0) {
+                // no truncation: every sector keeps the full dimension of the
+                // (far bond x physical) product space that couples to it
+                tr = iss;
+                for (auto& v : tr.dims) v = 1;
+                slot = tr;
+                SiteTensor unit;
+                try {
+                    unit = init_1siteTensor(c, site);
+                } catch (...) {
+                    slot = iss;
+                    throw;
+                }
+                const int n0 = c.symsecs(sb[0]).nrSecs, n1 = c.symsecs(sb[1]).nrSecs;
+                std::fill(tr.dims.begin(), tr.dims.end(), 0);
+                for (int b = 0; b < unit.nrblocks; ++b) {
+                    const QN q = unit.qnumbers[b];
+                    const int sec = is_left ? (int)(q / ((QN)n0 * n1)) : (int)(q % n0);
+                    tr.dims[sec] += (int)unit.block_size(b);
+                }
+                tr.totaldims = 0;
+                for (int v : tr.dims) tr.totaldims += v;
+            } else {
+                for (int i = 0; i < iss.nrSecs; ++i) {
+                    const double r = rng.uniform();
+                    if (r < 0.1) continue;
+                    const int dim = r < 0.2 ? 0 : 1 + (int)(rng.next() % (uint64_t)maxdim);
+                    tr.irreps.push_back(iss.irreps[i]);
+                    tr.dims.push_back(dim);
+                    tr.fcidims.push_back(iss.fcidims.empty() ? 0.0 : iss.fcidims[i]);
+                    tr.totaldims += dim;
+                }
+                tr.nrSecs = (int)tr.dims.size();
+            }
+            slot = tr;
+            try {
+                u.T = init_1siteTensor(c, site);
+                for (auto& v : u.T.tel) v = rng.normal();
+                std::vector<std::vector<int64_t>> tmpbb;
+                ROperators ns = init_rOperators_structure(c, pbond, is_left, false, &tmpbb);
+                u.nrhss = ns.nrhss;
+                u.bbh = ns.begin_blocks_of_hss;
+                u.qnumbers = ns.qnumbers;
+                u.hss_bb = tmpbb;
+                for (int k = 0; k < 3; ++k) u.site_ss[k].set(c.symsecs(sb[k]));
+            } catch (...) {
+                slot = iss;
+                throw;
+            }
+            slot = iss;
+            u.iss.set(iss);
+            u.mpo.set(c.symsecs(-1));
+            u.is_left = is_left;
+            for (auto& v : u.hss_bb)
+                if (v.empty()) v.assign(1, 0);
+            u.hss_bb_ptr.resize(u.hss_bb.size());
+            for (size_t h = 0; h < u.hss_bb.size(); ++h) u.hss_bb_ptr[h] = u.hss_bb[h].data();
+            u.bb_ptr.resize(P.nrops);
+            u.tel_ptr.resize(P.nrops);
+            for (int k = 0; k < P.nrops; ++k) {
+                u.bb_ptr[k] = P.beginblock[k].data();
+                u.tel_ptr[k] = P.tel.data() + P.op_offset[k];
+            }
+            u.built = true;
+        }
+        std::memset(d, 0, sizeof(*d));
+        d->nrSyms = c.sym.nrSyms;
+        for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) d->sgs[i] = c.sym.sgs[i];
+        d->is_left = u.is_left;
+        for (int k = 0; k < 3; ++k) d->site_ss[k] = u.site_ss[k].desc;
+        d->internal_ss = u.iss.desc;
+        d->mpo = u.mpo.desc;
+        d->T_nrblocks = u.T.nrblocks;
+        d->T_qnumbers = u.T.qnumbers.data();
+        d->T_beginblock = u.T.beginblock.data();
+        d->T_tel = u.T.tel.data();
+        d->nrhss = u.nrhss;
+        d->begin_blocks_of_hss = u.bbh.data();
+        d->qnumbers = u.qnumbers.data();
+        d->hss_beginblock = u.hss_bb_ptr.data();
+        std::memset(pops, 0, sizeof(*pops));
+        pops->bond = P.bond;
+        pops->is_left = P.is_left;
+        pops->P_operator = 1;
+        pops->nrhss = P.nrhss;
+        pops->begin_blocks_of_hss = P.begin_blocks_of_hss.data();
+        pops->qnumbers = P.qnumbers.data();
+        pops->nrops = P.nrops;
+        pops->hss_of_ops = P.hss_of_ops.data();
+        pops->beginblock = u.bb_ptr.data();
+        pops->tel = u.tel_ptr.data();
     });
 }
 

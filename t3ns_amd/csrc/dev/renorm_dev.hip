@@ -59,7 +59,7 @@ int grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(8192, n
 
 ROperators run_append(const AppendPlan& plan, const ROperators& old, hipStream_t s) {
     const ROperators& nr = plan.nr;
-    const int64_t nold = old.total_size(), nnew = nr.total_size();
+    const int64_t nnew = nr.total_size();
     // flat block index of every (op, block)
     std::vector<int> flat_of_op(nr.nrops + 1, 0);
     for (int k = 0; k < nr.nrops; ++k) flat_of_op[k + 1] = flat_of_op[k] + nr.nblocks_op(k);
@@ -97,6 +97,321 @@ ROperators run_append(const AppendPlan& plan, const ROperators& old, hipStream_t
     if (nflat)
         T3_HIP_CHECK(hipMemcpyAsync(keep.data(), dkeep.get(), nflat, hipMemcpyDeviceToHost, s));
     T3_HIP_CHECK(hipStreamSynchronize(s));
+    const Compaction cpt = compact_blocks(nr, keep);
+    ROperators out = nr;
+    out.beginblock = cpt.beginblock;
+    out.op_offset = cpt.op_offset;
+    const int64_t ncomp = cpt.op_offset[nr.nrops];
+    out.tel.assign(ncomp, 0.0);
+    if (ncomp) {
+        DevBuf<int64_t> cs, cd, cn;
+        DevBuf<double> dcomp(ncomp);
+        cs.upload(cpt.src, s);
+        cd.upload(cpt.dst, s);
+        cn.upload(cpt.n, s);
+        hipLaunchKernelGGL(gather_blocks_kernel, dim3(grid((int64_t)cpt.n.size())), dim3(kThreads), 0,
+                           s, cs.get(), cd.get(), cn.get(), (int)cpt.n.size(), dnew.get(), dcomp.get());
+        T3_HIP_CHECK(hipMemcpyAsync(out.tel.data(), dcomp.get(), ncomp * sizeof(double),
+                                    hipMemcpyDeviceToHost, s));
+        T3_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    T3_HIP_CHECK(hipGetLastError());
+    return out;
+}
+
+}  // namespace t3
+
+// ---------------------------------------------------------------------------
+// update_rOperators_physical: U[dst] = sum_t pref_t * A_t O_t B_t.
+// ---------------------------------------------------------------------------
+namespace t3 {
+
+namespace {
+
+constexpr int kTile = 64;   // output tile edge
+constexpr int kKStep = 16;  // reduction slab
+
+// Strided operand: element (i, k) at base[off + i * si + k * sk].
+struct Strided {
+    int64_t off;
+    int si, sk;
+};
+
+// Phase 1: X = P Q (M x N, column-major, leading dimension M) per tile.
+struct GemmTile {
+    Strided p, q;  // q as (k, j): off + k * si + j * sk
+    int64_t x_off;
+    int M, N, K;
+    int ti, tj;
+    int pad;
+};
+
+// Phase 2: U tile = sum_t pref_t * R_t S_t.
+struct SumTile {
+    int64_t u_off;
+    int M, N;
+    int ti, tj;
+    int task_begin, task_end;
+    int flat;
+    int pad;
+};
+struct SumTask {
+    Strided r, s;  // s as (k, j)
+    int K;
+    int pad;
+    double pref;
+};
+
+// One K-slab of a 64 x 64 tile product: loads P(64 x 16) and Q(16 x 64)
+// into LDS (zero padded) and accumulates acc[a][b] for rows tx + 16a,
+// columns ty + 16b.  The load pattern follows the unit-stride index so that
+// neighbouring lanes read neighbouring addresses.
+__device__ __forceinline__ void tile_slab(const double* __restrict__ base, const Strided& P,
+                                          const Strided& Q, int M, int N, int K, int i0, int j0,
+                                          int k0, double (*Ps)[kTile + 1],
+                                          double (*Qs)[kTile + 1], double acc[4][4]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        int i, k;
+        if (P.si == 1) {
+            i = t % kTile;
+            k = t / kTile + 4 * r;
+        } else {
+            k = t % kKStep;
+            i = t / kKStep + 16 * r;
+        }
+        const int gi = i0 + i, gk = k0 + k;
+        Ps[k][i] = (gi < M && gk < K) ? base[P.off + (int64_t)gi * P.si + (int64_t)gk * P.sk] : 0.0;
+        int j, kk;
+        if (Q.si == 1) {
+            kk = t % kKStep;
+            j = t / kKStep + 16 * r;
+        } else {
+            j = t % kTile;
+            kk = t / kTile + 4 * r;
+        }
+        const int gj = j0 + j, gkk = k0 + kk;
+        Qs[kk][j] =
+            (gj < N && gkk < K) ? base[Q.off + (int64_t)gkk * Q.si + (int64_t)gj * Q.sk] : 0.0;
+    }
+    __syncthreads();
+    const int tx = t % 16, ty = t / 16;
+#pragma unroll 4
+    for (int k = 0; k < kKStep; ++k) {
+        double pa[4], qb[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) pa[a] = Ps[k][tx + 16 * a];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) qb[b] = Qs[k][ty + 16 * b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = fma(pa[a], qb[b], acc[a][b]);
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256)
+    pupdate_first_kernel(const GemmTile* __restrict__ tiles, int ntiles, double* base) {
+    __shared__ double Ps[kKStep][kTile + 1];
+    __shared__ double Qs[kKStep][kTile + 1];
+    for (int it = blockIdx.x; it < ntiles; it += gridDim.x) {
+        const GemmTile g = tiles[it];
+        const int i0 = g.ti * kTile, j0 = g.tj * kTile;
+        double acc[4][4] = {};
+        for (int k0 = 0; k0 < g.K; k0 += kKStep)
+            tile_slab(base, g.p, g.q, g.M, g.N, g.K, i0, j0, k0, Ps, Qs, acc);
+        const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int j = j0 + ty + 16 * b;
+            if (j >= g.N) continue;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int i = i0 + tx + 16 * a;
+                if (i < g.M) base[g.x_off + i + (int64_t)j * g.M] = acc[a][b];
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+    pupdate_sum_kernel(const SumTile* __restrict__ tiles, int ntiles,
+                       const SumTask* __restrict__ tasks, const double* __restrict__ base,
+                       double* __restrict__ out, unsigned int* __restrict__ keep) {
+    __shared__ double Ps[kKStep][kTile + 1];
+    __shared__ double Qs[kKStep][kTile + 1];
+    for (int it = blockIdx.x; it < ntiles; it += gridDim.x) {
+        const SumTile g = tiles[it];
+        const int i0 = g.ti * kTile, j0 = g.tj * kTile;
+        double u[4][4] = {};
+        for (int t = g.task_begin; t < g.task_end; ++t) {
+            const SumTask tk = tasks[t];
+            double acc[4][4] = {};
+            for (int k0 = 0; k0 < tk.K; k0 += kKStep)
+                tile_slab(base, tk.r, tk.s, g.M, g.N, tk.K, i0, j0, k0, Ps, Qs, acc);
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) u[a][b] = fma(tk.pref, acc[a][b], u[a][b]);
+        }
+        const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+        int any = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int j = j0 + ty + 16 * b;
+            if (j >= g.N) continue;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int i = i0 + tx + 16 * a;
+                if (i < g.M) {
+                    out[g.u_off + i + (int64_t)j * g.M] = u[a][b];
+                    any |= fabs(u[a][b]) >= 1e-20;
+                }
+            }
+        }
+        any = __syncthreads_or(any);
+        if (threadIdx.x == 0 && any) atomicOr(&keep[g.flat], 1u);
+    }
+}
+
+int tiles_of(int n) { return (n + kTile - 1) / kTile; }
+
+}  // namespace
+
+ROperators run_pupdate(const PUpdatePlan& plan, const ROperators& pops, const SiteTensor& T,
+                       hipStream_t s, RenormWork* work) {
+    const ROperators& nr = plan.nr;
+    const int64_t nT = T.size(), nold = pops.total_size(), nnew = nr.total_size();
+    std::vector<int> flat_of_op(nr.nrops + 1, 0);
+    for (int k = 0; k < nr.nrops; ++k) flat_of_op[k + 1] = flat_of_op[k] + nr.nblocks_op(k);
+    const int nflat = flat_of_op[nr.nrops];
+
+    // Device arena: [T | old operators | workspace of the first products].
+    const int64_t oldoff = nT, workoff = nT + nold;
+    std::vector<GemmTile> first;
+    std::vector<SumTile> sums;
+    std::vector<SumTask> stasks;
+    int64_t wsize = 0;
+    double flops = 0;
+    for (const PUpdateGroup& g : plan.groups) {
+        SumTile st;
+        st.u_off = g.dst;
+        st.task_begin = (int)stasks.size();
+        st.flat = flat_of_op[g.op] + g.block;
+        st.pad = 0;
+        int N0 = -1, N1 = -1;
+        for (int ti = g.task_begin; ti < g.task_end; ++ti) {
+            const PUpdateTask& t = plan.tasks[ti];
+            N0 = t.N0;
+            N1 = t.N1;
+            const int64_t M0 = t.M0, M1 = t.M1;
+            // A = Th^T (left) or Th (right), N0 x M0; B = T (left) or T^T
+            // (right), M1 x N1; O is M0 x M1 column-major.
+            const Strided A = nr.is_left ? Strided{t.th, t.M0, 1} : Strided{t.th, 1, t.N0};
+            const Strided B = nr.is_left ? Strided{t.t, 1, t.M1} : Strided{t.t, t.N1, 1};
+            const Strided O = Strided{oldoff + t.src, 1, t.M0};
+            const double costA = (double)N0 * M0 * M1 + (double)N0 * M1 * N1;
+            const double costB = (double)M0 * M1 * N1 + (double)N0 * M0 * N1;
+            SumTask sk;
+            sk.pref = t.pref;
+            sk.pad = 0;
+            GemmTile ft;
+            ft.x_off = workoff + wsize;
+            ft.pad = 0;
+            if (costA <= costB) {
+                // W = A O (N0 x M1), then W B
+                ft.p = A;
+                ft.q = O;
+                ft.M = N0;
+                ft.N = (int)M1;
+                ft.K = (int)M0;
+                sk.r = Strided{ft.x_off, 1, N0};
+                sk.s = B;
+                sk.K = (int)M1;
+                wsize += (int64_t)N0 * M1;
+                flops += 2 * costA;
+            } else {
+                // V = O B (M0 x N1), then A V
+                ft.p = O;
+                ft.q = B;
+                ft.M = (int)M0;
+                ft.N = N1;
+                ft.K = (int)M1;
+                sk.r = A;
+                sk.s = Strided{ft.x_off, 1, (int)M0};
+                sk.K = (int)M0;
+                wsize += M0 * N1;
+                flops += 2 * costB;
+            }
+            for (int a = 0; a < tiles_of(ft.M); ++a)
+                for (int b = 0; b < tiles_of(ft.N); ++b) {
+                    ft.ti = a;
+                    ft.tj = b;
+                    first.push_back(ft);
+                }
+            stasks.push_back(sk);
+        }
+        st.task_end = (int)stasks.size();
+        st.M = N0;
+        st.N = N1;
+        for (int a = 0; a < tiles_of(N0); ++a)
+            for (int b = 0; b < tiles_of(N1); ++b) {
+                st.ti = a;
+                st.tj = b;
+                sums.push_back(st);
+            }
+    }
+    if (workoff + wsize > ((int64_t)1 << 33))
+        throw std::runtime_error("pUpdate: device workspace above 64 GiB");
+
+    DevBuf<double> arena(std::max<int64_t>(1, workoff + wsize));
+    if (nT) T3_HIP_CHECK(hipMemcpyAsync(arena.get(), T.tel.data(), nT * sizeof(double),
+                                        hipMemcpyHostToDevice, s));
+    if (nold)
+        T3_HIP_CHECK(hipMemcpyAsync(arena.get() + oldoff, pops.tel.data(), nold * sizeof(double),
+                                    hipMemcpyHostToDevice, s));
+    DevBuf<GemmTile> dfirst;
+    DevBuf<SumTile> dsums;
+    DevBuf<SumTask> dtasks;
+    DevBuf<double> dnew(std::max<int64_t>(1, nnew));
+    DevBuf<unsigned int> dkeep(std::max(1, nflat));
+    dfirst.upload(first, s);
+    dsums.upload(sums, s);
+    dtasks.upload(stasks, s);
+    hipLaunchKernelGGL(fill_zero<unsigned int>, dim3(grid((nflat + 255) / 256)), dim3(256), 0, s,
+                       dkeep.get(), (int64_t)nflat);
+    hipEvent_t e0, e1;
+    T3_HIP_CHECK(hipEventCreate(&e0));
+    T3_HIP_CHECK(hipEventCreate(&e1));
+    T3_HIP_CHECK(hipEventRecord(e0, s));
+    if (!first.empty())
+        hipLaunchKernelGGL(pupdate_first_kernel, dim3(grid((int64_t)first.size())), dim3(256), 0,
+                           s, dfirst.get(), (int)first.size(), arena.get());
+    if (!sums.empty())
+        hipLaunchKernelGGL(pupdate_sum_kernel, dim3(grid((int64_t)sums.size())), dim3(256), 0, s,
+                           dsums.get(), (int)sums.size(), dtasks.get(), arena.get(), dnew.get(),
+                           dkeep.get());
+    T3_HIP_CHECK(hipEventRecord(e1, s));
+    std::vector<unsigned int> keep32(nflat);
+    if (nflat)
+        T3_HIP_CHECK(hipMemcpyAsync(keep32.data(), dkeep.get(), nflat * sizeof(unsigned int),
+                                    hipMemcpyDeviceToHost, s));
+    T3_HIP_CHECK(hipStreamSynchronize(s));
+    float ms = 0;
+    T3_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    T3_HIP_CHECK(hipGetLastError());
+    if (work) {
+        work->flops = flops;
+        work->bytes = 8.0 * (double)(nT + nold + nnew);
+        work->kernel_ms = ms;
+    }
+
+    std::vector<unsigned char> keep(nflat);
+    for (int i = 0; i < nflat; ++i) keep[i] = keep32[i] != 0;
     const Compaction cpt = compact_blocks(nr, keep);
     ROperators out = nr;
     out.beginblock = cpt.beginblock;

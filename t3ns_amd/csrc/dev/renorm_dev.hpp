@@ -12,4 +12,17 @@ namespace t3 {
 // (src/sparseblocks.c:73-107), and the result returned with host data.
 ROperators run_append(const AppendPlan& plan, const ROperators& old, hipStream_t s);
 
+// Work of one device execution: algorithmic flops and bytes (inputs read
+// once, outputs written once) and the summed kernel time.
+struct RenormWork {
+    double flops = 0, bytes = 0, kernel_ms = 0;
+};
+
+// Runs an update_rOperators_physical plan on the device: every contribution
+// is two GEMMs (the cheaper association of A*O*B is chosen per block pair),
+// the first into a device workspace, the second summed over the block's
+// contributions in plan order; all-zero blocks are then dropped.
+ROperators run_pupdate(const PUpdatePlan& plan, const ROperators& pops, const SiteTensor& T,
+                       hipStream_t s, RenormWork* work);
+
 }  // namespace t3

@@ -168,3 +168,155 @@ Compaction compact_blocks(const ROperators& r, const std::vector<unsigned char>&
 }
 
 }  // namespace t3
+
+namespace t3 {
+
+namespace {
+int search_sector(const int* irrep, const Symsecs& ss, int nrsyms) {
+    for (int i = 0; i < ss.nrSecs; ++i) {
+        int k = 0;
+        for (; k < nrsyms; ++k)
+            if (ss.irreps[i][k] != irrep[k]) break;
+        if (k == nrsyms) return i;
+    }
+    return -1;
+}
+}  // namespace
+
+PUpdatePlan plan_pupdate(const PUpdateInput& in) {
+    const ROperators& po = *in.pops;
+    if (!po.P_operator) throw std::invalid_argument("pUpdate: operators are not physical");
+    const SiteTensor& T = *in.T;
+    const int il = in.is_left;
+    const int ns = in.sym.nrSyms;
+    PUpdatePlan P;
+    ROperators& nr = P.nr;
+    nr.bond = po.bond;
+    nr.is_left = il;
+    nr.P_operator = 0;
+    nr.nrhss = in.nrhss;
+    nr.begin_blocks_of_hss = in.begin_blocks_of_hss;
+    nr.qnumbers = in.qnumbers;
+    nr.nrops = po.nrops;
+    nr.hss_of_ops = po.hss_of_ops;
+    nr.beginblock.resize(nr.nrops);
+    for (int k = 0; k < nr.nrops; ++k) nr.beginblock[k] = in.hss_bb[nr.hss_of_ops[k]];
+    nr.finalize_offsets();
+
+    // uss: (new bond, new bond, MPO); oss[0] = oss[1]: site bonds with the
+    // updated bond internal; oss[2] = (internal, internal, MPO)
+    // (src/rOperators_pUpdate.c:140-176)
+    const int ub = il ? 2 : 0;
+    Symsecs uss[3] = {in.site_ss[ub], in.site_ss[ub], in.mpo};
+    Symsecs oss0[3] = {in.site_ss[0], in.site_ss[1], in.site_ss[2]};
+    oss0[ub] = in.iss;
+    Symsecs oss2[3] = {in.iss, in.iss, in.mpo};
+
+    // make_usb_to_osb: range of old blocks per new block
+    const int N = nr.begin_blocks_of_hss[nr.nrhss];
+    const int NN = po.begin_blocks_of_hss[po.nrhss];
+    std::vector<int> usb_to_osb(N + 1, NN);
+    usb_to_osb[0] = 0;
+    {
+        int osb = 0;
+        for (int usb = 0; usb < N; ++usb) {
+            int id[3], nid[3];
+            indexize3(id, nr.qnumbers[usb], uss);
+            for (int k = 0; k < 2; ++k) nid[k] = search_sector(uss[k].irreps[id[k]].data(), oss2[k], ns);
+            nid[2] = id[2];
+            const QN qn = qntypize3(nid, oss2);
+            for (; osb < NN; ++osb)
+                if (qn < po.qnumbers[3 * osb + 2]) {
+                    usb_to_osb[usb + 1] = osb;
+                    break;
+                }
+        }
+    }
+    std::vector<QN> Tqn(T.qnumbers.begin(), T.qnumbers.end());
+    for (int usb = 0; usb < N; ++usb) {
+        int newhss = 0;
+        while (usb >= nr.begin_blocks_of_hss[newhss + 1]) ++newhss;
+        const int usb2 = usb - nr.begin_blocks_of_hss[newhss];
+        // per old block: site blocks, dims and prefactor (select_site_blocks)
+        struct Aide {
+            int osb2;
+            int64_t th, t;
+            int M0, M1, N0, N1;
+            double pref;
+        };
+        std::vector<Aide> aides;
+        for (int osb = usb_to_osb[usb]; osb < usb_to_osb[usb + 1]; ++osb) {
+            const QN* cqn = &po.qnumbers[3 * osb];
+            int ids[3][3];
+            indexize3(ids[0], cqn[0], oss0);
+            indexize3(ids[1], cqn[1], oss0);
+            indexize3(ids[2], cqn[2], oss2);
+            int Tids[2][3];
+            bool ok = true;
+            for (int k = 0; k < 3; ++k) {
+                Tids[0][k] = search_sector(oss0[k].irreps[ids[1][k]].data(), in.site_ss[k], ns);
+                Tids[1][k] = search_sector(oss0[k].irreps[ids[0][k]].data(), in.site_ss[k], ns);
+                ok = ok && Tids[0][k] >= 0 && Tids[1][k] >= 0;
+            }
+            if (!ok) continue;
+            const QN tq = qntypize3(Tids[0], in.site_ss), thq = qntypize3(Tids[1], in.site_ss);
+            const int tsb = find_qn(Tqn.data(), T.nrblocks, tq);
+            const int thsb = find_qn(Tqn.data(), T.nrblocks, thq);
+            if (tsb < 0 || thsb < 0) continue;
+            if (T.block_size(tsb) == 0 || T.block_size(thsb) == 0) continue;
+            Aide a;
+            a.osb2 = osb - po.begin_blocks_of_hss[newhss];
+            a.th = T.beginblock[thsb];
+            a.t = T.beginblock[tsb];
+            a.N0 = in.site_ss[2 * il].dims[Tids[1][2 * il]];
+            a.N1 = in.site_ss[2 * il].dims[Tids[0][2 * il]];
+            a.M0 = in.site_ss[2 * !il].dims[Tids[1][2 * !il]] * in.site_ss[1].dims[Tids[1][1]];
+            a.M1 = in.site_ss[2 * !il].dims[Tids[0][2 * !il]] * in.site_ss[1].dims[Tids[0][1]];
+            const int* irr[3][3];
+            irr[0][0] = oss0[0].irreps[ids[0][0]].data();
+            irr[0][1] = oss0[1].irreps[ids[0][1]].data();
+            irr[0][2] = oss0[2].irreps[ids[0][2]].data();
+            irr[1][0] = oss0[0].irreps[ids[1][0]].data();
+            irr[1][1] = oss0[1].irreps[ids[1][1]].data();
+            irr[1][2] = oss0[2].irreps[ids[1][2]].data();
+            for (int j = 0; j < 3; ++j) irr[2][j] = oss2[j].irreps[ids[2][j]].data();
+            const int* adj[3] = {irr[0][0], irr[0][1], irr[0][2]};
+            a.pref = prefactor_adjoint(adj, il ? '3' : '1', in.sym) *
+                     prefactor_pUpdate(irr, il, in.sym);
+            aides.push_back(a);
+        }
+        for (int k = 0; k < po.nrops; ++k) {
+            if (po.hss_of_ops[k] != newhss) continue;
+            PUpdateGroup g;
+            g.op = k;
+            g.block = usb2;
+            g.dst = nr.op_offset[k] + nr.beginblock[k][usb2];
+            g.n = nr.block_size(k, usb2);
+            g.task_begin = (int)P.tasks.size();
+            if (g.n > 0) {
+                for (const Aide& a : aides) {
+                    if (po.block_size(k, a.osb2) == 0) continue;
+                    if ((int64_t)a.N0 * a.N1 != g.n ||
+                        (int64_t)a.M0 * a.M1 != po.block_size(k, a.osb2))
+                        throw std::runtime_error("pUpdate: block size mismatch");
+                    PUpdateTask t;
+                    t.dst = g.dst;
+                    t.src = po.op_offset[k] + po.beginblock[k][a.osb2];
+                    t.th = a.th;
+                    t.t = a.t;
+                    t.M0 = a.M0;
+                    t.M1 = a.M1;
+                    t.N0 = a.N0;
+                    t.N1 = a.N1;
+                    t.pref = a.pref;
+                    P.tasks.push_back(t);
+                }
+            }
+            g.task_end = (int)P.tasks.size();
+            if (g.task_end > g.task_begin) P.groups.push_back(g);
+        }
+    }
+    return P;
+}
+
+}  // namespace t3

@@ -52,6 +52,36 @@ struct AppendPlan {
 
 AppendPlan plan_append(const AppendInput& in);
 
+// Everything update_rOperators_physical reads.
+struct PUpdateInput {
+    SymInfo sym;
+    int is_left = 0;
+    Symsecs site_ss[3], iss, mpo;
+    const SiteTensor* T = nullptr;  // one-site tensor (qnumbers over site_ss)
+    int nrhss = 0;
+    std::vector<int> begin_blocks_of_hss;
+    std::vector<QN> qnumbers;
+    std::vector<std::vector<int64_t>> hss_bb;
+    const ROperators* pops = nullptr;  // physical operators on the internal bond
+};
+// One contribution O'[dst] += pref * Th^(T) * O[src] * T^(T) of one operator.
+struct PUpdateTask {
+    int64_t dst, src, th, t;  // offsets: new arena, old arena, tensor, tensor
+    int M0, M1, N0, N1;
+    double pref;
+};
+struct PUpdateGroup {
+    int task_begin, task_end;
+    int op, block;
+    int64_t dst, n;
+};
+struct PUpdatePlan {
+    ROperators nr;
+    std::vector<PUpdateGroup> groups;
+    std::vector<PUpdateTask> tasks;
+};
+PUpdatePlan plan_pupdate(const PUpdateInput& in);
+
 // Drops the blocks flagged empty: returns the compacted per-operator block
 // offsets and, for every kept block, (old offset, new offset, size).
 struct Compaction {

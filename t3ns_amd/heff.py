@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import FP32, FP64, AppendDesc, HeffDesc, RopsDesc, SiteDesc, check, lib
+from ._lib import FP32, FP64, AppendDesc, HeffDesc, PUpdateDesc, RopsDesc, SiteDesc, check, lib
 
 
 class Context:
@@ -109,6 +109,15 @@ class Step:
               "t3ns_step_append_desc")
         return d, o
 
+    def pupdate_problem(self, leg, seed=0, maxdim=8):
+        """(update descriptor, physical operators) behind physical leg `leg`:
+        the leg's bond truncated to a seeded subset of its internal sectors
+        (dims up to `maxdim`) and a seeded one-site tensor on its site."""
+        d, o = PUpdateDesc(), RopsDesc()
+        check(lib().t3ns_step_pupdate_desc(self._h, int(leg), int(seed), int(maxdim), C.byref(d),
+                                           C.byref(o)), "t3ns_step_pupdate_desc")
+        return d, o
+
     def close(self):
         if self._h:
             lib().t3ns_step_destroy(self._h)
@@ -155,6 +164,12 @@ class ROps:
             out.append((bb, tel))
         return out
 
+    def work(self):
+        """(flops, bytes, kernel ms) of the device execution that made these."""
+        f, b, ms = C.c_double(), C.c_double(), C.c_double()
+        check(lib().t3ns_rops_work(self._h, C.byref(f), C.byref(b), C.byref(ms)), "t3ns_rops_work")
+        return f.value, b.value, ms.value
+
     def close(self):
         if self._h:
             lib().t3ns_rops_destroy(self._h)
@@ -171,6 +186,17 @@ def append_phys(desc, old):
     """rOperators_append_phys (src/rOperators_pUpdate.c:665-673) on the device."""
     h = C.c_void_p()
     check(lib().t3ns_append_phys(C.byref(desc), C.byref(old), C.byref(h)), "t3ns_append_phys")
+    return ROps(h)
+
+
+def update_physical(desc, pops):
+    """update_rOperators_physical (src/rOperators_pUpdate.c:364-387) on the
+    device; `pops` is a RopsDesc or an ROps."""
+    if isinstance(pops, ROps):
+        pops = pops.desc
+    h = C.c_void_p()
+    check(lib().t3ns_update_physical(C.byref(desc), C.byref(pops), C.byref(h)),
+          "t3ns_update_physical")
     return ROps(h)
 
 
