@@ -257,6 +257,38 @@ int t3ns_update_physical(const t3ns_pupdate_desc* desc, const t3ns_rops_desc* po
  * src/optimize_network.c:255-279). */
 int t3ns_rops_work(const t3ns_rops* r, double* flops, double* bytes, double* kernel_ms);
 
+/* ------------------------------------------------------------------------
+ * makesiteTensor (src/siteTensor_init.c:785-800 with init_makeinfo 609-649,
+ * init_contractinfo 651-760, contractsiteTensors 768-783): the multisite
+ * tensor's elements from two to four one-site tensors, contracted over the
+ * internal bonds (the bonds the sites share, recognised by their ids in
+ * `new_ss`).  Multisite blocks with a missing or empty one-site block stay
+ * zero.  The one-site case of the reference is a pointer copy, not a
+ * computation, and is not offered here.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    double flops;      /* algorithmic flops of the device kernels */
+    double bytes;      /* inputs read once + output written once */
+    double kernel_ms;  /* elapsed time of the device kernels */
+} t3ns_work;
+
+typedef struct {
+    int nrsites;                         /* 2..4 */
+    int innersite;                       /* the branching site; the second of two physical sites */
+    t3ns_symsecs_desc new_ss[4][3];      /* multisite sectors (internal bonds at unit dims) */
+    t3ns_symsecs_desc old_ss[4][3];      /* sectors of the one-site tensors */
+    int nrblocks;                        /* multisite blocks */
+    const int64_t* qnumbers;             /* [nrblocks * nrsites] over new_ss */
+    const int64_t* beginblock;           /* [nrblocks + 1] */
+    int site_nrblocks[4];
+    const int64_t* site_qnumbers[4];     /* [site_nrblocks[i]] over old_ss[i], sorted */
+    const int64_t* site_beginblock[4];   /* [site_nrblocks[i] + 1] */
+    const double* site_tel[4];
+} t3ns_makesite_desc;
+
+/* tel: [beginblock[nrblocks]] doubles, overwritten.  work may be NULL. */
+int t3ns_make_site_tensor(const t3ns_makesite_desc* desc, double* tel, t3ns_work* work);
+
 /* Leg dimensions of every block of the multisite tensor in operator-leg
  * order (dims[3*b + leg]; leg 2 is 1 for two-site DMRG objects), as
  * fill_indexes derives them (src/Heff.c:695-739).  Returns the block count

@@ -68,6 +68,12 @@ int t3ns_step_append_desc(t3ns_step* s, int leg, uint64_t seed, t3ns_append_desc
  * until the next call for the same leg with another seed or maxdim. */
 int t3ns_step_pupdate_desc(t3ns_step* s, int leg, uint64_t seed, int maxdim, t3ns_pupdate_desc* d,
                            t3ns_rops_desc* pops);
+/* A makesiteTensor problem for a step: seeded N(0,1) one-site tensors of
+ * the step's sites over the sectors their bonds had before the multisite
+ * structure replaced the internal ones, and the step's multisite
+ * structure.  The arrays live in the step until the next call with another
+ * seed. */
+int t3ns_step_makesite_desc(t3ns_step* s, uint64_t seed, t3ns_makesite_desc* d);
 int t3ns_step_destroy(t3ns_step* s);
 int64_t t3ns_step_size(const t3ns_step* s);
 /* Fills the Heff boundary descriptors; pointers stay valid while s lives. */

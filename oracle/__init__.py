@@ -23,7 +23,7 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIBPATH):
             raise RuntimeError("oracle not built: run oracle/build.py")
-        from t3ns_amd._lib import AppendDesc, HeffDesc, PUpdateDesc, RopsDesc, SiteDesc
+        from t3ns_amd._lib import AppendDesc, HeffDesc, MakesiteDesc, PUpdateDesc, RopsDesc, SiteDesc
         L = C.CDLL(LIBPATH)
         P = C.c_void_p
         dp = C.POINTER(C.c_double)
@@ -55,6 +55,7 @@ def lib():
             "oracle_append_phys": (C.c_int64, [C.POINTER(AppendDesc), C.POINTER(RopsDesc), i64, dp]),
             "oracle_update_physical": (C.c_int64, [C.POINTER(PUpdateDesc), C.POINTER(RopsDesc),
                                                    i64, dp]),
+            "oracle_make_site_tensor": (C.c_int, [C.POINTER(MakesiteDesc), dp]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -181,3 +182,13 @@ def update_physical(desc, pops):
         pb += nb + 1
         pt += int(b[-1])
     return out
+
+
+def make_site_tensor(desc):
+    """Oracle makesiteTensor: the multisite elements."""
+    n = int(desc.beginblock[desc.nrblocks]) if desc.nrblocks else 0
+    out = np.zeros(max(n, 1))
+    rc = lib().oracle_make_site_tensor(C.byref(desc), out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != 0:
+        raise RuntimeError("oracle_make_site_tensor: sites not bound to the inner site")
+    return out[:n]

@@ -87,6 +87,12 @@ int64_t oracle_append_phys(const t3ns_append_desc* d, const t3ns_rops_desc* old,
 int64_t oracle_update_physical(const t3ns_pupdate_desc* d, const t3ns_rops_desc* pops,
                                int64_t* beginblock, double* tel);
 
+/* makesiteTensor's contraction (src/siteTensor_init.c:609-800) on the
+ * descriptor of include/t3ns_amd.h; tel: the multisite elements (zeroed
+ * here first).  Returns 0, or -1 when an outer site is not bound to the
+ * inner site. */
+int oracle_make_site_tensor(const t3ns_makesite_desc* d, double* tel);
+
 #ifdef __cplusplus
 }
 #endif

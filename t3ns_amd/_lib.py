@@ -66,6 +66,18 @@ class PUpdateDesc(C.Structure):
                 ("qnumbers", c_i64_p), ("hss_beginblock", C.POINTER(c_i64_p))]
 
 
+class Work(C.Structure):
+    _fields_ = [("flops", C.c_double), ("bytes", C.c_double), ("kernel_ms", C.c_double)]
+
+
+class MakesiteDesc(C.Structure):
+    _fields_ = [("nrsites", C.c_int), ("innersite", C.c_int),
+                ("new_ss", (SymsecsDesc * 3) * 4), ("old_ss", (SymsecsDesc * 3) * 4),
+                ("nrblocks", C.c_int), ("qnumbers", c_i64_p), ("beginblock", c_i64_p),
+                ("site_nrblocks", C.c_int * 4), ("site_qnumbers", c_i64_p * 4),
+                ("site_beginblock", c_i64_p * 4), ("site_tel", c_dbl_p * 4)]
+
+
 # void (*)(const double* x, double* y, void* stream, void* vdata)
 DEV_MATVEC = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)
 
@@ -94,6 +106,8 @@ def lib():
         "t3ns_update_physical": (C.c_int, [C.POINTER(PUpdateDesc), C.POINTER(RopsDesc),
                                            C.POINTER(P)]),
         "t3ns_rops_work": (C.c_int, [P, c_dbl_p, c_dbl_p, c_dbl_p]),
+        "t3ns_make_site_tensor": (C.c_int, [C.POINTER(MakesiteDesc), c_dbl_p, C.POINTER(Work)]),
+        "t3ns_step_makesite_desc": (C.c_int, [P, C.c_uint64, C.POINTER(MakesiteDesc)]),
         "t3ns_step_pupdate_desc": (C.c_int, [P, C.c_int, C.c_uint64, C.c_int,
                                              C.POINTER(PUpdateDesc), C.POINTER(RopsDesc)]),
         "t3ns_rops_view": (C.c_int, [P, C.POINTER(RopsDesc)]),

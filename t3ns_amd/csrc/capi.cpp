@@ -11,6 +11,7 @@
 #include "../../include/t3ns_host.h"
 #include "dev/davidson.hpp"
 #include "dev/heff_dev.hpp"
+#include "dev/sitetensor_dev.hpp"
 #include "dev/renorm_dev.hpp"
 #include "host/context.hpp"
 #include "host/heff_plan.hpp"
@@ -460,6 +461,46 @@ int t3ns_update_physical(const t3ns_pupdate_desc* d, const t3ns_rops_desc* pops,
         res->r = run_pupdate(plan, po, T, s, &res->work);
         res->views();
         *out = res.release();
+    });
+}
+
+int t3ns_make_site_tensor(const t3ns_makesite_desc* d, double* tel, t3ns_work* work) {
+    if (!d || !tel) return fail(T3NS_ERR_INVALID, "null argument");
+    return guarded([&] {
+        MakesiteInput in;
+        in.nrsites = d->nrsites;
+        in.innersite = d->innersite;
+        if (d->nrsites < 2 || d->nrsites > 4 || d->nrblocks < 0)
+            throw std::invalid_argument("makesite: needs 2-4 sites");
+        SiteTensor sites[4];
+        for (int i = 0; i < d->nrsites; ++i) {
+            for (int k = 0; k < 3; ++k) {
+                in.new_ss[i][k] = symsecs_from(d->new_ss[i][k]);
+                in.old_ss[i][k] = symsecs_from(d->old_ss[i][k]);
+            }
+            SiteTensor& t = sites[i];
+            t.nrsites = 1;
+            t.nrblocks = d->site_nrblocks[i];
+            if (t.nrblocks < 0 || (t.nrblocks && (!d->site_qnumbers[i] || !d->site_beginblock[i] ||
+                                                  !d->site_tel[i])))
+                throw std::invalid_argument("makesite: bad one-site tensor");
+            t.qnumbers.assign(d->site_qnumbers[i], d->site_qnumbers[i] + t.nrblocks);
+            t.beginblock.assign(1, 0);
+            if (t.nrblocks) t.beginblock.assign(d->site_beginblock[i], d->site_beginblock[i] + t.nrblocks + 1);
+            t.tel.assign(d->site_tel[i], d->site_tel[i] + t.size());
+            in.sites[i] = &sites[i];
+        }
+        in.qnumbers.assign(d->qnumbers, d->qnumbers + (size_t)d->nrblocks * d->nrsites);
+        in.beginblock.assign(d->beginblock, d->beginblock + d->nrblocks + 1);
+        const MakesitePlan plan = plan_makesite(in);
+        RenormWork w;
+        const std::vector<double> out = run_makesite(plan, in.sites, nullptr, &w);
+        std::copy(out.begin(), out.end(), tel);
+        if (work) {
+            work->flops = w.flops;
+            work->bytes = w.bytes;
+            work->kernel_ms = w.kernel_ms;
+        }
     });
 }
 

@@ -123,10 +123,19 @@ struct PUpdateStore {
     bool built = false;
 };
 
+struct MakesiteStore {
+    SymsecsFlat new_ss[4][3], old_ss[4][3];
+    SiteTensor sites[4];
+    uint64_t seed = 0;
+    bool built = false;
+};
+
 struct t3ns_step {
     t3ns_ctx* ctx = nullptr;
     AppendStore app[3];
     PUpdateStore pup[3];
+    MakesiteStore mks;
+    MultisiteInfo md;
     SiteTensor site;
     ROperators ops[3];
     InstructionSet merge_raw;
@@ -240,7 +249,7 @@ int t3ns_step_random(t3ns_step** out, t3ns_ctx* cx, const int* sites, int nsites
         Context& c = *cx->c;
         auto st = std::make_unique<t3ns_step>();
         st->ctx = cx;
-        MultisiteInfo md;
+        MultisiteInfo& md = st->md;
         st->site = make_multisite_structure(c, sites, nsites, &md);
         SiteTensor& T = st->site;
         Rng rng(seed);
@@ -580,6 +589,48 @@ int t3ns_step_pupdate_desc(t3ns_step* st, int leg, uint64_t seed, int maxdim,
         pops->hss_of_ops = P.hss_of_ops.data();
         pops->beginblock = u.bb_ptr.data();
         pops->tel = u.tel_ptr.data();
+    });
+}
+
+int t3ns_step_makesite_desc(t3ns_step* st, uint64_t seed, t3ns_makesite_desc* d) {
+    if (!st || !d) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        Context& c = *st->ctx->c;
+        const SiteTensor& M = st->site;
+        if (M.nrsites < 2) throw std::invalid_argument("step has a single site");
+        MakesiteStore& m = st->mks;
+        if (m.built && m.seed != seed) m = MakesiteStore{};
+        if (!m.built) {
+            m.seed = seed;
+            Rng rng(seed);
+            for (int i = 0; i < M.nrsites; ++i) {
+                const Symsecs* ss[3] = {&st->md.ssarr_old[i][0], &st->md.ssarr_old[i][1],
+                                        &st->md.ssarr_old[i][2]};
+                m.sites[i] = init_1siteTensor(c, M.sites[i], ss);
+                for (auto& v : m.sites[i].tel) v = rng.normal();
+                for (int k = 0; k < 3; ++k) {
+                    m.new_ss[i][k].set(st->md.ssarr[i][k]);
+                    m.old_ss[i][k].set(st->md.ssarr_old[i][k]);
+                }
+            }
+            m.built = true;
+        }
+        std::memset(d, 0, sizeof(*d));
+        d->nrsites = M.nrsites;
+        d->innersite = st->md.innersite;
+        for (int i = 0; i < M.nrsites; ++i) {
+            for (int k = 0; k < 3; ++k) {
+                d->new_ss[i][k] = m.new_ss[i][k].desc;
+                d->old_ss[i][k] = m.old_ss[i][k].desc;
+            }
+            d->site_nrblocks[i] = m.sites[i].nrblocks;
+            d->site_qnumbers[i] = m.sites[i].qnumbers.data();
+            d->site_beginblock[i] = m.sites[i].beginblock.data();
+            d->site_tel[i] = m.sites[i].tel.data();
+        }
+        d->nrblocks = M.nrblocks;
+        d->qnumbers = M.qnumbers.data();
+        d->beginblock = M.beginblock.data();
     });
 }
 

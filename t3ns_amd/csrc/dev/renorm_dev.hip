@@ -3,6 +3,7 @@
 // bandwidth-bound (each element is read once per contributing term and
 // written once).
 #include "renorm_dev.hpp"
+#include "tile_gemm.hpp"
 
 #include <algorithm>
 
@@ -128,23 +129,12 @@ namespace t3 {
 
 namespace {
 
-constexpr int kTile = 64;   // output tile edge
-constexpr int kKStep = 16;  // reduction slab
-
-// Strided operand: element (i, k) at base[off + i * si + k * sk].
-struct Strided {
-    int64_t off;
-    int si, sk;
-};
-
-// Phase 1: X = P Q (M x N, column-major, leading dimension M) per tile.
-struct GemmTile {
-    Strided p, q;  // q as (k, j): off + k * si + j * sk
-    int64_t x_off;
-    int M, N, K;
-    int ti, tj;
-    int pad;
-};
+using tile::GemmTile;
+using tile::kKStep;
+using tile::kTile;
+using tile::Strided;
+using tile::tile_slab;
+using tile::tiles_of;
 
 // Phase 2: U tile = sum_t pref_t * R_t S_t.
 struct SumTile {
@@ -161,80 +151,6 @@ struct SumTask {
     int pad;
     double pref;
 };
-
-// One K-slab of a 64 x 64 tile product: loads P(64 x 16) and Q(16 x 64)
-// into LDS (zero padded) and accumulates acc[a][b] for rows tx + 16a,
-// columns ty + 16b.  The load pattern follows the unit-stride index so that
-// neighbouring lanes read neighbouring addresses.
-__device__ __forceinline__ void tile_slab(const double* __restrict__ base, const Strided& P,
-                                          const Strided& Q, int M, int N, int K, int i0, int j0,
-                                          int k0, double (*Ps)[kTile + 1],
-                                          double (*Qs)[kTile + 1], double acc[4][4]) {
-    const int t = threadIdx.x;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        int i, k;
-        if (P.si == 1) {
-            i = t % kTile;
-            k = t / kTile + 4 * r;
-        } else {
-            k = t % kKStep;
-            i = t / kKStep + 16 * r;
-        }
-        const int gi = i0 + i, gk = k0 + k;
-        Ps[k][i] = (gi < M && gk < K) ? base[P.off + (int64_t)gi * P.si + (int64_t)gk * P.sk] : 0.0;
-        int j, kk;
-        if (Q.si == 1) {
-            kk = t % kKStep;
-            j = t / kKStep + 16 * r;
-        } else {
-            j = t % kTile;
-            kk = t / kTile + 4 * r;
-        }
-        const int gj = j0 + j, gkk = k0 + kk;
-        Qs[kk][j] =
-            (gj < N && gkk < K) ? base[Q.off + (int64_t)gkk * Q.si + (int64_t)gj * Q.sk] : 0.0;
-    }
-    __syncthreads();
-    const int tx = t % 16, ty = t / 16;
-#pragma unroll 4
-    for (int k = 0; k < kKStep; ++k) {
-        double pa[4], qb[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) pa[a] = Ps[k][tx + 16 * a];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) qb[b] = Qs[k][ty + 16 * b];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = fma(pa[a], qb[b], acc[a][b]);
-    }
-    __syncthreads();
-}
-
-__global__ void __launch_bounds__(256)
-    pupdate_first_kernel(const GemmTile* __restrict__ tiles, int ntiles, double* base) {
-    __shared__ double Ps[kKStep][kTile + 1];
-    __shared__ double Qs[kKStep][kTile + 1];
-    for (int it = blockIdx.x; it < ntiles; it += gridDim.x) {
-        const GemmTile g = tiles[it];
-        const int i0 = g.ti * kTile, j0 = g.tj * kTile;
-        double acc[4][4] = {};
-        for (int k0 = 0; k0 < g.K; k0 += kKStep)
-            tile_slab(base, g.p, g.q, g.M, g.N, g.K, i0, j0, k0, Ps, Qs, acc);
-        const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int j = j0 + ty + 16 * b;
-            if (j >= g.N) continue;
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const int i = i0 + tx + 16 * a;
-                if (i < g.M) base[g.x_off + i + (int64_t)j * g.M] = acc[a][b];
-            }
-        }
-    }
-}
 
 __global__ void __launch_bounds__(256)
     pupdate_sum_kernel(const SumTile* __restrict__ tiles, int ntiles,
@@ -275,8 +191,6 @@ __global__ void __launch_bounds__(256)
         if (threadIdx.x == 0 && any) atomicOr(&keep[g.flat], 1u);
     }
 }
-
-int tiles_of(int n) { return (n + kTile - 1) / kTile; }
 
 }  // namespace
 
@@ -319,11 +233,11 @@ ROperators run_pupdate(const PUpdatePlan& plan, const ROperators& pops, const Si
             sk.pad = 0;
             GemmTile ft;
             ft.x_off = workoff + wsize;
-            ft.pad = 0;
             if (costA <= costB) {
                 // W = A O (N0 x M1), then W B
                 ft.p = A;
                 ft.q = O;
+                ft.ldx = N0;
                 ft.M = N0;
                 ft.N = (int)M1;
                 ft.K = (int)M0;
@@ -336,6 +250,7 @@ ROperators run_pupdate(const PUpdatePlan& plan, const ROperators& pops, const Si
                 // V = O B (M0 x N1), then A V
                 ft.p = O;
                 ft.q = B;
+                ft.ldx = (int)M0;
                 ft.M = (int)M0;
                 ft.N = N1;
                 ft.K = (int)M1;
@@ -387,7 +302,7 @@ ROperators run_pupdate(const PUpdatePlan& plan, const ROperators& pops, const Si
     T3_HIP_CHECK(hipEventCreate(&e1));
     T3_HIP_CHECK(hipEventRecord(e0, s));
     if (!first.empty())
-        hipLaunchKernelGGL(pupdate_first_kernel, dim3(grid((int64_t)first.size())), dim3(256), 0,
+        hipLaunchKernelGGL(tile::gemm_tiles_kernel, dim3(grid((int64_t)first.size())), dim3(256), 0,
                            s, dfirst.get(), (int)first.size(), arena.get());
     if (!sums.empty())
         hipLaunchKernelGGL(pupdate_sum_kernel, dim3(grid((int64_t)sums.size())), dim3(256), 0, s,

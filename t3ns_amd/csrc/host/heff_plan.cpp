@@ -8,6 +8,9 @@
 namespace t3 {
 
 namespace {
+
+// Prefactors below this are exact zeros computed with rounding.
+constexpr double kZeroCoupling = 1e-14;
 const int kOrders[6][3] = {{0, 1, 2}, {1, 0, 2}, {0, 2, 1}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 
 struct BlockIdx {
@@ -306,7 +309,12 @@ HeffPlan build_heff_plan(const HeffInput& in) {
                         int idMPO[3];
                         mpo_ids_of_group(in, isdmrg, g, idMPO);
                         const double pref = calc_prefactor(in, isdmrg, posB, nb, ob, idMPO);
-                        if (std::fabs(pref) < 1e-20) continue;
+                        // The reference drops |pref| < 1e-20 (src/Heff.c:827), which keeps
+                        // or drops a coupling that is exactly zero depending on the rounding
+                        // of its 9j evaluation (values ~1e-18).  Nonzero couplings at the
+                        // spins that occur are many orders larger, so everything below
+                        // kZeroCoupling is an exact zero and its GEMMs are skipped.
+                        if (std::fabs(pref) < kZeroCoupling) continue;
                         PlanEntry e;
                         e.mpo_group = g;
                         e.prefactor = pref;

@@ -32,12 +32,16 @@ std::vector<int> sort_qn_tuples(const std::vector<QN>& qn, int w) {
 }  // namespace
 
 SiteTensor init_1siteTensor(const Context& c, int site) {
-    SiteTensor t;
-    t.nrsites = 1;
-    t.sites[0] = site;
     int b[3];
     c.netw.bonds_of_site(site, b);
     const Symsecs* ss[3] = {&c.symsecs(b[0]), &c.symsecs(b[1]), &c.symsecs(b[2])};
+    return init_1siteTensor(c, site, ss);
+}
+
+SiteTensor init_1siteTensor(const Context& c, int site, const Symsecs* const ss[3]) {
+    SiteTensor t;
+    t.nrsites = 1;
+    t.sites[0] = site;
     GoodSectors gs = c.find_good_sectors(ss, 1);
     std::vector<QN> qn;
     std::vector<int64_t> dims;

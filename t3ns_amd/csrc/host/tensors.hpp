@@ -52,6 +52,8 @@ struct ROperators {
 // Block structure of a one-site tensor from the valid sector triples
 // (src/siteTensor_init.c:64-108).  Data is zero-initialised.
 SiteTensor init_1siteTensor(const Context& c, int site);
+// The same over given sectors of the site's three bonds.
+SiteTensor init_1siteTensor(const Context& c, int site, const Symsecs* const ss[3]);
 
 // Structure of renormalised operators at a bond (src/rOperators_init.c:281-507).
 // Returns per-hss block offset arrays in `tmpbb`.

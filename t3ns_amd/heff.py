@@ -11,7 +11,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import FP32, FP64, AppendDesc, HeffDesc, PUpdateDesc, RopsDesc, SiteDesc, check, lib
+from ._lib import (FP32, FP64, AppendDesc, HeffDesc, MakesiteDesc, PUpdateDesc, RopsDesc, SiteDesc,
+                   Work, check, lib)
 
 
 class Context:
@@ -118,6 +119,13 @@ class Step:
                                            C.byref(o)), "t3ns_step_pupdate_desc")
         return d, o
 
+    def makesite_problem(self, seed=0):
+        """makesiteTensor descriptor: seeded one-site tensors of the step's
+        sites and the step's multisite structure."""
+        d = MakesiteDesc()
+        check(lib().t3ns_step_makesite_desc(self._h, int(seed), C.byref(d)), "t3ns_step_makesite_desc")
+        return d
+
     def close(self):
         if self._h:
             lib().t3ns_step_destroy(self._h)
@@ -198,6 +206,17 @@ def update_physical(desc, pops):
     check(lib().t3ns_update_physical(C.byref(desc), C.byref(pops), C.byref(h)),
           "t3ns_update_physical")
     return ROps(h)
+
+
+def make_site_tensor(desc):
+    """makesiteTensor (src/siteTensor_init.c:785-800) on the device:
+    (multisite elements, (flops, bytes, kernel ms))."""
+    n = int(desc.beginblock[desc.nrblocks]) if desc.nrblocks else 0
+    out = np.zeros(max(n, 1))
+    w = Work()
+    check(lib().t3ns_make_site_tensor(C.byref(desc), out.ctypes.data_as(_lib.c_dbl_p), C.byref(w)),
+          "t3ns_make_site_tensor")
+    return out[:n], (w.flops, w.bytes, w.kernel_ms)
 
 
 def plan_block_dims(desc, ops, site):

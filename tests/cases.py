@@ -28,4 +28,8 @@ QC_CASES = [
     ("ref_networks/10_T3NS", ["U1", "SU2", "D2h"], [14, 0, 0], 64, [5, 6, 7]),
     ("ref_networks/10_DMRG", ["U1", "SU2", "D2h"], [14, 0, 0], 64, [4, 5]),
     ("ref_networks/10_DMRG", ["U1", "U1"], [7, 7], 48, [2, 3]),
+    # physical site on the branching site's middle leg with its first leg
+    # external (the transposed/batched makesiteTensor products)
+    ("ref_networks/10_T3NS", ["U1", "U1"], [7, 7], 64, [7, 8]),
+    ("ref_networks/10_T3NS", ["U1", "SU2"], [14, 0], 64, [7, 8, 9]),
 ]

@@ -141,3 +141,40 @@ def orthogonal_site_tensor(pd, rng, keep):
             tel[bb[b]:bb[b + 1]] = part.T.flatten(order="F") if il else part.flatten(order="F")
     keep.append(tel)
     pd.T_tel = as_dbl(tel)
+
+
+def symsecs_dims(ss):
+    return [ss.dims[i] for i in range(ss.nrSecs)]
+
+
+def isometric_blocks(qn, bb, ss, leg, rng):
+    """Elements of a one-site tensor (blocks `qn`, offsets `bb`, sectors
+    `ss` of its three bonds) whose rows over leg `leg` (0 or 2) are
+    orthonormal within every sector of that leg, over all blocks of the
+    sector (columns: the other two legs).  Returns (elements, sectors where
+    that was possible: column count >= sector dim)."""
+    n0, n1 = ss[0].nrSecs, ss[1].nrSecs
+    qn = np.asarray(qn, dtype=np.int64)
+    bb = np.asarray(bb, dtype=np.int64)
+    sec = qn % n0 if leg == 0 else qn // (n0 * n1)
+    tel = rng.standard_normal(max(int(bb[-1]), 1))
+    good = set()
+    for s in np.unique(sec):
+        blks = np.nonzero(sec == s)[0]
+        d = ss[leg].dims[int(s)]
+        if d == 0:
+            continue
+        sizes = [int(bb[b + 1] - bb[b]) // d for b in blks]
+        R = sum(sizes)
+        if R < d:
+            continue
+        good.add(int(s))
+        Q, _ = np.linalg.qr(rng.standard_normal((R, d)))
+        X = Q.T  # d x R, orthonormal rows
+        c = 0
+        for b, m in zip(blks, sizes):
+            part = X[:, c:c + m]
+            c += m
+            # leg 0: block is (d x m) column-major; leg 2: (m x d)
+            tel[bb[b]:bb[b + 1]] = part.flatten(order="F") if leg == 0 else part.T.flatten(order="F")
+    return tel, good

@@ -9,6 +9,7 @@ import oracle as O
 import t3ns_amd as T
 from cases import HEFF_CASES, case_id
 from conftest import netw_text
+from plan_util import without_zero_couplings
 
 
 @pytest.mark.parametrize("case", HEFF_CASES, ids=case_id)
@@ -19,8 +20,11 @@ def test_plan_table_matches_oracle(case):
     d, o, s = st.descriptors()
     ints, prefs, flops = T.plan_table(d, o, s)
     oh = O.OracleHeff(d, o, s)
-    oi, op = oh.plan_table()
+    oi, op, ndropped = without_zero_couplings(*oh.plan_table())
     assert ints.shape[0] > 0
     assert np.array_equal(ints, oi)
     np.testing.assert_allclose(prefs, op, rtol=0, atol=1e-15)
-    assert flops == oh.plan_stats()["flops"]
+    if ndropped == 0:
+        assert flops == oh.plan_stats()["flops"]
+    else:
+        assert flops < oh.plan_stats()["flops"]
