@@ -258,6 +258,48 @@ int t3ns_update_physical(const t3ns_pupdate_desc* desc, const t3ns_rops_desc* po
 int t3ns_rops_work(const t3ns_rops* r, double* flops, double* bytes, double* kernel_ms);
 
 /* ------------------------------------------------------------------------
+ * Renormalisation: update_rOperators_branching (src/rOperators_bUpdate.c:
+ * 229-248 with update_unique_ops_T3NS 637-677, update_newblock_w_MPO_set
+ * 746-785, update_selected_blocks 1043-1072, prefactor_bUpdate
+ * src/symmetries.c:253-278, then sum_unique_rOperators
+ * src/rOperators_init.c:228-265): operators on one bond of a branching site
+ * from the operators on its two other bonds and the site tensor T,
+ *   new = sum pref * T_adj . (O0 x O1) . T   per bUpdate instruction.
+ * ops[0] sits on bond (update_case == 0 ? 1 : 0) and is a left operator;
+ * ops[1] on bond (update_case == 2 ? 1 : 2), left only for update_case 2.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int nrSyms;
+    int sgs[T3NS_MAX_SYMMETRIES];
+    int update_case;                 /* bond of the site carrying the new operators */
+    t3ns_symsecs_desc site_ss[3];    /* the site's bonds (bra and ket sectors alike) */
+    t3ns_symsecs_desc mpo;
+    int T_nrblocks;
+    const int64_t* T_qnumbers;       /* over site_ss, sorted */
+    const int64_t* T_beginblock;
+    const double* T_tel;
+    /* bUpdate instructions, sorted as the reference sorts them
+     * (src/instructions.c:152-186) */
+    int nr_instr;
+    const int* instr;                /* [nr_instr * 3]: op of ops[0], op of ops[1], new op */
+    const double* pref;
+    const int* hss_of_new;
+    /* per MPO sector h of a new operator, the pairs (sector of ops[0],
+     * sector of ops[1]) tprod[tprod_begin[h] .. tprod_begin[h+1]) that make
+     * it (tprods_ham, src/hamiltonian.c:227-251) */
+    const int* tprod_begin;          /* [nrhss + 1] */
+    const int* tprods;               /* [2 * tprod_begin[nrhss]] */
+    /* structure of the new operators (all blocks present) */
+    int nrhss;
+    const int* begin_blocks_of_hss;
+    const int64_t* qnumbers;
+    const int64_t* const* hss_beginblock;
+} t3ns_bupdate_desc;
+
+int t3ns_update_branching(const t3ns_bupdate_desc* desc, const t3ns_rops_desc* ops /* [2] */,
+                          t3ns_rops** out);
+
+/* ------------------------------------------------------------------------
  * makesiteTensor (src/siteTensor_init.c:785-800 with init_makeinfo 609-649,
  * init_contractinfo 651-760, contractsiteTensors 768-783): the multisite
  * tensor's elements from two to four one-site tensors, contracted over the

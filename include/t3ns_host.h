@@ -74,6 +74,13 @@ int t3ns_step_pupdate_desc(t3ns_step* s, int leg, uint64_t seed, int maxdim, t3n
  * structure.  The arrays live in the step until the next call with another
  * seed. */
 int t3ns_step_makesite_desc(t3ns_step* s, uint64_t seed, t3ns_makesite_desc* d);
+/* An update_rOperators_branching problem at branching site `site` for
+ * new operators on its bond `update_case`: seeded N(0,1) old operators on
+ * the two other bonds (vacuum operators at a border), a seeded site tensor,
+ * the bUpdate instructions and the new operators' structure.  The arrays
+ * live as long as the context. */
+int t3ns_ctx_bupdate_desc(t3ns_ctx* c, int site, int update_case, uint64_t seed,
+                          t3ns_bupdate_desc* d, t3ns_rops_desc* ops /* [2] */);
 int t3ns_step_destroy(t3ns_step* s);
 int64_t t3ns_step_size(const t3ns_step* s);
 /* Fills the Heff boundary descriptors; pointers stay valid while s lives. */

@@ -23,7 +23,8 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIBPATH):
             raise RuntimeError("oracle not built: run oracle/build.py")
-        from t3ns_amd._lib import AppendDesc, HeffDesc, MakesiteDesc, PUpdateDesc, RopsDesc, SiteDesc
+        from t3ns_amd._lib import (AppendDesc, BUpdateDesc, HeffDesc, MakesiteDesc, PUpdateDesc,
+                                   RopsDesc, SiteDesc)
         L = C.CDLL(LIBPATH)
         P = C.c_void_p
         dp = C.POINTER(C.c_double)
@@ -56,6 +57,9 @@ def lib():
             "oracle_update_physical": (C.c_int64, [C.POINTER(PUpdateDesc), C.POINTER(RopsDesc),
                                                    i64, dp]),
             "oracle_make_site_tensor": (C.c_int, [C.POINTER(MakesiteDesc), dp]),
+            "oracle_update_branching": (C.c_int64, [C.POINTER(BUpdateDesc), C.POINTER(RopsDesc),
+                                                    i64, dp]),
+            "oracle_set_bupdate_order": (None, [C.c_int]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -192,3 +196,32 @@ def make_site_tensor(desc):
     if rc != 0:
         raise RuntimeError("oracle_make_site_tensor: sites not bound to the inner site")
     return out[:n]
+
+
+def update_branching(desc, ops, order=-1):
+    """Oracle update_rOperators_branching: [(beginblock, elements)] per new
+    operator.  order 0..5 forces one contraction order for every block
+    (default: the cheapest, as the reference)."""
+    lib().oracle_set_bupdate_order(int(order))
+    nops = 0
+    for i in range(desc.nr_instr):
+        nops = max(nops, desc.instr[3 * i + 2] + 1)
+    nbb = 0
+    for k in range(nops):
+        h = desc.hss_of_new[k]
+        nbb += desc.begin_blocks_of_hss[h + 1] - desc.begin_blocks_of_hss[h] + 1
+    total = lib().oracle_update_branching(C.byref(desc), ops, None, None)
+    bb = np.zeros(max(nbb, 1), dtype=np.int64)
+    tel = np.zeros(max(total, 1))
+    lib().oracle_update_branching(C.byref(desc), ops, bb.ctypes.data_as(C.POINTER(C.c_int64)),
+                                  tel.ctypes.data_as(C.POINTER(C.c_double)))
+    lib().oracle_set_bupdate_order(-1)
+    out, pb, pt = [], 0, 0
+    for k in range(nops):
+        h = desc.hss_of_new[k]
+        nb = desc.begin_blocks_of_hss[h + 1] - desc.begin_blocks_of_hss[h]
+        b = bb[pb:pb + nb + 1].copy()
+        out.append((b, tel[pt:pt + int(b[-1])].copy()))
+        pb += nb + 1
+        pt += int(b[-1])
+    return out

@@ -87,6 +87,16 @@ int64_t oracle_append_phys(const t3ns_append_desc* d, const t3ns_rops_desc* old,
 int64_t oracle_update_physical(const t3ns_pupdate_desc* d, const t3ns_rops_desc* pops,
                                int64_t* beginblock, double* tel);
 
+/* update_rOperators_branching (src/rOperators_bUpdate.c:229-1101) on the
+ * descriptor of include/t3ns_amd.h with ops[2]; output as
+ * oracle_append_phys. */
+int64_t oracle_update_branching(const t3ns_bupdate_desc* d, const t3ns_rops_desc* ops,
+                                int64_t* beginblock, double* tel);
+/* Forces contraction order `way` (0..5 of ways_to_update,
+ * src/rOperators_bUpdate.c:813-820) for every block; -1 restores the
+ * reference's choice of the cheapest. */
+void oracle_set_bupdate_order(int way);
+
 /* makesiteTensor's contraction (src/siteTensor_init.c:609-800) on the
  * descriptor of include/t3ns_amd.h; tel: the multisite elements (zeroed
  * here first).  Returns 0, or -1 when an outer site is not bound to the

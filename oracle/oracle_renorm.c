@@ -362,3 +362,352 @@ int64_t oracle_update_physical(const t3ns_pupdate_desc* d, const t3ns_rops_desc*
         free(usb_to_osb);
         return total;
 }
+
+/* ------------------------------------------------------------------------
+ * update_rOperators_branching (src/rOperators_bUpdate.c:229-248):
+ * prepare_update_branching (292-313), init_uniqueOperators (255-290),
+ * make_qntens (315-383), init_instrhelper (423-493),
+ * initialize_indexhelper (496-532), update_unique_ops_T3NS (637-677),
+ * update_newblock_w_MPO_set (746-785), next_sb_tens (680-718),
+ * next_sb_sec_op (720-744), find_block_adj (787-806), ways_to_update /
+ * calc_contract / fillin_cinfo / how_to_update (813-1012),
+ * update_selected_blocks (1043-1072), get_tels_operators (1074-1101),
+ * then sum_unique_rOperators (src/rOperators_init.c:228-265).
+ * ---------------------------------------------------------------------- */
+
+enum { B_OPS1, B_OPS2, B_NEWOPS, B_TENS, B_ADJ, B_WORKBRA, B_WORKKET };
+enum { B_BRA, B_KET, B_MPO };
+
+static const int b_ways[6][3][3] = {
+        {{B_OPS1, B_TENS, B_WORKKET}, {B_OPS2, B_WORKKET, B_WORKBRA}, {B_ADJ, B_WORKBRA, B_NEWOPS}},
+        {{B_OPS2, B_TENS, B_WORKKET}, {B_OPS1, B_WORKKET, B_WORKBRA}, {B_ADJ, B_WORKBRA, B_NEWOPS}},
+        {{B_OPS1, B_ADJ, B_WORKBRA}, {B_OPS2, B_WORKBRA, B_WORKKET}, {B_WORKKET, B_TENS, B_NEWOPS}},
+        {{B_OPS2, B_ADJ, B_WORKBRA}, {B_OPS1, B_WORKBRA, B_WORKKET}, {B_WORKKET, B_TENS, B_NEWOPS}},
+        {{B_OPS1, B_TENS, B_WORKKET}, {B_OPS2, B_ADJ, B_WORKBRA}, {B_WORKBRA, B_WORKKET, B_NEWOPS}},
+        {{B_OPS2, B_TENS, B_WORKKET}, {B_OPS1, B_ADJ, B_WORKBRA}, {B_WORKBRA, B_WORKKET, B_NEWOPS}}};
+
+struct b_cinfo {
+        int tens[3];
+        int ta, tb;
+        int M, N, K, L;
+        int lda, ldb, ldc;
+        int64_t stride[3];
+};
+
+static void b_get_dims(int t, int* rd, int (*td)[2], int (*wd)[2], const int* id_ops)
+{
+        if (t < B_TENS) {
+                rd[B_BRA] = td[id_ops[t]][B_BRA];
+                rd[B_KET] = td[id_ops[t]][B_KET];
+        } else if (t < B_WORKBRA) {
+                const int bt = t == B_TENS ? B_KET : B_BRA;
+                for (int i = 0; i < 3; ++i) rd[i] = td[i][bt];
+        } else {
+                const int bt = t == B_WORKKET ? B_KET : B_BRA;
+                for (int i = 0; i < 3; ++i) rd[i] = wd[i][bt];
+        }
+}
+
+static const int b_bondz[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+
+static int b_ops_needed(int t, int bc, int btype, const int* d1, const int* d2, int (*wd)[2])
+{
+        const int* bs = b_bondz[bc];
+        if (t == B_WORKBRA || t == B_WORKKET) {
+                const int bt = t == B_WORKKET ? B_KET : B_BRA;
+                const int r = d1[0] * d1[1] * d2[bs[0]] * d2[bs[1]];
+                wd[bs[0]][bt] = d2[bs[0]];
+                wd[bs[1]][bt] = d2[bs[1]];
+                wd[bc][bt] = d1[!btype];
+                return r;
+        }
+        return d1[bs[0]] * d1[bs[1]] * d1[bc] * d2[bc];
+}
+
+static int b_calc(const int (*tens)[3], int (*td)[2], int (*wd)[2], const int* id_ops)
+{
+        int r = 0;
+        for (int i = 0; i < 3; ++i) {
+                const int bc = i == 2 ? id_ops[tens[i][2]] : id_ops[tens[i][0]];
+                const int btype = tens[i][1] == B_TENS || tens[i][1] == B_WORKKET ? B_KET : B_BRA;
+                int d1[3], d2[3];
+                b_get_dims(tens[i][0], d1, td, wd, id_ops);
+                b_get_dims(tens[i][1], d2, td, wd, id_ops);
+                r += b_ops_needed(tens[i][2], bc, btype, d1, d2, wd);
+        }
+        return r;
+}
+
+static void b_fill(struct b_cinfo* ci, const int (*tens)[3], int (*td)[2], int (*wd)[2], const int* id_ops)
+{
+        for (int i = 0; i < 3; ++i) {
+                struct b_cinfo* c = &ci[i];
+                memset(c, 0, sizeof(*c));
+                const int last = i == 2;
+                const int bc = last ? id_ops[tens[i][2]] : id_ops[tens[i][0]];
+                const int btype = tens[i][1] == B_TENS || tens[i][1] == B_WORKKET ? B_KET : B_BRA;
+                const int* bs = b_bondz[bc];
+                int d1[3], d2[3];
+                b_get_dims(tens[i][0], d1, td, wd, id_ops);
+                b_get_dims(tens[i][1], d2, td, wd, id_ops);
+                if (last) {
+                        c->ta = bc != 0;
+                        c->tb = bc == 0;
+                        c->M = d1[bc];
+                        c->N = d2[bc];
+                        if (bc == 1) {
+                                c->K = d2[bs[0]];
+                                c->L = d2[bs[1]];
+                                c->stride[0] = (int64_t) c->M * c->K;
+                                c->stride[1] = (int64_t) c->N * c->K;
+                        } else {
+                                c->K = d1[bs[0]] * d1[bs[1]];
+                                c->L = 1;
+                        }
+                        c->tens[0] = tens[i][0];
+                        c->tens[1] = tens[i][1];
+                        c->tens[2] = tens[i][2];
+                } else if (bc == 0) {
+                        c->ta = btype != B_KET;
+                        c->tb = 0;
+                        c->M = d1[!btype];
+                        c->N = d2[bs[0]] * d2[bs[1]];
+                        c->K = d1[btype];
+                        c->L = 1;
+                        c->tens[0] = tens[i][0];
+                        c->tens[1] = tens[i][1];
+                        c->tens[2] = tens[i][2];
+                } else if (bc == 1) {
+                        c->ta = 0;
+                        c->tb = btype == B_KET;
+                        c->M = d2[bs[0]];
+                        c->N = d1[!btype];
+                        c->K = d1[btype];
+                        c->L = d2[bs[1]];
+                        c->stride[0] = (int64_t) c->K * c->M;
+                        c->stride[2] = (int64_t) c->N * c->M;
+                        c->tens[0] = tens[i][1];
+                        c->tens[1] = tens[i][0];
+                        c->tens[2] = tens[i][2];
+                } else {
+                        c->ta = 0;
+                        c->tb = btype != B_BRA;
+                        c->M = d2[bs[0]] * d2[bs[1]];
+                        c->N = d1[!btype];
+                        c->K = d1[btype];
+                        c->L = 1;
+                        c->tens[0] = tens[i][1];
+                        c->tens[1] = tens[i][0];
+                        c->tens[2] = tens[i][2];
+                }
+                c->lda = !c->ta ? c->M : c->K;
+                c->ldb = !c->tb ? c->K : c->N;
+                c->ldc = c->M;
+        }
+}
+
+/* -1: the cheapest of the six contraction orders, as the reference;
+ * 0..5: that order for every block (for the order-independence check) */
+static int b_forced_way = -1;
+
+void oracle_set_bupdate_order(int way)
+{
+        b_forced_way = way;
+}
+
+static int64_t b_block_size(const t3ns_rops_desc* r, int op, int b)
+{
+        return r->beginblock[op][b + 1] - r->beginblock[op][b];
+}
+
+int64_t oracle_update_branching(const t3ns_bupdate_desc* d, const t3ns_rops_desc* ops,
+                                int64_t* out_bb, double* out_tel)
+{
+        const int uc = d->update_case;
+        const int H = d->mpo.nrSecs;
+        const int id_ops[3] = {uc == 0 ? 1 : 0, uc == 2 ? 1 : 2, uc};
+        const int looptype = uc == 0;
+        int maxdims[3][3];
+        for (int i = 0; i < 3; ++i) {
+                maxdims[i][B_BRA] = maxdims[i][B_KET] = d->site_ss[i].nrSecs;
+                maxdims[i][B_MPO] = H;
+        }
+        /* unique operators */
+        int nu = 0;
+        int* run_first = malloc(sizeof(int) * (d->nr_instr + 1));
+        int* run_of = malloc(sizeof(int) * (d->nr_instr + 1));
+        for (int i = 0; i < d->nr_instr; ++i) {
+                const int* a = d->instr + 3 * i;
+                const int* b = d->instr + 3 * (i - 1);
+                if (i == 0 || a[0] != b[0] || a[1] != b[1] || d->hss_of_new[a[2]] != d->hss_of_new[b[2]])
+                        run_first[nu++] = i;
+                run_of[i] = nu - 1;
+        }
+        double** utel = malloc(sizeof(double*) * (nu + 1));
+        int* u_hss = malloc(sizeof(int) * (nu + 1));
+        for (int u = 0; u < nu; ++u) {
+                const int h = d->hss_of_new[d->instr[3 * run_first[u] + 2]];
+                const int nb = d->begin_blocks_of_hss[h + 1] - d->begin_blocks_of_hss[h];
+                u_hss[u] = h;
+                utel[u] = calloc((size_t) d->hss_beginblock[h][nb] + 1, sizeof(double));
+        }
+        /* make_qntens: tensor blocks grouped by the two kets the loop fixes */
+        const int64_t divide = looptype ? (int64_t) maxdims[0][B_KET] * maxdims[1][B_KET] : maxdims[0][B_KET];
+        /* (the groups are searched by linear scan below, in block order) */
+
+        const int first = looptype ? B_OPS1 : B_OPS2, second = looptype ? B_OPS2 : B_OPS1;
+        const int N = d->begin_blocks_of_hss[d->nrhss];
+        for (int new_sb = 0; new_sb < N; ++new_sb) {
+                int id[3][3];
+                int64_t qn = d->qnumbers[new_sb];
+                const int nb_ = id_ops[B_NEWOPS];
+                id[nb_][B_BRA] = (int) (qn % maxdims[nb_][B_BRA]);
+                qn /= maxdims[nb_][B_BRA];
+                id[nb_][B_KET] = (int) (qn % maxdims[nb_][B_KET]);
+                id[nb_][B_MPO] = (int) (qn / maxdims[nb_][B_KET]);
+                const int hn = id[nb_][B_MPO];
+                const int sb_new = new_sb - d->begin_blocks_of_hss[hn];
+                for (int p = d->tprod_begin[hn]; p < d->tprod_begin[hn + 1]; ++p) {
+                        const int hss_ops[2] = {d->tprods[2 * p], d->tprods[2 * p + 1]};
+                        const t3ns_rops_desc* F = &ops[first];
+                        const t3ns_rops_desc* S = &ops[second];
+                        const int bf = id_ops[first], bs = id_ops[second];
+                        const int f0 = F->begin_blocks_of_hss[hss_ops[first]];
+                        const int nbf = F->begin_blocks_of_hss[hss_ops[first] + 1] - f0;
+                        for (int jf = 0; jf < nbf; ++jf) {
+                                int64_t q = F->qnumbers[f0 + jf];
+                                id[bf][B_BRA] = (int) (q % maxdims[bf][B_BRA]);
+                                q /= maxdims[bf][B_BRA];
+                                id[bf][B_KET] = (int) (q % maxdims[bf][B_KET]);
+                                id[bf][B_MPO] = (int) (q / maxdims[bf][B_KET]);
+                                id[bs][B_MPO] = hss_ops[second];
+                                const int64_t match = looptype ? id[0][B_KET] + (int64_t) id[1][B_KET] * maxdims[0][B_KET]
+                                                               : id[1][B_KET] + (int64_t) id[2][B_KET] * maxdims[1][B_KET];
+                                for (int tb = 0; tb < d->T_nrblocks; ++tb) {
+                                        const int64_t tq = d->T_qnumbers[tb];
+                                        if ((looptype ? tq % divide : tq / divide) != match) continue;
+                                        id[bs][B_KET] = (int) (looptype ? tq / divide : tq % divide);
+                                        const int s0 = S->begin_blocks_of_hss[hss_ops[second]];
+                                        const int nbs = S->begin_blocks_of_hss[hss_ops[second] + 1] - s0;
+                                        for (int js = 0; js < nbs; ++js) {
+                                                const int64_t sq = S->qnumbers[s0 + js];
+                                                if (sq / maxdims[bs][B_BRA] != id[bs][B_KET] + (int64_t) hss_ops[second] * maxdims[bs][B_KET])
+                                                        continue;
+                                                id[bs][B_BRA] = (int) (sq % maxdims[bs][B_BRA]);
+                                                /* find_block_adj */
+                                                int64_t aq = 0;
+                                                for (int i = 2; i >= 0; --i) aq = aq * maxdims[i][B_BRA] + id[i][B_BRA];
+                                                const int ab = bsearch_qn(aq, d->T_qnumbers, d->T_nrblocks);
+                                                if (ab < 0) continue;
+                                                int sb_op[3];
+                                                sb_op[first] = jf;
+                                                sb_op[second] = js;
+                                                sb_op[B_NEWOPS] = sb_new;
+                                                /* update_selected_blocks */
+                                                int irr[3][3][T3NS_MAX_SYMMETRIES];
+                                                int td[3][2];
+                                                for (int i = 0; i < 3; ++i) {
+                                                        memcpy(irr[i][0], irrep_of(&d->site_ss[i], id[i][B_BRA]), sizeof(irr[i][0]));
+                                                        memcpy(irr[i][1], irrep_of(&d->site_ss[i], id[i][B_KET]), sizeof(irr[i][1]));
+                                                        memcpy(irr[i][2], irrep_of(&d->mpo, id[i][B_MPO]), sizeof(irr[i][2]));
+                                                        td[i][B_BRA] = d->site_ss[i].dims[id[i][B_BRA]];
+                                                        td[i][B_KET] = d->site_ss[i].dims[id[i][B_KET]];
+                                                }
+                                                const double pref = oracle_prefactor_bUpdate(&irr[0][0][0], uc, d->sgs, d->nrSyms);
+                                                struct b_cinfo ci[3];
+                                                int wsz[2] = {0, 0};
+                                                int best = 0;
+                                                for (int w = 0; w < 6; ++w) {
+                                                        int wd[3][2];
+                                                        const int cost = b_calc(b_ways[w], td, wd, id_ops);
+                                                        if (b_forced_way >= 0 ? w == b_forced_way
+                                                                              : (w == 0 || best > cost)) {
+                                                                best = cost;
+                                                                b_fill(ci, b_ways[w], td, wd, id_ops);
+                                                                wsz[0] = wd[0][B_BRA] * wd[1][B_BRA] * wd[2][B_BRA];
+                                                                wsz[1] = wd[0][B_KET] * wd[1][B_KET] * wd[2][B_KET];
+                                                        }
+                                                }
+                                                double* tel[7] = {0};
+                                                tel[B_TENS] = (double*) d->T_tel + d->T_beginblock[tb];
+                                                tel[B_ADJ] = (double*) d->T_tel + d->T_beginblock[ab];
+                                                tel[B_WORKBRA] = malloc(sizeof(double) * ((size_t) wsz[0] + 1));
+                                                tel[B_WORKKET] = malloc(sizeof(double) * ((size_t) wsz[1] + 1));
+                                                /* find_matching_instr: unique runs of this MPO combination */
+                                                for (int u = 0; u < nu; ++u) {
+                                                        const int* ins = d->instr + 3 * run_first[u];
+                                                        if (ops[0].hss_of_ops[ins[0]] != hss_ops[0] ||
+                                                            ops[1].hss_of_ops[ins[1]] != hss_ops[1] ||
+                                                            d->hss_of_new[ins[2]] != hn)
+                                                                continue;
+                                                        if (b_block_size(&ops[0], ins[0], sb_op[B_OPS1]) == 0 ||
+                                                            b_block_size(&ops[1], ins[1], sb_op[B_OPS2]) == 0)
+                                                                continue;
+                                                        const int64_t* ubb = d->hss_beginblock[hn];
+                                                        if (ubb[sb_new + 1] == ubb[sb_new]) continue;
+                                                        tel[B_OPS1] = (double*) ops[0].tel[ins[0]] + ops[0].beginblock[ins[0]][sb_op[B_OPS1]];
+                                                        tel[B_OPS2] = (double*) ops[1].tel[ins[1]] + ops[1].beginblock[ins[1]][sb_op[B_OPS2]];
+                                                        tel[B_NEWOPS] = utel[u] + ubb[sb_new];
+                                                        for (int c = 0; c < 3; ++c) {
+                                                                const struct b_cinfo* x = &ci[c];
+                                                                for (int l = 0; l < x->L; ++l)
+                                                                        oracle_dgemm_batched(x->ta, x->tb, x->M, x->N, x->K, 1,
+                                                                                             c == 2 ? pref : 1.0,
+                                                                                             tel[x->tens[0]] + l * x->stride[0], x->lda, 0,
+                                                                                             tel[x->tens[1]] + l * x->stride[1], x->ldb, 0,
+                                                                                             c == 2 ? 1.0 : 0.0,
+                                                                                             tel[x->tens[2]] + l * x->stride[2], x->ldc, 0);
+                                                        }
+                                                }
+                                                free(tel[B_WORKBRA]);
+                                                free(tel[B_WORKKET]);
+                                        }
+                                }
+                        }
+                }
+        }
+        /* sum_unique_rOperators */
+        int nops = 0;
+        for (int i = 0; i < d->nr_instr; ++i)
+                if (d->instr[3 * i + 2] + 1 > nops) nops = d->instr[3 * i + 2] + 1;
+        double** ntel = malloc(sizeof(double*) * (nops + 1));
+        for (int k = 0; k < nops; ++k) {
+                const int h = d->hss_of_new[k];
+                const int nb = d->begin_blocks_of_hss[h + 1] - d->begin_blocks_of_hss[h];
+                ntel[k] = calloc((size_t) d->hss_beginblock[h][nb] + 1, sizeof(double));
+        }
+        for (int i = 0; i < d->nr_instr; ++i) {
+                const int k = d->instr[3 * i + 2];
+                const int h = d->hss_of_new[k];
+                const int nb = d->begin_blocks_of_hss[h + 1] - d->begin_blocks_of_hss[h];
+                const int64_t n = d->hss_beginblock[h][nb];
+                for (int64_t j = 0; j < n; ++j) ntel[k][j] += d->pref[i] * utel[run_of[i]][j];
+        }
+        int64_t total = 0, nbbo = 0;
+        for (int k = 0; k < nops; ++k) {
+                const int h = d->hss_of_new[k];
+                const int nb = d->begin_blocks_of_hss[h + 1] - d->begin_blocks_of_hss[h];
+                const int64_t* bb = d->hss_beginblock[h];
+                int64_t pos = 0;
+                if (out_bb) out_bb[nbbo] = 0;
+                for (int b = 0; b < nb; ++b) {
+                        int nonzero = 0;
+                        for (int64_t j = bb[b]; j < bb[b + 1]; ++j)
+                                if (!(fabs(ntel[k][j]) < 1e-20)) { nonzero = 1; break; }
+                        const int64_t n = nonzero ? bb[b + 1] - bb[b] : 0;
+                        if (out_tel && n) memcpy(out_tel + total + pos, ntel[k] + bb[b], sizeof(double) * n);
+                        pos += n;
+                        if (out_bb) out_bb[nbbo + b + 1] = pos;
+                }
+                nbbo += nb + 1;
+                total += pos;
+        }
+        for (int u = 0; u < nu; ++u) free(utel[u]);
+        for (int k = 0; k < nops; ++k) free(ntel[k]);
+        free(utel);
+        free(ntel);
+        free(u_hss);
+        free(run_first);
+        free(run_of);
+        return total;
+}

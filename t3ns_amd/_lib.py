@@ -66,6 +66,16 @@ class PUpdateDesc(C.Structure):
                 ("qnumbers", c_i64_p), ("hss_beginblock", C.POINTER(c_i64_p))]
 
 
+class BUpdateDesc(C.Structure):
+    _fields_ = [("nrSyms", C.c_int), ("sgs", C.c_int * MAX_SYMMETRIES), ("update_case", C.c_int),
+                ("site_ss", SymsecsDesc * 3), ("mpo", SymsecsDesc),
+                ("T_nrblocks", C.c_int), ("T_qnumbers", c_i64_p), ("T_beginblock", c_i64_p),
+                ("T_tel", c_dbl_p), ("nr_instr", C.c_int), ("instr", c_int_p), ("pref", c_dbl_p),
+                ("hss_of_new", c_int_p), ("tprod_begin", c_int_p), ("tprods", c_int_p),
+                ("nrhss", C.c_int), ("begin_blocks_of_hss", c_int_p), ("qnumbers", c_i64_p),
+                ("hss_beginblock", C.POINTER(c_i64_p))]
+
+
 class Work(C.Structure):
     _fields_ = [("flops", C.c_double), ("bytes", C.c_double), ("kernel_ms", C.c_double)]
 
@@ -106,6 +116,10 @@ def lib():
         "t3ns_update_physical": (C.c_int, [C.POINTER(PUpdateDesc), C.POINTER(RopsDesc),
                                            C.POINTER(P)]),
         "t3ns_rops_work": (C.c_int, [P, c_dbl_p, c_dbl_p, c_dbl_p]),
+        "t3ns_update_branching": (C.c_int, [C.POINTER(BUpdateDesc), C.POINTER(RopsDesc),
+                                            C.POINTER(P)]),
+        "t3ns_ctx_bupdate_desc": (C.c_int, [P, C.c_int, C.c_int, C.c_uint64, C.POINTER(BUpdateDesc),
+                                            C.POINTER(RopsDesc)]),
         "t3ns_make_site_tensor": (C.c_int, [C.POINTER(MakesiteDesc), c_dbl_p, C.POINTER(Work)]),
         "t3ns_step_makesite_desc": (C.c_int, [P, C.c_uint64, C.POINTER(MakesiteDesc)]),
         "t3ns_step_pupdate_desc": (C.c_int, [P, C.c_int, C.c_uint64, C.c_int,

@@ -464,6 +464,63 @@ int t3ns_update_physical(const t3ns_pupdate_desc* d, const t3ns_rops_desc* pops,
     });
 }
 
+int t3ns_update_branching(const t3ns_bupdate_desc* d, const t3ns_rops_desc* ops, t3ns_rops** out) {
+    if (!d || !ops || !out) return fail(T3NS_ERR_INVALID, "null argument");
+    return guarded([&] {
+        const ROperators o0 = rops_from(ops[0]);
+        const ROperators o1 = rops_from(ops[1]);
+        BUpdateInput in;
+        in.sym.nrSyms = d->nrSyms;
+        for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) in.sym.sgs[i] = d->sgs[i];
+        in.update_case = d->update_case;
+        for (int k = 0; k < 3; ++k) in.site_ss[k] = symsecs_from(d->site_ss[k]);
+        in.mpo = symsecs_from(d->mpo);
+        const int H = in.mpo.nrSecs;
+        if (d->nrhss != H || o0.nrhss != H || o1.nrhss != H)
+            throw std::invalid_argument("bUpdate: MPO sectors differ");
+        if (d->T_nrblocks < 0 || (d->T_nrblocks && (!d->T_qnumbers || !d->T_beginblock || !d->T_tel)))
+            throw std::invalid_argument("bUpdate: bad site tensor");
+        SiteTensor T;
+        T.nrsites = 1;
+        T.nrblocks = d->T_nrblocks;
+        T.qnumbers.assign(d->T_qnumbers, d->T_qnumbers + T.nrblocks);
+        T.beginblock.assign(1, 0);
+        if (T.nrblocks) T.beginblock.assign(d->T_beginblock, d->T_beginblock + T.nrblocks + 1);
+        T.tel.assign(d->T_tel, d->T_tel + T.size());
+        if (d->nr_instr < 0 || (d->nr_instr && (!d->instr || !d->pref || !d->hss_of_new)))
+            throw std::invalid_argument("bUpdate: bad instructions");
+        int nops = 0;
+        for (int i = 0; i < d->nr_instr; ++i) {
+            Instruction ins;
+            for (int k = 0; k < 3; ++k) ins.instr[k] = d->instr[3 * i + k];
+            ins.pref = d->pref[i];
+            if (ins.instr[2] < 0) throw std::invalid_argument("bUpdate: instruction out of range");
+            nops = std::max(nops, ins.instr[2] + 1);
+            in.instr.push_back(ins);
+        }
+        in.hss_of_new.assign(d->hss_of_new, d->hss_of_new + nops);
+        in.tprod_begin.assign(d->tprod_begin, d->tprod_begin + H + 1);
+        in.tprods.assign(d->tprods, d->tprods + 2 * (size_t)in.tprod_begin[H]);
+        in.nrhss = H;
+        in.begin_blocks_of_hss.assign(d->begin_blocks_of_hss, d->begin_blocks_of_hss + H + 1);
+        const int nb = in.begin_blocks_of_hss[H];
+        in.qnumbers.assign(d->qnumbers, d->qnumbers + nb);
+        in.hss_bb.resize(H);
+        for (int h = 0; h < H; ++h) {
+            const int n = in.begin_blocks_of_hss[h + 1] - in.begin_blocks_of_hss[h];
+            in.hss_bb[h].assign(d->hss_beginblock[h], d->hss_beginblock[h] + n + 1);
+        }
+        in.T = &T;
+        in.ops[0] = &o0;
+        in.ops[1] = &o1;
+        const BUpdatePlan plan = plan_bupdate(in);
+        auto res = std::make_unique<t3ns_rops>();
+        res->r = run_bupdate(plan, T, o0, o1, nullptr, &res->work);
+        res->views();
+        *out = res.release();
+    });
+}
+
 int t3ns_make_site_tensor(const t3ns_makesite_desc* d, double* tel, t3ns_work* work) {
     if (!d || !tel) return fail(T3NS_ERR_INVALID, "null argument");
     return guarded([&] {

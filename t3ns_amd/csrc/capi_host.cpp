@@ -88,8 +88,25 @@ void operator_types(Context& c, int bond, int is_left, int* nrops, std::vector<i
 }
 }  // namespace
 
+// An update_rOperators_branching problem: seeded old operators on two bonds
+// of a branching site, a seeded branching tensor and the flattened inputs.
+struct BUpdateStore {
+    ROperators ops[2];
+    std::vector<const int64_t*> bb_ptr[2];
+    std::vector<const double*> tel_ptr[2];
+    SymsecsFlat site_ss[3], mpo;
+    SiteTensor T;
+    std::vector<int> instr, hss_of_new, tprod_begin, tprods, bbh;
+    std::vector<double> pref;
+    std::vector<QN> qnumbers;
+    std::vector<std::vector<int64_t>> hss_bb;
+    std::vector<const int64_t*> hss_bb_ptr;
+    int update_case = -1, nrhss = 0;
+};
+
 struct t3ns_ctx {
     std::unique_ptr<Context> c;
+    std::vector<std::unique_ptr<BUpdateStore>> bups;
 };
 
 // An append problem of one physical leg of a step: the operators of the
@@ -631,6 +648,121 @@ int t3ns_step_makesite_desc(t3ns_step* st, uint64_t seed, t3ns_makesite_desc* d)
         d->nrblocks = M.nrblocks;
         d->qnumbers = M.qnumbers.data();
         d->beginblock = M.beginblock.data();
+    });
+}
+
+int t3ns_ctx_bupdate_desc(t3ns_ctx* cx, int site, int update_case, uint64_t seed,
+                          t3ns_bupdate_desc* d, t3ns_rops_desc* ops) {
+    if (!cx || !d || !ops || update_case < 0 || update_case > 2) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        Context& c = *cx->c;
+        if (site < 0 || site >= c.netw.sites || c.netw.is_psite(site))
+            throw std::invalid_argument("not a branching site");
+        auto st = std::make_unique<BUpdateStore>();
+        BUpdateStore& u = *st;
+        u.update_case = update_case;
+        int sb[3];
+        c.netw.bonds_of_site(site, sb);
+        // roles: ops[0] on bond (case 0 ? 1 : 0), left; ops[1] on bond
+        // (case 2 ? 1 : 2), left only for case 2 (src/rOperators_bUpdate.c:292-313)
+        const int obond[2] = {sb[update_case == 0 ? 1 : 0], sb[update_case == 2 ? 1 : 2]};
+        const int oleft[2] = {1, update_case == 2};
+        Rng rng(seed);
+        for (int r = 0; r < 2; ++r) {
+            ROperators o;
+            if (c.netw.bonds[obond[r]][!oleft[r]] == -1) {
+                o = vacuum_rOperators(c, obond[r], oleft[r]);
+            } else {
+                int nrops;
+                std::vector<int> hss;
+                std::vector<std::vector<int64_t>> tmpbb;
+                operator_types(c, obond[r], oleft[r], &nrops, &hss);
+                o = init_rOperators_structure(c, obond[r], oleft[r], false, &tmpbb);
+                o.nrops = nrops;
+                o.hss_of_ops = hss;
+                o.beginblock.resize(nrops);
+                for (int k = 0; k < nrops; ++k) o.beginblock[k] = tmpbb[hss[k]];
+                o.finalize_offsets();
+                o.tel.resize(o.total_size());
+                for (auto& v : o.tel) v = rng.normal();
+            }
+            u.ops[r] = std::move(o);
+            u.bb_ptr[r].resize(u.ops[r].nrops);
+            u.tel_ptr[r].resize(u.ops[r].nrops);
+            for (int k = 0; k < u.ops[r].nrops; ++k) {
+                u.bb_ptr[r][k] = u.ops[r].beginblock[k].data();
+                u.tel_ptr[r][k] = u.ops[r].tel.data() + u.ops[r].op_offset[k];
+            }
+        }
+        u.T = init_1siteTensor(c, site);
+        for (auto& v : u.T.tel) v = rng.normal();
+        for (int k = 0; k < 3; ++k) u.site_ss[k].set(c.symsecs(sb[k]));
+        u.mpo.set(c.symsecs(-1));
+        const int nbond = sb[update_case];
+        const int nleft = update_case == 2;
+        const InstructionSet& bs = c.fetch_bUpdate(nbond, nleft);
+        int nops = 0;
+        for (const auto& i : bs.instr) {
+            for (int k = 0; k < 3; ++k) u.instr.push_back(i.instr[k]);
+            u.pref.push_back(i.pref);
+            nops = std::max(nops, i.instr[2] + 1);
+        }
+        u.hss_of_new.assign(bs.hss_of_new.begin(), bs.hss_of_new.begin() + nops);
+        const int H = c.ham->nr_hss();
+        u.tprod_begin.assign(1, 0);
+        for (int h = 0; h < H; ++h) {
+            for (const auto& pr : c.ham->tprods_ham(h, site)) {
+                u.tprods.push_back(pr[0]);
+                u.tprods.push_back(pr[1]);
+            }
+            u.tprod_begin.push_back((int)u.tprods.size() / 2);
+        }
+        std::vector<std::vector<int64_t>> tmpbb;
+        ROperators ns = init_rOperators_structure(c, nbond, nleft, false, &tmpbb);
+        u.nrhss = ns.nrhss;
+        u.bbh = ns.begin_blocks_of_hss;
+        u.qnumbers = ns.qnumbers;
+        u.hss_bb = tmpbb;
+        for (auto& v : u.hss_bb)
+            if (v.empty()) v.assign(1, 0);
+        u.hss_bb_ptr.resize(u.hss_bb.size());
+        for (size_t h = 0; h < u.hss_bb.size(); ++h) u.hss_bb_ptr[h] = u.hss_bb[h].data();
+
+        std::memset(d, 0, sizeof(*d));
+        d->nrSyms = c.sym.nrSyms;
+        for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) d->sgs[i] = c.sym.sgs[i];
+        d->update_case = update_case;
+        for (int k = 0; k < 3; ++k) d->site_ss[k] = u.site_ss[k].desc;
+        d->mpo = u.mpo.desc;
+        d->T_nrblocks = u.T.nrblocks;
+        d->T_qnumbers = u.T.qnumbers.data();
+        d->T_beginblock = u.T.beginblock.data();
+        d->T_tel = u.T.tel.data();
+        d->nr_instr = (int)u.pref.size();
+        d->instr = u.instr.data();
+        d->pref = u.pref.data();
+        d->hss_of_new = u.hss_of_new.data();
+        d->tprod_begin = u.tprod_begin.data();
+        d->tprods = u.tprods.data();
+        d->nrhss = u.nrhss;
+        d->begin_blocks_of_hss = u.bbh.data();
+        d->qnumbers = u.qnumbers.data();
+        d->hss_beginblock = u.hss_bb_ptr.data();
+        for (int r = 0; r < 2; ++r) {
+            t3ns_rops_desc& o = ops[r];
+            std::memset(&o, 0, sizeof(o));
+            o.bond = u.ops[r].bond;
+            o.is_left = u.ops[r].is_left;
+            o.P_operator = 0;
+            o.nrhss = u.ops[r].nrhss;
+            o.begin_blocks_of_hss = u.ops[r].begin_blocks_of_hss.data();
+            o.qnumbers = u.ops[r].qnumbers.data();
+            o.nrops = u.ops[r].nrops;
+            o.hss_of_ops = u.ops[r].hss_of_ops.data();
+            o.beginblock = u.bb_ptr[r].data();
+            o.tel = u.tel_ptr[r].data();
+        }
+        cx->bups.push_back(std::move(st));
     });
 }
 

@@ -6,6 +6,8 @@
 #include "tile_gemm.hpp"
 
 #include <algorithm>
+#include <array>
+#include <unordered_map>
 
 namespace t3 {
 
@@ -58,41 +60,43 @@ __global__ void fill_zero(T* p, int64_t n) {
 int grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(8192, n)); }
 }  // namespace
 
-ROperators run_append(const AppendPlan& plan, const ROperators& old, hipStream_t s) {
-    const ROperators& nr = plan.nr;
+// dst groups = sum_t pref_t * (factor_t * src[src_t]) over a device source,
+// then the blocks left all zero are dropped (src/sparseblocks.c:73-107) and
+// the result copied to the host.
+static ROperators finish_sum(const ROperators& nr, const std::vector<AppendGroup>& groups,
+                             const std::vector<AppendTask>& tasks, const double* d_src, hipStream_t s) {
     const int64_t nnew = nr.total_size();
     // flat block index of every (op, block)
     std::vector<int> flat_of_op(nr.nrops + 1, 0);
     for (int k = 0; k < nr.nrops; ++k) flat_of_op[k + 1] = flat_of_op[k] + nr.nblocks_op(k);
     const int nflat = flat_of_op[nr.nrops];
-    std::vector<DevGroup> hg(plan.groups.size());
-    for (size_t i = 0; i < plan.groups.size(); ++i) {
-        const AppendGroup& g = plan.groups[i];
+    std::vector<DevGroup> hg(groups.size());
+    for (size_t i = 0; i < groups.size(); ++i) {
+        const AppendGroup& g = groups[i];
         hg[i] = DevGroup{g.dst, g.n, g.task_begin, g.task_end, flat_of_op[g.op] + g.block, 0};
     }
-    std::vector<int64_t> tsrc(plan.tasks.size());
-    std::vector<double> tfac(plan.tasks.size()), tpref(plan.tasks.size());
-    for (size_t i = 0; i < plan.tasks.size(); ++i) {
-        tsrc[i] = plan.tasks[i].src;
-        tfac[i] = plan.tasks[i].factor;
-        tpref[i] = plan.tasks[i].pref;
+    std::vector<int64_t> tsrc(tasks.size());
+    std::vector<double> tfac(tasks.size()), tpref(tasks.size());
+    for (size_t i = 0; i < tasks.size(); ++i) {
+        tsrc[i] = tasks[i].src;
+        tfac[i] = tasks[i].factor;
+        tpref[i] = tasks[i].pref;
     }
     DevBuf<DevGroup> dg;
     DevBuf<int64_t> dsrc;
-    DevBuf<double> dfac, dpref, dold, dnew;
+    DevBuf<double> dfac, dpref, dnew;
     DevBuf<unsigned char> dkeep;
     dg.upload(hg, s);
     dsrc.upload(tsrc, s);
     dfac.upload(tfac, s);
     dpref.upload(tpref, s);
-    dold.upload(old.tel, s);
     dnew.alloc(std::max<int64_t>(1, nnew));
     dkeep.alloc(std::max(1, nflat));
     hipLaunchKernelGGL(fill_zero<unsigned char>, dim3(grid((nflat + 255) / 256)), dim3(256), 0, s,
                        dkeep.get(), (int64_t)nflat);
     if (!hg.empty())
         hipLaunchKernelGGL(append_kernel, dim3(grid((int64_t)hg.size())), dim3(kThreads), 0, s,
-                           dg.get(), (int)hg.size(), dsrc.get(), dfac.get(), dpref.get(), dold.get(),
+                           dg.get(), (int)hg.size(), dsrc.get(), dfac.get(), dpref.get(), d_src,
                            dnew.get(), dkeep.get());
     std::vector<unsigned char> keep(nflat);
     if (nflat)
@@ -118,6 +122,13 @@ ROperators run_append(const AppendPlan& plan, const ROperators& old, hipStream_t
     }
     T3_HIP_CHECK(hipGetLastError());
     return out;
+}
+
+ROperators run_append(const AppendPlan& plan, const ROperators& old, hipStream_t s) {
+    DevBuf<double> dold;
+    dold.upload(old.tel, s);
+    if (old.tel.empty()) dold.alloc(1);
+    return finish_sum(plan.nr, plan.groups, plan.tasks, dold.get(), s);
 }
 
 }  // namespace t3
@@ -347,6 +358,244 @@ ROperators run_pupdate(const PUpdatePlan& plan, const ROperators& pops, const Si
     }
     T3_HIP_CHECK(hipGetLastError());
     return out;
+}
+
+}  // namespace t3
+
+// ---------------------------------------------------------------------------
+// update_rOperators_branching: U[dst] = sum_t pref_t * A_t . (O1_t x O2_t) . T_t,
+// then the instruction sums of the unique operators.
+// ---------------------------------------------------------------------------
+namespace t3 {
+
+namespace {
+
+// Y = X x_leg O: X has dims n[3] (column-major), O is (e x n[leg]) column-
+// major; Y has n[leg] replaced by e.
+void mode_tiles(std::vector<GemmTile>& out, int64_t x, const int n[3], int leg, int64_t o, int e,
+                int64_t y, double* flops) {
+    GemmTile g;
+    if (leg == 0) {
+        g.p = Strided{o, 1, e};
+        g.q = Strided{x, 1, n[0]};
+        g.M = e;
+        g.N = n[1] * n[2];
+        g.K = n[0];
+        g.x_off = y;
+        g.ldx = e;
+        for (int a = 0; a < tiles_of(g.M); ++a)
+            for (int b = 0; b < tiles_of(g.N); ++b) {
+                g.ti = a;
+                g.tj = b;
+                out.push_back(g);
+            }
+    } else if (leg == 2) {
+        g.p = Strided{x, 1, n[0] * n[1]};
+        g.q = Strided{o, e, 1};
+        g.M = n[0] * n[1];
+        g.N = e;
+        g.K = n[2];
+        g.x_off = y;
+        g.ldx = g.M;
+        for (int a = 0; a < tiles_of(g.M); ++a)
+            for (int b = 0; b < tiles_of(g.N); ++b) {
+                g.ti = a;
+                g.tj = b;
+                out.push_back(g);
+            }
+    } else {
+        for (int l = 0; l < n[2]; ++l) {
+            g.p = Strided{x + (int64_t)l * n[0] * n[1], 1, n[0]};
+            g.q = Strided{o, e, 1};
+            g.M = n[0];
+            g.N = e;
+            g.K = n[1];
+            g.x_off = y + (int64_t)l * n[0] * e;
+            g.ldx = n[0];
+            for (int a = 0; a < tiles_of(g.M); ++a)
+                for (int b = 0; b < tiles_of(g.N); ++b) {
+                    g.ti = a;
+                    g.tj = b;
+                    out.push_back(g);
+                }
+        }
+    }
+    *flops += 2.0 * e * (double)n[0] * n[1] * n[2];
+}
+
+struct KeyHash {
+    size_t operator()(const std::array<int64_t, 3>& k) const {
+        return std::hash<int64_t>()(k[0] * 1000003 + k[1] * 10007 + k[2]);
+    }
+};
+
+}  // namespace
+
+ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROperators& ops0,
+                       const ROperators& ops1, hipStream_t s, RenormWork* work) {
+    const int nl = plan.n_leg, xl = plan.x_leg, yl = plan.y_leg;
+    const int64_t nT = T.size(), n0 = ops0.total_size(), n1 = ops1.total_size();
+    const int64_t nU = plan.uniq.total_size();
+    // Arena: [T | ops0 | ops1 | unique operators | workspace]
+    const int64_t o0_base = nT, o1_base = nT + n0, u_base = nT + n0 + n1, w_base = u_base + nU;
+    // workspace per chunk of destinations
+    constexpr int64_t kWork = (int64_t)1 << 28;
+    struct Chunk {
+        std::vector<GemmTile> p1, p2;
+        std::vector<SumTile> sums;
+        std::vector<SumTask> tasks;
+    };
+    std::vector<Chunk> chunks;
+    double flops = 0;
+    int64_t wmax = 0;
+    size_t di = 0;
+    while (di < plan.dests.size()) {
+        Chunk c;
+        std::unordered_map<std::array<int64_t, 3>, int64_t, KeyHash> w1, w2;
+        int64_t wtop = 0;
+        for (; di < plan.dests.size(); ++di) {
+            const BUpdateDest& dd = plan.dests[di];
+            // workspace this destination needs at most
+            int64_t need = 0;
+            for (int t = dd.task_begin; t < dd.task_end; ++t) {
+                const BUpdateTask& tk = plan.tasks[t];
+                int64_t s1 = 1, s2 = 1;
+                for (int i = 0; i < 3; ++i) {
+                    s1 *= i == xl ? tk.e[i] : tk.d[i];
+                    s2 *= i == nl ? tk.d[i] : tk.e[i];
+                }
+                need += s1 + s2;
+            }
+            if (need > kWork) throw std::runtime_error("bUpdate: block needs more than 2 GiB of workspace");
+            if (wtop + need > kWork && !c.sums.empty()) break;
+            SumTile st;
+            st.u_off = u_base + dd.dst;
+            st.M = dd.M;
+            st.N = dd.N;
+            st.task_begin = (int)c.tasks.size();
+            st.flat = 0;
+            st.pad = 0;
+            for (int t = dd.task_begin; t < dd.task_end; ++t) {
+                const BUpdateTask& tk = plan.tasks[t];
+                // stage 1: W1 = T x_x O1 (shared by every task with the same T and O1 block)
+                int n1d[3] = {tk.d[0], tk.d[1], tk.d[2]};
+                const std::array<int64_t, 3> k1 = {tk.t, tk.o1, -1};
+                auto it1 = w1.find(k1);
+                int64_t w1off;
+                if (it1 == w1.end()) {
+                    w1off = w_base + wtop;
+                    int64_t sz = (int64_t)tk.e[xl];
+                    for (int i = 0; i < 3; ++i)
+                        if (i != xl) sz *= tk.d[i];
+                    wtop += sz;
+                    mode_tiles(c.p1, tk.t, n1d, xl, o0_base + tk.o1, tk.e[xl], w1off, &flops);
+                    w1.emplace(k1, w1off);
+                } else {
+                    w1off = it1->second;
+                }
+                // stage 2: W2 = W1 x_y O2
+                int n2d[3] = {tk.d[0], tk.d[1], tk.d[2]};
+                n2d[xl] = tk.e[xl];
+                const std::array<int64_t, 3> k2 = {tk.t, tk.o1, tk.o2};
+                auto it2 = w2.find(k2);
+                int64_t w2off;
+                if (it2 == w2.end()) {
+                    w2off = w_base + wtop;
+                    int64_t sz = (int64_t)tk.e[yl];
+                    for (int i = 0; i < 3; ++i)
+                        if (i != yl) sz *= n2d[i];
+                    wtop += sz;
+                    mode_tiles(c.p2, w1off, n2d, yl, o1_base + tk.o2, tk.e[yl], w2off, &flops);
+                    w2.emplace(k2, w2off);
+                } else {
+                    w2off = it2->second;
+                }
+                // stage 3: U += pref * A . W2 over the two legs of the old operators
+                const int* e = tk.e;
+                const int* d = tk.d;
+                SumTask sk;
+                sk.pad = 0;
+                sk.pref = tk.pref;
+                if (nl == 2) {
+                    sk.r = Strided{tk.a, e[0] * e[1], 1};
+                    sk.s = Strided{w2off, 1, e[0] * e[1]};
+                    sk.K = e[0] * e[1];
+                    c.tasks.push_back(sk);
+                } else if (nl == 0) {
+                    sk.r = Strided{tk.a, 1, e[0]};
+                    sk.s = Strided{w2off, d[0], 1};
+                    sk.K = e[1] * e[2];
+                    c.tasks.push_back(sk);
+                } else {
+                    for (int g = 0; g < e[2]; ++g) {
+                        sk.r = Strided{tk.a + (int64_t)g * e[0] * e[1], e[0], 1};
+                        sk.s = Strided{w2off + (int64_t)g * e[0] * d[1], 1, e[0]};
+                        sk.K = e[0];
+                        c.tasks.push_back(sk);
+                    }
+                }
+                flops += 2.0 * (double)e[0] * e[1] * e[2] * d[nl];
+            }
+            st.task_end = (int)c.tasks.size();
+            for (int a = 0; a < tiles_of(st.M); ++a)
+                for (int b = 0; b < tiles_of(st.N); ++b) {
+                    st.ti = a;
+                    st.tj = b;
+                    c.sums.push_back(st);
+                }
+        }
+        wmax = std::max(wmax, wtop);
+        chunks.push_back(std::move(c));
+    }
+
+    DevBuf<double> arena(std::max<int64_t>(1, w_base + wmax));
+    if (nT) T3_HIP_CHECK(hipMemcpyAsync(arena.get(), T.tel.data(), nT * sizeof(double),
+                                        hipMemcpyHostToDevice, s));
+    if (n0) T3_HIP_CHECK(hipMemcpyAsync(arena.get() + o0_base, ops0.tel.data(), n0 * sizeof(double),
+                                        hipMemcpyHostToDevice, s));
+    if (n1) T3_HIP_CHECK(hipMemcpyAsync(arena.get() + o1_base, ops1.tel.data(), n1 * sizeof(double),
+                                        hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(fill_zero<double>, dim3(grid((nU + 255) / 256)), dim3(256), 0, s,
+                       arena.get() + u_base, nU);
+    DevBuf<unsigned int> dkeep(1);
+    hipEvent_t e0, e1;
+    T3_HIP_CHECK(hipEventCreate(&e0));
+    T3_HIP_CHECK(hipEventCreate(&e1));
+    T3_HIP_CHECK(hipEventRecord(e0, s));
+    for (const Chunk& c : chunks) {
+        DevBuf<GemmTile> d1, d2;
+        DevBuf<SumTile> ds;
+        DevBuf<SumTask> dt;
+        d1.upload(c.p1, s);
+        d2.upload(c.p2, s);
+        ds.upload(c.sums, s);
+        dt.upload(c.tasks, s);
+        if (!c.p1.empty())
+            hipLaunchKernelGGL(tile::gemm_tiles_kernel, dim3(grid((int64_t)c.p1.size())), dim3(256), 0,
+                               s, d1.get(), (int)c.p1.size(), arena.get());
+        if (!c.p2.empty())
+            hipLaunchKernelGGL(tile::gemm_tiles_kernel, dim3(grid((int64_t)c.p2.size())), dim3(256), 0,
+                               s, d2.get(), (int)c.p2.size(), arena.get());
+        if (!c.sums.empty())
+            hipLaunchKernelGGL(pupdate_sum_kernel, dim3(grid((int64_t)c.sums.size())), dim3(256), 0, s,
+                               ds.get(), (int)c.sums.size(), dt.get(), arena.get(), arena.get(),
+                               dkeep.get());
+        // the chunk's tile lists are freed at scope end: wait for the kernels
+        T3_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    T3_HIP_CHECK(hipEventRecord(e1, s));
+    T3_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    T3_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    T3_HIP_CHECK(hipGetLastError());
+    if (work) {
+        work->flops = flops;
+        work->bytes = 8.0 * (double)(nT + n0 + n1 + plan.nr.total_size());
+        work->kernel_ms = ms;
+    }
+    return finish_sum(plan.nr, plan.groups, plan.stasks, arena.get() + u_base, s);
 }
 
 }  // namespace t3

@@ -22,6 +22,14 @@ struct RenormWork {
 // is two GEMMs (the cheaper association of A*O*B is chosen per block pair),
 // the first into a device workspace, the second summed over the block's
 // contributions in plan order; all-zero blocks are then dropped.
+// Runs an update_rOperators_branching plan: per contribution the tensor is
+// contracted with the two old operator blocks (two GEMM stages into a device
+// workspace, processed in chunks of destination blocks), the adjoint block
+// closes the contraction in a fixed-order sum per unique-operator block, and
+// sum_unique_rOperators with zero-block removal finishes on the device.
+ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROperators& ops0,
+                       const ROperators& ops1, hipStream_t s, RenormWork* work);
+
 ROperators run_pupdate(const PUpdatePlan& plan, const ROperators& pops, const SiteTensor& T,
                        hipStream_t s, RenormWork* work);
 

@@ -320,3 +320,229 @@ PUpdatePlan plan_pupdate(const PUpdateInput& in) {
 }
 
 }  // namespace t3
+
+namespace t3 {
+
+// update_rOperators_branching (src/rOperators_bUpdate.c:229-248): the
+// enumeration of update_unique_ops_T3NS (637-677), update_newblock_w_MPO_set
+// (746-785), next_sb_tens (680-718), next_sb_sec_op (720-744),
+// find_block_adj (787-806) and update_selected_blocks (1043-1072), then
+// sum_unique_rOperators (src/rOperators_init.c:228-265).
+BUpdatePlan plan_bupdate(const BUpdateInput& in) {
+    const int uc = in.update_case;
+    if (uc < 0 || uc > 2) throw std::invalid_argument("bUpdate: bad update case");
+    const ROperators& O0 = *in.ops[0];
+    const ROperators& O1 = *in.ops[1];
+    if (O0.P_operator || O1.P_operator) throw std::invalid_argument("bUpdate: physical operators");
+    const SiteTensor& T = *in.T;
+    const int H = in.mpo.nrSecs;
+    BUpdatePlan P;
+    // bond of each role: ops[0], ops[1], new (initialize_indexhelper, 518-520)
+    const int id_ops[3] = {uc == 0 ? 1 : 0, uc == 2 ? 1 : 2, uc};
+    P.x_leg = id_ops[0];
+    P.y_leg = id_ops[1];
+    P.n_leg = uc;
+    const int looptype = uc == 0;
+    const int nsec[3] = {in.site_ss[0].nrSecs, in.site_ss[1].nrSecs, in.site_ss[2].nrSecs};
+
+    // unique operators: runs of equal (op0, op1, MPO sector of the new op)
+    const int ni = (int)in.instr.size();
+    std::vector<int> run_first, run_of(ni);
+    for (int i = 0; i < ni; ++i) {
+        const Instruction& a = in.instr[i];
+        const bool fresh = i == 0 || a.instr[0] != in.instr[i - 1].instr[0] ||
+                           a.instr[1] != in.instr[i - 1].instr[1] ||
+                           in.hss_of_new[a.instr[2]] != in.hss_of_new[in.instr[i - 1].instr[2]];
+        if (fresh) run_first.push_back(i);
+        run_of[i] = (int)run_first.size() - 1;
+    }
+    const int nu = (int)run_first.size();
+    ROperators& U = P.uniq;
+    U.bond = -1;
+    U.is_left = uc == 2;
+    U.P_operator = 0;
+    U.nrhss = in.nrhss;
+    U.begin_blocks_of_hss = in.begin_blocks_of_hss;
+    U.qnumbers = in.qnumbers;
+    U.nrops = nu;
+    U.hss_of_ops.resize(nu);
+    U.beginblock.resize(nu);
+    for (int u = 0; u < nu; ++u) {
+        U.hss_of_ops[u] = in.hss_of_new[in.instr[run_first[u]].instr[2]];
+        U.beginblock[u] = in.hss_bb[U.hss_of_ops[u]];
+    }
+    U.finalize_offsets();
+
+    // instruction helper: MPO combination -> (first instruction, unique id)
+    // (init_instrhelper, 423-493)
+    std::vector<std::pair<QN, std::vector<std::array<int, 2>>>> combos;
+    for (int u = 0; u < nu; ++u) {
+        const int* ins = in.instr[run_first[u]].instr;
+        if (ins[0] < 0 || ins[0] >= O0.nrops || ins[1] < 0 || ins[1] >= O1.nrops)
+            throw std::invalid_argument("bUpdate: instruction out of range");
+        const QN key = (QN)O0.hss_of_ops[ins[0]] + (QN)O1.hss_of_ops[ins[1]] * H +
+                       (QN)in.hss_of_new[ins[2]] * H * H;
+        auto it = std::find_if(combos.begin(), combos.end(), [&](const auto& c) { return c.first == key; });
+        if (it == combos.end()) {
+            combos.push_back({key, {}});
+            it = combos.end() - 1;
+        }
+        it->second.push_back({run_first[u], u});
+    }
+    std::sort(combos.begin(), combos.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+
+    // tensor blocks grouped by the two kets the loop fixes (make_qntens, 304-383)
+    const QN divide = looptype ? (QN)nsec[0] * nsec[1] : (QN)nsec[0];
+    std::vector<std::pair<QN, std::vector<int>>> tgroups;
+    for (int b = 0; b < T.nrblocks; ++b) {
+        const QN key = looptype ? T.qnumbers[b] % divide : T.qnumbers[b] / divide;
+        auto it = std::find_if(tgroups.begin(), tgroups.end(), [&](const auto& g) { return g.first == key; });
+        if (it == tgroups.end()) {
+            tgroups.push_back({key, {}});
+            it = tgroups.end() - 1;
+        }
+        it->second.push_back(b);
+    }
+    std::sort(tgroups.begin(), tgroups.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+
+    const ROperators* ops[2] = {&O0, &O1};
+    auto ids_of = [&](QN qn, int bond, int* bra, int* ket, int* h) {
+        const QN n = nsec[bond];
+        *bra = (int)(qn % n);
+        *ket = (int)((qn / n) % n);
+        *h = (int)(qn / (n * n));
+    };
+    const int first = looptype ? 0 : 1, second = 1 - first;
+    const int N = in.begin_blocks_of_hss[in.nrhss];
+    std::vector<std::vector<BUpdateTask>> per_dest((size_t)nu * N);
+    for (int new_sb = 0; new_sb < N; ++new_sb) {
+        int bra[3], ket[3], hss[3];
+        ids_of(in.qnumbers[new_sb], uc, &bra[uc], &ket[uc], &hss[2]);
+        const int hn = hss[2];
+        const int nsb = new_sb - in.begin_blocks_of_hss[hn];
+        for (int p = in.tprod_begin[hn]; p < in.tprod_begin[hn + 1]; ++p) {
+            const int hss_ops[2] = {in.tprods[2 * p], in.tprods[2 * p + 1]};
+            const ROperators& F = *ops[first];
+            const ROperators& S = *ops[second];
+            const int bf = id_ops[first], bs = id_ops[second];
+            const int nbf = F.nblocks_hss(hss_ops[first]);
+            const QN* qf = F.qnumbers_hss(hss_ops[first]);
+            for (int jf = 0; jf < nbf; ++jf) {
+                int hf;
+                ids_of(qf[jf], bf, &bra[bf], &ket[bf], &hf);
+                const QN match = looptype ? (QN)ket[0] + (QN)ket[1] * nsec[0]
+                                          : (QN)ket[1] + (QN)ket[2] * nsec[1];
+                const auto tg = std::lower_bound(tgroups.begin(), tgroups.end(), match,
+                                                 [](const auto& g, QN v) { return g.first < v; });
+                if (tg == tgroups.end() || tg->first != match) continue;
+                for (int tb : tg->second) {
+                    ket[bs] = looptype ? (int)(T.qnumbers[tb] / divide) : (int)(T.qnumbers[tb] % divide);
+                    const int nbs = S.nblocks_hss(hss_ops[second]);
+                    const QN* qs = S.qnumbers_hss(hss_ops[second]);
+                    for (int js = 0; js < nbs; ++js) {
+                        int ks, hs;
+                        ids_of(qs[js], bs, &bra[bs], &ks, &hs);
+                        if (ks != ket[bs]) continue;
+                        const QN aqn = (QN)bra[0] + (QN)bra[1] * nsec[0] + (QN)bra[2] * nsec[0] * nsec[1];
+                        const int ab = find_qn(T.qnumbers.data(), T.nrblocks, aqn);
+                        if (ab < 0) continue;
+                        int sbop[2];
+                        sbop[first] = jf;
+                        sbop[second] = js;
+                        // prefactor over (bra, ket, MPO) irreps of the three bonds
+                        const int hb[3] = {hss_ops[0], hss_ops[1], hn};
+                        int hsite[3];
+                        for (int r = 0; r < 3; ++r) hsite[id_ops[r]] = hb[r];
+                        const int* irr[3][3];
+                        for (int i = 0; i < 3; ++i) {
+                            irr[i][0] = in.site_ss[i].irreps[bra[i]].data();
+                            irr[i][1] = in.site_ss[i].irreps[ket[i]].data();
+                            irr[i][2] = in.mpo.irreps[hsite[i]].data();
+                        }
+                        const double pref = prefactor_bUpdate(irr, uc, in.sym);
+                        // an exactly zero prefactor adds nothing
+                        if (pref == 0.0) continue;
+                        const QN key = (QN)hss_ops[0] + (QN)hss_ops[1] * H + (QN)hn * H * H;
+                        const auto cb = std::lower_bound(combos.begin(), combos.end(), key,
+                                                         [](const auto& c, QN v) { return c.first < v; });
+                        if (cb == combos.end() || cb->first != key) continue;
+                        const int64_t tsz = T.block_size(tb), asz = T.block_size(ab);
+                        if (tsz == 0 || asz == 0) continue;
+                        for (const auto& iu : cb->second) {
+                            const int* ins = in.instr[iu[0]].instr;
+                            const int u = iu[1];
+                            if (O0.block_size(ins[0], sbop[0]) == 0 || O1.block_size(ins[1], sbop[1]) == 0 ||
+                                U.block_size(u, nsb) == 0)
+                                continue;
+                            BUpdateTask t;
+                            t.t = T.beginblock[tb];
+                            t.a = T.beginblock[ab];
+                            t.o1 = O0.op_offset[ins[0]] + O0.beginblock[ins[0]][sbop[0]];
+                            t.o2 = O1.op_offset[ins[1]] + O1.beginblock[ins[1]][sbop[1]];
+                            for (int i = 0; i < 3; ++i) {
+                                t.d[i] = in.site_ss[i].dims[ket[i]];
+                                t.e[i] = in.site_ss[i].dims[bra[i]];
+                            }
+                            t.pref = pref;
+                            per_dest[(size_t)u * N + new_sb].push_back(t);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    for (int u = 0; u < nu; ++u) {
+        const int h = U.hss_of_ops[u];
+        for (int b = 0; b < U.nblocks_hss(h); ++b) {
+            auto& v = per_dest[(size_t)u * N + in.begin_blocks_of_hss[h] + b];
+            if (v.empty()) continue;
+            BUpdateDest dd;
+            dd.dst = U.op_offset[u] + U.beginblock[u][b];
+            dd.M = v[0].e[uc];
+            dd.N = v[0].d[uc];
+            dd.task_begin = (int)P.tasks.size();
+            P.tasks.insert(P.tasks.end(), v.begin(), v.end());
+            dd.task_end = (int)P.tasks.size();
+            P.dests.push_back(dd);
+        }
+    }
+
+    // sum_unique_rOperators
+    ROperators& nr = P.nr;
+    nr.bond = -1;
+    nr.is_left = U.is_left;
+    nr.P_operator = 0;
+    nr.nrhss = in.nrhss;
+    nr.begin_blocks_of_hss = in.begin_blocks_of_hss;
+    nr.qnumbers = in.qnumbers;
+    int nops = 0;
+    for (const auto& i : in.instr) nops = std::max(nops, i.instr[2] + 1);
+    nr.nrops = nops;
+    nr.hss_of_ops.assign(in.hss_of_new.begin(), in.hss_of_new.begin() + nops);
+    nr.beginblock.resize(nops);
+    for (int k = 0; k < nops; ++k) nr.beginblock[k] = in.hss_bb[nr.hss_of_ops[k]];
+    nr.finalize_offsets();
+    std::vector<std::vector<int>> instr_of(nops);
+    for (int i = 0; i < ni; ++i) instr_of[in.instr[i].instr[2]].push_back(i);
+    for (int k = 0; k < nops; ++k) {
+        const int nb = nr.nblocks_op(k);
+        for (int b = 0; b < nb; ++b) {
+            AppendGroup g;
+            g.dst = nr.op_offset[k] + nr.beginblock[k][b];
+            g.n = nr.block_size(k, b);
+            g.op = k;
+            g.block = b;
+            g.task_begin = (int)P.stasks.size();
+            for (int i : instr_of[k]) {
+                const int u = run_of[i];
+                P.stasks.push_back(AppendTask{U.op_offset[u] + U.beginblock[u][b], 1.0, in.instr[i].pref});
+            }
+            g.task_end = (int)P.stasks.size();
+            if (g.task_end > g.task_begin && g.n > 0) P.groups.push_back(g);
+            else P.stasks.resize(g.task_begin);
+        }
+    }
+    return P;
+}
+
+}  // namespace t3

@@ -82,6 +82,47 @@ struct PUpdatePlan {
 };
 PUpdatePlan plan_pupdate(const PUpdateInput& in);
 
+// Everything update_rOperators_branching reads.
+struct BUpdateInput {
+    SymInfo sym;
+    int update_case = -1;      // bond of the branching site carrying the new operators
+    Symsecs site_ss[3], mpo;   // the site's bonds (bra = ket sectors) and the MPO
+    const SiteTensor* T = nullptr;
+    std::vector<Instruction> instr;  // sorted bUpdate instructions
+    std::vector<int> hss_of_new;
+    std::vector<int> tprod_begin, tprods;  // per new MPO sector: (hss of op 0, hss of op 1) pairs
+    int nrhss = 0;
+    std::vector<int> begin_blocks_of_hss;
+    std::vector<QN> qnumbers;
+    std::vector<std::vector<int64_t>> hss_bb;
+    const ROperators* ops[2] = {nullptr, nullptr};
+};
+// One contribution U[dst] += pref * A . (O1 x O2) . T of one unique operator
+// block: A and T are the adjoint and ket blocks of the site tensor, O1 and O2
+// blocks of the two old operator sets.  d: ket dims, e: bra dims per bond.
+struct BUpdateTask {
+    int64_t t, a, o1, o2;  // offsets: tensor, tensor, ops[0] arena, ops[1] arena
+    int d[3], e[3];
+    double pref;
+};
+// All contributions of one unique-operator block, in the reference's order.
+struct BUpdateDest {
+    int64_t dst;  // offset in the unique-operator arena
+    int M, N;     // bra x ket dims of the block
+    int task_begin, task_end;
+};
+struct BUpdatePlan {
+    int n_leg = -1, x_leg = -1, y_leg = -1;  // bonds of the new operators, ops[0], ops[1]
+    ROperators uniq;  // unique operators (all blocks present)
+    std::vector<BUpdateDest> dests;
+    std::vector<BUpdateTask> tasks;
+    // sum_unique_rOperators: new operator blocks from unique blocks
+    ROperators nr;
+    std::vector<AppendGroup> groups;
+    std::vector<AppendTask> stasks;  // src: offset in the unique arena
+};
+BUpdatePlan plan_bupdate(const BUpdateInput& in);
+
 // Drops the blocks flagged empty: returns the compacted per-operator block
 // offsets and, for every kept block, (old offset, new offset, size).
 struct Compaction {

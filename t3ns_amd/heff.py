@@ -11,8 +11,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (FP32, FP64, AppendDesc, HeffDesc, MakesiteDesc, PUpdateDesc, RopsDesc, SiteDesc,
-                   Work, check, lib)
+from ._lib import (FP32, FP64, AppendDesc, BUpdateDesc, HeffDesc, MakesiteDesc, PUpdateDesc, RopsDesc,
+                   SiteDesc, Work, check, lib)
 
 
 class Context:
@@ -70,6 +70,14 @@ class Context:
         ns = (C.c_int * max(n, 1))()
         lib().t3ns_ctx_sweep_steps(self._h, int(maxsites), sites, ns, n)
         return [list(sites[4 * i:4 * i + ns[i]]) for i in range(n)]
+
+    def bupdate_problem(self, site, update_case, seed=0):
+        """(update descriptor, old operators [2]) of update_rOperators_branching
+        at branching site `site` for new operators on its bond `update_case`."""
+        d, o = BUpdateDesc(), (RopsDesc * 2)()
+        check(lib().t3ns_ctx_bupdate_desc(self._h, int(site), int(update_case), int(seed),
+                                          C.byref(d), o), "t3ns_ctx_bupdate_desc")
+        return d, o
 
     def close(self):
         if self._h:
@@ -205,6 +213,14 @@ def update_physical(desc, pops):
     h = C.c_void_p()
     check(lib().t3ns_update_physical(C.byref(desc), C.byref(pops), C.byref(h)),
           "t3ns_update_physical")
+    return ROps(h)
+
+
+def update_branching(desc, ops):
+    """update_rOperators_branching (src/rOperators_bUpdate.c:229-248) on the
+    device; `ops` is the array of the two old operator descriptors."""
+    h = C.c_void_p()
+    check(lib().t3ns_update_branching(C.byref(desc), ops, C.byref(h)), "t3ns_update_branching")
     return ROps(h)
 
 
