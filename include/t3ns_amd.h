@@ -213,6 +213,14 @@ typedef struct {
 /* Renormalised operators owned by the library (host copy). */
 typedef struct t3ns_rops t3ns_rops;
 
+/* Work of one device execution (renormalisation, site-tensor operations). */
+typedef struct {
+    double flops;      /* algorithmic flops of the device kernels */
+    double bytes;      /* inputs read once + output written once */
+    double kernel_ms;  /* elapsed time of the device kernels */
+} t3ns_work;
+
+
 /* rOperators_append_phys on the device: *out receives the new operators
  * (all-zero blocks dropped as kick_zero_blocks does, src/sparseblocks.c:73-107). */
 int t3ns_append_phys(const t3ns_append_desc* desc, const t3ns_rops_desc* old, t3ns_rops** out);
@@ -300,6 +308,59 @@ int t3ns_update_branching(const t3ns_bupdate_desc* desc, const t3ns_rops_desc* o
                           t3ns_rops** out);
 
 /* ------------------------------------------------------------------------
+ * decompose_siteTensor, one split (src/siteTensor_decompose.c:1403-1444,
+ * split_of_site, with init_svddata 587-887, svdblocks 1050-1072, selectS
+ * 1196-1289, SVD_copy_from_mem 989-1048, adapt_UV_tensors_and_kick_empties
+ * 1360-1401, norm_tensor src/siteTensor_misc.c:244-250): the site `siteV`
+ * of a multisite tensor A is split off by a truncated SVD per sector of the
+ * bond it shares with the rest.  U (the remaining sites, singular values
+ * multiplied in, normalised) and V (the site) come back with the cut bond's
+ * kept sectors.  Repeating the split gives the reference's HOSVD
+ * (1533-1585).  The SVDs run on the device (one-sided Jacobi); singular
+ * vectors agree with LAPACK's up to the sign of each pair.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int nrSyms;
+    int sgs[T3NS_MAX_SYMMETRIES];
+    int nrsites;                         /* 2..4 */
+    int siteV;                           /* index of the site split off */
+    t3ns_symsecs_desc ss[4][3];          /* sectors of every site's bonds (bond ids set) */
+    int nrblocks;
+    const int64_t* qnumbers;             /* [nrblocks * nrsites] */
+    const int64_t* beginblock;           /* [nrblocks + 1] */
+    const double* tel;                   /* A, normalised */
+    int minD, maxD;                      /* struct SvalSelect, include/siteTensor.h:425-431 */
+    double truncerr;
+    char truncType;                      /* 'E' (entropy) or 'N' (norm) */
+} t3ns_split_desc;
+
+typedef struct t3ns_split t3ns_split;
+typedef struct {
+    int U_nrsites, U_nrblocks;
+    const int64_t* U_qnumbers;           /* [U_nrblocks * U_nrsites] */
+    const int64_t* U_beginblock;
+    const double* U_tel;
+    int V_nrblocks;
+    const int64_t* V_qnumbers;
+    const int64_t* V_beginblock;
+    const double* V_tel;
+    int bond_nrSecs;                     /* the cut bond's kept sectors */
+    const int* bond_irreps;              /* [bond_nrSecs][T3NS_MAX_SYMMETRIES] */
+    const int* bond_dims;
+    int nrsectors;                       /* sectors before truncation */
+    const int* nsing;                    /* [nrsectors] singular values per sector */
+    const int* kept;                     /* [nrsectors] kept per sector */
+    const double* sing;                  /* all singular values, sector after sector, descending */
+    double norm[2], entropy[2];          /* before / after truncation (struct SelectRes) */
+    double norm_U;                       /* the norm divided out of U */
+    int sweeps;                          /* Jacobi sweeps */
+} t3ns_split_view;
+
+int t3ns_split_site(const t3ns_split_desc* desc, t3ns_split** out, t3ns_work* work);
+int t3ns_split_get(const t3ns_split* s, t3ns_split_view* view);
+int t3ns_split_destroy(t3ns_split* s);
+
+/* ------------------------------------------------------------------------
  * makesiteTensor (src/siteTensor_init.c:785-800 with init_makeinfo 609-649,
  * init_contractinfo 651-760, contractsiteTensors 768-783): the multisite
  * tensor's elements from two to four one-site tensors, contracted over the
@@ -308,12 +369,6 @@ int t3ns_update_branching(const t3ns_bupdate_desc* desc, const t3ns_rops_desc* o
  * zero.  The one-site case of the reference is a pointer copy, not a
  * computation, and is not offered here.
  * ---------------------------------------------------------------------- */
-typedef struct {
-    double flops;      /* algorithmic flops of the device kernels */
-    double bytes;      /* inputs read once + output written once */
-    double kernel_ms;  /* elapsed time of the device kernels */
-} t3ns_work;
-
 typedef struct {
     int nrsites;                         /* 2..4 */
     int innersite;                       /* the branching site; the second of two physical sites */

@@ -81,6 +81,10 @@ int t3ns_step_makesite_desc(t3ns_step* s, uint64_t seed, t3ns_makesite_desc* d);
  * live as long as the context. */
 int t3ns_ctx_bupdate_desc(t3ns_ctx* c, int site, int update_case, uint64_t seed,
                           t3ns_bupdate_desc* d, t3ns_rops_desc* ops /* [2] */);
+/* A split of the step's multisite tensor (its N(0,1) elements normalised):
+ * site index `siteV` is split off with the given truncation settings. */
+int t3ns_step_split_desc(t3ns_step* s, int siteV, int minD, int maxD, double truncerr,
+                         char truncType, t3ns_split_desc* d);
 int t3ns_step_destroy(t3ns_step* s);
 int64_t t3ns_step_size(const t3ns_step* s);
 /* Fills the Heff boundary descriptors; pointers stay valid while s lives. */

@@ -24,7 +24,7 @@ def lib():
         if not os.path.exists(LIBPATH):
             raise RuntimeError("oracle not built: run oracle/build.py")
         from t3ns_amd._lib import (AppendDesc, BUpdateDesc, HeffDesc, MakesiteDesc, PUpdateDesc,
-                                   RopsDesc, SiteDesc)
+                                   RopsDesc, SiteDesc, SplitDesc, SplitView)
         L = C.CDLL(LIBPATH)
         P = C.c_void_p
         dp = C.POINTER(C.c_double)
@@ -60,6 +60,8 @@ def lib():
             "oracle_update_branching": (C.c_int64, [C.POINTER(BUpdateDesc), C.POINTER(RopsDesc),
                                                     i64, dp]),
             "oracle_set_bupdate_order": (None, [C.c_int]),
+            "oracle_split_site": (C.c_int, [C.POINTER(SplitDesc), C.POINTER(SplitView)]),
+            "oracle_split_free": (None, [C.POINTER(SplitView)]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -225,3 +227,15 @@ def update_branching(desc, ops, order=-1):
         pb += nb + 1
         pt += int(b[-1])
     return out
+
+
+def split_site(desc):
+    """Oracle split_of_site: the result dict of t3ns_amd.split_site."""
+    from t3ns_amd._lib import SplitView, split_view_dict
+    v = SplitView()
+    if lib().oracle_split_site(C.byref(desc), C.byref(v)) != 0:
+        raise RuntimeError("oracle_split_site: site not cut by one bond")
+    try:
+        return split_view_dict(v)
+    finally:
+        lib().oracle_split_free(C.byref(v))

@@ -97,6 +97,12 @@ int64_t oracle_update_branching(const t3ns_bupdate_desc* d, const t3ns_rops_desc
  * reference's choice of the cheapest. */
 void oracle_set_bupdate_order(int way);
 
+/* split_of_site (src/siteTensor_decompose.c:1403-1444) on the descriptor
+ * of include/t3ns_amd.h; arrays of `out` are allocated here and released by
+ * oracle_split_free.  Returns 0, or -1 when the site is not cut by one bond. */
+int oracle_split_site(const t3ns_split_desc* d, t3ns_split_view* out);
+void oracle_split_free(t3ns_split_view* v);
+
 /* makesiteTensor's contraction (src/siteTensor_init.c:609-800) on the
  * descriptor of include/t3ns_amd.h; tel: the multisite elements (zeroed
  * here first).  Returns 0, or -1 when an outer site is not bound to the

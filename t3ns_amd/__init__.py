@@ -4,6 +4,6 @@ operator renormalisation for block-sparse three-legged tree tensor networks
 from ._lib import FP32, FP64, lib  # noqa: F401
 from .heff import (Context, Heff, ROps, Step, append_phys, davidson_device,  # noqa: F401
                    davidson_refined, make_site_tensor, plan_block_dims, plan_table,
-                   update_branching, update_physical)
+                   split_site, update_branching, update_physical)
 
 __all__ = ["Context", "Step", "Heff", "plan_table", "plan_block_dims", "FP64", "FP32", "lib"]

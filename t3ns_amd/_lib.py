@@ -6,6 +6,8 @@ loudly when it is missing: there is no Python fallback for any kernel.
 import ctypes as C
 import os
 
+import numpy as np
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libt3ns_amd.so")
 
@@ -76,6 +78,24 @@ class BUpdateDesc(C.Structure):
                 ("hss_beginblock", C.POINTER(c_i64_p))]
 
 
+class SplitDesc(C.Structure):
+    _fields_ = [("nrSyms", C.c_int), ("sgs", C.c_int * MAX_SYMMETRIES), ("nrsites", C.c_int),
+                ("siteV", C.c_int), ("ss", (SymsecsDesc * 3) * 4), ("nrblocks", C.c_int),
+                ("qnumbers", c_i64_p), ("beginblock", c_i64_p), ("tel", c_dbl_p),
+                ("minD", C.c_int), ("maxD", C.c_int), ("truncerr", C.c_double),
+                ("truncType", C.c_char)]
+
+
+class SplitView(C.Structure):
+    _fields_ = [("U_nrsites", C.c_int), ("U_nrblocks", C.c_int), ("U_qnumbers", c_i64_p),
+                ("U_beginblock", c_i64_p), ("U_tel", c_dbl_p), ("V_nrblocks", C.c_int),
+                ("V_qnumbers", c_i64_p), ("V_beginblock", c_i64_p), ("V_tel", c_dbl_p),
+                ("bond_nrSecs", C.c_int), ("bond_irreps", c_int_p), ("bond_dims", c_int_p),
+                ("nrsectors", C.c_int), ("nsing", c_int_p), ("kept", c_int_p), ("sing", c_dbl_p),
+                ("norm", C.c_double * 2), ("entropy", C.c_double * 2), ("norm_U", C.c_double),
+                ("sweeps", C.c_int)]
+
+
 class Work(C.Structure):
     _fields_ = [("flops", C.c_double), ("bytes", C.c_double), ("kernel_ms", C.c_double)]
 
@@ -120,6 +140,11 @@ def lib():
                                             C.POINTER(P)]),
         "t3ns_ctx_bupdate_desc": (C.c_int, [P, C.c_int, C.c_int, C.c_uint64, C.POINTER(BUpdateDesc),
                                             C.POINTER(RopsDesc)]),
+        "t3ns_split_site": (C.c_int, [C.POINTER(SplitDesc), C.POINTER(P), C.POINTER(Work)]),
+        "t3ns_split_get": (C.c_int, [P, C.POINTER(SplitView)]),
+        "t3ns_split_destroy": (C.c_int, [P]),
+        "t3ns_step_split_desc": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_char,
+                                           C.POINTER(SplitDesc)]),
         "t3ns_make_site_tensor": (C.c_int, [C.POINTER(MakesiteDesc), c_dbl_p, C.POINTER(Work)]),
         "t3ns_step_makesite_desc": (C.c_int, [P, C.c_uint64, C.POINTER(MakesiteDesc)]),
         "t3ns_step_pupdate_desc": (C.c_int, [P, C.c_int, C.c_uint64, C.c_int,
@@ -181,3 +206,30 @@ def exported_symbols():
             txt = open(os.path.join(inc, f)).read()
             names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*(t3ns_[a-z0-9_]+)\s*\(", txt, re.M)
     return sorted(set(names))
+
+
+def split_view_dict(v):
+    """The arrays of a split result view as numpy copies."""
+    def arr(p, n, dt):
+        return np.ctypeslib.as_array(p, shape=(max(n, 1),))[:n].astype(dt).copy() if n else np.zeros(0, dt)
+    nU = int(v.U_beginblock[v.U_nrblocks]) if v.U_nrblocks else 0
+    nV = int(v.V_beginblock[v.V_nrblocks]) if v.V_nrblocks else 0
+    ns = arr(v.nsing, v.nrsectors, np.int32)
+    return {
+        "U_nrsites": v.U_nrsites,
+        "U_qnumbers": arr(v.U_qnumbers, v.U_nrblocks * v.U_nrsites, np.int64),
+        "U_beginblock": arr(v.U_beginblock, v.U_nrblocks + 1, np.int64),
+        "U_tel": arr(v.U_tel, nU, np.float64),
+        "V_qnumbers": arr(v.V_qnumbers, v.V_nrblocks, np.int64),
+        "V_beginblock": arr(v.V_beginblock, v.V_nrblocks + 1, np.int64),
+        "V_tel": arr(v.V_tel, nV, np.float64),
+        "bond_irreps": arr(v.bond_irreps, v.bond_nrSecs * MAX_SYMMETRIES, np.int32).reshape(-1, MAX_SYMMETRIES),
+        "bond_dims": arr(v.bond_dims, v.bond_nrSecs, np.int32),
+        "nsing": ns,
+        "kept": arr(v.kept, v.nrsectors, np.int32),
+        "sing": arr(v.sing, int(ns.sum()), np.float64),
+        "norm": (v.norm[0], v.norm[1]),
+        "entropy": (v.entropy[0], v.entropy[1]),
+        "norm_U": v.norm_U,
+        "sweeps": v.sweeps,
+    }

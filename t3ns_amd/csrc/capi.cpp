@@ -10,6 +10,7 @@
 #include "../../include/t3ns_amd.h"
 #include "../../include/t3ns_host.h"
 #include "dev/davidson.hpp"
+#include "dev/decompose_dev.hpp"
 #include "dev/heff_dev.hpp"
 #include "dev/sitetensor_dev.hpp"
 #include "dev/renorm_dev.hpp"
@@ -519,6 +520,82 @@ int t3ns_update_branching(const t3ns_bupdate_desc* d, const t3ns_rops_desc* ops,
         res->views();
         *out = res.release();
     });
+}
+
+struct t3ns_split {
+    SplitResult r;
+    std::vector<int> irreps, nsing;
+    std::vector<double> sing;
+};
+
+int t3ns_split_site(const t3ns_split_desc* d, t3ns_split** out, t3ns_work* work) {
+    if (!d || !out) return fail(T3NS_ERR_INVALID, "null argument");
+    return guarded([&] {
+        SplitInput in;
+        in.sym.nrSyms = d->nrSyms;
+        for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) in.sym.sgs[i] = d->sgs[i];
+        in.nrsites = d->nrsites;
+        in.siteV = d->siteV;
+        if (d->nrsites < 2 || d->nrsites > 4 || d->nrblocks < 0)
+            throw std::invalid_argument("split: needs 2-4 sites");
+        for (int i = 0; i < d->nrsites; ++i)
+            for (int k = 0; k < 3; ++k) in.ss[i][k] = symsecs_from(d->ss[i][k]);
+        in.qnumbers.assign(d->qnumbers, d->qnumbers + (size_t)d->nrblocks * d->nrsites);
+        in.beginblock.assign(d->beginblock, d->beginblock + d->nrblocks + 1);
+        in.sel.minD = d->minD;
+        in.sel.maxD = d->maxD;
+        in.sel.truncerr = d->truncerr;
+        in.sel.truncType = d->truncType;
+        const std::vector<double> A(d->tel, d->tel + in.beginblock[d->nrblocks]);
+        const SplitPlan plan = plan_split(in);
+        auto res = std::make_unique<t3ns_split>();
+        RenormWork w;
+        res->r = run_split(in, plan, A, nullptr, &w);
+        for (const auto& ir : res->r.bond.irreps) res->irreps.insert(res->irreps.end(), ir.begin(), ir.end());
+        for (const auto& v : res->r.sing) {
+            res->nsing.push_back((int)v.size());
+            res->sing.insert(res->sing.end(), v.begin(), v.end());
+        }
+        if (work) {
+            work->flops = w.flops;
+            work->bytes = w.bytes;
+            work->kernel_ms = w.kernel_ms;
+        }
+        *out = res.release();
+    });
+}
+
+int t3ns_split_get(const t3ns_split* s, t3ns_split_view* v) {
+    if (!s || !v) return fail(T3NS_ERR_INVALID, "null argument");
+    const SplitResult& r = s->r;
+    v->U_nrsites = r.U.nrsites;
+    v->U_nrblocks = r.U.nrblocks;
+    v->U_qnumbers = r.U.qnumbers.data();
+    v->U_beginblock = r.U.beginblock.data();
+    v->U_tel = r.U.tel.data();
+    v->V_nrblocks = r.V.nrblocks;
+    v->V_qnumbers = r.V.qnumbers.data();
+    v->V_beginblock = r.V.beginblock.data();
+    v->V_tel = r.V.tel.data();
+    v->bond_nrSecs = r.bond.nrSecs;
+    v->bond_irreps = s->irreps.data();
+    v->bond_dims = r.bond.dims.data();
+    v->nrsectors = (int)r.kept.size();
+    v->nsing = s->nsing.data();
+    v->kept = r.kept.data();
+    v->sing = s->sing.data();
+    for (int i = 0; i < 2; ++i) {
+        v->norm[i] = r.res.norm[i];
+        v->entropy[i] = r.res.entropy[i];
+    }
+    v->norm_U = r.norm_U;
+    v->sweeps = r.sweeps;
+    return T3NS_OK;
+}
+
+int t3ns_split_destroy(t3ns_split* s) {
+    delete s;
+    return T3NS_OK;
 }
 
 int t3ns_make_site_tensor(const t3ns_makesite_desc* d, double* tel, t3ns_work* work) {

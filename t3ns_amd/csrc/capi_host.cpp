@@ -147,8 +147,14 @@ struct MakesiteStore {
     bool built = false;
 };
 
+struct SplitStore {
+    SymsecsFlat ss[4][3];
+    std::vector<double> A;
+};
+
 struct t3ns_step {
     t3ns_ctx* ctx = nullptr;
+    SplitStore spl;
     AppendStore app[3];
     PUpdateStore pup[3];
     MakesiteStore mks;
@@ -763,6 +769,42 @@ int t3ns_ctx_bupdate_desc(t3ns_ctx* cx, int site, int update_case, uint64_t seed
             o.tel = u.tel_ptr[r].data();
         }
         cx->bups.push_back(std::move(st));
+    });
+}
+
+int t3ns_step_split_desc(t3ns_step* st, int siteV, int minD, int maxD, double truncerr,
+                         char truncType, t3ns_split_desc* d) {
+    if (!st || !d) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        Context& c = *st->ctx->c;
+        const SiteTensor& M = st->site;
+        if (M.nrsites < 2 || siteV < 0 || siteV >= M.nrsites)
+            throw std::invalid_argument("split: bad site");
+        SplitStore& sp = st->spl;
+        if (sp.A.empty()) {
+            sp.A = M.tel;
+            double n = 0;
+            for (double v : sp.A) n += v * v;
+            n = std::sqrt(n);
+            for (auto& v : sp.A) v /= n;
+            for (int i = 0; i < M.nrsites; ++i)
+                for (int k = 0; k < 3; ++k) sp.ss[i][k].set(st->md.ssarr[i][k]);
+        }
+        std::memset(d, 0, sizeof(*d));
+        d->nrSyms = c.sym.nrSyms;
+        for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) d->sgs[i] = c.sym.sgs[i];
+        d->nrsites = M.nrsites;
+        d->siteV = siteV;
+        for (int i = 0; i < M.nrsites; ++i)
+            for (int k = 0; k < 3; ++k) d->ss[i][k] = sp.ss[i][k].desc;
+        d->nrblocks = M.nrblocks;
+        d->qnumbers = M.qnumbers.data();
+        d->beginblock = M.beginblock.data();
+        d->tel = sp.A.data();
+        d->minD = minD;
+        d->maxD = maxD;
+        d->truncerr = truncerr;
+        d->truncType = truncType;
     });
 }
 

@@ -12,7 +12,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (FP32, FP64, AppendDesc, BUpdateDesc, HeffDesc, MakesiteDesc, PUpdateDesc, RopsDesc,
-                   SiteDesc, Work, check, lib)
+                   SiteDesc, SplitDesc, SplitView, Work, check, lib, split_view_dict)
 
 
 class Context:
@@ -127,6 +127,14 @@ class Step:
                                            C.byref(o)), "t3ns_step_pupdate_desc")
         return d, o
 
+    def split_problem(self, siteV, maxD, minD=0, truncerr=0.0, trunc_type="E"):
+        """decompose_siteTensor split of site index `siteV` off the step's
+        (normalised) multisite tensor."""
+        d = SplitDesc()
+        check(lib().t3ns_step_split_desc(self._h, int(siteV), int(minD), int(maxD), float(truncerr),
+                                         trunc_type.encode(), C.byref(d)), "t3ns_step_split_desc")
+        return d
+
     def makesite_problem(self, seed=0):
         """makesiteTensor descriptor: seeded one-site tensors of the step's
         sites and the step's multisite structure."""
@@ -222,6 +230,21 @@ def update_branching(desc, ops):
     h = C.c_void_p()
     check(lib().t3ns_update_branching(C.byref(desc), ops, C.byref(h)), "t3ns_update_branching")
     return ROps(h)
+
+
+def split_site(desc):
+    """One split of decompose_siteTensor (src/siteTensor_decompose.c:
+    1403-1444) on the device: (result dict, (flops, bytes, kernel ms))."""
+    h = C.c_void_p()
+    w = Work()
+    check(lib().t3ns_split_site(C.byref(desc), C.byref(h), C.byref(w)), "t3ns_split_site")
+    try:
+        v = SplitView()
+        check(lib().t3ns_split_get(h, C.byref(v)), "t3ns_split_get")
+        out = split_view_dict(v)
+    finally:
+        lib().t3ns_split_destroy(h)
+    return out, (w.flops, w.bytes, w.kernel_ms)
 
 
 def make_site_tensor(desc):
