@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the baseline sample (0 disables it)")
     ap.add_argument("--precisions", default="f64,f32")
+    ap.add_argument("--no-rows", action="store_true",
+                    help="skip the renormalisation and site-tensor kernels")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (scale reference for HBM counters)")
     return ap.parse_args()
@@ -99,6 +101,78 @@ def cpu_baseline(step, x, target_seconds, flops_total):
                       "every %d-th output block (%d of %d blocks, %.1f%% of the matvec flops) in "
                       "%.1f s; matvec/s = sample flop rate / full-matvec flops"
                       % (stride, len(range(0, nb, stride)), nb, 100.0 * f / flops_total, t)}
+
+
+HBM_GBS = 8000.0
+RIDGE = PEAK_TFLOPS["f64"] * 1e12 / (HBM_GBS * 1e9)  # FP64 flop per byte
+
+
+def _row(flops, nbytes, ms, extra=None):
+    """Roofline line of one kernel group: the bound is the FP64 matrix peak
+    when its flop/byte ratio is above the ridge point, HBM otherwise."""
+    s = max(ms, 1e-9) * 1e-3
+    tf, gbs = flops / s / 1e12, nbytes / s / 1e9
+    intensity = flops / max(nbytes, 1.0)
+    if intensity >= RIDGE:
+        r = {"bound": "mfma", "achieved": tf, "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
+             "frac": tf / PEAK_TFLOPS["f64"]}
+    else:
+        r = {"bound": "hbm", "achieved": gbs, "peak": HBM_GBS, "unit": "GB/s", "frac": gbs / HBM_GBS}
+    r.update({"kernel_ms": ms, "flops": flops, "bytes": nbytes})
+    if extra:
+        r.update(extra)
+    return r
+
+
+def measure_rows(step, sites, bond_dim):
+    """The renormalisation and site-tensor rows on the same object: every
+    call runs twice and the second is reported (kernel time from device
+    events; flops/bytes algorithmic)."""
+    import t3ns_amd as T
+    from t3ns_amd import synthetic
+    rows = {}
+    d, o, s = step.descriptors()
+    for leg in range(3):
+        if o[leg].nrhss <= 0 or not o[leg].P_operator:
+            continue
+        ad, old = step.append_problem(leg, seed=3)
+        for _ in range(2):
+            r = T.append_phys(ad, old)
+        f, b, ms = r.work()
+        rows["append_phys_leg%d" % leg] = _row(f, b, ms)
+        pd, pops = step.pupdate_problem(leg, seed=5, maxdim=0)
+        for _ in range(2):
+            r = T.update_physical(pd, pops)
+        f, b, ms = r.work()
+        rows["update_physical_leg%d" % leg] = _row(f, b, ms)
+    md = step.makesite_problem(seed=7)
+    for _ in range(2):
+        _, (f, b, ms) = T.make_site_tensor(md)
+    rows["makesite"] = _row(f, b, ms)
+    for sv in range(s.nrsites):
+        try:
+            sd = step.split_problem(sv, maxD=bond_dim)
+        except RuntimeError:
+            continue
+        try:
+            for _ in range(2):
+                res, (f, b, ms) = T.split_site(sd)
+        except RuntimeError:
+            continue  # the branching site is not cut off by one bond
+        rows["split_site%d" % sv] = _row(f, b, ms, {"sweeps": res["sweeps"],
+                                                    "singular_values": int(res["nsing"].sum())})
+    fci = synthetic.qc_workload(24, 24, seed=1, rank=48)
+    with open(NETWORK) as fh:
+        ctx = T.Context(fh.read(), "QC", [], ["U1", "SU2"], [24, 0], bond_dim, fcidump=fci)
+    branch = [s.sites[i] for i in range(s.nrsites) if not s.site_is_phys[i]]
+    for uc in range(3 if branch else 0):
+        bd, bo = ctx.bupdate_problem(branch[0], uc, seed=3)
+        for _ in range(2):
+            r = T.update_branching(bd, bo)
+        f, b, ms = r.work()
+        rows["update_branching_case%d" % uc] = _row(f, b, ms)
+    ctx.close()
+    return rows
 
 
 def main():
@@ -204,6 +278,10 @@ def main():
                         "speedup_vs_f64": results["f32"]["matvec_per_s"] / r["matvec_per_s"],
                         "rel_err_vs_f64": err,
                         "roofline_frac": results["f32"]["tflops"] / PEAK_TFLOPS["f32"]}
+    if not args.no_rows:
+        t0 = time.perf_counter()
+        out["rows"] = measure_rows(step, sites, args.bond_dim)
+        print("# rows measured in %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(step, x, args.cpu_seconds, flops)
     else:

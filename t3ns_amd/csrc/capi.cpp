@@ -416,7 +416,7 @@ int t3ns_append_phys(const t3ns_append_desc* d, const t3ns_rops_desc* old, t3ns_
         const AppendPlan plan = plan_append(in);
         auto res = std::make_unique<t3ns_rops>();
         hipStream_t s = nullptr;
-        res->r = run_append(plan, o, s);
+        res->r = run_append(plan, o, s, &res->work);
         res->views();
         *out = res.release();
     });

@@ -64,7 +64,8 @@ int grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(8192, n
 // then the blocks left all zero are dropped (src/sparseblocks.c:73-107) and
 // the result copied to the host.
 static ROperators finish_sum(const ROperators& nr, const std::vector<AppendGroup>& groups,
-                             const std::vector<AppendTask>& tasks, const double* d_src, hipStream_t s) {
+                             const std::vector<AppendTask>& tasks, const double* d_src, hipStream_t s,
+                             float* kernel_ms = nullptr) {
     const int64_t nnew = nr.total_size();
     // flat block index of every (op, block)
     std::vector<int> flat_of_op(nr.nrops + 1, 0);
@@ -94,14 +95,24 @@ static ROperators finish_sum(const ROperators& nr, const std::vector<AppendGroup
     dkeep.alloc(std::max(1, nflat));
     hipLaunchKernelGGL(fill_zero<unsigned char>, dim3(grid((nflat + 255) / 256)), dim3(256), 0, s,
                        dkeep.get(), (int64_t)nflat);
+    hipEvent_t e0, e1;
+    T3_HIP_CHECK(hipEventCreate(&e0));
+    T3_HIP_CHECK(hipEventCreate(&e1));
+    T3_HIP_CHECK(hipEventRecord(e0, s));
     if (!hg.empty())
         hipLaunchKernelGGL(append_kernel, dim3(grid((int64_t)hg.size())), dim3(kThreads), 0, s,
                            dg.get(), (int)hg.size(), dsrc.get(), dfac.get(), dpref.get(), d_src,
                            dnew.get(), dkeep.get());
+    T3_HIP_CHECK(hipEventRecord(e1, s));
     std::vector<unsigned char> keep(nflat);
     if (nflat)
         T3_HIP_CHECK(hipMemcpyAsync(keep.data(), dkeep.get(), nflat, hipMemcpyDeviceToHost, s));
     T3_HIP_CHECK(hipStreamSynchronize(s));
+    float ms = 0;
+    T3_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (kernel_ms) *kernel_ms = ms;
     const Compaction cpt = compact_blocks(nr, keep);
     ROperators out = nr;
     out.beginblock = cpt.beginblock;
@@ -124,11 +135,24 @@ static ROperators finish_sum(const ROperators& nr, const std::vector<AppendGroup
     return out;
 }
 
-ROperators run_append(const AppendPlan& plan, const ROperators& old, hipStream_t s) {
+ROperators run_append(const AppendPlan& plan, const ROperators& old, hipStream_t s, RenormWork* work) {
     DevBuf<double> dold;
     dold.upload(old.tel, s);
     if (old.tel.empty()) dold.alloc(1);
-    return finish_sum(plan.nr, plan.groups, plan.tasks, dold.get(), s);
+    float ms = 0;
+    ROperators out = finish_sum(plan.nr, plan.groups, plan.tasks, dold.get(), s, &ms);
+    if (work) {
+        // each task reads its source block once; every new block is written once
+        double flops = 0, rd = 0;
+        for (const AppendGroup& g : plan.groups) {
+            rd += (double)g.n * (g.task_end - g.task_begin);
+            flops += 3.0 * g.n * (g.task_end - g.task_begin);
+        }
+        work->flops = flops;
+        work->bytes = 8.0 * (rd + (double)plan.nr.total_size());
+        work->kernel_ms = ms;
+    }
+    return out;
 }
 
 }  // namespace t3

@@ -10,7 +10,9 @@ namespace t3 {
 // Runs an append plan on the device: the old operator arena (host data of
 // in.old) is uploaded, the new arena computed, all-zero blocks dropped
 // (src/sparseblocks.c:73-107), and the result returned with host data.
-ROperators run_append(const AppendPlan& plan, const ROperators& old, hipStream_t s);
+struct RenormWork;
+ROperators run_append(const AppendPlan& plan, const ROperators& old, hipStream_t s,
+                      RenormWork* work = nullptr);
 
 // Work of one device execution: algorithmic flops and bytes (inputs read
 // once, outputs written once) and the summed kernel time.
