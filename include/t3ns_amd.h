@@ -361,6 +361,48 @@ int t3ns_split_get(const t3ns_split* s, t3ns_split_view* view);
 int t3ns_split_destroy(t3ns_split* s);
 
 /* ------------------------------------------------------------------------
+ * decompose_siteTensor for a one-site object: qr_step (src/siteTensor_
+ * decompose.c:1587-1630 with qr 313-355, qrblocks 270-311, multiplyR
+ * 416-490, R_svd 550-574).  A is QR-decomposed over the bond it shares with
+ * the next orthogonality centre C; Q replaces A (the bond's dim per sector
+ * becomes min(rows, dim)) and R is multiplied into C.  Householder QR on the
+ * device with LAPACK's sign convention; the entropy of R's spectrum is
+ * reported as the reference does.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int nrSyms;
+    int sgs[T3NS_MAX_SYMMETRIES];
+    t3ns_symsecs_desc ssA[3], ssC[3];    /* bonds of A and C (bond ids set) */
+    int A_nrblocks;
+    const int64_t* A_qnumbers;
+    const int64_t* A_beginblock;
+    const double* A_tel;
+    int C_nrblocks;
+    const int64_t* C_qnumbers;
+    const int64_t* C_beginblock;
+    const double* C_tel;
+} t3ns_qr_desc;
+
+typedef struct t3ns_qr t3ns_qr;
+typedef struct {
+    int nrblocks_Q;                      /* blocks and quantum numbers of A */
+    const int64_t* Q_beginblock;
+    const double* Q_tel;
+    int nrblocks_B;                      /* blocks and quantum numbers of C */
+    const int64_t* B_beginblock;
+    const double* B_tel;
+    int nrsectors;                       /* sectors of the shared bond */
+    const int* dims;                     /* its new dim per sector */
+    const int* nsing;
+    const double* sing;                  /* R's singular values, sector after sector */
+    double entropy;
+} t3ns_qr_view;
+
+int t3ns_qr_step(const t3ns_qr_desc* desc, t3ns_qr** out, t3ns_work* work);
+int t3ns_qr_get(const t3ns_qr* q, t3ns_qr_view* view);
+int t3ns_qr_destroy(t3ns_qr* q);
+
+/* ------------------------------------------------------------------------
  * makesiteTensor (src/siteTensor_init.c:785-800 with init_makeinfo 609-649,
  * init_contractinfo 651-760, contractsiteTensors 768-783): the multisite
  * tensor's elements from two to four one-site tensors, contracted over the

@@ -85,6 +85,10 @@ int t3ns_ctx_bupdate_desc(t3ns_ctx* c, int site, int update_case, uint64_t seed,
  * site index `siteV` is split off with the given truncation settings. */
 int t3ns_step_split_desc(t3ns_step* s, int siteV, int minD, int maxD, double truncerr,
                          char truncType, t3ns_split_desc* d);
+/* A qr_step problem: seeded N(0,1) one-site tensors of sites siteA and
+ * siteC (neighbours) over the context's sectors.  The arrays live as long
+ * as the context. */
+int t3ns_ctx_qr_desc(t3ns_ctx* c, int siteA, int siteC, uint64_t seed, t3ns_qr_desc* d);
 int t3ns_step_destroy(t3ns_step* s);
 int64_t t3ns_step_size(const t3ns_step* s);
 /* Fills the Heff boundary descriptors; pointers stay valid while s lives. */

@@ -24,7 +24,7 @@ def lib():
         if not os.path.exists(LIBPATH):
             raise RuntimeError("oracle not built: run oracle/build.py")
         from t3ns_amd._lib import (AppendDesc, BUpdateDesc, HeffDesc, MakesiteDesc, PUpdateDesc,
-                                   RopsDesc, SiteDesc, SplitDesc, SplitView)
+                                   QRDesc, QRView, RopsDesc, SiteDesc, SplitDesc, SplitView)
         L = C.CDLL(LIBPATH)
         P = C.c_void_p
         dp = C.POINTER(C.c_double)
@@ -62,6 +62,8 @@ def lib():
             "oracle_set_bupdate_order": (None, [C.c_int]),
             "oracle_split_site": (C.c_int, [C.POINTER(SplitDesc), C.POINTER(SplitView)]),
             "oracle_split_free": (None, [C.POINTER(SplitView)]),
+            "oracle_qr_step": (C.c_int, [C.POINTER(QRDesc), C.POINTER(QRView)]),
+            "oracle_qr_free": (None, [C.POINTER(QRView)]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -239,3 +241,15 @@ def split_site(desc):
         return split_view_dict(v)
     finally:
         lib().oracle_split_free(C.byref(v))
+
+
+def qr_step(desc):
+    """Oracle qr_step: the result dict of t3ns_amd.qr_step."""
+    from t3ns_amd._lib import QRView, qr_view_dict
+    v = QRView()
+    if lib().oracle_qr_step(C.byref(desc), C.byref(v)) != 0:
+        raise RuntimeError("oracle_qr_step: the tensors share no bond")
+    try:
+        return qr_view_dict(v)
+    finally:
+        lib().oracle_qr_free(C.byref(v))

@@ -103,6 +103,12 @@ void oracle_set_bupdate_order(int way);
 int oracle_split_site(const t3ns_split_desc* d, t3ns_split_view* out);
 void oracle_split_free(t3ns_split_view* v);
 
+/* qr_step (src/siteTensor_decompose.c:1587-1630); arrays of `out` are
+ * allocated here and released by oracle_qr_free.  Returns -1 when A and C
+ * share no bond. */
+int oracle_qr_step(const t3ns_qr_desc* d, t3ns_qr_view* out);
+void oracle_qr_free(t3ns_qr_view* v);
+
 /* makesiteTensor's contraction (src/siteTensor_init.c:609-800) on the
  * descriptor of include/t3ns_amd.h; tel: the multisite elements (zeroed
  * here first).  Returns 0, or -1 when an outer site is not bound to the

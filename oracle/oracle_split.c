@@ -1,6 +1,7 @@
 /* ORACLE -- test infrastructure, not product code.
  *
- * Plain-C restatement of split_of_site (src/siteTensor_decompose.c:
+ * Plain-C restatement of decompose_siteTensor's two paths.  First
+ * split_of_site (src/siteTensor_decompose.c:
  * 1403-1444): init_splitted_tens (633-660), make_r_count_svdinfos and
  * make_svdinfos (680-842), SVD_copy_to_mem (955-987), svdblocks (1050-1072;
  * LAPACK dgesdd replaced by a sequential one-sided Jacobi SVD written
@@ -464,6 +465,219 @@ void oracle_split_free(t3ns_split_view* v)
         free((void*) v->bond_dims);
         free((void*) v->nsing);
         free((void*) v->kept);
+        free((void*) v->sing);
+        memset(v, 0, sizeof(*v));
+}
+
+/* ------------------------------------------------------------------------
+ * qr_step (src/siteTensor_decompose.c:1587-1630): qr (313-355) with
+ * init_qrdata / makestart / make_Q (77-196), QR_copy_fromto_mem (214-253),
+ * qrblocks (255-311; dgeqrf and dorgqr restated below as unblocked
+ * Householder QR with LAPACK's reflector convention), multiplyR with makeB
+ * (377-490), R_svd (550-574) for the entropy.
+ * ---------------------------------------------------------------------- */
+
+/* dgeqr2 + dorg2r: A (m x n) -> R (k x n upper), Q (m x k), k = min(m, n) */
+static void householder(double* A, int m, int n, double* R, double* Q)
+{
+        const int k = m < n ? m : n;
+        double* tau = calloc(k + 1, sizeof(double));
+        for (int j = 0; j < k; ++j) {
+                double* col = A + (size_t) j * m;
+                double xn = 0;
+                for (int i = j + 1; i < m; ++i) xn += col[i] * col[i];
+                if (xn == 0) {
+                        tau[j] = 0;
+                } else {
+                        const double alpha = col[j];
+                        const double nrm = sqrt(alpha * alpha + xn);
+                        const double beta = alpha >= 0 ? -nrm : nrm;
+                        tau[j] = (beta - alpha) / beta;
+                        const double sc = 1 / (alpha - beta);
+                        for (int i = j + 1; i < m; ++i) col[i] *= sc;
+                        col[j] = beta;
+                }
+                for (int c = j + 1; c < n && tau[j] != 0; ++c) {
+                        double* x = A + (size_t) c * m;
+                        double w = x[j];
+                        for (int i = j + 1; i < m; ++i) w += col[i] * x[i];
+                        x[j] -= tau[j] * w;
+                        for (int i = j + 1; i < m; ++i) x[i] -= tau[j] * col[i] * w;
+                }
+        }
+        for (int j = 0; j < n; ++j)
+                for (int i = 0; i < k; ++i) R[i + (size_t) j * k] = i <= j ? A[i + (size_t) j * m] : 0;
+        for (int j = 0; j < k; ++j)
+                for (int i = 0; i < m; ++i) Q[i + (size_t) j * m] = i == j;
+        for (int jj = k - 1; jj >= 0; --jj) {
+                if (tau[jj] == 0) continue;
+                const double* v = A + (size_t) jj * m;
+                for (int c = jj; c < k; ++c) {
+                        double* x = Q + (size_t) c * m;
+                        double w = x[jj];
+                        for (int i = jj + 1; i < m; ++i) w += v[i] * x[i];
+                        x[jj] -= tau[jj] * w;
+                        for (int i = jj + 1; i < m; ++i) x[i] -= tau[jj] * v[i] * w;
+                }
+        }
+        free(tau);
+}
+
+int oracle_qr_step(const t3ns_qr_desc* d, t3ns_qr_view* out)
+{
+        int oc_id = -1, o_id = -1;
+        for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                        if (d->ssA[i].bond == d->ssC[j].bond) {
+                                oc_id = i;
+                                o_id = j;
+                        }
+        if (oc_id < 0) return -1;
+        const t3ns_symsecs_desc* ss = d->ssA;
+        const int nsec = ss[oc_id].nrSecs;
+        /* blocks of each sector in block order (stable sort of the bond index) */
+        int* sec_of = malloc(sizeof(int) * (d->A_nrblocks + 1));
+        int64_t* before = malloc(sizeof(int64_t) * (d->A_nrblocks + 1));
+        int64_t* after = malloc(sizeof(int64_t) * (d->A_nrblocks + 1));
+        int64_t* row0 = malloc(sizeof(int64_t) * (d->A_nrblocks + 1));
+        int64_t* M = calloc(nsec + 1, sizeof(int64_t));
+        for (int b = 0; b < d->A_nrblocks; ++b) {
+                int id[3];
+                idx3(id, d->A_qnumbers[b], ss);
+                sec_of[b] = id[oc_id];
+                before[b] = after[b] = 1;
+                for (int i = 0; i < oc_id; ++i) before[b] *= ss[i].dims[id[i]];
+                for (int i = oc_id + 1; i < 3; ++i) after[b] *= ss[i].dims[id[i]];
+                row0[b] = M[id[oc_id]];
+                if (d->A_beginblock[b + 1] > d->A_beginblock[b]) M[id[oc_id]] += before[b] * after[b];
+        }
+        int* K = calloc(nsec + 1, sizeof(int));
+        double** Rm = calloc(nsec + 1, sizeof(double*));
+        double** Qm = calloc(nsec + 1, sizeof(double*));
+        for (int s = 0; s < nsec; ++s) {
+                const int64_t N = M[s] ? ss[oc_id].dims[s] : 0;
+                K[s] = (int) (M[s] < N ? M[s] : N);
+                if (!K[s]) continue;
+                double* mat = calloc((size_t) M[s] * N + 1, sizeof(double));
+                for (int b = 0; b < d->A_nrblocks; ++b) {
+                        if (sec_of[b] != s || d->A_beginblock[b + 1] == d->A_beginblock[b]) continue;
+                        const double* src = d->A_tel + d->A_beginblock[b];
+                        for (int64_t o = 0; o < after[b]; ++o)
+                                for (int64_t n = 0; n < N; ++n)
+                                        for (int64_t m = 0; m < before[b]; ++m)
+                                                mat[row0[b] + m + o * before[b] + n * M[s]] =
+                                                        src[m + n * before[b] + o * before[b] * N];
+                }
+                Rm[s] = calloc((size_t) K[s] * N + 1, sizeof(double));
+                Qm[s] = calloc((size_t) M[s] * K[s] + 1, sizeof(double));
+                householder(mat, (int) M[s], (int) N, Rm[s], Qm[s]);
+                free(mat);
+        }
+        /* Q blocks (QR_copy_fromto_mem FROM_MEMORY) */
+        int64_t* Qbb = calloc((size_t) d->A_nrblocks + 2, sizeof(int64_t));
+        for (int b = 0; b < d->A_nrblocks; ++b) {
+                const int64_t sz = d->A_beginblock[b + 1] == d->A_beginblock[b] ? 0 : before[b] * K[sec_of[b]] * after[b];
+                Qbb[b + 1] = Qbb[b] + sz;
+        }
+        double* Qt = calloc((size_t) Qbb[d->A_nrblocks] + 1, sizeof(double));
+        for (int b = 0; b < d->A_nrblocks; ++b) {
+                const int s = sec_of[b];
+                if (Qbb[b + 1] == Qbb[b]) continue;
+                for (int64_t o = 0; o < after[b]; ++o)
+                        for (int64_t n = 0; n < K[s]; ++n)
+                                for (int64_t m = 0; m < before[b]; ++m)
+                                        Qt[Qbb[b] + m + n * before[b] + o * before[b] * K[s]] =
+                                                Qm[s][row0[b] + m + o * before[b] + n * M[s]];
+        }
+        /* multiplyR (bondR = 1): B = C contracted over its bond o_id with R */
+        const t3ns_symsecs_desc* sc = d->ssC;
+        int64_t* Bbb = calloc((size_t) d->C_nrblocks + 2, sizeof(int64_t));
+        for (int b = 0; b < d->C_nrblocks; ++b) {
+                int id[3];
+                idx3(id, d->C_qnumbers[b], sc);
+                const int64_t sz = d->C_beginblock[b + 1] - d->C_beginblock[b];
+                const int s = id[o_id];
+                Bbb[b + 1] = Bbb[b] + (K[s] == 0 || sz == 0 ? 0 : sz / sc[o_id].dims[s] * K[s]);
+        }
+        double* Bt = calloc((size_t) Bbb[d->C_nrblocks] + 1, sizeof(double));
+        for (int b = 0; b < d->C_nrblocks; ++b) {
+                if (Bbb[b + 1] == Bbb[b]) continue;
+                int id[3];
+                idx3(id, d->C_qnumbers[b], sc);
+                const int s = id[o_id];
+                const int N = sc[o_id].dims[s];
+                const double* A = d->C_tel + d->C_beginblock[b];
+                const double* R = Rm[s];
+                double* Bo = Bt + Bbb[b];
+                /* the reference's contractinfo (multiplyR 443-484) */
+                const int dim1A = sc[0].dims[id[0]] * (o_id == 1 ? 1 : sc[1].dims[id[1]]);
+                const int dim2A = sc[2].dims[id[2]] * (o_id == 1 ? 1 : sc[1].dims[id[1]]);
+                const int bondA = o_id, bondR = 1;
+                const int trB = !((bondR == 0) == (bondA != 0));
+                const int Mx = bondA == 0 ? K[s] : dim1A;
+                const int Nx = bondA == 0 ? dim2A : K[s];
+                const int Kx = N;
+                const int L = bondA == 1 ? dim2A : 1;
+                if (bondA == 0) {
+                        /* C = op(R) A */
+                        oracle_dgemm_batched(trB, 0, Mx, Nx, Kx, L, 1.0, R, trB ? Kx : Mx, 0, A, Kx,
+                                             (int64_t) Mx * Kx, 0.0, Bo, Mx, (int64_t) Mx * Nx);
+                } else {
+                        /* C = A op(R), batched over the last leg for the middle bond */
+                        oracle_dgemm_batched(0, trB, Mx, Nx, Kx, L, 1.0, A, Mx, (int64_t) Mx * Kx, R,
+                                             trB ? Nx : Kx, 0, 0.0, Bo, Mx, (int64_t) Mx * Nx);
+                }
+        }
+        /* R_svd and the entropy */
+        int* nsing = calloc(nsec + 1, sizeof(int));
+        int total = 0;
+        for (int s = 0; s < nsec; ++s) total += K[s];
+        double* sing = calloc(total + 1, sizeof(double));
+        double ent = 0;
+        int c = 0;
+        for (int s = 0; s < nsec; ++s) {
+                nsing[s] = K[s];
+                if (!K[s]) continue;
+                const int N = ss[oc_id].dims[s];
+                double* US = malloc(sizeof(double) * ((size_t) K[s] * K[s] + 1));
+                double* VT = malloc(sizeof(double) * ((size_t) K[s] * N + 1));
+                svd(Rm[s], K[s], N, sing + c, US, VT);
+                const int mult = multiplicity_of(ss[oc_id].irreps + (size_t) s * NS, d->sgs, d->nrSyms);
+                double e = 0;
+                for (int t = 0; t < K[s]; ++t) e += vn_entropy(sing[c + t] / sqrt((double) mult));
+                ent += mult * e;
+                c += K[s];
+                free(US);
+                free(VT);
+        }
+        memset(out, 0, sizeof(*out));
+        out->nrblocks_Q = d->A_nrblocks;
+        out->Q_beginblock = Qbb;
+        out->Q_tel = Qt;
+        out->nrblocks_B = d->C_nrblocks;
+        out->B_beginblock = Bbb;
+        out->B_tel = Bt;
+        out->nrsectors = nsec;
+        out->dims = K;
+        out->nsing = nsing;
+        out->sing = sing;
+        out->entropy = ent;
+        for (int s = 0; s < nsec; ++s) {
+                free(Rm[s]);
+                free(Qm[s]);
+        }
+        free(Rm); free(Qm); free(sec_of); free(before); free(after); free(row0); free(M);
+        return 0;
+}
+
+void oracle_qr_free(t3ns_qr_view* v)
+{
+        free((void*) v->Q_beginblock);
+        free((void*) v->Q_tel);
+        free((void*) v->B_beginblock);
+        free((void*) v->B_tel);
+        free((void*) v->dims);
+        free((void*) v->nsing);
         free((void*) v->sing);
         memset(v, 0, sizeof(*v));
 }

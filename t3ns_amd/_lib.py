@@ -96,6 +96,33 @@ class SplitView(C.Structure):
                 ("sweeps", C.c_int)]
 
 
+class QRDesc(C.Structure):
+    _fields_ = [("nrSyms", C.c_int), ("sgs", C.c_int * MAX_SYMMETRIES), ("ssA", SymsecsDesc * 3),
+                ("ssC", SymsecsDesc * 3), ("A_nrblocks", C.c_int), ("A_qnumbers", c_i64_p),
+                ("A_beginblock", c_i64_p), ("A_tel", c_dbl_p), ("C_nrblocks", C.c_int),
+                ("C_qnumbers", c_i64_p), ("C_beginblock", c_i64_p), ("C_tel", c_dbl_p)]
+
+
+class QRView(C.Structure):
+    _fields_ = [("nrblocks_Q", C.c_int), ("Q_beginblock", c_i64_p), ("Q_tel", c_dbl_p),
+                ("nrblocks_B", C.c_int), ("B_beginblock", c_i64_p), ("B_tel", c_dbl_p),
+                ("nrsectors", C.c_int), ("dims", c_int_p), ("nsing", c_int_p), ("sing", c_dbl_p),
+                ("entropy", C.c_double)]
+
+
+def qr_view_dict(v):
+    """The arrays of a qr_step result view as numpy copies."""
+    def arr(p, n, dt):
+        return np.ctypeslib.as_array(p, shape=(max(n, 1),))[:n].astype(dt).copy() if n else np.zeros(0, dt)
+    qbb = arr(v.Q_beginblock, v.nrblocks_Q + 1, np.int64)
+    bbb = arr(v.B_beginblock, v.nrblocks_B + 1, np.int64)
+    ns = arr(v.nsing, v.nrsectors, np.int32)
+    return {"Q_beginblock": qbb, "Q_tel": arr(v.Q_tel, int(qbb[-1]), np.float64),
+            "B_beginblock": bbb, "B_tel": arr(v.B_tel, int(bbb[-1]), np.float64),
+            "dims": arr(v.dims, v.nrsectors, np.int32), "nsing": ns,
+            "sing": arr(v.sing, int(ns.sum()), np.float64), "entropy": v.entropy}
+
+
 class Work(C.Structure):
     _fields_ = [("flops", C.c_double), ("bytes", C.c_double), ("kernel_ms", C.c_double)]
 
@@ -145,6 +172,10 @@ def lib():
         "t3ns_split_destroy": (C.c_int, [P]),
         "t3ns_step_split_desc": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_char,
                                            C.POINTER(SplitDesc)]),
+        "t3ns_qr_step": (C.c_int, [C.POINTER(QRDesc), C.POINTER(P), C.POINTER(Work)]),
+        "t3ns_qr_get": (C.c_int, [P, C.POINTER(QRView)]),
+        "t3ns_qr_destroy": (C.c_int, [P]),
+        "t3ns_ctx_qr_desc": (C.c_int, [P, C.c_int, C.c_int, C.c_uint64, C.POINTER(QRDesc)]),
         "t3ns_make_site_tensor": (C.c_int, [C.POINTER(MakesiteDesc), c_dbl_p, C.POINTER(Work)]),
         "t3ns_step_makesite_desc": (C.c_int, [P, C.c_uint64, C.POINTER(MakesiteDesc)]),
         "t3ns_step_pupdate_desc": (C.c_int, [P, C.c_int, C.c_uint64, C.c_int,

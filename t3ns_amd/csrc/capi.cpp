@@ -598,6 +598,78 @@ int t3ns_split_destroy(t3ns_split* s) {
     return T3NS_OK;
 }
 
+struct t3ns_qr {
+    QRResult r;
+    std::vector<int> nsing;
+    std::vector<double> sing;
+};
+
+namespace {
+SiteTensor site_from(int nrblocks, const int64_t* qn, const int64_t* bb, const double* tel) {
+    if (nrblocks < 0 || (nrblocks && (!qn || !bb || !tel))) throw std::invalid_argument("bad site tensor");
+    SiteTensor t;
+    t.nrsites = 1;
+    t.nrblocks = nrblocks;
+    t.qnumbers.assign(qn, qn + nrblocks);
+    t.beginblock.assign(1, 0);
+    if (nrblocks) t.beginblock.assign(bb, bb + nrblocks + 1);
+    t.tel.assign(tel, tel + t.size());
+    return t;
+}
+}  // namespace
+
+int t3ns_qr_step(const t3ns_qr_desc* d, t3ns_qr** out, t3ns_work* work) {
+    if (!d || !out) return fail(T3NS_ERR_INVALID, "null argument");
+    return guarded([&] {
+        QRInput in;
+        in.sym.nrSyms = d->nrSyms;
+        for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) in.sym.sgs[i] = d->sgs[i];
+        for (int k = 0; k < 3; ++k) {
+            in.ssA[k] = symsecs_from(d->ssA[k]);
+            in.ssC[k] = symsecs_from(d->ssC[k]);
+        }
+        const SiteTensor A = site_from(d->A_nrblocks, d->A_qnumbers, d->A_beginblock, d->A_tel);
+        const SiteTensor C = site_from(d->C_nrblocks, d->C_qnumbers, d->C_beginblock, d->C_tel);
+        in.A = &A;
+        in.C = &C;
+        const QRPlan plan = plan_qr(in);
+        auto res = std::make_unique<t3ns_qr>();
+        RenormWork w;
+        res->r = run_qr(in, plan, nullptr, &w);
+        for (const auto& v : res->r.sing) {
+            res->nsing.push_back((int)v.size());
+            res->sing.insert(res->sing.end(), v.begin(), v.end());
+        }
+        if (work) {
+            work->flops = w.flops;
+            work->bytes = w.bytes;
+            work->kernel_ms = w.kernel_ms;
+        }
+        *out = res.release();
+    });
+}
+
+int t3ns_qr_get(const t3ns_qr* q, t3ns_qr_view* v) {
+    if (!q || !v) return fail(T3NS_ERR_INVALID, "null argument");
+    v->nrblocks_Q = q->r.Q.nrblocks;
+    v->Q_beginblock = q->r.Q.beginblock.data();
+    v->Q_tel = q->r.Q.tel.data();
+    v->nrblocks_B = q->r.B.nrblocks;
+    v->B_beginblock = q->r.B.beginblock.data();
+    v->B_tel = q->r.B.tel.data();
+    v->nrsectors = (int)q->r.dims.size();
+    v->dims = q->r.dims.data();
+    v->nsing = q->nsing.data();
+    v->sing = q->sing.data();
+    v->entropy = q->r.entropy;
+    return T3NS_OK;
+}
+
+int t3ns_qr_destroy(t3ns_qr* q) {
+    delete q;
+    return T3NS_OK;
+}
+
 int t3ns_make_site_tensor(const t3ns_makesite_desc* d, double* tel, t3ns_work* work) {
     if (!d || !tel) return fail(T3NS_ERR_INVALID, "null argument");
     return guarded([&] {

@@ -104,9 +104,15 @@ struct BUpdateStore {
     int update_case = -1, nrhss = 0;
 };
 
+struct QRStore {
+    SymsecsFlat ssA[3], ssC[3];
+    SiteTensor A, C;
+};
+
 struct t3ns_ctx {
     std::unique_ptr<Context> c;
     std::vector<std::unique_ptr<BUpdateStore>> bups;
+    std::vector<std::unique_ptr<QRStore>> qrs;
 };
 
 // An append problem of one physical leg of a step: the operators of the
@@ -805,6 +811,45 @@ int t3ns_step_split_desc(t3ns_step* st, int siteV, int minD, int maxD, double tr
         d->maxD = maxD;
         d->truncerr = truncerr;
         d->truncType = truncType;
+    });
+}
+
+int t3ns_ctx_qr_desc(t3ns_ctx* cx, int siteA, int siteC, uint64_t seed, t3ns_qr_desc* d) {
+    if (!cx || !d) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        Context& c = *cx->c;
+        if (siteA < 0 || siteA >= c.netw.sites || siteC < 0 || siteC >= c.netw.sites)
+            throw std::invalid_argument("qr: bad site");
+        auto st = std::make_unique<QRStore>();
+        QRStore& q = *st;
+        Rng rng(seed);
+        q.A = init_1siteTensor(c, siteA);
+        for (auto& v : q.A.tel) v = rng.normal();
+        q.C = init_1siteTensor(c, siteC);
+        for (auto& v : q.C.tel) v = rng.normal();
+        int ba[3], bc[3];
+        c.netw.bonds_of_site(siteA, ba);
+        c.netw.bonds_of_site(siteC, bc);
+        for (int k = 0; k < 3; ++k) {
+            q.ssA[k].set(c.symsecs(ba[k]));
+            q.ssC[k].set(c.symsecs(bc[k]));
+        }
+        std::memset(d, 0, sizeof(*d));
+        d->nrSyms = c.sym.nrSyms;
+        for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) d->sgs[i] = c.sym.sgs[i];
+        for (int k = 0; k < 3; ++k) {
+            d->ssA[k] = q.ssA[k].desc;
+            d->ssC[k] = q.ssC[k].desc;
+        }
+        d->A_nrblocks = q.A.nrblocks;
+        d->A_qnumbers = q.A.qnumbers.data();
+        d->A_beginblock = q.A.beginblock.data();
+        d->A_tel = q.A.tel.data();
+        d->C_nrblocks = q.C.nrblocks;
+        d->C_qnumbers = q.C.qnumbers.data();
+        d->C_beginblock = q.C.beginblock.data();
+        d->C_tel = q.C.tel.data();
+        cx->qrs.push_back(std::move(st));
     });
 }
 

@@ -1,5 +1,6 @@
 // SVD split on the device (src/siteTensor_decompose.c:587-1444).
 #include "decompose_dev.hpp"
+#include "tile_gemm.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -508,6 +509,397 @@ SplitResult run_split(const SplitInput& in, const SplitPlan& P, const std::vecto
     };
     reform(P.U_qn, P.U_nrsites, csiteU, in.ss[P.id_csite], P.id_cbond, ubb, hu, &R.U);
     reform(P.V_qn, 1, 0, sV, P.id_bond, vbb, hv, &R.V);
+    return R;
+}
+
+}  // namespace t3
+
+// ---------------------------------------------------------------------------
+// qr_step: Householder QR per sector (dgeqrf + dorgqr semantics) and R
+// multiplied into the centre.
+// ---------------------------------------------------------------------------
+namespace t3 {
+
+namespace {
+
+struct QRMat {
+    int64_t a, tau, q, r;  // offsets: matrix (m x n), tau (k), Q (m x k), R (k x n)
+    int m, n, k, pad;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// One workgroup per matrix: Householder vectors below the diagonal, beta on
+// it (LAPACK dlarfg convention: beta = -sign(alpha) * norm), then R (upper
+// triangle) and the explicit Q of the first k columns.
+__global__ void __launch_bounds__(256)
+    householder_qr_kernel(const QRMat* __restrict__ mats, int nm, double* arena) {
+    __shared__ double red[256];
+    __shared__ double sh_tau, sh_scale;
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    for (int mi = blockIdx.x; mi < nm; mi += gridDim.x) {
+        const QRMat Q = mats[mi];
+        double* A = arena + Q.a;
+        double* tau = arena + Q.tau;
+        const int m = Q.m, n = Q.n;
+        for (int k = 0; k < Q.k; ++k) {
+            double* col = A + (int64_t)k * m;
+            double x2 = 0;
+            for (int i = k + 1 + threadIdx.x; i < m; i += 256) x2 = fma(col[i], col[i], x2);
+            red[threadIdx.x] = x2;
+            __syncthreads();
+            for (int st = 128; st > 0; st >>= 1) {
+                if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) {
+                const double alpha = col[k], xn2 = red[0];
+                if (xn2 == 0) {
+                    sh_tau = 0;
+                    sh_scale = 0;
+                } else {
+                    const double nrm = sqrt(alpha * alpha + xn2);
+                    const double beta = alpha >= 0 ? -nrm : nrm;
+                    sh_tau = (beta - alpha) / beta;
+                    sh_scale = 1 / (alpha - beta);
+                    col[k] = beta;
+                }
+                tau[k] = sh_tau;
+            }
+            __syncthreads();
+            const double t = sh_tau, sc = sh_scale;
+            if (t != 0)
+                for (int i = k + 1 + threadIdx.x; i < m; i += 256) col[i] *= sc;
+            __syncthreads();
+            // apply H = I - tau v v^T (v_k = 1) to columns k+1..n-1, one wave per column
+            if (t != 0)
+                for (int j = k + 1 + wave; j < n; j += 4) {
+                    double* cj = A + (int64_t)j * m;
+                    double w = lane == 0 ? cj[k] : 0.0;
+                    for (int i = k + 1 + lane; i < m; i += 64) w = fma(col[i], cj[i], w);
+                    w = wave_sum(w);
+                    if (lane == 0) cj[k] -= t * w;
+                    for (int i = k + 1 + lane; i < m; i += 64) cj[i] -= t * col[i] * w;
+                }
+            __syncthreads();
+        }
+        // R: the upper triangle of the first k rows (copy_to_R)
+        double* R = arena + Q.r;
+        for (int64_t e = threadIdx.x; e < (int64_t)Q.k * n; e += 256) {
+            const int i = (int)(e % Q.k), j = (int)(e / Q.k);
+            R[e] = i <= j ? A[i + (int64_t)j * m] : 0.0;
+        }
+        // Q = H_0 ... H_{k-1} applied to the first k columns of the identity
+        double* q = arena + Q.q;
+        for (int64_t e = threadIdx.x; e < (int64_t)m * Q.k; e += 256) {
+            const int i = (int)(e % m), j = (int)(e / m);
+            q[e] = i == j ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        for (int kk = Q.k - 1; kk >= 0; --kk) {
+            const double t = tau[kk];
+            if (t == 0) continue;  // uniform
+            const double* v = A + (int64_t)kk * m;
+            for (int j = kk + wave; j < Q.k; j += 4) {
+                double* cj = q + (int64_t)j * m;
+                double w = lane == 0 ? cj[kk] : 0.0;
+                for (int i = kk + 1 + lane; i < m; i += 64) w = fma(v[i], cj[i], w);
+                w = wave_sum(w);
+                if (lane == 0) cj[kk] -= t * w;
+                for (int i = kk + 1 + lane; i < m; i += 64) cj[i] -= t * v[i] * w;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// A block (before, N, after) into rows row0 + b + a * before, columns 0..N-1
+// of its sector matrix, and back from Q (before, K, after).
+struct QRMove {
+    int64_t blk, mat, row0;
+    int64_t before, after;
+    int64_t ld, cols;  // matrix rows (leading dimension), columns moved
+};
+
+__global__ void qr_gather_kernel(const QRMove* __restrict__ mv, int n, const double* __restrict__ src,
+                                 double* arena) {
+    for (int b = blockIdx.x; b < n; b += gridDim.x) {
+        const QRMove M = mv[b];
+        const int64_t tot = M.before * M.cols * M.after;
+        for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
+            const int64_t x = e % M.before, c = (e / M.before) % M.cols, y = e / (M.before * M.cols);
+            arena[M.mat + M.row0 + x + y * M.before + c * M.ld] = src[M.blk + e];
+        }
+    }
+}
+
+__global__ void qr_scatter_kernel(const QRMove* __restrict__ mv, int n, const double* arena,
+                                  double* __restrict__ dst) {
+    for (int b = blockIdx.x; b < n; b += gridDim.x) {
+        const QRMove M = mv[b];
+        const int64_t tot = M.before * M.cols * M.after;
+        for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
+            const int64_t x = e % M.before, c = (e / M.before) % M.cols, y = e / (M.before * M.cols);
+            dst[M.blk + e] = arena[M.mat + M.row0 + x + y * M.before + c * M.ld];
+        }
+    }
+}
+
+void mode_tiles_qr(std::vector<tile::GemmTile>& out, int64_t x, const int64_t n[3], int leg, int64_t o,
+                   int e, int64_t y) {
+    tile::GemmTile g;
+    auto push = [&]() {
+        for (int a = 0; a < tile::tiles_of(g.M); ++a)
+            for (int b = 0; b < tile::tiles_of(g.N); ++b) {
+                g.ti = a;
+                g.tj = b;
+                out.push_back(g);
+            }
+    };
+    if (leg == 0) {
+        g.p = tile::Strided{o, 1, e};
+        g.q = tile::Strided{x, 1, (int)n[0]};
+        g.M = e;
+        g.N = (int)(n[1] * n[2]);
+        g.K = (int)n[0];
+        g.x_off = y;
+        g.ldx = e;
+        push();
+    } else if (leg == 2) {
+        g.p = tile::Strided{x, 1, (int)(n[0] * n[1])};
+        g.q = tile::Strided{o, e, 1};
+        g.M = (int)(n[0] * n[1]);
+        g.N = e;
+        g.K = (int)n[2];
+        g.x_off = y;
+        g.ldx = g.M;
+        push();
+    } else {
+        for (int64_t l = 0; l < n[2]; ++l) {
+            g.p = tile::Strided{x + l * n[0] * n[1], 1, (int)n[0]};
+            g.q = tile::Strided{o, e, 1};
+            g.M = (int)n[0];
+            g.N = e;
+            g.K = (int)n[1];
+            g.x_off = y + l * n[0] * e;
+            g.ldx = (int)n[0];
+            push();
+        }
+    }
+}
+
+}  // namespace
+
+QRResult run_qr(const QRInput& in, const QRPlan& P, hipStream_t s, RenormWork* work) {
+    const SiteTensor& A = *in.A;
+    const SiteTensor& C = *in.C;
+    const int nsec = (int)P.sec.size();
+    QRResult R;
+    R.dims.assign(nsec, 0);
+    // arena: [A | C | per sector: matrix, tau, Q, R | B]
+    const int64_t a_base = 0, c_base = A.size();
+    int64_t top = c_base + C.size();
+    std::vector<QRMat> mats;
+    std::vector<int> mat_of(nsec, -1);
+    double flops = 0;
+    for (int ss = 0; ss < nsec; ++ss) {
+        const QRSector& S = P.sec[ss];
+        R.dims[ss] = (int)S.K;
+        if (S.K == 0) continue;
+        QRMat q;
+        q.m = (int)S.M;
+        q.n = (int)S.N;
+        q.k = (int)S.K;
+        q.pad = 0;
+        q.a = top;
+        top += S.M * S.N;
+        q.tau = top;
+        top += S.K;
+        q.q = top;
+        top += S.M * S.K;
+        q.r = top;
+        top += S.K * S.N;
+        mat_of[ss] = (int)mats.size();
+        mats.push_back(q);
+        flops += 4.0 * S.M * S.N * S.K;  // factorisation and explicit Q
+    }
+    // Q blocks: (before, K, after); B blocks: the centre with N -> K
+    std::vector<QRMove> gat, sca;
+    R.Q.nrsites = 1;
+    R.Q.sites[0] = A.sites[0];
+    R.Q.nrblocks = A.nrblocks;
+    R.Q.qnumbers = A.qnumbers;
+    R.Q.beginblock.assign(A.nrblocks + 1, 0);
+    for (int b = 0; b < A.nrblocks; ++b) {
+        const QRPlan::ABlock& ab = P.ablocks[b];
+        const QRSector& S = P.sec[ab.sector];
+        const int64_t sz = A.block_size(b) == 0 ? 0 : ab.before * S.K * ab.after;
+        R.Q.beginblock[b + 1] = R.Q.beginblock[b] + sz;
+    }
+    for (int ss = 0; ss < nsec; ++ss) {
+        const QRSector& S = P.sec[ss];
+        if (mat_of[ss] < 0) continue;
+        const QRMat& q = mats[mat_of[ss]];
+        for (size_t i = 0; i < S.blocks.size(); ++i) {
+            const int b = S.blocks[i];
+            const QRPlan::ABlock& ab = P.ablocks[b];
+            gat.push_back(QRMove{a_base + A.beginblock[b], q.a, S.rowstart[i], ab.before, ab.after, S.M, S.N});
+            sca.push_back(QRMove{R.Q.beginblock[b], q.q, S.rowstart[i], ab.before, ab.after, S.M, S.K});
+        }
+    }
+    // B = C x_bond R
+    const int bc = P.bondC;
+    R.B.nrsites = 1;
+    R.B.sites[0] = C.sites[0];
+    R.B.nrblocks = C.nrblocks;
+    R.B.qnumbers = C.qnumbers;
+    R.B.beginblock.assign(C.nrblocks + 1, 0);
+    std::vector<tile::GemmTile> tiles;
+    const int64_t b_base = top;
+    for (int b = 0; b < C.nrblocks; ++b) {
+        int id[3];
+        const QN n0 = in.ssC[0].nrSecs, n1 = in.ssC[1].nrSecs;
+        id[0] = (int)(C.qnumbers[b] % n0);
+        id[1] = (int)((C.qnumbers[b] / n0) % n1);
+        id[2] = (int)(C.qnumbers[b] / (n0 * n1));
+        const int ss = id[bc];
+        const int64_t K = (ss < nsec) ? P.sec[ss].K : 0;
+        int64_t nd[3];
+        for (int i = 0; i < 3; ++i) nd[i] = in.ssC[i].dims[id[i]];
+        const int64_t sz = (C.block_size(b) == 0 || K == 0) ? 0 : C.block_size(b) / nd[bc] * K;
+        R.B.beginblock[b + 1] = R.B.beginblock[b] + sz;
+        if (sz == 0) continue;
+        if (nd[bc] != P.sec[ss].N) throw std::runtime_error("qr: centre and tensor disagree on the bond");
+        mode_tiles_qr(tiles, c_base + C.beginblock[b], nd, bc, mats[mat_of[ss]].r, (int)K,
+                      b_base + R.B.beginblock[b]);
+        flops += 2.0 * (double)C.block_size(b) * K;
+    }
+    top = b_base + R.B.beginblock[C.nrblocks];
+    DevBuf<double> arena(std::max<int64_t>(1, top));
+    if (A.size())
+        T3_HIP_CHECK(hipMemcpyAsync(arena.get() + a_base, A.tel.data(), A.size() * sizeof(double),
+                                    hipMemcpyHostToDevice, s));
+    if (C.size())
+        T3_HIP_CHECK(hipMemcpyAsync(arena.get() + c_base, C.tel.data(), C.size() * sizeof(double),
+                                    hipMemcpyHostToDevice, s));
+    DevBuf<QRMat> dm;
+    DevBuf<QRMove> dg, dsc;
+    DevBuf<tile::GemmTile> dt;
+    dm.upload(mats, s);
+    dg.upload(gat, s);
+    dsc.upload(sca, s);
+    dt.upload(tiles, s);
+    DevBuf<double> dQ(std::max<int64_t>(1, R.Q.size()));
+    // sector matrices start zeroed
+    for (const QRMat& q : mats)
+        hipLaunchKernelGGL(zero_kernel<double>, dim3(gridn(((int64_t)q.m * q.n + 255) / 256)), dim3(256), 0,
+                           s, arena.get() + q.a, (int64_t)q.m * q.n);
+    hipEvent_t e0, e1;
+    T3_HIP_CHECK(hipEventCreate(&e0));
+    T3_HIP_CHECK(hipEventCreate(&e1));
+    T3_HIP_CHECK(hipEventRecord(e0, s));
+    if (!gat.empty())
+        hipLaunchKernelGGL(qr_gather_kernel, dim3(gridn((int64_t)gat.size())), dim3(256), 0, s, dg.get(),
+                           (int)gat.size(), arena.get(), arena.get());
+    if (!mats.empty())
+        hipLaunchKernelGGL(householder_qr_kernel, dim3(gridn((int64_t)mats.size())), dim3(256), 0, s,
+                           dm.get(), (int)mats.size(), arena.get());
+    if (!sca.empty())
+        hipLaunchKernelGGL(qr_scatter_kernel, dim3(gridn((int64_t)sca.size())), dim3(256), 0, s, dsc.get(),
+                           (int)sca.size(), arena.get(), dQ.get());
+    if (!tiles.empty())
+        hipLaunchKernelGGL(tile::gemm_tiles_kernel, dim3(gridn((int64_t)tiles.size())), dim3(256), 0, s,
+                           dt.get(), (int)tiles.size(), arena.get());
+    T3_HIP_CHECK(hipEventRecord(e1, s));
+    R.Q.tel.assign(R.Q.size(), 0.0);
+    R.B.tel.assign(R.B.size(), 0.0);
+    if (R.Q.size())
+        T3_HIP_CHECK(hipMemcpyAsync(R.Q.tel.data(), dQ.get(), R.Q.size() * sizeof(double),
+                                    hipMemcpyDeviceToHost, s));
+    if (R.B.size())
+        T3_HIP_CHECK(hipMemcpyAsync(R.B.tel.data(), arena.get() + b_base, R.B.size() * sizeof(double),
+                                    hipMemcpyDeviceToHost, s));
+    T3_HIP_CHECK(hipStreamSynchronize(s));
+    float ms = 0;
+    T3_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+
+    // singular values of R (R_svd, 550-574): Jacobi on R^T (n x k, n >= k)
+    std::vector<JacobiMat> jm;
+    int64_t jtop = 0;
+    for (int ss = 0; ss < nsec; ++ss) {
+        if (mat_of[ss] < 0) continue;
+        const QRMat& q = mats[mat_of[ss]];
+        JacobiMat J;
+        J.m = q.n;
+        J.n = q.k;
+        J.b = jtop;
+        jtop += (int64_t)J.m * J.n;
+        J.v = jtop;
+        jtop += (int64_t)J.n * J.n;
+        jm.push_back(J);
+    }
+    DevBuf<double> jar(std::max<int64_t>(1, jtop));
+    // the R matrices are small (k x n per sector): transpose them on the host
+    std::vector<double> hj(std::max<int64_t>(1, jtop), 0.0);
+    {
+        size_t t = 0;
+        std::vector<double> hr;
+        for (int ss = 0; ss < nsec; ++ss) {
+            if (mat_of[ss] < 0) continue;
+            const QRMat& q = mats[mat_of[ss]];
+            const JacobiMat& J = jm[t++];
+            hr.resize((size_t)q.k * q.n);
+            T3_HIP_CHECK(hipMemcpyAsync(hr.data(), arena.get() + q.r, hr.size() * sizeof(double),
+                                        hipMemcpyDeviceToHost, s));
+            T3_HIP_CHECK(hipStreamSynchronize(s));
+            for (int i = 0; i < q.k; ++i)
+                for (int j = 0; j < q.n; ++j) hj[J.b + j + (int64_t)i * q.n] = hr[i + (int64_t)j * q.k];
+        }
+    }
+    if (jtop) T3_HIP_CHECK(hipMemcpyAsync(jar.get(), hj.data(), jtop * sizeof(double), hipMemcpyHostToDevice, s));
+    jacobi_svd(jar.get(), jm, s, 40, 1e-15);
+    std::vector<double> hs(std::max<int64_t>(1, jtop));
+    if (jtop) T3_HIP_CHECK(hipMemcpyAsync(hs.data(), jar.get(), jtop * sizeof(double), hipMemcpyDeviceToHost, s));
+    T3_HIP_CHECK(hipStreamSynchronize(s));
+    T3_HIP_CHECK(hipGetLastError());
+    R.sing.assign(nsec, {});
+    {
+        size_t t = 0;
+        for (int ss = 0; ss < nsec; ++ss) {
+            if (mat_of[ss] < 0) continue;
+            const JacobiMat& J = jm[t++];
+            std::vector<double> sv;
+            for (int j = 0; j < J.n; ++j) {
+                double a = 0;
+                for (int i = 0; i < J.m; ++i) a += hs[J.b + i + (int64_t)j * J.m] * hs[J.b + i + (int64_t)j * J.m];
+                sv.push_back(std::sqrt(a));
+            }
+            std::sort(sv.begin(), sv.end(), [](double x, double y) { return x > y; });
+            R.sing[ss] = sv;
+        }
+    }
+    // calculateCostFunction(S, VonNeumannEntropy, 'A') over the bond's sectors
+    R.entropy = 0;
+    for (int ss = 0; ss < nsec; ++ss) {
+        const int mult = multiplicity(in.sym, in.ssA[P.bondA].irreps[ss].data());
+        const double mf = 1 / std::sqrt((double)mult);
+        double e = 0;
+        for (double v : R.sing[ss]) {
+            const double w = (v * mf) * (v * mf);
+            e += w < 1e-20 ? 0 : -w * std::log(w);
+        }
+        R.entropy += mult * e;
+    }
+    if (work) {
+        work->flops = flops;
+        work->bytes = 8.0 * (double)(A.size() + C.size() + R.Q.size() + R.B.size());
+        work->kernel_ms = ms;
+    }
     return R;
 }
 

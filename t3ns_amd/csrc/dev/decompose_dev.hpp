@@ -28,6 +28,17 @@ struct SplitResult {
 SplitResult run_split(const SplitInput& in, const SplitPlan& plan, const std::vector<double>& A,
                       hipStream_t s, RenormWork* work);
 
+struct QRResult {
+    // Q: the one-site tensor with the bond's dims reduced to min(rows, dim)
+    // per sector; B: the centre with R multiplied in.  Same blocks and
+    // quantum numbers as the inputs (blocks of empty sectors have size 0).
+    SiteTensor Q, B;
+    std::vector<int> dims;                  // new dims of the bond per sector
+    std::vector<std::vector<double>> sing;  // singular values of R per sector
+    double entropy = 0;                     // von Neumann entropy of the R spectrum
+};
+QRResult run_qr(const QRInput& in, const QRPlan& plan, hipStream_t s, RenormWork* work);
+
 // One-sided Jacobi SVD of column-major matrices on the device (rows >=
 // columns per matrix).  B is overwritten by U * Sigma (columns unsorted),
 // V (n x n) receives the right singular vectors.  Returns the sweeps run.

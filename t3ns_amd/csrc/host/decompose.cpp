@@ -211,3 +211,47 @@ std::vector<int> select_singular_values(const std::vector<std::vector<double>>& 
 }
 
 }  // namespace t3
+
+namespace t3 {
+
+QRPlan plan_qr(const QRInput& in) {
+    QRPlan P;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (in.ssA[i].bond == in.ssC[j].bond) {
+                P.bondA = i;
+                P.bondC = j;
+            }
+    if (P.bondA < 0) throw std::invalid_argument("qr: the tensors share no bond");
+    const Symsecs* ss = in.ssA;
+    const int b = P.bondA;
+    const int nsec = ss[b].nrSecs;
+    P.sec.resize(nsec);
+    const SiteTensor& A = *in.A;
+    P.ablocks.resize(A.nrblocks);
+    // blocks grouped by their sector of the bond, in block order (a stable
+    // sort of the indices, sort_indices 38-46 with makestart 77-97)
+    for (int k = 0; k < A.nrblocks; ++k) {
+        int id[3];
+        indexize3(id, A.qnumbers[k], ss);
+        const int s = id[b];
+        QRSector& S = P.sec[s];
+        int64_t before = 1, after = 1;
+        for (int i = 0; i < b; ++i) before *= ss[i].dims[id[i]];
+        for (int i = b + 1; i < 3; ++i) after *= ss[i].dims[id[i]];
+        P.ablocks[k] = QRPlan::ABlock{s, before, after};
+        if (A.block_size(k) == 0) continue;
+        S.blocks.push_back(k);
+        S.rowstart.push_back(S.M);
+        S.M += before * after;
+    }
+    for (int s = 0; s < nsec; ++s) {
+        QRSector& S = P.sec[s];
+        S.N = S.blocks.empty() ? 0 : ss[b].dims[s];
+        if (S.N == 0) S.M = 0;
+        S.K = std::min(S.M, S.N);
+    }
+    return P;
+}
+
+}  // namespace t3

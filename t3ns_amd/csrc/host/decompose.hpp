@@ -68,4 +68,32 @@ std::vector<int> select_singular_values(const std::vector<std::vector<double>>& 
                                         const std::vector<int>& mult, const SvalSelect& sel,
                                         SelectRes* res);
 
+// qr_step (src/siteTensor_decompose.c:1587-1630): the one-site tensor A is
+// QR-decomposed over its bond to the next orthogonality centre (qr 313-355,
+// qrblocks 270-311: per sector of that bond, the blocks with that sector
+// stacked row-wise in block order) and R multiplied into the centre
+// (multiplyR 416-490).
+struct QRInput {
+    SymInfo sym;
+    Symsecs ssA[3], ssC[3];  // bonds of A and of the centre (bond ids set)
+    const SiteTensor* A = nullptr;
+    const SiteTensor* C = nullptr;
+};
+struct QRSector {
+    std::vector<int> blocks;         // A blocks with this sector, in block order
+    std::vector<int64_t> rowstart;   // row offset of each block
+    int64_t M = 0, N = 0, K = 0;     // rows, columns (the sector's dim), min(M, N)
+};
+struct QRPlan {
+    int bondA = -1, bondC = -1;  // the shared bond's position in A and in the centre
+    std::vector<QRSector> sec;
+    // per A block: sector, dims before / at / after the bond
+    struct ABlock {
+        int sector;
+        int64_t before, after;
+    };
+    std::vector<ABlock> ablocks;
+};
+QRPlan plan_qr(const QRInput& in);
+
 }  // namespace t3

@@ -11,8 +11,9 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (FP32, FP64, AppendDesc, BUpdateDesc, HeffDesc, MakesiteDesc, PUpdateDesc, RopsDesc,
-                   SiteDesc, SplitDesc, SplitView, Work, check, lib, split_view_dict)
+from ._lib import (FP32, FP64, AppendDesc, BUpdateDesc, HeffDesc, MakesiteDesc, PUpdateDesc, QRDesc,
+                   QRView, RopsDesc, SiteDesc, SplitDesc, SplitView, Work, check, lib, qr_view_dict,
+                   split_view_dict)
 
 
 class Context:
@@ -70,6 +71,14 @@ class Context:
         ns = (C.c_int * max(n, 1))()
         lib().t3ns_ctx_sweep_steps(self._h, int(maxsites), sites, ns, n)
         return [list(sites[4 * i:4 * i + ns[i]]) for i in range(n)]
+
+    def qr_problem(self, site_a, site_c, seed=0):
+        """qr_step descriptor: seeded one-site tensors of neighbouring sites
+        site_a (decomposed) and site_c (the next orthogonality centre)."""
+        d = QRDesc()
+        check(lib().t3ns_ctx_qr_desc(self._h, int(site_a), int(site_c), int(seed), C.byref(d)),
+              "t3ns_ctx_qr_desc")
+        return d
 
     def bupdate_problem(self, site, update_case, seed=0):
         """(update descriptor, old operators [2]) of update_rOperators_branching
@@ -244,6 +253,21 @@ def split_site(desc):
         out = split_view_dict(v)
     finally:
         lib().t3ns_split_destroy(h)
+    return out, (w.flops, w.bytes, w.kernel_ms)
+
+
+def qr_step(desc):
+    """qr_step (src/siteTensor_decompose.c:1587-1630) on the device:
+    (result dict, (flops, bytes, kernel ms))."""
+    h = C.c_void_p()
+    w = Work()
+    check(lib().t3ns_qr_step(C.byref(desc), C.byref(h), C.byref(w)), "t3ns_qr_step")
+    try:
+        v = QRView()
+        check(lib().t3ns_qr_get(h, C.byref(v)), "t3ns_qr_get")
+        out = qr_view_dict(v)
+    finally:
+        lib().t3ns_qr_destroy(h)
     return out, (w.flops, w.bytes, w.kernel_ms)
 
 
