@@ -124,54 +124,77 @@ def _row(flops, nbytes, ms, extra=None):
     return r
 
 
-def measure_rows(step, sites, bond_dim):
-    """The renormalisation and site-tensor rows on the same object: every
-    call runs twice and the second is reported (kernel time from device
-    events; flops/bytes algorithmic)."""
+def row_problems(step, bond_dim, sites):
+    """(name, device call -> (flops, bytes, kernel ms), oracle call) for the
+    renormalisation and site-tensor rows on one step object."""
+    import oracle as O
     import t3ns_amd as T
     from t3ns_amd import synthetic
-    rows = {}
+    out = []
     d, o, s = step.descriptors()
+
+    def dev_rops(fn, *a):
+        return lambda: fn(*a).work()
+
     for leg in range(3):
         if o[leg].nrhss <= 0 or not o[leg].P_operator:
             continue
         ad, old = step.append_problem(leg, seed=3)
-        for _ in range(2):
-            r = T.append_phys(ad, old)
-        f, b, ms = r.work()
-        rows["append_phys_leg%d" % leg] = _row(f, b, ms)
+        out.append(("append_phys_leg%d" % leg, dev_rops(T.append_phys, ad, old),
+                    lambda ad=ad, old=old: O.append_phys(ad, old)))
         pd, pops = step.pupdate_problem(leg, seed=5, maxdim=0)
-        for _ in range(2):
-            r = T.update_physical(pd, pops)
-        f, b, ms = r.work()
-        rows["update_physical_leg%d" % leg] = _row(f, b, ms)
+        out.append(("update_physical_leg%d" % leg, dev_rops(T.update_physical, pd, pops),
+                    lambda pd=pd, pops=pops: O.update_physical(pd, pops)))
     md = step.makesite_problem(seed=7)
-    for _ in range(2):
-        _, (f, b, ms) = T.make_site_tensor(md)
-    rows["makesite"] = _row(f, b, ms)
+    out.append(("makesite", lambda: T.make_site_tensor(md)[1], lambda: O.make_site_tensor(md)))
     for sv in range(s.nrsites):
+        sd = step.split_problem(sv, maxD=bond_dim)
         try:
-            sd = step.split_problem(sv, maxD=bond_dim)
-        except RuntimeError:
-            continue
-        try:
-            for _ in range(2):
-                res, (f, b, ms) = T.split_site(sd)
+            O.split_site(step.split_problem(sv, maxD=1))
         except RuntimeError:
             continue  # the branching site is not cut off by one bond
-        rows["split_site%d" % sv] = _row(f, b, ms, {"sweeps": res["sweeps"],
-                                                    "singular_values": int(res["nsing"].sum())})
+        out.append(("split_site%d" % sv, lambda sd=sd: T.split_site(sd)[1], lambda sd=sd: O.split_site(sd)))
     fci = synthetic.qc_workload(24, 24, seed=1, rank=48)
     with open(NETWORK) as fh:
-        ctx = T.Context(fh.read(), "QC", [], ["U1", "SU2"], [24, 0], bond_dim, fcidump=fci)
+        netw = fh.read()
+    ctx = T.Context(netw, "QC", [], ["U1", "SU2"], [24, 0], bond_dim, fcidump=fci)
     branch = [s.sites[i] for i in range(s.nrsites) if not s.site_is_phys[i]]
     for uc in range(3 if branch else 0):
         bd, bo = ctx.bupdate_problem(branch[0], uc, seed=3)
-        for _ in range(2):
-            r = T.update_branching(bd, bo)
-        f, b, ms = r.work()
-        rows["update_branching_case%d" % uc] = _row(f, b, ms)
+        out.append(("update_branching_case%d" % uc, dev_rops(T.update_branching, bd, bo),
+                    lambda bd=bd, bo=bo: O.update_branching(bd, bo)))
+    qd = ctx.qr_problem(sites[0], sites[1], seed=3)
+    out.append(("qr_step", lambda: T.qr_step(qd)[1], lambda: O.qr_step(qd)))
+    return out, ctx
+
+
+def measure_rows(step, sites, bond_dim, cpu_dim=128):
+    """The renormalisation and site-tensor rows on the benchmark object:
+    every device call runs twice and the second is reported (kernel time
+    from device events, algorithmic flops and bytes).  The CPU baseline of a
+    row is the FP64 oracle restatement (one thread) on the same row of the
+    same step at bond dimension `cpu_dim` (the full size would take minutes
+    per row on one core); its rate is flops per second of that instance."""
+    rows = {}
+    probs, ctx = row_problems(step, bond_dim, sites)
+    for name, dev, _ in probs:
+        dev()
+        f, b, ms = dev()
+        rows[name] = _row(f, b, ms)
     ctx.close()
+    _, small = build_step(sites, cpu_dim)
+    sprobs, sctx = row_problems(small, cpu_dim, sites)
+    for name, dev, ora in sprobs:
+        if name not in rows:
+            continue
+        f, _, _ = dev()
+        t0 = time.perf_counter()
+        ora()
+        t = time.perf_counter() - t0
+        rows[name]["cpu_baseline"] = {"value": f / max(t, 1e-9) / 1e9, "unit": "GFLOP/s",
+                                      "seconds": t, "flops": f, "cores": 1, "kind": "port",
+                                      "sample": "same row of the same step at D=%d" % cpu_dim}
+    sctx.close()
     return rows
 
 

@@ -453,6 +453,24 @@ struct KeyHash {
     }
 };
 
+// dst (n0, n2, n1) = src (n0, n1, n2) with legs 1 and 2 swapped
+struct Swap12 {
+    int64_t src, dst;
+    int n0, n1, n2, pad;
+};
+
+__global__ void __launch_bounds__(256)
+    swap12_kernel(const Swap12* __restrict__ items, int n, double* arena) {
+    for (int it = blockIdx.x; it < n; it += gridDim.x) {
+        const Swap12 w = items[it];
+        const int64_t tot = (int64_t)w.n0 * w.n1 * w.n2;
+        for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
+            const int64_t i0 = e % w.n0, i2 = (e / w.n0) % w.n2, i1 = e / ((int64_t)w.n0 * w.n2);
+            arena[w.dst + e] = arena[w.src + i0 + i1 * w.n0 + i2 * (int64_t)w.n0 * w.n1];
+        }
+    }
+}
+
 }  // namespace
 
 ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROperators& ops0,
@@ -466,6 +484,7 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
     constexpr int64_t kWork = (int64_t)1 << 28;
     struct Chunk {
         std::vector<GemmTile> p1, p2;
+        std::vector<Swap12> swaps;  // new operators on leg 1: K made contiguous
         std::vector<SumTile> sums;
         std::vector<SumTask> tasks;
     };
@@ -475,7 +494,7 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
     size_t di = 0;
     while (di < plan.dests.size()) {
         Chunk c;
-        std::unordered_map<std::array<int64_t, 3>, int64_t, KeyHash> w1, w2;
+        std::unordered_map<std::array<int64_t, 3>, int64_t, KeyHash> w1, w2, sw;
         int64_t wtop = 0;
         for (; di < plan.dests.size(); ++di) {
             const BUpdateDest& dd = plan.dests[di];
@@ -489,6 +508,7 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
                     s2 *= i == nl ? tk.d[i] : tk.e[i];
                 }
                 need += s1 + s2;
+                if (nl == 1) need += s2 + (int64_t)tk.e[0] * tk.e[1] * tk.e[2];
             }
             if (need > kWork) throw std::runtime_error("bUpdate: block needs more than 2 GiB of workspace");
             if (wtop + need > kWork && !c.sums.empty()) break;
@@ -551,12 +571,24 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
                     sk.K = e[1] * e[2];
                     c.tasks.push_back(sk);
                 } else {
-                    for (int g = 0; g < e[2]; ++g) {
-                        sk.r = Strided{tk.a + (int64_t)g * e[0] * e[1], e[0], 1};
-                        sk.s = Strided{w2off + (int64_t)g * e[0] * d[1], 1, e[0]};
-                        sk.K = e[0];
-                        c.tasks.push_back(sk);
-                    }
+                    // U = sum_g A_g^T W2_g over the last leg: with legs 1 and 2
+                    // of A and W2 swapped, one product with K = e0 * e2
+                    auto swapped = [&](int64_t src, int n1, int64_t key2) {
+                        const std::array<int64_t, 3> key = {src, key2, -2};
+                        auto it = sw.find(key);
+                        if (it != sw.end()) return it->second;
+                        const int64_t off = w_base + wtop;
+                        wtop += (int64_t)e[0] * n1 * e[2];
+                        c.swaps.push_back(Swap12{src, off, e[0], n1, e[2], 0});
+                        sw.emplace(key, off);
+                        return off;
+                    };
+                    const int64_t a2 = swapped(tk.a, e[1], -1);
+                    const int64_t w2s = swapped(w2off, d[1], -3);
+                    sk.r = Strided{a2, e[0] * e[2], 1};
+                    sk.s = Strided{w2s, 1, e[0] * e[2]};
+                    sk.K = e[0] * e[2];
+                    c.tasks.push_back(sk);
                 }
                 flops += 2.0 * (double)e[0] * e[1] * e[2] * d[nl];
             }
@@ -600,6 +632,11 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
         if (!c.p2.empty())
             hipLaunchKernelGGL(tile::gemm_tiles_kernel, dim3(grid((int64_t)c.p2.size())), dim3(256), 0,
                                s, d2.get(), (int)c.p2.size(), arena.get());
+        DevBuf<Swap12> dw;
+        dw.upload(c.swaps, s);
+        if (!c.swaps.empty())
+            hipLaunchKernelGGL(swap12_kernel, dim3(grid((int64_t)c.swaps.size())), dim3(256), 0, s,
+                               dw.get(), (int)c.swaps.size(), arena.get());
         if (!c.sums.empty())
             hipLaunchKernelGGL(pupdate_sum_kernel, dim3(grid((int64_t)c.sums.size())), dim3(256), 0, s,
                                ds.get(), (int)c.sums.size(), dt.get(), arena.get(), arena.get(),
