@@ -158,7 +158,7 @@ __device__ __forceinline__ void row_mac_n(const T* __restrict__ S, const T* __re
     if (g.K & 3) {  // K tail: lanes past K contribute zero (their loads are skipped)
         const bool kv = (kfull << 2) + kk < g.K;
         const T* spc = kv ? sp : S + L.soff;
-        const T* opc = kv ? op : O + L.ooff;
+        const T* opc = kv ? op : O + (L.ooff - kk * g.dn);  // row 0 stays inside the block
         T av = kv ? spc[0] : (T)0;
         if (SCALE) av *= scale;
 #pragma unroll
@@ -201,6 +201,118 @@ __device__ __forceinline__ void row_put(T* __restrict__ Z, const Geo& g, const R
     }
 }
 
+// Row blocking, the transpose of the scheme above: a wave task covers up to
+// kTF row tiles and one column tile, so one B-operand (operator) load feeds
+// kTF matrix-core steps.  Chosen when a geometry has many fibers and few
+// output columns, where column blocking would reload every operator column
+// tile once per row tile.
+constexpr int kTF = 4;
+struct RowBlk {
+    int soff[kTF];   // A operand offsets (clamped fibers) at this lane's first k
+    int ooff;        // B operand offset of the column tile at this lane's first k
+    int tf0, ntf, tn;
+};
+template <typename T>
+__device__ __forceinline__ RowBlk row_blk(const Geo& g, int tf0, int tn) {
+    RowBlk B;
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 15, kk = lane >> 4;
+    B.tf0 = tf0;
+    B.ntf = min(kTF, g.tf - tf0);
+    B.tn = tn;
+#pragma unroll
+    for (int r = 0; r < kTF; ++r) {
+        const int f = min((tf0 + r) * 16 + i, g.F - 1);
+        B.soff[r] = (f % g.lo) + (f / g.lo) * g.lo * g.K + kk * g.lo;
+    }
+    const int n = min(tn * 16 + i, g.dn - 1);
+    B.ooff = n + kk * g.dn;
+    return B;
+}
+
+// acc[r] += (scale S)[row tile tf0 + r, :] O[column tile, :]^T, r < NTF.
+template <typename T, int NTF, bool SCALE>
+__device__ __forceinline__ void blk_mac_n(const T* __restrict__ S, const T* __restrict__ O,
+                                          const Geo& g, const RowBlk& B, T scale,
+                                          typename Mfma<T>::V (&acc)[kTF]) {
+    const int kk = (threadIdx.x & 63) >> 4;
+    const int sst = 4 * g.lo, ost = 4 * g.dn;
+    const T* op = O + B.ooff;
+    int so = 0;
+    const int kfull = uni(g.K >> 2);
+    constexpr int KS = 4;
+    for (int s0 = 0; s0 < kfull; s0 += KS) {
+        const int ns = uni(min(KS, kfull - s0));
+        T bv[KS];
+        T av[KS][NTF];
+#pragma unroll
+        for (int u = 0; u < KS; ++u) {
+            if (u < ns) {
+                bv[u] = op[u * ost];
+#pragma unroll
+                for (int r = 0; r < NTF; ++r) av[u][r] = S[B.soff[r] + so + u * sst];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < KS; ++u) {
+            if (u < ns) {
+#pragma unroll
+                for (int r = 0; r < NTF; ++r) {
+                    T a = av[u][r];
+                    if (SCALE) a *= scale;
+                    acc[r] = Mfma<T>::run(a, bv[u], acc[r]);
+                }
+            }
+        }
+        so += ns * sst;
+        op += ns * ost;
+    }
+    if (g.K & 3) {  // K tail: lanes past K contribute zero
+        const bool kv = (kfull << 2) + kk < g.K;
+        const T bv = kv ? op[0] : O[B.ooff - kk * g.dn];  // row 0 stays inside the block
+#pragma unroll
+        for (int r = 0; r < NTF; ++r) {
+            T a = kv ? S[B.soff[r] + so] : (T)0;
+            if (SCALE) a *= scale;
+            acc[r] = Mfma<T>::run(a, bv, acc[r]);
+        }
+    }
+}
+
+template <typename T, bool SCALE>
+__device__ __forceinline__ void blk_mac(const T* __restrict__ S, const T* __restrict__ O,
+                                        const Geo& g, const RowBlk& B, T scale,
+                                        typename Mfma<T>::V (&acc)[kTF]) {
+    switch (__builtin_amdgcn_readfirstlane(B.ntf)) {
+        case 1: blk_mac_n<T, 1, SCALE>(S, O, g, B, scale, acc); break;
+        case 2: blk_mac_n<T, 2, SCALE>(S, O, g, B, scale, acc); break;
+        case 3: blk_mac_n<T, 3, SCALE>(S, O, g, B, scale, acc); break;
+        default: blk_mac_n<T, 4, SCALE>(S, O, g, B, scale, acc); break;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void blk_put(T* __restrict__ Z, const Geo& g, const RowBlk& B,
+                                        const typename Mfma<T>::V (&acc)[kTF], bool add) {
+    const int lane = threadIdx.x & 63;
+    const int n = B.tn * 16 + (lane & 15);
+    if (n >= g.dn) return;
+#pragma unroll
+    for (int r = 0; r < kTF; ++r) {
+        if (r < B.ntf) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int fr = (B.tf0 + r) * 16 + Mfma<T>::row(lane, v);
+                if (fr < g.F) {
+                    T* d = Z + (fr % g.lo) + (fr / g.lo) * g.lo * g.dn + n * g.lo;
+                    if (add) *d += acc[r][v];
+                    else *d = acc[r][v];
+                }
+            }
+        }
+    }
+}
+
 // Tasks of one stage: (row tile, column-tile group, item group).
 struct StageTasks {
     int tf, tng, G;  // row tiles, column-tile groups, item groups
@@ -214,6 +326,22 @@ __device__ __forceinline__ StageTasks stage_tasks(const Geo& g, int nw) {
     t.G = base >= nw ? 1 : nw / base;
     t.total = base * t.G;
     return t;
+}
+// Row-blocked tasks: (row-tile group, column tile, item group); tf counts
+// row-tile groups here and tng column tiles.
+__device__ __forceinline__ StageTasks stage_tasks_rb(const Geo& g, int nw) {
+    StageTasks t;
+    t.tf = (g.tf + kTF - 1) / kTF;
+    t.tng = (g.dn + 15) >> 4;
+    const int base = t.tf * t.tng;
+    t.G = base >= nw ? 1 : nw / base;
+    t.total = base * t.G;
+    return t;
+}
+// Row blocking pays when the output has few column tiles and several row
+// tiles (every operator tile is then loaded once per row-tile group).
+__device__ __forceinline__ bool use_rows(const Geo& g) {
+    return ((g.dn + 15) >> 4) <= 2 && g.tf >= 2;
 }
 
 // All chunks of one pair.  Stage A: first-stage outputs of the chunk's
@@ -255,9 +383,14 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
         cur[pos[b]] = bd[b];
     }
     const Geo gC = make_geo(cur[0], cur[1], cur[2], pos[c], bd[c]);
-    const StageTasks kA = stage_tasks(gA, NW);
-    const StageTasks kB = stage_tasks(gB, NW);
+    const bool rbA = use_rows(gA), rbB = use_rows(gB);
+    const StageTasks kA = rbA ? stage_tasks_rb(gA, NW) : stage_tasks(gA, NW);
+    const StageTasks kB = rbB ? stage_tasks_rb(gB, NW) : stage_tasks(gB, NW);
     StageTasks kC = stage_tasks(gC, NW);
+    // stage C goes row-blocked only when its tasks alone occupy every wave
+    // (no item groups, hence no partial-tile reduction)
+    const StageTasks kCr = stage_tasks_rb(gC, NW);
+    const bool rbC = use_rows(gC) && kCr.tf * kCr.tng >= NW;
     // stage C item groups are reduced through LDS: keep the partial tiles
     // within NW x 256 values
     const int ntnC = (gC.dn + 15) >> 4;
@@ -282,7 +415,21 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
         }
         if (threadIdx.x < nr) rec->opa[threadIdx.x] = terms[runs[uni(C.run_begin) + threadIdx.x]].op[a];
         __syncthreads();
-        {  // stage A: W1[j] = O_a[j] x_a X
+        if (rbA) {  // stage A, row-blocked
+            const int base = kA.tf * kA.tng;
+            for (int task = wave; task < kA.total; task += NW) {
+                const int r = task % base, grp = task / base;
+                const RowBlk L = row_blk<T>(gA, (r % kA.tf) * kTF, r / kA.tf);
+                for (int j = grp; j < nr; j += kA.G) {
+                    V acc[kTF];
+#pragma unroll
+                    for (int q = 0; q < kTF; ++q) acc[q] = V{0, 0, 0, 0};
+                    blk_mac<T, false>(X, opa + rec->opa[j], gA, L, (T)1, acc);
+                    blk_put<T>(W1 + j * w1, gA, L, acc, false);
+                }
+            }
+            __syncthreads();
+        } else {  // stage A: W1[j] = O_a[j] x_a X
             const int base = kA.tf * kA.tng;
             for (int task = wave; task < kA.total; task += NW) {
                 const int r = task % base, grp = task / base;
@@ -297,7 +444,22 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             }
             __syncthreads();
         }
-        if (!isdmrg) {  // stage B: W2[t] = alpha_t O_b[t] x_b W1[slot t]
+        if (!isdmrg && rbB) {  // stage B, row-blocked
+            const int base = kB.tf * kB.tng;
+            for (int task = wave; task < kB.total; task += NW) {
+                const int r = task % base, grp = task / base;
+                const RowBlk L = row_blk<T>(gB, (r % kB.tf) * kTF, r / kB.tf);
+                for (int t = grp; t < nt; t += kB.G) {
+                    V acc[kTF];
+#pragma unroll
+                    for (int q = 0; q < kTF; ++q) acc[q] = V{0, 0, 0, 0};
+                    blk_mac<T, true>(W1 + rec->slot[t] * w1, opbb + rec->opb[t], gB, L,
+                                     (T)rec->alpha[t], acc);
+                    blk_put<T>(W2 + t * w2, gB, L, acc, false);
+                }
+            }
+            __syncthreads();
+        } else if (!isdmrg) {  // stage B: W2[t] = alpha_t O_b[t] x_b W1[slot t]
             const int base = kB.tf * kB.tng;
             for (int task = wave; task < kB.total; task += NW) {
                 const int r = task % base, grp = task / base;
@@ -313,7 +475,24 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             }
             __syncthreads();
         }
-        {  // stage C: Y += sum_t O_c[t] x_c W2[t]  (DMRG: alpha_t O_c[t] x_c W1[slot t])
+        if (rbC) {  // stage C, row-blocked (every task owns its output tiles)
+            const int base = kCr.tf * kCr.tng;
+            for (int task = wave; task < base; task += NW) {
+                const RowBlk L = row_blk<T>(gC, (task % kCr.tf) * kTF, task / kCr.tf);
+                V acc[kTF];
+#pragma unroll
+                for (int q = 0; q < kTF; ++q) acc[q] = V{0, 0, 0, 0};
+                for (int t = 0; t < nt; ++t) {
+                    if (isdmrg)
+                        blk_mac<T, true>(W1 + rec->slot[t] * w1, opc + rec->opc[t], gC, L,
+                                         (T)rec->alpha[t], acc);
+                    else
+                        blk_mac<T, false>(W2 + t * w2, opc + rec->opc[t], gC, L, (T)1, acc);
+                }
+                blk_put<T>(Y, gC, L, acc, true);
+            }
+            __syncthreads();
+        } else {  // stage C: Y += sum_t O_c[t] x_c W2[t]  (DMRG: alpha_t O_c[t] x_c W1[slot t])
             const int base = kC.tf * kC.tng;
             for (int task = wave; task < kC.total; task += NW) {
                 const int r = task % base, grp = task / base;
