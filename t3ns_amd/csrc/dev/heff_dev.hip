@@ -52,6 +52,7 @@ struct Mfma<float> {
 // i_hi lo dn.  Tiles are 16 fibers x 16 columns.
 struct Geo {
     int lo, K, F, dn, tf, tiles, ksteps;
+    double inv_lo;  // 1 / lo, for fiber_hi
 };
 // Values every lane of a wave agrees on, kept in scalar registers so that
 // loop control stays scalar.
@@ -72,7 +73,23 @@ __device__ __forceinline__ Geo make_geo(int s0, int s1, int s2, int p, int dn) {
     g.tf = (g.F + 15) >> 4;
     g.tiles = g.tf * ((dn + 15) >> 4);
     g.ksteps = (g.K + 3) >> 2;
+    g.inv_lo = 1.0 / g.lo;
     return g;
+}
+
+// f / lo for 0 <= f < 2^31: a double-precision estimate (off by at most
+// one) corrected by its remainder, a handful of vector instructions where
+// an integer division takes several dozen.
+__device__ __forceinline__ int fiber_hi(const Geo& g, int f) {
+    int q = (int)((double)f * g.inv_lo);
+    const int r = f - q * g.lo;
+    q += (r >= g.lo) - (r < 0);
+    return q;
+}
+// Offset of fiber f in a block whose mode-p leg has dimension n:
+// i_lo + i_hi lo n = f + i_hi lo (n - 1).
+__device__ __forceinline__ int fiber_off(const Geo& g, int f, int n) {
+    return f + fiber_hi(g, f) * g.lo * (n - 1);
 }
 
 // Register blocking: a wave task covers one 16-fiber row tile and up to
@@ -96,7 +113,7 @@ __device__ __forceinline__ RowLane row_lane(const Geo& g, int tf, int tn0) {
     const int lane = threadIdx.x & 63;
     const int i = lane & 15, kk = lane >> 4;
     const int f = min(tf * 16 + i, g.F - 1);
-    L.soff = (f % g.lo) + (f / g.lo) * g.lo * g.K + kk * g.lo;
+    L.soff = fiber_off(g, f, g.K) + kk * g.lo;
     L.tn0 = tn0;
     const int ntn_all = (g.dn + 15) >> 4;
     L.ntn = min(kTN, ntn_all - tn0);
@@ -105,7 +122,7 @@ __device__ __forceinline__ RowLane row_lane(const Geo& g, int tf, int tn0) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int fr = tf * 16 + Mfma<T>::row(lane, v);
-        L.rowoff[v] = fr < g.F ? (fr % g.lo) + (fr / g.lo) * g.lo * g.dn : -1;
+        L.rowoff[v] = fr < g.F ? fiber_off(g, fr, g.dn) : -1;
     }
     return L;
 }
@@ -114,6 +131,32 @@ __device__ __forceinline__ RowLane row_lane(const Geo& g, int tf, int tn0) {
 // j < NTN.  Loads are unconditional (clamped indices stay inside the
 // blocks) so the compiler can keep several in flight; only the K tail is
 // masked.
+// One slab of at most NU k-steps (ns of them live, ns >= 1): loads first,
+// then the matrix-core steps.
+template <typename T, int NTN, int NU, bool SCALE>
+__device__ __forceinline__ void row_slab(const T* sp, const T* op, const int (&coff)[NTN], int sst,
+                                         int ost, int ns, T scale,
+                                         typename Mfma<T>::V (&acc)[kTN]) {
+    T av[NU];
+    T bv[NU][NTN];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        const int uu = min(u, ns - 1);
+        av[u] = sp[uu * sst];
+#pragma unroll
+        for (int j = 0; j < NTN; ++j) bv[u][j] = op[uu * ost + coff[j]];
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        if (u < ns) {
+            T a = av[u];
+            if (SCALE) a *= scale;
+#pragma unroll
+            for (int j = 0; j < NTN; ++j) acc[j] = Mfma<T>::run(a, bv[u][j], acc[j]);
+        }
+    }
+}
+
 template <typename T, int NTN, bool SCALE>
 __device__ __forceinline__ void row_mac_n(const T* __restrict__ S, const T* __restrict__ O,
                                           const Geo& g, const RowLane& L, T scale,
@@ -129,29 +172,19 @@ __device__ __forceinline__ void row_mac_n(const T* __restrict__ S, const T* __re
     for (int j = 0; j < NTN; ++j) coff[j] = min((L.tn0 + j) * 16 + i, g.dn - 1) - n0;
     const int kfull = uni(g.K >> 2);
     // Full k-steps in slabs of KS: all loads of a slab are issued before its
-    // matrix-core steps so their latencies overlap.
+    // matrix-core steps so their latencies overlap.  The last, short slab
+    // loads unconditionally (re-reading its last step): guarded loads make
+    // the compiler drain the memory counter before each step.
     constexpr int KS = 4;
-    for (int s0 = 0; s0 < kfull; s0 += KS) {
-        const int ns = uni(min(KS, kfull - s0));
-        T av[KS];
-        T bv[KS][NTN];
-#pragma unroll
-        for (int u = 0; u < KS; ++u) {
-            if (u < ns) {
-                av[u] = sp[u * sst];
-#pragma unroll
-                for (int j = 0; j < NTN; ++j) bv[u][j] = op[u * ost + coff[j]];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < KS; ++u) {
-            if (u < ns) {
-                T a = av[u];
-                if (SCALE) a *= scale;
-#pragma unroll
-                for (int j = 0; j < NTN; ++j) acc[j] = Mfma<T>::run(a, bv[u][j], acc[j]);
-            }
-        }
+    int s0 = 0;
+    for (; s0 + KS <= kfull; s0 += KS) {
+        row_slab<T, NTN, KS, SCALE>(sp, op, coff, sst, ost, KS, scale, acc);
+        sp += KS * sst;
+        op += KS * ost;
+    }
+    if (s0 < kfull) {
+        const int ns = uni(kfull - s0);
+        row_slab<T, NTN, KS - 1, SCALE>(sp, op, coff, sst, ost, ns, scale, acc);
         sp += ns * sst;
         op += ns * ost;
     }
@@ -223,11 +256,38 @@ __device__ __forceinline__ RowBlk row_blk(const Geo& g, int tf0, int tn) {
 #pragma unroll
     for (int r = 0; r < kTF; ++r) {
         const int f = min((tf0 + r) * 16 + i, g.F - 1);
-        B.soff[r] = (f % g.lo) + (f / g.lo) * g.lo * g.K + kk * g.lo;
+        B.soff[r] = fiber_off(g, f, g.K) + kk * g.lo;
     }
     const int n = min(tn * 16 + i, g.dn - 1);
     B.ooff = n + kk * g.dn;
     return B;
+}
+
+// One slab of at most NU k-steps (ns of them live, ns >= 1) of blk_mac_n.
+template <typename T, int NTF, int NU, bool SCALE>
+__device__ __forceinline__ void blk_slab(const T* S, const T* op, const RowBlk& B, int so, int sst,
+                                         int ost, int ns, T scale,
+                                         typename Mfma<T>::V (&acc)[kTF]) {
+    T bv[NU];
+    T av[NU][NTF];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        const int uu = min(u, ns - 1);
+        bv[u] = op[uu * ost];
+#pragma unroll
+        for (int r = 0; r < NTF; ++r) av[u][r] = S[B.soff[r] + so + uu * sst];
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+        if (u < ns) {
+#pragma unroll
+            for (int r = 0; r < NTF; ++r) {
+                T a = av[u][r];
+                if (SCALE) a *= scale;
+                acc[r] = Mfma<T>::run(a, bv[u], acc[r]);
+            }
+        }
+    }
 }
 
 // acc[r] += (scale S)[row tile tf0 + r, :] O[column tile, :]^T, r < NTF.
@@ -241,29 +301,15 @@ __device__ __forceinline__ void blk_mac_n(const T* __restrict__ S, const T* __re
     int so = 0;
     const int kfull = uni(g.K >> 2);
     constexpr int KS = 4;
-    for (int s0 = 0; s0 < kfull; s0 += KS) {
-        const int ns = uni(min(KS, kfull - s0));
-        T bv[KS];
-        T av[KS][NTF];
-#pragma unroll
-        for (int u = 0; u < KS; ++u) {
-            if (u < ns) {
-                bv[u] = op[u * ost];
-#pragma unroll
-                for (int r = 0; r < NTF; ++r) av[u][r] = S[B.soff[r] + so + u * sst];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < KS; ++u) {
-            if (u < ns) {
-#pragma unroll
-                for (int r = 0; r < NTF; ++r) {
-                    T a = av[u][r];
-                    if (SCALE) a *= scale;
-                    acc[r] = Mfma<T>::run(a, bv[u], acc[r]);
-                }
-            }
-        }
+    int s0 = 0;
+    for (; s0 + KS <= kfull; s0 += KS) {
+        blk_slab<T, NTF, KS, SCALE>(S, op, B, so, sst, ost, KS, scale, acc);
+        so += KS * sst;
+        op += KS * ost;
+    }
+    if (s0 < kfull) {
+        const int ns = uni(kfull - s0);
+        blk_slab<T, NTF, KS - 1, SCALE>(S, op, B, so, sst, ost, ns, scale, acc);
         so += ns * sst;
         op += ns * ost;
     }
@@ -304,7 +350,7 @@ __device__ __forceinline__ void blk_put(T* __restrict__ Z, const Geo& g, const R
             for (int v = 0; v < 4; ++v) {
                 const int fr = (B.tf0 + r) * 16 + Mfma<T>::row(lane, v);
                 if (fr < g.F) {
-                    T* d = Z + (fr % g.lo) + (fr / g.lo) * g.lo * g.dn + n * g.lo;
+                    T* d = Z + fiber_off(g, fr, g.dn) + n * g.lo;
                     if (add) *d += acc[r][v];
                     else *d = acc[r][v];
                 }
