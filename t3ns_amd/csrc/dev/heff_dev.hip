@@ -359,6 +359,112 @@ __device__ __forceinline__ void blk_put(T* __restrict__ Z, const Geo& g, const R
     }
 }
 
+// Stage C sums every term of a chunk into one set of accumulators.  It
+// walks k-steps over groups of up to kTU terms, so that the operand loads of
+// all terms of a group are in flight together: a per-term walk over a short
+// K (legs of a few states) waits out one memory latency per term.
+constexpr int kTU = 4;
+__device__ __forceinline__ int64_t uni64(int64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+template <typename T>
+struct Terms {
+    const T* s[kTU];  // stage inputs
+    const T* o[kTU];  // operator blocks
+    T scale[kTU];
+    int n;  // live terms, >= 1
+};
+
+template <typename T, int NTN, bool SCALE>
+__device__ __forceinline__ void row_mac_terms_n(const Terms<T>& R, const Geo& g, const RowLane& L,
+                                                typename Mfma<T>::V (&acc)[kTN]) {
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 15, kk = lane >> 4;
+    const int n0 = min(L.tn0 * 16 + i, g.dn - 1);
+    int coff[NTN];
+#pragma unroll
+    for (int j = 0; j < NTN; ++j) coff[j] = min((L.tn0 + j) * 16 + i, g.dn - 1) - n0;
+    const int nks = uni(g.ksteps), nu = uni(R.n);
+    for (int ks = 0; ks < nks; ++ks) {
+        // lanes past K read k = 0 (in bounds) and contribute zero
+        const bool kv = ks * 4 + kk < g.K;
+        const int ao = L.soff + (kv ? ks * 4 * g.lo : -kk * g.lo);
+        const int bo = L.ooff + (kv ? ks * 4 * g.dn : -kk * g.dn);
+        T av[kTU];
+        T bv[kTU][NTN];
+#pragma unroll
+        for (int u = 0; u < kTU; ++u) {
+            av[u] = R.s[u][ao];
+#pragma unroll
+            for (int j = 0; j < NTN; ++j) bv[u][j] = R.o[u][bo + coff[j]];
+        }
+#pragma unroll
+        for (int u = 0; u < kTU; ++u) {
+            if (u < nu) {
+                T a = kv ? av[u] : (T)0;
+                if (SCALE) a *= R.scale[u];
+#pragma unroll
+                for (int j = 0; j < NTN; ++j) acc[j] = Mfma<T>::run(a, bv[u][j], acc[j]);
+            }
+        }
+    }
+}
+
+template <typename T, bool SCALE>
+__device__ __forceinline__ void row_mac_terms(const Terms<T>& R, const Geo& g, const RowLane& L,
+                                              typename Mfma<T>::V (&acc)[kTN]) {
+    switch (__builtin_amdgcn_readfirstlane(L.ntn)) {
+        case 1: row_mac_terms_n<T, 1, SCALE>(R, g, L, acc); break;
+        case 2: row_mac_terms_n<T, 2, SCALE>(R, g, L, acc); break;
+        case 3: row_mac_terms_n<T, 3, SCALE>(R, g, L, acc); break;
+        default: row_mac_terms_n<T, 4, SCALE>(R, g, L, acc); break;
+    }
+}
+
+template <typename T, int NTF, bool SCALE>
+__device__ __forceinline__ void blk_mac_terms_n(const Terms<T>& R, const Geo& g, const RowBlk& B,
+                                                typename Mfma<T>::V (&acc)[kTF]) {
+    const int kk = (threadIdx.x & 63) >> 4;
+    const int nks = uni(g.ksteps), nu = uni(R.n);
+    for (int ks = 0; ks < nks; ++ks) {
+        const bool kv = ks * 4 + kk < g.K;
+        const int so = kv ? ks * 4 * g.lo : -kk * g.lo;
+        const int bo = B.ooff + (kv ? ks * 4 * g.dn : -kk * g.dn);
+        T bv[kTU];
+        T av[kTU][NTF];
+#pragma unroll
+        for (int u = 0; u < kTU; ++u) {
+            bv[u] = R.o[u][bo];
+#pragma unroll
+            for (int r = 0; r < NTF; ++r) av[u][r] = R.s[u][B.soff[r] + so];
+        }
+#pragma unroll
+        for (int u = 0; u < kTU; ++u) {
+            if (u < nu) {
+#pragma unroll
+                for (int r = 0; r < NTF; ++r) {
+                    T a = kv ? av[u][r] : (T)0;
+                    if (SCALE) a *= R.scale[u];
+                    acc[r] = Mfma<T>::run(a, bv[u], acc[r]);
+                }
+            }
+        }
+    }
+}
+
+template <typename T, bool SCALE>
+__device__ __forceinline__ void blk_mac_terms(const Terms<T>& R, const Geo& g, const RowBlk& B,
+                                              typename Mfma<T>::V (&acc)[kTF]) {
+    switch (__builtin_amdgcn_readfirstlane(B.ntf)) {
+        case 1: blk_mac_terms_n<T, 1, SCALE>(R, g, B, acc); break;
+        case 2: blk_mac_terms_n<T, 2, SCALE>(R, g, B, acc); break;
+        case 3: blk_mac_terms_n<T, 3, SCALE>(R, g, B, acc); break;
+        default: blk_mac_terms_n<T, 4, SCALE>(R, g, B, acc); break;
+    }
+}
+
 // Tasks of one stage: (row tile, column-tile group, item group).
 struct StageTasks {
     int tf, tng, G;  // row tiles, column-tile groups, item groups
@@ -406,6 +512,27 @@ struct ChunkRec {
     int slot[kMaxChunkTerms];
 };
 
+// Terms t0, t0 + G, ... (at most kTU of them) of a chunk as stage C inputs.
+template <typename T>
+__device__ __forceinline__ Terms<T> c_terms(const ChunkRec* rec, int t0, int G, int nt, int isdmrg,
+                                            const T* W1, int w1, const T* W2, int w2, const T* opc) {
+    Terms<T> R;
+    R.n = uni(min(kTU, (nt - t0 + G - 1) / G));
+#pragma unroll
+    for (int u = 0; u < kTU; ++u) {
+        const int t = t0 + min(u, R.n - 1) * G;
+        R.o[u] = opc + uni64(rec->opc[t]);
+        if (isdmrg) {
+            R.s[u] = W1 + uni(rec->slot[t]) * w1;
+            R.scale[u] = (T)rec->alpha[t];
+        } else {
+            R.s[u] = W2 + t * w2;
+            R.scale[u] = (T)1;
+        }
+    }
+    return R;
+}
+
 template <typename T, int NT>
 __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                                             const DevChunk* __restrict__ chunks,
@@ -417,7 +544,7 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                                             const int* pos) {
     typedef typename Mfma<T>::V V;
     constexpr int NW = NT / 64;
-    const int wave = threadIdx.x >> 6;
+    const int wave = uni(threadIdx.x >> 6);  // wave-uniform: task loops stay scalar
     const int lane = threadIdx.x & 63;
     const int a = uni(P.ord[0]), b = uni(P.ord[1]), c = uni(isdmrg ? P.ord[1] : P.ord[2]);
     int cur[3] = {uni(P.d[map[0]]), uni(P.d[map[1]]), uni(P.d[map[2]])};
@@ -442,6 +569,45 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
     const int ntnC = (gC.dn + 15) >> 4;
     kC.G = gC.tf * ntnC >= NW || kC.tng > 1 ? 1 : NW / (gC.tf * ntnC);
     kC.total = kC.tf * kC.tng * kC.G;
+    // When every wave has at most one stage C task, its accumulators stay in
+    // registers across the pair's chunks and Y is read and written once per
+    // pair instead of once per chunk.
+    const bool pC = !rbC && kC.total <= NW;
+    const int baseC = kC.tf * kC.tng;
+    const int rC = wave % baseC, grpC = wave / baseC;
+    const RowLane LC = row_lane<T>(gC, rC % kC.tf, (rC / kC.tf) * kTN);
+    V accC[kTN];
+#pragma unroll
+    for (int q = 0; q < kTN; ++q) accC[q] = V{0, 0, 0, 0};
+    // stage C partial tiles of item groups, summed in group order into Y
+    auto reduce_parts = [&]() {
+        __syncthreads();
+        for (int r = wave; r < kC.tf; r += NW) {
+            const RowLane L = row_lane<T>(gC, r, 0);
+            V acc[kTN];
+#pragma unroll
+            for (int q = 0; q < kTN; ++q) {
+                acc[q] = V{0, 0, 0, 0};
+                if (q < L.ntn)
+                    for (int grp = 0; grp < kC.G; ++grp) {
+                        const T* pp = part + (((grp * kC.tf + r) * ntnC + q) * 64 + lane) * 4;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) acc[q][v] += pp[v];
+                    }
+            }
+            row_put<T>(Y, gC, L, acc, true);
+        }
+    };
+    auto store_part = [&](const V (&acc)[kTN], const RowLane& L, int grp, int r) {
+#pragma unroll
+        for (int q = 0; q < kTN; ++q) {
+            if (q < L.ntn) {
+                T* pp = part + (((grp * kC.tf + r) * ntnC + q) * 64 + lane) * 4;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) pp[v] = acc[q][v];
+            }
+        }
+    };
     const T* opa = a == 0 ? op0 : (a == 1 ? op1 : op2);
     const T* opbb = b == 0 ? op0 : (b == 1 ? op1 : op2);
     const T* opc = c == 0 ? op0 : (c == 1 ? op1 : op2);
@@ -528,62 +694,48 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                 V acc[kTF];
 #pragma unroll
                 for (int q = 0; q < kTF; ++q) acc[q] = V{0, 0, 0, 0};
-                for (int t = 0; t < nt; ++t) {
-                    if (isdmrg)
-                        blk_mac<T, true>(W1 + rec->slot[t] * w1, opc + rec->opc[t], gC, L,
-                                         (T)rec->alpha[t], acc);
-                    else
-                        blk_mac<T, false>(W2 + t * w2, opc + rec->opc[t], gC, L, (T)1, acc);
+                for (int t0 = 0; t0 < nt; t0 += kTU) {
+                    const Terms<T> R = c_terms<T>(rec, t0, 1, nt, isdmrg, W1, w1, W2, w2, opc);
+                    if (isdmrg) blk_mac_terms<T, true>(R, gC, L, acc);
+                    else blk_mac_terms<T, false>(R, gC, L, acc);
                 }
                 blk_put<T>(Y, gC, L, acc, true);
             }
             __syncthreads();
+        } else if (pC) {  // stage C into the pair's persistent accumulators
+            if (wave < kC.total) {
+                for (int t0 = grpC; t0 < nt; t0 += kC.G * kTU) {
+                    const Terms<T> R = c_terms<T>(rec, t0, kC.G, nt, isdmrg, W1, w1, W2, w2, opc);
+                    if (isdmrg) row_mac_terms<T, true>(R, gC, LC, accC);
+                    else row_mac_terms<T, false>(R, gC, LC, accC);
+                }
+            }
+            __syncthreads();
         } else {  // stage C: Y += sum_t O_c[t] x_c W2[t]  (DMRG: alpha_t O_c[t] x_c W1[slot t])
-            const int base = kC.tf * kC.tng;
             for (int task = wave; task < kC.total; task += NW) {
-                const int r = task % base, grp = task / base;
+                const int r = task % baseC, grp = task / baseC;
                 const RowLane L = row_lane<T>(gC, r % kC.tf, (r / kC.tf) * kTN);
                 V acc[kTN];
 #pragma unroll
                 for (int q = 0; q < kTN; ++q) acc[q] = V{0, 0, 0, 0};
-                for (int t = grp; t < nt; t += kC.G) {
-                    if (isdmrg)
-                        row_mac<T, true>(W1 + rec->slot[t] * w1, opc + rec->opc[t], gC, L,
-                                         (T)rec->alpha[t], acc);
-                    else
-                        row_mac<T, false>(W2 + t * w2, opc + rec->opc[t], gC, L, (T)1, acc);
+                for (int t0 = grp; t0 < nt; t0 += kC.G * kTU) {
+                    const Terms<T> R = c_terms<T>(rec, t0, kC.G, nt, isdmrg, W1, w1, W2, w2, opc);
+                    if (isdmrg) row_mac_terms<T, true>(R, gC, L, acc);
+                    else row_mac_terms<T, false>(R, gC, L, acc);
                 }
-                if (kC.G == 1) {
-                    row_put<T>(Y, gC, L, acc, true);
-                } else {
-#pragma unroll
-                    for (int q = 0; q < kTN; ++q) {
-                        if (q < L.ntn) {
-                            T* pp = part + (((grp * kC.tf + r) * ntnC + q) * 64 + lane) * 4;
-#pragma unroll
-                            for (int v = 0; v < 4; ++v) pp[v] = acc[q][v];
-                        }
-                    }
-                }
+                if (kC.G == 1) row_put<T>(Y, gC, L, acc, true);
+                else store_part(acc, L, grp, r);
             }
-            if (kC.G > 1) {
-                __syncthreads();
-                for (int r = wave; r < kC.tf; r += NW) {
-                    const RowLane L = row_lane<T>(gC, r, 0);
-                    V acc[kTN];
-#pragma unroll
-                    for (int q = 0; q < kTN; ++q) {
-                        acc[q] = V{0, 0, 0, 0};
-                        if (q < L.ntn)
-                            for (int grp = 0; grp < kC.G; ++grp) {
-                                const T* pp = part + (((grp * kC.tf + r) * ntnC + q) * 64 + lane) * 4;
-#pragma unroll
-                                for (int v = 0; v < 4; ++v) acc[q][v] += pp[v];
-                            }
-                    }
-                    row_put<T>(Y, gC, L, acc, true);
-                }
-            }
+            if (kC.G > 1) reduce_parts();
+            __syncthreads();
+        }
+    }
+    if (pC) {  // the pair's stage C sums into Y
+        if (kC.G == 1) {
+            if (wave < kC.total) row_put<T>(Y, gC, LC, accC, true);
+        } else {
+            if (wave < kC.total) store_part(accC, LC, grpC, rC);
+            reduce_parts();
             __syncthreads();
         }
     }
