@@ -218,20 +218,26 @@ template <typename T>
 __device__ __forceinline__ void row_put(T* __restrict__ Z, const Geo& g, const RowLane& L,
                                         const typename Mfma<T>::V (&acc)[kTN], bool add) {
     const int i = threadIdx.x & 15;
+    int off[kTN][4];  // -1: no element
 #pragma unroll
     for (int j = 0; j < kTN; ++j) {
         const int n = (L.tn0 + j) * 16 + i;
-        if (j < L.ntn && n < g.dn) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                if (L.rowoff[v] >= 0) {
-                    T* d = Z + L.rowoff[v] + n * g.lo;
-                    if (add) *d += acc[j][v];
-                    else *d = acc[j][v];
-                }
-            }
-        }
+        for (int v = 0; v < 4; ++v)
+            off[j][v] = j < L.ntn && n < g.dn && L.rowoff[v] >= 0 ? L.rowoff[v] + n * g.lo : -1;
     }
+    // all reads of an accumulating put are issued before its first write
+    // (element 0 stands in for missing ones), so they overlap
+    T old[kTN][4];
+#pragma unroll
+    for (int j = 0; j < kTN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) old[j][v] = add ? Z[max(off[j][v], 0)] : (T)0;
+#pragma unroll
+    for (int j = 0; j < kTN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            if (off[j][v] >= 0) Z[off[j][v]] = add ? old[j][v] + acc[j][v] : acc[j][v];
 }
 
 // Row blocking, the transpose of the scheme above: a wave task covers up to
@@ -342,21 +348,25 @@ __device__ __forceinline__ void blk_put(T* __restrict__ Z, const Geo& g, const R
                                         const typename Mfma<T>::V (&acc)[kTF], bool add) {
     const int lane = threadIdx.x & 63;
     const int n = B.tn * 16 + (lane & 15);
-    if (n >= g.dn) return;
+    int off[kTF][4];  // -1: no element
 #pragma unroll
-    for (int r = 0; r < kTF; ++r) {
-        if (r < B.ntf) {
+    for (int r = 0; r < kTF; ++r)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int fr = (B.tf0 + r) * 16 + Mfma<T>::row(lane, v);
-                if (fr < g.F) {
-                    T* d = Z + fiber_off(g, fr, g.dn) + n * g.lo;
-                    if (add) *d += acc[r][v];
-                    else *d = acc[r][v];
-                }
-            }
+        for (int v = 0; v < 4; ++v) {
+            const int fr = (B.tf0 + r) * 16 + Mfma<T>::row(lane, v);
+            off[r][v] = r < B.ntf && fr < g.F && n < g.dn ? fiber_off(g, fr, g.dn) + n * g.lo : -1;
         }
-    }
+    // reads first, as in row_put
+    T old[kTF][4];
+#pragma unroll
+    for (int r = 0; r < kTF; ++r)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) old[r][v] = add ? Z[max(off[r][v], 0)] : (T)0;
+#pragma unroll
+    for (int r = 0; r < kTF; ++r)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            if (off[r][v] >= 0) Z[off[r][v]] = add ? old[r][v] + acc[r][v] : acc[r][v];
 }
 
 // Stage C sums every term of a chunk into one set of accumulators.  It
