@@ -1,6 +1,7 @@
 // Device effective Hamiltonian: work-list construction and kernels.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <stdexcept>
@@ -753,9 +754,13 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
 
 // One workgroup owns one output block at a time (largest first) and
 // accumulates every contributing term into it: no atomics, a fixed
-// summation order.  Stage buffers live in LDS when the pair's chunks fit,
-// else in the workgroup's global scratch region.
-template <typename T, int CX, int C1, int C2, int NT, bool GLOBAL>
+// summation order.  Where the pair's input block and stage outputs live
+// depends on what fits in LDS (MODE):
+//   kModeLds     input block copied into LDS, stage outputs in LDS;
+//   kModeLdsW    input block read from global memory, stage outputs in LDS;
+//   kModeGlobal  stage outputs in the workgroup's global scratch region.
+enum { kModeLds = 0, kModeLdsW = 1, kModeGlobal = 2 };
+template <typename T, int CX, int C1, int C2, int NT, int MODE>
 __global__ void __launch_bounds__(NT)
     heff_matvec_kernel(const DevBlock* __restrict__ blocks, int nblocks,
                        const DevPair* __restrict__ pairs, const DevChunk* __restrict__ chunks,
@@ -764,13 +769,11 @@ __global__ void __launch_bounds__(NT)
                        const T* __restrict__ op2, const T* __restrict__ x, T* __restrict__ y,
                        T* __restrict__ scratch, int64_t scratch_stride, int* __restrict__ next,
                        T* __restrict__ partial, int isdmrg, int m0, int m1, int m2) {
-    // GLOBAL = false: the input block and the stage outputs live in LDS (all
-    // pairs of the units of this launch fit).  GLOBAL = true: they live in
-    // the workgroup's region of global scratch.
-    __shared__ T lds[(GLOBAL ? 0 : CX + C1 + C2) + kPartial];
+    constexpr int kStage = MODE == kModeGlobal ? 0 : (MODE == kModeLds ? CX : 0) + C1 + C2;
+    __shared__ T lds[kStage + kPartial];
     __shared__ ChunkRec rec;
     __shared__ int cur_block;
-    T* part = lds + (GLOBAL ? 0 : CX + C1 + C2);
+    T* part = lds + kStage;
     const int map[3] = {m0, m1, m2};
     int pos[3];
     pos[m0] = 0;
@@ -789,7 +792,10 @@ __global__ void __launch_bounds__(NT)
         T* Y = (B.to_part ? partial : y) + B.y_off;
         for (int pi = B.pair_begin; pi < B.pair_end; ++pi) {
             const DevPair P = pairs[pi];
-            if (GLOBAL) {
+            if (MODE == kModeLdsW) {
+                pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, lds,
+                                   lds + C1, Y, part, &rec, isdmrg, map, pos);
+            } else if (MODE == kModeGlobal) {
                 T* gw1 = scratch + (int64_t)blockIdx.x * scratch_stride;
                 pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, gw1,
                                    gw1 + scratch_stride / 2, Y, part, &rec, isdmrg, map, pos);
@@ -881,9 +887,21 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
 
     // Flatten the plan into work lists.
     const bool f32 = prec_ == PREC_FP32;
-    const int capx = f32 ? kCapX32 : kCapX64;
-    const int cap1 = f32 ? kCapW132 : kCapW164;
-    const int cap2 = f32 ? kCapW232 : kCapW264;
+    int capx = f32 ? kCapX32 : kCapX64;
+    int cap1 = f32 ? kCapW132 : kCapW164;
+    int cap2 = f32 ? kCapW232 : kCapW264;
+    // T3NS_HEFF_PLACEMENT=ldsw reads every input block from global memory,
+    // =global keeps every stage output in global scratch as well: the paths
+    // that large blocks take, forced on any problem (for tests).
+    if (const char* pl = std::getenv("T3NS_HEFF_PLACEMENT")) {
+        if (!std::strcmp(pl, "ldsw")) {
+            capx = 0;
+        } else if (!std::strcmp(pl, "global")) {
+            capx = cap1 = cap2 = 0;
+        } else if (std::strcmp(pl, "lds") != 0) {
+            throw std::runtime_error("T3NS_HEFF_PLACEMENT must be lds, ldsw or global");
+        }
+    }
     // Global chunk capacity: at least one stage output of the largest pair.
     int64_t gcap = kGlobalChunkElems;
     for (int sb = 0; sb < plan_.nrblocks; ++sb)
@@ -1061,7 +1079,8 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
         std::vector<int> sglob;
         double acc = 0;
         int beg = B.pair_begin;
-        auto is_glob = [&](int pi) { return !(hp[pi].xlds && hp[pi].wlds); };
+        // kernel of a pair: kModeLds, kModeLdsW or kModeGlobal
+        auto is_glob = [&](int pi) { return !hp[pi].wlds ? 2 : (!hp[pi].xlds ? 1 : 0); };
         for (int pi = B.pair_begin; pi < B.pair_end; ++pi) {
             if (pi > beg && (acc + pcost[pi] > unit_cap || is_glob(pi) != is_glob(beg))) {
                 slices.push_back({beg, pi});
@@ -1099,8 +1118,8 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
         }
         part_elems += (int64_t)slices.size() * ysz;
     }
-    // LDS units first, then global-scratch units; costliest first within
-    // each list so that the dynamic schedule drains evenly.
+    // Units in kernel order (kModeLds, kModeLdsW, kModeGlobal); costliest
+    // first within each list so that the dynamic schedule drains evenly.
     std::vector<int> idx(units.size());
     std::iota(idx.begin(), idx.end(), 0);
     std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
@@ -1108,12 +1127,13 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
         return ucost[a] > ucost[b];
     });
     std::vector<DevBlock> hbs;
-    nfast_ = 0;
+    nfast_ = nxg_ = 0;
     for (int i : idx) {
         hbs.push_back(units[i]);
-        nfast_ += !uglob[i];
+        nfast_ += uglob[i] == 0;
+        nxg_ += uglob[i] == 1;
     }
-    nglob_ = (int)hbs.size() - nfast_;
+    nglob_ = (int)hbs.size() - nfast_ - nxg_;
     stats_.nunits = (int64_t)hbs.size();
     stats_.nsplit_blocks = (int64_t)hs.size();
     nsplits_ = (int)hs.size();
@@ -1194,7 +1214,7 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     nwg_glob_ = std::max(1, std::min(nglob_, ncu));
     if (prec_ == PREC_FP32) scratch32_.alloc(nglob_ ? (size_t)nwg_glob_ * maxscratch : 1);
     else scratch64_.alloc(nglob_ ? (size_t)nwg_glob_ * maxscratch : 1);
-    counter_.alloc(2);
+    counter_.alloc(3);
     splits_.upload(hs, stream);
     if (f32) part32_.alloc(std::max<int64_t>(1, part_elems_));
     else part64_.alloc(std::max<int64_t>(1, part_elems_));
@@ -1208,22 +1228,29 @@ template <typename T, int CX, int C1, int C2>
 void HeffDevice::launch(const T* x, T* y, const T* const* ops, T* scratch, T* part,
                         hipStream_t s) {
     hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(size_)), dim3(256), 0, s, y, size_, (T)0);
-    hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s, counter_.get(), (int64_t)2, 0);
+    hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s, counter_.get(), (int64_t)3, 0);
     if (part_elems_)
         hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(part_elems_)), dim3(256), 0, s, part,
                            part_elems_, (T)0);
     if (nfast_)
-        hipLaunchKernelGGL((heff_matvec_kernel<T, CX, C1, C2, kThreads, false>),
+        hipLaunchKernelGGL((heff_matvec_kernel<T, CX, C1, C2, kThreads, kModeLds>),
                            dim3(std::min(nfast_, ncu_)), dim3(kThreads), 0, s, blocks_.get(),
                            nfast_, pairs_.get(), chunks_.get(), runs_.get(), terms_.get(), ops[0],
                            ops[1], ops[2], x, y, scratch, stats_.scratch_per_wg, counter_.get(),
                            part, isdmrg_, map_[0], map_[1], map_[2]);
+    if (nxg_)
+        hipLaunchKernelGGL((heff_matvec_kernel<T, CX, C1, C2, kThreads, kModeLdsW>),
+                           dim3(std::min(nxg_, ncu_)), dim3(kThreads), 0, s,
+                           blocks_.get() + nfast_, nxg_, pairs_.get(), chunks_.get(), runs_.get(),
+                           terms_.get(), ops[0], ops[1], ops[2], x, y, scratch,
+                           stats_.scratch_per_wg, counter_.get() + 1, part, isdmrg_, map_[0],
+                           map_[1], map_[2]);
     if (nglob_)
-        hipLaunchKernelGGL((heff_matvec_kernel<T, CX, C1, C2, kThreads, true>), dim3(nwg_glob_),
-                           dim3(kThreads), 0, s, blocks_.get() + nfast_, nglob_, pairs_.get(),
-                           chunks_.get(), runs_.get(), terms_.get(), ops[0], ops[1], ops[2], x, y,
-                           scratch, stats_.scratch_per_wg, counter_.get() + 1, part, isdmrg_,
-                           map_[0], map_[1], map_[2]);
+        hipLaunchKernelGGL((heff_matvec_kernel<T, CX, C1, C2, kThreads, kModeGlobal>),
+                           dim3(nwg_glob_), dim3(kThreads), 0, s, blocks_.get() + nfast_ + nxg_,
+                           nglob_, pairs_.get(), chunks_.get(), runs_.get(), terms_.get(), ops[0],
+                           ops[1], ops[2], x, y, scratch, stats_.scratch_per_wg,
+                           counter_.get() + 2, part, isdmrg_, map_[0], map_[1], map_[2]);
     if (nsplits_)
         hipLaunchKernelGGL((split_reduce_kernel<T>), dim3(std::min(nsplits_, 4096)), dim3(256), 0,
                            s, splits_.get(), nsplits_, part, y);

@@ -105,6 +105,7 @@ private:
     int nwg_glob_ = 1;
     int ncu_ = 256;
     int nfast_ = 0;   // units whose pairs all fit in LDS (first in the unit list)
+    int nxg_ = 0;     // units whose stage outputs fit in LDS but input blocks do not
     int nglob_ = 0;   // units that use global scratch
     DevBuf<DevBlock> blocks_;
     DevBuf<DevPair> pairs_;

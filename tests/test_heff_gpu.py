@@ -61,3 +61,34 @@ def test_matvec_fp32(gpu, case):
     y = h.matvec(x)
     yr = oh.matvec(x)
     assert np.linalg.norm(y - yr) / np.linalg.norm(yr) < 2e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("placement", ["ldsw", "global"])
+@pytest.mark.parametrize("case", HEFF_CASES, ids=case_id)
+def test_matvec_placements(gpu, case, placement, monkeypatch):
+    """The kernel paths of blocks too large for LDS (input block read from
+    global memory; stage outputs in global scratch), forced on every pair."""
+    monkeypatch.setenv("T3NS_HEFF_PLACEMENT", placement)
+    ctx, st, d, o, s = _setup(case)
+    print("case", case_id(case), placement, flush=True)
+    oh = O.OracleHeff(d, o, s)
+    x = np.random.default_rng(17).standard_normal(oh.size)
+    yr = oh.matvec(x)
+    for prec, tol in ((T.FP64, 1e-12), (T.FP32, 2e-5)):
+        h = T.Heff(d, o, s, prec, keepalive=st)
+        st_ = h.kernel_stats()
+        assert st_["pairs_global_x"] > 0
+        if placement == "global":
+            assert st_["pairs_global_w"] > 0
+        y = h.matvec(x)
+        assert np.linalg.norm(y - yr) / np.linalg.norm(yr) < tol
+        h.close()
+
+
+@pytest.mark.gpu
+def test_placement_option_rejects_unknown(gpu, monkeypatch):
+    monkeypatch.setenv("T3NS_HEFF_PLACEMENT", "bogus")
+    ctx, st, d, o, s = _setup(HEFF_CASES[0])
+    with pytest.raises(RuntimeError):
+        T.Heff(d, o, s, T.FP64, keepalive=st)
