@@ -14,10 +14,10 @@ namespace {
 constexpr int kThreads = 512;  // 8 waves
 constexpr int kWaves = kThreads / 64;
 constexpr int kDiagThreads = 256;
-// LDS elements of the input block, the first-stage outputs, the
-// second-stage outputs of one chunk and the stage-C partial tiles.
-constexpr int kCapX64 = 4096, kCapW164 = 6144, kCapW264 = 6144;
-constexpr int kCapX32 = 8192, kCapW132 = 12288, kCapW232 = 12288;
+// LDS pool of a workgroup (elements), laid out per pair: the input block,
+// then the first-stage outputs of a chunk, then its second-stage outputs
+// (chunks are cut to fit); the stage-C partial tiles have their own region.
+constexpr int kPool64 = 16384, kPool32 = 32768;
 constexpr int kPartial = kWaves * 256;
 constexpr int kMaxChunkTerms = 64;
 // Global scratch per workgroup for pairs whose stage outputs exceed LDS.
@@ -550,7 +550,7 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                                             const int* __restrict__ runs,
                                             const DevTerm* __restrict__ terms,
                                             const T* __restrict__ op0, const T* __restrict__ op1,
-                                            const T* __restrict__ op2, const T* X, T* W1, T* W2, T* Y,
+                                            const T* __restrict__ op2, const T* X, T* W1, T* Y,
                                             T* part, ChunkRec* rec, int isdmrg, const int* map,
                                             const int* pos) {
     typedef typename Mfma<T>::V V;
@@ -626,6 +626,7 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
     const int w1 = uni(P.w1), w2 = uni(P.w2);
     for (int ci = cb; ci < ce; ++ci) {
         const DevChunk C = chunks[ci];
+        T* const W2 = W1 + uni(C.w2_off);  // this chunk's second-stage outputs
         const int nr = uni(C.run_end - C.run_begin);
         const int nt = uni(C.term_end - C.term_begin);
         // chunk records into LDS
@@ -760,7 +761,7 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
 //   kModeLdsW    input block read from global memory, stage outputs in LDS;
 //   kModeGlobal  stage outputs in the workgroup's global scratch region.
 enum { kModeLds = 0, kModeLdsW = 1, kModeGlobal = 2 };
-template <typename T, int CX, int C1, int C2, int NT, int MODE>
+template <typename T, int POOL, int NT, int MODE>
 __global__ void __launch_bounds__(NT)
     heff_matvec_kernel(const DevBlock* __restrict__ blocks, int nblocks,
                        const DevPair* __restrict__ pairs, const DevChunk* __restrict__ chunks,
@@ -769,7 +770,7 @@ __global__ void __launch_bounds__(NT)
                        const T* __restrict__ op2, const T* __restrict__ x, T* __restrict__ y,
                        T* __restrict__ scratch, int64_t scratch_stride, int* __restrict__ next,
                        T* __restrict__ partial, int isdmrg, int m0, int m1, int m2) {
-    constexpr int kStage = MODE == kModeGlobal ? 0 : (MODE == kModeLds ? CX : 0) + C1 + C2;
+    constexpr int kStage = MODE == kModeGlobal ? 0 : POOL;
     __shared__ T lds[kStage + kPartial];
     __shared__ ChunkRec rec;
     __shared__ int cur_block;
@@ -793,19 +794,19 @@ __global__ void __launch_bounds__(NT)
         for (int pi = B.pair_begin; pi < B.pair_end; ++pi) {
             const DevPair P = pairs[pi];
             if (MODE == kModeLdsW) {
-                pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, lds,
-                                   lds + C1, Y, part, &rec, isdmrg, map, pos);
+                pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, lds, Y,
+                                   part, &rec, isdmrg, map, pos);
             } else if (MODE == kModeGlobal) {
                 T* gw1 = scratch + (int64_t)blockIdx.x * scratch_stride;
-                pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, gw1,
-                                   gw1 + scratch_stride / 2, Y, part, &rec, isdmrg, map, pos);
+                pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, gw1, Y,
+                                   part, &rec, isdmrg, map, pos);
             } else {
                 const T* X = x + P.x_off;
                 const int xsz = P.d[0] * P.d[1] * P.d[2];
                 for (int i = threadIdx.x; i < xsz; i += NT) lds[i] = X[i];
                 __syncthreads();
-                pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, lds, lds + CX,
-                                   lds + CX + C1, Y, part, &rec, isdmrg, map, pos);
+                pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, lds,
+                                   lds + P.w1_off, Y, part, &rec, isdmrg, map, pos);
             }
         }
     }
@@ -887,17 +888,17 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
 
     // Flatten the plan into work lists.
     const bool f32 = prec_ == PREC_FP32;
-    int capx = f32 ? kCapX32 : kCapX64;
-    int cap1 = f32 ? kCapW132 : kCapW164;
-    int cap2 = f32 ? kCapW232 : kCapW264;
+    int pool = f32 ? kPool32 : kPool64;
+    bool x_in_lds = true;
     // T3NS_HEFF_PLACEMENT=ldsw reads every input block from global memory,
     // =global keeps every stage output in global scratch as well: the paths
     // that large blocks take, forced on any problem (for tests).
     if (const char* pl = std::getenv("T3NS_HEFF_PLACEMENT")) {
         if (!std::strcmp(pl, "ldsw")) {
-            capx = 0;
+            x_in_lds = false;
         } else if (!std::strcmp(pl, "global")) {
-            capx = cap1 = cap2 = 0;
+            x_in_lds = false;
+            pool = 0;
         } else if (std::strcmp(pl, "lds") != 0) {
             throw std::runtime_error("T3NS_HEFF_PLACEMENT must be lds, ldsw or global");
         }
@@ -977,9 +978,13 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
                 else P.w2 = (int)sz;
             }
             const int64_t xsz = (int64_t)P.d[0] * P.d[1] * P.d[2];
-            P.xlds = xsz <= capx;
-            P.wlds = P.w1 <= cap1 && P.w2 <= cap2;
-            const int64_t c1 = P.wlds ? cap1 : gcap, c2 = P.wlds ? cap2 : gcap;
+            // LDS when one run and one term fit next to the input block
+            // (kModeLds) or without it (kModeLdsW)
+            const int64_t wmin = (int64_t)P.w1 + (isdmrg_ ? 0 : P.w2);
+            P.xlds = x_in_lds && xsz + wmin <= pool;
+            P.wlds = wmin <= pool;
+            P.w1_off = P.xlds ? (int)xsz : 0;
+            const int64_t avail = pool - P.w1_off;
             // chunks
             P.chunk_begin = (int)hc.size();
             DevChunk C{};
@@ -988,13 +993,15 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
             auto close = [&](int end) {
                 C.term_end = end;
                 C.run_end = (int)hr.size();
+                C.w2_off = P.wlds ? nruns * P.w1 : (int)gcap;
                 hc.push_back(C);
             };
             for (size_t t = tb; t < ht.size(); ++t) {
                 const bool newrun = nterms == 0 || ht[t].op[first] != prev;
+                const int64_t r1 = (int64_t)(nruns + newrun) * P.w1;
+                const int64_t r2 = isdmrg_ ? 0 : (int64_t)(nterms + 1) * P.w2;
                 const bool fits = nterms < kMaxChunkTerms &&
-                                  (int64_t)(nruns + newrun) * P.w1 <= c1 &&
-                                  (isdmrg_ || (int64_t)(nterms + 1) * P.w2 <= c2);
+                                  (P.wlds ? r1 + r2 <= avail : r1 <= gcap && r2 <= gcap);
                 if (nterms > 0 && !fits) {
                     close((int)t);
                     nruns = nterms = 0;
@@ -1224,7 +1231,7 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
 
 HeffDevice::~HeffDevice() = default;
 
-template <typename T, int CX, int C1, int C2>
+template <typename T, int POOL>
 void HeffDevice::launch(const T* x, T* y, const T* const* ops, T* scratch, T* part,
                         hipStream_t s) {
     hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(size_)), dim3(256), 0, s, y, size_, (T)0);
@@ -1233,20 +1240,20 @@ void HeffDevice::launch(const T* x, T* y, const T* const* ops, T* scratch, T* pa
         hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(part_elems_)), dim3(256), 0, s, part,
                            part_elems_, (T)0);
     if (nfast_)
-        hipLaunchKernelGGL((heff_matvec_kernel<T, CX, C1, C2, kThreads, kModeLds>),
+        hipLaunchKernelGGL((heff_matvec_kernel<T, POOL, kThreads, kModeLds>),
                            dim3(std::min(nfast_, ncu_)), dim3(kThreads), 0, s, blocks_.get(),
                            nfast_, pairs_.get(), chunks_.get(), runs_.get(), terms_.get(), ops[0],
                            ops[1], ops[2], x, y, scratch, stats_.scratch_per_wg, counter_.get(),
                            part, isdmrg_, map_[0], map_[1], map_[2]);
     if (nxg_)
-        hipLaunchKernelGGL((heff_matvec_kernel<T, CX, C1, C2, kThreads, kModeLdsW>),
+        hipLaunchKernelGGL((heff_matvec_kernel<T, POOL, kThreads, kModeLdsW>),
                            dim3(std::min(nxg_, ncu_)), dim3(kThreads), 0, s,
                            blocks_.get() + nfast_, nxg_, pairs_.get(), chunks_.get(), runs_.get(),
                            terms_.get(), ops[0], ops[1], ops[2], x, y, scratch,
                            stats_.scratch_per_wg, counter_.get() + 1, part, isdmrg_, map_[0],
                            map_[1], map_[2]);
     if (nglob_)
-        hipLaunchKernelGGL((heff_matvec_kernel<T, CX, C1, C2, kThreads, kModeGlobal>),
+        hipLaunchKernelGGL((heff_matvec_kernel<T, POOL, kThreads, kModeGlobal>),
                            dim3(nwg_glob_), dim3(kThreads), 0, s, blocks_.get() + nfast_ + nxg_,
                            nglob_, pairs_.get(), chunks_.get(), runs_.get(), terms_.get(), ops[0],
                            ops[1], ops[2], x, y, scratch, stats_.scratch_per_wg,
@@ -1259,13 +1266,13 @@ void HeffDevice::launch(const T* x, T* y, const T* const* ops, T* scratch, T* pa
 void HeffDevice::matvec(const double* dvec, double* dres, hipStream_t s) {
     if (prec_ == PREC_FP64) {
         const double* ops[3] = {ops64_[0].get(), ops64_[1].get(), ops64_[2].get()};
-        launch<double, kCapX64, kCapW164, kCapW264>(dvec, dres, ops, scratch64_.get(),
+        launch<double, kPool64>(dvec, dres, ops, scratch64_.get(),
                                                     part64_.get(), s);
     } else {
         hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(size_)), dim3(256), 0, s,
                            dvec, x32_.get(), size_);
         const float* ops[3] = {ops32_[0].get(), ops32_[1].get(), ops32_[2].get()};
-        launch<float, kCapX32, kCapW132, kCapW232>(x32_.get(), y32_.get(), ops, scratch32_.get(),
+        launch<float, kPool32>(x32_.get(), y32_.get(), ops, scratch32_.get(),
                                                    part32_.get(), s);
         hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(size_)), dim3(256), 0, s,
                            y32_.get(), dres, size_);

@@ -38,12 +38,14 @@ struct DevPair {       // one (output block, input block) pair
     int w2;            // elements of one second-stage output (0 for DMRG)
     int xlds;          // input block staged in LDS
     int wlds;          // stage outputs in LDS (else workgroup global scratch)
+    int w1_off;        // first-stage outputs: offset in the LDS pool (after the input block)
 };
 // Consecutive terms of a pair processed between two barriers: their
 // distinct first-stage operators ("runs") and the terms themselves.
 struct DevChunk {
     int term_begin, term_end;
     int run_begin, run_end;  // into the run-lead list
+    int w2_off;              // second-stage outputs: offset from the first-stage outputs
 };
 struct DevTerm {       // one instruction of one MPO group of a pair
     int64_t op[3];     // element offsets into the three operator arenas
@@ -93,7 +95,7 @@ public:
     void diagonal(double* ddiag, hipStream_t s);
 
 private:
-    template <typename T, int CX, int C1, int C2>
+    template <typename T, int POOL>
     void launch(const T* x, T* y, const T* const* ops, T* scratch, T* part, hipStream_t s);
 
     HeffPlan plan_;
