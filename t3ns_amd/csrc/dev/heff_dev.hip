@@ -19,7 +19,7 @@ constexpr int kDiagThreads = 256;
 // (chunks are cut to fit); the stage-C partial tiles have their own region.
 constexpr int kPool64 = 16384, kPool32 = 32768;
 constexpr int kPartial = kWaves * 256;
-constexpr int kMaxChunkTerms = 64;
+constexpr int kMaxChunkTerms = 256;  // chunk records staged by one thread each
 // Global scratch per workgroup for pairs whose stage outputs exceed LDS.
 constexpr int64_t kGlobalChunkElems = 1 << 18;
 const int hOrd[6][3] = {{0, 1, 2}, {1, 0, 2}, {0, 2, 1}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
