@@ -476,6 +476,92 @@ __device__ __forceinline__ void blk_mac_terms(const Terms<T>& R, const Geo& g, c
     }
 }
 
+// Vector-unit path for stages whose output leg has at most kVN states (a
+// matrix-core tile would be mostly padding there): one lane per fiber, the
+// fiber's dn outputs in registers, the operator values shared by the wave
+// (uniform loads).  A task is 64 fibers; items are split over groups of
+// waves as in the matrix-core path.
+constexpr int kVN = 4;
+struct VLane {
+    int soff;  // S offset of the lane's fiber (clamped to the last fiber)
+    int zoff;  // Z offset of the lane's fiber
+    int live;  // the fiber exists
+};
+__device__ __forceinline__ VLane vlane(const Geo& g, int ft) {
+    VLane L;
+    const int f = ft * 64 + (threadIdx.x & 63);
+    const int fc = min(f, g.F - 1);
+    const int hi = fiber_hi(g, fc);
+    L.soff = fc + hi * g.lo * (g.K - 1);
+    L.zoff = fc + hi * g.lo * (g.dn - 1);
+    L.live = f < g.F;
+    return L;
+}
+struct VTasks {
+    int nft, G, total;  // fiber tiles, item groups, tasks
+};
+__device__ __forceinline__ VTasks vtasks(const Geo& g, int nw) {
+    VTasks t;
+    t.nft = (g.F + 63) >> 6;
+    t.G = t.nft >= nw ? 1 : nw / t.nft;
+    t.total = t.nft * t.G;
+    return t;
+}
+
+// acc[n] += sum_k (scale S)[f, k] O[n, k], n < NB = dn (O is dn x K, column-major)
+template <typename T, int NB, bool SCALE>
+__device__ __forceinline__ void vmac_n(const T* __restrict__ S, const T* __restrict__ O,
+                                       const Geo& g, const VLane& L, T scale,
+                                       typename Mfma<T>::V& acc) {
+    const T* sp = S + L.soff;
+    const int K = uni(g.K), lo = uni(g.lo);
+    int k = 0;
+    for (; k + 4 <= K; k += 4) {
+        T a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = sp[(k + u) * lo];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const T av = SCALE ? a[u] * scale : a[u];
+#pragma unroll
+            for (int n = 0; n < NB; ++n) acc[n] = fma(av, O[(k + u) * NB + n], acc[n]);
+        }
+    }
+    for (; k < K; ++k) {
+        const T av = SCALE ? sp[k * lo] * scale : sp[k * lo];
+#pragma unroll
+        for (int n = 0; n < NB; ++n) acc[n] = fma(av, O[k * NB + n], acc[n]);
+    }
+}
+
+template <typename T, bool SCALE>
+__device__ __forceinline__ void vmac(const T* __restrict__ S, const T* __restrict__ O,
+                                     const Geo& g, const VLane& L, T scale,
+                                     typename Mfma<T>::V& acc) {
+    switch (uni(g.dn)) {
+        case 1: vmac_n<T, 1, SCALE>(S, O, g, L, scale, acc); break;
+        case 2: vmac_n<T, 2, SCALE>(S, O, g, L, scale, acc); break;
+        case 3: vmac_n<T, 3, SCALE>(S, O, g, L, scale, acc); break;
+        default: vmac_n<T, 4, SCALE>(S, O, g, L, scale, acc); break;
+    }
+}
+
+// Z[f, n] (+)= acc[n] for the lane's fiber; an accumulating put reads
+// before it writes.
+template <typename T>
+__device__ __forceinline__ void vput(T* __restrict__ Z, const Geo& g, const VLane& L,
+                                     const typename Mfma<T>::V& acc, bool add) {
+    if (!L.live) return;
+    T* z = Z + L.zoff;
+    const int lo = g.lo, dn = uni(g.dn);
+    T old[kVN];
+#pragma unroll
+    for (int n = 0; n < kVN; ++n) old[n] = add && n < dn ? z[n * lo] : (T)0;
+#pragma unroll
+    for (int n = 0; n < kVN; ++n)
+        if (n < dn) z[n * lo] = add ? old[n] + acc[n] : acc[n];
+}
+
 // Tasks of one stage: (row tile, column-tile group, item group).
 struct StageTasks {
     int tf, tng, G;  // row tiles, column-tile groups, item groups
@@ -567,6 +653,9 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
         cur[pos[b]] = bd[b];
     }
     const Geo gC = make_geo(cur[0], cur[1], cur[2], pos[c], bd[c]);
+    // stages whose output leg has at most kVN states take the vector-unit path
+    const bool vA = gA.dn <= kVN, vB = gB.dn <= kVN, vC = gC.dn <= kVN;
+    const VTasks tA = vtasks(gA, NW), tB = vtasks(gB, NW), tC = vtasks(gC, NW);
     const bool rbA = use_rows(gA), rbB = use_rows(gB);
     const StageTasks kA = rbA ? stage_tasks_rb(gA, NW) : stage_tasks(gA, NW);
     const StageTasks kB = rbB ? stage_tasks_rb(gB, NW) : stage_tasks(gB, NW);
@@ -583,7 +672,10 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
     // When every wave has at most one stage C task, its accumulators stay in
     // registers across the pair's chunks and Y is read and written once per
     // pair instead of once per chunk.
-    const bool pC = !rbC && kC.total <= NW;
+    const bool pC = !vC && !rbC && kC.total <= NW;
+    const bool pCv = vC && tC.total <= NW;  // the same on the vector-unit path
+    const VLane LV = vlane(gC, wave % tC.nft);
+    const int grpV = wave / tC.nft;
     const int baseC = kC.tf * kC.tng;
     const int rC = wave % baseC, grpC = wave / baseC;
     const RowLane LC = row_lane<T>(gC, rC % kC.tf, (rC / kC.tf) * kTN);
@@ -619,6 +711,25 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             }
         }
     };
+    // vector-unit stage C partials: (group, fiber tile) x 64 lanes x 4 values
+    auto vstore_part = [&](const V& acc, int grp, int ft) {
+        T* pp = part + ((grp * tC.nft + ft) * 64 + lane) * 4;
+#pragma unroll
+        for (int n = 0; n < 4; ++n) pp[n] = acc[n];
+    };
+    auto vreduce_parts = [&]() {
+        __syncthreads();
+        for (int ft = wave; ft < tC.nft; ft += NW) {
+            const VLane L = vlane(gC, ft);
+            V acc = V{0, 0, 0, 0};
+            for (int grp = 0; grp < tC.G; ++grp) {
+                const T* pp = part + ((grp * tC.nft + ft) * 64 + lane) * 4;
+#pragma unroll
+                for (int n = 0; n < 4; ++n) acc[n] += pp[n];
+            }
+            vput<T>(Y, gC, L, acc, true);
+        }
+    };
     const T* opa = a == 0 ? op0 : (a == 1 ? op1 : op2);
     const T* opbb = b == 0 ? op0 : (b == 1 ? op1 : op2);
     const T* opc = c == 0 ? op0 : (c == 1 ? op1 : op2);
@@ -639,7 +750,17 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
         }
         if (threadIdx.x < nr) rec->opa[threadIdx.x] = terms[runs[uni(C.run_begin) + threadIdx.x]].op[a];
         __syncthreads();
-        if (rbA) {  // stage A, row-blocked
+        if (vA) {  // stage A, vector unit
+            for (int task = wave; task < tA.total; task += NW) {
+                const VLane L = vlane(gA, task % tA.nft);
+                for (int j = task / tA.nft; j < nr; j += tA.G) {
+                    V acc = V{0, 0, 0, 0};
+                    vmac<T, false>(X, opa + uni64(rec->opa[j]), gA, L, (T)1, acc);
+                    vput<T>(W1 + j * w1, gA, L, acc, false);
+                }
+            }
+            __syncthreads();
+        } else if (rbA) {  // stage A, row-blocked
             const int base = kA.tf * kA.tng;
             for (int task = wave; task < kA.total; task += NW) {
                 const int r = task % base, grp = task / base;
@@ -668,7 +789,18 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             }
             __syncthreads();
         }
-        if (!isdmrg && rbB) {  // stage B, row-blocked
+        if (!isdmrg && vB) {  // stage B, vector unit
+            for (int task = wave; task < tB.total; task += NW) {
+                const VLane L = vlane(gB, task % tB.nft);
+                for (int t = task / tB.nft; t < nt; t += tB.G) {
+                    V acc = V{0, 0, 0, 0};
+                    vmac<T, true>(W1 + uni(rec->slot[t]) * w1, opbb + uni64(rec->opb[t]), gB, L,
+                                  (T)rec->alpha[t], acc);
+                    vput<T>(W2 + t * w2, gB, L, acc, false);
+                }
+            }
+            __syncthreads();
+        } else if (!isdmrg && rbB) {  // stage B, row-blocked
             const int base = kB.tf * kB.tng;
             for (int task = wave; task < kB.total; task += NW) {
                 const int r = task % base, grp = task / base;
@@ -699,7 +831,30 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             }
             __syncthreads();
         }
-        if (rbC) {  // stage C, row-blocked (every task owns its output tiles)
+        if (vC) {  // stage C, vector unit
+            auto cmac = [&](int t, const VLane& L, V& acc) {
+                if (isdmrg)
+                    vmac<T, true>(W1 + uni(rec->slot[t]) * w1, opc + uni64(rec->opc[t]), gC, L,
+                                  (T)rec->alpha[t], acc);
+                else
+                    vmac<T, false>(W2 + t * w2, opc + uni64(rec->opc[t]), gC, L, (T)1, acc);
+            };
+            if (pCv) {
+                if (wave < tC.total)
+                    for (int t = grpV; t < nt; t += tC.G) cmac(t, LV, accC[0]);
+            } else {
+                for (int task = wave; task < tC.total; task += NW) {
+                    const int ft = task % tC.nft, grp = task / tC.nft;
+                    const VLane L = vlane(gC, ft);
+                    V acc = V{0, 0, 0, 0};
+                    for (int t = grp; t < nt; t += tC.G) cmac(t, L, acc);
+                    if (tC.G == 1) vput<T>(Y, gC, L, acc, true);
+                    else vstore_part(acc, grp, ft);
+                }
+                if (tC.G > 1) vreduce_parts();
+            }
+            __syncthreads();
+        } else if (rbC) {  // stage C, row-blocked (every task owns its output tiles)
             const int base = kCr.tf * kCr.tng;
             for (int task = wave; task < base; task += NW) {
                 const RowBlk L = row_blk<T>(gC, (task % kCr.tf) * kTF, task / kCr.tf);
@@ -739,6 +894,15 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                 else store_part(acc, L, grp, r);
             }
             if (kC.G > 1) reduce_parts();
+            __syncthreads();
+        }
+    }
+    if (pCv) {  // the pair's vector-unit stage C sums into Y
+        if (tC.G == 1) {
+            if (wave < tC.total) vput<T>(Y, gC, LV, accC[0], true);
+        } else {
+            if (wave < tC.total) vstore_part(accC[0], grpV, wave % tC.nft);
+            vreduce_parts();
             __syncthreads();
         }
     }
