@@ -19,7 +19,7 @@ constexpr int kDiagThreads = 256;
 // (chunks are cut to fit); the stage-C partial tiles have their own region.
 constexpr int kPool64 = 16384, kPool32 = 32768;
 constexpr int kPartial = kWaves * 256;
-constexpr int kMaxChunkTerms = 256;  // chunk records staged by one thread each
+constexpr int kMaxChunkTerms = 128;  // chunk records staged by one thread each
 // Global scratch per workgroup for pairs whose stage outputs exceed LDS.
 constexpr int64_t kGlobalChunkElems = 1 << 18;
 const int hOrd[6][3] = {{0, 1, 2}, {1, 0, 2}, {0, 2, 1}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
@@ -637,7 +637,7 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                                             const DevTerm* __restrict__ terms,
                                             const T* __restrict__ op0, const T* __restrict__ op1,
                                             const T* __restrict__ op2, const T* X, T* W1, T* Y,
-                                            T* part, ChunkRec* rec, int isdmrg, const int* map,
+                                            T* part, ChunkRec* recs, int isdmrg, const int* map,
                                             const int* pos) {
     typedef typename Mfma<T>::V V;
     constexpr int NW = NT / 64;
@@ -740,16 +740,20 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
         T* const W2 = W1 + uni(C.w2_off);  // this chunk's second-stage outputs
         const int nr = uni(C.run_end - C.run_begin);
         const int nt = uni(C.term_end - C.term_begin);
-        // chunk records into LDS
-        if (threadIdx.x < nt) {
-            const DevTerm& tm = terms[uni(C.term_begin) + threadIdx.x];
-            rec->opb[threadIdx.x] = tm.op[b];
-            rec->opc[threadIdx.x] = tm.op[c];
-            rec->alpha[threadIdx.x] = tm.alpha;
-            rec->slot[threadIdx.x] = tm.slot;
+        // chunk records in LDS (double-buffered; the first chunk of a pair
+        // loads its own, the others were loaded during the previous chunk)
+        ChunkRec* const rec = recs + ((ci - cb) & 1);
+        if (ci == cb) {
+            if (threadIdx.x < nt) {
+                const DevTerm& tm = terms[uni(C.term_begin) + threadIdx.x];
+                rec->opb[threadIdx.x] = tm.op[b];
+                rec->opc[threadIdx.x] = tm.op[c];
+                rec->alpha[threadIdx.x] = tm.alpha;
+                rec->slot[threadIdx.x] = tm.slot;
+            }
+            if (threadIdx.x < nr) rec->opa[threadIdx.x] = terms[runs[uni(C.run_begin) + threadIdx.x]].op[a];
+            __syncthreads();
         }
-        if (threadIdx.x < nr) rec->opa[threadIdx.x] = terms[runs[uni(C.run_begin) + threadIdx.x]].op[a];
-        __syncthreads();
         if (vA) {  // stage A, vector unit
             for (int task = wave; task < tA.total; task += NW) {
                 const VLane L = vlane(gA, task % tA.nft);
@@ -831,6 +835,26 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             }
             __syncthreads();
         }
+        // The next chunk's records: loaded now, stored into the other
+        // buffer before the barrier that ends stage C (their latency
+        // overlaps stage C).
+        const bool pf = ci + 1 < ce;
+        int64_t pfb = 0, pfc = 0, pfa = 0;
+        double pfal = 0;
+        int pfsl = 0, pfnt = 0, pfnr = 0;
+        if (pf) {
+            const DevChunk C2 = chunks[ci + 1];
+            pfnt = uni(C2.term_end - C2.term_begin);
+            pfnr = uni(C2.run_end - C2.run_begin);
+            if (threadIdx.x < pfnt) {
+                const DevTerm& tm = terms[uni(C2.term_begin) + threadIdx.x];
+                pfb = tm.op[b];
+                pfc = tm.op[c];
+                pfal = tm.alpha;
+                pfsl = tm.slot;
+            }
+            if (threadIdx.x < pfnr) pfa = terms[runs[uni(C2.run_begin) + threadIdx.x]].op[a];
+        }
         if (vC) {  // stage C, vector unit
             auto cmac = [&](int t, const VLane& L, V& acc) {
                 if (isdmrg)
@@ -853,7 +877,6 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                 }
                 if (tC.G > 1) vreduce_parts();
             }
-            __syncthreads();
         } else if (rbC) {  // stage C, row-blocked (every task owns its output tiles)
             const int base = kCr.tf * kCr.tng;
             for (int task = wave; task < base; task += NW) {
@@ -868,7 +891,6 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                 }
                 blk_put<T>(Y, gC, L, acc, true);
             }
-            __syncthreads();
         } else if (pC) {  // stage C into the pair's persistent accumulators
             if (wave < kC.total) {
                 for (int t0 = grpC; t0 < nt; t0 += kC.G * kTU) {
@@ -877,7 +899,6 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                     else row_mac_terms<T, false>(R, gC, LC, accC);
                 }
             }
-            __syncthreads();
         } else {  // stage C: Y += sum_t O_c[t] x_c W2[t]  (DMRG: alpha_t O_c[t] x_c W1[slot t])
             for (int task = wave; task < kC.total; task += NW) {
                 const int r = task % baseC, grp = task / baseC;
@@ -894,8 +915,18 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                 else store_part(acc, L, grp, r);
             }
             if (kC.G > 1) reduce_parts();
-            __syncthreads();
         }
+        if (pf) {
+            ChunkRec* nx = recs + ((ci + 1 - cb) & 1);
+            if (threadIdx.x < pfnt) {
+                nx->opb[threadIdx.x] = pfb;
+                nx->opc[threadIdx.x] = pfc;
+                nx->alpha[threadIdx.x] = pfal;
+                nx->slot[threadIdx.x] = pfsl;
+            }
+            if (threadIdx.x < pfnr) nx->opa[threadIdx.x] = pfa;
+        }
+        __syncthreads();
     }
     if (pCv) {  // the pair's vector-unit stage C sums into Y
         if (tC.G == 1) {
@@ -936,7 +967,7 @@ __global__ void __launch_bounds__(NT)
                        T* __restrict__ partial, int isdmrg, int m0, int m1, int m2) {
     constexpr int kStage = MODE == kModeGlobal ? 0 : POOL;
     __shared__ T lds[kStage + kPartial];
-    __shared__ ChunkRec rec;
+    __shared__ ChunkRec rec[2];
     __shared__ int cur_block;
     T* part = lds + kStage;
     const int map[3] = {m0, m1, m2};
@@ -959,18 +990,18 @@ __global__ void __launch_bounds__(NT)
             const DevPair P = pairs[pi];
             if (MODE == kModeLdsW) {
                 pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, lds, Y,
-                                   part, &rec, isdmrg, map, pos);
+                                   part, rec, isdmrg, map, pos);
             } else if (MODE == kModeGlobal) {
                 T* gw1 = scratch + (int64_t)blockIdx.x * scratch_stride;
                 pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, gw1, Y,
-                                   part, &rec, isdmrg, map, pos);
+                                   part, rec, isdmrg, map, pos);
             } else {
                 const T* X = x + P.x_off;
                 const int xsz = P.d[0] * P.d[1] * P.d[2];
                 for (int i = threadIdx.x; i < xsz; i += NT) lds[i] = X[i];
                 __syncthreads();
                 pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, lds,
-                                   lds + P.w1_off, Y, part, &rec, isdmrg, map, pos);
+                                   lds + P.w1_off, Y, part, rec, isdmrg, map, pos);
             }
         }
     }
