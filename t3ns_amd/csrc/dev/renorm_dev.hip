@@ -165,10 +165,8 @@ namespace t3 {
 namespace {
 
 using tile::GemmTile;
-using tile::kKStep;
 using tile::kTile;
 using tile::Strided;
-using tile::tile_slab;
 using tile::tiles_of;
 
 // Phase 2: U tile = sum_t pref_t * R_t S_t.
@@ -191,34 +189,66 @@ __global__ void __launch_bounds__(256)
     pupdate_sum_kernel(const SumTile* __restrict__ tiles, int ntiles,
                        const SumTask* __restrict__ tasks, const double* __restrict__ base,
                        double* __restrict__ out, unsigned int* __restrict__ keep) {
-    __shared__ double Ps[kKStep][kTile + 1];
-    __shared__ double Qs[kKStep][kTile + 1];
+    __shared__ double Ps[tile::kKStep][kTile + 1];
+    __shared__ double Qs[tile::kKStep][kTile + 1];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     for (int it = blockIdx.x; it < ntiles; it += gridDim.x) {
         const SumTile g = tiles[it];
-        const int i0 = g.ti * kTile, j0 = g.tj * kTile;
-        double u[4][4] = {};
-        for (int t = g.task_begin; t < g.task_end; ++t) {
-            const SumTask tk = tasks[t];
-            double acc[4][4] = {};
-            for (int k0 = 0; k0 < tk.K; k0 += kKStep)
-                tile_slab(base, tk.r, tk.s, g.M, g.N, tk.K, i0, j0, k0, Ps, Qs, acc);
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) u[a][b] = fma(tk.pref, acc[a][b], u[a][b]);
-        }
-        const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+        const int ti0 = g.ti * kTile, j0 = g.tj * kTile;
         int any = 0;
+        // full tiles: LDS slabs, vector FMAs (every contribution has the
+        // tile's M and N; the choice is made once per tile on its first K)
+        const bool staged = g.task_end > g.task_begin &&
+                            tile::tile_staged(g.M, g.N, tasks[g.task_begin].K, ti0, j0);
+        if (staged) {
+            double u[4][4] = {};
+            for (int t = g.task_begin; t < g.task_end; ++t) {
+                const SumTask tk = tasks[t];
+                double acc[4][4] = {};
+                for (int k0 = 0; k0 < tk.K; k0 += tile::kKStep)
+                    tile::tile_slab(base, tk.r, tk.s, g.M, g.N, tk.K, ti0, j0, k0, Ps, Qs, acc);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int j = j0 + ty + 16 * b;
-            if (j >= g.N) continue;
+                for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const int i = i0 + tx + 16 * a;
-                if (i < g.M) {
-                    out[g.u_off + i + (int64_t)j * g.M] = u[a][b];
-                    any |= fabs(u[a][b]) >= 1e-20;
+                    for (int b = 0; b < 4; ++b) u[a][b] = fma(tk.pref, acc[a][b], u[a][b]);
+            }
+            const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int j = j0 + ty + 16 * b;
+                if (j >= g.N) continue;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int i = ti0 + tx + 16 * a;
+                    if (i < g.M) {
+                        out[g.u_off + i + (int64_t)j * g.M] = u[a][b];
+                        any |= fabs(u[a][b]) >= 1e-20;
+                    }
+                }
+            }
+        } else {
+            // other tiles: each wave's 16-row strip on the matrix cores, the
+            // prefactor scaling the left operand of each contribution
+            const int i0 = ti0 + 16 * w, nq = tile::tile_nq(g.N, j0);
+            if (i0 < g.M) {
+                tile::v4d u[4] = {};
+                for (int t = g.task_begin; t < g.task_end; ++t) {
+                    const SumTask tk = tasks[t];
+                    tile::wave_strip(base, tk.r, tk.s, g.M, g.N, tk.K, i0, j0, nq, tk.pref, u);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int j = j0 + 16 * q + (lane & 15);
+                    if (q >= nq || j >= g.N) continue;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int i = i0 + (lane >> 4) + 4 * v;
+                        if (i < g.M) {
+                            out[g.u_off + i + (int64_t)j * g.M] = u[q][v];
+                            any |= fabs(u[q][v]) >= 1e-20;
+                        }
+                    }
                 }
             }
         }
