@@ -38,7 +38,10 @@ def compile_one(src, is_hip):
     if is_hip:
         cmd = [HIPCC] + CXXFLAGS + HIPFLAGS + ["-x", "hip" if src.endswith(".hip") else "c++", "-c", src, "-o", obj]
         if not src.endswith(".hip"):
-            cmd = [HIPCC] + CXXFLAGS + ["-c", src, "-o", obj]
+            # host-only C++ that uses the HIP runtime API: no device pass
+            rocm_inc = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(HIPCC))), "include")
+            cmd = [HIPCC] + CXXFLAGS + ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I" + rocm_inc,
+                                        "-c", src, "-o", obj]
     else:
         cmd = ["g++"] + CXXFLAGS + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -604,6 +604,7 @@ struct t3ns_qr {
     std::vector<double> sing;
 };
 
+extern "C++" {  // C++ helper inside the C-ABI block
 namespace {
 SiteTensor site_from(int nrblocks, const int64_t* qn, const int64_t* bb, const double* tel) {
     if (nrblocks < 0 || (nrblocks && (!qn || !bb || !tel))) throw std::invalid_argument("bad site tensor");
@@ -617,6 +618,7 @@ SiteTensor site_from(int nrblocks, const int64_t* qn, const int64_t* bb, const d
     return t;
 }
 }  // namespace
+}  // extern "C++"
 
 int t3ns_qr_step(const t3ns_qr_desc* d, t3ns_qr** out, t3ns_work* work) {
     if (!d || !out) return fail(T3NS_ERR_INVALID, "null argument");
