@@ -604,10 +604,18 @@ __device__ __forceinline__ bool use_rows(const Geo& g) {
 // Operator offsets, prefactors and first-stage slots of one chunk, staged
 // in LDS at the start of the chunk.
 struct ChunkRec {
-    int64_t opa[kMaxChunkTerms], opb[kMaxChunkTerms], opc[kMaxChunkTerms];
-    double alpha[kMaxChunkTerms];
-    int slot[kMaxChunkTerms];
+    int64_t opa[kMaxChunkTerms];  // per run
+    int64_t opb[kMaxChunkTerms];  // per term
+    int64_t opc[kMaxChunkTerms];  // per group
+    double alpha[kMaxChunkTerms];  // per term
+    int slot[kMaxChunkTerms];      // per term
+    int cbeg[kMaxChunkTerms + 1];  // terms of group g: [cbeg[g], cbeg[g + 1])
 };
+
+// Stage B sums groups of terms only in the FP32 kernel (the host groups
+// terms on that path only).
+template <typename T>
+constexpr bool kGroupB = sizeof(T) == 4;
 
 // Terms t0, t0 + G, ... (at most kTU of them) of a chunk as stage C inputs.
 template <typename T>
@@ -740,6 +748,7 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
         T* const W2 = W1 + uni(C.w2_off);  // this chunk's second-stage outputs
         const int nr = uni(C.run_end - C.run_begin);
         const int nt = uni(C.term_end - C.term_begin);
+        const int ncs = uni(C.ncs);  // stage C items (groups of terms)
         // chunk records in LDS (double-buffered; the first chunk of a pair
         // loads its own, the others were loaded during the previous chunk)
         ChunkRec* const rec = recs + ((ci - cb) & 1);
@@ -747,9 +756,13 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             if (threadIdx.x < nt) {
                 const DevTerm& tm = terms[uni(C.term_begin) + threadIdx.x];
                 rec->opb[threadIdx.x] = tm.op[b];
-                rec->opc[threadIdx.x] = tm.op[c];
                 rec->alpha[threadIdx.x] = tm.alpha;
                 rec->slot[threadIdx.x] = tm.slot;
+            }
+            if (threadIdx.x <= ncs) {
+                const int g0 = runs[uni(C.run_end) + threadIdx.x];
+                rec->cbeg[threadIdx.x] = g0;
+                if (threadIdx.x < ncs) rec->opc[threadIdx.x] = terms[uni(C.term_begin) + g0].op[c];
             }
             if (threadIdx.x < nr) rec->opa[threadIdx.x] = terms[runs[uni(C.run_begin) + threadIdx.x]].op[a];
             __syncthreads();
@@ -796,11 +809,17 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
         if (!isdmrg && vB) {  // stage B, vector unit
             for (int task = wave; task < tB.total; task += NW) {
                 const VLane L = vlane(gB, task % tB.nft);
-                for (int t = task / tB.nft; t < nt; t += tB.G) {
+                for (int g = task / tB.nft; g < ncs; g += tB.G) {
                     V acc = V{0, 0, 0, 0};
-                    vmac<T, true>(W1 + uni(rec->slot[t]) * w1, opbb + uni64(rec->opb[t]), gB, L,
-                                  (T)rec->alpha[t], acc);
-                    vput<T>(W2 + t * w2, gB, L, acc, false);
+                    if constexpr (kGroupB<T>) {
+                        for (int t = uni(rec->cbeg[g]); t < uni(rec->cbeg[g + 1]); ++t)
+                            vmac<T, true>(W1 + uni(rec->slot[t]) * w1, opbb + uni64(rec->opb[t]), gB,
+                                          L, (T)rec->alpha[t], acc);
+                    } else {  // one term per group
+                        vmac<T, true>(W1 + uni(rec->slot[g]) * w1, opbb + uni64(rec->opb[g]), gB, L,
+                                      (T)rec->alpha[g], acc);
+                    }
+                    vput<T>(W2 + g * w2, gB, L, acc, false);
                 }
             }
             __syncthreads();
@@ -809,13 +828,19 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             for (int task = wave; task < kB.total; task += NW) {
                 const int r = task % base, grp = task / base;
                 const RowBlk L = row_blk<T>(gB, (r % kB.tf) * kTF, r / kB.tf);
-                for (int t = grp; t < nt; t += kB.G) {
+                for (int g = grp; g < ncs; g += kB.G) {
                     V acc[kTF];
 #pragma unroll
                     for (int q = 0; q < kTF; ++q) acc[q] = V{0, 0, 0, 0};
-                    blk_mac<T, true>(W1 + rec->slot[t] * w1, opbb + rec->opb[t], gB, L,
-                                     (T)rec->alpha[t], acc);
-                    blk_put<T>(W2 + t * w2, gB, L, acc, false);
+                    if constexpr (kGroupB<T>) {
+                        for (int t = uni(rec->cbeg[g]); t < uni(rec->cbeg[g + 1]); ++t)
+                            blk_mac<T, true>(W1 + rec->slot[t] * w1, opbb + rec->opb[t], gB, L,
+                                             (T)rec->alpha[t], acc);
+                    } else {  // one term per group
+                        blk_mac<T, true>(W1 + rec->slot[g] * w1, opbb + rec->opb[g], gB, L,
+                                         (T)rec->alpha[g], acc);
+                    }
+                    blk_put<T>(W2 + g * w2, gB, L, acc, false);
                 }
             }
             __syncthreads();
@@ -824,13 +849,19 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             for (int task = wave; task < kB.total; task += NW) {
                 const int r = task % base, grp = task / base;
                 const RowLane L = row_lane<T>(gB, r % kB.tf, (r / kB.tf) * kTN);
-                for (int t = grp; t < nt; t += kB.G) {
+                for (int g = grp; g < ncs; g += kB.G) {
                     V acc[kTN];
 #pragma unroll
                     for (int q = 0; q < kTN; ++q) acc[q] = V{0, 0, 0, 0};
-                    row_mac<T, true>(W1 + rec->slot[t] * w1, opbb + rec->opb[t], gB, L,
-                                     (T)rec->alpha[t], acc);
-                    row_put<T>(W2 + t * w2, gB, L, acc, false);
+                    if constexpr (kGroupB<T>) {
+                        for (int t = uni(rec->cbeg[g]); t < uni(rec->cbeg[g + 1]); ++t)
+                            row_mac<T, true>(W1 + rec->slot[t] * w1, opbb + rec->opb[t], gB, L,
+                                             (T)rec->alpha[t], acc);
+                    } else {  // one term per group
+                        row_mac<T, true>(W1 + rec->slot[g] * w1, opbb + rec->opb[g], gB, L,
+                                         (T)rec->alpha[g], acc);
+                    }
+                    row_put<T>(W2 + g * w2, gB, L, acc, false);
                 }
             }
             __syncthreads();
@@ -841,17 +872,21 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
         const bool pf = ci + 1 < ce;
         int64_t pfb = 0, pfc = 0, pfa = 0;
         double pfal = 0;
-        int pfsl = 0, pfnt = 0, pfnr = 0;
+        int pfsl = 0, pfnt = 0, pfnr = 0, pfnc = 0, pfg = 0;
         if (pf) {
             const DevChunk C2 = chunks[ci + 1];
             pfnt = uni(C2.term_end - C2.term_begin);
             pfnr = uni(C2.run_end - C2.run_begin);
+            pfnc = uni(C2.ncs);
             if (threadIdx.x < pfnt) {
                 const DevTerm& tm = terms[uni(C2.term_begin) + threadIdx.x];
                 pfb = tm.op[b];
-                pfc = tm.op[c];
                 pfal = tm.alpha;
                 pfsl = tm.slot;
+            }
+            if (threadIdx.x <= pfnc) {
+                pfg = runs[uni(C2.run_end) + threadIdx.x];
+                if (threadIdx.x < pfnc) pfc = terms[uni(C2.term_begin) + pfg].op[c];
             }
             if (threadIdx.x < pfnr) pfa = terms[runs[uni(C2.run_begin) + threadIdx.x]].op[a];
         }
@@ -865,13 +900,13 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             };
             if (pCv) {
                 if (wave < tC.total)
-                    for (int t = grpV; t < nt; t += tC.G) cmac(t, LV, accC[0]);
+                    for (int t = grpV; t < ncs; t += tC.G) cmac(t, LV, accC[0]);
             } else {
                 for (int task = wave; task < tC.total; task += NW) {
                     const int ft = task % tC.nft, grp = task / tC.nft;
                     const VLane L = vlane(gC, ft);
                     V acc = V{0, 0, 0, 0};
-                    for (int t = grp; t < nt; t += tC.G) cmac(t, L, acc);
+                    for (int t = grp; t < ncs; t += tC.G) cmac(t, L, acc);
                     if (tC.G == 1) vput<T>(Y, gC, L, acc, true);
                     else vstore_part(acc, grp, ft);
                 }
@@ -884,8 +919,8 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                 V acc[kTF];
 #pragma unroll
                 for (int q = 0; q < kTF; ++q) acc[q] = V{0, 0, 0, 0};
-                for (int t0 = 0; t0 < nt; t0 += kTU) {
-                    const Terms<T> R = c_terms<T>(rec, t0, 1, nt, isdmrg, W1, w1, W2, w2, opc);
+                for (int t0 = 0; t0 < ncs; t0 += kTU) {
+                    const Terms<T> R = c_terms<T>(rec, t0, 1, ncs, isdmrg, W1, w1, W2, w2, opc);
                     if (isdmrg) blk_mac_terms<T, true>(R, gC, L, acc);
                     else blk_mac_terms<T, false>(R, gC, L, acc);
                 }
@@ -893,8 +928,8 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             }
         } else if (pC) {  // stage C into the pair's persistent accumulators
             if (wave < kC.total) {
-                for (int t0 = grpC; t0 < nt; t0 += kC.G * kTU) {
-                    const Terms<T> R = c_terms<T>(rec, t0, kC.G, nt, isdmrg, W1, w1, W2, w2, opc);
+                for (int t0 = grpC; t0 < ncs; t0 += kC.G * kTU) {
+                    const Terms<T> R = c_terms<T>(rec, t0, kC.G, ncs, isdmrg, W1, w1, W2, w2, opc);
                     if (isdmrg) row_mac_terms<T, true>(R, gC, LC, accC);
                     else row_mac_terms<T, false>(R, gC, LC, accC);
                 }
@@ -906,8 +941,8 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                 V acc[kTN];
 #pragma unroll
                 for (int q = 0; q < kTN; ++q) acc[q] = V{0, 0, 0, 0};
-                for (int t0 = grp; t0 < nt; t0 += kC.G * kTU) {
-                    const Terms<T> R = c_terms<T>(rec, t0, kC.G, nt, isdmrg, W1, w1, W2, w2, opc);
+                for (int t0 = grp; t0 < ncs; t0 += kC.G * kTU) {
+                    const Terms<T> R = c_terms<T>(rec, t0, kC.G, ncs, isdmrg, W1, w1, W2, w2, opc);
                     if (isdmrg) row_mac_terms<T, true>(R, gC, L, acc);
                     else row_mac_terms<T, false>(R, gC, L, acc);
                 }
@@ -920,9 +955,12 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             ChunkRec* nx = recs + ((ci + 1 - cb) & 1);
             if (threadIdx.x < pfnt) {
                 nx->opb[threadIdx.x] = pfb;
-                nx->opc[threadIdx.x] = pfc;
                 nx->alpha[threadIdx.x] = pfal;
                 nx->slot[threadIdx.x] = pfsl;
+            }
+            if (threadIdx.x <= pfnc) {
+                nx->cbeg[threadIdx.x] = pfg;
+                if (threadIdx.x < pfnc) nx->opc[threadIdx.x] = pfc;
             }
             if (threadIdx.x < pfnr) nx->opa[threadIdx.x] = pfa;
         }
@@ -1185,16 +1223,47 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
             DevChunk C{};
             int nruns = 0, nterms = 0;
             int64_t prev = -1;
+            // last operators of the chunk's groups.  Terms are grouped on the
+            // FP32 path only; FP64 and DMRG objects keep one group per term
+            // (the FP64 kernel's stage B has no register room for the
+            // group loops).
+            const bool group = !isdmrg_ && f32;
+            const int last = isdmrg_ ? P.ord[1] : P.ord[2];
+            std::vector<int64_t> cops;
+            auto group_of = [&](const DevTerm& tm) {
+                if (!group) return -1;
+                for (size_t g = 0; g < cops.size(); ++g)
+                    if (cops[g] == tm.op[last]) return (int)g;
+                return -1;
+            };
             auto close = [&](int end) {
                 C.term_end = end;
                 C.run_end = (int)hr.size();
-                C.w2_off = P.wlds ? nruns * P.w1 : (int)gcap;
+                const int nt = end - C.term_begin;
+                C.ncs = group ? (int)cops.size() : nt;
+                // terms in group order (stable: each group keeps the order
+                // of its terms); run leads re-pointed at a term of the run
+                std::stable_sort(ht.begin() + C.term_begin, ht.begin() + end,
+                                 [](const DevTerm& x, const DevTerm& y) { return x.cslot < y.cslot; });
+                for (int r = 0; r < nruns; ++r) hr[C.run_begin + r] = -1;
+                for (int t = C.term_begin; t < end; ++t)
+                    if (hr[C.run_begin + ht[t].slot] < 0) hr[C.run_begin + ht[t].slot] = t;
+                for (int g = 0, t = 0; g <= C.ncs; ++g) {
+                    while (t < nt && ht[C.term_begin + t].cslot < g) ++t;
+                    hr.push_back(t);
+                }
+                const int64_t r1 = (int64_t)nruns * P.w1;
+                C.w2_off = P.wlds ? (int)r1 : (int)gcap;
                 hc.push_back(C);
             };
             for (size_t t = tb; t < ht.size(); ++t) {
                 const bool newrun = nterms == 0 || ht[t].op[first] != prev;
+                const bool newc = !group || nterms == 0 || group_of(ht[t]) < 0;
                 const int64_t r1 = (int64_t)(nruns + newrun) * P.w1;
-                const int64_t r2 = isdmrg_ ? 0 : (int64_t)(nterms + 1) * P.w2;
+                const int64_t r2 =
+                    isdmrg_ ? 0
+                            : (int64_t)((nterms == 0 ? 0 : (group ? (int)cops.size() : nterms)) + newc) *
+                                  P.w2;
                 const bool fits = nterms < kMaxChunkTerms &&
                                   (P.wlds ? r1 + r2 <= avail : r1 <= gcap && r2 <= gcap);
                 if (nterms > 0 && !fits) {
@@ -1205,12 +1274,23 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
                     C = DevChunk{};
                     C.term_begin = (int)t;
                     C.run_begin = (int)hr.size();
+                    cops.clear();
                 }
                 if (nterms == 0 || ht[t].op[first] != prev) {
                     hr.push_back((int)t);
                     ++nruns;
                 }
                 ht[t].slot = nruns - 1;
+                if (!group) {
+                    ht[t].cslot = nterms;
+                } else {
+                    int g = group_of(ht[t]);
+                    if (g < 0) {
+                        g = (int)cops.size();
+                        cops.push_back(ht[t].op[last]);
+                    }
+                    ht[t].cslot = g;
+                }
                 prev = ht[t].op[first];
                 ++nterms;
             }
@@ -1228,14 +1308,15 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
                     st[s2] = std::ceil(F / 16) * std::ceil(B.d[o] / 16.0) * std::ceil(cd[o] / 4.0);
                     cd[o] = B.d[o];
                 }
-                double nr = 0, ntt = 0;
+                double nr = 0, ntt = 0, ncs = 0;
                 for (int ci = P.chunk_begin; ci < P.chunk_end; ++ci) {
                     nr += hc[ci].run_end - hc[ci].run_begin;
                     ntt += hc[ci].term_end - hc[ci].term_begin;
+                    ncs += hc[ci].ncs;
                 }
                 stats_.stage_steps[0] += nr * st[0];
                 stats_.stage_steps[1] += ntt * st[1];
-                if (!isdmrg_) stats_.stage_steps[2] += ntt * st[2];
+                if (!isdmrg_) stats_.stage_steps[2] += ncs * st[2];
                 stats_.nruns += (int64_t)nr;
                 stats_.pairs_global_x += !P.xlds;
                 stats_.pairs_global_w += !P.wlds;
@@ -1246,7 +1327,7 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
                     const double nr = hc[ci].run_end - hc[ci].run_begin;
                     const double nt = hc[ci].term_end - hc[ci].term_begin;
                     c += nr * xsz * B.d[P.ord[0]] + nt * (double)P.w1 * B.d[P.ord[1]] +
-                         (isdmrg_ ? 0.0 : nt * (double)P.w2 * B.d[P.ord[2]]);
+                         (isdmrg_ ? 0.0 : hc[ci].ncs * (double)P.w2 * B.d[P.ord[2]]);
                 }
                 cost[sb] += c;
                 pcost.push_back(c);

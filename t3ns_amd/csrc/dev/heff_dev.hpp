@@ -42,16 +42,23 @@ struct DevPair {       // one (output block, input block) pair
 };
 // Consecutive terms of a pair processed between two barriers: their
 // distinct first-stage operators ("runs") and the terms themselves.
+// Terms sharing their last operator form a group: stage B sums the group's
+// terms into one second-stage output and stage C applies the shared
+// operator once per group.  A chunk's terms are ordered by group.  (Terms
+// are grouped on the FP32 path; FP64 and DMRG objects have one group per
+// term.)
 struct DevChunk {
     int term_begin, term_end;
-    int run_begin, run_end;  // into the run-lead list
+    int run_begin, run_end;  // into the run-lead list; the ncs + 1 group
+                             // offsets (terms, chunk-relative) follow at run_end
+    int ncs;                 // groups
     int w2_off;              // second-stage outputs: offset from the first-stage outputs
 };
 struct DevTerm {       // one instruction of one MPO group of a pair
     int64_t op[3];     // element offsets into the three operator arenas
     double alpha;      // instruction prefactor x symmetry prefactor
     int slot;          // first-stage output slot (run index within its chunk)
-    int pad;
+    int cslot;         // group of its last operator within its chunk
 };
 struct DevDiagBlock {
     int64_t off;
