@@ -128,12 +128,8 @@ __device__ __forceinline__ RowLane row_lane(const Geo& g, int tf, int tn0) {
     return L;
 }
 
-// acc[j] += (scale S)[row tile, :] O[column tile tn0 + j, :]^T over the whole K,
-// j < NTN.  Loads are unconditional (clamped indices stay inside the
-// blocks) so the compiler can keep several in flight; only the K tail is
-// masked.
-// One slab of at most NU k-steps (ns of them live, ns >= 1): loads first,
-// then the matrix-core steps.
+// One slab of at most NU k-steps (ns of them live, ns >= 1) of row_mac_n:
+// loads first, then the matrix-core steps.
 template <typename T, int NTN, int NU, bool SCALE>
 __device__ __forceinline__ void row_slab(const T* sp, const T* op, const int (&coff)[NTN], int sst,
                                          int ost, int ns, T scale,
@@ -158,6 +154,10 @@ __device__ __forceinline__ void row_slab(const T* sp, const T* op, const int (&c
     }
 }
 
+// acc[j] += (scale S)[row tile, :] O[column tile tn0 + j, :]^T over the whole K,
+// j < NTN.  Loads are unconditional (clamped indices stay inside the
+// blocks) so the compiler can keep several in flight; only the K tail is
+// masked.
 template <typename T, int NTN, bool SCALE>
 __device__ __forceinline__ void row_mac_n(const T* __restrict__ S, const T* __restrict__ O,
                                           const Geo& g, const RowLane& L, T scale,
@@ -593,16 +593,9 @@ __device__ __forceinline__ bool use_rows(const Geo& g) {
     return ((g.dn + 15) >> 4) <= 2 && g.tf >= 2;
 }
 
-// All chunks of one pair.  Stage A: first-stage outputs of the chunk's
-// distinct first operators; stage B: second-stage output of every term;
-// stage C: Y += sum over the chunk's terms of alpha_t O_c x_c W2_t, summed
-// in matrix-core accumulators per output tile.  Waves take (tile, group)
-// tasks: a wave keeps its tile's lane offsets and walks the operators or
-// terms of its group; when a geometry has fewer tiles than waves, items
-// are split over groups (stage C partial tiles are then reduced in a fixed
-// order).
-// Operator offsets, prefactors and first-stage slots of one chunk, staged
-// in LDS at the start of the chunk.
+// Operator offsets, prefactors, first-stage slots and term groups of one
+// chunk, staged in LDS (double-buffered: the next chunk's are loaded while
+// this one runs).
 struct ChunkRec {
     int64_t opa[kMaxChunkTerms];  // per run
     int64_t opb[kMaxChunkTerms];  // per term
@@ -638,6 +631,15 @@ __device__ __forceinline__ Terms<T> c_terms(const ChunkRec* rec, int t0, int G, 
     return R;
 }
 
+// All chunks of one pair.  Stage A: first-stage outputs of the chunk's
+// distinct first operators; stage B: second-stage output of every term
+// group; stage C: Y += sum over the groups of O_c x_c W2_g.  Each stage
+// picks its engine from its geometry (vector unit for output legs of at
+// most kVN states, else matrix cores, row-blocked for narrow outputs).
+// Waves take (tile, item group) tasks: a wave keeps its tile's lane offsets
+// and walks the items of its group; when a geometry has fewer tiles than
+// waves, items are split over groups (stage C partial tiles are then
+// reduced in a fixed order).
 template <typename T, int NT>
 __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                                             const DevChunk* __restrict__ chunks,
