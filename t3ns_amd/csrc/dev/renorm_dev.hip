@@ -185,6 +185,73 @@ struct SumTask {
     double pref;
 };
 
+// Output blocks of at most kDotEdge x kDotEdge states (a 16 x 16 strip
+// would be mostly padding) take their own kernel: the whole workgroup
+// splits the (contribution, k) reduction over its threads, neighbouring
+// threads taking neighbouring k, and sums the per-thread partials in a
+// fixed order (within each wave, then over the waves).
+constexpr int kDotEdge = 8;
+__host__ __device__ inline bool dot_tile(int M, int N) { return M <= kDotEdge && N <= kDotEdge; }
+
+__device__ __forceinline__ double wave_total(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(256)
+    pupdate_dot_kernel(const SumTile* __restrict__ tiles, int ntiles,
+                       const SumTask* __restrict__ tasks, const double* __restrict__ base,
+                       double* __restrict__ out, unsigned int* __restrict__ keep) {
+    __shared__ double red[4][kDotEdge * kDotEdge];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int it = blockIdx.x; it < ntiles; it += gridDim.x) {
+        const SumTile g = tiles[it];
+        double acc[kDotEdge][kDotEdge] = {};
+        for (int t = g.task_begin; t < g.task_end; ++t) {
+            const SumTask tk = tasks[t];
+            int64_t ro[kDotEdge], so[kDotEdge];  // rows / columns past the block repeat the last
+#pragma unroll
+            for (int a = 0; a < kDotEdge; ++a) {
+                ro[a] = tk.r.off + (int64_t)min(a, g.M - 1) * tk.r.si;
+                so[a] = tk.s.off + (int64_t)min(a, g.N - 1) * tk.s.sk;
+            }
+            for (int k = threadIdx.x; k < tk.K; k += 256) {
+                double r[kDotEdge], q[kDotEdge];
+#pragma unroll
+                for (int a = 0; a < kDotEdge; ++a) {
+                    r[a] = base[ro[a] + (int64_t)k * tk.r.sk] * tk.pref;
+                    q[a] = base[so[a] + (int64_t)k * tk.s.si];
+                }
+#pragma unroll
+                for (int a = 0; a < kDotEdge; ++a)
+#pragma unroll
+                    for (int b = 0; b < kDotEdge; ++b) acc[a][b] = fma(r[a], q[b], acc[a][b]);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < kDotEdge; ++a)
+#pragma unroll
+            for (int b = 0; b < kDotEdge; ++b) {
+                const double v = wave_total(acc[a][b]);
+                if (lane == 0) red[w][a * kDotEdge + b] = v;
+            }
+        __syncthreads();
+        int any = 0;
+        if (threadIdx.x < kDotEdge * kDotEdge) {
+            const int a = threadIdx.x / kDotEdge, b = threadIdx.x % kDotEdge;
+            const double v = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) +
+                             red[3][threadIdx.x];
+            if (a < g.M && b < g.N) {
+                out[g.u_off + a + (int64_t)b * g.M] = v;
+                any = fabs(v) >= 1e-20;
+            }
+        }
+        any = __syncthreads_or(any);  // also keeps red until every thread has read it
+        if (threadIdx.x == 0 && any) atomicOr(&keep[g.flat], 1u);
+    }
+}
+
 __global__ void __launch_bounds__(256)
     pupdate_sum_kernel(const SumTile* __restrict__ tiles, int ntiles,
                        const SumTask* __restrict__ tasks, const double* __restrict__ base,
@@ -193,68 +260,107 @@ __global__ void __launch_bounds__(256)
     __shared__ double Qs[tile::kKStep][kTile + 1];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    for (int it = blockIdx.x; it < ntiles; it += gridDim.x) {
-        const SumTile g = tiles[it];
-        const int ti0 = g.ti * kTile, j0 = g.tj * kTile;
+    // one wave's strips of tile g (rows i0 .. of the tile), the prefactor
+    // scaling the left operand of each contribution; returns "any kept"
+    auto strips = [&](const SumTile& g, int i_first, int i_step) {
+        const int ti0 = g.ti * kTile, j0 = g.tj * kTile, nq = tile::tile_nq(g.N, j0);
         int any = 0;
-        // full tiles: LDS slabs, vector FMAs (every contribution has the
-        // tile's M and N; the choice is made once per tile on its first K)
-        const bool staged = g.task_end > g.task_begin &&
-                            tile::tile_staged(g.M, g.N, tasks[g.task_begin].K, ti0, j0);
-        if (staged) {
-            double u[4][4] = {};
+        for (int i0 = ti0 + i_first; i0 < min(g.M, ti0 + kTile); i0 += i_step) {
+            tile::v4d u[4] = {};
             for (int t = g.task_begin; t < g.task_end; ++t) {
                 const SumTask tk = tasks[t];
-                double acc[4][4] = {};
-                for (int k0 = 0; k0 < tk.K; k0 += tile::kKStep)
-                    tile::tile_slab(base, tk.r, tk.s, g.M, g.N, tk.K, ti0, j0, k0, Ps, Qs, acc);
-#pragma unroll
-                for (int a = 0; a < 4; ++a)
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) u[a][b] = fma(tk.pref, acc[a][b], u[a][b]);
+                tile::wave_strip(base, tk.r, tk.s, g.M, g.N, tk.K, i0, j0, nq, tk.pref, u);
             }
-            const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int j = j0 + ty + 16 * b;
-                if (j >= g.N) continue;
+            for (int q = 0; q < 4; ++q) {
+                const int j = j0 + 16 * q + (lane & 15);
+                if (q >= nq || j >= g.N) continue;
 #pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    const int i = ti0 + tx + 16 * a;
+                for (int v = 0; v < 4; ++v) {
+                    const int i = i0 + (lane >> 4) + 4 * v;
                     if (i < g.M) {
-                        out[g.u_off + i + (int64_t)j * g.M] = u[a][b];
-                        any |= fabs(u[a][b]) >= 1e-20;
-                    }
-                }
-            }
-        } else {
-            // other tiles: each wave's 16-row strip on the matrix cores, the
-            // prefactor scaling the left operand of each contribution
-            const int i0 = ti0 + 16 * w, nq = tile::tile_nq(g.N, j0);
-            if (i0 < g.M) {
-                tile::v4d u[4] = {};
-                for (int t = g.task_begin; t < g.task_end; ++t) {
-                    const SumTask tk = tasks[t];
-                    tile::wave_strip(base, tk.r, tk.s, g.M, g.N, tk.K, i0, j0, nq, tk.pref, u);
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int j = j0 + 16 * q + (lane & 15);
-                    if (q >= nq || j >= g.N) continue;
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int i = i0 + (lane >> 4) + 4 * v;
-                        if (i < g.M) {
-                            out[g.u_off + i + (int64_t)j * g.M] = u[q][v];
-                            any |= fabs(u[q][v]) >= 1e-20;
-                        }
+                        out[g.u_off + i + (int64_t)j * g.M] = u[q][v];
+                        any |= fabs(u[q][v]) >= 1e-20;
                     }
                 }
             }
         }
-        any = __syncthreads_or(any);
-        if (threadIdx.x == 0 && any) atomicOr(&keep[g.flat], 1u);
+        return any;
+    };
+    for (int it = blockIdx.x * 4; it < ntiles; it += gridDim.x * 4) {
+        // tiles taken four at a time: when all four take the per-wave
+        // strips, each wave computes one tile, else the workgroup computes
+        // them one after another
+        int mine = 0;
+        if (it + w < ntiles) {
+            const SumTile g = tiles[it + w];
+            mine = g.task_end > g.task_begin &&
+                   tile::tile_staged(g.M, g.N, tasks[g.task_begin].K, g.ti * kTile, g.tj * kTile);
+        }
+        if (!__syncthreads_or(mine)) {
+            if (it + w < ntiles) {
+                const SumTile g = tiles[it + w];
+                const int any = __any(strips(g, 0, 16));
+                if (lane == 0 && any) atomicOr(&keep[g.flat], 1u);
+            }
+            continue;
+        }
+        for (int ti = it; ti < min(it + 4, ntiles); ++ti) {
+            const SumTile g = tiles[ti];
+            const int ti0 = g.ti * kTile, j0 = g.tj * kTile;
+            int any = 0;
+            if (g.task_end > g.task_begin &&
+                tile::tile_staged(g.M, g.N, tasks[g.task_begin].K, ti0, j0)) {
+                // full tiles: LDS slabs, vector FMAs
+                double u[4][4] = {};
+                for (int t = g.task_begin; t < g.task_end; ++t) {
+                    const SumTask tk = tasks[t];
+                    double acc[4][4] = {};
+                    for (int k0 = 0; k0 < tk.K; k0 += tile::kKStep)
+                        tile::tile_slab(base, tk.r, tk.s, g.M, g.N, tk.K, ti0, j0, k0, Ps, Qs, acc);
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) u[a][b] = fma(tk.pref, acc[a][b], u[a][b]);
+                }
+                const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int j = j0 + ty + 16 * b;
+                    if (j >= g.N) continue;
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        const int i = ti0 + tx + 16 * a;
+                        if (i < g.M) {
+                            out[g.u_off + i + (int64_t)j * g.M] = u[a][b];
+                            any |= fabs(u[a][b]) >= 1e-20;
+                        }
+                    }
+                }
+            } else {
+                any = strips(g, 16 * w, 64);  // the four waves' strips of the tile
+            }
+            any = __syncthreads_or(any);
+            if (threadIdx.x == 0 && any) atomicOr(&keep[g.flat], 1u);
+        }
     }
+}
+
+// Small-output tiles first (their count returned), then the others.
+int partition_sums(std::vector<SumTile>& sums) {
+    auto mid = std::stable_partition(sums.begin(), sums.end(),
+                                     [](const SumTile& g) { return dot_tile(g.M, g.N); });
+    return (int)(mid - sums.begin());
+}
+
+void launch_sums(const SumTile* d, int ndot, int n, const SumTask* tasks, const double* base,
+                 double* out, unsigned int* keep, hipStream_t s) {
+    if (ndot)
+        hipLaunchKernelGGL(pupdate_dot_kernel, dim3(grid(ndot)), dim3(256), 0, s, d, ndot, tasks, base,
+                           out, keep);
+    if (n > ndot)
+        hipLaunchKernelGGL(pupdate_sum_kernel, dim3(grid((n - ndot + 3) / 4)), dim3(256), 0, s, d + ndot,
+                           n - ndot, tasks, base, out, keep);
 }
 
 }  // namespace
@@ -358,6 +464,7 @@ ROperators run_pupdate(const PUpdatePlan& plan, const ROperators& pops, const Si
     DevBuf<double> dnew(std::max<int64_t>(1, nnew));
     DevBuf<unsigned int> dkeep(std::max(1, nflat));
     dfirst.upload(first, s);
+    const int ndot = partition_sums(sums);
     dsums.upload(sums, s);
     dtasks.upload(stasks, s);
     hipLaunchKernelGGL(fill_zero<unsigned int>, dim3(grid((nflat + 255) / 256)), dim3(256), 0, s,
@@ -369,10 +476,8 @@ ROperators run_pupdate(const PUpdatePlan& plan, const ROperators& pops, const Si
     if (!first.empty())
         hipLaunchKernelGGL(tile::gemm_tiles_kernel, dim3(grid((int64_t)first.size())), dim3(256), 0,
                            s, dfirst.get(), (int)first.size(), arena.get());
-    if (!sums.empty())
-        hipLaunchKernelGGL(pupdate_sum_kernel, dim3(grid((int64_t)sums.size())), dim3(256), 0, s,
-                           dsums.get(), (int)sums.size(), dtasks.get(), arena.get(), dnew.get(),
-                           dkeep.get());
+    launch_sums(dsums.get(), ndot, (int)sums.size(), dtasks.get(), arena.get(), dnew.get(),
+                dkeep.get(), s);
     T3_HIP_CHECK(hipEventRecord(e1, s));
     std::vector<unsigned int> keep32(nflat);
     if (nflat)
@@ -517,6 +622,7 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
         std::vector<Swap12> swaps;  // new operators on leg 1: K made contiguous
         std::vector<SumTile> sums;
         std::vector<SumTask> tasks;
+        int ndot = 0;  // small-output tiles, first in sums
     };
     std::vector<Chunk> chunks;
     double flops = 0;
@@ -631,6 +737,7 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
                 }
         }
         wmax = std::max(wmax, wtop);
+        c.ndot = partition_sums(c.sums);
         chunks.push_back(std::move(c));
     }
 
@@ -654,7 +761,7 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
         DevBuf<SumTask> dt;
         d1.upload(c.p1, s);
         d2.upload(c.p2, s);
-        ds.upload(c.sums, s);
+        ds.upload(c.sums, s);  // small-output tiles first (partitioned at chunk close)
         dt.upload(c.tasks, s);
         if (!c.p1.empty())
             hipLaunchKernelGGL(tile::gemm_tiles_kernel, dim3(grid((int64_t)c.p1.size())), dim3(256), 0,
@@ -667,10 +774,8 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
         if (!c.swaps.empty())
             hipLaunchKernelGGL(swap12_kernel, dim3(grid((int64_t)c.swaps.size())), dim3(256), 0, s,
                                dw.get(), (int)c.swaps.size(), arena.get());
-        if (!c.sums.empty())
-            hipLaunchKernelGGL(pupdate_sum_kernel, dim3(grid((int64_t)c.sums.size())), dim3(256), 0, s,
-                               ds.get(), (int)c.sums.size(), dt.get(), arena.get(), arena.get(),
-                               dkeep.get());
+        launch_sums(ds.get(), c.ndot, (int)c.sums.size(), dt.get(), arena.get(), arena.get(),
+                    dkeep.get(), s);
         // the chunk's tile lists are freed at scope end: wait for the kernels
         T3_HIP_CHECK(hipStreamSynchronize(s));
     }

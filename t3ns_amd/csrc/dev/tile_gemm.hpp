@@ -138,44 +138,75 @@ __device__ __forceinline__ int tile_nq(int N, int j0) { return min(4, (N - j0 + 
 // launch must not overlap).  256 threads = four 16-row strips per tile,
 // grid-stride over the tiles.
 // Internal linkage: every translation unit that launches it has its copy.
+// Stores a wave's strip (rows i0 .. i0 + 15) of a tile.
+__device__ __forceinline__ void strip_store(double* __restrict__ x, int ldx, int M, int N, int i0,
+                                            int j0, int nq, const v4d acc[4]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int j = j0 + 16 * q + (lane & 15);
+        if (q >= nq || j >= N) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = i0 + (lane >> 4) + 4 * v;
+            if (i < M) x[i + (int64_t)j * ldx] = acc[q][v];
+        }
+    }
+}
+
+// X tiles = P Q, all operands in one arena (inputs and outputs of one
+// launch must not overlap).  256 threads; tiles are taken four at a time:
+// when none of the four takes the LDS slabs, each wave computes one tile
+// (all its strips), else the workgroup computes them one after another
+// (four strips at once).  Internal linkage: every translation unit that
+// launches it has its copy.
 static __global__ void __launch_bounds__(256)
     gemm_tiles_kernel(const GemmTile* __restrict__ tiles, int ntiles, double* base) {
     __shared__ double Ps[kKStep][kTile + 1];
     __shared__ double Qs[kKStep][kTile + 1];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    for (int it = blockIdx.x; it < ntiles; it += gridDim.x) {
-        const GemmTile g = tiles[it];
-        const int ti0 = g.ti * kTile, j0 = g.tj * kTile;
-        if (tile_staged(g.M, g.N, g.K, ti0, j0)) {
-            double acc[4][4] = {};
-            for (int k0 = 0; k0 < g.K; k0 += kKStep)
-                tile_slab(base, g.p, g.q, g.M, g.N, g.K, ti0, j0, k0, Ps, Qs, acc);
-            const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int j = j0 + ty + 16 * b;
-                if (j >= g.N) continue;
-#pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    const int i = ti0 + tx + 16 * a;
-                    if (i < g.M) base[g.x_off + i + (int64_t)j * g.ldx] = acc[a][b];
-                }
+    for (int it = blockIdx.x * 4; it < ntiles; it += gridDim.x * 4) {
+        int mine = 0;
+        if (it + w < ntiles) {
+            const GemmTile g = tiles[it + w];
+            mine = tile_staged(g.M, g.N, g.K, g.ti * kTile, g.tj * kTile);
+        }
+        if (!__syncthreads_or(mine)) {  // one tile per wave
+            if (it + w >= ntiles) continue;
+            const GemmTile g = tiles[it + w];
+            const int ti0 = g.ti * kTile, j0 = g.tj * kTile, nq = tile_nq(g.N, j0);
+            for (int i0 = ti0; i0 < min(g.M, ti0 + kTile); i0 += 16) {
+                v4d acc[4] = {};
+                wave_strip(base, g.p, g.q, g.M, g.N, g.K, i0, j0, nq, 1.0, acc);
+                strip_store(base + g.x_off, g.ldx, g.M, g.N, i0, j0, nq, acc);
             }
             continue;
         }
-        const int i0 = ti0 + 16 * w, nq = tile_nq(g.N, j0);
-        if (i0 >= g.M) continue;
-        v4d acc[4] = {};
-        wave_strip(base, g.p, g.q, g.M, g.N, g.K, i0, j0, nq, 1.0, acc);
+        for (int t = it; t < min(it + 4, ntiles); ++t) {
+            const GemmTile g = tiles[t];
+            const int ti0 = g.ti * kTile, j0 = g.tj * kTile;
+            if (tile_staged(g.M, g.N, g.K, ti0, j0)) {
+                double acc[4][4] = {};
+                for (int k0 = 0; k0 < g.K; k0 += kKStep)
+                    tile_slab(base, g.p, g.q, g.M, g.N, g.K, ti0, j0, k0, Ps, Qs, acc);
+                const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int j = j0 + 16 * q + (lane & 15);
-            if (q >= nq || j >= g.N) continue;
+                for (int b = 0; b < 4; ++b) {
+                    const int j = j0 + ty + 16 * b;
+                    if (j >= g.N) continue;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = i0 + (lane >> 4) + 4 * v;
-                if (i < g.M) base[g.x_off + i + (int64_t)j * g.ldx] = acc[q][v];
+                    for (int a = 0; a < 4; ++a) {
+                        const int i = ti0 + tx + 16 * a;
+                        if (i < g.M) base[g.x_off + i + (int64_t)j * g.ldx] = acc[a][b];
+                    }
+                }
+            } else {
+                const int i0 = ti0 + 16 * w, nq = tile_nq(g.N, j0);
+                if (i0 < g.M) {
+                    v4d acc[4] = {};
+                    wave_strip(base, g.p, g.q, g.M, g.N, g.K, i0, j0, nq, 1.0, acc);
+                    strip_store(base + g.x_off, g.ldx, g.M, g.N, i0, j0, nq, acc);
+                }
             }
         }
     }
