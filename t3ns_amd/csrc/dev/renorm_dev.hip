@@ -755,10 +755,14 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
     T3_HIP_CHECK(hipEventCreate(&e0));
     T3_HIP_CHECK(hipEventCreate(&e1));
     T3_HIP_CHECK(hipEventRecord(e0, s));
+    // device tile lists, grown to the largest chunk and reused (the
+    // per-chunk synchronisation below keeps an upload from overwriting lists
+    // still in use)
+    DevBuf<GemmTile> d1, d2;
+    DevBuf<SumTile> ds;
+    DevBuf<SumTask> dt;
+    DevBuf<Swap12> dw;
     for (const Chunk& c : chunks) {
-        DevBuf<GemmTile> d1, d2;
-        DevBuf<SumTile> ds;
-        DevBuf<SumTask> dt;
         d1.upload(c.p1, s);
         d2.upload(c.p2, s);
         ds.upload(c.sums, s);  // small-output tiles first (partitioned at chunk close)
@@ -769,14 +773,13 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
         if (!c.p2.empty())
             hipLaunchKernelGGL(tile::gemm_tiles_kernel, dim3(grid((int64_t)c.p2.size())), dim3(256), 0,
                                s, d2.get(), (int)c.p2.size(), arena.get());
-        DevBuf<Swap12> dw;
         dw.upload(c.swaps, s);
         if (!c.swaps.empty())
             hipLaunchKernelGGL(swap12_kernel, dim3(grid((int64_t)c.swaps.size())), dim3(256), 0, s,
                                dw.get(), (int)c.swaps.size(), arena.get());
         launch_sums(ds.get(), c.ndot, (int)c.sums.size(), dt.get(), arena.get(), arena.get(),
                     dkeep.get(), s);
-        // the chunk's tile lists are freed at scope end: wait for the kernels
+        // the next chunk's uploads reuse the lists: wait for the kernels
         T3_HIP_CHECK(hipStreamSynchronize(s));
     }
     T3_HIP_CHECK(hipEventRecord(e1, s));
