@@ -563,21 +563,24 @@ void mode_tiles(std::vector<GemmTile>& out, int64_t x, const int n[3], int leg, 
                 out.push_back(g);
             }
     } else {
-        for (int l = 0; l < n[2]; ++l) {
-            g.p = Strided{x + (int64_t)l * n[0] * n[1], 1, n[0]};
-            g.q = Strided{o, e, 1};
-            g.M = n[0];
-            g.N = e;
-            g.K = n[1];
-            g.x_off = y + (int64_t)l * n[0] * e;
-            g.ldx = n[0];
-            for (int a = 0; a < tiles_of(g.M); ++a)
-                for (int b = 0; b < tiles_of(g.N); ++b) {
-                    g.ti = a;
-                    g.tj = b;
-                    out.push_back(g);
-                }
-        }
+        // the n[2] slices as one product: row (i, l) at i + l n0 n1 of the
+        // input and i + l n0 e of the output
+        g.p = Strided{x, 1, n[0]};
+        g.p.lo = n[0];
+        g.p.hi = (int64_t)n[0] * n[1];
+        g.q = Strided{o, e, 1};
+        g.M = n[0] * n[2];
+        g.N = e;
+        g.K = n[1];
+        g.x_off = y;
+        g.ldx = n[0];
+        g.xhi = (int64_t)n[0] * e;
+        for (int a = 0; a < tiles_of(g.M); ++a)
+            for (int b = 0; b < tiles_of(g.N); ++b) {
+                g.ti = a;
+                g.tj = b;
+                out.push_back(g);
+            }
     }
     *flops += 2.0 * e * (double)n[0] * n[1] * n[2];
 }

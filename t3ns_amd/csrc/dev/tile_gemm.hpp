@@ -18,7 +18,18 @@ constexpr int kTile = 64;  // output tile edge
 struct Strided {
     int64_t off;
     int si, sk;
+    // optional two-level first index (batched products folded into one):
+    // i -> (i % lo) si + (i / lo) hi; lo = 0: i si
+    int lo = 0;
+    int64_t hi = 0;
 };
+// FIB: the two-level form is in use (a template argument, so that plain
+// tiles carry no division)
+template <bool FIB>
+__host__ __device__ inline int64_t first_off(const Strided& s, int i) {
+    if (FIB) return (int64_t)(i % s.lo) * s.si + (int64_t)(i / s.lo) * s.hi;
+    return (int64_t)i * s.si;
+}
 
 // One output tile of X = P Q (M x N, column-major with leading dimension
 // ldx at x_off); q is read as (k, j).
@@ -28,7 +39,13 @@ struct GemmTile {
     int M, N, K;
     int ti, tj;
     int ldx;
+    int64_t xhi = 0;  // rows of X follow p.lo: i -> (i % lo) + (i / lo) xhi
 };
+template <bool FIB>
+__host__ __device__ inline int64_t x_row(const GemmTile& g, int i) {
+    if (FIB) return (int64_t)(i % g.p.lo) + (int64_t)(i / g.p.lo) * g.xhi;
+    return (int64_t)i;
+}
 
 inline int tiles_of(int n) { return (n + kTile - 1) / kTile; }
 
@@ -42,11 +59,12 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // straight from memory with clamped indices (rows past M and columns past
 // N repeat the last one and are never stored; k past K contributes zero),
 // four k-steps' loads issued before their matrix-core steps.
+template <bool FIB = false>
 __device__ __forceinline__ void wave_strip(const double* __restrict__ base, const Strided& P,
                                            const Strided& Q, int M, int N, int K, int i0,
                                            int j0, int nq, double scale, v4d acc[4]) {
     const int lane = threadIdx.x & 63, r = lane & 15, kk = lane >> 4;
-    const int64_t prow = P.off + (int64_t)min(i0 + r, M - 1) * P.si;
+    const int64_t prow = P.off + first_off<FIB>(P, min(i0 + r, M - 1));
     int64_t qcol[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) qcol[q] = Q.off + (int64_t)min(j0 + 16 * q + r, N - 1) * Q.sk;
@@ -78,6 +96,7 @@ __device__ __forceinline__ void wave_strip(const double* __restrict__ base, cons
 // ty = t / 16) with vector FMAs.  The load pattern follows the unit-stride
 // index so that neighbouring lanes read neighbouring addresses.
 constexpr int kKStep = 16;
+template <bool FIB = false>
 __device__ __forceinline__ void tile_slab(const double* __restrict__ base, const Strided& P,
                                           const Strided& Q, int M, int N, int K, int i0, int j0,
                                           int k0, double (*Ps)[kTile + 1],
@@ -94,7 +113,7 @@ __device__ __forceinline__ void tile_slab(const double* __restrict__ base, const
             i = t / kKStep + 16 * r;
         }
         const int gi = i0 + i, gk = k0 + k;
-        Ps[k][i] = (gi < M && gk < K) ? base[P.off + (int64_t)gi * P.si + (int64_t)gk * P.sk] : 0.0;
+        Ps[k][i] = (gi < M && gk < K) ? base[P.off + first_off<FIB>(P, gi) + (int64_t)gk * P.sk] : 0.0;
         int j, kk;
         if (Q.si == 1) {
             kk = t % kKStep;
@@ -139,17 +158,69 @@ __device__ __forceinline__ int tile_nq(int N, int j0) { return min(4, (N - j0 + 
 // grid-stride over the tiles.
 // Internal linkage: every translation unit that launches it has its copy.
 // Stores a wave's strip (rows i0 .. i0 + 15) of a tile.
-__device__ __forceinline__ void strip_store(double* __restrict__ x, int ldx, int M, int N, int i0,
+template <bool FIB>
+__device__ __forceinline__ void strip_store(double* __restrict__ base, const GemmTile& g, int i0,
                                             int j0, int nq, const v4d acc[4]) {
     const int lane = threadIdx.x & 63;
+    int64_t row[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) row[v] = x_row<FIB>(g, min(i0 + (lane >> 4) + 4 * v, g.M - 1));
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int j = j0 + 16 * q + (lane & 15);
-        if (q >= nq || j >= N) continue;
+        if (q >= nq || j >= g.N) continue;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int i = i0 + (lane >> 4) + 4 * v;
-            if (i < M) x[i + (int64_t)j * ldx] = acc[q][v];
+            if (i < g.M) base[g.x_off + row[v] + (int64_t)j * g.ldx] = acc[q][v];
+        }
+    }
+}
+
+// X tiles = P Q, all operands in one arena (inputs and outputs of one
+// launch must not overlap).  256 threads; tiles are taken four at a time:
+// when none of the four takes the LDS slabs, each wave computes one tile
+// (all its strips), else the workgroup computes them one after another
+// (four strips at once).  Internal linkage: every translation unit that
+// launches it has its copy.
+// One tile by one wave (all its strips).
+template <bool FIB>
+__device__ __forceinline__ void tile_by_wave(double* base, const GemmTile& g) {
+    const int ti0 = g.ti * kTile, j0 = g.tj * kTile, nq = tile_nq(g.N, j0);
+    for (int i0 = ti0; i0 < min(g.M, ti0 + kTile); i0 += 16) {
+        v4d acc[4] = {};
+        wave_strip<FIB>(base, g.p, g.q, g.M, g.N, g.K, i0, j0, nq, 1.0, acc);
+        strip_store<FIB>(base, g, i0, j0, nq, acc);
+    }
+}
+
+// One tile by the workgroup: LDS slabs for a full tile, else a strip per wave.
+template <bool FIB>
+__device__ __forceinline__ void tile_by_group(double* base, const GemmTile& g,
+                                              double (*Ps)[kTile + 1], double (*Qs)[kTile + 1]) {
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ti0 = g.ti * kTile, j0 = g.tj * kTile;
+    if (tile_staged(g.M, g.N, g.K, ti0, j0)) {
+        double acc[4][4] = {};
+        for (int k0 = 0; k0 < g.K; k0 += kKStep)
+            tile_slab<FIB>(base, g.p, g.q, g.M, g.N, g.K, ti0, j0, k0, Ps, Qs, acc);
+        const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int j = j0 + ty + 16 * b;
+            if (j >= g.N) continue;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int i = ti0 + tx + 16 * a;
+                if (i < g.M) base[g.x_off + x_row<FIB>(g, i) + (int64_t)j * g.ldx] = acc[a][b];
+            }
+        }
+    } else {
+        const int i0 = ti0 + 16 * w, nq = tile_nq(g.N, j0);
+        if (i0 < g.M) {
+            v4d acc[4] = {};
+            wave_strip<FIB>(base, g.p, g.q, g.M, g.N, g.K, i0, j0, nq, 1.0, acc);
+            strip_store<FIB>(base, g, i0, j0, nq, acc);
         }
     }
 }
@@ -174,40 +245,14 @@ static __global__ void __launch_bounds__(256)
         if (!__syncthreads_or(mine)) {  // one tile per wave
             if (it + w >= ntiles) continue;
             const GemmTile g = tiles[it + w];
-            const int ti0 = g.ti * kTile, j0 = g.tj * kTile, nq = tile_nq(g.N, j0);
-            for (int i0 = ti0; i0 < min(g.M, ti0 + kTile); i0 += 16) {
-                v4d acc[4] = {};
-                wave_strip(base, g.p, g.q, g.M, g.N, g.K, i0, j0, nq, 1.0, acc);
-                strip_store(base + g.x_off, g.ldx, g.M, g.N, i0, j0, nq, acc);
-            }
+            if (__builtin_amdgcn_readfirstlane(g.p.lo)) tile_by_wave<true>(base, g);
+            else tile_by_wave<false>(base, g);
             continue;
         }
         for (int t = it; t < min(it + 4, ntiles); ++t) {
             const GemmTile g = tiles[t];
-            const int ti0 = g.ti * kTile, j0 = g.tj * kTile;
-            if (tile_staged(g.M, g.N, g.K, ti0, j0)) {
-                double acc[4][4] = {};
-                for (int k0 = 0; k0 < g.K; k0 += kKStep)
-                    tile_slab(base, g.p, g.q, g.M, g.N, g.K, ti0, j0, k0, Ps, Qs, acc);
-                const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int j = j0 + ty + 16 * b;
-                    if (j >= g.N) continue;
-#pragma unroll
-                    for (int a = 0; a < 4; ++a) {
-                        const int i = ti0 + tx + 16 * a;
-                        if (i < g.M) base[g.x_off + i + (int64_t)j * g.ldx] = acc[a][b];
-                    }
-                }
-            } else {
-                const int i0 = ti0 + 16 * w, nq = tile_nq(g.N, j0);
-                if (i0 < g.M) {
-                    v4d acc[4] = {};
-                    wave_strip(base, g.p, g.q, g.M, g.N, g.K, i0, j0, nq, 1.0, acc);
-                    strip_store(base + g.x_off, g.ldx, g.M, g.N, i0, j0, nq, acc);
-                }
-            }
+            if (__builtin_amdgcn_readfirstlane(g.p.lo)) tile_by_group<true>(base, g, Ps, Qs);
+            else tile_by_group<false>(base, g, Ps, Qs);
         }
     }
 }
