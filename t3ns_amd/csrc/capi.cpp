@@ -68,7 +68,8 @@ Symsecs symsecs_from(const t3ns_symsecs_desc& d) {
     return s;
 }
 
-ROperators rops_from(const t3ns_rops_desc& d) {
+// copy = false: the operator data stays in the caller's arrays (tel_src)
+ROperators rops_from(const t3ns_rops_desc& d, bool copy = true) {
     ROperators r;
     r.bond = d.bond;
     r.is_left = d.is_left;
@@ -90,6 +91,12 @@ ROperators rops_from(const t3ns_rops_desc& d) {
         for (auto& v : r.beginblock[o]) v -= b0;
     }
     r.finalize_offsets();
+    if (!copy) {
+        r.tel_src.resize(d.nrops, nullptr);
+        for (int o = 0; o < d.nrops; ++o)
+            if (r.beginblock[o].back()) r.tel_src[o] = d.tel[o] + d.beginblock[o][0];
+        return r;
+    }
     r.tel.resize(r.total_size());
     for (int o = 0; o < d.nrops; ++o) {
         const int64_t n = r.beginblock[o].back();
@@ -137,7 +144,7 @@ void convert_heff_input(HeffHost* h, const t3ns_heff_desc* d, const t3ns_rops_de
     const bool isdmrg = (site->nrsites == 1 && site->site_is_phys[0]) ||
                         (site->nrsites == 2 && site->site_is_phys[0] && site->site_is_phys[1]);
     for (int k = 0; k < (isdmrg ? 2 : 3); ++k) {
-        h->ops[k] = rops_from(ops[k]);
+        h->ops[k] = rops_from(ops[k], /*copy=*/false);  // uploaded from the caller's arrays
         in.ops[k] = &h->ops[k];
         in.rops_on_site[k] = d->rops_on_site[k];
     }

@@ -1478,9 +1478,19 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     for (int k = 0; k < nops; ++k) {
         const ROperators& op = *in.ops[k];
         stats_.op_elems[k] = op.total_size();
-        if ((int64_t)op.tel.size() != op.total_size())
+        if ((int64_t)op.tel.size() == op.total_size()) {
+            ops64_[k].upload(op.tel, stream);
+        } else if ((int)op.tel_src.size() == op.nrops) {  // straight from the caller's arrays
+            ops64_[k].alloc(op.total_size());
+            for (int o = 0; o < op.nrops; ++o) {
+                const int64_t n = op.op_offset[o + 1] - op.op_offset[o];
+                if (n)
+                    T3_HIP_CHECK(hipMemcpyAsync(ops64_[k].get() + op.op_offset[o], op.tel_src[o],
+                                                n * sizeof(double), hipMemcpyHostToDevice, stream));
+            }
+        } else {
             throw std::runtime_error("HeffDevice: operator host data missing");
-        ops64_[k].upload(op.tel, stream);
+        }
     }
     if (prec_ == PREC_FP32) {
         for (int k = 0; k < nops; ++k) {

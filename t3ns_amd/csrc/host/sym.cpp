@@ -1,4 +1,6 @@
 // Symmetry algebra; see sym.hpp for the reference files this follows.
+#include <cstdint>
+#include <unordered_map>
 #include "sym.hpp"
 
 #include <cmath>
@@ -160,7 +162,30 @@ double wigner6j(int a, int b, int c, int d, int e, int f) {
     return (double)(sum * norm);
 }
 
+static double wigner9j_eval(int j1, int j2, int j3, int j4, int j5, int j6, int j7, int j8,
+                            int j9);
+
 double wigner9j(int j1, int j2, int j3, int j4, int j5, int j6, int j7, int j8, int j9) {
+    // Memoised: a Heff plan evaluates the same few symbols for millions of
+    // (block, block, MPO) triples.  Doubled spins below 128 pack into one key.
+    const int v[9] = {j1, j2, j3, j4, j5, j6, j7, j8, j9};
+    uint64_t key = 0;
+    bool packable = true;
+    for (int x : v) {
+        packable = packable && x >= 0 && x < 128;
+        key = (key << 7) | (uint64_t)(x & 127);
+    }
+    if (!packable) return wigner9j_eval(j1, j2, j3, j4, j5, j6, j7, j8, j9);
+    thread_local std::unordered_map<uint64_t, double> memo;
+    const auto it = memo.find(key);
+    if (it != memo.end()) return it->second;
+    const double w = wigner9j_eval(j1, j2, j3, j4, j5, j6, j7, j8, j9);
+    memo.emplace(key, w);
+    return w;
+}
+
+static double wigner9j_eval(int j1, int j2, int j3, int j4, int j5, int j6, int j7, int j8,
+                            int j9) {
     // {j1 j2 j3; j4 j5 j6; j7 j8 j9} = sum_x (-1)^(2x) (2x+1)
     //      {j1 j4 j7; j8 j9 x} {j2 j5 j8; j4 x j6} {j3 j6 j9; x j1 j2}
     if (triangle_bad(j1, j2, j3) || triangle_bad(j4, j5, j6) || triangle_bad(j7, j8, j9) ||

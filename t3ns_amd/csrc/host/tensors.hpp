@@ -36,6 +36,9 @@ struct ROperators {
     std::vector<std::vector<int64_t>> beginblock;
     std::vector<int64_t> op_offset;  // nrops + 1
     std::vector<double> tel;         // host arena (may be empty)
+    // Per-operator caller data when `tel` is not copied (the Heff setup
+    // uploads straight from it; valid only while the caller's arrays are).
+    std::vector<const double*> tel_src;
 
     int couplings() const { return P_operator ? 3 : 1; }
     int nblocks_hss(int hss) const { return begin_blocks_of_hss[hss + 1] - begin_blocks_of_hss[hss]; }
