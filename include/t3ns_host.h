@@ -43,6 +43,11 @@ int t3ns_ctx_destroy(t3ns_ctx* c);
 /* nr_bonds, sites, psites of the network; nrSyms and sgs after Z2 insertion. */
 int t3ns_ctx_info(const t3ns_ctx* c, int* nr_bonds, int* sites, int* psites, int* nrsyms,
                   int* sgs);
+/* The two sites that bond `bond` joins (netw.bonds, -1 at a border). */
+int t3ns_ctx_bond_sites(const t3ns_ctx* c, int bond, int* sites2);
+/* The three bonds of site `site` (get_bonds_of_site, src/network.c:430) and
+ * whether it is a physical site. */
+int t3ns_ctx_site_bonds(const t3ns_ctx* c, int site, int* bonds3, int* is_phys);
 /* Sector count and dims of bond `bond` (reference numbering). */
 int t3ns_ctx_bond(const t3ns_ctx* c, int bond, int* nrsecs, int* totaldims, int* dims_out,
                   int dims_cap);
@@ -81,6 +86,23 @@ int t3ns_step_makesite_desc(t3ns_step* s, uint64_t seed, t3ns_makesite_desc* d);
  * live as long as the context. */
 int t3ns_ctx_bupdate_desc(t3ns_ctx* c, int site, int update_case, uint64_t seed,
                           t3ns_bupdate_desc* d, t3ns_rops_desc* ops /* [2] */);
+/* The rOperators_append_phys problem on bond `bond` seen from side
+ * `is_left` (the physical site on that side is appended): seeded N(0,1)
+ * operators on the site's far bond (the unit operator at a border) as
+ * `old`, which a caller may replace by its own operators of that bond, and
+ * the append descriptor: the append of init_rops
+ * (src/optimize_network.c:436-451) and of the Heff legs next to a physical
+ * site (src/optimize_network.c:164-176).  The arrays live as long as the
+ * context. */
+int t3ns_ctx_append_desc(t3ns_ctx* c, int bond, int is_left, uint64_t seed, t3ns_append_desc* d,
+                         t3ns_rops_desc* old);
+/* The update_rOperators_physical problem on bond `bond` seen from side
+ * `is_left`: a seeded N(0,1) one-site tensor of the physical site on that
+ * side over the context's own sectors (no truncation) and the structure of
+ * the new operators; the physical operators are the caller's (the result
+ * of the append on the same bond).  The arrays live as long as the
+ * context. */
+int t3ns_ctx_pupdate_desc(t3ns_ctx* c, int bond, int is_left, uint64_t seed, t3ns_pupdate_desc* d);
 /* A split of the step's multisite tensor (its N(0,1) elements normalised):
  * site index `siteV` is split off with the given truncation settings. */
 int t3ns_step_split_desc(t3ns_step* s, int siteV, int minD, int maxD, double truncerr,

@@ -206,6 +206,11 @@ def lib():
         "t3ns_ctx_destroy": (C.c_int, [P]),
         "t3ns_ctx_info": (C.c_int, [P, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p]),
         "t3ns_ctx_bond": (C.c_int, [P, C.c_int, c_int_p, c_int_p, c_int_p, C.c_int]),
+        "t3ns_ctx_bond_sites": (C.c_int, [P, C.c_int, c_int_p]),
+        "t3ns_ctx_site_bonds": (C.c_int, [P, C.c_int, c_int_p, c_int_p]),
+        "t3ns_ctx_append_desc": (C.c_int, [P, C.c_int, C.c_int, C.c_uint64, C.POINTER(AppendDesc),
+                                           C.POINTER(RopsDesc)]),
+        "t3ns_ctx_pupdate_desc": (C.c_int, [P, C.c_int, C.c_int, C.c_uint64, C.POINTER(PUpdateDesc)]),
         "t3ns_step_random": (C.c_int, [C.POINTER(P), P, c_int_p, C.c_int, C.c_uint64]),
         "t3ns_step_destroy": (C.c_int, [P]),
         "t3ns_step_size": (C.c_int64, [P]),

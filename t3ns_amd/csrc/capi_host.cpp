@@ -109,11 +109,6 @@ struct QRStore {
     SiteTensor A, C;
 };
 
-struct t3ns_ctx {
-    std::unique_ptr<Context> c;
-    std::vector<std::unique_ptr<BUpdateStore>> bups;
-    std::vector<std::unique_ptr<QRStore>> qrs;
-};
 
 // An append problem of one physical leg of a step: the operators of the
 // far bond (old) and the flattened rOperators_append_phys inputs.
@@ -144,6 +139,14 @@ struct PUpdateStore {
     int is_left = 0, nrhss = 0, maxdim = -1;
     uint64_t seed = 0;
     bool built = false;
+};
+
+struct t3ns_ctx {
+    std::unique_ptr<Context> c;
+    std::vector<std::unique_ptr<BUpdateStore>> bups;
+    std::vector<std::unique_ptr<QRStore>> qrs;
+    std::vector<std::unique_ptr<AppendStore>> apps;
+    std::vector<std::unique_ptr<PUpdateStore>> pups;
 };
 
 struct MakesiteStore {
@@ -179,6 +182,226 @@ struct t3ns_step {
     std::vector<const int64_t*> bb_ptr[3];
     std::vector<const double*> tel_ptr[3];
 };
+
+void build_append(Context& c, int pbond, int is_left, uint64_t seed, AppendStore& a) {
+    // the appended site and the far bond of its operators
+    const int site = c.netw.bonds[pbond][!is_left];
+    int sb[3];
+    c.netw.bonds_of_site(site, sb);
+    const int b = sb[is_left ? 0 : 2];
+    int nrops;
+    std::vector<int> hss;
+    ROperators o;
+    if (c.netw.bonds[b][!is_left] == -1) {
+        o = vacuum_rOperators(c, b, is_left);
+    } else {
+        std::vector<std::vector<int64_t>> tmpbb;
+        operator_types(c, b, is_left, &nrops, &hss);
+        o = init_rOperators_structure(c, b, is_left, false, &tmpbb);
+        o.nrops = nrops;
+        o.hss_of_ops = hss;
+        o.beginblock.resize(nrops);
+        for (int k = 0; k < nrops; ++k) o.beginblock[k] = tmpbb[hss[k]];
+        o.finalize_offsets();
+        o.tel.resize(o.total_size());
+        Rng rng(seed);
+        for (auto& v : o.tel) v = rng.normal();
+    }
+    a.old = std::move(o);
+    a.is_left = is_left;
+    for (int k = 0; k < 3; ++k) a.site_ss[k].set(c.symsecs(sb[k]));
+    a.new_ss[0].set(c.symsecs(c.get_braT3NSbond(pbond)));
+    a.new_ss[1].set(c.symsecs(c.get_ketT3NSbond(pbond)));
+    a.new_ss[2].set(c.symsecs(-1));
+    a.old_ss[0].set(c.symsecs(c.get_braT3NSbond(b)));
+    a.old_ss[1].set(c.symsecs(c.get_ketT3NSbond(b)));
+    a.old_ss[2].set(c.symsecs(-1));
+    a.mpo.set(c.symsecs(-1));
+    const InstructionSet& ps = c.fetch_pUpdate(b, is_left);
+    int nops = 0, nso = 0;
+    for (const auto& i : ps.instr) {
+        for (int k = 0; k < 3; ++k) a.instr.push_back(i.instr[k]);
+        a.pref.push_back(i.pref);
+        nops = std::max(nops, i.instr[2] + 1);
+        nso = std::max(nso, i.instr[1] + 1);
+    }
+    a.hss_of_new.assign(ps.hss_of_new.begin(), ps.hss_of_new.begin() + nops);
+    a.nr_siteops = nso;
+    const Symsecs& phys = c.symsecs(sb[1]);
+    a.physdim = phys.nrSecs;
+    // site-operator ids may be sparse labels: tabulate the ones the
+    // instructions use
+    std::vector<char> used(nso, 0);
+    for (const auto& i : ps.instr) used[i.instr[1]] = 1;
+    a.siteop_hss.assign(nso, -1);
+    a.siteop_el.assign((size_t)nso * a.physdim * a.physdim, 0.0);
+    for (int so = 0; so < nso; ++so) {
+        if (!used[so]) continue;
+        a.siteop_hss[so] = c.ham->symsec_siteop(so, site);
+        for (int br = 0; br < a.physdim; ++br)
+            for (int kt = 0; kt < a.physdim; ++kt)
+                a.siteop_el[((size_t)so * a.physdim + br) * a.physdim + kt] =
+                    c.ham->el_siteop(so, br, kt);
+    }
+    const int H = c.ham->nr_hss();
+    a.tprod_begin.assign(1, 0);
+    for (int h = 0; h < H; ++h) {
+        for (const auto& pr : c.ham->tprods_ham(h, site)) {
+            a.tprods.push_back(pr[0]);
+            a.tprods.push_back(pr[1]);
+        }
+        a.tprod_begin.push_back((int)a.tprods.size() / 2);
+    }
+    std::vector<std::vector<int64_t>> tmpbb;
+    ROperators ns = init_rOperators_structure(c, pbond, is_left, true, &tmpbb);
+    a.nrhss = ns.nrhss;
+    a.bbh = ns.begin_blocks_of_hss;
+    a.qnumbers = ns.qnumbers;
+    a.hss_bb = tmpbb;
+    for (auto& v : a.hss_bb)
+        if (v.empty()) v.assign(1, 0);
+    a.hss_bb_ptr.resize(a.hss_bb.size());
+    for (size_t h = 0; h < a.hss_bb.size(); ++h) a.hss_bb_ptr[h] = a.hss_bb[h].data();
+    a.old_bb_ptr.resize(a.old.nrops);
+    a.old_tel_ptr.resize(a.old.nrops);
+    for (int k = 0; k < a.old.nrops; ++k) {
+        a.old_bb_ptr[k] = a.old.beginblock[k].data();
+        a.old_tel_ptr[k] = a.old.tel.data() + a.old.op_offset[k];
+    }
+    a.built = true;
+}
+
+void fill_append(const Context& c, AppendStore& a, t3ns_append_desc* d, t3ns_rops_desc* old) {
+    std::memset(d, 0, sizeof(*d));
+    d->nrSyms = c.sym.nrSyms;
+    for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) d->sgs[i] = c.sym.sgs[i];
+    d->is_left = a.is_left;
+    for (int k = 0; k < 3; ++k) {
+        d->site_ss[k] = a.site_ss[k].desc;
+        d->new_ss[k] = a.new_ss[k].desc;
+        d->old_ss[k] = a.old_ss[k].desc;
+    }
+    d->mpo = a.mpo.desc;
+    d->nr_instr = (int)a.pref.size();
+    d->instr = a.instr.data();
+    d->pref = a.pref.data();
+    d->hss_of_new = a.hss_of_new.data();
+    d->nr_siteops = a.nr_siteops;
+    d->siteop_hss = a.siteop_hss.data();
+    d->physdim = a.physdim;
+    d->siteop_el = a.siteop_el.data();
+    d->tprod_begin = a.tprod_begin.data();
+    d->tprods = a.tprods.data();
+    d->nrhss = a.nrhss;
+    d->begin_blocks_of_hss = a.bbh.data();
+    d->qnumbers = a.qnumbers.data();
+    d->hss_beginblock = a.hss_bb_ptr.data();
+    std::memset(old, 0, sizeof(*old));
+    old->bond = a.old.bond;
+    old->is_left = a.old.is_left;
+    old->P_operator = a.old.P_operator;
+    old->nrhss = a.old.nrhss;
+    old->begin_blocks_of_hss = a.old.begin_blocks_of_hss.data();
+    old->qnumbers = a.old.qnumbers.data();
+    old->nrops = a.old.nrops;
+    old->hss_of_ops = a.old.hss_of_ops.data();
+    old->beginblock = a.old_bb_ptr.data();
+    old->tel = a.old_tel_ptr.data();
+}
+
+void build_pupdate(Context& c, int pbond, int is_left, uint64_t seed, int maxdim, PUpdateStore& u) {
+    u.seed = seed;
+    u.maxdim = maxdim;
+    const int site = c.netw.bonds[pbond][!is_left];
+    int sb[3];
+    c.netw.bonds_of_site(site, sb);
+    // The internal sectors the physical operators were built on and
+    // a truncated set as a decomposition of the optimised tensor
+    // leaves it: sectors dropped, kept with dim 0, or kept with
+    // dims 1..maxdim.  maxdim < 0 keeps the context's own sectors.
+    const Symsecs iss = c.symsecs(pbond);
+    Symsecs tr;
+    tr.bond = iss.bond;
+    Rng rng(seed);
+    Symsecs& slot = c.symsecs_mut(pbond);
+    if (maxdim < 0) {
+        tr = iss;
+    } else if (maxdim == 0) {
+        // no truncation: every sector keeps the full dimension of the
+        // (far bond x physical) product space that couples to it
+        tr = iss;
+        for (auto& v : tr.dims) v = 1;
+        slot = tr;
+        SiteTensor unit;
+        try {
+            unit = init_1siteTensor(c, site);
+        } catch (...) {
+            slot = iss;
+            throw;
+        }
+        const int n0 = c.symsecs(sb[0]).nrSecs, n1 = c.symsecs(sb[1]).nrSecs;
+        std::fill(tr.dims.begin(), tr.dims.end(), 0);
+        for (int b = 0; b < unit.nrblocks; ++b) {
+            const QN q = unit.qnumbers[b];
+            const int sec = is_left ? (int)(q / ((QN)n0 * n1)) : (int)(q % n0);
+            tr.dims[sec] += (int)unit.block_size(b);
+        }
+        tr.totaldims = 0;
+        for (int v : tr.dims) tr.totaldims += v;
+    } else {
+        for (int i = 0; i < iss.nrSecs; ++i) {
+            const double r = rng.uniform();
+            if (r < 0.1) continue;
+            const int dim = r < 0.2 ? 0 : 1 + (int)(rng.next() % (uint64_t)maxdim);
+            tr.irreps.push_back(iss.irreps[i]);
+            tr.dims.push_back(dim);
+            tr.fcidims.push_back(iss.fcidims.empty() ? 0.0 : iss.fcidims[i]);
+            tr.totaldims += dim;
+        }
+        tr.nrSecs = (int)tr.dims.size();
+    }
+    slot = tr;
+    try {
+        u.T = init_1siteTensor(c, site);
+        for (auto& v : u.T.tel) v = rng.normal();
+        std::vector<std::vector<int64_t>> tmpbb;
+        ROperators ns = init_rOperators_structure(c, pbond, is_left, false, &tmpbb);
+        u.nrhss = ns.nrhss;
+        u.bbh = ns.begin_blocks_of_hss;
+        u.qnumbers = ns.qnumbers;
+        u.hss_bb = tmpbb;
+        for (int k = 0; k < 3; ++k) u.site_ss[k].set(c.symsecs(sb[k]));
+    } catch (...) {
+        slot = iss;
+        throw;
+    }
+    slot = iss;
+    u.iss.set(iss);
+    u.mpo.set(c.symsecs(-1));
+    u.is_left = is_left;
+    for (auto& v : u.hss_bb)
+        if (v.empty()) v.assign(1, 0);
+    u.hss_bb_ptr.resize(u.hss_bb.size());
+    for (size_t h = 0; h < u.hss_bb.size(); ++h) u.hss_bb_ptr[h] = u.hss_bb[h].data();
+}
+
+void fill_pupdate(const Context& c, PUpdateStore& u, t3ns_pupdate_desc* d) {
+    std::memset(d, 0, sizeof(*d));
+    d->nrSyms = c.sym.nrSyms;
+    for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) d->sgs[i] = c.sym.sgs[i];
+    d->is_left = u.is_left;
+    for (int k = 0; k < 3; ++k) d->site_ss[k] = u.site_ss[k].desc;
+    d->internal_ss = u.iss.desc;
+    d->mpo = u.mpo.desc;
+    d->T_nrblocks = u.T.nrblocks;
+    d->T_qnumbers = u.T.qnumbers.data();
+    d->T_beginblock = u.T.beginblock.data();
+    d->T_tel = u.T.tel.data();
+    d->nrhss = u.nrhss;
+    d->begin_blocks_of_hss = u.bbh.data();
+    d->qnumbers = u.qnumbers.data();
+    d->hss_beginblock = u.hss_bb_ptr.data();
+}
 
 extern "C" {
 
@@ -254,6 +477,20 @@ int t3ns_ctx_info(const t3ns_ctx* c, int* nr_bonds, int* sites, int* psites, int
     if (nrsyms) *nrsyms = x.sym.nrSyms;
     if (sgs)
         for (int i = 0; i < x.sym.nrSyms; ++i) sgs[i] = x.sym.sgs[i];
+    return T3NS_OK;
+}
+
+int t3ns_ctx_bond_sites(const t3ns_ctx* c, int bond, int* sites2) {
+    if (!c || !sites2 || bond < 0 || bond >= c->c->netw.nr_bonds) return T3NS_ERR_INVALID;
+    sites2[0] = c->c->netw.bonds[bond][0];
+    sites2[1] = c->c->netw.bonds[bond][1];
+    return T3NS_OK;
+}
+
+int t3ns_ctx_site_bonds(const t3ns_ctx* c, int site, int* bonds3, int* is_phys) {
+    if (!c || !bonds3 || site < 0 || site >= c->c->netw.sites) return T3NS_ERR_INVALID;
+    c->c->netw.bonds_of_site(site, bonds3);
+    if (is_phys) *is_phys = c->c->netw.is_psite(site) ? 1 : 0;
     return T3NS_OK;
 }
 
@@ -373,130 +610,8 @@ int t3ns_step_append_desc(t3ns_step* st, int leg, uint64_t seed, t3ns_append_des
         if (P.nrhss <= 0 || !P.P_operator)
             throw std::invalid_argument("step leg is not a physical operator");
         AppendStore& a = st->app[leg];
-        if (!a.built) {
-            const int is_left = P.is_left;
-            const int pbond = P.bond;
-            // the appended site and the far bond of its operators
-            const int site = c.netw.bonds[pbond][!is_left];
-            int sb[3];
-            c.netw.bonds_of_site(site, sb);
-            const int b = sb[is_left ? 0 : 2];
-            int nrops;
-            std::vector<int> hss;
-            ROperators o;
-            if (c.netw.bonds[b][!is_left] == -1) {
-                o = vacuum_rOperators(c, b, is_left);
-            } else {
-                std::vector<std::vector<int64_t>> tmpbb;
-                operator_types(c, b, is_left, &nrops, &hss);
-                o = init_rOperators_structure(c, b, is_left, false, &tmpbb);
-                o.nrops = nrops;
-                o.hss_of_ops = hss;
-                o.beginblock.resize(nrops);
-                for (int k = 0; k < nrops; ++k) o.beginblock[k] = tmpbb[hss[k]];
-                o.finalize_offsets();
-                o.tel.resize(o.total_size());
-                Rng rng(seed);
-                for (auto& v : o.tel) v = rng.normal();
-            }
-            a.old = std::move(o);
-            a.is_left = is_left;
-            for (int k = 0; k < 3; ++k) a.site_ss[k].set(c.symsecs(sb[k]));
-            a.new_ss[0].set(c.symsecs(c.get_braT3NSbond(pbond)));
-            a.new_ss[1].set(c.symsecs(c.get_ketT3NSbond(pbond)));
-            a.new_ss[2].set(c.symsecs(-1));
-            a.old_ss[0].set(c.symsecs(c.get_braT3NSbond(b)));
-            a.old_ss[1].set(c.symsecs(c.get_ketT3NSbond(b)));
-            a.old_ss[2].set(c.symsecs(-1));
-            a.mpo.set(c.symsecs(-1));
-            const InstructionSet& ps = c.fetch_pUpdate(b, is_left);
-            int nops = 0, nso = 0;
-            for (const auto& i : ps.instr) {
-                for (int k = 0; k < 3; ++k) a.instr.push_back(i.instr[k]);
-                a.pref.push_back(i.pref);
-                nops = std::max(nops, i.instr[2] + 1);
-                nso = std::max(nso, i.instr[1] + 1);
-            }
-            a.hss_of_new.assign(ps.hss_of_new.begin(), ps.hss_of_new.begin() + nops);
-            a.nr_siteops = nso;
-            const Symsecs& phys = c.symsecs(sb[1]);
-            a.physdim = phys.nrSecs;
-            // site-operator ids may be sparse labels: tabulate the ones the
-            // instructions use
-            std::vector<char> used(nso, 0);
-            for (const auto& i : ps.instr) used[i.instr[1]] = 1;
-            a.siteop_hss.assign(nso, -1);
-            a.siteop_el.assign((size_t)nso * a.physdim * a.physdim, 0.0);
-            for (int so = 0; so < nso; ++so) {
-                if (!used[so]) continue;
-                a.siteop_hss[so] = c.ham->symsec_siteop(so, site);
-                for (int br = 0; br < a.physdim; ++br)
-                    for (int kt = 0; kt < a.physdim; ++kt)
-                        a.siteop_el[((size_t)so * a.physdim + br) * a.physdim + kt] =
-                            c.ham->el_siteop(so, br, kt);
-            }
-            const int H = c.ham->nr_hss();
-            a.tprod_begin.assign(1, 0);
-            for (int h = 0; h < H; ++h) {
-                for (const auto& pr : c.ham->tprods_ham(h, site)) {
-                    a.tprods.push_back(pr[0]);
-                    a.tprods.push_back(pr[1]);
-                }
-                a.tprod_begin.push_back((int)a.tprods.size() / 2);
-            }
-            std::vector<std::vector<int64_t>> tmpbb;
-            ROperators ns = init_rOperators_structure(c, pbond, is_left, true, &tmpbb);
-            a.nrhss = ns.nrhss;
-            a.bbh = ns.begin_blocks_of_hss;
-            a.qnumbers = ns.qnumbers;
-            a.hss_bb = tmpbb;
-            for (auto& v : a.hss_bb)
-                if (v.empty()) v.assign(1, 0);
-            a.hss_bb_ptr.resize(a.hss_bb.size());
-            for (size_t h = 0; h < a.hss_bb.size(); ++h) a.hss_bb_ptr[h] = a.hss_bb[h].data();
-            a.old_bb_ptr.resize(a.old.nrops);
-            a.old_tel_ptr.resize(a.old.nrops);
-            for (int k = 0; k < a.old.nrops; ++k) {
-                a.old_bb_ptr[k] = a.old.beginblock[k].data();
-                a.old_tel_ptr[k] = a.old.tel.data() + a.old.op_offset[k];
-            }
-            a.built = true;
-        }
-        std::memset(d, 0, sizeof(*d));
-        d->nrSyms = c.sym.nrSyms;
-        for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) d->sgs[i] = c.sym.sgs[i];
-        d->is_left = a.is_left;
-        for (int k = 0; k < 3; ++k) {
-            d->site_ss[k] = a.site_ss[k].desc;
-            d->new_ss[k] = a.new_ss[k].desc;
-            d->old_ss[k] = a.old_ss[k].desc;
-        }
-        d->mpo = a.mpo.desc;
-        d->nr_instr = (int)a.pref.size();
-        d->instr = a.instr.data();
-        d->pref = a.pref.data();
-        d->hss_of_new = a.hss_of_new.data();
-        d->nr_siteops = a.nr_siteops;
-        d->siteop_hss = a.siteop_hss.data();
-        d->physdim = a.physdim;
-        d->siteop_el = a.siteop_el.data();
-        d->tprod_begin = a.tprod_begin.data();
-        d->tprods = a.tprods.data();
-        d->nrhss = a.nrhss;
-        d->begin_blocks_of_hss = a.bbh.data();
-        d->qnumbers = a.qnumbers.data();
-        d->hss_beginblock = a.hss_bb_ptr.data();
-        std::memset(old, 0, sizeof(*old));
-        old->bond = a.old.bond;
-        old->is_left = a.old.is_left;
-        old->P_operator = a.old.P_operator;
-        old->nrhss = a.old.nrhss;
-        old->begin_blocks_of_hss = a.old.begin_blocks_of_hss.data();
-        old->qnumbers = a.old.qnumbers.data();
-        old->nrops = a.old.nrops;
-        old->hss_of_ops = a.old.hss_of_ops.data();
-        old->beginblock = a.old_bb_ptr.data();
-        old->tel = a.old_tel_ptr.data();
+        if (!a.built) build_append(c, P.bond, P.is_left, seed, a);
+        fill_append(c, a, d, old);
     });
 }
 
@@ -511,79 +626,7 @@ int t3ns_step_pupdate_desc(t3ns_step* st, int leg, uint64_t seed, int maxdim,
         PUpdateStore& u = st->pup[leg];
         if (u.built && (u.seed != seed || u.maxdim != maxdim)) u = PUpdateStore{};
         if (!u.built) {
-            u.seed = seed;
-            u.maxdim = maxdim;
-            const int is_left = P.is_left;
-            const int pbond = P.bond;
-            const int site = c.netw.bonds[pbond][!is_left];
-            int sb[3];
-            c.netw.bonds_of_site(site, sb);
-            // The internal sectors the physical operators were built on and
-            // a truncated set as a decomposition of the optimised tensor
-            // leaves it: sectors dropped, kept with dim 0, or kept with
-            // dims 1..maxdim.
-            const Symsecs iss = c.symsecs(pbond);
-            Symsecs tr;
-            tr.bond = iss.bond;
-            Rng rng(seed);
-            Symsecs& slot = c.symsecs_mut(pbond);
-            if (maxdim == 0) {
-                // no truncation: every sector keeps the full dimension of the
-                // (far bond x physical) product space that couples to it
-                tr = iss;
-                for (auto& v : tr.dims) v = 1;
-                slot = tr;
-                SiteTensor unit;
-                try {
-                    unit = init_1siteTensor(c, site);
-                } catch (...) {
-                    slot = iss;
-                    throw;
-                }
-                const int n0 = c.symsecs(sb[0]).nrSecs, n1 = c.symsecs(sb[1]).nrSecs;
-                std::fill(tr.dims.begin(), tr.dims.end(), 0);
-                for (int b = 0; b < unit.nrblocks; ++b) {
-                    const QN q = unit.qnumbers[b];
-                    const int sec = is_left ? (int)(q / ((QN)n0 * n1)) : (int)(q % n0);
-                    tr.dims[sec] += (int)unit.block_size(b);
-                }
-                tr.totaldims = 0;
-                for (int v : tr.dims) tr.totaldims += v;
-            } else {
-                for (int i = 0; i < iss.nrSecs; ++i) {
-                    const double r = rng.uniform();
-                    if (r < 0.1) continue;
-                    const int dim = r < 0.2 ? 0 : 1 + (int)(rng.next() % (uint64_t)maxdim);
-                    tr.irreps.push_back(iss.irreps[i]);
-                    tr.dims.push_back(dim);
-                    tr.fcidims.push_back(iss.fcidims.empty() ? 0.0 : iss.fcidims[i]);
-                    tr.totaldims += dim;
-                }
-                tr.nrSecs = (int)tr.dims.size();
-            }
-            slot = tr;
-            try {
-                u.T = init_1siteTensor(c, site);
-                for (auto& v : u.T.tel) v = rng.normal();
-                std::vector<std::vector<int64_t>> tmpbb;
-                ROperators ns = init_rOperators_structure(c, pbond, is_left, false, &tmpbb);
-                u.nrhss = ns.nrhss;
-                u.bbh = ns.begin_blocks_of_hss;
-                u.qnumbers = ns.qnumbers;
-                u.hss_bb = tmpbb;
-                for (int k = 0; k < 3; ++k) u.site_ss[k].set(c.symsecs(sb[k]));
-            } catch (...) {
-                slot = iss;
-                throw;
-            }
-            slot = iss;
-            u.iss.set(iss);
-            u.mpo.set(c.symsecs(-1));
-            u.is_left = is_left;
-            for (auto& v : u.hss_bb)
-                if (v.empty()) v.assign(1, 0);
-            u.hss_bb_ptr.resize(u.hss_bb.size());
-            for (size_t h = 0; h < u.hss_bb.size(); ++h) u.hss_bb_ptr[h] = u.hss_bb[h].data();
+            build_pupdate(c, P.bond, P.is_left, seed, maxdim, u);
             u.bb_ptr.resize(P.nrops);
             u.tel_ptr.resize(P.nrops);
             for (int k = 0; k < P.nrops; ++k) {
@@ -592,21 +635,7 @@ int t3ns_step_pupdate_desc(t3ns_step* st, int leg, uint64_t seed, int maxdim,
             }
             u.built = true;
         }
-        std::memset(d, 0, sizeof(*d));
-        d->nrSyms = c.sym.nrSyms;
-        for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) d->sgs[i] = c.sym.sgs[i];
-        d->is_left = u.is_left;
-        for (int k = 0; k < 3; ++k) d->site_ss[k] = u.site_ss[k].desc;
-        d->internal_ss = u.iss.desc;
-        d->mpo = u.mpo.desc;
-        d->T_nrblocks = u.T.nrblocks;
-        d->T_qnumbers = u.T.qnumbers.data();
-        d->T_beginblock = u.T.beginblock.data();
-        d->T_tel = u.T.tel.data();
-        d->nrhss = u.nrhss;
-        d->begin_blocks_of_hss = u.bbh.data();
-        d->qnumbers = u.qnumbers.data();
-        d->hss_beginblock = u.hss_bb_ptr.data();
+        fill_pupdate(c, u, d);
         std::memset(pops, 0, sizeof(*pops));
         pops->bond = P.bond;
         pops->is_left = P.is_left;
@@ -775,6 +804,46 @@ int t3ns_ctx_bupdate_desc(t3ns_ctx* cx, int site, int update_case, uint64_t seed
             o.tel = u.tel_ptr[r].data();
         }
         cx->bups.push_back(std::move(st));
+    });
+}
+
+int t3ns_ctx_append_desc(t3ns_ctx* cx, int bond, int is_left, uint64_t seed, t3ns_append_desc* d,
+                         t3ns_rops_desc* old) {
+    if (!cx || !d || !old) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        Context& c = *cx->c;
+        if (bond < 0 || bond >= c.netw.nr_bonds || !c.netw.is_psite(c.netw.bonds[bond][!is_left]))
+            throw std::invalid_argument("append: the bond's site on that side is not physical");
+        auto a = std::make_unique<AppendStore>();
+        // The physical operators live on the bond's sectors at unit dims
+        // (src/optimize_network.c:440-453); the context keeps its dims.
+        Symsecs& slot = c.symsecs_mut(bond);
+        const Symsecs keep = slot;
+        std::fill(slot.dims.begin(), slot.dims.end(), 1);
+        slot.totaldims = slot.nrSecs;
+        try {
+            build_append(c, bond, is_left ? 1 : 0, seed, *a);
+        } catch (...) {
+            slot = keep;
+            throw;
+        }
+        slot = keep;
+        fill_append(c, *a, d, old);
+        cx->apps.push_back(std::move(a));
+    });
+}
+
+int t3ns_ctx_pupdate_desc(t3ns_ctx* cx, int bond, int is_left, uint64_t seed, t3ns_pupdate_desc* d) {
+    if (!cx || !d) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        Context& c = *cx->c;
+        if (bond < 0 || bond >= c.netw.nr_bonds || !c.netw.is_psite(c.netw.bonds[bond][!is_left]))
+            throw std::invalid_argument("pupdate: the bond's site on that side is not physical");
+        auto u = std::make_unique<PUpdateStore>();
+        build_pupdate(c, bond, is_left ? 1 : 0, seed, -1, *u);
+        u->built = true;
+        fill_pupdate(c, *u, d);
+        cx->pups.push_back(std::move(u));
     });
 }
 

@@ -1,0 +1,123 @@
+"""Renormalised operators of a network built from the Hamiltonian itself.
+
+init_rops (src/optimize_network.c:428-468) makes the operators of every bond
+from the vacuum at the borders inwards: a bond whose site on the covered side
+is physical gets rOperators_append_phys of that site to the operators of the
+site's far bond, then update_rOperators_physical with the site's tensor; a
+bond whose site is a branching site gets update_rOperators_branching of the
+operators on the site's two other bonds.  The operators of an optimisation
+step (src/optimize_network.c:164-176) are those of its external bonds, with
+the physical sites inside the step appended where the step's leg is a
+physical-operator leg.
+
+Everything runs through the library's device entry points; the operators are
+held on the host between calls as the reference holds them.  Any site
+tensors serve: the environment of a bond is T^dagger O T for whatever tensors
+the updates used, so the effective Hamiltonian of a step is symmetric, and
+with isometric tensors and no truncation it is the Hamiltonian itself on the
+step's space.
+"""
+from .heff import append_phys, update_branching, update_physical
+
+
+class Environment:
+    """Operators of bonds (both sides) of a context's network, built on demand.
+
+    `site_tensor(site, desc, keep)`, if given, may replace the seeded site
+    tensor of an update descriptor (desc.T_tel) before the update runs; the
+    arrays it makes go to `keep`.  `append`, `update_phys` and
+    `update_branch` are the three renormalisation calls (the device ones by
+    default); each returns an object with a `desc` (RopsDesc) attribute."""
+
+    def __init__(self, ctx, seed=0, site_tensor=None, append=append_phys,
+                 update_phys=update_physical, update_branch=update_branching):
+        self.ctx = ctx
+        self.seed = int(seed)
+        self.site_tensor = site_tensor
+        self._append, self._uphys, self._ubranch = append, update_phys, update_branch
+        self._ops = {}
+        self._keep = []
+        self._count = 0
+
+    def _next_seed(self):
+        self._count += 1
+        return self.seed * 1000003 + self._count
+
+    def _outer_site(self, bond, is_left):
+        left, right = self.ctx.bond_sites(bond)
+        return left if is_left else right
+
+    def is_border(self, bond, is_left):
+        """True when the covered side of `bond` is empty (vacuum operators)."""
+        return self._outer_site(bond, is_left) == -1
+
+    def physical(self, bond, is_left):
+        """Physical operators on `bond`: the site on side `is_left` appended to
+        the operators of its far bond (rOperators_append_phys)."""
+        key = ("P", int(bond), int(bool(is_left)))
+        if key in self._ops:
+            return self._ops[key]
+        L = int(bool(is_left))
+        site = self._outer_site(bond, L)
+        bonds, phys = self.ctx.site_bonds(site)
+        if not phys:
+            raise ValueError("site %d next to bond %d is not physical" % (site, bond))
+        ad, old = self.ctx.append_problem_at(bond, L, self._next_seed())
+        far = bonds[0] if L else bonds[2]
+        if not self.is_border(far, L):
+            old = self.operators(far, L).desc
+        r = self._append(ad, old)
+        r.desc.bond = bond
+        self._keep.append((ad, r))
+        self._ops[key] = r
+        return r
+
+    def operators(self, bond, is_left):
+        """Renormalised operators on `bond` covering its side `is_left`."""
+        key = ("R", int(bond), int(bool(is_left)))
+        if key in self._ops:
+            return self._ops[key]
+        L = int(bool(is_left))
+        site = self._outer_site(bond, L)
+        if site == -1:
+            raise ValueError("bond %d has a border on that side (vacuum operators)" % bond)
+        bonds, phys = self.ctx.site_bonds(site)
+        if phys:
+            P = self.physical(bond, L)
+            pd = self.ctx.pupdate_problem_at(bond, L, self._next_seed())
+            if self.site_tensor is not None:
+                self.site_tensor(site, pd, self._keep)
+            r = self._uphys(pd, P.desc)
+            self._keep.append(pd)
+        else:
+            case = bonds.index(bond)
+            d, o = self.ctx.bupdate_problem(site, case, self._next_seed())
+            # roles of the two old operators (src/rOperators_bUpdate.c:292-313)
+            obond = (bonds[1 if case == 0 else 0], bonds[1 if case == 2 else 2])
+            oleft = (1, int(case == 2))
+            for k in range(2):
+                if not self.is_border(obond[k], oleft[k]):
+                    o[k] = self.operators(obond[k], oleft[k]).desc
+            if self.site_tensor is not None:
+                self.site_tensor(site, d, self._keep)
+            r = self._ubranch(d, o)
+            self._keep.append((d, o))
+        r.desc.bond = bond
+        self._ops[key] = r
+        return r
+
+    def step_operators(self, step):
+        """(desc, ops[3], site) of `step` with its operators replaced by the
+        Hamiltonian's: physical-operator legs get the append of their step
+        site to the operators of the external bond, other legs those
+        operators themselves, border legs stay the vacuum."""
+        d, o, s = step.descriptors()
+        for i in range(3):
+            if o[i].nrhss <= 0:
+                continue
+            L = o[i].is_left
+            if o[i].P_operator:
+                o[i] = self.physical(o[i].bond, L).desc
+            elif not self.is_border(o[i].bond, L):
+                o[i] = self.operators(o[i].bond, L).desc
+        return d, o, s

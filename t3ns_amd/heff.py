@@ -88,6 +88,35 @@ class Context:
                                           C.byref(d), o), "t3ns_ctx_bupdate_desc")
         return d, o
 
+    def bond_sites(self, bond):
+        """(left site, right site) of bond `bond`, -1 at a border."""
+        out = (C.c_int * 2)()
+        check(lib().t3ns_ctx_bond_sites(self._h, int(bond), out), "t3ns_ctx_bond_sites")
+        return out[0], out[1]
+
+    def site_bonds(self, site):
+        """([three bonds], is physical) of site `site`."""
+        out, ph = (C.c_int * 3)(), C.c_int()
+        check(lib().t3ns_ctx_site_bonds(self._h, int(site), out, C.byref(ph)), "t3ns_ctx_site_bonds")
+        return list(out), bool(ph.value)
+
+    def append_problem_at(self, bond, is_left, seed=0):
+        """(append descriptor, seeded old operators) of appending the
+        physical site on side `is_left` of bond `bond` to the operators of
+        that site's far bond."""
+        d, o = AppendDesc(), RopsDesc()
+        check(lib().t3ns_ctx_append_desc(self._h, int(bond), int(is_left), int(seed), C.byref(d),
+                                         C.byref(o)), "t3ns_ctx_append_desc")
+        return d, o
+
+    def pupdate_problem_at(self, bond, is_left, seed=0):
+        """update_rOperators_physical descriptor on bond `bond` (side
+        `is_left`) with a seeded one-site tensor over the context's sectors."""
+        d = PUpdateDesc()
+        check(lib().t3ns_ctx_pupdate_desc(self._h, int(bond), int(is_left), int(seed), C.byref(d)),
+              "t3ns_ctx_pupdate_desc")
+        return d
+
     def close(self):
         if self._h:
             lib().t3ns_ctx_destroy(self._h)
