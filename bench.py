@@ -13,7 +13,11 @@ the exact block structure of the renormalised operators of that step.
 Prints one JSON line: matvec/s of the plain FP64 path (value) and of the
 FP32 path, the roofline of the matvec kernel (HIP events on its stream), and
 the CPU baseline (the FP64 restatement in oracle/, OpenMP over output blocks,
-timed on a bounded sample of output blocks on this host).
+timed on a bounded sample of output blocks on this host).  The "solve" entry
+is the ground-state solve of the same step on the Hamiltonian's own
+operators (renormalised on the device from the vacuum inwards): plain FP64
+Davidson against FP32 matvecs with FP64 refinement, energies, iterations,
+seconds and |dE|.
 """
 import argparse
 import json
@@ -55,6 +59,10 @@ def parse():
     ap.add_argument("--precisions", default="f64,f32")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the renormalisation and site-tensor kernels")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="skip the ground-state solve on the Hamiltonian's own operators")
+    ap.add_argument("--solve-tol", type=float, default=1e-5,
+                    help="Davidson residual tolerance of the solve leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (scale reference for HBM counters)")
     return ap.parse_args()
@@ -198,6 +206,44 @@ def measure_rows(step, sites, bond_dim, cpu_dim=128):
     return rows
 
 
+def measure_solve(ctx, step, tol):
+    """Lowest eigenpair of the step's effective Hamiltonian built from the
+    Hamiltonian's own operators: every bond's operators renormalised on the
+    device from the vacuum inwards (t3ns_amd.environment, init_rops of
+    src/optimize_network.c:428-468) with seeded site tensors scaled to unit
+    rows.  Plain FP64 Davidson against FP32 matvecs with FP64 refinement,
+    from the same guess to the same residual tolerance."""
+    import t3ns_amd as T
+    from t3ns_amd.environment import Environment, normalised_rows
+    t0 = time.perf_counter()
+    env = Environment(ctx, seed=1, site_tensor=normalised_rows)
+    d, o, s = env.step_operators(step)
+    t_env = time.perf_counter() - t0
+    h64 = T.Heff(d, o, s, T.FP64, keepalive=(step, env))
+    h32 = T.Heff(d, o, s, T.FP32, keepalive=(step, env))
+    print("# solve: operators in %.1f s, flops/matvec %.4g" % (t_env, h64.stats()["flops"]),
+          file=sys.stderr, flush=True)
+    guess = np.random.default_rng(2).standard_normal(h64.size)
+    diag = h64.diagonal()
+    t0 = time.perf_counter()
+    e64, _, its, rc = h64.davidson(guess, tol=tol, max_its=400, diagonal=diag)
+    t64 = time.perf_counter() - t0
+    print("# solve FP64: E %.12f its %d (%.1f s)" % (e64, its, t64), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    em, _, its_lo, its_hi, rcm = T.davidson_refined(h32, h64, guess, tol=tol, switch_tol=tol,
+                                                    max_its=400, diagonal=diag)
+    tm = time.perf_counter() - t0
+    print("# solve mixed: E %.12f its %d+%d (%.1f s)" % (em, its_lo, its_hi, tm),
+          file=sys.stderr, flush=True)
+    return {"operators": "device renormalisation chain from the vacuum, seeded site tensors "
+                         "with unit rows", "operators_s": t_env,
+            "flops_per_matvec": h64.stats()["flops"], "tol": tol,
+            "fp64": {"energy": e64, "its": its, "status": rc, "seconds": t64},
+            "mixed": {"energy": em, "its_f32": its_lo, "its_f64": its_hi, "status": rcm,
+                      "seconds": tm},
+            "abs_dE": abs(em - e64), "speedup_vs_fp64": t64 / tm}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -301,6 +347,8 @@ def main():
                         "speedup_vs_f64": results["f32"]["matvec_per_s"] / r["matvec_per_s"],
                         "rel_err_vs_f64": err,
                         "roofline_frac": results["f32"]["tflops"] / PEAK_TFLOPS["f32"]}
+    if not args.no_solve:
+        out["solve"] = measure_solve(ctx, step, args.solve_tol)
     if not args.no_rows:
         t0 = time.perf_counter()
         out["rows"] = measure_rows(step, sites, args.bond_dim)

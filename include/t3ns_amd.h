@@ -158,9 +158,16 @@ int t3ns_davidson(t3ns_heff* h, double* result, double* energy, int max_vecs, in
                   double tol, int max_its, const double* diagonal, int* its);
 
 /* The same eigenpair with a reduced-precision operator for the bulk of the
- * iterations: Davidson with h_low's matvec (e.g. T3NS_FP32) until the
- * residual is below switch_tol, then Davidson with the FP64 operator h64
- * from that vector down to tol.  Same return codes. */
+ * iterations.  The reduced operator's relative error e is measured on the
+ * guess (one matvec of each); Davidson with h_low's matvec (e.g.
+ * T3NS_FP32) runs until the residual is below max(tol, switch_tol) or
+ * below 2 e |theta| (where its eigenvector is as close to the FP64 one as
+ * e allows).  If that met tol, one FP64 matvec gives the FP64 Rayleigh
+ * quotient and residual, the result when that residual is below tol
+ * (its64 = 0); otherwise the same search space (its keep_deflate lowest
+ * Ritz vectors, their images recomputed with h64) continues with FP64
+ * matvecs down to tol.  The energy is an FP64 Rayleigh quotient.  Same
+ * return codes. */
 int t3ns_davidson_refined(t3ns_heff* h_low, t3ns_heff* h64, double* result, double* energy,
                           int max_vecs, int keep_deflate, double tol, double switch_tol,
                           int max_its, const double* diagonal, int* its_low, int* its64);

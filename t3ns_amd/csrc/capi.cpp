@@ -283,6 +283,12 @@ int t3ns_davidson(t3ns_heff* h, double* result, double* energy, int max_vecs, in
     return rc != T3NS_OK ? rc : status;
 }
 
+namespace {
+// The reduced-precision phase of the refined solve stops at a residual of
+// kFloorFactor * (measured relative matvec error) * |theta|.
+constexpr double kFloorFactor = 2.0;
+}  // namespace
+
 int t3ns_davidson_refined(t3ns_heff* h_low, t3ns_heff* h64, double* result, double* energy,
                           int max_vecs, int keep_deflate, double tol, double switch_tol,
                           int max_its, const double* diagonal, int* its_low, int* its64) {
@@ -302,20 +308,38 @@ int t3ns_davidson_refined(t3ns_heff* h_low, t3ns_heff* h64, double* result, doub
             h64->dev->diagonal(d.get(), s);
         HeffDevice* lo = h_low->dev.get();
         HeffDevice* hi = h64->dev.get();
-        // Phase 1: reduced-precision matvecs until the residual reaches
-        // switch_tol.  Phase 2: FP64 matvecs from that vector down to tol
-        // (the FP64 Rayleigh-Ritz and residual correct the reduced-precision
-        // operator error).
-        DeviceDavidson d1(n, max_vecs, keep_deflate, s);
-        const DavidsonResult R1 = d1.solve(
-            x.get(), d.get(), [lo](const double* a, double* b, hipStream_t st) { lo->matvec(a, b, st); },
-            std::max(tol, switch_tol), max_its);
+        // The reduced-precision operator's relative error, measured on the
+        // guess, sets how far its iterations can usefully go: an
+        // eigenvector of it is as close to the FP64 one as that error
+        // allows once the residual is below about kFloorFactor * error *
+        // |theta|.  Phase 1: reduced-precision matvecs down to max(tol,
+        // switch_tol) or that floor.  Then, if phase 1 met tol, one FP64
+        // matvec gives the FP64 Rayleigh quotient and residual, the result
+        // when that residual meets tol; otherwise phase 2 resumes the same
+        // search space (its lowest Ritz vectors, images recomputed in FP64)
+        // with FP64 matvecs down to tol.
+        const DevMatvec mvlo = [lo](const double* a, double* b, hipStream_t st) { lo->matvec(a, b, st); };
+        const DevMatvec mv64 = [hi](const double* a, double* b, hipStream_t st) { hi->matvec(a, b, st); };
+        DeviceDavidson dv(n, max_vecs, keep_deflate, s);
+        const double rel = dv.relative_difference(x.get(), mv64, mvlo);
+        const DavidsonResult R1 = dv.solve(x.get(), d.get(), mvlo, std::max(tol, switch_tol), max_its,
+                                           kFloorFactor * rel);
         if (R1.status < 0) throw std::domain_error("davidson: projected eigenproblem failed");
-        DeviceDavidson d2(n, max_vecs, keep_deflate, s);
-        const DavidsonResult R2 = d2.solve(
-            x.get(), d.get(), [hi](const double* a, double* b, hipStream_t st) { hi->matvec(a, b, st); },
-            tol, max_its);
-        if (R2.status < 0) throw std::domain_error("davidson: projected eigenproblem failed");
+        DavidsonResult R2;
+        bool done = false;
+        if (R1.status == 0 && R1.residual <= tol) {
+            DeviceDavidson chk(n, 2, 1, s);
+            const auto rq = chk.rayleigh(x.get(), mv64);
+            if (rq.second <= tol) {
+                R2.energy = rq.first;
+                R2.residual = rq.second;
+                done = true;
+            }
+        }
+        if (!done) {
+            R2 = dv.resume(x.get(), d.get(), mv64, tol, std::max(1, max_its - R1.its));
+            if (R2.status < 0) throw std::domain_error("davidson: projected eigenproblem failed");
+        }
         T3_HIP_CHECK(hipMemcpyAsync(result, x.get(), n * sizeof(double), hipMemcpyDeviceToHost, s));
         T3_HIP_CHECK(hipStreamSynchronize(s));
         *energy = R2.energy;

@@ -73,6 +73,14 @@ __global__ void scale_kernel(double* __restrict__ v, int64_t n, double a) {
         v[i] *= a;
 }
 
+// t = y - theta x
+__global__ void residual_kernel(const double* __restrict__ y, const double* __restrict__ x,
+                                double theta, double* __restrict__ t, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        t[i] = y[i] - theta * x[i];
+}
+
 __global__ void copy_kernel(const double* __restrict__ a, double* __restrict__ b, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -260,16 +268,64 @@ void DeviceDavidson::deflate() {
     }
 }
 
-DavidsonResult DeviceDavidson::solve(double* x, const double* diag, const DevMatvec& mv,
-                                     double tol, int max_its) {
-    DavidsonResult R;
-    m_ = 0;
-    hipLaunchKernelGGL(copy_kernel, dim3(grid(size_)), dim3(256), 0, s_, x, t_.get(), size_);
-    double residual = tol * 10;
-    double energy = 0;
-    int its = 0;
+std::pair<double, double> DeviceDavidson::rayleigh(const double* x, const DevMatvec& mv) {
+    double* y = VA_.get();
+    mv(x, y, s_);
+    double xx, xy;
+    dots(x, size_, 1, x, &xx);
+    dots(x, size_, 1, y, &xy);
+    if (!(xx > 0)) throw std::runtime_error("davidson: zero vector");
+    const double theta = xy / xx;
+    hipLaunchKernelGGL(residual_kernel, dim3(grid(size_)), dim3(256), 0, s_, y, x, theta, t_.get(),
+                       size_);
+    return {theta, norm(t_.get()) / std::sqrt(xx)};
+}
+
+double DeviceDavidson::relative_difference(const double* x, const DevMatvec& a, const DevMatvec& b) {
+    double* ya = VA_.get();
+    double* yb = t_.get();
+    a(x, ya, s_);
+    b(x, yb, s_);
+    const double na = norm(ya);
+    hipLaunchKernelGGL(residual_kernel, dim3(grid(size_)), dim3(256), 0, s_, ya, yb, 1.0, tmp_.get(),
+                       size_);
+    return na > 0 ? norm(tmp_.get()) / na : 0.0;
+}
+
+// Ritz vector x = V y, residual t = VA y - theta x (src/davidson.c:187-202),
+// then the preconditioned correction of t against x (src/davidson.c:
+// 204-257).  Returns the residual norm; theta is eigval_[0].
+double DeviceDavidson::ritz_and_correct(double* x, const double* diag) {
     std::vector<double> c(mv_);
+    const double theta = eigval_[0];
+    combine(V_.get(), m_, eigv_.data(), x, 0.0);
+    combine(VA_.get(), m_, eigv_.data(), t_.get(), 0.0);
+    for (int j = 0; j < m_; ++j) c[j] = -theta * eigv_[j];
+    combine(V_.get(), m_, c.data(), t_.get(), 1.0);
+    const double residual = norm(t_.get());
+    hipLaunchKernelGGL(precond_dots_kernel, dim3(nblk_), dim3(kRedThreads), 0, s_, x, t_.get(),
+                       diag, theta, size_, partial_.get());
+    hipLaunchKernelGGL(finish_kernel, dim3(2), dim3(kRedThreads), 0, s_, partial_.get(), nblk_,
+                       2, out_.get());
+    double uk[2];
+    T3_HIP_CHECK(hipMemcpyAsync(uk, out_.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
+    T3_HIP_CHECK(hipStreamSynchronize(s_));
+    const double alpha = -uk[0] / uk[1];
+    hipLaunchKernelGGL(precond_apply_kernel, dim3(grid(size_)), dim3(256), 0, s_, t_.get(), x,
+                       diag, theta, alpha, size_);
+    return residual;
+}
+
+DavidsonResult DeviceDavidson::iterate(double* x, const double* diag, const DevMatvec& mv,
+                                       double tol, int max_its, double rel_floor, int its,
+                                       double residual) {
+    DavidsonResult R;
+    bool floor_hit = false;
     while (residual > tol && its < max_its) {
+        if (rel_floor > 0 && m_ > 0 && residual <= rel_floor * std::fabs(eigval_[0])) {
+            floor_hit = true;
+            break;
+        }
         orthonormalise(t_.get());
         double* Vm = V_.get() + (int64_t)m_ * size_;
         double* VAm = VA_.get() + (int64_t)m_ * size_;
@@ -288,34 +344,47 @@ DavidsonResult DeviceDavidson::solve(double* x, const double* diag, const DevMat
                 return R;
             }
         }
-        // residual (src/davidson.c:187-202): x = V y, t = VA y - theta x
-        const double theta = eigval_[0];
-        combine(V_.get(), m_, eigv_.data(), x, 0.0);
-        combine(VA_.get(), m_, eigv_.data(), t_.get(), 0.0);
-        for (int j = 0; j < m_; ++j) c[j] = -theta * eigv_[j];
-        combine(V_.get(), m_, c.data(), t_.get(), 1.0);
-        residual = norm(t_.get());
-        energy = theta;
+        residual = ritz_and_correct(x, diag);
         ++its;
-        // preconditioned correction of the residual t against the Ritz
-        // vector x (src/davidson.c:204-257)
-        hipLaunchKernelGGL(precond_dots_kernel, dim3(nblk_), dim3(kRedThreads), 0, s_, x, t_.get(),
-                           diag, theta, size_, partial_.get());
-        hipLaunchKernelGGL(finish_kernel, dim3(2), dim3(kRedThreads), 0, s_, partial_.get(), nblk_,
-                           2, out_.get());
-        double uk[2];
-        T3_HIP_CHECK(hipMemcpyAsync(uk, out_.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
-        T3_HIP_CHECK(hipStreamSynchronize(s_));
-        const double alpha = -uk[0] / uk[1];
-        hipLaunchKernelGGL(precond_apply_kernel, dim3(grid(size_)), dim3(256), 0, s_, t_.get(), x,
-                           diag, theta, alpha, size_);
     }
     T3_HIP_CHECK(hipStreamSynchronize(s_));
-    R.energy = energy;
+    R.energy = m_ > 0 ? eigval_[0] : 0.0;
     R.residual = residual;
     R.its = its;
-    R.status = residual > tol ? 1 : 0;
+    R.status = residual <= tol ? 0 : floor_hit ? 2 : 1;
     return R;
+}
+
+DavidsonResult DeviceDavidson::solve(double* x, const double* diag, const DevMatvec& mv,
+                                     double tol, int max_its, double rel_floor) {
+    m_ = 0;
+    hipLaunchKernelGGL(copy_kernel, dim3(grid(size_)), dim3(256), 0, s_, x, t_.get(), size_);
+    return iterate(x, diag, mv, tol, max_its, rel_floor, 0, HUGE_VAL);
+}
+
+DavidsonResult DeviceDavidson::resume(double* x, const double* diag, const DevMatvec& mv,
+                                      double tol, int max_its) {
+    if (m_ == 0) return solve(x, diag, mv, tol, max_its);
+    if (m_ > keep_) {
+        // keep the lowest Ritz vectors of the current projected problem
+        for (int k = 0; k < keep_; ++k)
+            combine(V_.get(), m_, eigv_.data() + (size_t)k * mv_, tmp_.get() + (int64_t)k * size_, 0.0);
+        hipLaunchKernelGGL(copy_kernel, dim3(grid(size_ * keep_)), dim3(256), 0, s_, tmp_.get(),
+                           V_.get(), size_ * keep_);
+        m_ = keep_;
+    }
+    std::fill(sub_.begin(), sub_.end(), 0.0);
+    for (int k = 0; k < m_; ++k) {
+        mv(V_.get() + (int64_t)k * size_, VA_.get() + (int64_t)k * size_, s_);
+        expand(k);
+    }
+    DavidsonResult R;
+    if (eigsolve() != 0) {
+        R.status = -1;
+        return R;
+    }
+    const double residual = ritz_and_correct(x, diag);
+    return iterate(x, diag, mv, tol, max_its, 0.0, 1, residual);
 }
 
 }  // namespace t3

@@ -14,6 +14,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <utility>
 #include <vector>
 
 #include "hip_util.hpp"
@@ -27,7 +28,7 @@ struct DavidsonResult {
     double energy = 0;
     double residual = 0;
     int its = 0;
-    int status = 0;  // 0 converged, 1 max_its reached, -1 eigensolver failure
+    int status = 0;  // 0 converged, 1 max_its reached, 2 at the precision floor, -1 failure
 };
 
 class DeviceDavidson {
@@ -35,8 +36,21 @@ public:
     DeviceDavidson(int64_t size, int max_vecs, int keep_deflate, hipStream_t s);
     // x (device, size): initial guess in, eigenvector out.  diag (device):
     // the operator's diagonal for the preconditioner.
+    // rel_floor > 0: also stop (status 2) once the residual is below
+    // rel_floor * |theta|, the accuracy limit of a reduced-precision matvec
+    // whose relative error is about rel_floor.
     DavidsonResult solve(double* x, const double* diag, const DevMatvec& mv, double tol,
-                         int max_its);
+                         int max_its, double rel_floor = 0);
+    // Continues the last solve with another matvec: the basis is deflated
+    // to its keep_deflate lowest Ritz vectors, their images are recomputed
+    // with mv, and the iteration goes on from there.
+    DavidsonResult resume(double* x, const double* diag, const DevMatvec& mv, double tol,
+                          int max_its);
+    // Rayleigh quotient of x (device) under mv and the norm of its
+    // residual, both relative to |x|: (x.Ax / x.x, |Ax - theta x| / |x|).
+    std::pair<double, double> rayleigh(const double* x, const DevMatvec& mv);
+    // |A x - B x| / |A x| for two matvecs of one operator.
+    double relative_difference(const double* x, const DevMatvec& a, const DevMatvec& b);
 
 private:
     void dots(const double* A, int64_t lda, int n, const double* b, double* host_out);
@@ -46,6 +60,9 @@ private:
     void expand(int m);
     int eigsolve();
     void deflate();
+    double ritz_and_correct(double* x, const double* diag);
+    DavidsonResult iterate(double* x, const double* diag, const DevMatvec& mv, double tol,
+                           int max_its, double rel_floor, int its, double residual);
 
     int64_t size_;
     int mv_, keep_;

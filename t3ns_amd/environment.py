@@ -17,7 +17,49 @@ the updates used, so the effective Hamiltonian of a step is symmetric, and
 with isometric tensors and no truncation it is the Hamiltonian itself on the
 step's space.
 """
+import numpy as np
+
 from .heff import append_phys, update_branching, update_physical
+
+
+def normalised_rows(site, desc, keep):
+    """site_tensor hook: scales the seeded tensor of an update descriptor so
+    that every state of the updated bond (leg 2 of a left physical update,
+    leg 0 of a right one, leg `update_case` of a branching update) has unit
+    norm over the tensor's other two legs.  Rows of N(0,1) entries in high
+    dimension are nearly orthogonal, so the tensor is close to an isometry
+    and the operators keep the Hamiltonian's scale through any number of
+    updates (unscaled N(0,1) tensors multiply it by about the row length at
+    every update)."""
+    leg = desc.update_case if hasattr(desc, "update_case") else (2 if desc.is_left else 0)
+    ss = desc.site_ss
+    n0, n1 = ss[0].nrSecs, ss[1].nrSecs
+    nb = desc.T_nrblocks
+    if nb == 0:
+        return
+    qn = np.ctypeslib.as_array(desc.T_qnumbers, shape=(nb,)).astype(np.int64)
+    bb = np.ctypeslib.as_array(desc.T_beginblock, shape=(nb + 1,)).astype(np.int64)
+    tel = np.ctypeslib.as_array(desc.T_tel, shape=(max(int(bb[-1]), 1),))
+    ids = np.stack([qn % n0, (qn // n0) % n1, qn // (n0 * n1)], axis=1)
+    dims = [np.array([ss[k].dims[i] for i in range(ss[k].nrSecs)], dtype=np.int64) for k in range(3)]
+    other = tuple(a for a in range(3) if a != leg)
+    sq = {}
+    views = []
+    for b in range(nb):
+        n = int(bb[b + 1] - bb[b])
+        if n == 0:
+            continue
+        shape = tuple(int(dims[k][ids[b, k]]) for k in range(3))
+        v = tel[bb[b]:bb[b + 1]].reshape(shape, order="F")
+        sec = int(ids[b, leg])
+        part = (v * v).sum(axis=other)
+        sq[sec] = sq[sec] + part if sec in sq else part
+        views.append((v, sec))
+    for v, sec in views:
+        scale = 1.0 / np.sqrt(np.maximum(sq[sec], 1e-300))
+        shp = [1, 1, 1]
+        shp[leg] = -1
+        v *= scale.reshape(shp)
 
 
 class Environment:
