@@ -17,7 +17,10 @@ timed on a bounded sample of output blocks on this host).  The "solve" entry
 is the ground-state solve of the same step on the Hamiltonian's own
 operators (renormalised on the device from the vacuum inwards): plain FP64
 Davidson against FP32 matvecs with FP64 refinement, energies, iterations,
-seconds and |dE|.
+seconds and |dE|.  The "sweep" entry is the two-site DMRG of the 10-site
+NN_HUBBARD chain (t = 1, U = 4, half filling, D = 64, up to 4 sweeps,
+E_CONV = 1e-10) with every step on the device: seconds and energies, FP64
+and mixed.
 """
 import argparse
 import json
@@ -61,6 +64,8 @@ def parse():
                     help="skip the renormalisation and site-tensor kernels")
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the ground-state solve on the Hamiltonian's own operators")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the DMRG sweep of the 10-site Hubbard chain")
     ap.add_argument("--solve-tol", type=float, default=1e-5,
                     help="Davidson residual tolerance of the solve leg")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -244,6 +249,31 @@ def measure_solve(ctx, step, tol):
             "abs_dE": abs(em - e64), "speedup_vs_fp64": t64 / tm}
 
 
+def measure_sweep(sweeps=4, e_conv=1e-10):
+    """Two-site DMRG of the NN_HUBBARD(t = 1, U = 4) 10-site chain
+    (makenetwork.py 10 DMRG), 10 electrons (U1 x U1: 5 up, 5 down), D = 64,
+    up to 4 sweeps with E_CONV = 1e-10, seeded canonical start: every step
+    on the device (makesiteTensor, renormalisation, Heff + Davidson, split),
+    FP64 and FP32 matvecs with FP64 refinement."""
+    import t3ns_amd as T
+    from t3ns_amd.sweep import ChainDMRG, DeviceSolver
+    with open(os.path.join(ROOT, "tests", "golden", "networks", "10_DMRG.netw")) as f:
+        netw = f.read()
+    out = {"workload": "nn_hubbard_10_chain_D64", "max_sweeps": sweeps, "e_conv": e_conv}
+    for prec in ("f64", "mixed"):
+        ctx = T.Context(netw, "NN_HUBBARD", [1.0, 4.0], ["U1", "U1"], [5, 5], 64)
+        t0 = time.perf_counter()
+        dm = ChainDMRG(ctx, 64, seed=1, solver=DeviceSolver(prec), tol=1e-8)
+        e = dm.run(sweeps, e_conv=e_conv)
+        t = time.perf_counter() - t0
+        out[prec] = {"energy": e, "sweeps": len(dm.energies), "seconds": t,
+                     "seconds_per_sweep": t / len(dm.energies), "sweep_energies": dm.energies}
+        print("# sweep %s: E %.12f in %d sweeps, %.2f s" % (prec, e, len(dm.energies), t),
+              file=sys.stderr, flush=True)
+    out["abs_dE"] = abs(out["mixed"]["energy"] - out["f64"]["energy"])
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -349,6 +379,8 @@ def main():
                         "roofline_frac": results["f32"]["tflops"] / PEAK_TFLOPS["f32"]}
     if not args.no_solve:
         out["solve"] = measure_solve(ctx, step, args.solve_tol)
+    if not args.no_sweep:
+        out["sweep"] = measure_sweep()
     if not args.no_rows:
         t0 = time.perf_counter()
         out["rows"] = measure_rows(step, sites, args.bond_dim)

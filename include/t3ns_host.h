@@ -48,6 +48,15 @@ int t3ns_ctx_bond_sites(const t3ns_ctx* c, int bond, int* sites2);
 /* The three bonds of site `site` (get_bonds_of_site, src/network.c:430) and
  * whether it is a physical site. */
 int t3ns_ctx_site_bonds(const t3ns_ctx* c, int site, int* bonds3, int* is_phys);
+/* Replaces the sectors of bond `bond` (irreps [nrsecs][T3NS_MAX_SYMMETRIES],
+ * dims [nrsecs]), as a decomposition leaves them: the bookkeeper update of
+ * adapt_UV_tensors_and_kick_empties (src/siteTensor_decompose.c:1360-1401). */
+int t3ns_ctx_set_bond(t3ns_ctx* c, int bond, int nrsecs, const int* irreps, const int* dims);
+/* A one-site tensor of site `site` over the context's current sectors
+ * (init_1siteTensor, src/siteTensor_init.c:64-108) with seeded N(0,1) elements;
+ * the arrays live as long as the context and the elements are writable. */
+int t3ns_ctx_site_tensor(t3ns_ctx* c, int site, uint64_t seed, int* nrblocks, const int64_t** qnumbers,
+                         const int64_t** beginblock, double** tel);
 /* Sector count and dims of bond `bond` (reference numbering). */
 int t3ns_ctx_bond(const t3ns_ctx* c, int bond, int* nrsecs, int* totaldims, int* dims_out,
                   int dims_cap);

@@ -147,6 +147,7 @@ struct t3ns_ctx {
     std::vector<std::unique_ptr<QRStore>> qrs;
     std::vector<std::unique_ptr<AppendStore>> apps;
     std::vector<std::unique_ptr<PUpdateStore>> pups;
+    std::vector<std::unique_ptr<SiteTensor>> sites;
 };
 
 struct MakesiteStore {
@@ -492,6 +493,49 @@ int t3ns_ctx_site_bonds(const t3ns_ctx* c, int site, int* bonds3, int* is_phys) 
     c->c->netw.bonds_of_site(site, bonds3);
     if (is_phys) *is_phys = c->c->netw.is_psite(site) ? 1 : 0;
     return T3NS_OK;
+}
+
+int t3ns_ctx_set_bond(t3ns_ctx* c, int bond, int nrsecs, const int* irreps, const int* dims) {
+    if (!c || nrsecs < 0 || (nrsecs && (!irreps || !dims))) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        Context& x = *c->c;
+        if (bond < 0 || bond >= x.netw.nr_bonds) throw std::invalid_argument("set_bond: bad bond");
+        Symsecs& slot = x.symsecs_mut(bond);
+        Symsecs ns;
+        ns.bond = slot.bond;
+        ns.nrSecs = nrsecs;
+        ns.irreps.resize(nrsecs);
+        ns.dims.assign(dims, dims + nrsecs);
+        ns.fcidims.assign(nrsecs, 0.0);
+        for (int i = 0; i < nrsecs; ++i) {
+            if (dims[i] < 0) throw std::invalid_argument("set_bond: negative dimension");
+            for (int k = 0; k < T3NS_MAX_SYMMETRIES; ++k)
+                ns.irreps[i][k] = irreps[i * T3NS_MAX_SYMMETRIES + k];
+            // keep the full-CI dimension of a sector the bond had before
+            for (int j = 0; j < slot.nrSecs; ++j)
+                if (x.irreps_equal(slot.irreps[j].data(), ns.irreps[i].data()) && !slot.fcidims.empty())
+                    ns.fcidims[i] = slot.fcidims[j];
+            ns.totaldims += dims[i];
+        }
+        slot = std::move(ns);
+    });
+}
+
+int t3ns_ctx_site_tensor(t3ns_ctx* c, int site, uint64_t seed, int* nrblocks, const int64_t** qnumbers,
+                         const int64_t** beginblock, double** tel) {
+    if (!c || !nrblocks || !qnumbers || !beginblock || !tel) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        Context& x = *c->c;
+        if (site < 0 || site >= x.netw.sites) throw std::invalid_argument("site_tensor: bad site");
+        auto t = std::make_unique<SiteTensor>(init_1siteTensor(x, site));
+        Rng rng(seed);
+        for (auto& v : t->tel) v = rng.normal();
+        *nrblocks = t->nrblocks;
+        *qnumbers = t->qnumbers.data();
+        *beginblock = t->beginblock.data();
+        *tel = t->tel.data();
+        c->sites.push_back(std::move(t));
+    });
 }
 
 int t3ns_ctx_bond(const t3ns_ctx* c, int bond, int* nrsecs, int* totaldims, int* dims_out,

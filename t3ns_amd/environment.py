@@ -77,8 +77,8 @@ class Environment:
         self.seed = int(seed)
         self.site_tensor = site_tensor
         self._append, self._uphys, self._ubranch = append, update_phys, update_branch
-        self._ops = {}
-        self._keep = []
+        self._ops = {}    # key -> (operators, arrays they were made from)
+        self._cover = {}  # (bond, side) -> sites on that side
         self._count = 0
 
     def _next_seed(self):
@@ -95,10 +95,10 @@ class Environment:
 
     def physical(self, bond, is_left):
         """Physical operators on `bond`: the site on side `is_left` appended to
-        the operators of its far bond (rOperators_append_phys)."""
-        key = ("P", int(bond), int(bool(is_left)))
-        if key in self._ops:
-            return self._ops[key]
+        the operators of its far bond (rOperators_append_phys).  Not cached:
+        they live on the bond's current sectors, which differ between a step
+        where the bond is internal (the product sectors of the step's
+        tensor) and one where it is not."""
         L = int(bool(is_left))
         site = self._outer_site(bond, L)
         bonds, phys = self.ctx.site_bonds(site)
@@ -110,27 +110,27 @@ class Environment:
             old = self.operators(far, L).desc
         r = self._append(ad, old)
         r.desc.bond = bond
-        self._keep.append((ad, r))
-        self._ops[key] = r
+        r.made_from = ad
         return r
 
     def operators(self, bond, is_left):
         """Renormalised operators on `bond` covering its side `is_left`."""
         key = ("R", int(bond), int(bool(is_left)))
         if key in self._ops:
-            return self._ops[key]
+            return self._ops[key][0]
         L = int(bool(is_left))
         site = self._outer_site(bond, L)
         if site == -1:
             raise ValueError("bond %d has a border on that side (vacuum operators)" % bond)
         bonds, phys = self.ctx.site_bonds(site)
+        keep = []
         if phys:
             P = self.physical(bond, L)
             pd = self.ctx.pupdate_problem_at(bond, L, self._next_seed())
             if self.site_tensor is not None:
-                self.site_tensor(site, pd, self._keep)
+                self.site_tensor(site, pd, keep)
             r = self._uphys(pd, P.desc)
-            self._keep.append(pd)
+            keep.append(pd)
         else:
             case = bonds.index(bond)
             d, o = self.ctx.bupdate_problem(site, case, self._next_seed())
@@ -141,12 +141,41 @@ class Environment:
                 if not self.is_border(obond[k], oleft[k]):
                     o[k] = self.operators(obond[k], oleft[k]).desc
             if self.site_tensor is not None:
-                self.site_tensor(site, d, self._keep)
+                self.site_tensor(site, d, keep)
             r = self._ubranch(d, o)
-            self._keep.append((d, o))
+            keep.append((d, o))
         r.desc.bond = bond
-        self._ops[key] = r
+        self._ops[key] = (r, keep)
         return r
+
+    def covered(self, bond, is_left):
+        """The sites on side `is_left` of `bond`."""
+        key = (int(bond), int(bool(is_left)))
+        if key not in self._cover:
+            nr_bonds = self.ctx.info()["nr_bonds"]
+            start = self._outer_site(bond, is_left)
+            seen, stack = set(), [start] if start != -1 else []
+            while stack:
+                s = stack.pop()
+                if s in seen:
+                    continue
+                seen.add(s)
+                for b in self.ctx.site_bonds(s)[0]:
+                    if b == bond or b < 0 or b >= nr_bonds:
+                        continue
+                    for o in self.ctx.bond_sites(b):
+                        if o != -1 and o not in seen:
+                            stack.append(o)
+            self._cover[key] = frozenset(seen)
+        return self._cover[key]
+
+    def invalidate(self, sites):
+        """Drops every operator set that covers one of `sites` (their
+        tensors changed); they are rebuilt on the next request."""
+        sites = set(sites)
+        for key in list(self._ops):
+            if self.covered(key[1], key[2]) & sites:
+                del self._ops[key]
 
     def step_operators(self, step):
         """(desc, ops[3], site) of `step` with its operators replaced by the
@@ -154,12 +183,15 @@ class Environment:
         site to the operators of the external bond, other legs those
         operators themselves, border legs stay the vacuum."""
         d, o, s = step.descriptors()
+        self._step_ops = []
         for i in range(3):
             if o[i].nrhss <= 0:
                 continue
             L = o[i].is_left
             if o[i].P_operator:
-                o[i] = self.physical(o[i].bond, L).desc
+                r = self.physical(o[i].bond, L)
+                self._step_ops.append(r)  # alive until the next step's request
+                o[i] = r.desc
             elif not self.is_border(o[i].bond, L):
                 o[i] = self.operators(o[i].bond, L).desc
         return d, o, s

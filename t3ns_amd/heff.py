@@ -100,6 +100,30 @@ class Context:
         check(lib().t3ns_ctx_site_bonds(self._h, int(site), out, C.byref(ph)), "t3ns_ctx_site_bonds")
         return list(out), bool(ph.value)
 
+    def set_bond(self, bond, irreps, dims):
+        """Replaces the sectors of bond `bond`: irreps [n][MAX_SYMMETRIES],
+        dims [n] (as a decomposition leaves them)."""
+        ir = np.ascontiguousarray(irreps, dtype=np.int32).reshape(-1)
+        dm = np.ascontiguousarray(dims, dtype=np.int32)
+        if ir.size != dm.size * _lib.MAX_SYMMETRIES:
+            raise ValueError("set_bond: irreps and dims disagree")
+        check(lib().t3ns_ctx_set_bond(self._h, int(bond), int(dm.size),
+                                      ir.ctypes.data_as(_lib.c_int_p), dm.ctypes.data_as(_lib.c_int_p)),
+              "t3ns_ctx_set_bond")
+
+    def site_tensor(self, site, seed=0):
+        """One-site tensor of `site` over the current sectors with seeded
+        N(0,1) elements: (qnumbers, beginblock, elements) as numpy copies."""
+        n = C.c_int()
+        qn, bb, tel = _lib.c_i64_p(), _lib.c_i64_p(), _lib.c_dbl_p()
+        check(lib().t3ns_ctx_site_tensor(self._h, int(site), int(seed), C.byref(n), C.byref(qn),
+                                         C.byref(bb), C.byref(tel)), "t3ns_ctx_site_tensor")
+        nb = n.value
+        q = np.ctypeslib.as_array(qn, shape=(max(nb, 1),))[:nb].copy()
+        b = np.ctypeslib.as_array(bb, shape=(nb + 1,)).copy()
+        t = np.ctypeslib.as_array(tel, shape=(max(int(b[-1]), 1),))[:int(b[-1])].copy()
+        return q, b, t
+
     def append_problem_at(self, bond, is_left, seed=0):
         """(append descriptor, seeded old operators) of appending the
         physical site on side `is_left` of bond `bond` to the operators of

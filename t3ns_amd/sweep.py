@@ -1,0 +1,212 @@
+"""Two-site DMRG sweeps on chain networks.
+
+The optimisation loop of src/optimize_network.c:488-570 (execute_optScheme,
+optimize_network: one step per site pair of the sweep order of
+src/network.c:529-727) for the two-site DMRG case, on the library's entry
+points: per step, makesiteTensor of the pair from its one-site tensors
+(src/siteTensor_init.c:785-800), the step's operators
+(src/optimize_network.c:164-176, through Environment), the lowest
+eigenpair of its effective Hamiltonian (Davidson, FP64 or FP32 with FP64
+refinement), one split of decompose_siteTensor with truncation
+(src/siteTensor_decompose.c:1403-1444) that leaves the pair's site shared
+with the next step as the orthogonality centre, the cut bond's new sectors
+written into the context (the bookkeeper update of
+adapt_UV_tensors_and_kick_empties), and the operators covering the two
+sites dropped so that the next step renormalises them afresh
+(postprocess_rOperators, src/optimize_network.c:236-311).
+
+The starting state is seeded and in canonical form toward the first step:
+every other site tensor is an isometry onto its bond toward the first pair
+(the reference starts from a random state and a renormalisation pass,
+src/optimize_network.c:574-640).  Tree networks (branching sites, three-
+and four-site steps with HOSVD) are not covered here.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .environment import Environment
+from .heff import FP32, FP64, Heff, Step, davidson_refined, make_site_tensor, split_site
+
+
+class SiteData:
+    """A one-site tensor on the host: qnumbers, beginblock, elements."""
+
+    def __init__(self, qn, bb, tel):
+        self.qn = np.ascontiguousarray(qn, dtype=np.int64)
+        self.bb = np.ascontiguousarray(bb, dtype=np.int64)
+        self.tel = np.ascontiguousarray(tel if len(tel) else np.zeros(1), dtype=np.float64)
+
+    @property
+    def nrblocks(self):
+        return len(self.qn)
+
+    def pointers(self):
+        return (self.qn.ctypes.data_as(_lib.c_i64_p), self.bb.ctypes.data_as(_lib.c_i64_p),
+                self.tel.ctypes.data_as(_lib.c_dbl_p))
+
+
+def _site_dims(ctx, site):
+    bonds, _ = ctx.site_bonds(site)
+    return [np.array(ctx.bond_dims(b), dtype=np.int64) for b in bonds]
+
+
+def isometrise(ctx, site, data, leg, rng):
+    """Makes `data` an isometry onto leg `leg`: for every sector of that
+    leg (d states) the rows over all the tensor's other index pairs (c >= d
+    of them) are orthonormal, drawn from `rng`."""
+    dims = _site_dims(ctx, site)
+    n0, n1 = len(dims[0]), len(dims[1])
+    ids = np.stack([data.qn % n0, (data.qn // n0) % n1, data.qn // (n0 * n1)], axis=1)
+    views = {}
+    for b in range(data.nrblocks):
+        if data.bb[b + 1] == data.bb[b]:
+            continue
+        shape = tuple(int(dims[k][ids[b, k]]) for k in range(3))
+        v = data.tel[data.bb[b]:data.bb[b + 1]].reshape(shape, order="F")
+        views.setdefault(int(ids[b, leg]), []).append(np.moveaxis(v, leg, 0))
+    for sec, vs in views.items():
+        d = vs[0].shape[0]
+        sizes = [v[0].size for v in vs]
+        c = sum(sizes)
+        if c < d:
+            raise ValueError("sector %d of site %d is larger than its product space" % (sec, site))
+        Q, _ = np.linalg.qr(rng.standard_normal((c, d)))
+        rows = Q.T
+        k = 0
+        for v, m in zip(vs, sizes):
+            v[...] = rows[:, k:k + m].reshape(v.shape)
+            k += m
+
+
+class DeviceSolver:
+    """The device entry points for the steps: makesiteTensor, Heff +
+    Davidson (precision "f64" or "mixed"), split."""
+
+    env = {}
+
+    def __init__(self, precision="f64", max_vecs=30, keep_deflate=2, max_its=300):
+        if precision not in ("f64", "mixed"):
+            raise ValueError("precision is f64 or mixed")
+        self.precision = precision
+        self.max_vecs, self.keep, self.max_its = max_vecs, keep_deflate, max_its
+
+    def make_site_tensor(self, desc):
+        return make_site_tensor(desc)[0]
+
+    def solve(self, d, o, s, guess, tol, keep):
+        h64 = Heff(d, o, s, FP64, keepalive=keep)
+        try:
+            if self.precision == "f64":
+                e, x, its, rc = h64.davidson(guess, self.max_vecs, self.keep, tol, self.max_its)
+                return e, x, (its, 0)
+            h32 = Heff(d, o, s, FP32, keepalive=keep)
+            try:
+                e, x, i1, i2, rc = davidson_refined(h32, h64, guess, self.max_vecs, self.keep, tol,
+                                                    tol, self.max_its)
+                return e, x, (i1, i2)
+            finally:
+                h32.close()
+        finally:
+            h64.close()
+
+    def split(self, desc):
+        return split_site(desc)[0]
+
+
+class ChainDMRG:
+    """Two-site DMRG over the sweep order of a chain network.
+
+    `solver` provides make_site_tensor(desc), solve(d, o, s, guess, tol,
+    keep) -> (energy, vector, iterations) and split(desc) -> split dict,
+    and `solver.env` the renormalisation calls for the Environment (all
+    device calls by default)."""
+
+    def __init__(self, ctx, max_dim, seed=0, solver=None, tol=1e-8, truncerr=0.0):
+        info = ctx.info()
+        if info["psites"] != info["sites"]:
+            raise ValueError("ChainDMRG needs a chain network (no branching sites)")
+        self.ctx, self.max_dim, self.tol, self.truncerr = ctx, int(max_dim), float(tol), truncerr
+        self.solver = solver if solver is not None else DeviceSolver()
+        self.seed = int(seed)
+        self.steps = ctx.sweep_steps(2)
+        first = self.steps[0]
+        rng = np.random.default_rng(seed)
+        self.tensors = {}
+        for s in range(info["sites"]):
+            data = SiteData(*ctx.site_tensor(s, seed * 7919 + s + 1))
+            if s not in first:
+                isometrise(ctx, s, data, 2 if s < min(first) else 0, rng)
+            self.tensors[s] = data
+        self.env = Environment(ctx, seed, site_tensor=self._site_tensor, **self.solver.env)
+        self.energies = []
+
+    def _site_tensor(self, site, desc, keep):
+        data = self.tensors[site]
+        qn, bb, tel = data.pointers()
+        desc.T_nrblocks = data.nrblocks
+        desc.T_qnumbers, desc.T_beginblock, desc.T_tel = qn, bb, tel
+        keep.append(data)
+
+    def _leaving_site(self, k):
+        """The site of step k that the orthogonality centre leaves: the one
+        the next step (or, where the sweep turns on the same pair, the one
+        after) does not hold."""
+        sites = set(self.steps[k])
+        n = len(self.steps)
+        for j in (1, 2):
+            common = sites & set(self.steps[(k + j) % n])
+            if len(common) == 1:
+                return (sites - common).pop()
+        return self.steps[k][-1]
+
+    def step(self, k):
+        sites = self.steps[k]
+        st = Step(self.ctx, sites, seed=self.seed)
+        try:
+            d, o, s = st.descriptors()
+            order = [s.sites[i] for i in range(s.nrsites)]
+            mk = st.makesite_problem(self.seed)
+            for i, site in enumerate(order):
+                data = self.tensors[site]
+                qn, bb, tel = data.pointers()
+                mk.site_nrblocks[i] = data.nrblocks
+                mk.site_qnumbers[i], mk.site_beginblock[i], mk.site_tel[i] = qn, bb, tel
+            x0 = np.array(self.solver.make_site_tensor(mk), dtype=np.float64)
+            x0 /= np.linalg.norm(x0)
+            d, o, s = self.env.step_operators(st)
+            e, x, its = self.solver.solve(d, o, s, x0, self.tol, (st, self.env))
+            vsite = self._leaving_site(k)
+            sd = st.split_problem(order.index(vsite), self.max_dim, 0, self.truncerr)
+            xn = np.ascontiguousarray(x / np.linalg.norm(x))
+            sd.tel = xn.ctypes.data_as(_lib.c_dbl_p)
+            res = self.solver.split(sd)
+            usite = order[1 - order.index(vsite)]
+            cut = (set(self.ctx.site_bonds(sites[0])[0]) & set(self.ctx.site_bonds(sites[1])[0])).pop()
+            self.ctx.set_bond(cut, res["bond_irreps"], res["bond_dims"])
+            self.tensors[vsite] = SiteData(res["V_qnumbers"], res["V_beginblock"], res["V_tel"])
+            self.tensors[usite] = SiteData(res["U_qnumbers"], res["U_beginblock"], res["U_tel"])
+            self.env.invalidate(sites)
+            return {"sites": list(sites), "energy": e, "its": its,
+                    "discarded": 1.0 - res["norm"][1] ** 2 / max(res["norm"][0] ** 2, 1e-300),
+                    "bond_dim": int(np.sum(res["bond_dims"]))}
+        finally:
+            st.close()
+
+    def sweep(self):
+        """One sweep; returns the lowest step energy (the reference reports
+        the minimum, src/optimize_network.c:650-654) and the step records."""
+        recs = [self.step(k) for k in range(len(self.steps))]
+        e = min(r["energy"] for r in recs)
+        self.energies.append(e)
+        return e, recs
+
+    def run(self, sweeps, e_conv=0.0):
+        """Up to `sweeps` sweeps, stopping when two consecutive sweep energies
+        differ by less than e_conv (src/optimize_network.c:410-415)."""
+        for i in range(sweeps):
+            self.sweep()
+            if i > 0 and abs(self.energies[-1] - self.energies[-2]) < e_conv:
+                break
+        return min(self.energies)

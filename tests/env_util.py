@@ -171,3 +171,30 @@ def hubbard_edges(ctx):
             for j in range(i + 1, 3):
                 edges.add((min(nb[i], nb[j]), max(nb[i], nb[j])))
     return sorted(edges)
+
+
+class OracleSolver:
+    """The oracle's makesiteTensor, Heff eigenpair (dense below 800 states,
+    Lanczos above) and split, in the form t3ns_amd.sweep.ChainDMRG takes."""
+
+    env = ORACLE_BACKEND
+
+    def make_site_tensor(self, desc):
+        return O.make_site_tensor(desc)
+
+    def solve(self, d, o, s, guess, tol, keep):
+        import scipy.sparse.linalg as sla
+        oh = O.OracleHeff(d, o, s)
+        try:
+            n = len(guess)
+            if n <= 800:
+                w, v = np.linalg.eigh(dense(oh.matvec, n))
+                return w[0], v[:, 0], (0, 0)
+            A = sla.LinearOperator((n, n), matvec=oh.matvec, dtype=float)
+            w, v = sla.eigsh(A, k=1, which="SA", v0=guess, tol=1e-12)
+            return w[0], v[:, 0], (0, 0)
+        finally:
+            oh.close()
+
+    def split(self, desc):
+        return O.split_site(desc)

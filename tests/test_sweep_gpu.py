@@ -1,0 +1,44 @@
+"""Two-site DMRG sweeps on the device (t3ns_amd.sweep.ChainDMRG with the
+device makesiteTensor, renormalisation, Heff + Davidson and split): the
+reference's N2/STO-3G regression energy -107.648250974014 within its
+tolerance 1e-8 (tests/test1.c.in:99), FP64 and FP32 matvecs with FP64
+refinement, and the 10-site Hubbard chain at D = 64 against the same sweep
+run on the oracle (1e-8 Eh)."""
+import pytest
+
+import t3ns_amd as T
+from conftest import netw_text
+from env_util import OracleSolver
+from t3ns_amd.sweep import ChainDMRG, DeviceSolver
+from test_environment_oracle import N2_FCI
+from test_qc_heff import fcidump
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+@pytest.mark.parametrize("sy", [(["U1", "SU2"], [14, 0]), (["U1", "U1"], [7, 7])],
+                         ids=lambda s: "x".join(s[0]))
+def test_device_sweep_n2_reference_energy(gpu, sy, prec):
+    syms, target = sy
+    print("case n2", syms, prec, flush=True)
+    ctx = T.Context(netw_text("ref_networks/10_DMRG"), "QC", [], syms, target, 1000,
+                    fcidump=fcidump())
+    dm = ChainDMRG(ctx, 1000, seed=1, solver=DeviceSolver(prec))
+    e = dm.run(4, e_conv=1e-10)
+    print("E %.12f sweeps %d" % (e, len(dm.energies)))
+    assert abs(e - N2_FCI) < 1e-8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+def test_device_sweep_hubbard_10_matches_oracle(gpu, prec):
+    print("case hubbard 10 D=64", prec, flush=True)
+
+    def sweep(solver):
+        ctx = T.Context(netw_text("10_DMRG"), "NN_HUBBARD", [1.0, 4.0], ["U1", "U1"], [5, 5], 64)
+        dm = ChainDMRG(ctx, 64, seed=1, solver=solver)
+        return dm.run(3)
+    e_dev = sweep(DeviceSolver(prec))
+    e_ref = sweep(OracleSolver())
+    print("E device %.12f oracle %.12f" % (e_dev, e_ref))
+    assert abs(e_dev - e_ref) < 1e-8

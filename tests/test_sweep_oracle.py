@@ -1,0 +1,48 @@
+"""Two-site DMRG sweeps (t3ns_amd.sweep.ChainDMRG, the optimisation loop of
+src/optimize_network.c:488-570) run entirely on the oracle: makesiteTensor,
+operator renormalisation, the Heff eigenpair and the split with truncation.
+
+What the sweep must give: the exact ground-state energy where nothing is
+truncated (the 6-site Hubbard chain against exact diagonalisation), a
+variational energy that does not rise from sweep to sweep where it is
+(10 sites at D = 16), and the reference's own regression energy for
+N2/STO-3G on its 10-site network, -107.648250974014 within its tolerance
+1e-8 (tests/test1.c.in:99; D = 1000 keeps every state)."""
+import numpy as np
+import pytest
+
+import t3ns_amd as T
+from conftest import netw_text
+from env_util import OracleSolver, hubbard_edges, hubbard_ground_energy
+from t3ns_amd.sweep import ChainDMRG
+from test_environment_oracle import N2_FCI
+from test_qc_heff import fcidump
+
+
+@pytest.mark.parametrize("sy", [(["U1", "U1"], [3, 3]), (["U1", "SU2"], [6, 0])],
+                         ids=lambda s: "x".join(s[0]))
+def test_sweep_hubbard_6_exact(sy):
+    syms, target = sy
+    ctx = T.Context(netw_text("6_DMRG"), "NN_HUBBARD", [1.0, 4.0], syms, target, 64)
+    dm = ChainDMRG(ctx, 64, seed=1, solver=OracleSolver())
+    e = dm.run(2)
+    exact = hubbard_ground_energy(6, hubbard_edges(ctx), 3, 3, 1.0, 4.0)
+    assert abs(e - exact) < 1e-10
+
+
+def test_sweep_hubbard_10_truncated_is_variational():
+    ctx = T.Context(netw_text("10_DMRG"), "NN_HUBBARD", [1.0, 4.0], ["U1", "U1"], [5, 5], 16)
+    dm = ChainDMRG(ctx, 16, seed=1, solver=OracleSolver())
+    dm.run(3)
+    exact = -5.380618820415  # sparse ED of the 10-site chain (tests/test_environment_gpu.py)
+    es = dm.energies
+    assert all(b <= a + 1e-10 for a, b in zip(es, es[1:]))
+    assert exact - 1e-10 <= es[-1] < exact + 0.01
+
+
+def test_sweep_n2_reproduces_reference_energy():
+    ctx = T.Context(netw_text("ref_networks/10_DMRG"), "QC", [], ["U1", "SU2", "D2h"], [14, 0, 0],
+                    1000, fcidump=fcidump())
+    dm = ChainDMRG(ctx, 1000, seed=1, solver=OracleSolver())
+    e = dm.run(4, e_conv=1e-10)
+    assert abs(e - N2_FCI) < 1e-8
