@@ -1,4 +1,4 @@
-"""Two-site DMRG sweeps on chain networks.
+"""Two-site sweeps: DMRG on chain networks and two-site T3NS sweeps on trees.
 
 The optimisation loop of src/optimize_network.c:488-570 (execute_optScheme,
 optimize_network: one step per site pair of the sweep order of
@@ -18,8 +18,8 @@ sites dropped so that the next step renormalises them afresh
 The starting state is seeded and in canonical form toward the first step:
 every other site tensor is an isometry onto its bond toward the first pair
 (the reference starts from a random state and a renormalisation pass,
-src/optimize_network.c:574-640).  Tree networks (branching sites, three-
-and four-site steps with HOSVD) are not covered here.
+src/optimize_network.c:574-640).  Three- and four-site steps (HOSVD over
+repeated splits) are not covered here.
 """
 import ctypes as C
 
@@ -116,7 +116,8 @@ class DeviceSolver:
 
 
 class ChainDMRG:
-    """Two-site DMRG over the sweep order of a chain network.
+    """Two-site sweeps over the sweep order of a network: DMRG on chains and,
+    on trees, steps of a physical site and a neighbouring branching site.
 
     `solver` provides make_site_tensor(desc), solve(d, o, s, guess, tol,
     keep) -> (energy, vector, iterations) and split(desc) -> split dict,
@@ -125,22 +126,39 @@ class ChainDMRG:
 
     def __init__(self, ctx, max_dim, seed=0, solver=None, tol=1e-8, truncerr=0.0):
         info = ctx.info()
-        if info["psites"] != info["sites"]:
-            raise ValueError("ChainDMRG needs a chain network (no branching sites)")
         self.ctx, self.max_dim, self.tol, self.truncerr = ctx, int(max_dim), float(tol), truncerr
         self.solver = solver if solver is not None else DeviceSolver()
         self.seed = int(seed)
         self.steps = ctx.sweep_steps(2)
-        first = self.steps[0]
+        toward = self._legs_toward(self.steps[0], info["nr_bonds"])
         rng = np.random.default_rng(seed)
         self.tensors = {}
         for s in range(info["sites"]):
             data = SiteData(*ctx.site_tensor(s, seed * 7919 + s + 1))
-            if s not in first:
-                isometrise(ctx, s, data, 2 if s < min(first) else 0, rng)
+            if s in toward:
+                isometrise(ctx, s, data, toward[s], rng)
             self.tensors[s] = data
         self.env = Environment(ctx, seed, site_tensor=self._site_tensor, **self.solver.env)
         self.energies = []
+
+    def _legs_toward(self, sites, nr_bonds):
+        """For every site outside `sites`: the leg of its bond on the path to
+        them (breadth-first from the step outwards)."""
+        toward, frontier, seen = {}, list(sites), set(sites)
+        while frontier:
+            nxt = []
+            for u in frontier:
+                for b in self.ctx.site_bonds(u)[0]:
+                    if b < 0 or b >= nr_bonds:
+                        continue
+                    for t in self.ctx.bond_sites(b):
+                        if t == -1 or t in seen:
+                            continue
+                        seen.add(t)
+                        toward[t] = self.ctx.site_bonds(t)[0].index(b)
+                        nxt.append(t)
+            frontier = nxt
+        return toward
 
     def _site_tensor(self, site, desc, keep):
         data = self.tensors[site]

@@ -2,8 +2,10 @@
 device makesiteTensor, renormalisation, Heff + Davidson and split): the
 reference's N2/STO-3G regression energy -107.648250974014 within its
 tolerance 1e-8 (tests/test1.c.in:99), FP64 and FP32 matvecs with FP64
-refinement, and the 10-site Hubbard chain at D = 64 against the same sweep
-run on the oracle (1e-8 Eh)."""
+refinement, on the reference's 10_T3NS tree (its test1 network) and 10-site
+chain with the four symmetry sets of tests/test1.c.in:34-41; and the
+10-site Hubbard chain at D = 64 against the same sweep run on the oracle
+(1e-8 Eh)."""
 import pytest
 
 import t3ns_amd as T
@@ -14,14 +16,19 @@ from test_environment_oracle import N2_FCI
 from test_qc_heff import fcidump
 
 
+# the four symmetry sets of tests/test1.c.in:34-41
+TEST1_SYMS = [(["U1", "U1"], [7, 7]), (["U1", "U1", "D2h"], [7, 7, 0]), (["U1", "SU2"], [14, 0]),
+              (["U1", "SU2", "D2h"], [14, 0, 0])]
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec", ["f64", "mixed"])
-@pytest.mark.parametrize("sy", [(["U1", "SU2"], [14, 0]), (["U1", "U1"], [7, 7])],
-                         ids=lambda s: "x".join(s[0]))
-def test_device_sweep_n2_reference_energy(gpu, sy, prec):
+@pytest.mark.parametrize("sy", TEST1_SYMS, ids=lambda s: "x".join(s[0]))
+@pytest.mark.parametrize("net", ["10_T3NS", "10_DMRG"])
+def test_device_sweep_n2_reference_energy(gpu, net, sy, prec):
     syms, target = sy
-    print("case n2", syms, prec, flush=True)
-    ctx = T.Context(netw_text("ref_networks/10_DMRG"), "QC", [], syms, target, 1000,
+    print("case n2", net, syms, prec, flush=True)
+    ctx = T.Context(netw_text("ref_networks/" + net), "QC", [], syms, target, 1000,
                     fcidump=fcidump())
     dm = ChainDMRG(ctx, 1000, seed=1, solver=DeviceSolver(prec))
     e = dm.run(4, e_conv=1e-10)

@@ -6,8 +6,9 @@ What the sweep must give: the exact ground-state energy where nothing is
 truncated (the 6-site Hubbard chain against exact diagonalisation), a
 variational energy that does not rise from sweep to sweep where it is
 (10 sites at D = 16), and the reference's own regression energy for
-N2/STO-3G on its 10-site network, -107.648250974014 within its tolerance
-1e-8 (tests/test1.c.in:99; D = 1000 keeps every state)."""
+N2/STO-3G, -107.648250974014 within its tolerance 1e-8
+(tests/test1.c.in:99; D = 1000 keeps every state), on its 10-site chain
+and on its 10_T3NS tree."""
 import numpy as np
 import pytest
 
@@ -40,9 +41,15 @@ def test_sweep_hubbard_10_truncated_is_variational():
     assert exact - 1e-10 <= es[-1] < exact + 0.01
 
 
-def test_sweep_n2_reproduces_reference_energy():
-    ctx = T.Context(netw_text("ref_networks/10_DMRG"), "QC", [], ["U1", "SU2", "D2h"], [14, 0, 0],
-                    1000, fcidump=fcidump())
+@pytest.mark.parametrize("case", [("10_DMRG", ["U1", "SU2", "D2h"], [14, 0, 0]),
+                                  ("10_T3NS", ["U1", "SU2"], [14, 0])],
+                         ids=lambda c: "%s-%s" % (c[0], "x".join(c[1])))
+def test_sweep_n2_reproduces_reference_energy(case):
+    """On the reference's 10_T3NS network the sweep passes its branching
+    sites (steps of a physical and a branching site, branching updates)."""
+    name, syms, target = case
+    ctx = T.Context(netw_text("ref_networks/" + name), "QC", [], syms, target, 1000,
+                    fcidump=fcidump())
     dm = ChainDMRG(ctx, 1000, seed=1, solver=OracleSolver())
     e = dm.run(4, e_conv=1e-10)
     assert abs(e - N2_FCI) < 1e-8
