@@ -124,9 +124,10 @@ class ChainDMRG:
     and `solver.env` the renormalisation calls for the Environment (all
     device calls by default)."""
 
-    def __init__(self, ctx, max_dim, seed=0, solver=None, tol=1e-8, truncerr=0.0):
+    def __init__(self, ctx, max_dim, seed=0, solver=None, tol=1e-8, truncerr=0.0, min_dim=0):
         info = ctx.info()
         self.ctx, self.max_dim, self.tol, self.truncerr = ctx, int(max_dim), float(tol), truncerr
+        self.min_dim = int(min_dim)  # struct SvalSelect: minD, maxD, truncerr
         self.solver = solver if solver is not None else DeviceSolver()
         self.seed = int(seed)
         self.steps = ctx.sweep_steps(2)
@@ -196,7 +197,7 @@ class ChainDMRG:
             d, o, s = self.env.step_operators(st)
             e, x, its = self.solver.solve(d, o, s, x0, self.tol, (st, self.env))
             vsite = self._leaving_site(k)
-            sd = st.split_problem(order.index(vsite), self.max_dim, 0, self.truncerr)
+            sd = st.split_problem(order.index(vsite), self.max_dim, self.min_dim, self.truncerr)
             xn = np.ascontiguousarray(x / np.linalg.norm(x))
             sd.tel = xn.ctypes.data_as(_lib.c_dbl_p)
             res = self.solver.split(sd)
