@@ -48,6 +48,11 @@ int t3ns_ctx_bond_sites(const t3ns_ctx* c, int bond, int* sites2);
 /* The three bonds of site `site` (get_bonds_of_site, src/network.c:430) and
  * whether it is a physical site. */
 int t3ns_ctx_site_bonds(const t3ns_ctx* c, int site, int* bonds3, int* is_phys);
+/* on = 0: the problem builders (t3ns_step_random, t3ns_step_*_desc,
+ * t3ns_ctx_*_desc except t3ns_ctx_site_tensor) give structure only, with
+ * zero elements, for callers that put their own data in; on = 1 (the
+ * default): seeded N(0,1) elements as documented at each builder. */
+int t3ns_ctx_fill_seeded(t3ns_ctx* c, int on);
 /* Replaces the sectors of bond `bond` (irreps [nrsecs][T3NS_MAX_SYMMETRIES],
  * dims [nrsecs]), as a decomposition leaves them: the bookkeeper update of
  * adapt_UV_tensors_and_kick_empties (src/siteTensor_decompose.c:1360-1401). */

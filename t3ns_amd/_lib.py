@@ -208,6 +208,7 @@ def lib():
         "t3ns_ctx_bond": (C.c_int, [P, C.c_int, c_int_p, c_int_p, c_int_p, C.c_int]),
         "t3ns_ctx_bond_sites": (C.c_int, [P, C.c_int, c_int_p]),
         "t3ns_ctx_set_bond": (C.c_int, [P, C.c_int, C.c_int, c_int_p, c_int_p]),
+        "t3ns_ctx_fill_seeded": (C.c_int, [P, C.c_int]),
         "t3ns_ctx_site_tensor": (C.c_int, [P, C.c_int, C.c_uint64, c_int_p, C.POINTER(c_i64_p),
                                            C.POINTER(c_i64_p), C.POINTER(c_dbl_p)]),
         "t3ns_ctx_site_bonds": (C.c_int, [P, C.c_int, c_int_p, c_int_p]),

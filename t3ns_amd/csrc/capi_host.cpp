@@ -206,7 +206,8 @@ void build_append(Context& c, int pbond, int is_left, uint64_t seed, AppendStore
         o.finalize_offsets();
         o.tel.resize(o.total_size());
         Rng rng(seed);
-        for (auto& v : o.tel) v = rng.normal();
+        if (c.fill_seeded)
+            for (auto& v : o.tel) v = rng.normal();
     }
     a.old = std::move(o);
     a.is_left = is_left;
@@ -364,7 +365,8 @@ void build_pupdate(Context& c, int pbond, int is_left, uint64_t seed, int maxdim
     slot = tr;
     try {
         u.T = init_1siteTensor(c, site);
-        for (auto& v : u.T.tel) v = rng.normal();
+        if (c.fill_seeded)
+            for (auto& v : u.T.tel) v = rng.normal();
         std::vector<std::vector<int64_t>> tmpbb;
         ROperators ns = init_rOperators_structure(c, pbond, is_left, false, &tmpbb);
         u.nrhss = ns.nrhss;
@@ -495,6 +497,12 @@ int t3ns_ctx_site_bonds(const t3ns_ctx* c, int site, int* bonds3, int* is_phys) 
     return T3NS_OK;
 }
 
+int t3ns_ctx_fill_seeded(t3ns_ctx* c, int on) {
+    if (!c) return T3NS_ERR_INVALID;
+    c->c->fill_seeded = on != 0;
+    return T3NS_OK;
+}
+
 int t3ns_ctx_set_bond(t3ns_ctx* c, int bond, int nrsecs, const int* irreps, const int* dims) {
     if (!c || nrsecs < 0 || (nrsecs && (!irreps || !dims))) return T3NS_ERR_INVALID;
     return guarded_host([&] {
@@ -610,7 +618,8 @@ int t3ns_step_random(t3ns_step** out, t3ns_ctx* cx, const int* sites, int nsites
             for (int o = 0; o < nrops; ++o) r.beginblock[o] = tmpbb[hss[o]];
             r.finalize_offsets();
             r.tel.resize(r.total_size());
-            for (auto& v : r.tel) v = rng.normal();
+            if (c.fill_seeded)
+                for (auto& v : r.tel) v = rng.normal();
             st->ops[i] = std::move(r);
         }
         // Heff descriptor pieces
@@ -640,7 +649,8 @@ int t3ns_step_random(t3ns_step** out, t3ns_ctx* cx, const int* sites, int nsites
             for (int k = 0; k < 3; ++k) st->merge_instr.push_back(ins.instr[k]);
             st->merge_pref.push_back(ins.pref);
         }
-        for (auto& v : T.tel) v = rng.normal();
+        if (c.fill_seeded)
+            for (auto& v : T.tel) v = rng.normal();
         *out = st.release();
     });
 }
@@ -709,7 +719,8 @@ int t3ns_step_makesite_desc(t3ns_step* st, uint64_t seed, t3ns_makesite_desc* d)
                 const Symsecs* ss[3] = {&st->md.ssarr_old[i][0], &st->md.ssarr_old[i][1],
                                         &st->md.ssarr_old[i][2]};
                 m.sites[i] = init_1siteTensor(c, M.sites[i], ss);
-                for (auto& v : m.sites[i].tel) v = rng.normal();
+                if (c.fill_seeded)
+                    for (auto& v : m.sites[i].tel) v = rng.normal();
                 for (int k = 0; k < 3; ++k) {
                     m.new_ss[i][k].set(st->md.ssarr[i][k]);
                     m.old_ss[i][k].set(st->md.ssarr_old[i][k]);
@@ -769,7 +780,8 @@ int t3ns_ctx_bupdate_desc(t3ns_ctx* cx, int site, int update_case, uint64_t seed
                 for (int k = 0; k < nrops; ++k) o.beginblock[k] = tmpbb[hss[k]];
                 o.finalize_offsets();
                 o.tel.resize(o.total_size());
-                for (auto& v : o.tel) v = rng.normal();
+                if (c.fill_seeded)
+                    for (auto& v : o.tel) v = rng.normal();
             }
             u.ops[r] = std::move(o);
             u.bb_ptr[r].resize(u.ops[r].nrops);
@@ -780,7 +792,8 @@ int t3ns_ctx_bupdate_desc(t3ns_ctx* cx, int site, int update_case, uint64_t seed
             }
         }
         u.T = init_1siteTensor(c, site);
-        for (auto& v : u.T.tel) v = rng.normal();
+        if (c.fill_seeded)
+            for (auto& v : u.T.tel) v = rng.normal();
         for (int k = 0; k < 3; ++k) u.site_ss[k].set(c.symsecs(sb[k]));
         u.mpo.set(c.symsecs(-1));
         const int nbond = sb[update_case];

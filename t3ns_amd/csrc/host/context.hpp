@@ -40,6 +40,10 @@ public:
     std::unique_ptr<Model> ham;
     std::vector<Symsecs> v_symsecs;
     std::vector<Symsecs> p_symsecs;
+    // Whether the host layer's problem builders fill operators and tensors
+    // with seeded values (off: structure only, elements zero, for callers
+    // that supply their own data).
+    bool fill_seeded = true;
 
     // Symmetry sectors of a bond index in the reference numbering:
     // [0, 2*nr_bonds) virtual (ket, bra), >= 2*nr_bonds physical, -1 MPO

@@ -100,6 +100,11 @@ class Context:
         check(lib().t3ns_ctx_site_bonds(self._h, int(site), out, C.byref(ph)), "t3ns_ctx_site_bonds")
         return list(out), bool(ph.value)
 
+    def fill_seeded(self, on):
+        """Whether problem builders fill seeded elements (default) or give
+        structure only with zero elements."""
+        check(lib().t3ns_ctx_fill_seeded(self._h, int(bool(on))), "t3ns_ctx_fill_seeded")
+
     def set_bond(self, bond, irreps, dims):
         """Replaces the sectors of bond `bond`: irreps [n][MAX_SYMMETRIES],
         dims [n] (as a decomposition leaves them)."""

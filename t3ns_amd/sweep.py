@@ -139,6 +139,9 @@ class ChainDMRG:
             if s in toward:
                 isometrise(ctx, s, data, toward[s], rng)
             self.tensors[s] = data
+        # every element the steps use comes from here: the context's problem
+        # builders give structure only from now on
+        ctx.fill_seeded(False)
         self.env = Environment(ctx, seed, site_tensor=self._site_tensor, **self.solver.env)
         self.energies = []
 
