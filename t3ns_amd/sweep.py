@@ -1,4 +1,4 @@
-"""Two-site sweeps: DMRG on chain networks and two-site T3NS sweeps on trees.
+"""Sweeps: DMRG on chain networks and T3NS sweeps on trees.
 
 The optimisation loop of src/optimize_network.c:488-570 (execute_optScheme,
 optimize_network: one step per site pair of the sweep order of
@@ -7,19 +7,18 @@ points: per step, makesiteTensor of the pair from its one-site tensors
 (src/siteTensor_init.c:785-800), the step's operators
 (src/optimize_network.c:164-176, through Environment), the lowest
 eigenpair of its effective Hamiltonian (Davidson, FP64 or FP32 with FP64
-refinement), one split of decompose_siteTensor with truncation
-(src/siteTensor_decompose.c:1403-1444) that leaves the pair's site shared
-with the next step as the orthogonality centre, the cut bond's new sectors
-written into the context (the bookkeeper update of
-adapt_UV_tensors_and_kick_empties), and the operators covering the two
+refinement), decompose_siteTensor with truncation by repeated splits
+(src/siteTensor_decompose.c:1403-1444, 1533-1585) that leave a site shared
+with the next step as the orthogonality centre, each cut bond's new
+sectors written into the context (the bookkeeper update of
+adapt_UV_tensors_and_kick_empties), and the operators covering the step's
 sites dropped so that the next step renormalises them afresh
 (postprocess_rOperators, src/optimize_network.c:236-311).
 
 The starting state is seeded and in canonical form toward the first step:
 every other site tensor is an isometry onto its bond toward the first pair
 (the reference starts from a random state and a renormalisation pass,
-src/optimize_network.c:574-640).  Three- and four-site steps (HOSVD over
-repeated splits) are not covered here.
+src/optimize_network.c:574-640).
 """
 import ctypes as C
 
@@ -116,21 +115,24 @@ class DeviceSolver:
 
 
 class ChainDMRG:
-    """Two-site sweeps over the sweep order of a network: DMRG on chains and,
-    on trees, steps of a physical site and a neighbouring branching site.
+    """Sweeps over the sweep order of a network for steps of up to
+    `site_size` sites (src/network.c:529-727): two-site DMRG on chains; on
+    trees, two-site steps and, with site_size 3 or 4, the {physical,
+    branching, physical} objects where the sweep passes a branching site.
 
     `solver` provides make_site_tensor(desc), solve(d, o, s, guess, tol,
     keep) -> (energy, vector, iterations) and split(desc) -> split dict,
     and `solver.env` the renormalisation calls for the Environment (all
     device calls by default)."""
 
-    def __init__(self, ctx, max_dim, seed=0, solver=None, tol=1e-8, truncerr=0.0, min_dim=0):
+    def __init__(self, ctx, max_dim, seed=0, solver=None, tol=1e-8, truncerr=0.0, min_dim=0,
+                 site_size=2):
         info = ctx.info()
         self.ctx, self.max_dim, self.tol, self.truncerr = ctx, int(max_dim), float(tol), truncerr
         self.min_dim = int(min_dim)  # struct SvalSelect: minD, maxD, truncerr
         self.solver = solver if solver is not None else DeviceSolver()
         self.seed = int(seed)
-        self.steps = ctx.sweep_steps(2)
+        self.steps = ctx.sweep_steps(int(site_size))
         toward = self._legs_toward(self.steps[0], info["nr_bonds"])
         rng = np.random.default_rng(seed)
         self.tensors = {}
@@ -171,17 +173,76 @@ class ChainDMRG:
         desc.T_qnumbers, desc.T_beginblock, desc.T_tel = qn, bb, tel
         keep.append(data)
 
-    def _leaving_site(self, k):
-        """The site of step k that the orthogonality centre leaves: the one
-        the next step (or, where the sweep turns on the same pair, the one
-        after) does not hold."""
-        sites = set(self.steps[k])
+    def _centre_site(self, k):
+        """The site of step k that keeps the orthogonality centre: one the
+        next step also holds (or, where the sweep turns on the same sites,
+        the step after)."""
+        sites = self.steps[k]
         n = len(self.steps)
         for j in (1, 2):
-            common = sites & set(self.steps[(k + j) % n])
-            if len(common) == 1:
-                return (sites - common).pop()
-        return self.steps[k][-1]
+            nxt = self.steps[(k + j) % n]
+            common = [t for t in sites if t in nxt]
+            if 0 < len(common) < len(sites):
+                return common[0]
+        return sites[0]
+
+    def _common_bond(self, a, b):
+        return (set(self.ctx.site_bonds(a)[0]) & set(self.ctx.site_bonds(b)[0])).pop()
+
+    def _decompose(self, sd, order, centre):
+        """decompose_siteTensor by repeated splits (the HOSVD of
+        src/siteTensor_decompose.c:1533-1585): a site of the remaining
+        object that is a leaf of it and not the centre is split off (an
+        isometry onto the cut bond), the cut bond's kept sectors go into the
+        context, and the rest (singular values in) is split again until the
+        centre alone remains.  Returns the cut bonds' dimensions."""
+        remaining = list(order)
+        dims = []
+        keep = []
+        while len(remaining) > 1:
+            leaf = next(t for t in remaining if t != centre and sum(
+                1 for u in remaining if u != t and self._adjacent(t, u)) == 1)
+            nb = next(u for u in remaining if u != leaf and self._adjacent(leaf, u))
+            sd.siteV = remaining.index(leaf)
+            res = self.solver.split(sd)
+            cut = self._common_bond(leaf, nb)
+            self.ctx.set_bond(cut, res["bond_irreps"], res["bond_dims"])
+            dims.append(int(np.sum(res["bond_dims"])))
+            self.tensors[leaf] = SiteData(res["V_qnumbers"], res["V_beginblock"], res["V_tel"])
+            idx = remaining.index(leaf)
+            remaining.pop(idx)
+            if len(remaining) == 1:
+                self.tensors[remaining[0]] = SiteData(res["U_qnumbers"], res["U_beginblock"],
+                                                      res["U_tel"])
+                break
+            # the split of what is left: the remaining sites' sectors, the cut
+            # bond's replaced by its kept ones
+            irr = np.ascontiguousarray(res["bond_irreps"], dtype=np.int32).reshape(-1)
+            dm = np.ascontiguousarray(res["bond_dims"], dtype=np.int32)
+            U = (np.ascontiguousarray(res["U_qnumbers"]), np.ascontiguousarray(res["U_beginblock"]),
+                 np.ascontiguousarray(res["U_tel"] if len(res["U_tel"]) else np.zeros(1)))
+            keep += [irr, dm, U]
+            nsd = _lib.SplitDesc()
+            nsd.nrSyms, nsd.sgs = sd.nrSyms, sd.sgs
+            nsd.nrsites = len(remaining)
+            rows = [i for i in range(len(remaining) + 1) if i != idx]
+            for j, i in enumerate(rows):
+                for leg in range(3):
+                    ssd = sd.ss[i][leg]
+                    if ssd.bond == cut:
+                        ssd = _lib.SymsecsDesc(cut, len(dm), irr.ctypes.data_as(_lib.c_int_p),
+                                               dm.ctypes.data_as(_lib.c_int_p))
+                    nsd.ss[j][leg] = ssd
+            nsd.nrblocks = len(U[0]) // len(remaining)
+            nsd.qnumbers = U[0].ctypes.data_as(_lib.c_i64_p)
+            nsd.beginblock = U[1].ctypes.data_as(_lib.c_i64_p)
+            nsd.tel = U[2].ctypes.data_as(_lib.c_dbl_p)
+            nsd.minD, nsd.maxD, nsd.truncerr, nsd.truncType = sd.minD, sd.maxD, sd.truncerr, sd.truncType
+            sd = nsd
+        return dims
+
+    def _adjacent(self, a, b):
+        return bool(set(self.ctx.site_bonds(a)[0]) & set(self.ctx.site_bonds(b)[0]))
 
     def step(self, k):
         sites = self.steps[k]
@@ -199,20 +260,13 @@ class ChainDMRG:
             x0 /= np.linalg.norm(x0)
             d, o, s = self.env.step_operators(st)
             e, x, its = self.solver.solve(d, o, s, x0, self.tol, (st, self.env))
-            vsite = self._leaving_site(k)
-            sd = st.split_problem(order.index(vsite), self.max_dim, self.min_dim, self.truncerr)
+            centre = self._centre_site(k)
+            sd = st.split_problem(0, self.max_dim, self.min_dim, self.truncerr)
             xn = np.ascontiguousarray(x / np.linalg.norm(x))
             sd.tel = xn.ctypes.data_as(_lib.c_dbl_p)
-            res = self.solver.split(sd)
-            usite = order[1 - order.index(vsite)]
-            cut = (set(self.ctx.site_bonds(sites[0])[0]) & set(self.ctx.site_bonds(sites[1])[0])).pop()
-            self.ctx.set_bond(cut, res["bond_irreps"], res["bond_dims"])
-            self.tensors[vsite] = SiteData(res["V_qnumbers"], res["V_beginblock"], res["V_tel"])
-            self.tensors[usite] = SiteData(res["U_qnumbers"], res["U_beginblock"], res["U_tel"])
+            dims = self._decompose(sd, order, centre)
             self.env.invalidate(sites)
-            return {"sites": list(sites), "energy": e, "its": its,
-                    "discarded": 1.0 - res["norm"][1] ** 2 / max(res["norm"][0] ** 2, 1e-300),
-                    "bond_dim": int(np.sum(res["bond_dims"]))}
+            return {"sites": list(sites), "energy": e, "its": its, "bond_dim": max(dims)}
         finally:
             st.close()
 

@@ -49,3 +49,20 @@ def test_device_sweep_hubbard_10_matches_oracle(gpu, prec):
     e_ref = sweep(OracleSolver())
     print("E device %.12f oracle %.12f" % (e_dev, e_ref))
     assert abs(e_dev - e_ref) < 1e-8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("site_size", [3, 4])
+@pytest.mark.parametrize("sy", [TEST1_SYMS[0], TEST1_SYMS[3]], ids=lambda s: "x".join(s[0]))
+def test_device_multisite_sweep_n2(gpu, sy, site_size):
+    """Three- and four-site steps around the branching sites of the
+    reference's 10_T3NS network (HOSVD by repeated splits)."""
+    syms, target = sy
+    print("case n2 multisite", syms, site_size, flush=True)
+    ctx = T.Context(netw_text("ref_networks/10_T3NS"), "QC", [], syms, target, 1000,
+                    fcidump=fcidump())
+    dm = ChainDMRG(ctx, 1000, seed=1, solver=DeviceSolver("f64"), site_size=site_size)
+    e = dm.run(4, e_conv=1e-10)
+    print("E %.12f sweeps %d" % (e, len(dm.energies)))
+    assert abs(e - N2_FCI) < 1e-8
+

@@ -53,3 +53,17 @@ def test_sweep_n2_reproduces_reference_energy(case):
     dm = ChainDMRG(ctx, 1000, seed=1, solver=OracleSolver())
     e = dm.run(4, e_conv=1e-10)
     assert abs(e - N2_FCI) < 1e-8
+
+
+@pytest.mark.parametrize("site_size", [3, 4])
+def test_sweep_multisite_steps_n2(site_size):
+    """Three- and four-site steps around the branching sites of 10_T3NS,
+    decomposed by repeated splits (HOSVD, src/siteTensor_decompose.c:
+    1533-1585)."""
+    ctx = T.Context(netw_text("ref_networks/10_T3NS"), "QC", [], ["U1", "SU2"], [14, 0], 1000,
+                    fcidump=fcidump())
+    dm = ChainDMRG(ctx, 1000, seed=1, solver=OracleSolver(), site_size=site_size)
+    assert any(len(st) == site_size for st in dm.steps)
+    e = dm.run(3, e_conv=1e-10)
+    assert abs(e - N2_FCI) < 1e-8
+
