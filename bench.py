@@ -20,7 +20,8 @@ Davidson against FP32 matvecs with FP64 refinement, energies, iterations,
 seconds and |dE|.  The "sweep" entry is the two-site DMRG of the 10-site
 NN_HUBBARD chain (t = 1, U = 4, half filling, D = 64, up to 4 sweeps,
 E_CONV = 1e-10) with every step on the device: seconds and energies, FP64
-and mixed.
+and mixed; and the reference's LiF regression sweep (tests/test3.c.in,
+singlet) with its reference energy.
 """
 import argparse
 import json
@@ -271,6 +272,43 @@ def measure_sweep(sweeps=4, e_conv=1e-10):
         print("# sweep %s: E %.12f in %d sweeps, %.2f s" % (prec, e, len(dm.energies), t),
               file=sys.stderr, flush=True)
     out["abs_dE"] = abs(out["mixed"]["energy"] - out["f64"]["energy"])
+    out["lif"] = measure_lif_sweep()
+    return out
+
+
+def measure_lif_sweep():
+    """The reference's LiF regression run (tests/test3.c.in, singlet): 25
+    orbitals, 6 electrons, Z2 x U1 x SU2 x C2v, lif_T3NS tree, two-site
+    steps, minD = maxD = 100 for up to 4 sweeps (E_CONV 1e-4) then 140
+    until E_CONV 1e-6, truncation 1e-4, Davidson tolerance 1e-6 (10
+    iterations per step); reference energy -107.1152169273."""
+    import gzip
+    import t3ns_amd as T
+    from t3ns_amd.sweep import ChainDMRG, DeviceSolver
+    gold = os.path.join(ROOT, "tests", "golden")
+    with gzip.open(os.path.join(gold, "fcidumps", "LiF_3.05.FCIDUMP.gz"), "rt") as f:
+        fci = f.read()
+    with open(os.path.join(gold, "ref_networks", "lif_T3NS.netw")) as f:
+        netw = f.read()
+    out = {"workload": "lif_t3ns_test3_singlet", "reference_energy": -107.1152169273}
+    for prec in ("f64", "mixed"):
+        ctx = T.Context(netw, "QC", [], ["U1", "SU2", "C2v"], [6, 0, 0], 100, fcidump=fci)
+        solver = DeviceSolver(prec, max_its=10)
+        t0 = time.perf_counter()
+        dm = ChainDMRG(ctx, 100, seed=1, solver=solver, tol=1e-6, truncerr=1e-4, min_dim=100)
+        for D, sweeps, e_conv in ((100, 4, 1e-4), (140, 100, 1e-6)):
+            dm.max_dim = dm.min_dim = D
+            for i in range(sweeps):
+                dm.sweep()
+                if i > 0 and abs(dm.energies[-1] - dm.energies[-2]) < e_conv:
+                    break
+        t = time.perf_counter() - t0
+        e = min(dm.energies)
+        out[prec] = {"energy": e, "abs_err_vs_reference": abs(e - out["reference_energy"]),
+                     "sweeps": len(dm.energies), "seconds": t,
+                     "seconds_per_sweep": t / len(dm.energies)}
+        print("# lif sweep %s: E %.10f in %d sweeps, %.2f s" % (prec, e, len(dm.energies), t),
+              file=sys.stderr, flush=True)
     return out
 
 

@@ -85,11 +85,16 @@ class DeviceSolver:
 
     env = {}
 
-    def __init__(self, precision="f64", max_vecs=30, keep_deflate=2, max_its=300):
+    def __init__(self, precision="f64", max_vecs=30, keep_deflate=2, max_its=300,
+                 mixed_min_size=200000):
+        """mixed_min_size: below this many elements a step's Heff is
+        launch-bound, FP32 matvecs save nothing and the refinement adds
+        matvecs, so "mixed" solves such steps in FP64."""
         if precision not in ("f64", "mixed"):
             raise ValueError("precision is f64 or mixed")
         self.precision = precision
         self.max_vecs, self.keep, self.max_its = max_vecs, keep_deflate, max_its
+        self.mixed_min_size = int(mixed_min_size)
 
     def make_site_tensor(self, desc):
         return make_site_tensor(desc)[0]
@@ -97,7 +102,7 @@ class DeviceSolver:
     def solve(self, d, o, s, guess, tol, keep):
         h64 = Heff(d, o, s, FP64, keepalive=keep)
         try:
-            if self.precision == "f64":
+            if self.precision == "f64" or h64.size < self.mixed_min_size:
                 e, x, its, rc = h64.davidson(guess, self.max_vecs, self.keep, tol, self.max_its)
                 return e, x, (its, 0)
             h32 = Heff(d, o, s, FP32, keepalive=keep)

@@ -30,7 +30,7 @@ def test_device_sweep_n2_reference_energy(gpu, net, sy, prec):
     print("case n2", net, syms, prec, flush=True)
     ctx = T.Context(netw_text("ref_networks/" + net), "QC", [], syms, target, 1000,
                     fcidump=fcidump())
-    dm = ChainDMRG(ctx, 1000, seed=1, solver=DeviceSolver(prec))
+    dm = ChainDMRG(ctx, 1000, seed=1, solver=DeviceSolver(prec, mixed_min_size=0))
     e = dm.run(4, e_conv=1e-10)
     print("E %.12f sweeps %d" % (e, len(dm.energies)))
     assert abs(e - N2_FCI) < 1e-8
@@ -45,7 +45,7 @@ def test_device_sweep_hubbard_10_matches_oracle(gpu, prec):
         ctx = T.Context(netw_text("10_DMRG"), "NN_HUBBARD", [1.0, 4.0], ["U1", "U1"], [5, 5], 64)
         dm = ChainDMRG(ctx, 64, seed=1, solver=solver)
         return dm.run(3)
-    e_dev = sweep(DeviceSolver(prec))
+    e_dev = sweep(DeviceSolver(prec, mixed_min_size=0))
     e_ref = sweep(OracleSolver())
     print("E device %.12f oracle %.12f" % (e_dev, e_ref))
     assert abs(e_dev - e_ref) < 1e-8
