@@ -257,14 +257,14 @@ def measure_sweep(sweeps=4, e_conv=1e-10):
     on the device (makesiteTensor, renormalisation, Heff + Davidson, split),
     FP64 and FP32 matvecs with FP64 refinement."""
     import t3ns_amd as T
-    from t3ns_amd.sweep import ChainDMRG, DeviceSolver
+    from t3ns_amd.sweep import Sweeper, DeviceSolver
     with open(os.path.join(ROOT, "tests", "golden", "networks", "10_DMRG.netw")) as f:
         netw = f.read()
     out = {"workload": "nn_hubbard_10_chain_D64", "max_sweeps": sweeps, "e_conv": e_conv}
     for prec in ("f64", "mixed"):
         ctx = T.Context(netw, "NN_HUBBARD", [1.0, 4.0], ["U1", "U1"], [5, 5], 64)
         t0 = time.perf_counter()
-        dm = ChainDMRG(ctx, 64, seed=1, solver=DeviceSolver(prec), tol=1e-8)
+        dm = Sweeper(ctx, 64, seed=1, solver=DeviceSolver(prec), tol=1e-8)
         e = dm.run(sweeps, e_conv=e_conv)
         t = time.perf_counter() - t0
         out[prec] = {"energy": e, "sweeps": len(dm.energies), "seconds": t,
@@ -284,7 +284,7 @@ def measure_lif_sweep():
     iterations per step); reference energy -107.1152169273."""
     import gzip
     import t3ns_amd as T
-    from t3ns_amd.sweep import ChainDMRG, DeviceSolver
+    from t3ns_amd.sweep import Sweeper, DeviceSolver
     gold = os.path.join(ROOT, "tests", "golden")
     with gzip.open(os.path.join(gold, "fcidumps", "LiF_3.05.FCIDUMP.gz"), "rt") as f:
         fci = f.read()
@@ -295,7 +295,7 @@ def measure_lif_sweep():
         ctx = T.Context(netw, "QC", [], ["U1", "SU2", "C2v"], [6, 0, 0], 100, fcidump=fci)
         solver = DeviceSolver(prec, max_its=10)
         t0 = time.perf_counter()
-        dm = ChainDMRG(ctx, 100, seed=1, solver=solver, tol=1e-6, truncerr=1e-4, min_dim=100)
+        dm = Sweeper(ctx, 100, seed=1, solver=solver, tol=1e-6, truncerr=1e-4, min_dim=100)
         for D, sweeps, e_conv in ((100, 4, 1e-4), (140, 100, 1e-6)):
             dm.max_dim = dm.min_dim = D
             for i in range(sweeps):

@@ -119,7 +119,7 @@ class DeviceSolver:
         return split_site(desc)[0]
 
 
-class ChainDMRG:
+class Sweeper:
     """Sweeps over the sweep order of a network for steps of up to
     `site_size` sites (src/network.c:529-727): two-site DMRG on chains; on
     trees, two-site steps and, with site_size 3 or 4, the {physical,

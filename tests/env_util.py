@@ -175,7 +175,7 @@ def hubbard_edges(ctx):
 
 class OracleSolver:
     """The oracle's makesiteTensor, Heff eigenpair (dense below 800 states,
-    Lanczos above) and split, in the form t3ns_amd.sweep.ChainDMRG takes."""
+    Lanczos above) and split, in the form t3ns_amd.sweep.Sweeper takes."""
 
     env = ORACLE_BACKEND
 

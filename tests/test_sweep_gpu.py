@@ -1,4 +1,4 @@
-"""Two-site DMRG sweeps on the device (t3ns_amd.sweep.ChainDMRG with the
+"""Two-site DMRG sweeps on the device (t3ns_amd.sweep.Sweeper with the
 device makesiteTensor, renormalisation, Heff + Davidson and split): the
 reference's N2/STO-3G regression energy -107.648250974014 within its
 tolerance 1e-8 (tests/test1.c.in:99), FP64 and FP32 matvecs with FP64
@@ -11,7 +11,7 @@ import pytest
 import t3ns_amd as T
 from conftest import netw_text
 from env_util import OracleSolver
-from t3ns_amd.sweep import ChainDMRG, DeviceSolver
+from t3ns_amd.sweep import Sweeper, DeviceSolver
 from test_environment_oracle import N2_FCI
 from test_qc_heff import fcidump
 
@@ -30,7 +30,7 @@ def test_device_sweep_n2_reference_energy(gpu, net, sy, prec):
     print("case n2", net, syms, prec, flush=True)
     ctx = T.Context(netw_text("ref_networks/" + net), "QC", [], syms, target, 1000,
                     fcidump=fcidump())
-    dm = ChainDMRG(ctx, 1000, seed=1, solver=DeviceSolver(prec, mixed_min_size=0))
+    dm = Sweeper(ctx, 1000, seed=1, solver=DeviceSolver(prec, mixed_min_size=0))
     e = dm.run(4, e_conv=1e-10)
     print("E %.12f sweeps %d" % (e, len(dm.energies)))
     assert abs(e - N2_FCI) < 1e-8
@@ -43,7 +43,7 @@ def test_device_sweep_hubbard_10_matches_oracle(gpu, prec):
 
     def sweep(solver):
         ctx = T.Context(netw_text("10_DMRG"), "NN_HUBBARD", [1.0, 4.0], ["U1", "U1"], [5, 5], 64)
-        dm = ChainDMRG(ctx, 64, seed=1, solver=solver)
+        dm = Sweeper(ctx, 64, seed=1, solver=solver)
         return dm.run(3)
     e_dev = sweep(DeviceSolver(prec, mixed_min_size=0))
     e_ref = sweep(OracleSolver())
@@ -61,7 +61,7 @@ def test_device_multisite_sweep_n2(gpu, sy, site_size):
     print("case n2 multisite", syms, site_size, flush=True)
     ctx = T.Context(netw_text("ref_networks/10_T3NS"), "QC", [], syms, target, 1000,
                     fcidump=fcidump())
-    dm = ChainDMRG(ctx, 1000, seed=1, solver=DeviceSolver("f64"), site_size=site_size)
+    dm = Sweeper(ctx, 1000, seed=1, solver=DeviceSolver("f64"), site_size=site_size)
     e = dm.run(4, e_conv=1e-10)
     print("E %.12f sweeps %d" % (e, len(dm.energies)))
     assert abs(e - N2_FCI) < 1e-8

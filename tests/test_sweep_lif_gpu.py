@@ -16,7 +16,7 @@ import pytest
 
 import t3ns_amd as T
 from conftest import GOLDEN
-from t3ns_amd.sweep import ChainDMRG, DeviceSolver
+from t3ns_amd.sweep import Sweeper, DeviceSolver
 
 CASES = [
     # (target (N, 2S, C2v irrep), D of the second regime, energy, source)
@@ -42,7 +42,7 @@ def test_device_sweep_lif_reference_energy(gpu, case):
     netw, fci = _inputs()
     ctx = T.Context(netw, "QC", [], ["U1", "SU2", "C2v"], target, 100, fcidump=fci)
     solver = DeviceSolver("f64", max_its=10)
-    dm = ChainDMRG(ctx, 100, seed=1, solver=solver, tol=1e-6, truncerr=1e-4, min_dim=100)
+    dm = Sweeper(ctx, 100, seed=1, solver=solver, tol=1e-6, truncerr=1e-4, min_dim=100)
     for D, sweeps, its, e_conv in ((100, 4, 10, 1e-4), (D2, 100, 10, 1e-6)):
         dm.max_dim = dm.min_dim = D
         solver.max_its = its

@@ -1,4 +1,4 @@
-"""Two-site DMRG sweeps (t3ns_amd.sweep.ChainDMRG, the optimisation loop of
+"""Two-site DMRG sweeps (t3ns_amd.sweep.Sweeper, the optimisation loop of
 src/optimize_network.c:488-570) run entirely on the oracle: makesiteTensor,
 operator renormalisation, the Heff eigenpair and the split with truncation.
 
@@ -15,7 +15,7 @@ import pytest
 import t3ns_amd as T
 from conftest import netw_text
 from env_util import OracleSolver, hubbard_edges, hubbard_ground_energy
-from t3ns_amd.sweep import ChainDMRG
+from t3ns_amd.sweep import Sweeper
 from test_environment_oracle import N2_FCI
 from test_qc_heff import fcidump
 
@@ -25,7 +25,7 @@ from test_qc_heff import fcidump
 def test_sweep_hubbard_6_exact(sy):
     syms, target = sy
     ctx = T.Context(netw_text("6_DMRG"), "NN_HUBBARD", [1.0, 4.0], syms, target, 64)
-    dm = ChainDMRG(ctx, 64, seed=1, solver=OracleSolver())
+    dm = Sweeper(ctx, 64, seed=1, solver=OracleSolver())
     e = dm.run(2)
     exact = hubbard_ground_energy(6, hubbard_edges(ctx), 3, 3, 1.0, 4.0)
     assert abs(e - exact) < 1e-10
@@ -33,7 +33,7 @@ def test_sweep_hubbard_6_exact(sy):
 
 def test_sweep_hubbard_10_truncated_is_variational():
     ctx = T.Context(netw_text("10_DMRG"), "NN_HUBBARD", [1.0, 4.0], ["U1", "U1"], [5, 5], 16)
-    dm = ChainDMRG(ctx, 16, seed=1, solver=OracleSolver())
+    dm = Sweeper(ctx, 16, seed=1, solver=OracleSolver())
     dm.run(3)
     exact = -5.380618820415  # sparse ED of the 10-site chain (tests/test_environment_gpu.py)
     es = dm.energies
@@ -50,7 +50,7 @@ def test_sweep_n2_reproduces_reference_energy(case):
     name, syms, target = case
     ctx = T.Context(netw_text("ref_networks/" + name), "QC", [], syms, target, 1000,
                     fcidump=fcidump())
-    dm = ChainDMRG(ctx, 1000, seed=1, solver=OracleSolver())
+    dm = Sweeper(ctx, 1000, seed=1, solver=OracleSolver())
     e = dm.run(4, e_conv=1e-10)
     assert abs(e - N2_FCI) < 1e-8
 
@@ -62,7 +62,7 @@ def test_sweep_multisite_steps_n2(site_size):
     1533-1585)."""
     ctx = T.Context(netw_text("ref_networks/10_T3NS"), "QC", [], ["U1", "SU2"], [14, 0], 1000,
                     fcidump=fcidump())
-    dm = ChainDMRG(ctx, 1000, seed=1, solver=OracleSolver(), site_size=site_size)
+    dm = Sweeper(ctx, 1000, seed=1, solver=OracleSolver(), site_size=site_size)
     assert any(len(st) == site_size for st in dm.steps)
     e = dm.run(3, e_conv=1e-10)
     assert abs(e - N2_FCI) < 1e-8
