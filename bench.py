@@ -117,6 +117,27 @@ def cpu_baseline(step, x, target_seconds, flops_total):
                       % (stride, len(range(0, nb, stride)), nb, 100.0 * f / flops_total, t)}
 
 
+def cpu_sweep_baseline(sweeps=4, e_conv=1e-10):
+    """The sweep leg's Hubbard chain (10 sites, D = 64) with every step on
+    the oracle, the FP64 restatement of the reference's CPU path (OpenMP
+    matvec, Lanczos eigensolver, renormalisation, split), on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import t3ns_amd as T
+    from env_util import OracleSolver
+    from t3ns_amd.sweep import Sweeper
+    with open(os.path.join(ROOT, "tests", "golden", "networks", "10_DMRG.netw")) as f:
+        netw = f.read()
+    ctx = T.Context(netw, "NN_HUBBARD", [1.0, 4.0], ["U1", "U1"], [5, 5], 64)
+    t0 = time.perf_counter()
+    dm = Sweeper(ctx, 64, seed=1, solver=OracleSolver(), tol=1e-8)
+    e = dm.run(sweeps, e_conv=e_conv)
+    t = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"workload": "nn_hubbard_10_chain_D64", "energy": e, "sweeps": len(dm.energies),
+            "seconds": t, "seconds_per_sweep": t / len(dm.energies), "cores": threads,
+            "kind": "port"}
+
+
 HBM_GBS = 8000.0
 RIDGE = PEAK_TFLOPS["f64"] * 1e12 / (HBM_GBS * 1e9)  # FP64 flop per byte
 
@@ -425,6 +446,8 @@ def main():
         print("# rows measured in %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(step, x, args.cpu_seconds, flops)
+        if not args.no_sweep:
+            out["cpu_baseline"]["sweep"] = cpu_sweep_baseline()
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
