@@ -20,7 +20,6 @@ every other site tensor is an isometry onto its bond toward the first pair
 (the reference starts from a random state and a renormalisation pass,
 src/optimize_network.c:574-640).
 """
-import ctypes as C
 
 import numpy as np
 

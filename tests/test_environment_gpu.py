@@ -19,7 +19,7 @@ import t3ns_amd as T
 from conftest import netw_text
 from env_util import ORACLE_BACKEND, hubbard_edges, hubbard_ground_energy, isometric_tensors
 from t3ns_amd.environment import Environment
-from test_environment_oracle import N2_FCI, SYM_CASES, TREE7, network
+from test_environment_oracle import N2_FCI, SYM_CASES, network
 from test_qc_heff import fcidump
 
 

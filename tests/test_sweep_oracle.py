@@ -9,7 +9,6 @@ variational energy that does not rise from sweep to sweep where it is
 N2/STO-3G, -107.648250974014 within its tolerance 1e-8
 (tests/test1.c.in:99; D = 1000 keeps every state), on its 10-site chain
 and on its 10_T3NS tree."""
-import numpy as np
 import pytest
 
 import t3ns_amd as T
