@@ -15,7 +15,7 @@ import t3ns_amd as T
 from conftest import netw_text
 from env_util import OracleSolver, hubbard_edges, hubbard_ground_energy
 from t3ns_amd.sweep import Sweeper
-from test_environment_oracle import N2_FCI
+from test_environment_oracle import N2_FCI, TREE7
 from test_qc_heff import fcidump
 
 
@@ -65,4 +65,21 @@ def test_sweep_multisite_steps_n2(site_size):
     assert any(len(st) == site_size for st in dm.steps)
     e = dm.run(3, e_conv=1e-10)
     assert abs(e - N2_FCI) < 1e-8
+
+
+@pytest.mark.parametrize("sy", [(["U1", "U1"], [3, 3]), (["U1", "SU2"], [6, 0])],
+                         ids=lambda s: "x".join(s[0]))
+def test_sweep_hubbard_tree_exact(sy):
+    """NN_HUBBARD on a 7-site tree with one branching site (two-site steps,
+    the sweep passes it with branching updates): nothing is truncated, so
+    the sweep ends on the exact energy of the tree's hopping graph.  (Its
+    three-site sweep order holds a {branching, physical, physical} chain,
+    which the multisite structure does not take; the makenetwork.py trees
+    give {physical, branching, physical} objects only.)"""
+    syms, target = sy
+    ctx = T.Context(TREE7, "NN_HUBBARD", [1.0, 4.0], syms, target, 256)
+    dm = Sweeper(ctx, 256, seed=1, solver=OracleSolver())
+    e = dm.run(3, e_conv=1e-12)
+    exact = hubbard_ground_energy(6, hubbard_edges(ctx), 3, 3, 1.0, 4.0)
+    assert abs(e - exact) < 1e-10
 
