@@ -12,7 +12,8 @@ import t3ns_amd as T
 from conftest import netw_text
 from env_util import OracleSolver
 from t3ns_amd.sweep import Sweeper, DeviceSolver
-from test_environment_oracle import N2_FCI
+from test_environment_oracle import N2_FCI, TREE7
+from env_util import hubbard_edges, hubbard_ground_energy
 from test_qc_heff import fcidump
 
 
@@ -65,4 +66,19 @@ def test_device_multisite_sweep_n2(gpu, sy, site_size):
     e = dm.run(4, e_conv=1e-10)
     print("E %.12f sweeps %d" % (e, len(dm.energies)))
     assert abs(e - N2_FCI) < 1e-8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+def test_device_sweep_hubbard_tree_exact(gpu, prec):
+    """NN_HUBBARD (t = 1, U = 4) on a 7-site tree with one branching site,
+    U1 x SU2, untruncated: the device sweep ends on the exact energy of the
+    tree's hopping graph (1e-9)."""
+    print("case hubbard tree7", prec, flush=True)
+    ctx = T.Context(TREE7, "NN_HUBBARD", [1.0, 4.0], ["U1", "SU2"], [6, 0], 256)
+    dm = Sweeper(ctx, 256, seed=1, solver=DeviceSolver(prec, mixed_min_size=0))
+    e = dm.run(3, e_conv=1e-12)
+    exact = hubbard_ground_energy(6, hubbard_edges(ctx), 3, 3, 1.0, 4.0)
+    print("E %.12f exact %.12f" % (e, exact))
+    assert abs(e - exact) < 1e-9
 
