@@ -314,17 +314,10 @@ def measure_lif_sweep():
     out = {"workload": "lif_t3ns_test3_singlet", "reference_energy": -107.1152169273}
     for prec in ("f64", "mixed"):
         ctx = T.Context(netw, "QC", [], ["U1", "SU2", "C2v"], [6, 0, 0], 100, fcidump=fci)
-        solver = DeviceSolver(prec, max_its=10)
         t0 = time.perf_counter()
-        dm = Sweeper(ctx, 100, seed=1, solver=solver, tol=1e-6, truncerr=1e-4, min_dim=100)
-        for D, sweeps, e_conv in ((100, 4, 1e-4), (140, 100, 1e-6)):
-            dm.max_dim = dm.min_dim = D
-            for i in range(sweeps):
-                dm.sweep()
-                if i > 0 and abs(dm.energies[-1] - dm.energies[-2]) < e_conv:
-                    break
+        dm = Sweeper(ctx, 100, seed=1, solver=DeviceSolver(prec))
+        e = dm.run_scheme([(100, 100, 1e-4, 1e-6, 4, 10, 1e-4), (140, 140, 1e-4, 1e-6, 100, 10, 1e-6)])
         t = time.perf_counter() - t0
-        e = min(dm.energies)
         out[prec] = {"energy": e, "abs_err_vs_reference": abs(e - out["reference_energy"]),
                      "sweeps": len(dm.energies), "seconds": t,
                      "seconds_per_sweep": t / len(dm.energies)}

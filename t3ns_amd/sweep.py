@@ -282,6 +282,25 @@ class Sweeper:
         self.energies.append(e)
         return e, recs
 
+    def run_scheme(self, regimes):
+        """execute_optScheme (src/optimize_network.c:640-660, struct regime of
+        include/optScheme.h): per regime (min_dim, max_dim, truncerr,
+        davidson_tol, max_sweeps, davidson_max_its, e_conv), sweeps at those
+        settings until two consecutive sweep energies differ by less than
+        e_conv; returns the lowest energy over all regimes."""
+        for min_dim, max_dim, truncerr, tol, sweeps, its, e_conv in regimes:
+            self.min_dim, self.max_dim, self.truncerr, self.tol = int(min_dim), int(max_dim), truncerr, tol
+            if hasattr(self.solver, "max_its"):
+                self.solver.max_its = int(its)
+            start = len(self.energies)
+            for i in range(sweeps):
+                self.sweep()
+                if i > 0 and abs(self.energies[-1] - self.energies[-2]) < e_conv:
+                    break
+            if len(self.energies) == start:
+                raise ValueError("a regime with no sweeps")
+        return min(self.energies)
+
     def run(self, sweeps, e_conv=0.0):
         """Up to `sweeps` sweeps, stopping when two consecutive sweep energies
         differ by less than e_conv (src/optimize_network.c:410-415)."""

@@ -41,17 +41,11 @@ def test_device_sweep_lif_reference_energy(gpu, case):
     print("case lif", target, D2, flush=True)
     netw, fci = _inputs()
     ctx = T.Context(netw, "QC", [], ["U1", "SU2", "C2v"], target, 100, fcidump=fci)
-    solver = DeviceSolver("f64", max_its=10)
-    dm = Sweeper(ctx, 100, seed=1, solver=solver, tol=1e-6, truncerr=1e-4, min_dim=100)
-    for D, sweeps, its, e_conv in ((100, 4, 10, 1e-4), (D2, 100, 10, 1e-6)):
-        dm.max_dim = dm.min_dim = D
-        solver.max_its = its
-        start = len(dm.energies)
-        for i in range(sweeps):
-            dm.sweep()
-            if i > 0 and abs(dm.energies[-1] - dm.energies[-2]) < e_conv:
-                break
-        print("D %d: %d sweeps, E %.10f" % (D, len(dm.energies) - start, dm.energies[-1]), flush=True)
-    e = min(dm.energies)
+    dm = Sweeper(ctx, 100, seed=1, solver=DeviceSolver("f64"))
+    # the regimes of the reference's test (minD, maxD, truncerr, Davidson
+    # tolerance, sweeps, Davidson iterations, E_CONV), 10 iterations in the
+    # first regime instead of 2 (see above)
+    e = dm.run_scheme([(100, 100, 1e-4, 1e-6, 4, 10, 1e-4), (D2, D2, 1e-4, 1e-6, 100, 10, 1e-6)])
+    print("sweep energies", ["%.10f" % v for v in dm.energies], flush=True)
     print("E %.10f reference %.10f (%s)" % (e, want, src))
     assert abs(e - want) < 1e-5

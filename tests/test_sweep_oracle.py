@@ -83,3 +83,14 @@ def test_sweep_hubbard_tree_exact(sy):
     exact = hubbard_ground_energy(6, hubbard_edges(ctx), 3, 3, 1.0, 4.0)
     assert abs(e - exact) < 1e-10
 
+
+def test_scheme_n2_two_regimes():
+    """Two regimes as in tests/test1.c.in:86-89 (a first truncated one, then
+    one keeping every state): the lowest energy is the full-CI energy."""
+    ctx = T.Context(netw_text("ref_networks/10_T3NS"), "QC", [], ["U1", "SU2"], [14, 0], 64,
+                    fcidump=fcidump())
+    dm = Sweeper(ctx, 64, seed=1, solver=OracleSolver())
+    e = dm.run_scheme([(16, 16, 1e-4, 1e-6, 2, 2, 1e-6), (1000, 1000, 1e-4, 1e-8, 4, 10, 1e-10)])
+    assert dm.energies[0] > N2_FCI + 1e-6  # the first regime truncates
+    assert abs(e - N2_FCI) < 1e-8
+
