@@ -118,9 +118,10 @@ def cpu_baseline(step, x, target_seconds, flops_total):
 
 
 def cpu_sweep_baseline(sweeps=4, e_conv=1e-10):
-    """The sweep leg's Hubbard chain (10 sites, D = 64) with every step on
-    the oracle, the FP64 restatement of the reference's CPU path (OpenMP
-    matvec, Lanczos eigensolver, renormalisation, split), on this host."""
+    """The sweep leg's Hubbard chain (10 sites, D = 64, the same N(0, 1)
+    seed-0 start) with every step on the oracle, the FP64 restatement of
+    the reference's CPU path (OpenMP matvec, Lanczos eigensolver,
+    renormalisation, split), on this host."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import t3ns_amd as T
     from env_util import OracleSolver
@@ -129,7 +130,7 @@ def cpu_sweep_baseline(sweeps=4, e_conv=1e-10):
         netw = f.read()
     ctx = T.Context(netw, "NN_HUBBARD", [1.0, 4.0], ["U1", "U1"], [5, 5], 64)
     t0 = time.perf_counter()
-    dm = Sweeper(ctx, 64, seed=1, solver=OracleSolver(), tol=1e-8)
+    dm = Sweeper(ctx, 64, seed=0, solver=OracleSolver(), tol=1e-5, start="normal")
     e = dm.run(sweeps, e_conv=e_conv)
     t = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -274,23 +275,31 @@ def measure_solve(ctx, step, tol):
 def measure_sweep(sweeps=4, e_conv=1e-10):
     """Two-site DMRG of the NN_HUBBARD(t = 1, U = 4) 10-site chain
     (makenetwork.py 10 DMRG), 10 electrons (U1 x U1: 5 up, 5 down), D = 64,
-    up to 4 sweeps with E_CONV = 1e-10, seeded canonical start: every step
-    on the device (makesiteTensor, renormalisation, Heff + Davidson, split),
-    FP64 and FP32 matvecs with FP64 refinement."""
+    up to 4 sweeps with E_CONV = 1e-10, the reference's default Davidson
+    settings (tolerance 1e-5, 100 iterations); start: every block entry
+    N(0, 1) from PCG64 seed 0 in site then block order, canonicalised by QR
+    toward the first step.  Every step on the device (makesiteTensor,
+    renormalisation, Heff + Davidson, split), FP64, and FP32 matvecs with
+    FP64 refinement on every step (the counts of FP32 and FP64 steps and
+    iterations are reported)."""
     import t3ns_amd as T
     from t3ns_amd.sweep import Sweeper, DeviceSolver
     with open(os.path.join(ROOT, "tests", "golden", "networks", "10_DMRG.netw")) as f:
         netw = f.read()
-    out = {"workload": "nn_hubbard_10_chain_D64", "max_sweeps": sweeps, "e_conv": e_conv}
+    out = {"workload": "nn_hubbard_10_chain_D64", "max_sweeps": sweeps, "e_conv": e_conv,
+           "davidson_tol": 1e-5, "davidson_max_its": 100, "start": "N(0,1) seed 0, QR canonical"}
     for prec in ("f64", "mixed"):
         ctx = T.Context(netw, "NN_HUBBARD", [1.0, 4.0], ["U1", "U1"], [5, 5], 64)
         t0 = time.perf_counter()
-        dm = Sweeper(ctx, 64, seed=1, solver=DeviceSolver(prec), tol=1e-8)
+        solver = DeviceSolver(prec, max_its=100, mixed_min_size=0)
+        dm = Sweeper(ctx, 64, seed=0, solver=solver, tol=1e-5, start="normal")
         e = dm.run(sweeps, e_conv=e_conv)
         t = time.perf_counter() - t0
         out[prec] = {"energy": e, "sweeps": len(dm.energies), "seconds": t,
-                     "seconds_per_sweep": t / len(dm.energies), "sweep_energies": dm.energies}
-        print("# sweep %s: E %.12f in %d sweeps, %.2f s" % (prec, e, len(dm.energies), t),
+                     "seconds_per_sweep": t / len(dm.energies), "sweep_energies": dm.energies,
+                     "counts": dict(solver.counts)}
+        print("# sweep %s: E %.12f in %d sweeps, %.2f s %s" % (prec, e, len(dm.energies), t,
+                                                            solver.counts),
               file=sys.stderr, flush=True)
     out["abs_dE"] = abs(out["mixed"]["energy"] - out["f64"]["energy"])
     out["lif"] = measure_lif_sweep()
@@ -300,12 +309,14 @@ def measure_sweep(sweeps=4, e_conv=1e-10):
 def measure_lif_sweep():
     """The reference's LiF regression run (tests/test3.c.in, singlet): 25
     orbitals, 6 electrons, Z2 x U1 x SU2 x C2v, lif_T3NS tree, two-site
-    steps, minD = maxD = 100 for up to 4 sweeps (E_CONV 1e-4) then 140
-    until E_CONV 1e-6, truncation 1e-4, Davidson tolerance 1e-6 (10
-    iterations per step); reference energy -107.1152169273."""
+    steps, the scheme of tests/test3.c.in:81-84 (minD = maxD = 100, 4
+    Davidson iterations per step, at most 2 sweeps, E_CONV 1e-4; then 140,
+    100 iterations, at most 10 sweeps, E_CONV 1e-6; truncation 1e-4,
+    Davidson tolerance 1e-6); reference energy -107.1152169273.  The mixed
+    leg runs FP32 matvecs with FP64 refinement on every step."""
     import gzip
     import t3ns_amd as T
-    from t3ns_amd.sweep import Sweeper, DeviceSolver
+    from t3ns_amd.sweep import DeviceSolver, Regime, Sweeper
     gold = os.path.join(ROOT, "tests", "golden")
     with gzip.open(os.path.join(gold, "fcidumps", "LiF_3.05.FCIDUMP.gz"), "rt") as f:
         fci = f.read()
@@ -315,12 +326,18 @@ def measure_lif_sweep():
     for prec in ("f64", "mixed"):
         ctx = T.Context(netw, "QC", [], ["U1", "SU2", "C2v"], [6, 0, 0], 100, fcidump=fci)
         t0 = time.perf_counter()
-        dm = Sweeper(ctx, 100, seed=1, solver=DeviceSolver(prec))
-        e = dm.run_scheme([(100, 100, 1e-4, 1e-6, 4, 10, 1e-4), (140, 140, 1e-4, 1e-6, 100, 10, 1e-6)])
+        solver = DeviceSolver(prec, mixed_min_size=0)
+        dm = Sweeper(ctx, 100, seed=1, solver=solver)
+        # the reference's scheme (tests/test3.c.in:81-84): 4 Davidson
+        # iterations x at most 2 sweeps at D = 100, then 100 iterations x at
+        # most 10 sweeps at D = 140
+        e = dm.run_scheme([Regime(100, 100, 1e-4, 1e-6, 4, 2, 1e-4),
+                           Regime(140, 140, 1e-4, 1e-6, 100, 10, 1e-6)])
         t = time.perf_counter() - t0
         out[prec] = {"energy": e, "abs_err_vs_reference": abs(e - out["reference_energy"]),
                      "sweeps": len(dm.energies), "seconds": t,
-                     "seconds_per_sweep": t / len(dm.energies)}
+                     "seconds_per_sweep": t / len(dm.energies),
+                     "regime_energies": dm.regime_energies, "counts": dict(solver.counts)}
         print("# lif sweep %s: E %.10f in %d sweeps, %.2f s" % (prec, e, len(dm.energies), t),
               file=sys.stderr, flush=True)
     return out

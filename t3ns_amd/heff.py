@@ -350,6 +350,22 @@ def plan_block_dims(desc, ops, site):
     return dims[:n]
 
 
+def _vec(a, n, what, writable=False):
+    """`a` as a C-contiguous float64 array of shape (n,); raises ValueError
+    otherwise (the C side reads or writes exactly n doubles).  A writable
+    output must already have that form (no silent copy)."""
+    if writable:
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.shape == (n,)
+                and a.flags.c_contiguous and a.flags.writeable):
+            raise ValueError("%s must be a writable C-contiguous float64 array of %d elements"
+                             % (what, n))
+        return a
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.shape != (n,):
+        raise ValueError("%s has shape %s, the operator needs (%d,)" % (what, a.shape, n))
+    return a
+
+
 class Heff:
     """init_Heffdata (include/Heff.h:166) on the device."""
 
@@ -371,11 +387,11 @@ class Heff:
         return self._h
 
     def matvec(self, vec, out=None):
-        vec = np.ascontiguousarray(vec, dtype=np.float64)
-        if vec.shape != (self.size,):
-            raise ValueError("vector has %s elements, Heff needs %d" % (vec.shape, self.size))
+        vec = _vec(vec, self.size, "vector")
         if out is None:
             out = np.empty(self.size, dtype=np.float64)
+        else:
+            out = _vec(out, self.size, "out", writable=True)
         check(lib().t3ns_heff_matvec(self._h, vec.ctypes.data_as(_lib.c_dbl_p),
                                      out.ctypes.data_as(_lib.c_dbl_p)), "t3ns_heff_matvec")
         return out
@@ -400,9 +416,9 @@ class Heff:
         """Lowest eigenpair (davidson, include/davidson.h:56-60): returns
         (energy, eigenvector, iterations, status) with status 0 converged,
         1 stopped at max_its."""
-        x = np.array(guess, dtype=np.float64, copy=True)
+        x = np.array(_vec(guess, self.size, "guess"), copy=True)
         e, its = C.c_double(), C.c_int()
-        dg = None if diagonal is None else np.ascontiguousarray(diagonal, dtype=np.float64)
+        dg = None if diagonal is None else _vec(diagonal, self.size, "diagonal")
         rc = lib().t3ns_davidson(self._h, x.ctypes.data_as(_lib.c_dbl_p), C.byref(e),
                                  int(max_vecs), int(keep_deflate), float(tol), int(max_its),
                                  None if dg is None else dg.ctypes.data_as(_lib.c_dbl_p),
@@ -443,9 +459,11 @@ def davidson_refined(h_low, h64, guess, max_vecs=30, keep_deflate=2, tol=1e-8, s
     """Davidson with h_low's (reduced-precision) matvec down to switch_tol,
     then FP64 matvecs down to tol.  Returns (energy, vector, its_low,
     its64, status)."""
-    x = np.array(guess, dtype=np.float64, copy=True)
+    if h_low.size != h64.size:
+        raise ValueError("the two operators differ in size")
+    x = np.array(_vec(guess, h64.size, "guess"), copy=True)
     e, i1, i2 = C.c_double(), C.c_int(), C.c_int()
-    dg = None if diagonal is None else np.ascontiguousarray(diagonal, dtype=np.float64)
+    dg = None if diagonal is None else _vec(diagonal, h64.size, "diagonal")
     rc = lib().t3ns_davidson_refined(h_low.handle, h64.handle, x.ctypes.data_as(_lib.c_dbl_p),
                                      C.byref(e), int(max_vecs), int(keep_deflate), float(tol),
                                      float(switch_tol), int(max_its),

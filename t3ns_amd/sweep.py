@@ -21,11 +21,22 @@ every other site tensor is an isometry onto its bond toward the first pair
 src/optimize_network.c:574-640).
 """
 
+from collections import namedtuple
+
 import numpy as np
 
 from . import _lib
 from .environment import Environment
 from .heff import FP32, FP64, Heff, Step, davidson_refined, make_site_tensor, split_site
+
+
+# One regime of an optimisation scheme, fields in the order of struct regime
+# (include/optScheme.h:28-45): the truncation (minD, maxD, truncerr of
+# struct SvalSelect), the Davidson tolerance and iteration limit, the sweep
+# limit and the energy convergence between sweeps.  (The reference's
+# sitesize and noise fields are the Sweeper's site_size and zero here.)
+Regime = namedtuple("Regime", ["min_dim", "max_dim", "truncerr", "davidson_tol",
+                               "davidson_max_its", "max_sweeps", "e_conv"])
 
 
 class SiteData:
@@ -50,10 +61,13 @@ def _site_dims(ctx, site):
     return [np.array(ctx.bond_dims(b), dtype=np.int64) for b in bonds]
 
 
-def isometrise(ctx, site, data, leg, rng):
+def isometrise(ctx, site, data, leg, rng=None):
     """Makes `data` an isometry onto leg `leg`: for every sector of that
     leg (d states) the rows over all the tensor's other index pairs (c >= d
-    of them) are orthonormal, drawn from `rng`."""
+    of them) are orthonormal.  With `rng` they are drawn from it; without,
+    they are the Q factor of the QR decomposition of the tensor's own rows
+    in that sector (the canonicalisation of a given state, as the
+    reference's qr of its random start, src/optimize_network.c:493-515)."""
     dims = _site_dims(ctx, site)
     n0, n1 = len(dims[0]), len(dims[1])
     ids = np.stack([data.qn % n0, (data.qn // n0) % n1, data.qn // (n0 * n1)], axis=1)
@@ -70,7 +84,11 @@ def isometrise(ctx, site, data, leg, rng):
         c = sum(sizes)
         if c < d:
             raise ValueError("sector %d of site %d is larger than its product space" % (sec, site))
-        Q, _ = np.linalg.qr(rng.standard_normal((c, d)))
+        if rng is not None:
+            M = rng.standard_normal((c, d))
+        else:
+            M = np.concatenate([v.reshape(d, -1) for v in vs], axis=1).T
+        Q, _ = np.linalg.qr(M)
         rows = Q.T
         k = 0
         for v, m in zip(vs, sizes):
@@ -94,6 +112,9 @@ class DeviceSolver:
         self.precision = precision
         self.max_vecs, self.keep, self.max_its = max_vecs, keep_deflate, max_its
         self.mixed_min_size = int(mixed_min_size)
+        # steps solved with FP32 matvecs (plus FP64 refinement) and in FP64
+        # alone, and the Davidson iterations of each kind
+        self.counts = {"steps_f32": 0, "steps_f64": 0, "its_f32": 0, "its_f64": 0}
 
     def make_site_tensor(self, desc):
         return make_site_tensor(desc)[0]
@@ -103,11 +124,16 @@ class DeviceSolver:
         try:
             if self.precision == "f64" or h64.size < self.mixed_min_size:
                 e, x, its, rc = h64.davidson(guess, self.max_vecs, self.keep, tol, self.max_its)
-                return e, x, (its, 0)
+                self.counts["steps_f64"] += 1
+                self.counts["its_f64"] += its
+                return e, x, (0, its)
             h32 = Heff(d, o, s, FP32, keepalive=keep)
             try:
                 e, x, i1, i2, rc = davidson_refined(h32, h64, guess, self.max_vecs, self.keep, tol,
                                                     tol, self.max_its)
+                self.counts["steps_f32"] += 1
+                self.counts["its_f32"] += i1
+                self.counts["its_f64"] += i2
                 return e, x, (i1, i2)
             finally:
                 h32.close()
@@ -130,7 +156,15 @@ class Sweeper:
     device calls by default)."""
 
     def __init__(self, ctx, max_dim, seed=0, solver=None, tol=1e-8, truncerr=0.0, min_dim=0,
-                 site_size=2):
+                 site_size=2, start="seeded"):
+        """start: "seeded" -- every site tensor from the context's seeded
+        fill, the ones outside the first step replaced by random isometries
+        toward it; "normal" -- every block entry N(0, 1) from one PCG64
+        generator seeded `seed`, in site order then block order, the sites
+        outside the first step then made isometries toward it by a QR of
+        their own entries per sector."""
+        if start not in ("seeded", "normal"):
+            raise ValueError("start is seeded or normal")
         info = ctx.info()
         self.ctx, self.max_dim, self.tol, self.truncerr = ctx, int(max_dim), float(tol), truncerr
         self.min_dim = int(min_dim)  # struct SvalSelect: minD, maxD, truncerr
@@ -142,7 +176,11 @@ class Sweeper:
         self.tensors = {}
         for s in range(info["sites"]):
             data = SiteData(*ctx.site_tensor(s, seed * 7919 + s + 1))
-            if s in toward:
+            if start == "normal":
+                data.tel[:] = rng.standard_normal(len(data.tel))
+                if s in toward:
+                    isometrise(ctx, s, data, toward[s])
+            elif s in toward:
                 isometrise(ctx, s, data, toward[s], rng)
             self.tensors[s] = data
         # every element the steps use comes from here: the context's problem
@@ -283,23 +321,28 @@ class Sweeper:
         return e, recs
 
     def run_scheme(self, regimes):
-        """execute_optScheme (src/optimize_network.c:640-660, struct regime of
-        include/optScheme.h): per regime (min_dim, max_dim, truncerr,
-        davidson_tol, max_sweeps, davidson_max_its, e_conv), sweeps at those
-        settings until two consecutive sweep energies differ by less than
-        e_conv; returns the lowest energy over all regimes."""
-        for min_dim, max_dim, truncerr, tol, sweeps, its, e_conv in regimes:
-            self.min_dim, self.max_dim, self.truncerr, self.tol = int(min_dim), int(max_dim), truncerr, tol
-            if hasattr(self.solver, "max_its"):
-                self.solver.max_its = int(its)
-            start = len(self.energies)
-            for i in range(sweeps):
-                self.sweep()
-                if i > 0 and abs(self.energies[-1] - self.energies[-2]) < e_conv:
-                    break
-            if len(self.energies) == start:
+        """execute_optScheme (src/optimize_network.c:640-660): every regime
+        (a Regime, or a tuple in its field order, which is that of struct
+        regime, include/optScheme.h:28-45) runs sweeps at its settings until
+        two consecutive sweep energies differ by less than its e_conv or
+        max_sweeps sweeps have run (execute_regime,
+        src/optimize_network.c:395-420); a regime's energy is its last
+        sweep's, and the result is the lowest regime energy."""
+        self.regime_energies = []
+        for reg in regimes:
+            reg = Regime(*reg)
+            if reg.max_sweeps < 1:
                 raise ValueError("a regime with no sweeps")
-        return min(self.energies)
+            self.min_dim, self.max_dim = int(reg.min_dim), int(reg.max_dim)
+            self.truncerr, self.tol = reg.truncerr, reg.davidson_tol
+            if hasattr(self.solver, "max_its"):
+                self.solver.max_its = int(reg.davidson_max_its)
+            for i in range(int(reg.max_sweeps)):
+                self.sweep()
+                if i > 0 and abs(self.energies[-1] - self.energies[-2]) < reg.e_conv:
+                    break
+            self.regime_energies.append(self.energies[-1])
+        return min(self.regime_energies)
 
     def run(self, sweeps, e_conv=0.0):
         """Up to `sweeps` sweeps, stopping when two consecutive sweep energies

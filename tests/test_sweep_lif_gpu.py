@@ -1,14 +1,13 @@
 """The reference's LiF regression energies (tests/test3.c.in, test4.c.in)
 from device two-site sweeps on its lif_T3NS network (25 orbitals, 6
-electrons, Z2 x U1 x SU2 x C2v): two regimes at fixed bond dimension
-(minD = maxD, truncation 1e-4 'E', Davidson tolerance 1e-6; the second
-until the sweep energy changes by less than 1e-6), energies within the
-reference's tolerance 1e-5.
-
-Our start state differs from the reference's (a seeded canonical state
-rather than its random initialisation), so the first regime runs 10
-Davidson iterations per step instead of the reference's 2: with 2, this
-start loses sectors early and stays 5.5 mEh above the singlet minimum."""
+electrons, Z2 x U1 x SU2 x C2v) under the reference's own optimisation
+scheme (tests/test3.c.in:81-84, test4.c.in:81-84; struct regime,
+include/optScheme.h:28-45): regime 1 at minD = maxD = 100, truncation 1e-4
+'E', Davidson tolerance 1e-6, 4 Davidson iterations per step, at most 2
+sweeps, E_CONV 1e-4; regime 2 at D = 140 (test4: 100), 100 iterations, at
+most 10 sweeps, E_CONV 1e-6.  Energies within the reference's tolerance
+1e-5.  The start state is a seeded canonical one (the reference's is
+random), so the path to the minimum differs, not the scheme."""
 import gzip
 import os
 
@@ -16,7 +15,7 @@ import pytest
 
 import t3ns_amd as T
 from conftest import GOLDEN
-from t3ns_amd.sweep import Sweeper, DeviceSolver
+from t3ns_amd.sweep import DeviceSolver, Regime, Sweeper
 
 CASES = [
     # (target (N, 2S, C2v irrep), D of the second regime, energy, source)
@@ -24,6 +23,12 @@ CASES = [
     ([6, 2, 0], 140, -106.8754093145, "tests/test3.c.in:86"),
     ([6, 0, 1], 100, -106.8889308194337247, "tests/test4.c.in:86"),
 ]
+
+
+def lif_scheme(D2):
+    """The two regimes of tests/test3.c.in:81-84 (second regime D2)."""
+    return [Regime(100, 100, 1e-4, 1e-6, davidson_max_its=4, max_sweeps=2, e_conv=1e-4),
+            Regime(D2, D2, 1e-4, 1e-6, davidson_max_its=100, max_sweeps=10, e_conv=1e-6)]
 
 
 def _inputs():
@@ -42,10 +47,7 @@ def test_device_sweep_lif_reference_energy(gpu, case):
     netw, fci = _inputs()
     ctx = T.Context(netw, "QC", [], ["U1", "SU2", "C2v"], target, 100, fcidump=fci)
     dm = Sweeper(ctx, 100, seed=1, solver=DeviceSolver("f64"))
-    # the regimes of the reference's test (minD, maxD, truncerr, Davidson
-    # tolerance, sweeps, Davidson iterations, E_CONV), 10 iterations in the
-    # first regime instead of 2 (see above)
-    e = dm.run_scheme([(100, 100, 1e-4, 1e-6, 4, 10, 1e-4), (D2, D2, 1e-4, 1e-6, 100, 10, 1e-6)])
+    e = dm.run_scheme(lif_scheme(D2))
     print("sweep energies", ["%.10f" % v for v in dm.energies], flush=True)
     print("E %.10f reference %.10f (%s)" % (e, want, src))
     assert abs(e - want) < 1e-5

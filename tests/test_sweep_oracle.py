@@ -14,7 +14,7 @@ import pytest
 import t3ns_amd as T
 from conftest import netw_text
 from env_util import OracleSolver, hubbard_edges, hubbard_ground_energy
-from t3ns_amd.sweep import Sweeper
+from t3ns_amd.sweep import Regime, Sweeper
 from test_environment_oracle import N2_FCI, TREE7
 from test_qc_heff import fcidump
 
@@ -84,13 +84,52 @@ def test_sweep_hubbard_tree_exact(sy):
     assert abs(e - exact) < 1e-10
 
 
-def test_scheme_n2_two_regimes():
-    """Two regimes as in tests/test1.c.in:86-89 (a first truncated one, then
-    one keeping every state): the lowest energy is the full-CI energy."""
-    ctx = T.Context(netw_text("ref_networks/10_T3NS"), "QC", [], ["U1", "SU2"], [14, 0], 64,
+def test_scheme_n2_reference_regimes():
+    """The scheme of tests/test1.c.in:85-88: two regimes at D = 1000
+    (truncation 1e-4 'E', Davidson tolerance 1e-6, E_CONV 1e-8), the first
+    with 4 Davidson iterations and at most 2 sweeps, the second with 100
+    iterations and at most 10 sweeps; the result is the reference's N2
+    energy within its 1e-8 (tests/test1.c.in:99).  (The oracle's solver
+    converges every step; the iteration limit is exercised on the device.)"""
+    ctx = T.Context(netw_text("ref_networks/10_T3NS"), "QC", [], ["U1", "SU2"], [14, 0], 1000,
                     fcidump=fcidump())
-    dm = Sweeper(ctx, 64, seed=1, solver=OracleSolver())
-    e = dm.run_scheme([(16, 16, 1e-4, 1e-6, 2, 2, 1e-6), (1000, 1000, 1e-4, 1e-8, 4, 10, 1e-10)])
-    assert dm.energies[0] > N2_FCI + 1e-6  # the first regime truncates
+    dm = Sweeper(ctx, 1000, seed=1, solver=OracleSolver())
+    e = dm.run_scheme([Regime(1000, 1000, 1e-4, 1e-6, 4, 2, 1e-8),
+                       Regime(1000, 1000, 1e-4, 1e-6, 100, 10, 1e-8)])
+    assert len(dm.regime_energies) == 2
     assert abs(e - N2_FCI) < 1e-8
+
+
+class _ScriptedSweeper(Sweeper):
+    """run_scheme's bookkeeping on scripted sweep energies: no tensors."""
+
+    def __init__(self, energies):
+        self.script = list(energies)
+        self.energies = []
+        self.solver = type("S", (), {"max_its": None})()
+        self.seen = []
+
+    def sweep(self):
+        self.seen.append((self.min_dim, self.max_dim, self.truncerr, self.tol,
+                          self.solver.max_its))
+        self.energies.append(self.script.pop(0))
+        return self.energies[-1], []
+
+
+def test_scheme_regime_fields_and_stopping():
+    """Fields in struct regime's order (include/optScheme.h:28-45) reach the
+    sweep; a regime stops after max_sweeps or when two consecutive sweep
+    energies differ by less than e_conv (src/optimize_network.c:406-415);
+    the result is the lowest of the regimes' last-sweep energies
+    (src/optimize_network.c:647-654), not the lowest sweep energy."""
+    sw = _ScriptedSweeper([-1.0, -1.5, -1.2, -1.25, -1.2500001, -9.0])
+    e = sw.run_scheme([Regime(10, 20, 1e-3, 1e-5, davidson_max_its=4, max_sweeps=2, e_conv=1e-9),
+                       Regime(30, 40, 1e-4, 1e-6, davidson_max_its=100, max_sweeps=10, e_conv=1e-6)])
+    assert sw.seen[:2] == [(10, 20, 1e-3, 1e-5, 4)] * 2
+    assert sw.seen[2:] == [(30, 40, 1e-4, 1e-6, 100)] * 3
+    assert sw.regime_energies == [-1.5, -1.2500001]
+    assert e == -1.5
+    # the lowest sweep energy of a regime that goes on to rise is not its result
+    sw = _ScriptedSweeper([-2.0, -1.0, -1.0])
+    assert sw.run_scheme([(1, 1, 0.0, 1e-6, 4, 3, 1e-6)]) == -1.0
 
