@@ -180,6 +180,22 @@ int t3ns_davidson_dev(int64_t size, double* dx, const double* ddiag, t3ns_dev_ma
                       void* vdata, void* stream, double* energy, int max_vecs, int keep_deflate,
                       double tol, int max_its, int* its);
 
+/* t3ns_davidson_dev without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc/hipFree and no allocating library
+ * routine; the solver's buffers (search vectors, images, reductions) live
+ * in a fixed 64 MiB device array per GPU, held until the process ends, so
+ * (2 max_vecs + 1 + keep_deflate) * size doubles (plus small reduction
+ * buffers) must fit in it: about 133 000 elements at max_vecs = 30; larger
+ * sizes and max_vecs above 64 return T3NS_ERR_INVALID.  Every call on a GPU
+ * shares that GPU's array and leaves its contents behind, so calls that
+ * overlap on one GPU race.  The callback must not allocate either for the
+ * whole solve to be allocation-free (t3ns_heff_matvec_device does not). */
+int t3ns_davidson_dev_noalloc(int64_t size, double* dx, const double* ddiag,
+                              t3ns_dev_matvec matvec, void* vdata, void* stream, double* energy,
+                              int max_vecs, int keep_deflate, double tol, int max_its, int* its);
+
 /* ------------------------------------------------------------------------
  * Renormalisation: appending a physical site to renormalised operators.
  * rOperators_append_phys(nr, or) (src/rOperators_pUpdate.c:665-673 with

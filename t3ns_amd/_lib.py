@@ -190,6 +190,8 @@ def lib():
                                             C.c_double, C.c_int, c_dbl_p, c_int_p, c_int_p]),
         "t3ns_davidson_dev": (C.c_int, [C.c_int64, P, P, DEV_MATVEC, P, P, c_dbl_p, C.c_int,
                                         C.c_int, C.c_double, C.c_int, c_int_p]),
+        "t3ns_davidson_dev_noalloc": (C.c_int, [C.c_int64, P, P, DEV_MATVEC, P, P, c_dbl_p,
+                                                C.c_int, C.c_int, C.c_double, C.c_int, c_int_p]),
         "t3ns_heff_size": (C.c_int64, [P]),
         "t3ns_heff_stats": (C.c_int, [P, c_i64_p, c_i64_p, c_i64_p, c_dbl_p]),
         "t3ns_heff_plan_table": (C.c_int64, [P, c_int_p, c_dbl_p]),

@@ -372,6 +372,26 @@ int t3ns_davidson_dev(int64_t size, double* dx, const double* ddiag, t3ns_dev_ma
     return rc != T3NS_OK ? rc : status;
 }
 
+int t3ns_davidson_dev_noalloc(int64_t size, double* dx, const double* ddiag,
+                              t3ns_dev_matvec matvec, void* vdata, void* stream, double* energy,
+                              int max_vecs, int keep_deflate, double tol, int max_its, int* its) {
+    if (size <= 0 || !dx || !ddiag || !matvec || !energy) return fail(T3NS_ERR_INVALID, "bad argument");
+    int status = 0;
+    const int rc = guarded([&] {
+        const DavidsonResult R = davidson_noalloc(
+            size, dx, ddiag,
+            [matvec, vdata](const double* a, double* b, hipStream_t st) {
+                matvec(a, b, (void*)st, vdata);
+            },
+            (hipStream_t)stream, tol, max_vecs, keep_deflate, max_its);
+        if (R.status < 0) throw std::domain_error("davidson: projected eigenproblem failed");
+        *energy = R.energy;
+        if (its) *its = R.its;
+        status = R.status;
+    });
+    return rc != T3NS_OK ? rc : status;
+}
+
 }  // extern "C"
 
 struct t3ns_rops {

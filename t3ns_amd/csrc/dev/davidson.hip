@@ -5,7 +5,9 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <mutex>
 #include <stdexcept>
+#include <string>
 
 namespace t3 {
 
@@ -133,11 +135,15 @@ __global__ void precond_apply_kernel(double* __restrict__ t, const double* __res
 int grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(4096, (n + 255) / 256)); }
 }  // namespace
 
-int jacobi_eigh(int n, const double* A, int ld, double* w, double* V) {
-    std::vector<double> a((size_t)n * n), v((size_t)n * n, 0.0);
+int jacobi_eigh(int n, const double* A, int ld, double* w, double* V, double* wa, double* wv) {
+    if (n < 1 || n > kMaxDavidsonVecs) return 1;
+    double* a = wa;
+    double* v = wv;
     for (int j = 0; j < n; ++j)
-        for (int i = 0; i < n; ++i) a[i + (size_t)j * n] = A[std::min(i, j) + (size_t)std::max(i, j) * ld];
-    for (int i = 0; i < n; ++i) v[i + (size_t)i * n] = 1.0;
+        for (int i = 0; i < n; ++i) {
+            a[i + (size_t)j * n] = A[std::min(i, j) + (size_t)std::max(i, j) * ld];
+            v[i + (size_t)j * n] = i == j ? 1.0 : 0.0;
+        }
     auto at = [&](int i, int j) -> double& { return a[i + (size_t)j * n]; };
     for (int sweep = 0; sweep < 100; ++sweep) {
         double off = 0, tot = 0;
@@ -172,9 +178,16 @@ int jacobi_eigh(int n, const double* A, int ld, double* w, double* V) {
                 }
             }
     }
-    std::vector<int> idx(n);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return at(x, x) < at(y, y); });
+    // ascending eigenvalues, equal ones in index order (stable insertion sort)
+    int idx[kMaxDavidsonVecs];
+    for (int j = 0; j < n; ++j) {
+        int k = j;
+        while (k > 0 && at(idx[k - 1], idx[k - 1]) > at(j, j)) {
+            idx[k] = idx[k - 1];
+            --k;
+        }
+        idx[k] = j;
+    }
     for (int j = 0; j < n; ++j) {
         w[j] = at(idx[j], idx[j]);
         for (int i = 0; i < n; ++i) V[i + (size_t)j * ld] = v[i + (size_t)idx[j] * n];
@@ -184,40 +197,83 @@ int jacobi_eigh(int n, const double* A, int ld, double* w, double* V) {
     return 0;
 }
 
-DeviceDavidson::DeviceDavidson(int64_t size, int max_vecs, int keep_deflate, hipStream_t s)
-    : size_(size), mv_(max_vecs), keep_(keep_deflate), s_(s) {
+namespace {
+constexpr size_t kAlign = 32;  // doubles: 256 bytes
+size_t rounded(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
+}  // namespace
+
+void DeviceDavidson::setup(int64_t size, int max_vecs, int keep_deflate) {
     if (size <= 0 || max_vecs < 2 || keep_deflate < 1 || keep_deflate >= max_vecs)
         throw std::invalid_argument("davidson: bad sizes");
+    if (max_vecs > kMaxDavidsonVecs)
+        throw std::invalid_argument("davidson: max_vecs above " + std::to_string(kMaxDavidsonVecs));
+    size_ = size;
+    mv_ = max_vecs;
+    keep_ = keep_deflate;
     if (max_vecs > size) mv_ = (int)std::max<int64_t>(2, size);
     if (keep_ >= mv_) keep_ = mv_ - 1;
-    V_.alloc((size_t)size * mv_);
-    VA_.alloc((size_t)size * mv_);
-    t_.alloc(size);
-    tmp_.alloc((size_t)size * keep_);
     nblk_ = (int)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (size + kRedThreads - 1) / kRedThreads));
-    partial_.alloc((size_t)nblk_ * (mv_ + 2));
-    coef_.alloc(mv_);
-    out_.alloc(mv_ + 2);
-    sub_.assign((size_t)mv_ * mv_, 0.0);
-    eigv_.assign((size_t)mv_ * mv_, 0.0);
-    eigval_.assign(mv_, 0.0);
+    std::fill(sub_, sub_ + kMaxDavidsonVecs * kMaxDavidsonVecs, 0.0);
+    std::fill(eigv_, eigv_ + kMaxDavidsonVecs * kMaxDavidsonVecs, 0.0);
+    std::fill(eigval_, eigval_ + kMaxDavidsonVecs, 0.0);
+}
+
+size_t DeviceDavidson::workspace_elems(int64_t size, int max_vecs, int keep_deflate) {
+    int mv = max_vecs, keep = keep_deflate;
+    if (mv > size) mv = (int)std::max<int64_t>(2, size);
+    if (keep >= mv) keep = mv - 1;
+    const size_t nb = (size_t)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (size + kRedThreads - 1) / kRedThreads));
+    return 2 * rounded((size_t)size * mv) + rounded(size) + rounded((size_t)size * keep) +
+           rounded(nb * (mv + 2)) + rounded(mv) + rounded(mv + 2);
+}
+
+void DeviceDavidson::carve(double* ws) {
+    double* p = ws;
+    auto take = [&](size_t n) {
+        double* q = p;
+        p += rounded(n);
+        return q;
+    };
+    V_ = take((size_t)size_ * mv_);
+    VA_ = take((size_t)size_ * mv_);
+    t_ = take(size_);
+    tmp_ = take((size_t)size_ * keep_);
+    partial_ = take((size_t)nblk_ * (mv_ + 2));
+    coef_ = take(mv_);
+    out_ = take(mv_ + 2);
+}
+
+DeviceDavidson::DeviceDavidson(int64_t size, int max_vecs, int keep_deflate, hipStream_t s)
+    : s_(s) {
+    setup(size, max_vecs, keep_deflate);
+    own_.alloc(workspace_elems(size, max_vecs, keep_deflate));
+    carve(own_.get());
+}
+
+DeviceDavidson::DeviceDavidson(int64_t size, int max_vecs, int keep_deflate, hipStream_t s,
+                               double* ws, size_t ws_elems)
+    : s_(s) {
+    setup(size, max_vecs, keep_deflate);
+    if (!ws || ws_elems < workspace_elems(size, max_vecs, keep_deflate))
+        throw std::invalid_argument("davidson: workspace too small");
+    carve(ws);
 }
 
 void DeviceDavidson::dots(const double* A, int64_t lda, int n, const double* b, double* host_out) {
     if (n == 0) return;
     hipLaunchKernelGGL(dots_kernel, dim3(nblk_), dim3(kRedThreads), 0, s_, A, lda, n, b, size_,
-                       partial_.get());
+                       partial_);
     hipLaunchKernelGGL(finish_kernel, dim3(std::min(n, 64)), dim3(kRedThreads), 0, s_,
-                       partial_.get(), nblk_, n, out_.get());
-    T3_HIP_CHECK(hipMemcpyAsync(host_out, out_.get(), n * sizeof(double), hipMemcpyDeviceToHost, s_));
+                       partial_, nblk_, n, out_);
+    T3_HIP_CHECK(hipMemcpyAsync(host_out, out_, n * sizeof(double), hipMemcpyDeviceToHost, s_));
     T3_HIP_CHECK(hipStreamSynchronize(s_));
 }
 
 void DeviceDavidson::combine(const double* A, int n, const double* coef_host, double* out,
                              double beta) {
-    T3_HIP_CHECK(hipMemcpyAsync(coef_.get(), coef_host, n * sizeof(double), hipMemcpyHostToDevice, s_));
+    T3_HIP_CHECK(hipMemcpyAsync(coef_, coef_host, n * sizeof(double), hipMemcpyHostToDevice, s_));
     hipLaunchKernelGGL(combine_kernel, dim3(grid(size_)), dim3(256), 0, s_, A, size_, n,
-                       coef_.get(), out, beta);
+                       coef_, out, beta);
 }
 
 double DeviceDavidson::norm(const double* v) {
@@ -228,11 +284,11 @@ double DeviceDavidson::norm(const double* v) {
 
 // t <- orthonormal complement of t against V[:, :m] (two CGS passes).
 void DeviceDavidson::orthonormalise(double* t) {
-    std::vector<double> c(mv_);
+    double* c = hc_;
     for (int pass = 0; pass < 2 && m_ > 0; ++pass) {
-        dots(V_.get(), size_, m_, t, c.data());
+        dots(V_, size_, m_, t, c);
         for (int j = 0; j < m_; ++j) c[j] = -c[j];
-        combine(V_.get(), m_, c.data(), t, 1.0);
+        combine(V_, m_, c, t, 1.0);
     }
     const double nrm = norm(t);
     if (!(nrm > 0)) throw std::runtime_error("davidson: search vector vanished");
@@ -242,25 +298,25 @@ void DeviceDavidson::orthonormalise(double* t) {
 // Column m of the projected matrix: V[:, i] . VA[:, m], i <= m
 // (src/davidson.c:132-145).
 void DeviceDavidson::expand(int m) {
-    std::vector<double> c(m + 1);
-    dots(V_.get(), size_, m + 1, VA_.get() + (int64_t)m * size_, c.data());
+    double* c = hc_;
+    dots(V_, size_, m + 1, VA_ + (int64_t)m * size_, c);
     for (int i = 0; i <= m; ++i) sub_[i + (size_t)m * mv_] = c[i];
 }
 
 int DeviceDavidson::eigsolve() {
-    return jacobi_eigh(m_, sub_.data(), mv_, eigval_.data(), eigv_.data());
+    return jacobi_eigh(m_, sub_, mv_, eigval_, eigv_, ja_, jv_);
 }
 
 // Keep the keep_ lowest Ritz vectors (src/davidson.c:164-185).
 void DeviceDavidson::deflate() {
     for (int which = 0; which < 2; ++which) {
-        double* B = which == 0 ? V_.get() : VA_.get();
+        double* B = which == 0 ? V_ : VA_;
         for (int k = 0; k < keep_; ++k)
-            combine(B, m_, eigv_.data() + (size_t)k * mv_, tmp_.get() + (int64_t)k * size_, 0.0);
-        hipLaunchKernelGGL(copy_kernel, dim3(grid(size_ * keep_)), dim3(256), 0, s_, tmp_.get(), B,
+            combine(B, m_, eigv_ + (size_t)k * mv_, tmp_ + (int64_t)k * size_, 0.0);
+        hipLaunchKernelGGL(copy_kernel, dim3(grid(size_ * keep_)), dim3(256), 0, s_, tmp_, B,
                            size_ * keep_);
     }
-    std::fill(sub_.begin(), sub_.end(), 0.0);
+    std::fill(sub_, sub_ + (size_t)mv_ * mv_, 0.0);
     m_ = 0;
     while (m_ < keep_) {
         expand(m_);
@@ -269,49 +325,49 @@ void DeviceDavidson::deflate() {
 }
 
 std::pair<double, double> DeviceDavidson::rayleigh(const double* x, const DevMatvec& mv) {
-    double* y = VA_.get();
+    double* y = VA_;
     mv(x, y, s_);
     double xx, xy;
     dots(x, size_, 1, x, &xx);
     dots(x, size_, 1, y, &xy);
     if (!(xx > 0)) throw std::runtime_error("davidson: zero vector");
     const double theta = xy / xx;
-    hipLaunchKernelGGL(residual_kernel, dim3(grid(size_)), dim3(256), 0, s_, y, x, theta, t_.get(),
+    hipLaunchKernelGGL(residual_kernel, dim3(grid(size_)), dim3(256), 0, s_, y, x, theta, t_,
                        size_);
-    return {theta, norm(t_.get()) / std::sqrt(xx)};
+    return {theta, norm(t_) / std::sqrt(xx)};
 }
 
 double DeviceDavidson::relative_difference(const double* x, const DevMatvec& a, const DevMatvec& b) {
-    double* ya = VA_.get();
-    double* yb = t_.get();
+    double* ya = VA_;
+    double* yb = t_;
     a(x, ya, s_);
     b(x, yb, s_);
     const double na = norm(ya);
-    hipLaunchKernelGGL(residual_kernel, dim3(grid(size_)), dim3(256), 0, s_, ya, yb, 1.0, tmp_.get(),
+    hipLaunchKernelGGL(residual_kernel, dim3(grid(size_)), dim3(256), 0, s_, ya, yb, 1.0, tmp_,
                        size_);
-    return na > 0 ? norm(tmp_.get()) / na : 0.0;
+    return na > 0 ? norm(tmp_) / na : 0.0;
 }
 
 // Ritz vector x = V y, residual t = VA y - theta x (src/davidson.c:187-202),
 // then the preconditioned correction of t against x (src/davidson.c:
 // 204-257).  Returns the residual norm; theta is eigval_[0].
 double DeviceDavidson::ritz_and_correct(double* x, const double* diag) {
-    std::vector<double> c(mv_);
+    double* c = hc_;
     const double theta = eigval_[0];
-    combine(V_.get(), m_, eigv_.data(), x, 0.0);
-    combine(VA_.get(), m_, eigv_.data(), t_.get(), 0.0);
+    combine(V_, m_, eigv_, x, 0.0);
+    combine(VA_, m_, eigv_, t_, 0.0);
     for (int j = 0; j < m_; ++j) c[j] = -theta * eigv_[j];
-    combine(V_.get(), m_, c.data(), t_.get(), 1.0);
-    const double residual = norm(t_.get());
-    hipLaunchKernelGGL(precond_dots_kernel, dim3(nblk_), dim3(kRedThreads), 0, s_, x, t_.get(),
-                       diag, theta, size_, partial_.get());
-    hipLaunchKernelGGL(finish_kernel, dim3(2), dim3(kRedThreads), 0, s_, partial_.get(), nblk_,
-                       2, out_.get());
+    combine(V_, m_, c, t_, 1.0);
+    const double residual = norm(t_);
+    hipLaunchKernelGGL(precond_dots_kernel, dim3(nblk_), dim3(kRedThreads), 0, s_, x, t_,
+                       diag, theta, size_, partial_);
+    hipLaunchKernelGGL(finish_kernel, dim3(2), dim3(kRedThreads), 0, s_, partial_, nblk_,
+                       2, out_);
     double uk[2];
-    T3_HIP_CHECK(hipMemcpyAsync(uk, out_.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
+    T3_HIP_CHECK(hipMemcpyAsync(uk, out_, 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
     T3_HIP_CHECK(hipStreamSynchronize(s_));
     const double alpha = -uk[0] / uk[1];
-    hipLaunchKernelGGL(precond_apply_kernel, dim3(grid(size_)), dim3(256), 0, s_, t_.get(), x,
+    hipLaunchKernelGGL(precond_apply_kernel, dim3(grid(size_)), dim3(256), 0, s_, t_, x,
                        diag, theta, alpha, size_);
     return residual;
 }
@@ -326,10 +382,10 @@ DavidsonResult DeviceDavidson::iterate(double* x, const double* diag, const DevM
             floor_hit = true;
             break;
         }
-        orthonormalise(t_.get());
-        double* Vm = V_.get() + (int64_t)m_ * size_;
-        double* VAm = VA_.get() + (int64_t)m_ * size_;
-        hipLaunchKernelGGL(copy_kernel, dim3(grid(size_)), dim3(256), 0, s_, t_.get(), Vm, size_);
+        orthonormalise(t_);
+        double* Vm = V_ + (int64_t)m_ * size_;
+        double* VAm = VA_ + (int64_t)m_ * size_;
+        hipLaunchKernelGGL(copy_kernel, dim3(grid(size_)), dim3(256), 0, s_, t_, Vm, size_);
         mv(Vm, VAm, s_);
         expand(m_);
         ++m_;
@@ -358,7 +414,7 @@ DavidsonResult DeviceDavidson::iterate(double* x, const double* diag, const DevM
 DavidsonResult DeviceDavidson::solve(double* x, const double* diag, const DevMatvec& mv,
                                      double tol, int max_its, double rel_floor) {
     m_ = 0;
-    hipLaunchKernelGGL(copy_kernel, dim3(grid(size_)), dim3(256), 0, s_, x, t_.get(), size_);
+    hipLaunchKernelGGL(copy_kernel, dim3(grid(size_)), dim3(256), 0, s_, x, t_, size_);
     return iterate(x, diag, mv, tol, max_its, rel_floor, 0, HUGE_VAL);
 }
 
@@ -368,14 +424,14 @@ DavidsonResult DeviceDavidson::resume(double* x, const double* diag, const DevMa
     if (m_ > keep_) {
         // keep the lowest Ritz vectors of the current projected problem
         for (int k = 0; k < keep_; ++k)
-            combine(V_.get(), m_, eigv_.data() + (size_t)k * mv_, tmp_.get() + (int64_t)k * size_, 0.0);
-        hipLaunchKernelGGL(copy_kernel, dim3(grid(size_ * keep_)), dim3(256), 0, s_, tmp_.get(),
-                           V_.get(), size_ * keep_);
+            combine(V_, m_, eigv_ + (size_t)k * mv_, tmp_ + (int64_t)k * size_, 0.0);
+        hipLaunchKernelGGL(copy_kernel, dim3(grid(size_ * keep_)), dim3(256), 0, s_, tmp_,
+                           V_, size_ * keep_);
         m_ = keep_;
     }
-    std::fill(sub_.begin(), sub_.end(), 0.0);
+    std::fill(sub_, sub_ + (size_t)mv_ * mv_, 0.0);
     for (int k = 0; k < m_; ++k) {
-        mv(V_.get() + (int64_t)k * size_, VA_.get() + (int64_t)k * size_, s_);
+        mv(V_ + (int64_t)k * size_, VA_ + (int64_t)k * size_, s_);
         expand(k);
     }
     DavidsonResult R;
@@ -385,6 +441,54 @@ DavidsonResult DeviceDavidson::resume(double* x, const double* diag, const DevMa
     }
     const double residual = ritz_and_correct(x, diag);
     return iterate(x, diag, mv, tol, max_its, 0.0, 1, residual);
+}
+
+// ---------------------------------------------------------------------------
+// Davidson without allocation.  The solver's device buffers live in this
+// file-scope array: every device the library runs on holds its own copy
+// (kNoAllocBytes) until the process ends.  Host code may not use the
+// array's name as a device address; each device's address comes from
+// hipGetSymbolAddress, taken once per device with that device current, and
+// only that pointer is handed to kernels and copies.  Every call on a
+// device shares that device's array and leaves its contents behind: the
+// solver writes every buffer before reading it, so nothing needs clearing,
+// and calls that overlap on one device race.
+namespace {
+__device__ __align__(256) double scratch[kNoAllocBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+double* scratch_for_current_device() {
+    int dev = 0;
+    T3_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("HIP error: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        T3_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+DavidsonResult davidson_noalloc(int64_t size, double* dx, const double* ddiag,
+                                const DevMatvec& mv, hipStream_t s, double tol, int max_vecs,
+                                int keep_deflate, int max_its) {
+    const size_t cap = kNoAllocBytes / sizeof(double);
+    if (size <= 0 || max_vecs < 2 || keep_deflate < 1 || keep_deflate >= max_vecs)
+        throw std::invalid_argument("davidson: bad sizes");
+    const size_t need = DeviceDavidson::workspace_elems(size, max_vecs, keep_deflate);
+    if (need > cap)
+        throw std::invalid_argument(
+            "davidson_noalloc: " + std::to_string(size) + " elements x " + std::to_string(max_vecs) +
+            " vectors need " + std::to_string(need * sizeof(double)) +
+            " bytes of workspace, more than the fixed " + std::to_string(kNoAllocBytes) + " bytes");
+    double* ws = scratch_for_current_device();
+    DeviceDavidson dav(size, max_vecs, keep_deflate, s, ws, cap);
+    return dav.solve(dx, ddiag, mv, tol, max_its);
 }
 
 }  // namespace t3

@@ -21,6 +21,10 @@
 
 namespace t3 {
 
+// Largest search space (max_vecs) the solver takes; its host-side
+// projected problem lives in fixed arrays of this size (no heap use).
+constexpr int kMaxDavidsonVecs = 64;
+
 // y = A x on device pointers (FP64), on the given stream.
 using DevMatvec = std::function<void(const double* x, double* y, hipStream_t s)>;
 
@@ -33,7 +37,15 @@ struct DavidsonResult {
 
 class DeviceDavidson {
 public:
+    // Allocates its device buffers (hipMalloc).
     DeviceDavidson(int64_t size, int max_vecs, int keep_deflate, hipStream_t s);
+    // Takes its device buffers from `ws` (ws_elems doubles, at least
+    // workspace_elems(size, max_vecs, keep_deflate)) and allocates nothing.
+    DeviceDavidson(int64_t size, int max_vecs, int keep_deflate, hipStream_t s, double* ws,
+                   size_t ws_elems);
+    // Device doubles the solver needs for these sizes (each buffer rounded
+    // up to 256 bytes).
+    static size_t workspace_elems(int64_t size, int max_vecs, int keep_deflate);
     // x (device, size): initial guess in, eigenvector out.  diag (device):
     // the operator's diagonal for the preconditioner.
     // rel_floor > 0: also stop (status 2) once the residual is below
@@ -64,20 +76,40 @@ private:
     DavidsonResult iterate(double* x, const double* diag, const DevMatvec& mv, double tol,
                            int max_its, double rel_floor, int its, double residual);
 
+    void setup(int64_t size, int max_vecs, int keep_deflate);
+    void carve(double* ws);
+
     int64_t size_;
     int mv_, keep_;
     hipStream_t s_;
-    DevBuf<double> V_, VA_, t_, tmp_;
-    DevBuf<double> coef_, partial_, out_;
-    std::vector<double> sub_, eigv_, eigval_;
+    DevBuf<double> own_;  // the device buffers when the solver allocates them
+    double *V_ = nullptr, *VA_ = nullptr, *t_ = nullptr, *tmp_ = nullptr;
+    double *coef_ = nullptr, *partial_ = nullptr, *out_ = nullptr;
+    // projected matrix, its eigenvectors (both mv_ x mv_, leading dimension
+    // mv_), eigenvalues, host coefficients and the eigensolver's work
+    double sub_[kMaxDavidsonVecs * kMaxDavidsonVecs];
+    double eigv_[kMaxDavidsonVecs * kMaxDavidsonVecs];
+    double eigval_[kMaxDavidsonVecs];
+    double hc_[kMaxDavidsonVecs + 2];
+    double ja_[kMaxDavidsonVecs * kMaxDavidsonVecs];
+    double jv_[kMaxDavidsonVecs * kMaxDavidsonVecs];
     int m_ = 0;
     int nblk_ = 0;
 };
 
 // Symmetric eigendecomposition of a small dense matrix (column-major n x n
-// with leading dimension ld) by cyclic Jacobi rotations; eigenvalues in
-// ascending order, eigenvectors as columns of V (ld).  Returns 0 on
-// success.
-int jacobi_eigh(int n, const double* A, int ld, double* w, double* V);
+// with leading dimension ld, n <= kMaxDavidsonVecs) by cyclic Jacobi
+// rotations; eigenvalues in ascending order, eigenvectors as columns of V
+// (ld).  wa, wv: work arrays of n * n doubles.  Returns 0 on success.
+int jacobi_eigh(int n, const double* A, int ld, double* w, double* V, double* wa, double* wv);
+
+// Davidson on caller data without any allocation: the solver's device
+// buffers come from a file-scope device array of kNoAllocBytes on the
+// current device (shared by every such call on that device).  Throws
+// std::invalid_argument when the sizes need more than that.
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+DavidsonResult davidson_noalloc(int64_t size, double* dx, const double* ddiag,
+                                const DevMatvec& mv, hipStream_t s, double tol, int max_vecs,
+                                int keep_deflate, int max_its);
 
 }  // namespace t3

@@ -488,6 +488,21 @@ def davidson_device(size, dx_ptr, ddiag_ptr, matvec, stream_ptr=0, max_vecs=30, 
     return e.value, its.value, rc
 
 
+def davidson_device_noalloc(size, dx_ptr, ddiag_ptr, matvec, stream_ptr=0, max_vecs=30,
+                            keep_deflate=2, tol=1e-8, max_its=100):
+    """davidson_device through t3ns_davidson_dev_noalloc: no device memory
+    is allocated (the solver's buffers are the library's fixed per-GPU
+    array; see include/t3ns_amd.h for the size limit)."""
+    cb = _lib.DEV_MATVEC(lambda x, y, s, v: matvec(x, y, s))
+    e, its = C.c_double(), C.c_int()
+    rc = lib().t3ns_davidson_dev_noalloc(int(size), C.c_void_p(dx_ptr), C.c_void_p(ddiag_ptr), cb,
+                                         None, C.c_void_p(stream_ptr), C.byref(e), int(max_vecs),
+                                         int(keep_deflate), float(tol), int(max_its), C.byref(its))
+    if rc < 0:
+        check(rc, "t3ns_davidson_dev_noalloc")
+    return e.value, its.value, rc
+
+
 # Reference-shaped entry points ------------------------------------------------
 
 def init_Heffdata(Operators, siteObject, desc, precision=FP64):
