@@ -197,6 +197,50 @@ int t3ns_davidson_dev_noalloc(int64_t size, double* dx, const double* ddiag,
                               int max_vecs, int keep_deflate, double tol, int max_its, int* its);
 
 /* ------------------------------------------------------------------------
+ * Reference-shaped entry points: the signatures the reference's caller
+ * optimize_siteTensor (src/optimize_network.c:188-226) uses, so a shim maps
+ * its calls one to one (init_Heffdata -> t3ns_heff_init, destroy_Heffdata ->
+ * t3ns_heff_destroy; the rest below).
+ * ---------------------------------------------------------------------- */
+
+/* matvecT3NS(vec, result, vdata) of include/Heff.h:177 (src/Heff.c:1049):
+ * vdata is the t3ns_heff handle; host arrays of t3ns_heff_size() doubles,
+ * result overwritten.  void like the reference: a failure prints the
+ * message and exits (EXIT_FAILURE), as the reference's fatal errors do
+ * (src/macros.c:49-57). */
+void t3ns_matvecT3NS(const double* vec, double* result, void* vdata);
+
+/* The same operator as a t3ns_dev_matvec callback (device arrays, stream),
+ * vdata the t3ns_heff handle: t3ns_davidson_dev(size, dx, ddiag,
+ * t3ns_heff_dev_matvec, h, ...) runs a solve with no host round trips.
+ * Failures print and exit like t3ns_matvecT3NS. */
+void t3ns_heff_dev_matvec(const double* dx, double* dy, void* stream, void* vdata);
+
+/* make_diagonal (include/Heff.h:185, src/Heff.c:1268-1312): a malloc'ed
+ * array of t3ns_heff_size() doubles that the caller frees with free();
+ * NULL on error (message in t3ns_last_error). */
+double* t3ns_make_diagonal(t3ns_heff* h);
+
+/* davidson (include/davidson.h:56-60, src/davidson.c:259-328) with the
+ * reference's arguments: host guess in / eigenvector out, host diagonal,
+ * and the caller's host matvec callback (e.g. t3ns_matvecT3NS with the
+ * t3ns_heff handle as vdat).  The search space stays on the device; each
+ * matvec copies one vector to the host and its image back.  Returns 0
+ * converged, 1 stopped at max_its, -1 on failure (the reference's dsyev
+ * failure, or a bad argument or device error: message in t3ns_last_error). */
+int t3ns_davidson_cb(double* result, double* energy, int size, int max_vecs, int keep_deflate,
+                     double davidson_tol, int max_its, const double* diagonal,
+                     void (*matvec)(const double*, double*, void*), void* vdat);
+
+/* sparse_eigensolve (include/wrapper_solvers.h:56-60,
+ * src/wrapper_solvers.c:107-128): solver "D" is t3ns_davidson_cb; any
+ * other solver (PRIMME) returns -1. */
+int t3ns_sparse_eigensolve(double* result, double* energy, int size, int max_vecs,
+                           int keep_deflate, double tol, int max_its, const double* diagonal,
+                           void (*matvec)(const double*, double*, void*), void* vdat,
+                           const char solver[]);
+
+/* ------------------------------------------------------------------------
  * Renormalisation: appending a physical site to renormalised operators.
  * rOperators_append_phys(nr, or) (src/rOperators_pUpdate.c:665-673 with
  * unique_rOperators_ap 642-663, pAppend_block 537-640, sum_unique_rOperators

@@ -9,6 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libt3ns_amd.so")
+DRIVER = os.path.join(HERE, "t3ns_optimize_step")
 ARCH = os.environ.get("T3NS_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -64,9 +65,27 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(LIB + ".tmp", LIB)
+    build_driver()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", DRIVER)
     return LIB
+
+
+def build_driver():
+    """The C driver that replays the reference's optimize_siteTensor call
+    sequence through the C ABI (driver/optimize_step.c), linked against the
+    in-tree library."""
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
+    cmd = ["gcc", "-O2", "-std=gnu11", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           "-I" + os.path.join(os.path.dirname(HERE), "include"), "-I" + os.path.join(rocm, "include"),
+           os.path.join(HERE, "driver", "optimize_step.c"), "-o", DRIVER + ".tmp",
+           "-L" + HERE, "-lt3ns_amd", "-L" + os.path.join(rocm, "lib"), "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(rocm, "lib")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed: %s\n%s" % (" ".join(cmd), r.stderr))
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -1,11 +1,14 @@
 // C ABI of t3ns_amd (include/t3ns_amd.h, include/t3ns_host.h).
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/t3ns_amd.h"
 #include "../../include/t3ns_host.h"
@@ -390,6 +393,95 @@ int t3ns_davidson_dev_noalloc(int64_t size, double* dx, const double* ddiag,
         status = R.status;
     });
     return rc != T3NS_OK ? rc : status;
+}
+
+/* Reference-shaped entry points (include/Heff.h:166-192,
+ * include/davidson.h:56-60, include/wrapper_solvers.h:56-60). */
+
+void t3ns_matvecT3NS(const double* vec, double* result, void* vdata) {
+    // void like the reference: a failure is fatal there (src/macros.c:49-57)
+    const int rc = t3ns_heff_matvec(static_cast<t3ns_heff*>(vdata), vec, result);
+    if (rc != T3NS_OK) {
+        std::fprintf(stderr, "matvecT3NS: %s\n", g_last_error.c_str());
+        std::exit(EXIT_FAILURE);
+    }
+}
+
+void t3ns_heff_dev_matvec(const double* dx, double* dy, void* stream, void* vdata) {
+    const int rc = t3ns_heff_matvec_device(static_cast<t3ns_heff*>(vdata), dx, dy, stream);
+    if (rc != T3NS_OK) {
+        std::fprintf(stderr, "t3ns_heff_dev_matvec: %s\n", g_last_error.c_str());
+        std::exit(EXIT_FAILURE);
+    }
+}
+
+double* t3ns_make_diagonal(t3ns_heff* h) {
+    if (!h) {
+        fail(T3NS_ERR_INVALID, "null argument");
+        return nullptr;
+    }
+    double* d = static_cast<double*>(std::malloc(sizeof(double) * (size_t)h->dev->size()));
+    if (!d) {
+        fail(T3NS_ERR_NOMEM, "make_diagonal: host allocation failed");
+        return nullptr;
+    }
+    if (t3ns_heff_diagonal(h, d) != T3NS_OK) {
+        std::free(d);
+        return nullptr;
+    }
+    return d;
+}
+
+int t3ns_davidson_cb(double* result, double* energy, int size, int max_vecs, int keep_deflate,
+                     double davidson_tol, int max_its, const double* diagonal,
+                     void (*matvec)(const double*, double*, void*), void* vdat) {
+    if (!result || !energy || !diagonal || !matvec || size <= 0) {
+        fail(T3NS_ERR_INVALID, "davidson: bad argument");
+        return -1;
+    }
+    int status = 0;
+    const int rc = guarded([&] {
+        const int64_t n = size;
+        hipStream_t s = nullptr;
+        T3_HIP_CHECK(hipStreamCreate(&s));
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } guard{s};
+        DevBuf<double> x(n), d(n);
+        T3_HIP_CHECK(hipMemcpyAsync(x.get(), result, n * sizeof(double), hipMemcpyHostToDevice, s));
+        T3_HIP_CHECK(hipMemcpyAsync(d.get(), diagonal, n * sizeof(double), hipMemcpyHostToDevice, s));
+        std::vector<double> hx(n), hy(n);
+        // the caller's matvec works on host arrays: each search vector goes
+        // to the host and its image comes back
+        const DevMatvec mv = [&](const double* a, double* b, hipStream_t st) {
+            T3_HIP_CHECK(hipMemcpyAsync(hx.data(), a, n * sizeof(double), hipMemcpyDeviceToHost, st));
+            T3_HIP_CHECK(hipStreamSynchronize(st));
+            matvec(hx.data(), hy.data(), vdat);
+            T3_HIP_CHECK(hipMemcpyAsync(b, hy.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+            T3_HIP_CHECK(hipStreamSynchronize(st));
+        };
+        DeviceDavidson dav(n, max_vecs, keep_deflate, s);
+        const DavidsonResult R = dav.solve(x.get(), d.get(), mv, davidson_tol, max_its);
+        if (R.status < 0) throw std::domain_error("davidson: projected eigenproblem failed");
+        T3_HIP_CHECK(hipMemcpyAsync(result, x.get(), n * sizeof(double), hipMemcpyDeviceToHost, s));
+        T3_HIP_CHECK(hipStreamSynchronize(s));
+        *energy = R.energy;
+        status = R.status;
+    });
+    return rc != T3NS_OK ? -1 : status;
+}
+
+int t3ns_sparse_eigensolve(double* result, double* energy, int size, int max_vecs,
+                           int keep_deflate, double tol, int max_its, const double* diagonal,
+                           void (*matvec)(const double*, double*, void*), void* vdat,
+                           const char solver[]) {
+    if (!solver || std::strcmp(solver, "D") != 0) {
+        fail(T3NS_ERR_INVALID, "sparse_eigensolve: only the Davidson solver \"D\" is built");
+        return -1;
+    }
+    return t3ns_davidson_cb(result, energy, size, max_vecs, keep_deflate, tol, max_its, diagonal,
+                            matvec, vdat);
 }
 
 }  // extern "C"

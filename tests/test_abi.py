@@ -13,7 +13,7 @@ def _declared():
         with open(os.path.join(ROOT, "include", h)) as f:
             txt = f.read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-        for m in re.finditer(r"\b(t3ns_[a-z0-9_]+)\s*\(", txt):
+        for m in re.finditer(r"\b(t3ns_[A-Za-z0-9_]+)\s*\(", txt):
             names.add(m.group(1))
     names.discard("t3ns_dev_matvec")
     return names
