@@ -67,6 +67,12 @@ def parse():
                     help="skip the ground-state solve on the Hamiltonian's own operators")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the DMRG sweep of the 10-site Hubbard chain")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the served-solve stream")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests of the served-solve stream (the first ones of the seed-3 stream)")
+    ap.add_argument("--serve-rate", type=float, default=4.0,
+                    help="Poisson arrival rate of the served-solve stream (requests/s)")
     ap.add_argument("--solve-tol", type=float, default=1e-5,
                     help="Davidson residual tolerance of the solve leg")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -343,6 +349,49 @@ def measure_lif_sweep():
     return out
 
 
+def measure_serving(n, rate, slots=8):
+    """The served-solve stream (NN_HUBBARD chains and trees, L ~ U{8..32},
+    U ~ U[1, 8], D ~ U{64, 128, 256, 512}, half filling, U1 x U1, 4 sweeps
+    each, Poisson arrivals at `rate`/s, seed 3), its first n requests:
+    served with continuous admission (slots, HBM budget) against the same
+    solves run one at a time in FP64 in arrival order (a FIFO queue under
+    the same arrivals).  Reports solves/s, p50/p99 latency of both and the
+    largest energy difference between a served result and the same solve
+    alone."""
+    from t3ns_amd import serve
+    reqs = serve.hubbard_stream(n, rate, seed=3)
+    t0 = time.perf_counter()
+    recs = serve.Server(slots=slots).run(reqs)
+    t_served = time.perf_counter() - t0
+    served = serve.latency_summary(recs)
+    print("# serve: %d solves in %.1f s, p50 %.2f s p99 %.2f s" %
+          (n, t_served, served["p50_s"], served["p99_s"]), file=sys.stderr, flush=True)
+    # one at a time, plain FP64, arrival order: service times, then the
+    # FIFO schedule under the same arrivals
+    service, energies = [], []
+    for r in reqs:
+        s0 = time.perf_counter()
+        e, _ = serve.solve(r)
+        service.append(time.perf_counter() - s0)
+        energies.append(e)
+    clock, lat = 0.0, []
+    for r, sv in zip(reqs, service):
+        clock = max(clock, r.arrival) + sv
+        lat.append(clock - r.arrival)
+    span = clock - reqs[0].arrival
+    alone = {"solves": n, "solves_per_s": n / span, "p50_s": float(np.percentile(lat, 50)),
+             "p99_s": float(np.percentile(lat, 99)), "makespan_s": span,
+             "service_s_total": float(sum(service))}
+    print("# serve alone: %.1f s of service, p50 %.2f s p99 %.2f s" %
+          (sum(service), alone["p50_s"], alone["p99_s"]), file=sys.stderr, flush=True)
+    dE = max(abs(rec["energy"] - e) for rec, e in zip(recs, energies))
+    return {"workload": "nn_hubbard_stream_seed3", "requests": n, "rate_per_s": rate,
+            "slots": slots, "precision": "f64", "served": served, "one_at_a_time_f64": alone,
+            "speedup_solves_per_s": served["solves_per_s"] / alone["solves_per_s"],
+            "max_abs_dE_served_vs_alone": dE,
+            "requests_LD": [[r.geometry, r.L, r.D] for r in reqs]}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -450,6 +499,8 @@ def main():
         out["solve"] = measure_solve(ctx, step, args.solve_tol)
     if not args.no_sweep:
         out["sweep"] = measure_sweep()
+    if not args.no_serve:
+        out["serve"] = measure_serving(args.serve_requests, args.serve_rate)
     if not args.no_rows:
         t0 = time.perf_counter()
         out["rows"] = measure_rows(step, sites, args.bond_dim)

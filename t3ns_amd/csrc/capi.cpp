@@ -558,7 +558,7 @@ int t3ns_append_phys(const t3ns_append_desc* d, const t3ns_rops_desc* old, t3ns_
         const AppendInput in = append_input_from(d, &o);
         const AppendPlan plan = plan_append(in);
         auto res = std::make_unique<t3ns_rops>();
-        hipStream_t s = nullptr;
+        hipStream_t s = hipStreamPerThread;
         res->r = run_append(plan, o, s, &res->work);
         res->views();
         *out = res.release();
@@ -601,7 +601,7 @@ int t3ns_update_physical(const t3ns_pupdate_desc* d, const t3ns_rops_desc* pops,
         in.pops = &po;
         const PUpdatePlan plan = plan_pupdate(in);
         auto res = std::make_unique<t3ns_rops>();
-        hipStream_t s = nullptr;
+        hipStream_t s = hipStreamPerThread;
         res->r = run_pupdate(plan, po, T, s, &res->work);
         res->views();
         *out = res.release();
@@ -659,7 +659,7 @@ int t3ns_update_branching(const t3ns_bupdate_desc* d, const t3ns_rops_desc* ops,
         in.ops[1] = &o1;
         const BUpdatePlan plan = plan_bupdate(in);
         auto res = std::make_unique<t3ns_rops>();
-        res->r = run_bupdate(plan, T, o0, o1, nullptr, &res->work);
+        res->r = run_bupdate(plan, T, o0, o1, hipStreamPerThread, &res->work);
         res->views();
         *out = res.release();
     });
@@ -693,7 +693,7 @@ int t3ns_split_site(const t3ns_split_desc* d, t3ns_split** out, t3ns_work* work)
         const SplitPlan plan = plan_split(in);
         auto res = std::make_unique<t3ns_split>();
         RenormWork w;
-        res->r = run_split(in, plan, A, nullptr, &w);
+        res->r = run_split(in, plan, A, hipStreamPerThread, &w);
         for (const auto& ir : res->r.bond.irreps) res->irreps.insert(res->irreps.end(), ir.begin(), ir.end());
         for (const auto& v : res->r.sing) {
             res->nsing.push_back((int)v.size());
@@ -780,7 +780,7 @@ int t3ns_qr_step(const t3ns_qr_desc* d, t3ns_qr** out, t3ns_work* work) {
         const QRPlan plan = plan_qr(in);
         auto res = std::make_unique<t3ns_qr>();
         RenormWork w;
-        res->r = run_qr(in, plan, nullptr, &w);
+        res->r = run_qr(in, plan, hipStreamPerThread, &w);
         for (const auto& v : res->r.sing) {
             res->nsing.push_back((int)v.size());
             res->sing.insert(res->sing.end(), v.begin(), v.end());
@@ -845,7 +845,7 @@ int t3ns_make_site_tensor(const t3ns_makesite_desc* d, double* tel, t3ns_work* w
         in.beginblock.assign(d->beginblock, d->beginblock + d->nrblocks + 1);
         const MakesitePlan plan = plan_makesite(in);
         RenormWork w;
-        const std::vector<double> out = run_makesite(plan, in.sites, nullptr, &w);
+        const std::vector<double> out = run_makesite(plan, in.sites, hipStreamPerThread, &w);
         std::copy(out.begin(), out.end(), tel);
         if (work) {
             work->flops = w.flops;

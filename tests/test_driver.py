@@ -5,7 +5,8 @@ t3ns_sparse_eigensolve with t3ns_matvecT3NS as the host callback,
 t3ns_heff_destroy, free; then the device-callback solve (t3ns_davidson_dev
 with t3ns_heff_dev_matvec).  On the GPU its energies and eigenvector must
 equal those of the ctypes path (Heff.davidson on the same step and guess)
-to 1e-12 relative (energy) and 1e-10 (vector, up to sign)."""
+to 1e-12 relative (energy) and 1e-10 (vector, up to sign), with the same
+status and iteration count."""
 import json
 import os
 import subprocess
@@ -54,14 +55,17 @@ def test_driver_matches_ctypes_path(gpu, case, tmp_path):
     cmd = [DRIVER, "--netw", path] + model + ["--sym", ",".join(sy), "--ts",
                                               ",".join(map(str, ts)), "--D", str(D), "--sites",
                                               ",".join(map(str, sites)), "--seed", "7", "--guess",
-                                              str(guess), "--tol", "1e-10", "--its", "300",
+                                              str(guess), "--tol", "1e-7", "--its", "500",
                                               "--out", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     res = json.loads(r.stdout.strip().splitlines()[-1])
     assert res["size"] == h.size
-    e, x, its, rc = h.davidson(x0, 30, 2, 1e-10, 300)
-    assert rc == 0 and res["status"] == 0 and res["status_dev"] == 0
+    e, x, its, rc = h.davidson(x0, 30, 2, 1e-7, 500)
+    # same iteration on the same operator: the same status (0 converged, 1
+    # at the iteration limit; the seeded operators are not a Hamiltonian, so
+    # convergence is not the point)
+    assert res["status"] == rc and res["status_dev"] == rc and res["its_dev"] == its
     assert abs(res["energy"] - e) <= 1e-12 * abs(e), (res["energy"], e)
     assert abs(res["energy_dev"] - e) <= 1e-12 * abs(e), (res["energy_dev"], e)
     xv = np.fromfile(out, dtype=np.float64)
