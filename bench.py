@@ -69,7 +69,7 @@ def parse():
                     help="skip the DMRG sweep of the 10-site Hubbard chain")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the served-solve stream")
-    ap.add_argument("--serve-requests", type=int, default=24,
+    ap.add_argument("--serve-requests", type=int, default=12,
                     help="requests of the served-solve stream (the first ones of the seed-3 stream)")
     ap.add_argument("--serve-rate", type=float, default=4.0,
                     help="Poisson arrival rate of the served-solve stream (requests/s)")
@@ -353,7 +353,8 @@ def measure_serving(n, rate, slots=8):
     """The served-solve stream (NN_HUBBARD chains and trees, L ~ U{8..32},
     U ~ U[1, 8], D ~ U{64, 128, 256, 512}, half filling, U1 x U1, 4 sweeps
     each, Poisson arrivals at `rate`/s, seed 3), its first n requests:
-    served with continuous admission (slots, HBM budget) against the same
+    served with continuous admission (slots, HBM budget; each admitted
+    solve on its own host thread and streams) against the same
     solves run one at a time in FP64 in arrival order (a FIFO queue under
     the same arrivals).  Reports solves/s, p50/p99 latency of both and the
     largest energy difference between a served result and the same solve
@@ -386,7 +387,7 @@ def measure_serving(n, rate, slots=8):
           (sum(service), alone["p50_s"], alone["p99_s"]), file=sys.stderr, flush=True)
     dE = max(abs(rec["energy"] - e) for rec, e in zip(recs, energies))
     return {"workload": "nn_hubbard_stream_seed3", "requests": n, "rate_per_s": rate,
-            "slots": slots, "precision": "f64", "served": served, "one_at_a_time_f64": alone,
+            "slots": slots, "workers": "one host thread per slot", "precision": "f64", "served": served, "one_at_a_time_f64": alone,
             "speedup_solves_per_s": served["solves_per_s"] / alone["solves_per_s"],
             "max_abs_dE_served_vs_alone": dE,
             "requests_LD": [[r.geometry, r.L, r.D] for r in reqs]}

@@ -8,9 +8,9 @@ src/optimize_network.c:640-660; its globals become one Context per solve
 here).  The server admits requests as they arrive while a solve slot is free
 and the request's HBM footprint fits in what the running solves leave of
 the budget, runs every admitted solve on its own host thread (its own
-Context, HIP streams and device buffers), so the work of concurrent solves
-overlaps on the device, and records arrival, admission and completion
-times.  Requests that do not fit wait; a later, smaller request may be
+Context, HIP streams and device buffers) or, with a WorkerPool, in its own
+worker process, so the work of concurrent solves overlaps on the device,
+and records arrival, admission and completion times.  Requests that do not fit wait; a later, smaller request may be
 admitted before them when it fits (no slot is held idle).
 
 Every solve is deterministic and independent of what runs beside it (no
@@ -75,6 +75,45 @@ def solve(req, precision="f64"):
         return e, list(dm.energies)
     finally:
         ctx.close()
+
+
+def _worker_init():
+    # one HIP context per worker process, made before the stream starts
+    from ._lib import lib
+    lib().t3ns_device_count()
+
+
+class WorkerPool:
+    """Solves in worker processes (a solve's host work -- the sweep loop,
+    planning, the small eigenproblems -- then runs on its own core instead of
+    sharing one interpreter with the other solves; the device work of all
+    workers overlaps on the one GPU).  Use as a context manager and pass as
+    Server's solve_fn."""
+
+    def __init__(self, workers):
+        import concurrent.futures as cf
+        import multiprocessing as mp
+        self.pool = cf.ProcessPoolExecutor(max_workers=int(workers),
+                                           mp_context=mp.get_context("spawn"),
+                                           initializer=_worker_init)
+        # start every worker now (their start-up is not part of a request)
+        list(self.pool.map(_ping, range(int(workers))))
+
+    def __call__(self, req, precision):
+        return self.pool.submit(solve, req, precision).result()
+
+    def close(self):
+        self.pool.shutdown(wait=True)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _ping(i):
+    return i
 
 
 class Server:

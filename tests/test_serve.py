@@ -97,3 +97,16 @@ def test_served_chain_reaches_exact_energy(gpu):
     rec = Server(slots=1).run([req])[0]
     exact = hubbard_ground_energy(8, [(i, i + 1) for i in range(7)], 4, 4, 1.0, 4.0)
     assert abs(rec["energy"] - exact) < 1e-9, (rec["energy"], exact)
+
+
+@pytest.mark.gpu
+def test_served_in_worker_processes_equal_alone(gpu):
+    """The same through worker processes (WorkerPool): results equal the
+    solves run alone in this process."""
+    reqs = [Request(i, ("chain", "tree")[i % 2], 8 + i, 3.0 + i, 32, 2, 0.005 * i) for i in range(4)]
+    print("worker-pool requests", [(r.geometry, r.L) for r in reqs], flush=True)
+    with serve.WorkerPool(2) as pool:
+        recs = Server(slots=2, solve_fn=pool).run(reqs)
+    for req, rec in zip(reqs, recs):
+        e, _ = serve.solve(req)
+        assert abs(rec["energy"] - e) <= 1e-10 * abs(e), (req, rec["energy"], e)
