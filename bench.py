@@ -35,20 +35,31 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
-PROFILE = os.path.join(ROOT, "profiles", "heff_matvec_qc24_t3ns_D512.md")
 
 
-def profiled_traffic(sites, D):
+def profiled_traffic(workload, sites, D):
     """HBM bytes per FP64 matvec from the committed PMC summary of this
     workload (FETCH_SIZE/WRITE_SIZE, copy-calibrated), or None."""
-    if sites != [18, 31, 17] or D != 512 or not os.path.exists(PROFILE):
+    prof = os.path.join(ROOT, "profiles", "heff_matvec_%s_t3ns_D%d.md" % (workload, D))
+    if sites != WORKLOADS[workload]["sites"] or D != WORKLOADS[workload]["bond_dim"] or \
+            not os.path.exists(prof):
         return None
-    with open(PROFILE) as f:
+    with open(prof) as f:
         for line in f:
             if line.startswith("traffic_bytes_per_matvec:"):
                 return float(line.split(":")[1])
     return None
 NETWORK = os.path.join(ROOT, "tests", "golden", "networks", "24_T3NS.netw")
+# The matvec workloads: synthetic FCIDUMP (orbitals = electrons, seed, ERI
+# rank), U1 x SU2, the makenetwork.py tree, bond dimension and step sites.
+WORKLOADS = {
+    # three-site {physical, branching, physical} step of the QC-24 system
+    "qc24": {"norb": 24, "seed": 1, "rank": 48, "bond_dim": 512, "sites": [18, 31, 17]},
+    # two-site step 8 of the sweep of the QC-50 system at D = 2048: a
+    # branching site whose two virtual bonds carry 2083 and 2066 reduced
+    # states (29.5 M elements, 2.9e14 flops per matvec)
+    "qc50": {"norb": 50, "seed": 4, "rank": 100, "bond_dim": 2048, "sites": [69, 42]},
+}
 
 
 def parse():
@@ -56,8 +67,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--sites", default="18,31,17")
-    ap.add_argument("--bond-dim", type=int, default=512)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="qc24",
+                    help="qc24: the QC-24 T3NS D=512 three-site step (default, with every other "
+                         "leg); qc50: the QC-50 T3NS D=2048 two-site step (matvec legs only)")
+    ap.add_argument("--sites", default=None, help="step sites (default: the workload's)")
+    ap.add_argument("--bond-dim", type=int, default=None, help="default: the workload's")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the baseline sample (0 disables it)")
     ap.add_argument("--precisions", default="f64,f32")
@@ -80,13 +94,15 @@ def parse():
     return ap.parse_args()
 
 
-def build_step(sites, D):
+def build_step(sites, D, workload="qc24"):
     import t3ns_amd as T
     from t3ns_amd import synthetic
-    fci = synthetic.qc_workload(24, 24, seed=1, rank=48)
-    with open(NETWORK) as f:
+    w = WORKLOADS[workload]
+    n = w["norb"]
+    fci = synthetic.qc_workload(n, n, seed=w["seed"], rank=w["rank"])
+    with open(os.path.join(ROOT, "tests", "golden", "networks", "%d_T3NS.netw" % n)) as f:
         netw = f.read()
-    ctx = T.Context(netw, "QC", [], ["U1", "SU2"], [24, 0], D, fcidump=fci)
+    ctx = T.Context(netw, "QC", [], ["U1", "SU2"], [n, 0], D, fcidump=fci)
     step = T.Step(ctx, sites, seed=1)
     return ctx, step
 
@@ -95,7 +111,10 @@ def cpu_baseline(step, x, target_seconds, flops_total):
     """FP64 oracle matvec over a strided sample of output blocks."""
     import oracle as O
     d, o, s = step.descriptors()
+    t0 = time.perf_counter()
     oh = O.OracleHeff(d, o, s)
+    print("# cpu baseline: oracle plan in %.1f s" % (time.perf_counter() - t0), file=sys.stderr,
+          flush=True)
     bf = oh.block_flops()
     out = np.zeros(oh.size)
     nb = oh.nrblocks
@@ -106,6 +125,8 @@ def cpu_baseline(step, x, target_seconds, flops_total):
     t_cal = time.perf_counter() - t0
     f_cal = bf[0:nb:stride].sum()
     rate = f_cal / max(t_cal, 1e-9)
+    print("# cpu baseline: calibration %.1f s, %.3g flop/s" % (t_cal, rate), file=sys.stderr,
+          flush=True)
     want = rate * target_seconds
     frac = min(1.0, want / max(flops_total, 1.0))
     stride = max(1, int(round(1.0 / max(frac, 1e-9))))
@@ -412,9 +433,14 @@ def main():
         print(json.dumps({"copy_bytes": n * 8}))
         return
 
-    sites = [int(v) for v in args.sites.split(",")]
+    wl = WORKLOADS[args.workload]
+    sites = [int(v) for v in args.sites.split(",")] if args.sites else list(wl["sites"])
+    if args.bond_dim is None:
+        args.bond_dim = wl["bond_dim"]
+    if args.workload != "qc24":  # the other legs belong to the default workload
+        args.no_solve = args.no_sweep = args.no_rows = args.no_serve = True
     t0 = time.perf_counter()
-    ctx, step = build_step(sites, args.bond_dim)
+    ctx, step = build_step(sites, args.bond_dim, args.workload)
     t_setup = time.perf_counter() - t0
     print("# setup %.1f s, vector size %d" % (t_setup, step.size), file=sys.stderr, flush=True)
     rng = np.random.default_rng(1)
@@ -468,7 +494,8 @@ def main():
     r = results[main_prec]
     flops = r["flops"]
     out = {
-        "metric": "Heff matvec/s (QC-24 T3NS, D=512, 3-site branching step)",
+        "metric": ("Heff matvec/s (QC-24 T3NS, D=512, 3-site branching step)" if args.workload == "qc24"
+                   else "Heff matvec/s (QC-50 T3NS, D=2048, 2-site branching step)"),
         "value": r["matvec_per_s"],
         "unit": "matvec/s",
         "n_gpus": 1,
@@ -479,14 +506,16 @@ def main():
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": main_prec,
-        "data": "synthetic (FCIDUMP seed 1; N(0,1) operator blocks and input vector)",
-        "config": {"workload": "qc24_t3ns_D%d_sites%s" % (args.bond_dim, "-".join(map(str, sites))),
-                   "norb": 24, "nelec": 24, "symmetry": "U1xSU2", "bond_dim": args.bond_dim,
+        "data": "synthetic (FCIDUMP seed %d; N(0,1) operator blocks and input vector)" % wl["seed"],
+        "config": {"workload": "%s_t3ns_D%d_sites%s" % (args.workload, args.bond_dim,
+                                                         "-".join(map(str, sites))),
+                   "norb": wl["norb"], "nelec": wl["norb"], "symmetry": "U1xSU2",
+                   "bond_dim": args.bond_dim,
                    "sites": sites, "vector_size": step.size,
                    "flops_per_matvec": flops, "setup_s": t_setup},
         "roofline": {"bound": "mfma", "achieved": r["tflops"], "peak": PEAK_TFLOPS[main_prec],
                      "unit": "TFLOP/s", "frac": r["tflops"] / PEAK_TFLOPS[main_prec],
-                     "traffic": profiled_traffic(sites, args.bond_dim) if main_prec == "f64"
+                     "traffic": profiled_traffic(args.workload, sites, args.bond_dim) if main_prec == "f64"
                      else None},
     }
     if "f32" in results and "f64" in results:

@@ -10,8 +10,9 @@ and the request's HBM footprint fits in what the running solves leave of
 the budget, runs every admitted solve on its own host thread (its own
 Context, HIP streams and device buffers) or, with a WorkerPool, in its own
 worker process, so the work of concurrent solves overlaps on the device,
-and records arrival, admission and completion times.  Requests that do not fit wait; a later, smaller request may be
-admitted before them when it fits (no slot is held idle).
+and records arrival, admission and completion times.  Requests that do not
+fit wait; a later, smaller request may be admitted before them when it fits
+(no slot is held idle).
 
 Every solve is deterministic and independent of what runs beside it (no
 shared device state; fixed-order reductions), so a result in the mix equals
