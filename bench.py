@@ -107,41 +107,53 @@ def build_step(sites, D, workload="qc24"):
     return ctx, step
 
 
-def cpu_baseline(step, x, target_seconds, flops_total):
-    """FP64 oracle matvec over a strided sample of output blocks."""
+def cpu_baseline(step, x, target_seconds, flops_total, min_fraction=0.25, max_seconds=40.0):
+    """FP64 oracle matvec over a strided sample of output blocks, its GEMMs
+    on OpenBLAS (one BLAS thread per OpenMP thread, as the reference runs
+    cblas_dgemm inside its OpenMP loop over output blocks).  The sample
+    covers at least `min_fraction` of the matvec's flops when that fits in
+    about max_seconds, else what `target_seconds` allows."""
     import oracle as O
+    blas = O.use_blas()
     d, o, s = step.descriptors()
     t0 = time.perf_counter()
     oh = O.OracleHeff(d, o, s)
-    print("# cpu baseline: oracle plan in %.1f s" % (time.perf_counter() - t0), file=sys.stderr,
-          flush=True)
+    print("# cpu baseline: oracle plan in %.1f s (BLAS %s)" % (time.perf_counter() - t0, blas),
+          file=sys.stderr, flush=True)
     bf = oh.block_flops()
     out = np.zeros(oh.size)
     nb = oh.nrblocks
-    # calibrate on a small sample, then size the sample to the target time
-    stride = max(1, nb // 64)
-    t0 = time.perf_counter()
-    oh.matvec_blocks(x, out, 0, nb, stride)
-    t_cal = time.perf_counter() - t0
-    f_cal = bf[0:nb:stride].sum()
+    # calibrate on a strided subset of about 2 s of work at a guessed
+    # 5 GFLOP/s per thread (the OpenMP loop runs over the subset's blocks)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    want_cal = 2.0 * 5e9 * threads
+    cal = nb
+    while cal > 1 and bf[0:nb:cal].sum() < want_cal:
+        cal = max(1, cal // 2)
+    c0 = time.perf_counter()
+    oh.matvec_blocks(x, out, 0, nb, cal)
+    t_cal = time.perf_counter() - c0
+    f_cal = bf[0:nb:cal].sum()
     rate = f_cal / max(t_cal, 1e-9)
     print("# cpu baseline: calibration %.1f s, %.3g flop/s" % (t_cal, rate), file=sys.stderr,
           flush=True)
-    want = rate * target_seconds
-    frac = min(1.0, want / max(flops_total, 1.0))
-    stride = max(1, int(round(1.0 / max(frac, 1e-9))))
+    frac = min(1.0, rate * target_seconds / max(flops_total, 1.0))
+    if rate * max_seconds >= min_fraction * flops_total:
+        frac = max(frac, min_fraction)
+    stride = max(1, int(1.0 / max(frac, 1e-9)))
     t0 = time.perf_counter()
     oh.matvec_blocks(x, out, 0, nb, stride)
     t = time.perf_counter() - t0
     f = bf[0:nb:stride].sum()
     oh.close()
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": (f / t) / flops_total, "unit": "matvec/s", "cores": threads,
             "kind": "port",
-            "sample": "FP64 oracle restatement of src/Heff.c matvec, OpenMP over output blocks, "
-                      "every %d-th output block (%d of %d blocks, %.1f%% of the matvec flops) in "
-                      "%.1f s; matvec/s = sample flop rate / full-matvec flops"
-                      % (stride, len(range(0, nb, stride)), nb, 100.0 * f / flops_total, t)}
+            "sample": "FP64 oracle restatement of src/Heff.c matvec (GEMMs on OpenBLAS %s), "
+                      "OpenMP over output blocks, every %d-th output block (%d of %d blocks, "
+                      "%.1f%% of the matvec flops) in %.1f s; matvec/s = sample flop rate / "
+                      "full-matvec flops" % ("on" if blas else "unavailable: loop nest", stride,
+                                             len(range(0, nb, stride)), nb,
+                                             100.0 * f / flops_total, t)}
 
 
 def cpu_sweep_baseline(sweeps=4, e_conv=1e-10):
@@ -150,8 +162,10 @@ def cpu_sweep_baseline(sweeps=4, e_conv=1e-10):
     the reference's CPU path (OpenMP matvec, Lanczos eigensolver,
     renormalisation, split), on this host."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as O
     import t3ns_amd as T
     from env_util import OracleSolver
+    O.use_blas()
     from t3ns_amd.sweep import Sweeper
     with open(os.path.join(ROOT, "tests", "golden", "networks", "10_DMRG.netw")) as f:
         netw = f.read()
@@ -231,32 +245,49 @@ def row_problems(step, bond_dim, sites):
     return out, ctx
 
 
-def measure_rows(step, sites, bond_dim, cpu_dim=128):
+def measure_rows(step, sites, bond_dim, cpu_dim=128, same_size_flops=4e10):
     """The renormalisation and site-tensor rows on the benchmark object:
     every device call runs twice and the second is reported (kernel time
     from device events, algorithmic flops and bytes).  The CPU baseline of a
-    row is the FP64 oracle restatement (one thread) on the same row of the
-    same step at bond dimension `cpu_dim` (the full size would take minutes
-    per row on one core); its rate is flops per second of that instance."""
+    row is the FP64 oracle restatement (GEMMs on OpenBLAS, the host's
+    OpenMP threads where the restatement has them) on the same row of the
+    same step: at the same size when the row has at most same_size_flops
+    flops, else at bond dimension `cpu_dim` (the full size would take
+    minutes); its rate is flops per second of the instance it ran."""
+    import oracle as O
+    O.use_blas()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     rows = {}
     probs, ctx = row_problems(step, bond_dim, sites)
-    for name, dev, _ in probs:
+    for name, dev, ora in probs:
         dev()
         f, b, ms = dev()
         rows[name] = _row(f, b, ms)
+        if f <= same_size_flops:
+            t0 = time.perf_counter()
+            ora()
+            t = time.perf_counter() - t0
+            rows[name]["cpu_baseline"] = {"value": f / max(t, 1e-9) / 1e9, "unit": "GFLOP/s",
+                                          "seconds": t, "flops": f, "cores": threads,
+                                          "kind": "port",
+                                          "sample": "same row of the same step, same size"}
     ctx.close()
+    if all("cpu_baseline" in r for r in rows.values()):
+        return rows
     _, small = build_step(sites, cpu_dim)
     sprobs, sctx = row_problems(small, cpu_dim, sites)
     for name, dev, ora in sprobs:
-        if name not in rows:
+        if name not in rows or "cpu_baseline" in rows[name]:
             continue
         f, _, _ = dev()
         t0 = time.perf_counter()
         ora()
         t = time.perf_counter() - t0
         rows[name]["cpu_baseline"] = {"value": f / max(t, 1e-9) / 1e9, "unit": "GFLOP/s",
-                                      "seconds": t, "flops": f, "cores": 1, "kind": "port",
-                                      "sample": "same row of the same step at D=%d" % cpu_dim}
+                                      "seconds": t, "flops": f, "cores": threads, "kind": "port",
+                                      "sample": "same row of the same step at D=%d (the row at "
+                                                "D=%d has %.3g flops)" % (cpu_dim, bond_dim,
+                                                                          rows[name]["flops"])}
     sctx.close()
     return rows
 

@@ -69,8 +69,40 @@ def lib():
             f = getattr(L, n)
             f.restype = r
             f.argtypes = a
+        L.oracle_use_blas.restype = C.c_int
+        L.oracle_use_blas.argtypes = [C.c_char_p]
+        L.oracle_blas_active.restype = C.c_int
         _lib = L
     return _lib
+
+
+def _openblas_path():
+    """scipy's bundled OpenBLAS (CBLAS symbols with the scipy_ prefix), the
+    GEMM the oracle uses in place of the reference's cblas_dgemm; None if
+    scipy is not installed."""
+    import glob
+    try:
+        import scipy
+    except ImportError:
+        return None
+    libs = glob.glob(os.path.join(os.path.dirname(os.path.dirname(scipy.__file__)), "scipy.libs",
+                                  "libscipy_openblas*.so"))
+    return libs[0] if libs else None
+
+
+def use_blas():
+    """Runs the oracle's GEMMs on OpenBLAS from here on (the CPU baseline:
+    the reference calls cblas_dgemm).  The checker itself keeps the plain
+    loop nest unless this is called.  Returns whether BLAS is in use."""
+    path = _openblas_path()
+    if path and not lib().oracle_blas_active():
+        lib().oracle_use_blas(path.encode())
+    return blas_active()
+
+
+def blas_active():
+    """True when the oracle's GEMMs run on OpenBLAS."""
+    return bool(lib().oracle_blas_active())
 
 
 def ref():

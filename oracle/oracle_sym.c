@@ -4,6 +4,7 @@
  * src/symmetry_su2.c:26-120 and the Racah formula of src/Wigner.c:245-330.
  * Irrep arrays are flat int arrays with T3NS_MAX_SYMMETRIES entries per irrep.
  */
+#include <dlfcn.h>
 #include <math.h>
 #include <stdlib.h>
 
@@ -217,11 +218,43 @@ double oracle_prefactor_bUpdate(const int* irr, int u, const int* sgs, int nrsy)
         return p;
 }
 
-/* src/sparseblocks.c:133-158 restated as a plain loop nest */
+/* The reference's do_contract calls cblas_dgemm (src/sparseblocks.c:144,
+ * 153).  oracle_use_blas() points the oracle at an installed CBLAS dgemm
+ * (scipy's OpenBLAS, symbol prefix scipy_) so the CPU baseline runs the
+ * same kind of GEMM; without it the loop nest below serves.  CBLAS
+ * constants: column major 102, no transpose 111, transpose 112. */
+typedef void (*cblas_dgemm_fn)(int, int, int, int, int, int, double, const double*, int,
+                               const double*, int, double, double*, int);
+static cblas_dgemm_fn blas_dgemm = NULL;
+
+int oracle_use_blas(const char* path)
+{
+        void* h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+        if (!h) return -1;
+        cblas_dgemm_fn f = (cblas_dgemm_fn) dlsym(h, "scipy_cblas_dgemm");
+        void (*threads)(int) = (void (*)(int)) dlsym(h, "scipy_openblas_set_num_threads");
+        if (!f || !threads) return -1;
+        threads(1);  /* one BLAS thread inside each OpenMP thread, as the reference runs */
+        blas_dgemm = f;
+        return 0;
+}
+
+int oracle_blas_active(void) { return blas_dgemm != NULL; }
+
+/* src/sparseblocks.c:133-158: C_l = beta C_l + alpha op(A_l) op(B_l), l < L */
 void oracle_dgemm_batched(int ta, int tb, int M, int N, int K, int L, double alpha,
                           const double* A, int lda, int64_t sA, const double* B, int ldb,
                           int64_t sB, double beta, double* C, int ldc, int64_t sC)
 {
+        /* BLAS only where its argument rules hold (leading dimensions at
+         * least the stored rows); it would refuse the call otherwise */
+        if (blas_dgemm && M > 0 && N > 0 && K > 0 && lda >= (ta ? K : M) &&
+            ldb >= (tb ? N : K) && ldc >= M) {
+                for (int l = 0; l < L; ++l)
+                        blas_dgemm(102, ta ? 112 : 111, tb ? 112 : 111, M, N, K, alpha, A + l * sA,
+                                   lda, B + l * sB, ldb, beta, C + l * sC, ldc);
+                return;
+        }
         for (int l = 0; l < L; ++l) {
                 const double* Al = A + l * sA;
                 const double* Bl = B + l * sB;
