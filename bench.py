@@ -34,7 +34,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+# matrix-core peaks of the arithmetic each path runs (the BF16 path stores
+# operators in BF16 and computes on FP32 matrix cores)
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 157.3}
 
 
 def profiled_traffic(workload, sites, D):
@@ -74,7 +76,7 @@ def parse():
     ap.add_argument("--bond-dim", type=int, default=None, help="default: the workload's")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the baseline sample (0 disables it)")
-    ap.add_argument("--precisions", default="f64,f32")
+    ap.add_argument("--precisions", default="f64,f32,bf16")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the renormalisation and site-tensor kernels")
     ap.add_argument("--no-solve", action="store_true",
@@ -245,7 +247,7 @@ def row_problems(step, bond_dim, sites):
     return out, ctx
 
 
-def measure_rows(step, sites, bond_dim, cpu_dim=128, same_size_flops=4e10):
+def measure_rows(step, sites, bond_dim, cpu_dim=128, same_size_flops=1e10):
     """The renormalisation and site-tensor rows on the benchmark object:
     every device call runs twice and the second is reported (kernel time
     from device events, algorithmic flops and bytes).  The CPU baseline of a
@@ -292,6 +294,25 @@ def measure_rows(step, sites, bond_dim, cpu_dim=128, same_size_flops=4e10):
     return rows
 
 
+def _time_matvec(h, steps=3):
+    """ms per device-resident matvec of a Heff handle (HIP events on the
+    launch stream, one warm-up call first)."""
+    import torch
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    dx = torch.from_numpy(np.random.default_rng(7).standard_normal(h.size)).to(dev)
+    dy = torch.empty_like(dx)
+    torch.cuda.synchronize()
+    h.matvec_device(dx.data_ptr(), dy.data_ptr(), stream.cuda_stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        h.matvec_device(dx.data_ptr(), dy.data_ptr(), stream.cuda_stream)
+    e1.record(stream)
+    stream.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
 def measure_solve(ctx, step, tol):
     """Lowest eigenpair of the step's effective Hamiltonian built from the
     Hamiltonian's own operators: every bond's operators renormalised on the
@@ -309,6 +330,13 @@ def measure_solve(ctx, step, tol):
     h32 = T.Heff(d, o, s, T.FP32, keepalive=(step, env))
     print("# solve: operators in %.1f s, flops/matvec %.4g" % (t_env, h64.stats()["flops"]),
           file=sys.stderr, flush=True)
+    fl = h64.stats()["flops"]
+    rates = {}
+    for name, h in (("f64", h64), ("f32", h32)):
+        ms = _time_matvec(h)
+        rates[name] = {"ms_per_matvec": ms, "tflops": fl / (ms * 1e-3) / 1e12,
+                       "roofline_frac": fl / (ms * 1e-3) / 1e12 / PEAK_TFLOPS[name]}
+    print("# solve matvec rates %s" % rates, file=sys.stderr, flush=True)
     guess = np.random.default_rng(2).standard_normal(h64.size)
     diag = h64.diagonal()
     t0 = time.perf_counter()
@@ -323,7 +351,7 @@ def measure_solve(ctx, step, tol):
           file=sys.stderr, flush=True)
     return {"operators": "device renormalisation chain from the vacuum, seeded site tensors "
                          "with unit rows", "operators_s": t_env,
-            "flops_per_matvec": h64.stats()["flops"], "tol": tol,
+            "flops_per_matvec": h64.stats()["flops"], "tol": tol, "matvec": rates,
             "fp64": {"energy": e64, "its": its, "status": rc, "seconds": t64},
             "mixed": {"energy": em, "its_f32": its_lo, "its_f64": its_hi, "status": rcm,
                       "seconds": tm},
@@ -486,7 +514,7 @@ def main():
     heffs = {}
     for prec in args.precisions.split(","):
         t0 = time.perf_counter()
-        h = T.Heff.from_step(step, T.FP64 if prec == "f64" else T.FP32)
+        h = T.Heff.from_step(step, {"f64": T.FP64, "f32": T.FP32, "bf16": T.BF16}[prec])
         t_init = time.perf_counter() - t0
         heffs[prec] = h
         print("# %s: Heff init %.1f s" % (prec, t_init), file=sys.stderr, flush=True)
@@ -556,6 +584,16 @@ def main():
                         "speedup_vs_f64": results["f32"]["matvec_per_s"] / r["matvec_per_s"],
                         "rel_err_vs_f64": err,
                         "roofline_frac": results["f32"]["tflops"] / PEAK_TFLOPS["f32"]}
+    if "bf16" in results and "f64" in results:
+        err = float(np.linalg.norm(ys["bf16"] - ys["f64"]) / np.linalg.norm(ys["f64"]))
+        out.setdefault("mixed", {})["bf16"] = {
+            "storage": "bf16 operator blocks, per-block power-of-two scales",
+            "arithmetic": "f32 matrix cores, f32 accumulation",
+            "matvec_per_s": results["bf16"]["matvec_per_s"],
+            "ms_per_matvec": results["bf16"]["ms_per_matvec"],
+            "speedup_vs_f64": results["bf16"]["matvec_per_s"] / r["matvec_per_s"],
+            "rel_err_vs_f64": err,
+            "roofline_frac": results["bf16"]["tflops"] / PEAK_TFLOPS["bf16"]}
     if not args.no_solve:
         out["solve"] = measure_solve(ctx, step, args.solve_tol)
     if not args.no_sweep:

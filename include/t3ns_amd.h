@@ -39,7 +39,10 @@ enum t3ns_status {
 
 enum t3ns_precision {
     T3NS_FP64 = 0, /* plain path: FP64 storage and arithmetic */
-    T3NS_FP32 = 1  /* FP32 storage and arithmetic for the bulk matvec */
+    T3NS_FP32 = 1, /* FP32 storage and arithmetic for the bulk matvec */
+    T3NS_BF16 = 2  /* BF16 operator storage, each operator block scaled by a
+                      power of two; FP32 matrix-core arithmetic and
+                      accumulation, FP32 stage intermediates */
 };
 
 /* Symmetry sectors of one bond: struct symsecs, include/symsecs.h:32-54. */

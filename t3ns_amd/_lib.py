@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libt3ns_amd.so")
 
 MAX_SYMMETRIES = 5
-FP64, FP32 = 0, 1
+FP64, FP32, BF16 = 0, 1, 2
 ERRORS = {0: "ok", -1: "invalid argument", -2: "device error", -3: "out of memory", -4: "solver failure"}
 
 # symmetry group enum (include/symmetries.h:34)

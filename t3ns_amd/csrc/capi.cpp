@@ -212,7 +212,7 @@ int t3ns_device_count(void) {
 int t3ns_heff_init(t3ns_heff** out, const t3ns_heff_desc* d, const t3ns_rops_desc* ops,
                    const t3ns_site_desc* site, int precision) {
     if (!out || !d || !ops || !site) return fail(T3NS_ERR_INVALID, "null argument");
-    if (precision != T3NS_FP64 && precision != T3NS_FP32)
+    if (precision != T3NS_FP64 && precision != T3NS_FP32 && precision != T3NS_BF16)
         return fail(T3NS_ERR_INVALID, "unknown precision");
     return guarded([&] {
         if (t3ns_device_count() < 1) throw std::runtime_error("HIP error: no device visible");

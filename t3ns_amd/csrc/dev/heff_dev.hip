@@ -46,6 +46,13 @@ struct Mfma<float> {
     static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
+// Operator values as the arithmetic type: FP64 and FP32 arenas as they
+// are, BF16 arenas (the upper 16 bits of an FP32 value, per-block scaled)
+// widened to FP32.
+__device__ __forceinline__ double cvt(double v) { return v; }
+__device__ __forceinline__ float cvt(float v) { return v; }
+__device__ __forceinline__ float cvt(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
+
 // Geometry of a mode-p product of a three-leg block S (memory dims s0, s1,
 // s2; first fastest) with a column-major (dn x s_p) operator block O:
 //     Z[f, k'] = sum_k S[f, k] O[k', k],   f = (i_lo, i_hi) a fiber,
@@ -130,8 +137,8 @@ __device__ __forceinline__ RowLane row_lane(const Geo& g, int tf, int tn0) {
 
 // One slab of at most NU k-steps (ns of them live, ns >= 1) of row_mac_n:
 // loads first, then the matrix-core steps.
-template <typename T, int NTN, int NU, bool SCALE>
-__device__ __forceinline__ void row_slab(const T* sp, const T* op, const int (&coff)[NTN], int sst,
+template <typename T, typename OT, int NTN, int NU, bool SCALE>
+__device__ __forceinline__ void row_slab(const T* sp, const OT* op, const int (&coff)[NTN], int sst,
                                          int ost, int ns, T scale,
                                          typename Mfma<T>::V (&acc)[kTN]) {
     T av[NU];
@@ -141,7 +148,7 @@ __device__ __forceinline__ void row_slab(const T* sp, const T* op, const int (&c
         const int uu = min(u, ns - 1);
         av[u] = sp[uu * sst];
 #pragma unroll
-        for (int j = 0; j < NTN; ++j) bv[u][j] = op[uu * ost + coff[j]];
+        for (int j = 0; j < NTN; ++j) bv[u][j] = cvt(op[uu * ost + coff[j]]);
     }
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
@@ -158,14 +165,14 @@ __device__ __forceinline__ void row_slab(const T* sp, const T* op, const int (&c
 // j < NTN.  Loads are unconditional (clamped indices stay inside the
 // blocks) so the compiler can keep several in flight; only the K tail is
 // masked.
-template <typename T, int NTN, bool SCALE>
-__device__ __forceinline__ void row_mac_n(const T* __restrict__ S, const T* __restrict__ O,
+template <typename T, typename OT, int NTN, bool SCALE>
+__device__ __forceinline__ void row_mac_n(const T* __restrict__ S, const OT* __restrict__ O,
                                           const Geo& g, const RowLane& L, T scale,
                                           typename Mfma<T>::V (&acc)[kTN]) {
     const int lane = threadIdx.x & 63;
     const int i = lane & 15, kk = lane >> 4;
     const T* sp = S + L.soff;
-    const T* op = O + L.ooff;
+    const OT* op = O + L.ooff;
     const int sst = 4 * g.lo, ost = 4 * g.dn;
     const int n0 = min(L.tn0 * 16 + i, g.dn - 1);
     int coff[NTN];
@@ -179,39 +186,39 @@ __device__ __forceinline__ void row_mac_n(const T* __restrict__ S, const T* __re
     constexpr int KS = 4;
     int s0 = 0;
     for (; s0 + KS <= kfull; s0 += KS) {
-        row_slab<T, NTN, KS, SCALE>(sp, op, coff, sst, ost, KS, scale, acc);
+        row_slab<T, OT, NTN, KS, SCALE>(sp, op, coff, sst, ost, KS, scale, acc);
         sp += KS * sst;
         op += KS * ost;
     }
     if (s0 < kfull) {
         const int ns = uni(kfull - s0);
-        row_slab<T, NTN, KS - 1, SCALE>(sp, op, coff, sst, ost, ns, scale, acc);
+        row_slab<T, OT, NTN, KS - 1, SCALE>(sp, op, coff, sst, ost, ns, scale, acc);
         sp += ns * sst;
         op += ns * ost;
     }
     if (g.K & 3) {  // K tail: lanes past K contribute zero (their loads are skipped)
         const bool kv = (kfull << 2) + kk < g.K;
         const T* spc = kv ? sp : S + L.soff;
-        const T* opc = kv ? op : O + (L.ooff - kk * g.dn);  // row 0 stays inside the block
+        const OT* opc = kv ? op : O + (L.ooff - kk * g.dn);  // row 0 stays inside the block
         T av = kv ? spc[0] : (T)0;
         if (SCALE) av *= scale;
 #pragma unroll
         for (int j = 0; j < NTN; ++j) {
-            const T bv = opc[coff[j]];
+            const T bv = cvt(opc[coff[j]]);
             acc[j] = Mfma<T>::run(av, bv, acc[j]);
         }
     }
 }
 
-template <typename T, bool SCALE>
-__device__ __forceinline__ void row_mac(const T* __restrict__ S, const T* __restrict__ O,
+template <typename T, typename OT, bool SCALE>
+__device__ __forceinline__ void row_mac(const T* __restrict__ S, const OT* __restrict__ O,
                                         const Geo& g, const RowLane& L, T scale,
                                         typename Mfma<T>::V (&acc)[kTN]) {
     switch (__builtin_amdgcn_readfirstlane(L.ntn)) {
-        case 1: row_mac_n<T, 1, SCALE>(S, O, g, L, scale, acc); break;
-        case 2: row_mac_n<T, 2, SCALE>(S, O, g, L, scale, acc); break;
-        case 3: row_mac_n<T, 3, SCALE>(S, O, g, L, scale, acc); break;
-        default: row_mac_n<T, 4, SCALE>(S, O, g, L, scale, acc); break;
+        case 1: row_mac_n<T, OT, 1, SCALE>(S, O, g, L, scale, acc); break;
+        case 2: row_mac_n<T, OT, 2, SCALE>(S, O, g, L, scale, acc); break;
+        case 3: row_mac_n<T, OT, 3, SCALE>(S, O, g, L, scale, acc); break;
+        default: row_mac_n<T, OT, 4, SCALE>(S, O, g, L, scale, acc); break;
     }
 }
 
@@ -271,8 +278,8 @@ __device__ __forceinline__ RowBlk row_blk(const Geo& g, int tf0, int tn) {
 }
 
 // One slab of at most NU k-steps (ns of them live, ns >= 1) of blk_mac_n.
-template <typename T, int NTF, int NU, bool SCALE>
-__device__ __forceinline__ void blk_slab(const T* S, const T* op, const RowBlk& B, int so, int sst,
+template <typename T, typename OT, int NTF, int NU, bool SCALE>
+__device__ __forceinline__ void blk_slab(const T* S, const OT* op, const RowBlk& B, int so, int sst,
                                          int ost, int ns, T scale,
                                          typename Mfma<T>::V (&acc)[kTF]) {
     T bv[NU];
@@ -280,7 +287,7 @@ __device__ __forceinline__ void blk_slab(const T* S, const T* op, const RowBlk& 
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
         const int uu = min(u, ns - 1);
-        bv[u] = op[uu * ost];
+        bv[u] = cvt(op[uu * ost]);
 #pragma unroll
         for (int r = 0; r < NTF; ++r) av[u][r] = S[B.soff[r] + so + uu * sst];
     }
@@ -298,31 +305,31 @@ __device__ __forceinline__ void blk_slab(const T* S, const T* op, const RowBlk& 
 }
 
 // acc[r] += (scale S)[row tile tf0 + r, :] O[column tile, :]^T, r < NTF.
-template <typename T, int NTF, bool SCALE>
-__device__ __forceinline__ void blk_mac_n(const T* __restrict__ S, const T* __restrict__ O,
+template <typename T, typename OT, int NTF, bool SCALE>
+__device__ __forceinline__ void blk_mac_n(const T* __restrict__ S, const OT* __restrict__ O,
                                           const Geo& g, const RowBlk& B, T scale,
                                           typename Mfma<T>::V (&acc)[kTF]) {
     const int kk = (threadIdx.x & 63) >> 4;
     const int sst = 4 * g.lo, ost = 4 * g.dn;
-    const T* op = O + B.ooff;
+    const OT* op = O + B.ooff;
     int so = 0;
     const int kfull = uni(g.K >> 2);
     constexpr int KS = 4;
     int s0 = 0;
     for (; s0 + KS <= kfull; s0 += KS) {
-        blk_slab<T, NTF, KS, SCALE>(S, op, B, so, sst, ost, KS, scale, acc);
+        blk_slab<T, OT, NTF, KS, SCALE>(S, op, B, so, sst, ost, KS, scale, acc);
         so += KS * sst;
         op += KS * ost;
     }
     if (s0 < kfull) {
         const int ns = uni(kfull - s0);
-        blk_slab<T, NTF, KS - 1, SCALE>(S, op, B, so, sst, ost, ns, scale, acc);
+        blk_slab<T, OT, NTF, KS - 1, SCALE>(S, op, B, so, sst, ost, ns, scale, acc);
         so += ns * sst;
         op += ns * ost;
     }
     if (g.K & 3) {  // K tail: lanes past K contribute zero
         const bool kv = (kfull << 2) + kk < g.K;
-        const T bv = kv ? op[0] : O[B.ooff - kk * g.dn];  // row 0 stays inside the block
+        const T bv = cvt(kv ? op[0] : O[B.ooff - kk * g.dn]);  // row 0 stays inside the block
 #pragma unroll
         for (int r = 0; r < NTF; ++r) {
             T a = kv ? S[B.soff[r] + so] : (T)0;
@@ -332,15 +339,15 @@ __device__ __forceinline__ void blk_mac_n(const T* __restrict__ S, const T* __re
     }
 }
 
-template <typename T, bool SCALE>
-__device__ __forceinline__ void blk_mac(const T* __restrict__ S, const T* __restrict__ O,
+template <typename T, typename OT, bool SCALE>
+__device__ __forceinline__ void blk_mac(const T* __restrict__ S, const OT* __restrict__ O,
                                         const Geo& g, const RowBlk& B, T scale,
                                         typename Mfma<T>::V (&acc)[kTF]) {
     switch (__builtin_amdgcn_readfirstlane(B.ntf)) {
-        case 1: blk_mac_n<T, 1, SCALE>(S, O, g, B, scale, acc); break;
-        case 2: blk_mac_n<T, 2, SCALE>(S, O, g, B, scale, acc); break;
-        case 3: blk_mac_n<T, 3, SCALE>(S, O, g, B, scale, acc); break;
-        default: blk_mac_n<T, 4, SCALE>(S, O, g, B, scale, acc); break;
+        case 1: blk_mac_n<T, OT, 1, SCALE>(S, O, g, B, scale, acc); break;
+        case 2: blk_mac_n<T, OT, 2, SCALE>(S, O, g, B, scale, acc); break;
+        case 3: blk_mac_n<T, OT, 3, SCALE>(S, O, g, B, scale, acc); break;
+        default: blk_mac_n<T, OT, 4, SCALE>(S, O, g, B, scale, acc); break;
     }
 }
 
@@ -380,16 +387,16 @@ __device__ __forceinline__ int64_t uni64(int64_t v) {
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
-template <typename T>
+template <typename T, typename OT>
 struct Terms {
-    const T* s[kTU];  // stage inputs
-    const T* o[kTU];  // operator blocks
+    const T* s[kTU];   // stage inputs
+    const OT* o[kTU];  // operator blocks
     T scale[kTU];
     int n;  // live terms, >= 1
 };
 
-template <typename T, int NTN, bool SCALE>
-__device__ __forceinline__ void row_mac_terms_n(const Terms<T>& R, const Geo& g, const RowLane& L,
+template <typename T, typename OT, int NTN, bool SCALE>
+__device__ __forceinline__ void row_mac_terms_n(const Terms<T, OT>& R, const Geo& g, const RowLane& L,
                                                 typename Mfma<T>::V (&acc)[kTN]) {
     const int lane = threadIdx.x & 63;
     const int i = lane & 15, kk = lane >> 4;
@@ -409,7 +416,7 @@ __device__ __forceinline__ void row_mac_terms_n(const Terms<T>& R, const Geo& g,
         for (int u = 0; u < kTU; ++u) {
             av[u] = R.s[u][ao];
 #pragma unroll
-            for (int j = 0; j < NTN; ++j) bv[u][j] = R.o[u][bo + coff[j]];
+            for (int j = 0; j < NTN; ++j) bv[u][j] = cvt(R.o[u][bo + coff[j]]);
         }
 #pragma unroll
         for (int u = 0; u < kTU; ++u) {
@@ -423,19 +430,19 @@ __device__ __forceinline__ void row_mac_terms_n(const Terms<T>& R, const Geo& g,
     }
 }
 
-template <typename T, bool SCALE>
-__device__ __forceinline__ void row_mac_terms(const Terms<T>& R, const Geo& g, const RowLane& L,
+template <typename T, typename OT, bool SCALE>
+__device__ __forceinline__ void row_mac_terms(const Terms<T, OT>& R, const Geo& g, const RowLane& L,
                                               typename Mfma<T>::V (&acc)[kTN]) {
     switch (__builtin_amdgcn_readfirstlane(L.ntn)) {
-        case 1: row_mac_terms_n<T, 1, SCALE>(R, g, L, acc); break;
-        case 2: row_mac_terms_n<T, 2, SCALE>(R, g, L, acc); break;
-        case 3: row_mac_terms_n<T, 3, SCALE>(R, g, L, acc); break;
-        default: row_mac_terms_n<T, 4, SCALE>(R, g, L, acc); break;
+        case 1: row_mac_terms_n<T, OT, 1, SCALE>(R, g, L, acc); break;
+        case 2: row_mac_terms_n<T, OT, 2, SCALE>(R, g, L, acc); break;
+        case 3: row_mac_terms_n<T, OT, 3, SCALE>(R, g, L, acc); break;
+        default: row_mac_terms_n<T, OT, 4, SCALE>(R, g, L, acc); break;
     }
 }
 
-template <typename T, int NTF, bool SCALE>
-__device__ __forceinline__ void blk_mac_terms_n(const Terms<T>& R, const Geo& g, const RowBlk& B,
+template <typename T, typename OT, int NTF, bool SCALE>
+__device__ __forceinline__ void blk_mac_terms_n(const Terms<T, OT>& R, const Geo& g, const RowBlk& B,
                                                 typename Mfma<T>::V (&acc)[kTF]) {
     const int kk = (threadIdx.x & 63) >> 4;
     const int nks = uni(g.ksteps), nu = uni(R.n);
@@ -447,7 +454,7 @@ __device__ __forceinline__ void blk_mac_terms_n(const Terms<T>& R, const Geo& g,
         T av[kTU][NTF];
 #pragma unroll
         for (int u = 0; u < kTU; ++u) {
-            bv[u] = R.o[u][bo];
+            bv[u] = cvt(R.o[u][bo]);
 #pragma unroll
             for (int r = 0; r < NTF; ++r) av[u][r] = R.s[u][B.soff[r] + so];
         }
@@ -465,14 +472,14 @@ __device__ __forceinline__ void blk_mac_terms_n(const Terms<T>& R, const Geo& g,
     }
 }
 
-template <typename T, bool SCALE>
-__device__ __forceinline__ void blk_mac_terms(const Terms<T>& R, const Geo& g, const RowBlk& B,
+template <typename T, typename OT, bool SCALE>
+__device__ __forceinline__ void blk_mac_terms(const Terms<T, OT>& R, const Geo& g, const RowBlk& B,
                                               typename Mfma<T>::V (&acc)[kTF]) {
     switch (__builtin_amdgcn_readfirstlane(B.ntf)) {
-        case 1: blk_mac_terms_n<T, 1, SCALE>(R, g, B, acc); break;
-        case 2: blk_mac_terms_n<T, 2, SCALE>(R, g, B, acc); break;
-        case 3: blk_mac_terms_n<T, 3, SCALE>(R, g, B, acc); break;
-        default: blk_mac_terms_n<T, 4, SCALE>(R, g, B, acc); break;
+        case 1: blk_mac_terms_n<T, OT, 1, SCALE>(R, g, B, acc); break;
+        case 2: blk_mac_terms_n<T, OT, 2, SCALE>(R, g, B, acc); break;
+        case 3: blk_mac_terms_n<T, OT, 3, SCALE>(R, g, B, acc); break;
+        default: blk_mac_terms_n<T, OT, 4, SCALE>(R, g, B, acc); break;
     }
 }
 
@@ -509,8 +516,8 @@ __device__ __forceinline__ VTasks vtasks(const Geo& g, int nw) {
 }
 
 // acc[n] += sum_k (scale S)[f, k] O[n, k], n < NB = dn (O is dn x K, column-major)
-template <typename T, int NB, bool SCALE>
-__device__ __forceinline__ void vmac_n(const T* __restrict__ S, const T* __restrict__ O,
+template <typename T, typename OT, int NB, bool SCALE>
+__device__ __forceinline__ void vmac_n(const T* __restrict__ S, const OT* __restrict__ O,
                                        const Geo& g, const VLane& L, T scale,
                                        typename Mfma<T>::V& acc) {
     const T* sp = S + L.soff;
@@ -524,25 +531,25 @@ __device__ __forceinline__ void vmac_n(const T* __restrict__ S, const T* __restr
         for (int u = 0; u < 4; ++u) {
             const T av = SCALE ? a[u] * scale : a[u];
 #pragma unroll
-            for (int n = 0; n < NB; ++n) acc[n] = fma(av, O[(k + u) * NB + n], acc[n]);
+            for (int n = 0; n < NB; ++n) acc[n] = fma(av, (T)cvt(O[(k + u) * NB + n]), acc[n]);
         }
     }
     for (; k < K; ++k) {
         const T av = SCALE ? sp[k * lo] * scale : sp[k * lo];
 #pragma unroll
-        for (int n = 0; n < NB; ++n) acc[n] = fma(av, O[k * NB + n], acc[n]);
+        for (int n = 0; n < NB; ++n) acc[n] = fma(av, (T)cvt(O[k * NB + n]), acc[n]);
     }
 }
 
-template <typename T, bool SCALE>
-__device__ __forceinline__ void vmac(const T* __restrict__ S, const T* __restrict__ O,
+template <typename T, typename OT, bool SCALE>
+__device__ __forceinline__ void vmac(const T* __restrict__ S, const OT* __restrict__ O,
                                      const Geo& g, const VLane& L, T scale,
                                      typename Mfma<T>::V& acc) {
     switch (uni(g.dn)) {
-        case 1: vmac_n<T, 1, SCALE>(S, O, g, L, scale, acc); break;
-        case 2: vmac_n<T, 2, SCALE>(S, O, g, L, scale, acc); break;
-        case 3: vmac_n<T, 3, SCALE>(S, O, g, L, scale, acc); break;
-        default: vmac_n<T, 4, SCALE>(S, O, g, L, scale, acc); break;
+        case 1: vmac_n<T, OT, 1, SCALE>(S, O, g, L, scale, acc); break;
+        case 2: vmac_n<T, OT, 2, SCALE>(S, O, g, L, scale, acc); break;
+        case 3: vmac_n<T, OT, 3, SCALE>(S, O, g, L, scale, acc); break;
+        default: vmac_n<T, OT, 4, SCALE>(S, O, g, L, scale, acc); break;
     }
 }
 
@@ -611,10 +618,11 @@ template <typename T>
 constexpr bool kGroupB = sizeof(T) == 4;
 
 // Terms t0, t0 + G, ... (at most kTU of them) of a chunk as stage C inputs.
-template <typename T>
-__device__ __forceinline__ Terms<T> c_terms(const ChunkRec* rec, int t0, int G, int nt, int isdmrg,
-                                            const T* W1, int w1, const T* W2, int w2, const T* opc) {
-    Terms<T> R;
+template <typename T, typename OT>
+__device__ __forceinline__ Terms<T, OT> c_terms(const ChunkRec* rec, int t0, int G, int nt,
+                                                int isdmrg, const T* W1, int w1, const T* W2,
+                                                int w2, const OT* opc) {
+    Terms<T, OT> R;
     R.n = uni(min(kTU, (nt - t0 + G - 1) / G));
 #pragma unroll
     for (int u = 0; u < kTU; ++u) {
@@ -640,13 +648,13 @@ __device__ __forceinline__ Terms<T> c_terms(const ChunkRec* rec, int t0, int G, 
 // and walks the items of its group; when a geometry has fewer tiles than
 // waves, items are split over groups (stage C partial tiles are then
 // reduced in a fixed order).
-template <typename T, int NT>
+template <typename T, typename OT, int NT>
 __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                                             const DevChunk* __restrict__ chunks,
                                             const int* __restrict__ runs,
                                             const DevTerm* __restrict__ terms,
-                                            const T* __restrict__ op0, const T* __restrict__ op1,
-                                            const T* __restrict__ op2, const T* X, T* W1, T* Y,
+                                            const OT* __restrict__ op0, const OT* __restrict__ op1,
+                                            const OT* __restrict__ op2, const T* X, T* W1, T* Y,
                                             T* part, ChunkRec* recs, int isdmrg, const int* map,
                                             const int* pos) {
     typedef typename Mfma<T>::V V;
@@ -740,9 +748,9 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             vput<T>(Y, gC, L, acc, true);
         }
     };
-    const T* opa = a == 0 ? op0 : (a == 1 ? op1 : op2);
-    const T* opbb = b == 0 ? op0 : (b == 1 ? op1 : op2);
-    const T* opc = c == 0 ? op0 : (c == 1 ? op1 : op2);
+    const OT* opa = a == 0 ? op0 : (a == 1 ? op1 : op2);
+    const OT* opbb = b == 0 ? op0 : (b == 1 ? op1 : op2);
+    const OT* opc = c == 0 ? op0 : (c == 1 ? op1 : op2);
     const int cb = uni(P.chunk_begin), ce = uni(P.chunk_end);
     const int w1 = uni(P.w1), w2 = uni(P.w2);
     for (int ci = cb; ci < ce; ++ci) {
@@ -774,7 +782,7 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                 const VLane L = vlane(gA, task % tA.nft);
                 for (int j = task / tA.nft; j < nr; j += tA.G) {
                     V acc = V{0, 0, 0, 0};
-                    vmac<T, false>(X, opa + uni64(rec->opa[j]), gA, L, (T)1, acc);
+                    vmac<T, OT, false>(X, opa + uni64(rec->opa[j]), gA, L, (T)1, acc);
                     vput<T>(W1 + j * w1, gA, L, acc, false);
                 }
             }
@@ -788,7 +796,7 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                     V acc[kTF];
 #pragma unroll
                     for (int q = 0; q < kTF; ++q) acc[q] = V{0, 0, 0, 0};
-                    blk_mac<T, false>(X, opa + rec->opa[j], gA, L, (T)1, acc);
+                    blk_mac<T, OT, false>(X, opa + rec->opa[j], gA, L, (T)1, acc);
                     blk_put<T>(W1 + j * w1, gA, L, acc, false);
                 }
             }
@@ -802,7 +810,7 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                     V acc[kTN];
 #pragma unroll
                     for (int q = 0; q < kTN; ++q) acc[q] = V{0, 0, 0, 0};
-                    row_mac<T, false>(X, opa + rec->opa[j], gA, L, (T)1, acc);
+                    row_mac<T, OT, false>(X, opa + rec->opa[j], gA, L, (T)1, acc);
                     row_put<T>(W1 + j * w1, gA, L, acc, false);
                 }
             }
@@ -815,10 +823,10 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                     V acc = V{0, 0, 0, 0};
                     if constexpr (kGroupB<T>) {
                         for (int t = uni(rec->cbeg[g]); t < uni(rec->cbeg[g + 1]); ++t)
-                            vmac<T, true>(W1 + uni(rec->slot[t]) * w1, opbb + uni64(rec->opb[t]), gB,
+                            vmac<T, OT, true>(W1 + uni(rec->slot[t]) * w1, opbb + uni64(rec->opb[t]), gB,
                                           L, (T)rec->alpha[t], acc);
                     } else {  // one term per group
-                        vmac<T, true>(W1 + uni(rec->slot[g]) * w1, opbb + uni64(rec->opb[g]), gB, L,
+                        vmac<T, OT, true>(W1 + uni(rec->slot[g]) * w1, opbb + uni64(rec->opb[g]), gB, L,
                                       (T)rec->alpha[g], acc);
                     }
                     vput<T>(W2 + g * w2, gB, L, acc, false);
@@ -836,10 +844,10 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                     for (int q = 0; q < kTF; ++q) acc[q] = V{0, 0, 0, 0};
                     if constexpr (kGroupB<T>) {
                         for (int t = uni(rec->cbeg[g]); t < uni(rec->cbeg[g + 1]); ++t)
-                            blk_mac<T, true>(W1 + rec->slot[t] * w1, opbb + rec->opb[t], gB, L,
+                            blk_mac<T, OT, true>(W1 + rec->slot[t] * w1, opbb + rec->opb[t], gB, L,
                                              (T)rec->alpha[t], acc);
                     } else {  // one term per group
-                        blk_mac<T, true>(W1 + rec->slot[g] * w1, opbb + rec->opb[g], gB, L,
+                        blk_mac<T, OT, true>(W1 + rec->slot[g] * w1, opbb + rec->opb[g], gB, L,
                                          (T)rec->alpha[g], acc);
                     }
                     blk_put<T>(W2 + g * w2, gB, L, acc, false);
@@ -857,10 +865,10 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                     for (int q = 0; q < kTN; ++q) acc[q] = V{0, 0, 0, 0};
                     if constexpr (kGroupB<T>) {
                         for (int t = uni(rec->cbeg[g]); t < uni(rec->cbeg[g + 1]); ++t)
-                            row_mac<T, true>(W1 + rec->slot[t] * w1, opbb + rec->opb[t], gB, L,
+                            row_mac<T, OT, true>(W1 + rec->slot[t] * w1, opbb + rec->opb[t], gB, L,
                                              (T)rec->alpha[t], acc);
                     } else {  // one term per group
-                        row_mac<T, true>(W1 + rec->slot[g] * w1, opbb + rec->opb[g], gB, L,
+                        row_mac<T, OT, true>(W1 + rec->slot[g] * w1, opbb + rec->opb[g], gB, L,
                                          (T)rec->alpha[g], acc);
                     }
                     row_put<T>(W2 + g * w2, gB, L, acc, false);
@@ -895,10 +903,10 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
         if (vC) {  // stage C, vector unit
             auto cmac = [&](int t, const VLane& L, V& acc) {
                 if (isdmrg)
-                    vmac<T, true>(W1 + uni(rec->slot[t]) * w1, opc + uni64(rec->opc[t]), gC, L,
+                    vmac<T, OT, true>(W1 + uni(rec->slot[t]) * w1, opc + uni64(rec->opc[t]), gC, L,
                                   (T)rec->alpha[t], acc);
                 else
-                    vmac<T, false>(W2 + t * w2, opc + uni64(rec->opc[t]), gC, L, (T)1, acc);
+                    vmac<T, OT, false>(W2 + t * w2, opc + uni64(rec->opc[t]), gC, L, (T)1, acc);
             };
             if (pCv) {
                 if (wave < tC.total)
@@ -922,18 +930,18 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
 #pragma unroll
                 for (int q = 0; q < kTF; ++q) acc[q] = V{0, 0, 0, 0};
                 for (int t0 = 0; t0 < ncs; t0 += kTU) {
-                    const Terms<T> R = c_terms<T>(rec, t0, 1, ncs, isdmrg, W1, w1, W2, w2, opc);
-                    if (isdmrg) blk_mac_terms<T, true>(R, gC, L, acc);
-                    else blk_mac_terms<T, false>(R, gC, L, acc);
+                    const Terms<T, OT> R = c_terms<T, OT>(rec, t0, 1, ncs, isdmrg, W1, w1, W2, w2, opc);
+                    if (isdmrg) blk_mac_terms<T, OT, true>(R, gC, L, acc);
+                    else blk_mac_terms<T, OT, false>(R, gC, L, acc);
                 }
                 blk_put<T>(Y, gC, L, acc, true);
             }
         } else if (pC) {  // stage C into the pair's persistent accumulators
             if (wave < kC.total) {
                 for (int t0 = grpC; t0 < ncs; t0 += kC.G * kTU) {
-                    const Terms<T> R = c_terms<T>(rec, t0, kC.G, ncs, isdmrg, W1, w1, W2, w2, opc);
-                    if (isdmrg) row_mac_terms<T, true>(R, gC, LC, accC);
-                    else row_mac_terms<T, false>(R, gC, LC, accC);
+                    const Terms<T, OT> R = c_terms<T, OT>(rec, t0, kC.G, ncs, isdmrg, W1, w1, W2, w2, opc);
+                    if (isdmrg) row_mac_terms<T, OT, true>(R, gC, LC, accC);
+                    else row_mac_terms<T, OT, false>(R, gC, LC, accC);
                 }
             }
         } else {  // stage C: Y += sum_t O_c[t] x_c W2[t]  (DMRG: alpha_t O_c[t] x_c W1[slot t])
@@ -944,9 +952,9 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
 #pragma unroll
                 for (int q = 0; q < kTN; ++q) acc[q] = V{0, 0, 0, 0};
                 for (int t0 = grp; t0 < ncs; t0 += kC.G * kTU) {
-                    const Terms<T> R = c_terms<T>(rec, t0, kC.G, ncs, isdmrg, W1, w1, W2, w2, opc);
-                    if (isdmrg) row_mac_terms<T, true>(R, gC, L, acc);
-                    else row_mac_terms<T, false>(R, gC, L, acc);
+                    const Terms<T, OT> R = c_terms<T, OT>(rec, t0, kC.G, ncs, isdmrg, W1, w1, W2, w2, opc);
+                    if (isdmrg) row_mac_terms<T, OT, true>(R, gC, L, acc);
+                    else row_mac_terms<T, OT, false>(R, gC, L, acc);
                 }
                 if (kC.G == 1) row_put<T>(Y, gC, L, acc, true);
                 else store_part(acc, L, grp, r);
@@ -996,13 +1004,13 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
 //   kModeLdsW    input block read from global memory, stage outputs in LDS;
 //   kModeGlobal  stage outputs in the workgroup's global scratch region.
 enum { kModeLds = 0, kModeLdsW = 1, kModeGlobal = 2 };
-template <typename T, int POOL, int NT, int MODE>
+template <typename T, typename OT, int POOL, int NT, int MODE>
 __global__ void __launch_bounds__(NT)
     heff_matvec_kernel(const DevBlock* __restrict__ blocks, int nblocks,
                        const DevPair* __restrict__ pairs, const DevChunk* __restrict__ chunks,
                        const int* __restrict__ runs, const DevTerm* __restrict__ terms,
-                       const T* __restrict__ op0, const T* __restrict__ op1,
-                       const T* __restrict__ op2, const T* __restrict__ x, T* __restrict__ y,
+                       const OT* __restrict__ op0, const OT* __restrict__ op1,
+                       const OT* __restrict__ op2, const T* __restrict__ x, T* __restrict__ y,
                        T* __restrict__ scratch, int64_t scratch_stride, int* __restrict__ next,
                        T* __restrict__ partial, int isdmrg, int m0, int m1, int m2) {
     constexpr int kStage = MODE == kModeGlobal ? 0 : POOL;
@@ -1029,18 +1037,18 @@ __global__ void __launch_bounds__(NT)
         for (int pi = B.pair_begin; pi < B.pair_end; ++pi) {
             const DevPair P = pairs[pi];
             if (MODE == kModeLdsW) {
-                pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, lds, Y,
+                pair_chunks<T, OT, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, lds, Y,
                                    part, rec, isdmrg, map, pos);
             } else if (MODE == kModeGlobal) {
                 T* gw1 = scratch + (int64_t)blockIdx.x * scratch_stride;
-                pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, gw1, Y,
+                pair_chunks<T, OT, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, gw1, Y,
                                    part, rec, isdmrg, map, pos);
             } else {
                 const T* X = x + P.x_off;
                 const int xsz = P.d[0] * P.d[1] * P.d[2];
                 for (int i = threadIdx.x; i < xsz; i += NT) lds[i] = X[i];
                 __syncthreads();
-                pair_chunks<T, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, lds,
+                pair_chunks<T, OT, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, lds,
                                    lds + P.w1_off, Y, part, rec, isdmrg, map, pos);
             }
         }
@@ -1089,6 +1097,18 @@ __global__ void convert_kernel(const A* __restrict__ src, B* __restrict__ dst, i
         dst[i] = (B)src[i];
 }
 
+// dst = BF16(src * scale) with round to nearest even (the upper 16 bits of
+// the FP32 value).
+__global__ void to_bf16_kernel(const double* __restrict__ src, const float* __restrict__ scale,
+                               uint16_t* __restrict__ dst, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t u = __float_as_uint((float)(src[i] * (double)scale[i]));
+        u += 0x7fffu + ((u >> 16) & 1u);
+        dst[i] = (uint16_t)(u >> 16);
+    }
+}
+
 // Output blocks split over several work units: sum their partial copies
 // in a fixed order.
 template <typename T>
@@ -1121,8 +1141,36 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     const int nops = isdmrg_ ? 2 : 3;
     const InstructionSet& is = *in.iset;
 
-    // Flatten the plan into work lists.
-    const bool f32 = prec_ == PREC_FP32;
+    // Flatten the plan into work lists.  FP32 and BF16 share the FP32
+    // arithmetic (and its pool and term grouping).
+    const bool f32 = prec_ != PREC_FP64;
+    const bool bf16 = prec_ == PREC_BF16;
+    // BF16: every operator block is stored divided by a power of two at or
+    // above its largest magnitude; the scales of a term's three blocks go
+    // into its prefactor (exact, powers of two).
+    std::vector<std::vector<std::vector<float>>> bscale(3);
+    if (bf16) {
+        for (int k = 0; k < nops; ++k) {
+            const ROperators& op = *in.ops[k];
+            bscale[k].resize(op.nrops);
+            for (int o = 0; o < op.nrops; ++o) {
+                const int nb = (int)op.beginblock[o].size() - 1;
+                bscale[k][o].assign(std::max(nb, 0), 1.0f);
+                const double* src = (int64_t)op.tel.size() == op.total_size()
+                                        ? op.tel.data() + op.op_offset[o]
+                                        : (o < (int)op.tel_src.size() ? op.tel_src[o] : nullptr);
+                if (!src) continue;
+                for (int b = 0; b < nb; ++b) {
+                    double m = 0;
+                    for (int64_t e = op.beginblock[o][b]; e < op.beginblock[o][b + 1]; ++e)
+                        m = std::max(m, std::fabs(src[e]));
+                    int ex = 0;
+                    if (m > 0) std::frexp(m, &ex);  // m < 2^ex
+                    bscale[k][o][b] = (float)std::ldexp(1.0, ex);
+                }
+            }
+        }
+    }
     int pool = f32 ? kPool32 : kPool64;
     bool x_in_lds = true;
     // T3NS_HEFF_PLACEMENT=ldsw reads every input block from global memory,
@@ -1191,6 +1239,9 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
                     }
                     if (!ok) continue;
                     t.alpha = ins.pref * e.prefactor;
+                    if (bf16)
+                        for (int k = 0; k < nops; ++k)
+                            t.alpha *= bscale[k][ins.instr[k]][e.sbops[k]];
                     ht.push_back(t);
                 }
             }
@@ -1500,6 +1551,28 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
                 hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0,
                                    stream, ops64_[k].get(), ops32_[k].get(), n);
         }
+    }
+    if (bf16) {
+        // per-element scale of the BF16 arena: the reciprocal block scale,
+        // applied by the conversion kernel (block ranges cover the arena)
+        for (int k = 0; k < nops; ++k) {
+            const ROperators& op = *in.ops[k];
+            const int64_t n = stats_.op_elems[k];
+            std::vector<float> inv(std::max<int64_t>(n, 1), 1.0f);
+            for (int o = 0; o < op.nrops; ++o)
+                for (size_t b = 0; b + 1 < op.beginblock[o].size(); ++b)
+                    for (int64_t e = op.beginblock[o][b]; e < op.beginblock[o][b + 1]; ++e)
+                        inv[op.op_offset[o] + e] = 1.0f / bscale[k][o][b];
+            DevBuf<float> dinv;
+            dinv.upload(inv, stream);
+            ops16_[k].alloc(std::max<int64_t>(n, 1));
+            if (n)
+                hipLaunchKernelGGL(to_bf16_kernel, dim3(grid_for(n)), dim3(256), 0, stream,
+                                   ops64_[k].get(), dinv.get(), ops16_[k].get(), n);
+            T3_HIP_CHECK(hipStreamSynchronize(stream));
+        }
+    }
+    if (f32) {
         x32_.alloc(size_);
         y32_.alloc(size_);
     }
@@ -1507,7 +1580,7 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     // workgroups of the global-scratch kernel.
     ncu_ = ncu;
     nwg_glob_ = std::max(1, std::min(nglob_, ncu));
-    if (prec_ == PREC_FP32) scratch32_.alloc(nglob_ ? (size_t)nwg_glob_ * maxscratch : 1);
+    if (f32) scratch32_.alloc(nglob_ ? (size_t)nwg_glob_ * maxscratch : 1);
     else scratch64_.alloc(nglob_ ? (size_t)nwg_glob_ * maxscratch : 1);
     counter_.alloc(3);
     splits_.upload(hs, stream);
@@ -1519,8 +1592,8 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
 
 HeffDevice::~HeffDevice() = default;
 
-template <typename T, int POOL>
-void HeffDevice::launch(const T* x, T* y, const T* const* ops, T* scratch, T* part,
+template <typename T, typename OT, int POOL>
+void HeffDevice::launch(const T* x, T* y, const OT* const* ops, T* scratch, T* part,
                         hipStream_t s) {
     hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(size_)), dim3(256), 0, s, y, size_, (T)0);
     hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s, counter_.get(), (int64_t)3, 0);
@@ -1528,20 +1601,20 @@ void HeffDevice::launch(const T* x, T* y, const T* const* ops, T* scratch, T* pa
         hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(part_elems_)), dim3(256), 0, s, part,
                            part_elems_, (T)0);
     if (nfast_)
-        hipLaunchKernelGGL((heff_matvec_kernel<T, POOL, kThreads, kModeLds>),
+        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeLds>),
                            dim3(std::min(nfast_, ncu_)), dim3(kThreads), 0, s, blocks_.get(),
                            nfast_, pairs_.get(), chunks_.get(), runs_.get(), terms_.get(), ops[0],
                            ops[1], ops[2], x, y, scratch, stats_.scratch_per_wg, counter_.get(),
                            part, isdmrg_, map_[0], map_[1], map_[2]);
     if (nxg_)
-        hipLaunchKernelGGL((heff_matvec_kernel<T, POOL, kThreads, kModeLdsW>),
+        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeLdsW>),
                            dim3(std::min(nxg_, ncu_)), dim3(kThreads), 0, s,
                            blocks_.get() + nfast_, nxg_, pairs_.get(), chunks_.get(), runs_.get(),
                            terms_.get(), ops[0], ops[1], ops[2], x, y, scratch,
                            stats_.scratch_per_wg, counter_.get() + 1, part, isdmrg_, map_[0],
                            map_[1], map_[2]);
     if (nglob_)
-        hipLaunchKernelGGL((heff_matvec_kernel<T, POOL, kThreads, kModeGlobal>),
+        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeGlobal>),
                            dim3(nwg_glob_), dim3(kThreads), 0, s, blocks_.get() + nfast_ + nxg_,
                            nglob_, pairs_.get(), chunks_.get(), runs_.get(), terms_.get(), ops[0],
                            ops[1], ops[2], x, y, scratch, stats_.scratch_per_wg,
@@ -1554,14 +1627,19 @@ void HeffDevice::launch(const T* x, T* y, const T* const* ops, T* scratch, T* pa
 void HeffDevice::matvec(const double* dvec, double* dres, hipStream_t s) {
     if (prec_ == PREC_FP64) {
         const double* ops[3] = {ops64_[0].get(), ops64_[1].get(), ops64_[2].get()};
-        launch<double, kPool64>(dvec, dres, ops, scratch64_.get(),
-                                                    part64_.get(), s);
+        launch<double, double, kPool64>(dvec, dres, ops, scratch64_.get(), part64_.get(), s);
     } else {
         hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(size_)), dim3(256), 0, s,
                            dvec, x32_.get(), size_);
-        const float* ops[3] = {ops32_[0].get(), ops32_[1].get(), ops32_[2].get()};
-        launch<float, kPool32>(x32_.get(), y32_.get(), ops, scratch32_.get(),
-                                                   part32_.get(), s);
+        if (prec_ == PREC_BF16) {
+            const uint16_t* ops[3] = {ops16_[0].get(), ops16_[1].get(), ops16_[2].get()};
+            launch<float, uint16_t, kPool32>(x32_.get(), y32_.get(), ops, scratch32_.get(),
+                                             part32_.get(), s);
+        } else {
+            const float* ops[3] = {ops32_[0].get(), ops32_[1].get(), ops32_[2].get()};
+            launch<float, float, kPool32>(x32_.get(), y32_.get(), ops, scratch32_.get(),
+                                          part32_.get(), s);
+        }
         hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(size_)), dim3(256), 0, s,
                            y32_.get(), dres, size_);
     }

@@ -13,7 +13,7 @@
 
 namespace t3 {
 
-enum Precision : int { PREC_FP64 = 0, PREC_FP32 = 1 };
+enum Precision : int { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16 = 2 };
 
 struct DevBlock {      // one work unit: an output block or a slice of its pairs
     int64_t y_off;     // element offset of the destination (vector or partials)
@@ -102,8 +102,8 @@ public:
     void diagonal(double* ddiag, hipStream_t s);
 
 private:
-    template <typename T, int POOL>
-    void launch(const T* x, T* y, const T* const* ops, T* scratch, T* part, hipStream_t s);
+    template <typename T, typename OT, int POOL>
+    void launch(const T* x, T* y, const OT* const* ops, T* scratch, T* part, hipStream_t s);
 
     HeffPlan plan_;
     HeffStats stats_;
@@ -125,6 +125,7 @@ private:
     DevBuf<DevTerm> dterms_;
     DevBuf<double> ops64_[3];
     DevBuf<float> ops32_[3];
+    DevBuf<uint16_t> ops16_[3];  // BF16 arenas, each block divided by its power-of-two scale
     DevBuf<double> scratch64_;
     DevBuf<float> scratch32_;
     DevBuf<float> x32_, y32_;
