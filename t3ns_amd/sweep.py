@@ -21,7 +21,8 @@ every other site tensor is an isometry onto its bond toward the first pair
 src/optimize_network.c:574-640).
 """
 
-from collections import namedtuple
+import time
+from collections import Counter, namedtuple
 
 import numpy as np
 
@@ -115,25 +116,36 @@ class DeviceSolver:
         # steps solved with FP32 matvecs (plus FP64 refinement) and in FP64
         # alone, and the Davidson iterations of each kind
         self.counts = {"steps_f32": 0, "steps_f64": 0, "its_f32": 0, "its_f64": 0}
+        # seconds in the Heff set-up (plan, upload: the reference's "Heff:
+        # prepare data") and in Davidson (diagonal and matvecs)
+        self.timers = Counter()
 
     def make_site_tensor(self, desc):
         return make_site_tensor(desc)[0]
 
     def solve(self, d, o, s, guess, tol, keep):
+        t0 = time.perf_counter()
         h64 = Heff(d, o, s, FP64, keepalive=keep)
+        self.timers["Heff: prepare data"] += time.perf_counter() - t0
+        t0 = time.perf_counter()
         try:
             if self.precision == "f64" or h64.size < self.mixed_min_size:
                 e, x, its, rc = h64.davidson(guess, self.max_vecs, self.keep, tol, self.max_its)
                 self.counts["steps_f64"] += 1
                 self.counts["its_f64"] += its
+                self.timers["Heff: davidson"] += time.perf_counter() - t0
                 return e, x, (0, its)
+            t1 = time.perf_counter()
             h32 = Heff(d, o, s, FP32, keepalive=keep)
+            self.timers["Heff: prepare data"] += time.perf_counter() - t1
             try:
+                t2 = time.perf_counter()
                 e, x, i1, i2, rc = davidson_refined(h32, h64, guess, self.max_vecs, self.keep, tol,
                                                     tol, self.max_its)
                 self.counts["steps_f32"] += 1
                 self.counts["its_f32"] += i1
                 self.counts["its_f64"] += i2
+                self.timers["Heff: davidson"] += time.perf_counter() - t2
                 return e, x, (i1, i2)
             finally:
                 h32.close()
@@ -188,6 +200,10 @@ class Sweeper:
         ctx.fill_seeded(False)
         self.env = Environment(ctx, seed, site_tensor=self._site_tensor, **self.solver.env)
         self.energies = []
+        # seconds per phase, named as the reference's timers
+        # (src/optimize_network.c:43-58); the solve covers Heff set-up and
+        # Davidson (the solver may split it further)
+        self.timers = Counter()
 
     def _legs_toward(self, sites, nr_bonds):
         """For every site outside `sites`: the leg of its bond on the path to
@@ -288,6 +304,8 @@ class Sweeper:
 
     def step(self, k):
         sites = self.steps[k]
+        tm = self.timers
+        t0 = time.perf_counter()
         st = Step(self.ctx, sites, seed=self.seed)
         try:
             d, o, s = st.descriptors()
@@ -300,14 +318,21 @@ class Sweeper:
                 mk.site_qnumbers[i], mk.site_beginblock[i], mk.site_tel[i] = qn, bb, tel
             x0 = np.array(self.solver.make_site_tensor(mk), dtype=np.float64)
             x0 /= np.linalg.norm(x0)
+            t1 = time.perf_counter()
+            tm["siteTensor: make multisite tensor"] += t1 - t0
             d, o, s = self.env.step_operators(st)
+            t2 = time.perf_counter()
+            tm["rOperators: append and update"] += t2 - t1
             e, x, its = self.solver.solve(d, o, s, x0, self.tol, (st, self.env))
+            t3 = time.perf_counter()
+            tm["Heff: solve"] += t3 - t2
             centre = self._centre_site(k)
             sd = st.split_problem(0, self.max_dim, self.min_dim, self.truncerr)
             xn = np.ascontiguousarray(x / np.linalg.norm(x))
             sd.tel = xn.ctypes.data_as(_lib.c_dbl_p)
             dims = self._decompose(sd, order, centre)
             self.env.invalidate(sites)
+            tm["siteTensor: decompose"] += time.perf_counter() - t3
             return {"sites": list(sites), "energy": e, "its": its, "bond_dim": max(dims)}
         finally:
             st.close()

@@ -1,0 +1,69 @@
+"""The D2h sweep workload: synthetic FCIDUMP with 36 orbitals and 36
+electrons (t3ns_amd.synthetic, seed 2, D2h irreps round-robin, integrals
+masked to D2h invariance), Z2 x U1 x SU2 x D2h, target (36, 0, Ag), the
+T3NS tree of makenetwork.py 36, two-site steps.  At D = 64: a device sweep
+in FP64 and one with FP32 matvecs and FP64 refinement on every step give
+sweep energies within 1e-6 Eh of each other, the second sweep does not
+rise above the first (variational), and the Heff of a step in the middle
+of the tree matches the oracle on sampled output blocks (1e-12)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import t3ns_amd as T
+from t3ns_amd import synthetic
+from t3ns_amd.sweep import DeviceSolver, Sweeper
+
+NETW = os.path.join(os.path.dirname(__file__), "golden", "networks", "36_T3NS.netw")
+
+
+def _ctx(D):
+    fci = synthetic.qc_workload(36, 36, seed=2, pg="D2h")
+    with open(NETW) as f:
+        return T.Context(f.read(), "QC", [], ["U1", "SU2", "D2h"], [36, 0, 0], D, fcidump=fci)
+
+
+@pytest.mark.gpu
+def test_qc36_d2h_sweeps_fp64_and_mixed(gpu):
+    energies = {}
+    for prec in ("f64", "mixed"):
+        print("qc36 sweep", prec, flush=True)
+        ctx = _ctx(64)
+        dm = Sweeper(ctx, 64, seed=2, solver=DeviceSolver(prec, mixed_min_size=0), tol=1e-7)
+        dm.sweep()
+        dm.sweep()
+        energies[prec] = list(dm.energies)
+        print(prec, ["%.10f" % e for e in dm.energies], dict(dm.timers), flush=True)
+        assert dm.energies[1] <= dm.energies[0] + 1e-9
+        ctx.close()
+    assert np.allclose(energies["f64"], energies["mixed"], rtol=0, atol=1e-6), energies
+
+
+@pytest.mark.gpu
+def test_qc36_d2h_matvec_sampled_blocks(gpu):
+    ctx = _ctx(256)
+    steps = ctx.sweep_steps(2)
+    sites = steps[len(steps) // 2]
+    print("qc36 step", sites, flush=True)
+    st = T.Step(ctx, sites, seed=1)
+    d, o, s = st.descriptors()
+    h = T.Heff(d, o, s, T.FP64, keepalive=st)
+    x = np.random.default_rng(5).standard_normal(h.size)
+    y = h.matvec(x)
+    oh = O.OracleHeff(d, o, s)
+    bf = oh.block_flops()
+    bb = np.ctypeslib.as_array(s.beginblock, shape=(s.nrblocks + 1,))
+    live = np.where(bf > 0)[0]
+    picks = sorted({int(live[np.argmax(bf[live] * (bf[live] <= 5e9))]),
+                    int(live[np.argsort(bf[live])[len(live) // 2]]), int(live[0])})
+    out = np.zeros(oh.size)
+    for b in picks:
+        oh.matvec_blocks(x, out, b, b + 1, 1)
+        r = slice(int(bb[b]), int(bb[b + 1]))
+        err = np.linalg.norm(y[r] - out[r]) / np.linalg.norm(out[r])
+        assert err < 1e-12, (b, err)
+    oh.close()
+    h.close()
+    ctx.close()
