@@ -2,6 +2,7 @@
 #include "renorm.hpp"
 
 #include <algorithm>
+#include <string>
 #include <cmath>
 #include <stdexcept>
 
@@ -328,6 +329,13 @@ namespace t3 {
 // (746-785), next_sb_tens (680-718), next_sb_sec_op (720-744),
 // find_block_adj (787-806) and update_selected_blocks (1043-1072), then
 // sum_unique_rOperators (src/rOperators_init.c:228-265).
+// The branching update is one small GEMM chain per (new block, tensor
+// block, adjoint block, instruction); with many tiny symmetry sectors (e.g.
+// 36 orbitals under D2h) their number reaches 1e8-1e9, whose task list
+// alone outgrows host memory.  Past this count the planner stops with an
+// error instead.
+constexpr int64_t kMaxBUpdateTasks = 100000000;
+
 BUpdatePlan plan_bupdate(const BUpdateInput& in) {
     const int uc = in.update_case;
     if (uc < 0 || uc > 2) throw std::invalid_argument("bUpdate: bad update case");
@@ -415,6 +423,7 @@ BUpdatePlan plan_bupdate(const BUpdateInput& in) {
     const int first = looptype ? 0 : 1, second = 1 - first;
     const int N = in.begin_blocks_of_hss[in.nrhss];
     std::vector<std::vector<BUpdateTask>> per_dest((size_t)nu * N);
+    int64_t ntasks = 0;
     for (int new_sb = 0; new_sb < N; ++new_sb) {
         int bra[3], ket[3], hss[3];
         ids_of(in.qnumbers[new_sb], uc, &bra[uc], &ket[uc], &hss[2]);
@@ -435,11 +444,26 @@ BUpdatePlan plan_bupdate(const BUpdateInput& in) {
                 const auto tg = std::lower_bound(tgroups.begin(), tgroups.end(), match,
                                                  [](const auto& g, QN v) { return g.first < v; });
                 if (tg == tgroups.end() || tg->first != match) continue;
+                const int nbs = S.nblocks_hss(hss_ops[second]);
+                const QN* qs = S.qnumbers_hss(hss_ops[second]);
+                // the second operator's blocks of one MPO sector are sorted by
+                // (bra + ket n + hss n^2): the blocks with a given ket form
+                // one contiguous run, found by bisection (scanned in order,
+                // as a full scan would visit them)
+                const QN nb2 = nsec[bs];
+                bool runs_ok = nbs > 0;
+                for (int js = 1; js < nbs && runs_ok; ++js) runs_ok = qs[js - 1] < qs[js];
+                const QN h0 = nbs > 0 ? qs[0] / (nb2 * nb2) : 0;
+                if (runs_ok) runs_ok = qs[nbs - 1] / (nb2 * nb2) == h0;
                 for (int tb : tg->second) {
                     ket[bs] = looptype ? (int)(T.qnumbers[tb] / divide) : (int)(T.qnumbers[tb] % divide);
-                    const int nbs = S.nblocks_hss(hss_ops[second]);
-                    const QN* qs = S.qnumbers_hss(hss_ops[second]);
-                    for (int js = 0; js < nbs; ++js) {
+                    int js_lo = 0, js_hi = nbs;
+                    if (runs_ok) {
+                        const QN lo = (QN)ket[bs] * nb2 + h0 * nb2 * nb2;
+                        js_lo = (int)(std::lower_bound(qs, qs + nbs, lo) - qs);
+                        js_hi = (int)(std::lower_bound(qs + js_lo, qs + nbs, lo + nb2) - qs);
+                    }
+                    for (int js = js_lo; js < js_hi; ++js) {
                         int ks, hs;
                         ids_of(qs[js], bs, &bra[bs], &ks, &hs);
                         if (ks != ket[bs]) continue;
@@ -485,6 +509,11 @@ BUpdatePlan plan_bupdate(const BUpdateInput& in) {
                             }
                             t.pref = pref;
                             per_dest[(size_t)u * N + new_sb].push_back(t);
+                            if (++ntasks > kMaxBUpdateTasks)
+                                throw std::runtime_error(
+                                    "bUpdate: more than " + std::to_string(kMaxBUpdateTasks) +
+                                    " block products (tiny symmetry sectors); the plan would not "
+                                    "fit in host memory");
                         }
                     }
                 }
