@@ -1132,6 +1132,76 @@ inline int grid_for(int64_t n) {
 }
 }  // namespace
 
+namespace {
+// Device cost of contracting one pair in order `ord` (operator legs in
+// contraction order), in 16x16x4 matrix-core steps: every stage product
+// padded to 16 x 16 output tiles and k-steps of 4 (stages whose output leg
+// has at most 4 states run on the vector unit: 64 fibers per wave, one FMA
+// per output state and k, a quarter of a step's work), times the products
+// of the stage (first-stage operators, terms, stage-C groups), plus a fixed
+// price per chunk of terms (the barriers and memory latency of a chunk,
+// measured at about a thousand steps' worth) with chunks as the LDS pool
+// allows them.
+double order_cost(const int (&din)[3], const int (&dout)[3], const int (&ord)[3], int nst,
+                  const double (&nprod)[3], double nterms, int pool, double chunk_steps) {
+    double cur[3] = {(double)din[0], (double)din[1], (double)din[2]};
+    double steps = 0, wsz[2] = {0, 0};
+    for (int s = 0; s < nst; ++s) {
+        const int leg = ord[s];
+        const double K = cur[leg], dn = dout[leg];
+        double F = 1;
+        for (int k = 0; k < 3; ++k)
+            if (k != leg) F *= cur[k];
+        const double c = dn <= 4 ? std::ceil(F / 64) * K * dn / 4.0
+                                 : std::ceil(F / 16) * std::ceil(dn / 16) * std::ceil(K / 4);
+        steps += c * nprod[s];
+        cur[leg] = dn;
+        if (s < 2) wsz[s] = cur[0] * cur[1] * cur[2];
+    }
+    const double xsz = (double)din[0] * din[1] * din[2];
+    const double avail = std::max(1.0, pool - xsz);
+    const double need = nprod[0] * wsz[0] + (nst == 3 ? nterms * wsz[1] : 0.0);
+    const double chunks = std::max(std::ceil(need / avail), std::ceil(nterms / kMaxChunkTerms));
+    return steps + chunk_steps * chunks;
+}
+
+// Picks P.ord among the contraction orders (6 for three legs, 2 for DMRG)
+// by order_cost; terms: the pair's terms (distinct operator blocks per leg
+// give the first-stage and stage-C product counts).
+void choose_order(DevPair& P, const int (&dout)[3], const DevTerm* t, int nt, int isdmrg, int pool,
+                  bool grouped, double chunk_steps) {
+    double distinct[3] = {0, 0, 0};
+    std::vector<int64_t> v(nt);
+    for (int k = 0; k < (isdmrg ? 2 : 3); ++k) {
+        for (int i = 0; i < nt; ++i) v[i] = t[i].op[k];
+        std::sort(v.begin(), v.end());
+        distinct[k] = (double)(std::unique(v.begin(), v.end()) - v.begin());
+    }
+    double best = HUGE_VAL;
+    int bo[3] = {P.ord[0], P.ord[1], P.ord[2]};
+    const int norders = isdmrg ? 2 : 6;
+    for (int o = 0; o < norders; ++o) {
+        int ord[3];
+        if (isdmrg) {
+            ord[0] = o;
+            ord[1] = 1 - o;
+            ord[2] = 2;
+        } else {
+            for (int k = 0; k < 3; ++k) ord[k] = hOrd[o][k];
+        }
+        const int last = isdmrg ? ord[1] : ord[2];
+        const double nprod[3] = {distinct[ord[0]], isdmrg ? (double)nt : (double)nt,
+                                 grouped && !isdmrg ? distinct[last] : (double)nt};
+        const double c = order_cost(P.d, dout, ord, isdmrg ? 2 : 3, nprod, nt, pool, chunk_steps);
+        if (c < best * (1 - 1e-12)) {
+            best = c;
+            for (int k = 0; k < 3; ++k) bo[k] = ord[k];
+        }
+    }
+    for (int k = 0; k < 3; ++k) P.ord[k] = bo[k];
+}
+}  // namespace
+
 HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     : prec_(precision) {
     plan_ = build_heff_plan(in);
@@ -1186,6 +1256,19 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
             throw std::runtime_error("T3NS_HEFF_PLACEMENT must be lds, ldsw or global");
         }
     }
+    // Contraction order of each pair: the reference's (fewest flops,
+    // src/Heff.c:629-666) or, by default, the one a device cost model
+    // prefers (T3NS_HEFF_ORDER=reference|model).
+    bool order_by_model = true;
+    if (const char* ov = std::getenv("T3NS_HEFF_ORDER")) {
+        if (!std::strcmp(ov, "reference")) order_by_model = false;
+        else if (std::strcmp(ov, "model") != 0)
+            throw std::runtime_error("T3NS_HEFF_ORDER must be reference or model");
+    }
+    // price of a chunk in the order model, in matrix-core steps
+    // (T3NS_HEFF_CHUNK_STEPS overrides it)
+    double chunk_steps = 1000;
+    if (const char* cs = std::getenv("T3NS_HEFF_CHUNK_STEPS")) chunk_steps = std::atof(cs);
     // Global chunk capacity: at least one stage output of the largest pair.
     int64_t gcap = kGlobalChunkElems;
     for (int sb = 0; sb < plan_.nrblocks; ++sb)
@@ -1248,6 +1331,9 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
             if ((int64_t)ht.size() > INT32_MAX)
                 throw std::runtime_error("HeffDevice: more than 2^31 terms");
             if (ht.size() == tb) continue;
+            if (order_by_model)
+                choose_order(P, B.d, ht.data() + tb, (int)(ht.size() - tb), isdmrg_, pool, f32,
+                             chunk_steps);
             // Terms sorted by their first-stage operator block: runs of equal
             // first operators share one first-stage output.
             const int first = P.ord[0];
