@@ -92,3 +92,21 @@ def test_placement_option_rejects_unknown(gpu, monkeypatch):
     ctx, st, d, o, s = _setup(HEFF_CASES[0])
     with pytest.raises(RuntimeError):
         T.Heff(d, o, s, T.FP64, keepalive=st)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", HEFF_CASES, ids=case_id)
+def test_matvec_reference_order(gpu, case, monkeypatch):
+    """The reference's contraction orders (T3NS_HEFF_ORDER=reference) give
+    the same matvec as the cost-model orders (default), both against the
+    oracle (FP64 1e-12)."""
+    ctx, st, d, o, s = _setup(case)
+    oh = O.OracleHeff(d, o, s)
+    x = np.random.default_rng(23).standard_normal(oh.size)
+    yr = oh.matvec(x)
+    for order in ("reference", "model"):
+        monkeypatch.setenv("T3NS_HEFF_ORDER", order)
+        h = T.Heff(d, o, s, T.FP64, keepalive=st)
+        y = h.matvec(x)
+        assert np.linalg.norm(y - yr) / np.linalg.norm(yr) < 1e-12, order
+        h.close()
