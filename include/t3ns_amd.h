@@ -116,6 +116,13 @@ int t3ns_heff_matvec(t3ns_heff* h, const double* vec, double* result);
  * may be NULL for the default stream). */
 int t3ns_heff_matvec_device(t3ns_heff* h, const double* dvec, double* dresult, void* stream);
 
+/* on != 0: matvecs on a stream other than the default ones replay a HIP
+ * graph captured at the first call with the same (input, output) device
+ * arrays, one launch instead of the matvec's sequence of kernels (the
+ * Davidson of this library reuses its search-vector slots, so its matvecs
+ * replay after the first max_vecs); on = 0 (default) drops them. */
+int t3ns_heff_use_graphs(t3ns_heff* h, int on);
+
 /* make_diagonal (include/Heff.h:185): diag(Heff) into a host array. */
 int t3ns_heff_diagonal(t3ns_heff* h, double* diag);
 

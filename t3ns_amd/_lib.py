@@ -190,6 +190,7 @@ def lib():
                                             C.c_double, C.c_int, c_dbl_p, c_int_p, c_int_p]),
         "t3ns_davidson_dev": (C.c_int, [C.c_int64, P, P, DEV_MATVEC, P, P, c_dbl_p, C.c_int,
                                         C.c_int, C.c_double, C.c_int, c_int_p]),
+        "t3ns_heff_use_graphs": (C.c_int, [P, C.c_int]),
         "t3ns_davidson_dev_noalloc": (C.c_int, [C.c_int64, P, P, DEV_MATVEC, P, P, c_dbl_p,
                                                 C.c_int, C.c_int, C.c_double, C.c_int, c_int_p]),
         "t3ns_heff_size": (C.c_int64, [P]),

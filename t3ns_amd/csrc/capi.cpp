@@ -245,6 +245,11 @@ int t3ns_heff_matvec_device(t3ns_heff* h, const double* dvec, double* dres, void
     return guarded([&] { h->dev->matvec(dvec, dres, (hipStream_t)stream); });
 }
 
+int t3ns_heff_use_graphs(t3ns_heff* h, int on) {
+    if (!h) return fail(T3NS_ERR_INVALID, "null argument");
+    return guarded([&] { h->dev->set_graphs(on != 0); });
+}
+
 int t3ns_heff_diagonal(t3ns_heff* h, double* diag) {
     if (!h || !diag) return fail(T3NS_ERR_INVALID, "null argument");
     return guarded([&] {

@@ -1676,7 +1676,7 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     T3_HIP_CHECK(hipStreamSynchronize(stream));
 }
 
-HeffDevice::~HeffDevice() = default;
+HeffDevice::~HeffDevice() { drop_graphs(); }
 
 template <typename T, typename OT, int POOL>
 void HeffDevice::launch(const T* x, T* y, const OT* const* ops, T* scratch, T* part,
@@ -1711,6 +1711,45 @@ void HeffDevice::launch(const T* x, T* y, const OT* const* ops, T* scratch, T* p
 }
 
 void HeffDevice::matvec(const double* dvec, double* dres, hipStream_t s) {
+    // Graph replay: the launch sequence of one (input, output) pair of
+    // device arrays is captured once and replayed (a captured stream only;
+    // the legacy default stream cannot be captured).  Every node is a
+    // kernel (the fills included), so a replay repeats exactly the eager
+    // work.
+    if (use_graphs_ && s != nullptr && s != hipStreamPerThread) {
+        hipGraphExec_t exec = nullptr;
+        for (const auto& g : graphs_)
+            if (g.x == dvec && g.y == dres) exec = g.exec;
+        if (!exec) {
+            hipGraph_t graph = nullptr;
+            T3_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            enqueue(dvec, dres, s);
+            T3_HIP_CHECK(hipStreamEndCapture(s, &graph));
+            T3_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            T3_HIP_CHECK(hipGraphDestroy(graph));
+            if (graphs_.size() >= kMaxGraphs) {  // oldest out
+                T3_HIP_CHECK(hipGraphExecDestroy(graphs_.front().exec));
+                graphs_.erase(graphs_.begin());
+            }
+            graphs_.push_back({dvec, dres, exec});
+        }
+        T3_HIP_CHECK(hipGraphLaunch(exec, s));
+        return;
+    }
+    enqueue(dvec, dres, s);
+}
+
+void HeffDevice::set_graphs(bool on) {
+    use_graphs_ = on;
+    if (!on) drop_graphs();
+}
+
+void HeffDevice::drop_graphs() {
+    for (const auto& g : graphs_) (void)hipGraphExecDestroy(g.exec);
+    graphs_.clear();
+}
+
+void HeffDevice::enqueue(const double* dvec, double* dres, hipStream_t s) {
     if (prec_ == PREC_FP64) {
         const double* ops[3] = {ops64_[0].get(), ops64_[1].get(), ops64_[2].get()};
         launch<double, double, kPool64>(dvec, dres, ops, scratch64_.get(), part64_.get(), s);

@@ -99,9 +99,17 @@ public:
 
     // Device pointers, FP64 in and out.
     void matvec(const double* dvec, double* dres, hipStream_t s);
+    // Replays matvecs from captured HIP graphs, one per (input, output)
+    // pair of arrays (up to kMaxGraphs, oldest dropped first); off by
+    // default.
+    void set_graphs(bool on);
+    bool graphs() const { return use_graphs_; }
+    static constexpr size_t kMaxGraphs = 64;
     void diagonal(double* ddiag, hipStream_t s);
 
 private:
+    void enqueue(const double* dvec, double* dres, hipStream_t s);
+    void drop_graphs();
     template <typename T, typename OT, int POOL>
     void launch(const T* x, T* y, const OT* const* ops, T* scratch, T* part, hipStream_t s);
 
@@ -136,6 +144,13 @@ private:
     int64_t part_elems_ = 0;
     int nsplits_ = 0;
     int ndblocks_ = 0;
+    struct Graph {
+        const double* x;
+        double* y;
+        hipGraphExec_t exec;
+    };
+    bool use_graphs_ = false;
+    std::vector<Graph> graphs_;
 };
 
 }  // namespace t3

@@ -400,6 +400,11 @@ class Heff:
         check(lib().t3ns_heff_matvec_device(self._h, C.c_void_p(dvec_ptr), C.c_void_p(dres_ptr),
                                             C.c_void_p(stream_ptr or 0)), "t3ns_heff_matvec_device")
 
+    def use_graphs(self, on=True):
+        """Replay matvecs on a non-default stream from captured HIP graphs
+        (t3ns_heff_use_graphs)."""
+        check(lib().t3ns_heff_use_graphs(self._h, int(bool(on))), "t3ns_heff_use_graphs")
+
     def diagonal(self):
         out = np.empty(self.size, dtype=np.float64)
         check(lib().t3ns_heff_diagonal(self._h, out.ctypes.data_as(_lib.c_dbl_p)), "t3ns_heff_diagonal")
