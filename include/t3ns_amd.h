@@ -139,8 +139,8 @@ int t3ns_heff_stats(const t3ns_heff* h, int64_t* nblocks, int64_t* npairs, int64
  * pairs whose input block is read from global memory, pairs whose stage
  * outputs use global scratch, matrix-core steps per matvec (total and per
  * stage A/B/C), global scratch elements per workgroup, work units, output
- * blocks split over several units.  Returns the number of statistics
- * available. */
+ * blocks split over several units, pairs whose middle stage is fused into
+ * the last.  Returns the number of statistics available. */
 int t3ns_heff_kernel_stats(const t3ns_heff* h, double* out, int n);
 
 /* The dispatch table the reference builds in its first matvec

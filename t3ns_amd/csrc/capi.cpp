@@ -916,7 +916,7 @@ int t3ns_heff_kernel_stats(const t3ns_heff* h, double* out, int n) {
     const double v[] = {(double)s.nchunks, (double)s.nruns, (double)s.pairs_global_x,
                         (double)s.pairs_global_w, s.mfma_steps, s.stage_steps[0],
                         s.stage_steps[1], s.stage_steps[2], (double)s.scratch_per_wg,
-                        (double)s.nunits, (double)s.nsplit_blocks};
+                        (double)s.nunits, (double)s.nsplit_blocks, (double)s.pairs_fused};
     const int m = (int)(sizeof(v) / sizeof(v[0]));
     for (int i = 0; i < n && i < m; ++i) out[i] = v[i];
     return m;

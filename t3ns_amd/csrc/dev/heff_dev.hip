@@ -441,6 +441,114 @@ __device__ __forceinline__ void row_mac_terms(const Terms<T, OT>& R, const Geo& 
     }
 }
 
+// Fused middle stage.  When the middle leg of a three-stage order has at
+// most kFuseN states in and out, its product is not materialised: stage C
+// builds each A operand on the fly from the first-stage output,
+//     W2[f, k] = sum_b Bs[b'(f), b] W1[f(b), k],
+// with Bs = alpha_t O_b (at most kFuseN x kFuseN values per term, staged
+// in LDS per chunk, padded to kFuseN x kFuseN), so the middle stage has no
+// barrier, no vector-unit tasks and no second-stage outputs in LDS (a chunk
+// then holds many more terms).
+constexpr int kFuseN = 4;
+constexpr int kFuseStage = kFuseN * kFuseN;  // staged values per term
+constexpr int kFTU = 2;                      // terms per k-step walk (register budget)
+struct FLane {
+    int w1off;  // first-stage output offset of this lane's fiber at b = 0, k = its first k
+    int bpr;    // the fiber's index b' on the middle leg
+};
+// Fiber f of the stage C geometry over memory dims (e0, e1, e2) (contracted
+// position pc, middle leg at position pb) mapped to the first-stage output,
+// whose memory dims equal these except at pb (f0, f1 are its first two).
+__device__ __forceinline__ FLane fused_lane(const Geo& g, int tf, int e0, int e1, int f0, int f1,
+                                            int pb, int pc) {
+    FLane L;
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 15, kk = lane >> 4;
+    const int f = min(tf * 16 + i, g.F - 1);
+    int c0, c1, c2;
+    if (pc == 0) {
+        c0 = 0;
+        c1 = f % e1;
+        c2 = f / e1;
+    } else if (pc == 1) {
+        c0 = f % e0;
+        c1 = 0;
+        c2 = f / e0;
+    } else {
+        c0 = f % e0;
+        c1 = f / e0;
+        c2 = 0;
+    }
+    L.bpr = pb == 0 ? c0 : (pb == 1 ? c1 : c2);
+    c0 = pb == 0 ? 0 : c0;
+    c1 = pb == 1 ? 0 : c1;
+    c2 = pb == 2 ? 0 : c2;
+    const int sk = pc == 0 ? 1 : (pc == 1 ? f0 : f0 * f1);
+    L.w1off = c0 + f0 * (c1 + f1 * c2) + kk * sk;
+    return L;
+}
+template <typename T, typename OT>
+struct FTerms {
+    const T* s[kFTU];   // first-stage outputs
+    const T* b[kFTU];   // staged alpha_t O_b blocks (LDS)
+    const OT* o[kFTU];  // stage C operator blocks
+    int n;              // live terms, >= 1
+};
+// acc[j] += W2_t[row tile, :] O_c[column tile tn0 + j, :]^T summed over the
+// terms of R, W2_t built from W1 as above (Kb: middle-leg states in, sb:
+// its stride in W1, sk4: four k-steps of the contracted leg in W1).
+template <typename T, typename OT, int NTN>
+__device__ __forceinline__ void row_mac_fused_n(const FTerms<T, OT>& R, const Geo& g,
+                                                const RowLane& L, const FLane& F, int Kb, int sb,
+                                                int sk4, typename Mfma<T>::V (&acc)[kTN]) {
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 15, kk = lane >> 4;
+    const int n0 = min(L.tn0 * 16 + i, g.dn - 1);
+    int coff[NTN];
+#pragma unroll
+    for (int j = 0; j < NTN; ++j) coff[j] = min((L.tn0 + j) * 16 + i, g.dn - 1) - n0;
+    const int nks = uni(g.ksteps), nu = uni(R.n);
+    const int sk = sk4 >> 2;
+    for (int ks = 0; ks < nks; ++ks) {
+        // lanes past K read k = 0 (in bounds) and contribute zero
+        const bool kv = ks * 4 + kk < g.K;
+        const int ao = F.w1off + (kv ? ks * sk4 : -kk * sk);
+        const int bo = L.ooff + (kv ? ks * 4 * g.dn : -kk * g.dn);
+        T av[kFTU];
+        T bv[kFTU][NTN];
+#pragma unroll
+        for (int u = 0; u < kFTU; ++u) {
+            T a = 0;
+#pragma unroll
+            for (int q = 0; q < kFuseN; ++q)
+                if (q < Kb) a = fma(R.b[u][F.bpr + kFuseN * q], R.s[u][ao + q * sb], a);
+            av[u] = a;
+#pragma unroll
+            for (int j = 0; j < NTN; ++j) bv[u][j] = cvt(R.o[u][bo + coff[j]]);
+        }
+#pragma unroll
+        for (int u = 0; u < kFTU; ++u) {
+            if (u < nu) {
+                const T a = kv ? av[u] : (T)0;
+#pragma unroll
+                for (int j = 0; j < NTN; ++j) acc[j] = Mfma<T>::run(a, bv[u][j], acc[j]);
+            }
+        }
+    }
+}
+
+template <typename T, typename OT>
+__device__ __forceinline__ void row_mac_fused(const FTerms<T, OT>& R, const Geo& g, const RowLane& L,
+                                              const FLane& F, int Kb, int sb, int sk4,
+                                              typename Mfma<T>::V (&acc)[kTN]) {
+    switch (__builtin_amdgcn_readfirstlane(L.ntn)) {
+        case 1: row_mac_fused_n<T, OT, 1>(R, g, L, F, Kb, sb, sk4, acc); break;
+        case 2: row_mac_fused_n<T, OT, 2>(R, g, L, F, Kb, sb, sk4, acc); break;
+        case 3: row_mac_fused_n<T, OT, 3>(R, g, L, F, Kb, sb, sk4, acc); break;
+        default: row_mac_fused_n<T, OT, 4>(R, g, L, F, Kb, sb, sk4, acc); break;
+    }
+}
+
 template <typename T, typename OT, int NTF, bool SCALE>
 __device__ __forceinline__ void blk_mac_terms_n(const Terms<T, OT>& R, const Geo& g, const RowBlk& B,
                                                 typename Mfma<T>::V (&acc)[kTF]) {
@@ -616,6 +724,10 @@ struct ChunkRec {
 // terms on that path only).
 template <typename T>
 constexpr bool kGroupB = sizeof(T) == 4;
+// The fused middle stage is built for the FP64 kernel (FP32 groups terms
+// in stage B instead) when stage outputs live in LDS.
+template <typename T>
+constexpr bool kFused = sizeof(T) == 8;
 
 // Terms t0, t0 + G, ... (at most kTU of them) of a chunk as stage C inputs.
 template <typename T, typename OT>
@@ -648,29 +760,49 @@ __device__ __forceinline__ Terms<T, OT> c_terms(const ChunkRec* rec, int t0, int
 // and walks the items of its group; when a geometry has fewer tiles than
 // waves, items are split over groups (stage C partial tiles are then
 // reduced in a fixed order).
-template <typename T, typename OT, int NT>
+template <typename T, typename OT, int NT, bool FZ>
 __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                                             const DevChunk* __restrict__ chunks,
                                             const int* __restrict__ runs,
                                             const DevTerm* __restrict__ terms,
                                             const OT* __restrict__ op0, const OT* __restrict__ op1,
                                             const OT* __restrict__ op2, const T* X, T* W1, T* Y,
-                                            T* part, ChunkRec* recs, int isdmrg, const int* map,
-                                            const int* pos) {
+                                            T* part, ChunkRec* recs, int isdmrg, int m0, int m1,
+                                            int m2) {
     typedef typename Mfma<T>::V V;
     constexpr int NW = NT / 64;
     const int wave = uni(threadIdx.x >> 6);  // wave-uniform: task loops stay scalar
     const int lane = threadIdx.x & 63;
     const int a = uni(P.ord[0]), b = uni(P.ord[1]), c = uni(isdmrg ? P.ord[1] : P.ord[2]);
-    int cur[3] = {uni(P.d[map[0]]), uni(P.d[map[1]]), uni(P.d[map[2]])};
-    const Geo gA = make_geo(cur[0], cur[1], cur[2], pos[a], bd[a]);
-    cur[pos[a]] = bd[a];
+    // Leg dimensions through the stages.  Only constant indices into
+    // arrays, selects otherwise: a runtime index would put the array in
+    // scratch memory.
+    const int pd0 = uni(P.d[0]), pd1 = uni(P.d[1]), pd2 = uni(P.d[2]);
+    const int bd0 = uni(bd[0]), bd1 = uni(bd[1]), bd2 = uni(bd[2]);
+    auto posof = [&](int k) { return k == m0 ? 0 : (k == m1 ? 1 : 2); };
+    auto bdof = [&](int k) { return k == 0 ? bd0 : (k == 1 ? bd1 : bd2); };
+    auto pdof = [&](int k) { return k == 0 ? pd0 : (k == 1 ? pd1 : pd2); };
+    int cur0 = pdof(m0), cur1 = pdof(m1), cur2 = pdof(m2);
+    auto setcur = [&](int p, int v) {
+        cur0 = p == 0 ? v : cur0;
+        cur1 = p == 1 ? v : cur1;
+        cur2 = p == 2 ? v : cur2;
+    };
+    const Geo gA = make_geo(cur0, cur1, cur2, posof(a), bdof(a));
+    setcur(posof(a), bdof(a));
+    const int w1d0 = cur0, w1d1 = cur1;  // first-stage output dims (first two)
     Geo gB = gA;
     if (!isdmrg) {
-        gB = make_geo(cur[0], cur[1], cur[2], pos[b], bd[b]);
-        cur[pos[b]] = bd[b];
+        gB = make_geo(cur0, cur1, cur2, posof(b), bdof(b));
+        setcur(posof(b), bdof(b));
     }
-    const Geo gC = make_geo(cur[0], cur[1], cur[2], pos[c], bd[c]);
+    const Geo gC = make_geo(cur0, cur1, cur2, posof(c), bdof(c));
+    // fused middle stage (three-leg objects only): its input states, its
+    // stride in W1 and four k-steps of stage C's contracted leg in W1
+    const bool fz = FZ && uni(P.fuse) != 0;
+    const int fKb = gB.K, fpb = posof(b), fpc = posof(c);
+    const int fsb = fpb == 0 ? 1 : (fpb == 1 ? w1d0 : w1d0 * w1d1);
+    const int fsk4 = 4 * (fpc == 0 ? 1 : (fpc == 1 ? w1d0 : w1d0 * w1d1));
     // stages whose output leg has at most kVN states take the vector-unit path
     const bool vA = gA.dn <= kVN, vB = gB.dn <= kVN, vC = gC.dn <= kVN;
     const VTasks tA = vtasks(gA, NW), tB = vtasks(gB, NW), tC = vtasks(gC, NW);
@@ -681,7 +813,7 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
     // stage C goes row-blocked only when its tasks alone occupy every wave
     // (no item groups, hence no partial-tile reduction)
     const StageTasks kCr = stage_tasks_rb(gC, NW);
-    const bool rbC = use_rows(gC) && kCr.tf * kCr.tng >= NW;
+    const bool rbC = !fz && use_rows(gC) && kCr.tf * kCr.tng >= NW;
     // stage C item groups are reduced through LDS: keep the partial tiles
     // within NW x 256 values
     const int ntnC = (gC.dn + 15) >> 4;
@@ -690,7 +822,7 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
     // When every wave has at most one stage C task, its accumulators stay in
     // registers across the pair's chunks and Y is read and written once per
     // pair instead of once per chunk.
-    const bool pC = !vC && !rbC && kC.total <= NW;
+    const bool pC = !fz && !vC && !rbC && kC.total <= NW;
     const bool pCv = vC && tC.total <= NW;  // the same on the vector-unit path
     const VLane LV = vlane(gC, wave % tC.nft);
     const int grpV = wave / tC.nft;
@@ -777,6 +909,9 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             if (threadIdx.x < nr) rec->opa[threadIdx.x] = terms[runs[uni(C.run_begin) + threadIdx.x]].op[a];
             __syncthreads();
         }
+        // fused middle stage: alpha_t O_b of every term, loaded now and
+        // staged in LDS (after W1) at the end of stage A
+        constexpr int kFuseRegs = kMaxChunkTerms * kFuseStage / NT;
         if (vA) {  // stage A, vector unit
             for (int task = wave; task < tA.total; task += NW) {
                 const VLane L = vlane(gA, task % tA.nft);
@@ -786,7 +921,6 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                     vput<T>(W1 + j * w1, gA, L, acc, false);
                 }
             }
-            __syncthreads();
         } else if (rbA) {  // stage A, row-blocked
             const int base = kA.tf * kA.tng;
             for (int task = wave; task < kA.total; task += NW) {
@@ -800,7 +934,6 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                     blk_put<T>(W1 + j * w1, gA, L, acc, false);
                 }
             }
-            __syncthreads();
         } else {  // stage A: W1[j] = O_a[j] x_a X
             const int base = kA.tf * kA.tng;
             for (int task = wave; task < kA.total; task += NW) {
@@ -814,9 +947,22 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                     row_put<T>(W1 + j * w1, gA, L, acc, false);
                 }
             }
-            __syncthreads();
         }
-        if (!isdmrg && vB) {  // stage B, vector unit
+        if (fz) {
+            const int dnb = uni(gB.dn);
+#pragma unroll
+            for (int r = 0; r < kFuseRegs; ++r) {
+                const int idx = threadIdx.x + r * NT;
+                const int t = idx / kFuseStage, e = idx % kFuseStage;
+                const int bp = e % kFuseN, bq = e / kFuseN;
+                T v = (T)0;
+                if (t < nt && bp < dnb && bq < fKb)
+                    v = (T)rec->alpha[t] * (T)cvt(opbb[rec->opb[t] + bp + bq * dnb]);
+                if (idx < nt * kFuseStage) W2[idx] = v;
+            }
+        }
+        __syncthreads();
+        if (!isdmrg && !fz && vB) {  // stage B, vector unit
             for (int task = wave; task < tB.total; task += NW) {
                 const VLane L = vlane(gB, task % tB.nft);
                 for (int g = task / tB.nft; g < ncs; g += tB.G) {
@@ -833,7 +979,7 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                 }
             }
             __syncthreads();
-        } else if (!isdmrg && rbB) {  // stage B, row-blocked
+        } else if (!isdmrg && !fz && rbB) {  // stage B, row-blocked
             const int base = kB.tf * kB.tng;
             for (int task = wave; task < kB.total; task += NW) {
                 const int r = task % base, grp = task / base;
@@ -854,7 +1000,7 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                 }
             }
             __syncthreads();
-        } else if (!isdmrg) {  // stage B: W2[t] = alpha_t O_b[t] x_b W1[slot t]
+        } else if (!isdmrg && !fz) {  // stage B: W2[t] = alpha_t O_b[t] x_b W1[slot t]
             const int base = kB.tf * kB.tng;
             for (int task = wave; task < kB.total; task += NW) {
                 const int r = task % base, grp = task / base;
@@ -900,7 +1046,35 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             }
             if (threadIdx.x < pfnr) pfa = terms[runs[uni(C2.run_begin) + threadIdx.x]].op[a];
         }
-        if (vC) {  // stage C, vector unit
+        if (fz) {  // stage C with the middle stage fused (one term per group)
+            auto fterms = [&](int t0, int G) {
+                FTerms<T, OT> R;
+                R.n = uni(min(kFTU, (nt - t0 + G - 1) / G));
+#pragma unroll
+                for (int u = 0; u < kFTU; ++u) {
+                    const int t = t0 + min(u, R.n - 1) * G;
+                    R.s[u] = W1 + uni(rec->slot[t]) * w1;
+                    R.b[u] = W2 + t * kFuseStage;
+                    R.o[u] = opc + uni64(rec->opc[t]);
+                }
+                return R;
+            };
+            {  // Y is read and written once per chunk (a fused pair has few chunks)
+                for (int task = wave; task < kC.total; task += NW) {
+                    const int r = task % baseC, grp = task / baseC;
+                    const RowLane L = row_lane<T>(gC, r % kC.tf, (r / kC.tf) * kTN);
+                    const FLane FL = fused_lane(gC, r % kC.tf, cur0, cur1, w1d0, w1d1, fpb, fpc);
+                    V acc[kTN];
+#pragma unroll
+                    for (int q = 0; q < kTN; ++q) acc[q] = V{0, 0, 0, 0};
+                    for (int t0 = grp; t0 < nt; t0 += kC.G * kFTU)
+                        row_mac_fused<T, OT>(fterms(t0, kC.G), gC, L, FL, fKb, fsb, fsk4, acc);
+                    if (kC.G == 1) row_put<T>(Y, gC, L, acc, true);
+                    else store_part(acc, L, grp, r);
+                }
+                if (kC.G > 1) reduce_parts();
+            }
+        } else if (vC) {  // stage C, vector unit
             auto cmac = [&](int t, const VLane& L, V& acc) {
                 if (isdmrg)
                     vmac<T, OT, true>(W1 + uni(rec->slot[t]) * w1, opc + uni64(rec->opc[t]), gC, L,
@@ -1012,22 +1186,33 @@ __global__ void __launch_bounds__(NT)
                        const OT* __restrict__ op0, const OT* __restrict__ op1,
                        const OT* __restrict__ op2, const T* __restrict__ x, T* __restrict__ y,
                        T* __restrict__ scratch, int64_t scratch_stride, int* __restrict__ next,
-                       T* __restrict__ partial, int isdmrg, int m0, int m1, int m2) {
+                       const int* __restrict__ seg, int nseg, T* __restrict__ partial, int isdmrg,
+                       int m0, int m1, int m2) {
     constexpr int kStage = MODE == kModeGlobal ? 0 : POOL;
     __shared__ T lds[kStage + kPartial];
     __shared__ ChunkRec rec[2];
     __shared__ int cur_block;
     T* part = lds + kStage;
-    const int map[3] = {m0, m1, m2};
-    int pos[3];
-    pos[m0] = 0;
-    pos[m1] = 1;
-    pos[m2] = 2;
-    // Units are taken from a shared counter in cost order (dynamic
-    // longest-first schedule); every workgroup leaves once it runs past the
-    // end of the list.
+    // Units are taken from shared counters, one per segment of the unit
+    // list: a workgroup drains the segment of its own XCD first
+    // (workgroups are dispatched to the XCDs round-robin, so blockIdx mod 8
+    // is the XCD when there are 8 segments), then the others in turn; it
+    // leaves once every segment is drained.  One segment: one shared
+    // longest-first list.
+    const int home = (int)(blockIdx.x % (unsigned)nseg);
     for (;;) {
-        if (threadIdx.x == 0) cur_block = atomicAdd(next, 1);
+        if (threadIdx.x == 0) {
+            int u = nblocks;
+            for (int k = 0; k < nseg; ++k) {
+                const int sg = home + k < nseg ? home + k : home + k - nseg;
+                const int i = atomicAdd(next + sg, 1);
+                if (i < seg[sg + 1] - seg[sg]) {
+                    u = seg[sg] + i;
+                    break;
+                }
+            }
+            cur_block = u;
+        }
         __syncthreads();
         const int bi = cur_block;
         __syncthreads();
@@ -1037,19 +1222,19 @@ __global__ void __launch_bounds__(NT)
         for (int pi = B.pair_begin; pi < B.pair_end; ++pi) {
             const DevPair P = pairs[pi];
             if (MODE == kModeLdsW) {
-                pair_chunks<T, OT, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, lds, Y,
-                                   part, rec, isdmrg, map, pos);
+                pair_chunks<T, OT, NT, kFused<T> && MODE != kModeGlobal>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, lds, Y,
+                                   part, rec, isdmrg, m0, m1, m2);
             } else if (MODE == kModeGlobal) {
                 T* gw1 = scratch + (int64_t)blockIdx.x * scratch_stride;
-                pair_chunks<T, OT, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, gw1, Y,
-                                   part, rec, isdmrg, map, pos);
+                pair_chunks<T, OT, NT, kFused<T> && MODE != kModeGlobal>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, gw1, Y,
+                                   part, rec, isdmrg, m0, m1, m2);
             } else {
                 const T* X = x + P.x_off;
                 const int xsz = P.d[0] * P.d[1] * P.d[2];
                 for (int i = threadIdx.x; i < xsz; i += NT) lds[i] = X[i];
                 __syncthreads();
-                pair_chunks<T, OT, NT>(P, B.d, chunks, runs, terms, op0, op1, op2, lds,
-                                   lds + P.w1_off, Y, part, rec, isdmrg, map, pos);
+                pair_chunks<T, OT, NT, kFused<T> && MODE != kModeGlobal>(P, B.d, chunks, runs, terms, op0, op1, op2, lds,
+                                   lds + P.w1_off, Y, part, rec, isdmrg, m0, m1, m2);
             }
         }
     }
@@ -1165,11 +1350,48 @@ double order_cost(const int (&din)[3], const int (&dout)[3], const int (&ord)[3]
     return steps + chunk_steps * chunks;
 }
 
+// The same for a three-leg order whose middle stage is fused into stage C
+// (middle leg of at most kFuseN states in and out): stage A over the
+// distinct first operators, stage C over every term with its A operands
+// built on the fly (`fuse_over`: their extra work per matrix-core step), no
+// middle stage; a chunk holds its first-stage outputs and kFuseStage staged
+// values per term.  Returns HUGE_VAL where the form does not apply.
+double order_cost_fused(const int (&din)[3], const int (&dout)[3], const int (&ord)[3],
+                        double nfirst, double nterms, int pool, double chunk_steps,
+                        double fuse_over) {
+    const int a = ord[0], m = ord[1], c = ord[2];
+    if (din[m] > kFuseN || dout[m] > kFuseN || dout[c] <= kVN) return HUGE_VAL;
+    double cur[3] = {(double)din[0], (double)din[1], (double)din[2]};
+    auto fibers = [&](int leg) {
+        double F = 1;
+        for (int k = 0; k < 3; ++k)
+            if (k != leg) F *= cur[k];
+        return F;
+    };
+    double F = fibers(a), K = cur[a], dn = dout[a];
+    const double cA = dn <= 4 ? std::ceil(F / 64) * K * dn / 4.0
+                              : std::ceil(F / 16) * std::ceil(dn / 16) * std::ceil(K / 4);
+    cur[a] = dn;
+    const double w1 = cur[0] * cur[1] * cur[2];
+    cur[m] = dout[m];
+    F = fibers(c);
+    K = cur[c];
+    dn = dout[c];
+    const double cC = std::ceil(F / 16) * std::ceil(dn / 16) * std::ceil(K / 4) * (1 + fuse_over);
+    const double xsz = (double)din[0] * din[1] * din[2];
+    if (w1 + kFuseStage > pool) return HUGE_VAL;
+    const double avail = std::max(1.0, pool - (xsz + w1 + kFuseStage <= pool ? xsz : 0.0));
+    const double need = nfirst * w1 + nterms * kFuseStage;
+    const double chunks = std::max(std::ceil(need / avail), std::ceil(nterms / kMaxChunkTerms));
+    return cA * nfirst + cC * nterms + chunk_steps * chunks;
+}
+
 // Picks P.ord among the contraction orders (6 for three legs, 2 for DMRG)
-// by order_cost; terms: the pair's terms (distinct operator blocks per leg
-// give the first-stage and stage-C product counts).
+// by order_cost (and, with `fuse`, the fused-middle form by
+// order_cost_fused); terms: the pair's terms (distinct operator blocks per
+// leg give the first-stage and stage-C product counts).
 void choose_order(DevPair& P, const int (&dout)[3], const DevTerm* t, int nt, int isdmrg, int pool,
-                  bool grouped, double chunk_steps) {
+                  bool grouped, double chunk_steps, bool fuse, double fuse_over) {
     double distinct[3] = {0, 0, 0};
     std::vector<int64_t> v(nt);
     for (int k = 0; k < (isdmrg ? 2 : 3); ++k) {
@@ -1179,6 +1401,7 @@ void choose_order(DevPair& P, const int (&dout)[3], const DevTerm* t, int nt, in
     }
     double best = HUGE_VAL;
     int bo[3] = {P.ord[0], P.ord[1], P.ord[2]};
+    int bf = 0;
     const int norders = isdmrg ? 2 : 6;
     for (int o = 0; o < norders; ++o) {
         int ord[3];
@@ -1195,10 +1418,21 @@ void choose_order(DevPair& P, const int (&dout)[3], const DevTerm* t, int nt, in
         const double c = order_cost(P.d, dout, ord, isdmrg ? 2 : 3, nprod, nt, pool, chunk_steps);
         if (c < best * (1 - 1e-12)) {
             best = c;
+            bf = 0;
             for (int k = 0; k < 3; ++k) bo[k] = ord[k];
+        }
+        if (fuse && !isdmrg) {
+            const double cf = order_cost_fused(P.d, dout, ord, distinct[ord[0]], nt, pool,
+                                               chunk_steps, fuse_over);
+            if (cf < best * (1 - 1e-12)) {
+                best = cf;
+                bf = 1;
+                for (int k = 0; k < 3; ++k) bo[k] = ord[k];
+            }
         }
     }
     for (int k = 0; k < 3; ++k) P.ord[k] = bo[k];
+    P.fuse = bf;
 }
 }  // namespace
 
@@ -1269,6 +1503,13 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     // (T3NS_HEFF_CHUNK_STEPS overrides it)
     double chunk_steps = 1000;
     if (const char* cs = std::getenv("T3NS_HEFF_CHUNK_STEPS")) chunk_steps = std::atof(cs);
+    // Fused middle stage (FP64 three-leg objects, cost-model orders):
+    // T3NS_HEFF_FUSE=0 turns it off; T3NS_HEFF_FUSE_OVER prices the extra
+    // work of its stage C operands per matrix-core step.
+    bool fuse = !f32 && order_by_model;
+    if (const char* fv = std::getenv("T3NS_HEFF_FUSE")) fuse = fuse && std::atoi(fv) != 0;
+    double fuse_over = 0.25;
+    if (const char* fo = std::getenv("T3NS_HEFF_FUSE_OVER")) fuse_over = std::atof(fo);
     // Global chunk capacity: at least one stage output of the largest pair.
     int64_t gcap = kGlobalChunkElems;
     for (int sb = 0; sb < plan_.nrblocks; ++sb)
@@ -1333,7 +1574,7 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
             if (ht.size() == tb) continue;
             if (order_by_model)
                 choose_order(P, B.d, ht.data() + tb, (int)(ht.size() - tb), isdmrg_, pool, f32,
-                             chunk_steps);
+                             chunk_steps, fuse, fuse_over);
             // Terms sorted by their first-stage operator block: runs of equal
             // first operators share one first-stage output.
             const int first = P.ord[0];
@@ -1352,7 +1593,7 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
             const int64_t xsz = (int64_t)P.d[0] * P.d[1] * P.d[2];
             // LDS when one run and one term fit next to the input block
             // (kModeLds) or without it (kModeLdsW)
-            const int64_t wmin = (int64_t)P.w1 + (isdmrg_ ? 0 : P.w2);
+            const int64_t wmin = (int64_t)P.w1 + (isdmrg_ ? 0 : (P.fuse ? kFuseStage : P.w2));
             P.xlds = x_in_lds && xsz + wmin <= pool;
             P.wlds = wmin <= pool;
             P.w1_off = P.xlds ? (int)xsz : 0;
@@ -1401,8 +1642,9 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
                 const int64_t r1 = (int64_t)(nruns + newrun) * P.w1;
                 const int64_t r2 =
                     isdmrg_ ? 0
-                            : (int64_t)((nterms == 0 ? 0 : (group ? (int)cops.size() : nterms)) + newc) *
-                                  P.w2;
+                    : P.fuse ? (int64_t)(nterms + 1) * kFuseStage
+                             : (int64_t)((nterms == 0 ? 0 : (group ? (int)cops.size() : nterms)) + newc) *
+                                   P.w2;
                 const bool fits = nterms < kMaxChunkTerms &&
                                   (P.wlds ? r1 + r2 <= avail : r1 <= gcap && r2 <= gcap);
                 if (nterms > 0 && !fits) {
@@ -1454,7 +1696,8 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
                     ncs += hc[ci].ncs;
                 }
                 stats_.stage_steps[0] += nr * st[0];
-                stats_.stage_steps[1] += ntt * st[1];
+                stats_.stage_steps[1] += P.fuse ? 0.0 : ntt * st[1];
+                stats_.pairs_fused += P.fuse;
                 if (!isdmrg_) stats_.stage_steps[2] += ncs * st[2];
                 stats_.nruns += (int64_t)nr;
                 stats_.pairs_global_x += !P.xlds;
@@ -1489,6 +1732,7 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     std::vector<DevBlock> units;
     std::vector<double> ucost;
     std::vector<int> uglob;
+    std::vector<int> usb;  // output block of each unit
     std::vector<DevSplit> hs;
     int64_t part_elems = 0;
     for (const DevBlock& B : hb) {
@@ -1516,10 +1760,12 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
         slices.push_back({beg, B.pair_end});
         scost.push_back(acc);
         sglob.push_back(is_glob(beg));
+        const int sbi = (int)(&B - hb.data());
         if (slices.size() == 1) {
             units.push_back(B);
             ucost.push_back(scost[0]);
             uglob.push_back(sglob[0]);
+            usb.push_back(sbi);
             continue;
         }
         DevSplit S{};
@@ -1537,25 +1783,79 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
             units.push_back(U);
             ucost.push_back(scost[k]);
             uglob.push_back(sglob[k]);
+            usb.push_back(sbi);
         }
         part_elems += (int64_t)slices.size() * ysz;
     }
-    // Units in kernel order (kModeLds, kModeLdsW, kModeGlobal); costliest
-    // first within each list so that the dynamic schedule drains evenly.
-    std::vector<int> idx(units.size());
-    std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
-        if (uglob[a] != uglob[b]) return uglob[a] < uglob[b];
-        return ucost[a] > ucost[b];
-    });
-    std::vector<DevBlock> hbs;
-    nfast_ = nxg_ = 0;
-    for (int i : idx) {
-        hbs.push_back(units[i]);
-        nfast_ += uglob[i] == 0;
-        nxg_ += uglob[i] == 1;
+    // Units in kernel order (kModeLds, kModeLdsW, kModeGlobal), each list
+    // ordered by the schedule (T3NS_HEFF_SCHED):
+    //   cost      costliest first, one shared list (dynamic longest-first);
+    //   block     output-block order (neighbouring blocks share sectors,
+    //             hence operator blocks), the cheapest tenth of the cost
+    //             moved to the end, costliest first, so the schedule drains
+    //             evenly;
+    //   xcd       output-block order cut into one segment of equal cost
+    //             per XCD (8), costliest first within a segment; a
+    //             workgroup drains its XCD's segment, then the others;
+    //   xcdblock  the same segments, each in the block order above.
+    int sched = 0;
+    if (const char* sv = std::getenv("T3NS_HEFF_SCHED")) {
+        const char* names[4] = {"cost", "block", "xcd", "xcdblock"};
+        sched = -1;
+        for (int k = 0; k < 4; ++k)
+            if (!std::strcmp(sv, names[k])) sched = k;
+        if (sched < 0) throw std::runtime_error("T3NS_HEFF_SCHED must be cost, block, xcd or xcdblock");
     }
-    nglob_ = (int)hbs.size() - nfast_ - nxg_;
+    auto by_cost = [&](std::vector<int>& v) {
+        std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return ucost[a] > ucost[b]; });
+    };
+    // block order with the cheapest tenth of the cost at the end
+    auto by_block = [&](std::vector<int>& v) {
+        std::vector<int> c = v;
+        std::stable_sort(c.begin(), c.end(), [&](int a, int b) { return ucost[a] < ucost[b]; });
+        double tot = 0, acc = 0;
+        for (int i : v) tot += ucost[i];
+        std::vector<char> tail(units.size(), 0);
+        for (int i : c) {
+            if (acc + ucost[i] > 0.1 * tot) break;
+            acc += ucost[i];
+            tail[i] = 1;
+        }
+        std::vector<int> head, rest;
+        for (int i : v) (tail[i] ? rest : head).push_back(i);  // v is in block order
+        by_cost(rest);
+        v = head;
+        v.insert(v.end(), rest.begin(), rest.end());
+    };
+    std::vector<DevBlock> hbs;
+    std::vector<int> hseg;
+    for (int m = 0; m < 3; ++m) {
+        std::vector<int> v;
+        for (size_t i = 0; i < units.size(); ++i)
+            if (uglob[i] == m) v.push_back((int)i);  // block order
+        const int nseg = (sched >= 2 && m == 0 && v.size() >= 64) ? kSegments : 1;
+        std::vector<std::vector<int>> segs(nseg);
+        double tot = 0;
+        for (int i : v) tot += ucost[i];
+        double acc = 0;
+        for (int i : v) {
+            const int sg = std::min(nseg - 1, (int)(acc / tot * nseg));
+            segs[sg].push_back(i);
+            acc += ucost[i];
+        }
+        nseg_[m] = nseg;
+        const int base = (int)hbs.size();
+        for (int sg = 0; sg <= kSegments; ++sg) hseg.push_back(0);
+        int* sb = hseg.data() + (kSegments + 1) * m;
+        for (int sg = 0; sg < nseg; ++sg) {
+            if (sched == 0 || sched == 2) by_cost(segs[sg]);
+            else by_block(segs[sg]);
+            sb[sg] = (int)hbs.size() - base;
+            for (int i : segs[sg]) hbs.push_back(units[i]);
+        }
+        sb[nseg] = (int)hbs.size() - base;
+        (m == 0 ? nfast_ : m == 1 ? nxg_ : nglob_) = (int)hbs.size() - base;
+    }
     stats_.nunits = (int64_t)hbs.size();
     stats_.nsplit_blocks = (int64_t)hs.size();
     nsplits_ = (int)hs.size();
@@ -1668,7 +1968,8 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     nwg_glob_ = std::max(1, std::min(nglob_, ncu));
     if (f32) scratch32_.alloc(nglob_ ? (size_t)nwg_glob_ * maxscratch : 1);
     else scratch64_.alloc(nglob_ ? (size_t)nwg_glob_ * maxscratch : 1);
-    counter_.alloc(3);
+    counter_.alloc(3 * kSegments);
+    seg_.upload(hseg, stream);
     splits_.upload(hs, stream);
     if (f32) part32_.alloc(std::max<int64_t>(1, part_elems_));
     else part64_.alloc(std::max<int64_t>(1, part_elems_));
@@ -1682,7 +1983,8 @@ template <typename T, typename OT, int POOL>
 void HeffDevice::launch(const T* x, T* y, const OT* const* ops, T* scratch, T* part,
                         hipStream_t s) {
     hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(size_)), dim3(256), 0, s, y, size_, (T)0);
-    hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s, counter_.get(), (int64_t)3, 0);
+    hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s, counter_.get(),
+                       (int64_t)(3 * kSegments), 0);
     if (part_elems_)
         hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(part_elems_)), dim3(256), 0, s, part,
                            part_elems_, (T)0);
@@ -1691,20 +1993,22 @@ void HeffDevice::launch(const T* x, T* y, const OT* const* ops, T* scratch, T* p
                            dim3(std::min(nfast_, ncu_)), dim3(kThreads), 0, s, blocks_.get(),
                            nfast_, pairs_.get(), chunks_.get(), runs_.get(), terms_.get(), ops[0],
                            ops[1], ops[2], x, y, scratch, stats_.scratch_per_wg, counter_.get(),
-                           part, isdmrg_, map_[0], map_[1], map_[2]);
+                           seg_.get(), nseg_[0], part, isdmrg_, map_[0], map_[1], map_[2]);
     if (nxg_)
         hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeLdsW>),
                            dim3(std::min(nxg_, ncu_)), dim3(kThreads), 0, s,
                            blocks_.get() + nfast_, nxg_, pairs_.get(), chunks_.get(), runs_.get(),
                            terms_.get(), ops[0], ops[1], ops[2], x, y, scratch,
-                           stats_.scratch_per_wg, counter_.get() + 1, part, isdmrg_, map_[0],
-                           map_[1], map_[2]);
+                           stats_.scratch_per_wg, counter_.get() + kSegments,
+                           seg_.get() + (kSegments + 1), nseg_[1], part, isdmrg_, map_[0], map_[1],
+                           map_[2]);
     if (nglob_)
         hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeGlobal>),
                            dim3(nwg_glob_), dim3(kThreads), 0, s, blocks_.get() + nfast_ + nxg_,
                            nglob_, pairs_.get(), chunks_.get(), runs_.get(), terms_.get(), ops[0],
                            ops[1], ops[2], x, y, scratch, stats_.scratch_per_wg,
-                           counter_.get() + 2, part, isdmrg_, map_[0], map_[1], map_[2]);
+                           counter_.get() + 2 * kSegments, seg_.get() + 2 * (kSegments + 1),
+                           nseg_[2], part, isdmrg_, map_[0], map_[1], map_[2]);
     if (nsplits_)
         hipLaunchKernelGGL((split_reduce_kernel<T>), dim3(std::min(nsplits_, 4096)), dim3(256), 0,
                            s, splits_.get(), nsplits_, part, y);

@@ -39,6 +39,7 @@ struct DevPair {       // one (output block, input block) pair
     int xlds;          // input block staged in LDS
     int wlds;          // stage outputs in LDS (else workgroup global scratch)
     int w1_off;        // first-stage outputs: offset in the LDS pool (after the input block)
+    int fuse;          // middle stage fused into stage C (chunks stage alpha O_b, no W2)
 };
 // Consecutive terms of a pair processed between two barriers: their
 // distinct first-stage operators ("runs") and the terms themselves.
@@ -79,6 +80,7 @@ struct HeffStats {
     int64_t nsplit_blocks = 0;
     int64_t pairs_global_x = 0;  // input block read from global memory
     int64_t pairs_global_w = 0;  // stage outputs in global scratch
+    int64_t pairs_fused = 0;     // pairs whose middle stage is fused into stage C
     double mfma_steps = 0;       // 16x16x4 matrix-core steps of one matvec
     double stage_steps[3] = {0, 0, 0};
 };
@@ -137,7 +139,12 @@ private:
     DevBuf<double> scratch64_;
     DevBuf<float> scratch32_;
     DevBuf<float> x32_, y32_;
-    DevBuf<int> counter_;  // next output block of the dynamic schedule
+    // Unit schedule: per kernel, up to kSegments segments of the unit list
+    // (one per XCD), each with its own counter.
+    static constexpr int kSegments = 8;
+    DevBuf<int> counter_;  // next unit of each segment
+    DevBuf<int> seg_;      // segment boundaries, kSegments + 1 per kernel
+    int nseg_[3] = {1, 1, 1};
     DevBuf<DevSplit> splits_;
     DevBuf<double> part64_;
     DevBuf<float> part32_;

@@ -434,7 +434,8 @@ class Heff:
 
     def kernel_stats(self):
         names = ["chunks", "runs", "pairs_global_x", "pairs_global_w", "mfma_steps",
-                 "steps_A", "steps_B", "steps_C", "scratch_per_wg", "units", "split_blocks"]
+                 "steps_A", "steps_B", "steps_C", "scratch_per_wg", "units", "split_blocks",
+                 "pairs_fused"]
         buf = (C.c_double * len(names))()
         check(lib().t3ns_heff_kernel_stats(self._h, buf, len(names)) < 0 and -1 or 0)
         return dict(zip(names, list(buf)))
