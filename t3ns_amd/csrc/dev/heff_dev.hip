@@ -487,20 +487,23 @@ __device__ __forceinline__ FLane fused_lane(const Geo& g, int tf, int e0, int e1
     L.w1off = c0 + f0 * (c1 + f1 * c2) + kk * sk;
     return L;
 }
+// Stage C items of the fused form: groups of terms sharing their last
+// operator (the A operand of a group sums its terms' W2).
 template <typename T, typename OT>
 struct FTerms {
-    const T* s[kFTU];   // first-stage outputs
-    const T* b[kFTU];   // staged alpha_t O_b blocks (LDS)
-    const OT* o[kFTU];  // stage C operator blocks
-    int n;              // live terms, >= 1
+    int tb[kFTU], te[kFTU];  // terms of each group (chunk-relative)
+    const OT* o[kFTU];       // stage C operator blocks
+    int n;                   // live groups, >= 1
 };
-// acc[j] += W2_t[row tile, :] O_c[column tile tn0 + j, :]^T summed over the
-// terms of R, W2_t built from W1 as above (Kb: middle-leg states in, sb:
-// its stride in W1, sk4: four k-steps of the contracted leg in W1).
+// acc[j] += sum over the groups g of R and their terms t of
+// W2_t[row tile, :] O_c,g[column tile tn0 + j, :]^T, W2_t built from W1 as
+// above (W1 + slot_t w1; bst: the staged blocks; Kb: middle-leg states in,
+// sb: its stride in W1, sk4: four k-steps of the contracted leg in W1).
 template <typename T, typename OT, int NTN>
 __device__ __forceinline__ void row_mac_fused_n(const FTerms<T, OT>& R, const Geo& g,
-                                                const RowLane& L, const FLane& F, int Kb, int sb,
-                                                int sk4, typename Mfma<T>::V (&acc)[kTN]) {
+                                                const RowLane& L, const FLane& F, const T* W1,
+                                                int w1, const T* bst, const int* slot, int Kb,
+                                                int sb, int sk4, typename Mfma<T>::V (&acc)[kTN]) {
     const int lane = threadIdx.x & 63;
     const int i = lane & 15, kk = lane >> 4;
     const int n0 = min(L.tn0 * 16 + i, g.dn - 1);
@@ -519,9 +522,13 @@ __device__ __forceinline__ void row_mac_fused_n(const FTerms<T, OT>& R, const Ge
 #pragma unroll
         for (int u = 0; u < kFTU; ++u) {
             T a = 0;
+            for (int t = R.tb[u]; t < R.te[u]; ++t) {
+                const T* sp = W1 + uni(slot[t]) * w1 + ao;
+                const T* bp = bst + t * kFuseStage + F.bpr;
 #pragma unroll
-            for (int q = 0; q < kFuseN; ++q)
-                if (q < Kb) a = fma(R.b[u][F.bpr + kFuseN * q], R.s[u][ao + q * sb], a);
+                for (int q = 0; q < kFuseN; ++q)
+                    if (q < Kb) a = fma(bp[kFuseN * q], sp[q * sb], a);
+            }
             av[u] = a;
 #pragma unroll
             for (int j = 0; j < NTN; ++j) bv[u][j] = cvt(R.o[u][bo + coff[j]]);
@@ -539,13 +546,14 @@ __device__ __forceinline__ void row_mac_fused_n(const FTerms<T, OT>& R, const Ge
 
 template <typename T, typename OT>
 __device__ __forceinline__ void row_mac_fused(const FTerms<T, OT>& R, const Geo& g, const RowLane& L,
-                                              const FLane& F, int Kb, int sb, int sk4,
+                                              const FLane& F, const T* W1, int w1, const T* bst,
+                                              const int* slot, int Kb, int sb, int sk4,
                                               typename Mfma<T>::V (&acc)[kTN]) {
     switch (__builtin_amdgcn_readfirstlane(L.ntn)) {
-        case 1: row_mac_fused_n<T, OT, 1>(R, g, L, F, Kb, sb, sk4, acc); break;
-        case 2: row_mac_fused_n<T, OT, 2>(R, g, L, F, Kb, sb, sk4, acc); break;
-        case 3: row_mac_fused_n<T, OT, 3>(R, g, L, F, Kb, sb, sk4, acc); break;
-        default: row_mac_fused_n<T, OT, 4>(R, g, L, F, Kb, sb, sk4, acc); break;
+        case 1: row_mac_fused_n<T, OT, 1>(R, g, L, F, W1, w1, bst, slot, Kb, sb, sk4, acc); break;
+        case 2: row_mac_fused_n<T, OT, 2>(R, g, L, F, W1, w1, bst, slot, Kb, sb, sk4, acc); break;
+        case 3: row_mac_fused_n<T, OT, 3>(R, g, L, F, W1, w1, bst, slot, Kb, sb, sk4, acc); break;
+        default: row_mac_fused_n<T, OT, 4>(R, g, L, F, W1, w1, bst, slot, Kb, sb, sk4, acc); break;
     }
 }
 
@@ -724,10 +732,9 @@ struct ChunkRec {
 // terms on that path only).
 template <typename T>
 constexpr bool kGroupB = sizeof(T) == 4;
-// The fused middle stage is built for the FP64 kernel (FP32 groups terms
-// in stage B instead) when stage outputs live in LDS.
+// The fused middle stage is built where stage outputs live in LDS.
 template <typename T>
-constexpr bool kFused = sizeof(T) == 8;
+constexpr bool kFused = true;
 
 // Terms t0, t0 + G, ... (at most kTU of them) of a chunk as stage C inputs.
 template <typename T, typename OT>
@@ -1046,16 +1053,16 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             }
             if (threadIdx.x < pfnr) pfa = terms[runs[uni(C2.run_begin) + threadIdx.x]].op[a];
         }
-        if (fz) {  // stage C with the middle stage fused (one term per group)
-            auto fterms = [&](int t0, int G) {
+        if (fz) {  // stage C with the middle stage fused, over groups of terms
+            auto fterms = [&](int g0, int G) {
                 FTerms<T, OT> R;
-                R.n = uni(min(kFTU, (nt - t0 + G - 1) / G));
+                R.n = uni(min(kFTU, (ncs - g0 + G - 1) / G));
 #pragma unroll
                 for (int u = 0; u < kFTU; ++u) {
-                    const int t = t0 + min(u, R.n - 1) * G;
-                    R.s[u] = W1 + uni(rec->slot[t]) * w1;
-                    R.b[u] = W2 + t * kFuseStage;
-                    R.o[u] = opc + uni64(rec->opc[t]);
+                    const int gg = g0 + min(u, R.n - 1) * G;
+                    R.tb[u] = uni(rec->cbeg[gg]);
+                    R.te[u] = uni(rec->cbeg[gg + 1]);
+                    R.o[u] = opc + uni64(rec->opc[gg]);
                 }
                 return R;
             };
@@ -1067,8 +1074,9 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                     V acc[kTN];
 #pragma unroll
                     for (int q = 0; q < kTN; ++q) acc[q] = V{0, 0, 0, 0};
-                    for (int t0 = grp; t0 < nt; t0 += kC.G * kFTU)
-                        row_mac_fused<T, OT>(fterms(t0, kC.G), gC, L, FL, fKb, fsb, fsk4, acc);
+                    for (int g0 = grp; g0 < ncs; g0 += kC.G * kFTU)
+                        row_mac_fused<T, OT>(fterms(g0, kC.G), gC, L, FL, W1, w1, W2, rec->slot,
+                                             fKb, fsb, fsk4, acc);
                     if (kC.G == 1) row_put<T>(Y, gC, L, acc, true);
                     else store_part(acc, L, grp, r);
                 }
@@ -1352,12 +1360,14 @@ double order_cost(const int (&din)[3], const int (&dout)[3], const int (&ord)[3]
 
 // The same for a three-leg order whose middle stage is fused into stage C
 // (middle leg of at most kFuseN states in and out): stage A over the
-// distinct first operators, stage C over every term with its A operands
-// built on the fly (`fuse_over`: their extra work per matrix-core step), no
-// middle stage; a chunk holds its first-stage outputs and kFuseStage staged
-// values per term.  Returns HUGE_VAL where the form does not apply.
+// distinct first operators, stage C over the distinct last operators (terms
+// sharing one are summed in its A operands), the A operands built on the
+// fly from every term (`fuse_over`: that work per fiber tile and k-step, in
+// matrix-core steps), no middle stage; a chunk holds its first-stage
+// outputs and kFuseStage staged values per term.  Returns HUGE_VAL where
+// the form does not apply.
 double order_cost_fused(const int (&din)[3], const int (&dout)[3], const int (&ord)[3],
-                        double nfirst, double nterms, int pool, double chunk_steps,
+                        double nfirst, double nlast, double nterms, int pool, double chunk_steps,
                         double fuse_over) {
     const int a = ord[0], m = ord[1], c = ord[2];
     if (din[m] > kFuseN || dout[m] > kFuseN || dout[c] <= kVN) return HUGE_VAL;
@@ -1377,13 +1387,14 @@ double order_cost_fused(const int (&din)[3], const int (&dout)[3], const int (&o
     F = fibers(c);
     K = cur[c];
     dn = dout[c];
-    const double cC = std::ceil(F / 16) * std::ceil(dn / 16) * std::ceil(K / 4) * (1 + fuse_over);
+    const double cC = std::ceil(F / 16) * std::ceil(dn / 16) * std::ceil(K / 4);
+    const double cOp = std::ceil(F / 16) * std::ceil(K / 4) * fuse_over;
     const double xsz = (double)din[0] * din[1] * din[2];
     if (w1 + kFuseStage > pool) return HUGE_VAL;
     const double avail = std::max(1.0, pool - (xsz + w1 + kFuseStage <= pool ? xsz : 0.0));
     const double need = nfirst * w1 + nterms * kFuseStage;
     const double chunks = std::max(std::ceil(need / avail), std::ceil(nterms / kMaxChunkTerms));
-    return cA * nfirst + cC * nterms + chunk_steps * chunks;
+    return cA * nfirst + cC * nlast + cOp * nterms + chunk_steps * chunks;
 }
 
 // Picks P.ord among the contraction orders (6 for three legs, 2 for DMRG)
@@ -1422,8 +1433,8 @@ void choose_order(DevPair& P, const int (&dout)[3], const DevTerm* t, int nt, in
             for (int k = 0; k < 3; ++k) bo[k] = ord[k];
         }
         if (fuse && !isdmrg) {
-            const double cf = order_cost_fused(P.d, dout, ord, distinct[ord[0]], nt, pool,
-                                               chunk_steps, fuse_over);
+            const double cf = order_cost_fused(P.d, dout, ord, distinct[ord[0]], distinct[ord[2]],
+                                               nt, pool, chunk_steps, fuse_over);
             if (cf < best * (1 - 1e-12)) {
                 best = cf;
                 bf = 1;
@@ -1506,9 +1517,11 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     // Fused middle stage (FP64 three-leg objects, cost-model orders):
     // T3NS_HEFF_FUSE=0 turns it off; T3NS_HEFF_FUSE_OVER prices the extra
     // work of its stage C operands per matrix-core step.
-    bool fuse = !f32 && order_by_model;
+    bool fuse = order_by_model;
     if (const char* fv = std::getenv("T3NS_HEFF_FUSE")) fuse = fuse && std::atoi(fv) != 0;
-    double fuse_over = 0.25;
+    if (f32)
+        if (const char* fv = std::getenv("T3NS_HEFF_FUSE32")) fuse = fuse && std::atoi(fv) != 0;
+    double fuse_over = 2.0;
     if (const char* fo = std::getenv("T3NS_HEFF_FUSE_OVER")) fuse_over = std::atof(fo);
     // Global chunk capacity: at least one stage output of the largest pair.
     int64_t gcap = kGlobalChunkElems;
@@ -1604,10 +1617,10 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
             int nruns = 0, nterms = 0;
             int64_t prev = -1;
             // last operators of the chunk's groups.  Terms are grouped on the
-            // FP32 path only; FP64 and DMRG objects keep one group per term
-            // (the FP64 kernel's stage B has no register room for the
-            // group loops).
-            const bool group = !isdmrg_ && f32;
+            // FP32 path and in fused FP64 pairs; other FP64 pairs and DMRG
+            // objects keep one group per term (the FP64 kernel's stage B has
+            // no register room for the group loops).
+            const bool group = !isdmrg_ && (f32 || P.fuse);
             const int last = isdmrg_ ? P.ord[1] : P.ord[2];
             std::vector<int64_t> cops;
             auto group_of = [&](const DevTerm& tm) {
