@@ -77,6 +77,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the baseline sample (0 disables it)")
     ap.add_argument("--precisions", default="f64,f32,bf16")
+    ap.add_argument("--autotune-trials", type=int, default=6,
+                    help="work-list settings the autotuner times before the timed matvecs of "
+                         "the first precision (0: the defaults, untuned)")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the renormalisation and site-tensor kernels")
     ap.add_argument("--no-solve", action="store_true",
@@ -518,6 +521,12 @@ def main():
         t_init = time.perf_counter() - t0
         heffs[prec] = h
         print("# %s: Heff init %.1f s" % (prec, t_init), file=sys.stderr, flush=True)
+        tune = None
+        if args.autotune_trials > 0 and prec == args.precisions.split(",")[0]:
+            t0 = time.perf_counter()
+            tune = h.autotune(max_trials=args.autotune_trials)
+            tune["seconds"] = time.perf_counter() - t0
+            print("# %s: autotune %s" % (prec, tune), file=sys.stderr, flush=True)
         dy = torch.empty_like(dx)
         torch.cuda.synchronize()
         for _ in range(args.warmup):
@@ -545,7 +554,7 @@ def main():
         ys[prec] = dy.cpu().numpy()
         results[prec] = {"ms_per_matvec": ms, "matvec_per_s": 1000.0 / ms,
                          "wall_s": wall, "init_s": t_init, "flops": st["flops"],
-                         "tflops": st["flops"] / (ms * 1e-3) / 1e12}
+                         "tflops": st["flops"] / (ms * 1e-3) / 1e12, "tuning": tune}
         print("# %s: %.3f ms/matvec, %.2f TFLOP/s (init %.1f s)" %
               (prec, ms, results[prec]["tflops"], t_init), file=sys.stderr, flush=True)
 
@@ -594,6 +603,7 @@ def main():
             "speedup_vs_f64": results["bf16"]["matvec_per_s"] / r["matvec_per_s"],
             "rel_err_vs_f64": err,
             "roofline_frac": results["bf16"]["tflops"] / PEAK_TFLOPS["bf16"]}
+    out["autotune"] = {p: results[p]["tuning"] for p in results}
     if not args.no_solve:
         out["solve"] = measure_solve(ctx, step, args.solve_tol)
     if not args.no_sweep:

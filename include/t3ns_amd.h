@@ -135,6 +135,23 @@ int64_t t3ns_heff_size(const t3ns_heff* h);
 int t3ns_heff_stats(const t3ns_heff* h, int64_t* nblocks, int64_t* npairs, int64_t* nterms,
                     double* flops);
 
+/* Autotuner (no counterpart in the reference: its matvec has no tunable
+ * work decomposition).  Times matvecs under at most max_trials settings of
+ * the work-list cost model (chunk price, fused-operand price, fused middle
+ * stage on/off, unit schedule), keeps the fastest and records it for the
+ * object's block-size histogram; an object whose histogram was tuned before
+ * in this process takes the recorded setting without timing.  out[0..n):
+ * chunk price, fused-operand price, fused (0/1), schedule, ms per matvec
+ * under the result, ms under the setting before tuning (0 when taken from
+ * the record), settings timed, taken from the record (0/1).  Results change
+ * by rounding only. */
+int t3ns_heff_autotune(t3ns_heff* h, int max_trials, double* out, int n);
+/* The work-list setting t[0..4) (as out[0..4) above) read or imposed. */
+int t3ns_heff_get_tuning(const t3ns_heff* h, double* t);
+int t3ns_heff_set_tuning(t3ns_heff* h, const double* t);
+/* Forgets every recorded tuning of this process. */
+void t3ns_tune_cache_clear(void);
+
 /* Work-list statistics of the device kernels, out[0..n): chunks, runs,
  * pairs whose input block is read from global memory, pairs whose stage
  * outputs use global scratch, matrix-core steps per matvec (total and per

@@ -197,6 +197,10 @@ def lib():
         "t3ns_heff_stats": (C.c_int, [P, c_i64_p, c_i64_p, c_i64_p, c_dbl_p]),
         "t3ns_heff_plan_table": (C.c_int64, [P, c_int_p, c_dbl_p]),
         "t3ns_heff_kernel_stats": (C.c_int, [P, c_dbl_p, C.c_int]),
+        "t3ns_heff_autotune": (C.c_int, [P, C.c_int, c_dbl_p, C.c_int]),
+        "t3ns_heff_get_tuning": (C.c_int, [P, c_dbl_p]),
+        "t3ns_heff_set_tuning": (C.c_int, [P, c_dbl_p]),
+        "t3ns_tune_cache_clear": (None, []),
         "t3ns_plan_table": (C.c_int64, [C.POINTER(HeffDesc), C.POINTER(RopsDesc),
                                         C.POINTER(SiteDesc), c_int_p, c_dbl_p, c_dbl_p]),
         "t3ns_plan_block_dims": (C.c_int, [C.POINTER(HeffDesc), C.POINTER(RopsDesc),

@@ -17,6 +17,7 @@
 #include "dev/heff_dev.hpp"
 #include "dev/sitetensor_dev.hpp"
 #include "dev/renorm_dev.hpp"
+#include "heff_tune.hpp"
 #include "host/context.hpp"
 #include "host/heff_plan.hpp"
 #include "host/tensors.hpp"
@@ -249,6 +250,50 @@ int t3ns_heff_use_graphs(t3ns_heff* h, int on) {
     if (!h) return fail(T3NS_ERR_INVALID, "null argument");
     return guarded([&] { h->dev->set_graphs(on != 0); });
 }
+
+int t3ns_heff_autotune(t3ns_heff* h, int max_trials, double* out, int n) {
+    if (!h || (n > 0 && !out)) return fail(T3NS_ERR_INVALID, "null argument");
+    if (max_trials < 1) return fail(T3NS_ERR_INVALID, "max_trials must be at least 1");
+    return guarded([&] {
+        const size_t sz = h->dev->size();
+        std::vector<double> x(sz);
+        for (size_t i = 0; i < sz; ++i) x[i] = 0.1 * (double)(i % 7 + 1);
+        T3_HIP_CHECK(hipMemcpyAsync(h->dvec.get(), x.data(), sz * sizeof(double),
+                                    hipMemcpyHostToDevice, h->stream));
+        const TuneReport R = autotune_heff(*h->dev, h->host.in, h->stream, h->dvec.get(),
+                                           h->dres.get(), max_trials);
+        const double v[] = {R.best.chunk_steps, R.best.fuse_over, (double)R.best.fuse,
+                            (double)R.best.sched,  R.ms, R.baseline_ms, (double)R.trials,
+                            R.cached ? 1.0 : 0.0};
+        for (int i = 0; i < n && i < 8; ++i) out[i] = v[i];
+    });
+}
+
+int t3ns_heff_set_tuning(t3ns_heff* h, const double* t) {
+    if (!h || !t) return fail(T3NS_ERR_INVALID, "null argument");
+    if (!(t[0] >= 0) || !(t[1] >= 0) || t[3] < 0 || t[3] > 3)
+        return fail(T3NS_ERR_INVALID, "tuning out of range");
+    return guarded([&] {
+        HeffTuning q;
+        q.chunk_steps = t[0];
+        q.fuse_over = t[1];
+        q.fuse = t[2] != 0;
+        q.sched = (int)t[3];
+        h->dev->retune(h->host.in, q, h->stream);
+    });
+}
+
+int t3ns_heff_get_tuning(const t3ns_heff* h, double* t) {
+    if (!h || !t) return fail(T3NS_ERR_INVALID, "null argument");
+    const HeffTuning& q = h->dev->tuning();
+    t[0] = q.chunk_steps;
+    t[1] = q.fuse_over;
+    t[2] = q.fuse;
+    t[3] = q.sched;
+    return T3NS_OK;
+}
+
+void t3ns_tune_cache_clear(void) { clear_tune_cache(); }
 
 int t3ns_heff_diagonal(t3ns_heff* h, double* diag) {
     if (!h || !diag) return fail(T3NS_ERR_INVALID, "null argument");

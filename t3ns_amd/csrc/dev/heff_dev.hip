@@ -519,6 +519,12 @@ __device__ __forceinline__ void row_mac_fused_n(const FTerms<T, OT>& R, const Ge
         const int bo = L.ooff + (kv ? ks * 4 * g.dn : -kk * g.dn);
         T av[kFTU];
         T bv[kFTU][NTN];
+        // operator loads first: their memory latency overlaps the LDS work
+        // of the A operands
+#pragma unroll
+        for (int u = 0; u < kFTU; ++u)
+#pragma unroll
+            for (int j = 0; j < NTN; ++j) bv[u][j] = cvt(R.o[u][bo + coff[j]]);
 #pragma unroll
         for (int u = 0; u < kFTU; ++u) {
             T a = 0;
@@ -530,8 +536,6 @@ __device__ __forceinline__ void row_mac_fused_n(const FTerms<T, OT>& R, const Ge
                     if (q < Kb) a = fma(bp[kFuseN * q], sp[q * sb], a);
             }
             av[u] = a;
-#pragma unroll
-            for (int j = 0; j < NTN; ++j) bv[u][j] = cvt(R.o[u][bo + coff[j]]);
         }
 #pragma unroll
         for (int u = 0; u < kFTU; ++u) {
@@ -1447,7 +1451,23 @@ void choose_order(DevPair& P, const int (&dout)[3], const DevTerm* t, int nt, in
 }
 }  // namespace
 
-HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
+HeffTuning HeffTuning::from_env() {
+    HeffTuning t;
+    if (const char* cs = std::getenv("T3NS_HEFF_CHUNK_STEPS")) t.chunk_steps = std::atof(cs);
+    if (const char* fv = std::getenv("T3NS_HEFF_FUSE")) t.fuse = std::atoi(fv) != 0;
+    if (const char* fo = std::getenv("T3NS_HEFF_FUSE_OVER")) t.fuse_over = std::atof(fo);
+    if (const char* sv = std::getenv("T3NS_HEFF_SCHED")) {
+        const char* names[4] = {"cost", "block", "xcd", "xcdblock"};
+        t.sched = -1;
+        for (int k = 0; k < 4; ++k)
+            if (!std::strcmp(sv, names[k])) t.sched = k;
+        if (t.sched < 0) throw std::runtime_error("T3NS_HEFF_SCHED must be cost, block, xcd or xcdblock");
+    }
+    return t;
+}
+
+HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream,
+                       const HeffTuning* tuning)
     : prec_(precision) {
     plan_ = build_heff_plan(in);
     isdmrg_ = plan_.isdmrg;
@@ -1455,15 +1475,14 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     size_ = in.site->size();
     const int nops = isdmrg_ ? 2 : 3;
     const InstructionSet& is = *in.iset;
-
-    // Flatten the plan into work lists.  FP32 and BF16 share the FP32
-    // arithmetic (and its pool and term grouping).
+    tune_ = tuning ? *tuning : HeffTuning::from_env();
     const bool f32 = prec_ != PREC_FP64;
     const bool bf16 = prec_ == PREC_BF16;
     // BF16: every operator block is stored divided by a power of two at or
     // above its largest magnitude; the scales of a term's three blocks go
     // into its prefactor (exact, powers of two).
-    std::vector<std::vector<std::vector<float>>> bscale(3);
+    std::vector<std::vector<std::vector<float>>>& bscale = bscale_;
+    bscale.assign(3, {});
     if (bf16) {
         for (int k = 0; k < nops; ++k) {
             const ROperators& op = *in.ops[k];
@@ -1486,6 +1505,113 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
             }
         }
     }
+    // Diagonal work list.
+    std::vector<DiagEntry> de = build_heff_diag_plan(in, plan_);
+    std::vector<DevDiagBlock> hdb;
+    std::vector<DevTerm> hdt;
+    for (size_t i = 0; i < de.size();) {
+        const int sb = de[i].sb;
+        DevDiagBlock D{};
+        D.off = plan_.beginblock[sb];
+        for (int k = 0; k < 3; ++k) D.d[k] = plan_.dims[sb][k];
+        D.term_begin = (int)hdt.size();
+        for (; i < de.size() && de[i].sb == sb; ++i) {
+            const DiagEntry& e = de[i];
+            for (int j = is.MPOc_beg[e.mpo_group]; j < is.MPOc_beg[e.mpo_group + 1]; ++j) {
+                const Instruction& ins = is.instr[j];
+                DevTerm t{};
+                bool ok = true;
+                for (int k = 0; k < nops; ++k) {
+                    const ROperators& op = *in.ops[k];
+                    const int o = ins.instr[k];
+                    if (op.block_size(o, e.sbops[k]) == 0) {
+                        ok = false;
+                        break;
+                    }
+                    t.op[k] = op.op_offset[o] + op.beginblock[o][e.sbops[k]];
+                }
+                if (!ok) continue;
+                t.alpha = ins.pref * e.prefactor;
+                hdt.push_back(t);
+            }
+        }
+        D.term_end = (int)hdt.size();
+        if (D.term_end > D.term_begin) hdb.push_back(D);
+    }
+    ndblocks_ = (int)hdb.size();
+
+    dblocks_.upload(hdb, stream);
+    dterms_.upload(hdt, stream);
+    build_lists(in, stream);
+    for (int k = 0; k < nops; ++k) {
+        const ROperators& op = *in.ops[k];
+        stats_.op_elems[k] = op.total_size();
+        if ((int64_t)op.tel.size() == op.total_size()) {
+            ops64_[k].upload(op.tel, stream);
+        } else if ((int)op.tel_src.size() == op.nrops) {  // straight from the caller's arrays
+            ops64_[k].alloc(op.total_size());
+            for (int o = 0; o < op.nrops; ++o) {
+                const int64_t n = op.op_offset[o + 1] - op.op_offset[o];
+                if (n)
+                    T3_HIP_CHECK(hipMemcpyAsync(ops64_[k].get() + op.op_offset[o], op.tel_src[o],
+                                                n * sizeof(double), hipMemcpyHostToDevice, stream));
+            }
+        } else {
+            throw std::runtime_error("HeffDevice: operator host data missing");
+        }
+    }
+    if (prec_ == PREC_FP32) {
+        for (int k = 0; k < nops; ++k) {
+            const int64_t n = stats_.op_elems[k];
+            ops32_[k].alloc(n);
+            if (n)
+                hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0,
+                                   stream, ops64_[k].get(), ops32_[k].get(), n);
+        }
+    }
+    if (bf16) {
+        // per-element scale of the BF16 arena: the reciprocal block scale,
+        // applied by the conversion kernel (block ranges cover the arena)
+        for (int k = 0; k < nops; ++k) {
+            const ROperators& op = *in.ops[k];
+            const int64_t n = stats_.op_elems[k];
+            std::vector<float> inv(std::max<int64_t>(n, 1), 1.0f);
+            for (int o = 0; o < op.nrops; ++o)
+                for (size_t b = 0; b + 1 < op.beginblock[o].size(); ++b)
+                    for (int64_t e = op.beginblock[o][b]; e < op.beginblock[o][b + 1]; ++e)
+                        inv[op.op_offset[o] + e] = 1.0f / bscale[k][o][b];
+            DevBuf<float> dinv;
+            dinv.upload(inv, stream);
+            ops16_[k].alloc(std::max<int64_t>(n, 1));
+            if (n)
+                hipLaunchKernelGGL(to_bf16_kernel, dim3(grid_for(n)), dim3(256), 0, stream,
+                                   ops64_[k].get(), dinv.get(), ops16_[k].get(), n);
+            T3_HIP_CHECK(hipStreamSynchronize(stream));
+        }
+    }
+    if (f32) {
+        x32_.alloc(size_);
+        y32_.alloc(size_);
+    }
+    T3_HIP_CHECK(hipGetLastError());
+    T3_HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+// Work lists of the current tuning (everything the matvec kernels read
+// besides the vectors and operator arenas).  Rebuilt by retune().
+void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
+    drop_graphs();  // captured launches point at the old lists
+    {
+        const HeffStats old = stats_;
+        stats_ = HeffStats{};
+        for (int k = 0; k < 3; ++k) stats_.op_elems[k] = old.op_elems[k];
+    }
+    const int nops = isdmrg_ ? 2 : 3;
+    const InstructionSet& is = *in.iset;
+    // FP32 and BF16 share the FP32 arithmetic (and its pool and term
+    // grouping).
+    const bool f32 = prec_ != PREC_FP64;
+    const bool bf16 = prec_ == PREC_BF16;
     int pool = f32 ? kPool32 : kPool64;
     bool x_in_lds = true;
     // T3NS_HEFF_PLACEMENT=ldsw reads every input block from global memory,
@@ -1510,19 +1636,10 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
         else if (std::strcmp(ov, "model") != 0)
             throw std::runtime_error("T3NS_HEFF_ORDER must be reference or model");
     }
-    // price of a chunk in the order model, in matrix-core steps
-    // (T3NS_HEFF_CHUNK_STEPS overrides it)
-    double chunk_steps = 1000;
-    if (const char* cs = std::getenv("T3NS_HEFF_CHUNK_STEPS")) chunk_steps = std::atof(cs);
-    // Fused middle stage (FP64 three-leg objects, cost-model orders):
-    // T3NS_HEFF_FUSE=0 turns it off; T3NS_HEFF_FUSE_OVER prices the extra
-    // work of its stage C operands per matrix-core step.
-    bool fuse = order_by_model;
-    if (const char* fv = std::getenv("T3NS_HEFF_FUSE")) fuse = fuse && std::atoi(fv) != 0;
-    if (f32)
-        if (const char* fv = std::getenv("T3NS_HEFF_FUSE32")) fuse = fuse && std::atoi(fv) != 0;
-    double fuse_over = 2.0;
-    if (const char* fo = std::getenv("T3NS_HEFF_FUSE_OVER")) fuse_over = std::atof(fo);
+    // cost-model prices of this build (HeffTuning)
+    const double chunk_steps = tune_.chunk_steps;
+    const bool fuse = order_by_model && tune_.fuse;
+    const double fuse_over = tune_.fuse_over;
     // Global chunk capacity: at least one stage output of the largest pair.
     int64_t gcap = kGlobalChunkElems;
     for (int sb = 0; sb < plan_.nrblocks; ++sb)
@@ -1578,7 +1695,7 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
                     t.alpha = ins.pref * e.prefactor;
                     if (bf16)
                         for (int k = 0; k < nops; ++k)
-                            t.alpha *= bscale[k][ins.instr[k]][e.sbops[k]];
+                            t.alpha *= bscale_[k][ins.instr[k]][e.sbops[k]];
                     ht.push_back(t);
                 }
             }
@@ -1811,14 +1928,7 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     //             per XCD (8), costliest first within a segment; a
     //             workgroup drains its XCD's segment, then the others;
     //   xcdblock  the same segments, each in the block order above.
-    int sched = 0;
-    if (const char* sv = std::getenv("T3NS_HEFF_SCHED")) {
-        const char* names[4] = {"cost", "block", "xcd", "xcdblock"};
-        sched = -1;
-        for (int k = 0; k < 4; ++k)
-            if (!std::strcmp(sv, names[k])) sched = k;
-        if (sched < 0) throw std::runtime_error("T3NS_HEFF_SCHED must be cost, block, xcd or xcdblock");
-    }
+    const int sched = tune_.sched;
     auto by_cost = [&](std::vector<int>& v) {
         std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return ucost[a] > ucost[b]; });
     };
@@ -1874,41 +1984,6 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     nsplits_ = (int)hs.size();
     part_elems_ = part_elems;
 
-    // Diagonal work list.
-    std::vector<DiagEntry> de = build_heff_diag_plan(in, plan_);
-    std::vector<DevDiagBlock> hdb;
-    std::vector<DevTerm> hdt;
-    for (size_t i = 0; i < de.size();) {
-        const int sb = de[i].sb;
-        DevDiagBlock D{};
-        D.off = plan_.beginblock[sb];
-        for (int k = 0; k < 3; ++k) D.d[k] = plan_.dims[sb][k];
-        D.term_begin = (int)hdt.size();
-        for (; i < de.size() && de[i].sb == sb; ++i) {
-            const DiagEntry& e = de[i];
-            for (int j = is.MPOc_beg[e.mpo_group]; j < is.MPOc_beg[e.mpo_group + 1]; ++j) {
-                const Instruction& ins = is.instr[j];
-                DevTerm t{};
-                bool ok = true;
-                for (int k = 0; k < nops; ++k) {
-                    const ROperators& op = *in.ops[k];
-                    const int o = ins.instr[k];
-                    if (op.block_size(o, e.sbops[k]) == 0) {
-                        ok = false;
-                        break;
-                    }
-                    t.op[k] = op.op_offset[o] + op.beginblock[o][e.sbops[k]];
-                }
-                if (!ok) continue;
-                t.alpha = ins.pref * e.prefactor;
-                hdt.push_back(t);
-            }
-        }
-        D.term_end = (int)hdt.size();
-        if (D.term_end > D.term_begin) hdb.push_back(D);
-    }
-    ndblocks_ = (int)hdb.size();
-
     stats_.nblocks = (int64_t)hbs.size();
     stats_.npairs = (int64_t)hp.size();
     stats_.nterms = (int64_t)ht.size();
@@ -1923,58 +1998,6 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     chunks_.upload(hc, stream);
     runs_.upload(hr, stream);
     terms_.upload(ht, stream);
-    dblocks_.upload(hdb, stream);
-    dterms_.upload(hdt, stream);
-    for (int k = 0; k < nops; ++k) {
-        const ROperators& op = *in.ops[k];
-        stats_.op_elems[k] = op.total_size();
-        if ((int64_t)op.tel.size() == op.total_size()) {
-            ops64_[k].upload(op.tel, stream);
-        } else if ((int)op.tel_src.size() == op.nrops) {  // straight from the caller's arrays
-            ops64_[k].alloc(op.total_size());
-            for (int o = 0; o < op.nrops; ++o) {
-                const int64_t n = op.op_offset[o + 1] - op.op_offset[o];
-                if (n)
-                    T3_HIP_CHECK(hipMemcpyAsync(ops64_[k].get() + op.op_offset[o], op.tel_src[o],
-                                                n * sizeof(double), hipMemcpyHostToDevice, stream));
-            }
-        } else {
-            throw std::runtime_error("HeffDevice: operator host data missing");
-        }
-    }
-    if (prec_ == PREC_FP32) {
-        for (int k = 0; k < nops; ++k) {
-            const int64_t n = stats_.op_elems[k];
-            ops32_[k].alloc(n);
-            if (n)
-                hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0,
-                                   stream, ops64_[k].get(), ops32_[k].get(), n);
-        }
-    }
-    if (bf16) {
-        // per-element scale of the BF16 arena: the reciprocal block scale,
-        // applied by the conversion kernel (block ranges cover the arena)
-        for (int k = 0; k < nops; ++k) {
-            const ROperators& op = *in.ops[k];
-            const int64_t n = stats_.op_elems[k];
-            std::vector<float> inv(std::max<int64_t>(n, 1), 1.0f);
-            for (int o = 0; o < op.nrops; ++o)
-                for (size_t b = 0; b + 1 < op.beginblock[o].size(); ++b)
-                    for (int64_t e = op.beginblock[o][b]; e < op.beginblock[o][b + 1]; ++e)
-                        inv[op.op_offset[o] + e] = 1.0f / bscale[k][o][b];
-            DevBuf<float> dinv;
-            dinv.upload(inv, stream);
-            ops16_[k].alloc(std::max<int64_t>(n, 1));
-            if (n)
-                hipLaunchKernelGGL(to_bf16_kernel, dim3(grid_for(n)), dim3(256), 0, stream,
-                                   ops64_[k].get(), dinv.get(), ops16_[k].get(), n);
-            T3_HIP_CHECK(hipStreamSynchronize(stream));
-        }
-    }
-    if (f32) {
-        x32_.alloc(size_);
-        y32_.alloc(size_);
-    }
     // Persistent grids: one workgroup per CU; global scratch only for the
     // workgroups of the global-scratch kernel.
     ncu_ = ncu;
@@ -1989,6 +2012,12 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream)
     T3_HIP_CHECK(hipGetLastError());
     T3_HIP_CHECK(hipStreamSynchronize(stream));
 }
+
+void HeffDevice::retune(const HeffInput& in, const HeffTuning& t, hipStream_t stream) {
+    tune_ = t;
+    build_lists(in, stream);
+}
+
 
 HeffDevice::~HeffDevice() { drop_graphs(); }
 

@@ -85,11 +85,27 @@ struct HeffStats {
     double stage_steps[3] = {0, 0, 0};
 };
 
+// Prices of the cost model that flattens the plan into work lists: they
+// decide contraction orders, the fused middle stage and chunking, hence the
+// kernels' work.  Defaults measured on MI355X; the autotuner
+// (t3ns_heff_autotune) picks per block-size histogram.
+struct HeffTuning {
+    double chunk_steps = 1000;  // a chunk of terms (barriers, memory latency), in matrix-core steps
+    double fuse_over = 2.0;     // fused operand work per fiber tile and k-step, in steps
+    int fuse = 1;               // fused middle stage allowed
+    int sched = 0;              // unit schedule: 0 cost, 1 block, 2 xcd, 3 xcdblock
+    // The defaults with T3NS_HEFF_CHUNK_STEPS, T3NS_HEFF_FUSE,
+    // T3NS_HEFF_FUSE_OVER and T3NS_HEFF_SCHED applied.
+    static HeffTuning from_env();
+};
+
 class HeffDevice {
 public:
     // Builds the plan tables and uploads the operator arenas (host data in
-    // in.ops[k]->tel).  `stream` may be null (default stream).
-    HeffDevice(const HeffInput& in, int precision, hipStream_t stream);
+    // in.ops[k]->tel).  `stream` may be null (default stream); `tuning`
+    // null: HeffTuning::from_env().
+    HeffDevice(const HeffInput& in, int precision, hipStream_t stream,
+               const HeffTuning* tuning = nullptr);
     ~HeffDevice();
     HeffDevice(const HeffDevice&) = delete;
     HeffDevice& operator=(const HeffDevice&) = delete;
@@ -108,8 +124,14 @@ public:
     bool graphs() const { return use_graphs_; }
     static constexpr size_t kMaxGraphs = 64;
     void diagonal(double* ddiag, hipStream_t s);
+    // Rebuilds the work lists under another tuning (`in` as given to the
+    // constructor; operator arenas are kept).  Results change by rounding
+    // only (contraction orders may differ).
+    void retune(const HeffInput& in, const HeffTuning& t, hipStream_t s);
+    const HeffTuning& tuning() const { return tune_; }
 
 private:
+    void build_lists(const HeffInput& in, hipStream_t stream);
     void enqueue(const double* dvec, double* dres, hipStream_t s);
     void drop_graphs();
     template <typename T, typename OT, int POOL>
@@ -117,6 +139,8 @@ private:
 
     HeffPlan plan_;
     HeffStats stats_;
+    HeffTuning tune_;
+    std::vector<std::vector<std::vector<float>>> bscale_;  // BF16 block scales [leg][op][block]
     int prec_;
     int isdmrg_;
     int map_[3];

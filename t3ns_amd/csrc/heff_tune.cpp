@@ -1,0 +1,141 @@
+// Autotuner of the effective-Hamiltonian work lists: the cost-model prices
+// that decide contraction orders, the fused middle stage and chunking
+// (HeffTuning) are chosen by timing real matvecs on the device, and the
+// choice is remembered per block-size histogram of the multisite object,
+// so later objects of the same shape (the same step in the next sweep, or
+// a sibling step) reuse it without timing.  No counterpart in the
+// reference, whose matvec has no tunable work decomposition.
+#include "heff_tune.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <string>
+
+namespace t3 {
+
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::string, HeffTuning>& tune_cache() {
+    static std::map<std::string, HeffTuning> m;
+    return m;
+}
+
+bool same(const HeffTuning& a, const HeffTuning& b) {
+    return a.chunk_steps == b.chunk_steps && a.fuse_over == b.fuse_over && a.fuse == b.fuse &&
+           a.sched == b.sched;
+}
+
+// Milliseconds of one matvec: one untimed, then the best of `reps`.
+double time_matvec(HeffDevice& dev, const double* x, double* y, hipStream_t s, int reps) {
+    hipEvent_t e0, e1;
+    T3_HIP_CHECK(hipEventCreate(&e0));
+    T3_HIP_CHECK(hipEventCreate(&e1));
+    dev.matvec(x, y, s);
+    double best = HUGE_VAL;
+    for (int r = 0; r < reps; ++r) {
+        T3_HIP_CHECK(hipEventRecord(e0, s));
+        dev.matvec(x, y, s);
+        T3_HIP_CHECK(hipEventRecord(e1, s));
+        T3_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        T3_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, (double)ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
+}  // namespace
+
+std::string heff_signature(const HeffDevice& dev) {
+    // Output blocks binned by log2 of their size; per bin, log2 of the
+    // count rounded to halves.  With the precision and the object kind.
+    const HeffPlan& P = dev.plan();
+    std::map<int, int64_t> hist;
+    for (int b = 0; b < P.nrblocks; ++b) {
+        const double sz = (double)P.dims[b][0] * P.dims[b][1] * P.dims[b][2];
+        hist[sz > 0 ? (int)std::floor(std::log2(sz)) : -1] += 1;
+    }
+    std::string key = "p" + std::to_string(dev.precision()) + (P.isdmrg ? "d" : "t");
+    for (const auto& kv : hist)
+        key += ":" + std::to_string(kv.first) + "=" +
+               std::to_string((int)std::lround(2 * std::log2((double)kv.second)));
+    return key;
+}
+
+TuneReport autotune_heff(HeffDevice& dev, const HeffInput& in, hipStream_t s, const double* x,
+                         double* y, int max_trials) {
+    TuneReport R;
+    R.key = heff_signature(dev);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = tune_cache().find(R.key);
+        if (it != tune_cache().end()) {
+            R.cached = true;
+            R.best = it->second;
+        }
+    }
+    if (R.cached) {
+        if (!same(R.best, dev.tuning())) dev.retune(in, R.best, s);
+        R.ms = time_matvec(dev, x, y, s, 1);
+        R.trials = 0;
+        return R;
+    }
+    // Coordinate search from the current tuning: the fused operand price,
+    // the chunk price, then the fused form off.
+    HeffTuning best = dev.tuning();
+    const int reps = 2;
+    double best_ms = time_matvec(dev, x, y, s, reps);
+    R.baseline_ms = best_ms;
+    int trials = 1;
+    HeffTuning cur = best;  // what the device lists are built for
+    auto trial = [&](const HeffTuning& t) {
+        if (trials >= max_trials || same(t, best)) return;
+        dev.retune(in, t, s);
+        cur = t;
+        ++trials;
+        const double ms = time_matvec(dev, x, y, s, reps);
+        if (ms < best_ms * 0.99) {
+            best_ms = ms;
+            best = t;
+        }
+    };
+    const HeffTuning base = best;
+    for (double fo : {base.fuse_over * 0.5, base.fuse_over * 2}) {
+        HeffTuning t = best;
+        t.fuse_over = fo;
+        trial(t);
+    }
+    for (double cs : {base.chunk_steps * 0.3, base.chunk_steps * 3}) {
+        HeffTuning t = best;
+        t.chunk_steps = cs;
+        trial(t);
+    }
+    {
+        HeffTuning t = best;
+        t.fuse = 0;
+        trial(t);
+    }
+    if (!same(cur, best)) dev.retune(in, best, s);
+    R.best = best;
+    R.ms = best_ms;
+    R.trials = trials;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_cache()[R.key] = best;
+    }
+    return R;
+}
+
+void clear_tune_cache() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_cache().clear();
+}
+
+}  // namespace t3

@@ -104,15 +104,19 @@ class DeviceSolver:
     env = {}
 
     def __init__(self, precision="f64", max_vecs=30, keep_deflate=2, max_its=300,
-                 mixed_min_size=200000):
+                 mixed_min_size=200000, autotune_min_size=500000):
         """mixed_min_size: below this many elements a step's Heff is
         launch-bound, FP32 matvecs save nothing and the refinement adds
-        matvecs, so "mixed" solves such steps in FP64."""
+        matvecs, so "mixed" solves such steps in FP64.  autotune_min_size:
+        steps of at least this many elements have their work lists
+        autotuned (Heff.autotune; shapes seen before reuse their setting);
+        None turns it off."""
         if precision not in ("f64", "mixed"):
             raise ValueError("precision is f64 or mixed")
         self.precision = precision
         self.max_vecs, self.keep, self.max_its = max_vecs, keep_deflate, max_its
         self.mixed_min_size = int(mixed_min_size)
+        self.autotune_min_size = autotune_min_size
         # steps solved with FP32 matvecs (plus FP64 refinement) and in FP64
         # alone, and the Davidson iterations of each kind
         self.counts = {"steps_f32": 0, "steps_f64": 0, "its_f32": 0, "its_f64": 0}
@@ -127,6 +131,10 @@ class DeviceSolver:
         t0 = time.perf_counter()
         h64 = Heff(d, o, s, FP64, keepalive=keep)
         self.timers["Heff: prepare data"] += time.perf_counter() - t0
+        if self.autotune_min_size is not None and h64.size >= self.autotune_min_size:
+            t0 = time.perf_counter()
+            h64.autotune()
+            self.timers["Heff: autotune"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         try:
             if self.precision == "f64" or h64.size < self.mixed_min_size:
@@ -138,6 +146,10 @@ class DeviceSolver:
             t1 = time.perf_counter()
             h32 = Heff(d, o, s, FP32, keepalive=keep)
             self.timers["Heff: prepare data"] += time.perf_counter() - t1
+            if self.autotune_min_size is not None and h32.size >= self.autotune_min_size:
+                t1 = time.perf_counter()
+                h32.autotune()
+                self.timers["Heff: autotune"] += time.perf_counter() - t1
             try:
                 t2 = time.perf_counter()
                 e, x, i1, i2, rc = davidson_refined(h32, h64, guess, self.max_vecs, self.keep, tol,
