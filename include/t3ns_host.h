@@ -131,6 +131,33 @@ int64_t t3ns_step_size(const t3ns_step* s);
 int t3ns_step_heff_desc(t3ns_step* s, t3ns_heff_desc* desc, t3ns_rops_desc* ops,
                         t3ns_site_desc* site);
 
+/* Checkpoints in the reference's HDF5 layout (T3NScalc.h5,
+ * src/io_to_disk.c:446-487): /network, /bookkeeper (the context's current
+ * bond sectors), /hamiltonian and /T3NS (one single-site tensor per network
+ * site: nrblocks[i] blocks, qnumbers[i][nrblocks[i]], beginblock[i]
+ * [nrblocks[i] + 1], tel[i][beginblock[i][nrblocks[i]]]); /rOps holds nrOps
+ * and no operators (a restored calculation rebuilds them from the
+ * tensors).  HDF5 is loaded at run time (libhdf5.so.103, or the path in
+ * T3NS_HDF5_LIB).  Replaces write_to_disk (src/io_to_disk.c:446-466). */
+int t3ns_ckpt_write(const t3ns_ctx* c, const char* path, int nsites, const int* nrblocks,
+                    const int64_t* const* qnumbers, const int64_t* const* beginblock,
+                    const double* const* tel);
+
+/* Reads a checkpoint (written here or by the reference: its rOperator
+ * groups are counted, not read).  Replaces read_from_disk
+ * (src/io_to_disk.c:468-487) with an object in place of the globals. */
+typedef struct t3ns_ckpt t3ns_ckpt;
+int t3ns_ckpt_open(t3ns_ckpt** k, const char* path);
+int t3ns_ckpt_info(const t3ns_ckpt* k, int* sites, int* nr_bonds, int* ham_type,
+                   int* nr_rops_groups);
+/* Site tensor `site`: pointers valid until t3ns_ckpt_close. */
+int t3ns_ckpt_site(const t3ns_ckpt* k, int site, int* nrsites, int* nrblocks,
+                   const int64_t** qnumbers, const int64_t** beginblock, const double** tel);
+/* A new context with the checkpoint's network, symmetries, Hamiltonian and
+ * bond sectors. */
+int t3ns_ckpt_context(const t3ns_ckpt* k, t3ns_ctx** c);
+int t3ns_ckpt_close(t3ns_ckpt* k);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,8 @@ ARCH = os.environ.get("T3NS_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 CXXFLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function"]
+# HDF5 headers for the checkpoint module (the library itself is dlopen'ed)
+HDF5_INC = os.environ.get("T3NS_HDF5_INCLUDE", "/opt/conda/include")
 # Packed FP32 ops disabled for all device code.
 HIPFLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -45,6 +47,10 @@ def compile_one(src, is_hip):
                                         "-c", src, "-o", obj]
     else:
         cmd = ["g++"] + CXXFLAGS + ["-c", src, "-o", obj]
+        with open(src) as fh:
+            if "<hdf5.h>" in fh.read() and os.path.exists(os.path.join(HDF5_INC, "hdf5.h")):
+                # declarations only: the HDF5 library is loaded at run time
+                cmd[1:1] = ["-I" + HDF5_INC]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "../../include/t3ns_host.h"
+#include "host/checkpoint.hpp"
 #include "host/context.hpp"
 #include "host/tensors.hpp"
 
@@ -1036,6 +1037,82 @@ int t3ns_step_heff_desc(t3ns_step* s, t3ns_heff_desc* d, t3ns_rops_desc* ops, t3
         site->qnumbers = T.qnumbers.data();
         site->beginblock = T.beginblock.data();
     });
+}
+
+int t3ns_ckpt_write(const t3ns_ctx* c, const char* path, int nsites, const int* nrblocks,
+                    const int64_t* const* qnumbers, const int64_t* const* beginblock,
+                    const double* const* tel) {
+    if (!c || !path || nsites < 0 || (nsites > 0 && (!nrblocks || !qnumbers || !beginblock || !tel)))
+        return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        if (nsites != c->c->netw.sites)
+            throw std::invalid_argument("checkpoint: one tensor per network site expected");
+        std::vector<SiteBlocks> ts(nsites);
+        for (int i = 0; i < nsites; ++i) {
+            const int nb = nrblocks[i];
+            if (nb < 0 || (nb > 0 && (!qnumbers[i] || !beginblock[i])))
+                throw std::invalid_argument("checkpoint: bad site tensor " + std::to_string(i));
+            SiteBlocks& t = ts[i];
+            t.sites = {i};
+            t.qnumbers.assign(qnumbers[i], qnumbers[i] + nb);
+            t.beginblock.assign(1, 0);
+            if (nb > 0) t.beginblock.assign(beginblock[i], beginblock[i] + nb + 1);
+            const int64_t n = t.beginblock.back();
+            if (n < 0) throw std::invalid_argument("checkpoint: bad block offsets");
+            if (n > 0 && !tel[i]) throw std::invalid_argument("checkpoint: missing elements");
+            t.tel.assign(tel[i], tel[i] + n);
+        }
+        write_checkpoint(path, *c->c, ts);
+    });
+}
+
+struct t3ns_ckpt {
+    CheckpointData d;
+};
+
+int t3ns_ckpt_open(t3ns_ckpt** k, const char* path) {
+    if (!k || !path) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        auto p = std::make_unique<t3ns_ckpt>();
+        p->d = read_checkpoint(path);
+        *k = p.release();
+    });
+}
+
+int t3ns_ckpt_info(const t3ns_ckpt* k, int* sites, int* nr_bonds, int* ham_type,
+                   int* nr_rops_groups) {
+    if (!k) return T3NS_ERR_INVALID;
+    if (sites) *sites = k->d.netw.sites;
+    if (nr_bonds) *nr_bonds = k->d.netw.nr_bonds;
+    if (ham_type) *ham_type = k->d.ham.type;
+    if (nr_rops_groups) *nr_rops_groups = k->d.nr_rops_groups;
+    return T3NS_OK;
+}
+
+int t3ns_ckpt_site(const t3ns_ckpt* k, int site, int* nrsites, int* nrblocks,
+                   const int64_t** qnumbers, const int64_t** beginblock, const double** tel) {
+    if (!k || site < 0 || site >= (int)k->d.tensors.size()) return T3NS_ERR_INVALID;
+    const SiteBlocks& t = k->d.tensors[site];
+    if (nrsites) *nrsites = (int)t.sites.size();
+    if (nrblocks) *nrblocks = t.nrblocks();
+    if (qnumbers) *qnumbers = t.qnumbers.data();
+    if (beginblock) *beginblock = t.beginblock.data();
+    if (tel) *tel = t.tel.data();
+    return T3NS_OK;
+}
+
+int t3ns_ckpt_context(const t3ns_ckpt* k, t3ns_ctx** out) {
+    if (!k || !out) return T3NS_ERR_INVALID;
+    return guarded_host([&] {
+        auto ctx = std::make_unique<t3ns_ctx>();
+        ctx->c = context_from_checkpoint(k->d);
+        *out = ctx.release();
+    });
+}
+
+int t3ns_ckpt_close(t3ns_ckpt* k) {
+    delete k;
+    return T3NS_OK;
 }
 
 }  // extern "C"

@@ -17,6 +17,14 @@ const int kMpoSU2[3][2] = {{-1, 1}, {0, 0}, {1, 1}};
 
 class Hubbard final : public Model {
 public:
+    HamData ham_data() const override {
+        HamData d;
+        d.type = HAM_NN_HUBBARD;
+        d.su2 = su2_;
+        d.t = t_;
+        d.U = U_;
+        return d;
+    }
     Hubbard(double t, double U, bool su2, bool quirk) : t_(t), U_(U), su2_(su2), quirk_(quirk) {
         const int n = su2_ ? 3 : 5;
         mpo_.bond = -1;

@@ -38,10 +38,27 @@ struct InstructionSet {  // include/instructions.h:36-66
     int nr_instr() const { return (int)instr.size(); }
 };
 
+// Hamiltonian data as the reference checkpoints it (src/hamiltonian.c:
+// 327-345, src/hamiltonian_qc.c:640-654, src/hamiltonian_nn_hubbard.c:
+// 261-270).
+struct HamData {
+    int type = 0;                // enum hamtypes (include/hamiltonian.h:30): 1 QC, 2 NN_HUBBARD
+    int su2 = 0;
+    int norb = 0;                // QC
+    std::vector<int> orbirrep;   // QC: psi4 irrep of every orbital
+    std::vector<double> Vijkl;   // QC: norb^4, the one-body part folded in
+    double core_energy = 0;      // QC
+    int has_seniority = 0;       // QC
+    double t = 0, U = 0;         // NN_HUBBARD
+};
+enum HamType { HAM_QC = 1, HAM_NN_HUBBARD = 2 };
+
 class Model {
 public:
     virtual ~Model() = default;
     virtual std::string name() const = 0;
+    // What a checkpoint stores of the Hamiltonian.
+    virtual HamData ham_data() const = 0;
     virtual Symsecs phys_symsecs(int orbital) const = 0;
     virtual const Symsecs& mpo_symsecs() const = 0;
     int nr_hss() const { return mpo_symsecs().nrSecs; }
@@ -75,5 +92,9 @@ std::unique_ptr<Model> make_hubbard_model(double t, double U, bool su2,
 // src/opType.c, src/opType_qc.c, src/instructions_qc.c).  su2 selects U1xSU2,
 // otherwise U1xU1; a point group is taken from the symmetry list at attach().
 std::unique_ptr<Model> make_qc_model(const std::string& fcidump_text, bool su2);
+
+// The model a checkpoint describes (QC from its stored integrals, which
+// already hold the folded one-body part; NN_HUBBARD from t, U, su2).
+std::unique_ptr<Model> make_model_from_data(const HamData& d);
 
 }  // namespace t3

@@ -87,7 +87,20 @@ Network Network::from_file(const std::string& path) {
 
 Network Network::from_bonds(int nsites, const std::vector<int>& s2o,
                             const std::vector<std::array<int, 2>>& bonds) {
+    return from_bonds(nsites, s2o, bonds, {});
+}
+
+Network Network::from_bonds(int nsites, const std::vector<int>& s2o,
+                            const std::vector<std::array<int, 2>>& bonds,
+                            const std::vector<int>& sweep) {
     Network n;
+    if ((int)s2o.size() != nsites) throw std::runtime_error("network: sitetoorb of the wrong length");
+    for (const auto& b : bonds)
+        for (int k = 0; k < 2; ++k)
+            if (b[k] < -1 || b[k] >= nsites) throw std::runtime_error("network: illegal site in bond list");
+    for (int v : sweep)
+        if (v < 0 || v >= (int)bonds.size()) throw std::runtime_error("network: illegal bond in sweep");
+    n.sweep = sweep;
     n.sites = nsites;
     n.sitetoorb = s2o;
     n.psites = 0;

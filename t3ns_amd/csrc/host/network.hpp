@@ -39,6 +39,10 @@ public:
     // Builds a network from an explicit bond list and site->orbital map.
     static Network from_bonds(int nsites, const std::vector<int>& sitetoorb,
                               const std::vector<std::array<int, 2>>& bonds);
+    // The same with a given sweep order (empty: the default simple sweep).
+    static Network from_bonds(int nsites, const std::vector<int>& sitetoorb,
+                              const std::vector<std::array<int, 2>>& bonds,
+                              const std::vector<int>& sweep);
 
     bool is_psite(int site) const { return sitetoorb[site] >= 0; }
     bool is_pbond(int bond) const { return bond >= 2 * nr_bonds; }

@@ -79,7 +79,30 @@ struct OpType {
 class QC final : public Model {
 public:
     QC(const std::string& text, bool su2) : su2_(su2) { read_fcidump(text); }
+    // From checkpointed integrals (one-body part already folded in,
+    // orbital irreps in psi4 order).
+    explicit QC(const HamData& d) : su2_(d.su2 != 0), restored_(true) {
+        norb_ = d.norb;
+        orbirrep_ = d.orbirrep;
+        V_ = d.Vijkl;
+        core_ = d.core_energy;
+        if (norb_ <= 0 || (int)orbirrep_.size() != norb_ ||
+            V_.size() != (size_t)norb_ * norb_ * norb_ * norb_)
+            throw std::runtime_error("QC: checkpointed integrals of the wrong size");
+        if (d.has_seniority) throw std::runtime_error("QC: seniority symmetry is not supported");
+    }
     std::string name() const override { return "QC"; }
+    HamData ham_data() const override {
+        HamData d;
+        d.type = HAM_QC;
+        d.su2 = su2_;
+        d.norb = norb_;
+        d.orbirrep = orbirrep_;
+        d.Vijkl = V_;
+        d.core_energy = core_;
+        d.has_seniority = 0;
+        return d;
+    }
 
     void attach(const Network& n, const SymInfo& s) override {
         netw_ = &n;
@@ -90,6 +113,11 @@ public:
         npg_ = pg_ == -1 ? 1 : pg_nr_irreps(pg_);
         if (norb_ != n.psites)
             throw std::runtime_error("FCIDUMP orbital count differs from the network's physical sites");
+        if (restored_) {  // integrals and irreps as checkpointed
+            prepare_mpo();
+            init_optype_array();
+            return;
+        }
         orbirrep_.assign(norb_, 0);
         if (pg_ != -1) {
             if ((int)orbsym_.size() != norb_) throw std::runtime_error("FCIDUMP lacks ORBSYM");
@@ -224,6 +252,7 @@ public:
 
 private:
     bool su2_;
+    bool restored_ = false;
     int norb_ = 0;
     std::vector<int> orbsym_, orbirrep_;
     double core_ = 0;
@@ -902,6 +931,13 @@ private:
 
 std::unique_ptr<Model> make_qc_model(const std::string& text, bool su2) {
     return std::unique_ptr<Model>(new QC(text, su2));
+}
+
+std::unique_ptr<Model> make_model_from_data(const HamData& d) {
+    if (d.type == HAM_QC) return std::unique_ptr<Model>(new QC(d));
+    if (d.type == HAM_NN_HUBBARD) return make_hubbard_model(d.t, d.U, d.su2 != 0);
+    throw std::runtime_error("checkpoint: Hamiltonian type " + std::to_string(d.type) +
+                             " is not supported");
 }
 
 }  // namespace t3

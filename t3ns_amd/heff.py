@@ -44,6 +44,12 @@ class Context:
                   "t3ns_ctx_create")
         self._h = h
 
+    @classmethod
+    def _from_handle(cls, h):
+        obj = cls.__new__(cls)
+        obj._h = h
+        return obj
+
     @property
     def handle(self):
         return self._h

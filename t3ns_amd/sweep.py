@@ -180,13 +180,15 @@ class Sweeper:
     device calls by default)."""
 
     def __init__(self, ctx, max_dim, seed=0, solver=None, tol=1e-8, truncerr=0.0, min_dim=0,
-                 site_size=2, start="seeded"):
+                 site_size=2, start="seeded", tensors=None):
         """start: "seeded" -- every site tensor from the context's seeded
         fill, the ones outside the first step replaced by random isometries
         toward it; "normal" -- every block entry N(0, 1) from one PCG64
         generator seeded `seed`, in site order then block order, the sites
         outside the first step then made isometries toward it by a QR of
-        their own entries per sector."""
+        their own entries per sector.  tensors: {site: SiteData} to start
+        from instead (a state in canonical form toward the first step, such
+        as a checkpoint written after whole sweeps)."""
         if start not in ("seeded", "normal"):
             raise ValueError("start is seeded or normal")
         info = ctx.info()
@@ -199,6 +201,10 @@ class Sweeper:
         rng = np.random.default_rng(seed)
         self.tensors = {}
         for s in range(info["sites"]):
+            if tensors is not None:
+                t = tensors[s]
+                self.tensors[s] = SiteData(t.qn.copy(), t.bb.copy(), t.tel.copy())
+                continue
             data = SiteData(*ctx.site_tensor(s, seed * 7919 + s + 1))
             if start == "normal":
                 data.tel[:] = rng.standard_normal(len(data.tel))
@@ -216,6 +222,22 @@ class Sweeper:
         # (src/optimize_network.c:43-58); the solve covers Heff set-up and
         # Davidson (the solver may split it further)
         self.timers = Counter()
+
+    def save(self, path):
+        """Checkpoint of the current state (network, bond sectors,
+        Hamiltonian, site tensors) in the reference's HDF5 layout
+        (t3ns_amd.checkpoint); written between sweeps it restores to the
+        same calculation."""
+        from .checkpoint import write_checkpoint
+        write_checkpoint(path, self.ctx, self.tensors)
+
+    @classmethod
+    def restore(cls, path, max_dim, **kw):
+        """A Sweeper continuing from a checkpoint written by save() (or by
+        the reference's write_to_disk after whole sweeps)."""
+        from .checkpoint import read_checkpoint
+        ctx, tensors, _ = read_checkpoint(path)
+        return cls(ctx, max_dim, tensors=tensors, **kw)
 
     def _legs_toward(self, sites, nr_bonds):
         """For every site outside `sites`: the leg of its bond on the path to
