@@ -102,6 +102,45 @@ static void jacobi(double* B, int m, int n, double* V)
         }
 }
 
+/* Rows t of VT (k x n, row-major over the stride k) whose singular value is
+ * at most 1e-12 of the largest carry no direction (the rotated column they
+ * come from is rounding noise, or zero); dgesdd still returns orthonormal
+ * rows there.  They are replaced, in order, by the first unit vector e_c
+ * (c = 0, 1, ...) that keeps at least half its norm after two
+ * Gram-Schmidt passes against every other good or completed row. */
+static void complete_rows(double* VT, const double* S, int k, int n)
+{
+        double smax = 0;
+        for (int t = 0; t < k; ++t) smax = S[t] > smax ? S[t] : smax;
+        const double thr = 1e-12 * smax;
+        char* ok = malloc(k + 1);
+        for (int t = 0; t < k; ++t) ok[t] = S[t] > thr;
+        double* v = malloc(sizeof(double) * (n + 1));
+        for (int t = 0; t < k; ++t) {
+                if (ok[t]) continue;
+                for (int c = 0; c < n; ++c) {
+                        for (int i = 0; i < n; ++i) v[i] = i == c;
+                        for (int pass = 0; pass < 2; ++pass)
+                                for (int r = 0; r < k; ++r) {
+                                        if (!ok[r]) continue;
+                                        double d = 0;
+                                        for (int i = 0; i < n; ++i) d += VT[r + (size_t) i * k] * v[i];
+                                        for (int i = 0; i < n; ++i) v[i] -= d * VT[r + (size_t) i * k];
+                                }
+                        double nr = 0;
+                        for (int i = 0; i < n; ++i) nr += v[i] * v[i];
+                        nr = sqrt(nr);
+                        if (nr > 0.5) {
+                                for (int i = 0; i < n; ++i) VT[t + (size_t) i * k] = v[i] / nr;
+                                ok[t] = 1;
+                                break;
+                        }
+                }
+        }
+        free(v);
+        free(ok);
+}
+
 /* dgesdd 'S' semantics: US = U * diag(S) (m x k), VT (k x n), S descending, k = min(m, n) */
 static void svd(const double* A, int m, int n, double* S, double* US, double* VT)
 {
@@ -141,6 +180,7 @@ static void svd(const double* A, int m, int n, double* S, double* US, double* VT
                                 VT[t + (size_t) i * nn] = sv[j] > 0 ? B[i + (size_t) j * n] / sv[j] : 0;
                 }
         }
+        if (tr) complete_rows(VT, S, nn, n);
         free(B);
         free(V);
         free(sv);

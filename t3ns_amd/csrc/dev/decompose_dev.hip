@@ -348,6 +348,70 @@ SplitResult run_split(const SplitInput& in, const SplitPlan& P, const std::vecto
     for (int i = 0; i < nU; ++i) ubb[i + 1] = ubb[i] + usz[i];
     for (int i = 0; i < nV; ++i) vbb[i + 1] = vbb[i] + vsz[i];
 
+    // Null directions of transposed sectors: a kept singular value at most
+    // 1e-12 of the sector's largest has no direction of its own (its rotated
+    // column is rounding noise or zero, and dividing by the value would not
+    // give a unit vector).  Such columns are replaced, in singular-value
+    // order, by the first unit vector e_c that keeps at least half its norm
+    // after two Gram-Schmidt passes against the sector's other good or
+    // completed columns (normalised), so V keeps orthonormal rows as
+    // dgesdd's VT does.  Rare: only sectors with such values kept are
+    // brought to the host.
+    std::vector<std::vector<char>> unit(nsec);
+    for (int ss = 0; ss < nsec; ++ss) {
+        const int mi = mat_of[ss];
+        if (mi < 0 || !trans[ss] || R.kept[ss] == 0) continue;
+        const std::vector<double>& sg = R.sing[ss];
+        const double thr = 1e-12 * (sg.empty() ? 0.0 : sg[0]);
+        const int n = mats[mi].n, m = mats[mi].m;
+        bool need = false;
+        for (int t = 0; t < R.kept[ss]; ++t) need = need || sg[t] <= thr;
+        if (!need) continue;
+        std::vector<double> B((size_t)m * n);
+        T3_HIP_CHECK(hipMemcpyAsync(B.data(), arena.get() + mats[mi].b, B.size() * sizeof(double),
+                                    hipMemcpyDeviceToHost, s));
+        T3_HIP_CHECK(hipStreamSynchronize(s));
+        std::vector<char> ok(n, 0);
+        std::vector<std::vector<double>> rows(n);  // unit rows by singular-value rank
+        for (int t = 0; t < n; ++t)
+            if (sg[t] > thr) {
+                ok[t] = 1;
+                const double* c = B.data() + (size_t)order[ss][t] * m;
+                rows[t].assign(c, c + m);
+                for (double& x : rows[t]) x /= sg[t];
+            }
+        unit[ss].assign(R.kept[ss], 0);
+        std::vector<double> v(m);
+        for (int t = 0; t < R.kept[ss]; ++t) {
+            if (ok[t]) continue;
+            for (int c = 0; c < m; ++c) {
+                std::fill(v.begin(), v.end(), 0.0);
+                v[c] = 1.0;
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int r = 0; r < n; ++r) {
+                        if (!ok[r]) continue;
+                        double d = 0;
+                        for (int i = 0; i < m; ++i) d += rows[r][i] * v[i];
+                        for (int i = 0; i < m; ++i) v[i] -= d * rows[r][i];
+                    }
+                double nr = 0;
+                for (double x : v) nr += x * x;
+                nr = std::sqrt(nr);
+                if (nr > 0.5) {
+                    for (double& x : v) x /= nr;
+                    rows[t] = v;
+                    ok[t] = 1;
+                    unit[ss][t] = 1;
+                    std::copy(v.begin(), v.end(), B.begin() + (size_t)order[ss][t] * m);
+                    break;
+                }
+            }
+        }
+        T3_HIP_CHECK(hipMemcpyAsync(arena.get() + mats[mi].b, B.data(), B.size() * sizeof(double),
+                                    hipMemcpyHostToDevice, s));
+        T3_HIP_CHECK(hipStreamSynchronize(s));
+    }
+
     // scatter: U gets the singular vectors times the singular values, V the
     // right singular vectors (SVD_copy_from_mem)
     std::vector<int> perm;
@@ -362,7 +426,8 @@ SplitResult run_split(const SplitInput& in, const SplitPlan& P, const std::vecto
             // rows >= columns: B holds U * Sigma, V the right vectors;
             // transposed: B holds (right vectors) * Sigma, V the left vectors
             uscale.push_back(trans[ss] ? sg : 1.0);
-            vscale.push_back(trans[ss] ? (sg > 0 ? 1.0 / sg : 0.0) : 1.0);
+            const bool u1 = !unit[ss].empty() && unit[ss][t];  // completed: already a unit vector
+            vscale.push_back(trans[ss] ? (u1 ? 1.0 : (sg > 0 ? 1.0 / sg : 0.0)) : 1.0);
         }
     }
     const Symsecs* sV = in.ss[P.id_siteV];

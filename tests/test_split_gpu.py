@@ -100,3 +100,19 @@ def test_split_qc24_d512(gpu, sv):
         ctx = T.Context(f.read(), "QC", [], ["U1", "SU2"], [24, 0], 512, fcidump=fci)
     st = T.Step(ctx, [18, 31, 17], seed=1)
     _compare(st.split_problem(sv, maxD=512))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [c for c in CASES if c[0] == "qc"][:4], ids=_cid)
+def test_split_null_singular_values_match_oracle(gpu, case):
+    """Kept null singular values: the device completes V's rows as the
+    oracle does (orthonormal rows, checked on the oracle in
+    test_split_oracle.py)."""
+    from test_split_oracle import keep_first_block
+    ctx, st = _step(case)
+    d0, o, s = st.descriptors()
+    for sv in splittable(st, s.nrsites):
+        print("case", _cid(case), "site", sv, "null values", flush=True)
+        d = st.split_problem(sv, maxD=100000)
+        keep_first_block(d)
+        _compare(d)

@@ -129,82 +129,117 @@ def test_split_reconstructs_and_is_an_svd(case):
         d = st.split_problem(sv, maxD=100000)
         r = O.split_site(d)
         assert int(r["kept"].sum()) == int(r["nsing"].sum()) or np.min(r["sing"]) < 1e-300
-        S = Split(d, r)
-        # reconstruction: A[a, v, c] = norm_U * sum_k U[a, k, c] V[x, k, y]
-        for qn, blk in _a_blocks(d):
-            tup = []
-            for i in S.incl:
-                tup.append(S.renumber(i, qn[i])[0] if i == S.cs else qn[i])
-            qv, sec = S.renumber(sv, qn[sv])
-            k = r["bond_dims"][S.newid[sec]] if S.newid[sec] >= 0 else 0
-            if k == 0:
-                assert np.max(np.abs(blk)) < 1e-12 if blk.size else True
-                continue
-            U, V = S.ublock(tup), S.vblock(qv)
-            assert U is not None and V is not None
-            # U's legs before its cut leg (U's site and leg order)
-            t0 = 1
-            for i in S.incl:
-                di, _ = S.dims(i, qn[i], cutdim=1)
-                if i < S.cs:
-                    t0 *= int(np.prod(di))
-                elif i == S.cs:
-                    t0 *= int(np.prod(di[:S.cb])) if S.cb else 1
-            vd, _ = S.dims(sv, qn[sv], cutdim=1)
-            v0 = int(np.prod(vd[:S.vb])) if S.vb else 1
-            v2 = int(np.prod(vd[S.vb + 1:]))
-            t2 = U.size // (t0 * k)
-            # rows: U's non-cut legs in order; columns: V's legs
-            Ur = np.moveaxis(U.reshape((t0, k, t2), order="F"), 1, 2).reshape((t0 * t2, k), order="F")
-            Vc = np.moveaxis(V.reshape((v0, k, v2), order="F"), 1, 2).reshape((v0 * v2, k), order="F")
-            rc = r["norm_U"] * Ur @ Vc.T
-            # A's block: (sites before V, V, sites after V), column-major
-            a0 = int(np.prod([np.prod(S.dims(i, qn[i], cutdim=1)[0]) for i in S.incl if i < sv] or [1]))
-            a2 = rc.shape[0] // a0
-            rec = np.moveaxis(rc.reshape((a0, a2, v0 * v2), order="F"), 2, 1)
-            np.testing.assert_allclose(rec.flatten(order="F"), blk, rtol=0, atol=1e-12)
-        # V orthonormal rows and U orthogonal columns per kept sector
-        gram_v, gram_u = {}, {}
-        for b in range(len(r["V_beginblock"]) - 1):
-            q = r["V_qnumbers"][b]
-            ids = _ids(q, [d.ss[sv][kk].nrSecs if kk != S.vb else len(S.bond_irr) for kk in range(3)])
-            sec = ids[S.vb]
-            k = r["bond_dims"][sec]
-            V = r["V_tel"][r["V_beginblock"][b]:r["V_beginblock"][b + 1]]
-            x = V.size // k
-            v0 = int(np.prod([d.ss[sv][kk].dims[ids[kk]] for kk in range(S.vb)])) if S.vb else 1
-            Vm = np.moveaxis(V.reshape((v0, k, x // v0), order="F"), 1, 0).reshape(k, -1)
-            gram_v[sec] = gram_v.get(sec, 0) + Vm @ Vm.T
-        for sec, g in gram_v.items():
-            np.testing.assert_allclose(g, np.eye(g.shape[0]), rtol=0, atol=1e-11)
-        nu = r["U_nrsites"]
-        uq = r["U_qnumbers"].reshape(-1, nu)
-        ci = S.incl.index(S.cs)
-        for b in range(len(r["U_beginblock"]) - 1):
-            q = int(uq[b][ci])
-            ns_ = [d.ss[S.cs][kk].nrSecs if kk != S.cb else len(S.bond_irr) for kk in range(3)]
-            ids = _ids(q, ns_)
-            sec = ids[S.cb]
-            k = r["bond_dims"][sec]
-            U = r["U_tel"][r["U_beginblock"][b]:r["U_beginblock"][b + 1]]
-            t0 = 1
-            for j, i in enumerate(S.incl):
-                di = [d.ss[i][kk].dims[x] for kk, x in enumerate(_ids(int(uq[b][j]), ns_ if i == S.cs else
-                                                                        S.nsec(i)))]
-                if i < S.cs:
-                    t0 *= int(np.prod(di))
-                elif i == S.cs:
-                    t0 *= int(np.prod(di[:S.cb])) if S.cb else 1
-            Ur = np.moveaxis(U.reshape((t0, k, U.size // (t0 * k)), order="F"), 1, 2).reshape((-1, k), order="F")
-            gram_u[sec] = gram_u.get(sec, 0) + Ur.T @ Ur
-        offs = np.concatenate([[0], np.cumsum(r["nsing"])])
-        for new, g in gram_u.items():
-            old_sec = [s_ for s_, n_ in S.newid.items() if n_ == new][0]
-            sv_ = r["sing"][offs[old_sec]:offs[old_sec] + r["kept"][old_sec]] / r["norm_U"]
-            np.testing.assert_allclose(g, np.diag(sv_ ** 2), rtol=0, atol=1e-12)
-        for s_ in range(len(r["nsing"])):
-            assert np.all(np.diff(r["sing"][offs[s_]:offs[s_ + 1]]) <= 0)
-        # truncation keeps the largest values; the norm after is their weight
+        check_split(d, r)
+
+
+def keep_first_block(d):
+    """Zeroes every block of the split problem's tensor but the first: most
+    sector matrices become zero and their singular values null."""
+    n = d.beginblock[d.nrblocks]
+    A = np.ctypeslib.as_array(d.tel, shape=(n,))
+    A[d.beginblock[1]:] = 0
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c[0] == "qc"][:4], ids=_cid)
+def test_split_null_singular_values_keep_v_orthonormal(case):
+    """Kept null singular values (minD above the rank, as in the reference's
+    LiF regimes with minD = maxD): V still has orthonormal rows per sector,
+    as dgesdd's VT does (src/siteTensor_decompose.c:1060)."""
+    ctx, st = _step(case)
+    d0, o, s = st.descriptors()
+    nnull = 0
+    for sv in splittable(st, s.nrsites):
+        d = st.split_problem(sv, maxD=100000)
+        keep_first_block(d)
+        r = O.split_site(d)
+        assert int(r["kept"].sum()) == int(r["nsing"].sum())
+        nnull += int(np.sum(r["sing"] == 0))
+        check_split(d, r, null_ok=True)
+    assert nnull > 0
+
+
+def check_split(d, r, null_ok=False):
+    """Reconstruction of A from U and V, orthonormal rows of V and
+    orthogonal columns of U with the singular values as norms, per kept
+    sector, descending values, the reported norm."""
+    sv = d.siteV
+    S = Split(d, r)
+    # reconstruction: A[a, v, c] = norm_U * sum_k U[a, k, c] V[x, k, y]
+    for qn, blk in _a_blocks(d):
+        tup = []
+        for i in S.incl:
+            tup.append(S.renumber(i, qn[i])[0] if i == S.cs else qn[i])
+        qv, sec = S.renumber(sv, qn[sv])
+        k = r["bond_dims"][S.newid[sec]] if S.newid[sec] >= 0 else 0
+        if k == 0:
+            assert np.max(np.abs(blk)) < 1e-12 if blk.size else True
+            continue
+        U, V = S.ublock(tup), S.vblock(qv)
+        assert U is not None and V is not None
+        # U's legs before its cut leg (U's site and leg order)
+        t0 = 1
+        for i in S.incl:
+            di, _ = S.dims(i, qn[i], cutdim=1)
+            if i < S.cs:
+                t0 *= int(np.prod(di))
+            elif i == S.cs:
+                t0 *= int(np.prod(di[:S.cb])) if S.cb else 1
+        vd, _ = S.dims(sv, qn[sv], cutdim=1)
+        v0 = int(np.prod(vd[:S.vb])) if S.vb else 1
+        v2 = int(np.prod(vd[S.vb + 1:]))
+        t2 = U.size // (t0 * k)
+        # rows: U's non-cut legs in order; columns: V's legs
+        Ur = np.moveaxis(U.reshape((t0, k, t2), order="F"), 1, 2).reshape((t0 * t2, k), order="F")
+        Vc = np.moveaxis(V.reshape((v0, k, v2), order="F"), 1, 2).reshape((v0 * v2, k), order="F")
+        rc = r["norm_U"] * Ur @ Vc.T
+        # A's block: (sites before V, V, sites after V), column-major
+        a0 = int(np.prod([np.prod(S.dims(i, qn[i], cutdim=1)[0]) for i in S.incl if i < sv] or [1]))
+        a2 = rc.shape[0] // a0
+        rec = np.moveaxis(rc.reshape((a0, a2, v0 * v2), order="F"), 2, 1)
+        np.testing.assert_allclose(rec.flatten(order="F"), blk, rtol=0, atol=1e-12)
+    # V orthonormal rows and U orthogonal columns per kept sector
+    gram_v, gram_u = {}, {}
+    for b in range(len(r["V_beginblock"]) - 1):
+        q = r["V_qnumbers"][b]
+        ids = _ids(q, [d.ss[sv][kk].nrSecs if kk != S.vb else len(S.bond_irr) for kk in range(3)])
+        sec = ids[S.vb]
+        k = r["bond_dims"][sec]
+        V = r["V_tel"][r["V_beginblock"][b]:r["V_beginblock"][b + 1]]
+        x = V.size // k
+        v0 = int(np.prod([d.ss[sv][kk].dims[ids[kk]] for kk in range(S.vb)])) if S.vb else 1
+        Vm = np.moveaxis(V.reshape((v0, k, x // v0), order="F"), 1, 0).reshape(k, -1)
+        gram_v[sec] = gram_v.get(sec, 0) + Vm @ Vm.T
+    for sec, g in gram_v.items():
+        np.testing.assert_allclose(g, np.eye(g.shape[0]), rtol=0, atol=1e-11)
+    nu = r["U_nrsites"]
+    uq = r["U_qnumbers"].reshape(-1, nu)
+    ci = S.incl.index(S.cs)
+    for b in range(len(r["U_beginblock"]) - 1):
+        q = int(uq[b][ci])
+        ns_ = [d.ss[S.cs][kk].nrSecs if kk != S.cb else len(S.bond_irr) for kk in range(3)]
+        ids = _ids(q, ns_)
+        sec = ids[S.cb]
+        k = r["bond_dims"][sec]
+        U = r["U_tel"][r["U_beginblock"][b]:r["U_beginblock"][b + 1]]
+        t0 = 1
+        for j, i in enumerate(S.incl):
+            di = [d.ss[i][kk].dims[x] for kk, x in enumerate(_ids(int(uq[b][j]), ns_ if i == S.cs else
+                                                                    S.nsec(i)))]
+            if i < S.cs:
+                t0 *= int(np.prod(di))
+            elif i == S.cs:
+                t0 *= int(np.prod(di[:S.cb])) if S.cb else 1
+        Ur = np.moveaxis(U.reshape((t0, k, U.size // (t0 * k)), order="F"), 1, 2).reshape((-1, k), order="F")
+        gram_u[sec] = gram_u.get(sec, 0) + Ur.T @ Ur
+    offs = np.concatenate([[0], np.cumsum(r["nsing"])])
+    for new, g in gram_u.items():
+        old_sec = [s_ for s_, n_ in S.newid.items() if n_ == new][0]
+        sv_ = r["sing"][offs[old_sec]:offs[old_sec] + r["kept"][old_sec]] / r["norm_U"]
+        np.testing.assert_allclose(g, np.diag(sv_ ** 2), rtol=0, atol=1e-12)
+    for s_ in range(len(r["nsing"])):
+        assert np.all(np.diff(r["sing"][offs[s_]:offs[s_ + 1]]) <= 0)
+    # truncation keeps the largest values; the norm after is their weight
+    if not null_ok:
         assert abs(r["norm"][0] - 1) < 1e-10
 
 
