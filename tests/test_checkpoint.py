@@ -164,3 +164,18 @@ def test_restored_sweep_continues_exactly(tmp_path, model):
     c = Sweeper.restore(path, D, seed=1, solver=OracleSolver())
     c.sweep()
     assert abs(c.energies[-1] - a.energies[-1]) < 1e-10, (c.energies, a.energies)
+
+
+@pytest.mark.gpu
+def test_restored_device_sweep_continues_exactly(gpu, tmp_path):
+    """The same on the device (fixed-order sums: bitwise reproducible)."""
+    print("case hubbard 10 chain D16 device sweep restore", flush=True)
+    a = Sweeper(_hubbard_ctx(), 16, seed=1)
+    a.run(2)
+    b = Sweeper(_hubbard_ctx(), 16, seed=1)
+    b.sweep()
+    path = tmp_path / "dev.h5"
+    b.save(path)
+    c = Sweeper.restore(path, 16, seed=1)
+    c.sweep()
+    assert abs(c.energies[-1] - a.energies[-1]) < 1e-12, (c.energies, a.energies)
