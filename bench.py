@@ -88,6 +88,8 @@ def parse():
                     help="skip the DMRG sweep of the 10-site Hubbard chain")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the served-solve stream")
+    ap.add_argument("--qc24-sweep", type=int, default=0, metavar="D",
+                    help="also run one two-site sweep of the QC-24 system at bond dimension D")
     ap.add_argument("--serve-requests", type=int, default=12,
                     help="requests of the served-solve stream (the first ones of the seed-3 stream)")
     ap.add_argument("--serve-rate", type=float, default=4.0,
@@ -361,6 +363,14 @@ def measure_solve(ctx, step, tol):
             "abs_dE": abs(em - e64), "speedup_vs_fp64": t64 / tm}
 
 
+def sweep_phases(dm, solver):
+    """Seconds per phase of a sweep run under the reference's timer names
+    (src/optimize_network.c:43-58): the sweep loop's and the solver's."""
+    ph = dict(dm.timers)
+    ph.update(solver.timers)
+    return {k: round(v, 4) for k, v in sorted(ph.items())}
+
+
 def measure_sweep(sweeps=4, e_conv=1e-10):
     """Two-site DMRG of the NN_HUBBARD(t = 1, U = 4) 10-site chain
     (makenetwork.py 10 DMRG), 10 electrons (U1 x U1: 5 up, 5 down), D = 64,
@@ -386,7 +396,7 @@ def measure_sweep(sweeps=4, e_conv=1e-10):
         t = time.perf_counter() - t0
         out[prec] = {"energy": e, "sweeps": len(dm.energies), "seconds": t,
                      "seconds_per_sweep": t / len(dm.energies), "sweep_energies": dm.energies,
-                     "counts": dict(solver.counts)}
+                     "counts": dict(solver.counts), "phases": sweep_phases(dm, solver)}
         print("# sweep %s: E %.12f in %d sweeps, %.2f s %s" % (prec, e, len(dm.energies), t,
                                                             solver.counts),
               file=sys.stderr, flush=True)
@@ -426,9 +436,38 @@ def measure_lif_sweep():
         out[prec] = {"energy": e, "abs_err_vs_reference": abs(e - out["reference_energy"]),
                      "sweeps": len(dm.energies), "seconds": t,
                      "seconds_per_sweep": t / len(dm.energies),
-                     "regime_energies": dm.regime_energies, "counts": dict(solver.counts)}
+                     "regime_energies": dm.regime_energies, "counts": dict(solver.counts),
+                     "phases": sweep_phases(dm, solver)}
         print("# lif sweep %s: E %.10f in %d sweeps, %.2f s" % (prec, e, len(dm.energies), t),
               file=sys.stderr, flush=True)
+    return out
+
+
+def measure_qc24_sweep(D=128, sweeps=1):
+    """One two-site sweep of the QC-24 system (the matvec workload's
+    Hamiltonian: synthetic FCIDUMP seed 1, 24 orbitals and electrons, ERI
+    rank 48, U1 x SU2, makenetwork.py 24 tree) at bond dimension D from the
+    seeded canonical start, FP64 and mixed: seconds, per-phase seconds,
+    energies."""
+    import t3ns_amd as T
+    from t3ns_amd import synthetic
+    from t3ns_amd.sweep import DeviceSolver, Sweeper
+    fci = synthetic.qc_workload(24, 24, seed=1, rank=48)
+    with open(NETWORK) as f:
+        netw = f.read()
+    out = {"workload": "qc24_t3ns_D%d" % D, "sweeps": sweeps, "davidson_tol": 1e-5}
+    for prec in ("f64", "mixed"):
+        ctx = T.Context(netw, "QC", [], ["U1", "SU2"], [24, 0], D, fcidump=fci)
+        t0 = time.perf_counter()
+        solver = DeviceSolver(prec, max_its=100, mixed_min_size=0)
+        dm = Sweeper(ctx, D, seed=1, solver=solver, tol=1e-5)
+        e = dm.run(sweeps)
+        t = time.perf_counter() - t0
+        out[prec] = {"energy": e, "seconds": t, "seconds_per_sweep": t / len(dm.energies),
+                     "counts": dict(solver.counts), "phases": sweep_phases(dm, solver)}
+        print("# qc24 sweep %s: E %.10f, %.2f s %s" % (prec, e, t, out[prec]["phases"]),
+              file=sys.stderr, flush=True)
+    out["abs_dE"] = abs(out["mixed"]["energy"] - out["f64"]["energy"])
     return out
 
 
@@ -608,6 +647,8 @@ def main():
         out["solve"] = measure_solve(ctx, step, args.solve_tol)
     if not args.no_sweep:
         out["sweep"] = measure_sweep()
+        if args.qc24_sweep:
+            out["sweep"]["qc24"] = measure_qc24_sweep(args.qc24_sweep)
     if not args.no_serve:
         out["serve"] = measure_serving(args.serve_requests, args.serve_rate)
     if not args.no_rows:

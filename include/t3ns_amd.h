@@ -401,6 +401,13 @@ typedef struct {
 int t3ns_update_branching(const t3ns_bupdate_desc* desc, const t3ns_rops_desc* ops /* [2] */,
                           t3ns_rops** out);
 
+/* The host half of t3ns_update_branching alone (no device): out[0..n) =
+ * seconds taking in the descriptor (operator and tensor copies), seconds
+ * planning, block products planned, unique-operator blocks written.  For
+ * sizing and timing the planner. */
+int t3ns_bupdate_plan_stats(const t3ns_bupdate_desc* d, const t3ns_rops_desc* ops, double* out,
+                            int n);
+
 /* ------------------------------------------------------------------------
  * decompose_siteTensor, one split (src/siteTensor_decompose.c:1403-1444,
  * split_of_site, with init_svddata 587-887, svdblocks 1050-1072, selectS

@@ -1,6 +1,7 @@
 // C ABI of t3ns_amd (include/t3ns_amd.h, include/t3ns_host.h).
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -658,12 +659,13 @@ int t3ns_update_physical(const t3ns_pupdate_desc* d, const t3ns_rops_desc* pops,
     });
 }
 
-int t3ns_update_branching(const t3ns_bupdate_desc* d, const t3ns_rops_desc* ops, t3ns_rops** out) {
-    if (!d || !ops || !out) return fail(T3NS_ERR_INVALID, "null argument");
-    return guarded([&] {
-        const ROperators o0 = rops_from(ops[0]);
-        const ROperators o1 = rops_from(ops[1]);
-        BUpdateInput in;
+namespace {
+// The branching-update input of a descriptor (the operators and tensor
+// copied into o0, o1, T, which `in` points at).
+void bupdate_input(const t3ns_bupdate_desc* d, const t3ns_rops_desc* ops, ROperators& o0,
+                   ROperators& o1, SiteTensor& T, BUpdateInput& in) {
+        o0 = rops_from(ops[0]);
+        o1 = rops_from(ops[1]);
         in.sym.nrSyms = d->nrSyms;
         for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) in.sym.sgs[i] = d->sgs[i];
         in.update_case = d->update_case;
@@ -674,7 +676,6 @@ int t3ns_update_branching(const t3ns_bupdate_desc* d, const t3ns_rops_desc* ops,
             throw std::invalid_argument("bUpdate: MPO sectors differ");
         if (d->T_nrblocks < 0 || (d->T_nrblocks && (!d->T_qnumbers || !d->T_beginblock || !d->T_tel)))
             throw std::invalid_argument("bUpdate: bad site tensor");
-        SiteTensor T;
         T.nrsites = 1;
         T.nrblocks = d->T_nrblocks;
         T.qnumbers.assign(d->T_qnumbers, d->T_qnumbers + T.nrblocks);
@@ -707,11 +708,40 @@ int t3ns_update_branching(const t3ns_bupdate_desc* d, const t3ns_rops_desc* ops,
         in.T = &T;
         in.ops[0] = &o0;
         in.ops[1] = &o1;
+}
+}  // namespace
+
+int t3ns_update_branching(const t3ns_bupdate_desc* d, const t3ns_rops_desc* ops, t3ns_rops** out) {
+    if (!d || !ops || !out) return fail(T3NS_ERR_INVALID, "null argument");
+    return guarded([&] {
+        ROperators o0, o1;
+        SiteTensor T;
+        BUpdateInput in;
+        bupdate_input(d, ops, o0, o1, T, in);
         const BUpdatePlan plan = plan_bupdate(in);
         auto res = std::make_unique<t3ns_rops>();
         res->r = run_bupdate(plan, T, o0, o1, hipStreamPerThread, &res->work);
         res->views();
         *out = res.release();
+    });
+}
+
+int t3ns_bupdate_plan_stats(const t3ns_bupdate_desc* d, const t3ns_rops_desc* ops, double* out,
+                            int n) {
+    if (!d || !ops || (n > 0 && !out)) return fail(T3NS_ERR_INVALID, "null argument");
+    return guarded([&] {
+        const auto t0 = std::chrono::steady_clock::now();
+        ROperators o0, o1;
+        SiteTensor T;
+        BUpdateInput in;
+        bupdate_input(d, ops, o0, o1, T, in);
+        const auto t1 = std::chrono::steady_clock::now();
+        const BUpdatePlan plan = plan_bupdate(in);
+        const auto t2 = std::chrono::steady_clock::now();
+        const double v[] = {std::chrono::duration<double>(t1 - t0).count(),
+                            std::chrono::duration<double>(t2 - t1).count(),
+                            (double)plan.tasks.size(), (double)plan.dests.size()};
+        for (int i = 0; i < n && i < 4; ++i) out[i] = v[i];
     });
 }
 

@@ -3,6 +3,7 @@
 // bandwidth-bound (each element is read once per contributing term and
 // written once).
 #include "renorm_dev.hpp"
+#include "../host/host_threads.hpp"
 #include "tile_gemm.hpp"
 
 #include <algorithm>
@@ -627,121 +628,154 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
         std::vector<SumTask> tasks;
         int ndot = 0;  // small-output tiles, first in sums
     };
+    // Tile lists are built on host threads over ranges of destinations of
+    // about equal task counts, each range cut into its own chunks (a chunk
+    // shares first- and second-stage products among its tasks); the chunks
+    // run in range order.  The sums per destination keep their task order,
+    // so the results do not depend on the ranges.
+    const size_t ndest = plan.dests.size();
+    const int nranges = (int)std::min<size_t>(std::max<size_t>(ndest, 1), 64);
+    std::vector<size_t> rbeg(nranges + 1, ndest);
+    {
+        const double per = (double)plan.tasks.size() / nranges;
+        size_t d = 0;
+        for (int r = 0; r < nranges; ++r) {
+            rbeg[r] = d;
+            const double target = per * (r + 1);
+            while (d < ndest && (double)plan.dests[d].task_end <= target) ++d;
+        }
+        rbeg[0] = 0;
+    }
+    std::vector<std::vector<Chunk>> rchunks(nranges);
+    std::vector<double> rflops(nranges, 0.0);
+    std::vector<int64_t> rwmax(nranges, 0);
+    auto build_range = [&](int r) {
+        std::vector<Chunk>& chunks = rchunks[r];
+        double& flops = rflops[r];
+        int64_t& wmax = rwmax[r];
+        size_t di = rbeg[r];
+        const size_t dend = rbeg[r + 1];
+        while (di < dend) {
+            Chunk c;
+            std::unordered_map<std::array<int64_t, 3>, int64_t, KeyHash> w1, w2, sw;
+            int64_t wtop = 0;
+            for (; di < dend; ++di) {
+                const BUpdateDest& dd = plan.dests[di];
+                // workspace this destination needs at most
+                int64_t need = 0;
+                for (int t = dd.task_begin; t < dd.task_end; ++t) {
+                    const BUpdateTask& tk = plan.tasks[t];
+                    int64_t s1 = 1, s2 = 1;
+                    for (int i = 0; i < 3; ++i) {
+                        s1 *= i == xl ? tk.e[i] : tk.d[i];
+                        s2 *= i == nl ? tk.d[i] : tk.e[i];
+                    }
+                    need += s1 + s2;
+                    if (nl == 1) need += s2 + (int64_t)tk.e[0] * tk.e[1] * tk.e[2];
+                }
+                if (need > kWork) throw std::runtime_error("bUpdate: block needs more than 2 GiB of workspace");
+                if (wtop + need > kWork && !c.sums.empty()) break;
+                SumTile st;
+                st.u_off = u_base + dd.dst;
+                st.M = dd.M;
+                st.N = dd.N;
+                st.task_begin = (int)c.tasks.size();
+                st.flat = 0;
+                st.pad = 0;
+                for (int t = dd.task_begin; t < dd.task_end; ++t) {
+                    const BUpdateTask& tk = plan.tasks[t];
+                    // stage 1: W1 = T x_x O1 (shared by every task with the same T and O1 block)
+                    int n1d[3] = {tk.d[0], tk.d[1], tk.d[2]};
+                    const std::array<int64_t, 3> k1 = {tk.t, tk.o1, -1};
+                    auto it1 = w1.find(k1);
+                    int64_t w1off;
+                    if (it1 == w1.end()) {
+                        w1off = w_base + wtop;
+                        int64_t sz = (int64_t)tk.e[xl];
+                        for (int i = 0; i < 3; ++i)
+                            if (i != xl) sz *= tk.d[i];
+                        wtop += sz;
+                        mode_tiles(c.p1, tk.t, n1d, xl, o0_base + tk.o1, tk.e[xl], w1off, &flops);
+                        w1.emplace(k1, w1off);
+                    } else {
+                        w1off = it1->second;
+                    }
+                    // stage 2: W2 = W1 x_y O2
+                    int n2d[3] = {tk.d[0], tk.d[1], tk.d[2]};
+                    n2d[xl] = tk.e[xl];
+                    const std::array<int64_t, 3> k2 = {tk.t, tk.o1, tk.o2};
+                    auto it2 = w2.find(k2);
+                    int64_t w2off;
+                    if (it2 == w2.end()) {
+                        w2off = w_base + wtop;
+                        int64_t sz = (int64_t)tk.e[yl];
+                        for (int i = 0; i < 3; ++i)
+                            if (i != yl) sz *= n2d[i];
+                        wtop += sz;
+                        mode_tiles(c.p2, w1off, n2d, yl, o1_base + tk.o2, tk.e[yl], w2off, &flops);
+                        w2.emplace(k2, w2off);
+                    } else {
+                        w2off = it2->second;
+                    }
+                    // stage 3: U += pref * A . W2 over the two legs of the old operators
+                    const int* e = tk.e;
+                    const int* d = tk.d;
+                    SumTask sk;
+                    sk.pad = 0;
+                    sk.pref = tk.pref;
+                    if (nl == 2) {
+                        sk.r = Strided{tk.a, e[0] * e[1], 1};
+                        sk.s = Strided{w2off, 1, e[0] * e[1]};
+                        sk.K = e[0] * e[1];
+                        c.tasks.push_back(sk);
+                    } else if (nl == 0) {
+                        sk.r = Strided{tk.a, 1, e[0]};
+                        sk.s = Strided{w2off, d[0], 1};
+                        sk.K = e[1] * e[2];
+                        c.tasks.push_back(sk);
+                    } else {
+                        // U = sum_g A_g^T W2_g over the last leg: with legs 1 and 2
+                        // of A and W2 swapped, one product with K = e0 * e2
+                        auto swapped = [&](int64_t src, int n1, int64_t key2) {
+                            const std::array<int64_t, 3> key = {src, key2, -2};
+                            auto it = sw.find(key);
+                            if (it != sw.end()) return it->second;
+                            const int64_t off = w_base + wtop;
+                            wtop += (int64_t)e[0] * n1 * e[2];
+                            c.swaps.push_back(Swap12{src, off, e[0], n1, e[2], 0});
+                            sw.emplace(key, off);
+                            return off;
+                        };
+                        const int64_t a2 = swapped(tk.a, e[1], -1);
+                        const int64_t w2s = swapped(w2off, d[1], -3);
+                        sk.r = Strided{a2, e[0] * e[2], 1};
+                        sk.s = Strided{w2s, 1, e[0] * e[2]};
+                        sk.K = e[0] * e[2];
+                        c.tasks.push_back(sk);
+                    }
+                    flops += 2.0 * (double)e[0] * e[1] * e[2] * d[nl];
+                }
+                st.task_end = (int)c.tasks.size();
+                for (int a = 0; a < tiles_of(st.M); ++a)
+                    for (int b = 0; b < tiles_of(st.N); ++b) {
+                        st.ti = a;
+                        st.tj = b;
+                        c.sums.push_back(st);
+                    }
+            }
+            wmax = std::max(wmax, wtop);
+            c.ndot = partition_sums(c.sums);
+            chunks.push_back(std::move(c));
+        }
+    };
+    run_blocks_threaded(nranges, build_range);
     std::vector<Chunk> chunks;
     double flops = 0;
     int64_t wmax = 0;
-    size_t di = 0;
-    while (di < plan.dests.size()) {
-        Chunk c;
-        std::unordered_map<std::array<int64_t, 3>, int64_t, KeyHash> w1, w2, sw;
-        int64_t wtop = 0;
-        for (; di < plan.dests.size(); ++di) {
-            const BUpdateDest& dd = plan.dests[di];
-            // workspace this destination needs at most
-            int64_t need = 0;
-            for (int t = dd.task_begin; t < dd.task_end; ++t) {
-                const BUpdateTask& tk = plan.tasks[t];
-                int64_t s1 = 1, s2 = 1;
-                for (int i = 0; i < 3; ++i) {
-                    s1 *= i == xl ? tk.e[i] : tk.d[i];
-                    s2 *= i == nl ? tk.d[i] : tk.e[i];
-                }
-                need += s1 + s2;
-                if (nl == 1) need += s2 + (int64_t)tk.e[0] * tk.e[1] * tk.e[2];
-            }
-            if (need > kWork) throw std::runtime_error("bUpdate: block needs more than 2 GiB of workspace");
-            if (wtop + need > kWork && !c.sums.empty()) break;
-            SumTile st;
-            st.u_off = u_base + dd.dst;
-            st.M = dd.M;
-            st.N = dd.N;
-            st.task_begin = (int)c.tasks.size();
-            st.flat = 0;
-            st.pad = 0;
-            for (int t = dd.task_begin; t < dd.task_end; ++t) {
-                const BUpdateTask& tk = plan.tasks[t];
-                // stage 1: W1 = T x_x O1 (shared by every task with the same T and O1 block)
-                int n1d[3] = {tk.d[0], tk.d[1], tk.d[2]};
-                const std::array<int64_t, 3> k1 = {tk.t, tk.o1, -1};
-                auto it1 = w1.find(k1);
-                int64_t w1off;
-                if (it1 == w1.end()) {
-                    w1off = w_base + wtop;
-                    int64_t sz = (int64_t)tk.e[xl];
-                    for (int i = 0; i < 3; ++i)
-                        if (i != xl) sz *= tk.d[i];
-                    wtop += sz;
-                    mode_tiles(c.p1, tk.t, n1d, xl, o0_base + tk.o1, tk.e[xl], w1off, &flops);
-                    w1.emplace(k1, w1off);
-                } else {
-                    w1off = it1->second;
-                }
-                // stage 2: W2 = W1 x_y O2
-                int n2d[3] = {tk.d[0], tk.d[1], tk.d[2]};
-                n2d[xl] = tk.e[xl];
-                const std::array<int64_t, 3> k2 = {tk.t, tk.o1, tk.o2};
-                auto it2 = w2.find(k2);
-                int64_t w2off;
-                if (it2 == w2.end()) {
-                    w2off = w_base + wtop;
-                    int64_t sz = (int64_t)tk.e[yl];
-                    for (int i = 0; i < 3; ++i)
-                        if (i != yl) sz *= n2d[i];
-                    wtop += sz;
-                    mode_tiles(c.p2, w1off, n2d, yl, o1_base + tk.o2, tk.e[yl], w2off, &flops);
-                    w2.emplace(k2, w2off);
-                } else {
-                    w2off = it2->second;
-                }
-                // stage 3: U += pref * A . W2 over the two legs of the old operators
-                const int* e = tk.e;
-                const int* d = tk.d;
-                SumTask sk;
-                sk.pad = 0;
-                sk.pref = tk.pref;
-                if (nl == 2) {
-                    sk.r = Strided{tk.a, e[0] * e[1], 1};
-                    sk.s = Strided{w2off, 1, e[0] * e[1]};
-                    sk.K = e[0] * e[1];
-                    c.tasks.push_back(sk);
-                } else if (nl == 0) {
-                    sk.r = Strided{tk.a, 1, e[0]};
-                    sk.s = Strided{w2off, d[0], 1};
-                    sk.K = e[1] * e[2];
-                    c.tasks.push_back(sk);
-                } else {
-                    // U = sum_g A_g^T W2_g over the last leg: with legs 1 and 2
-                    // of A and W2 swapped, one product with K = e0 * e2
-                    auto swapped = [&](int64_t src, int n1, int64_t key2) {
-                        const std::array<int64_t, 3> key = {src, key2, -2};
-                        auto it = sw.find(key);
-                        if (it != sw.end()) return it->second;
-                        const int64_t off = w_base + wtop;
-                        wtop += (int64_t)e[0] * n1 * e[2];
-                        c.swaps.push_back(Swap12{src, off, e[0], n1, e[2], 0});
-                        sw.emplace(key, off);
-                        return off;
-                    };
-                    const int64_t a2 = swapped(tk.a, e[1], -1);
-                    const int64_t w2s = swapped(w2off, d[1], -3);
-                    sk.r = Strided{a2, e[0] * e[2], 1};
-                    sk.s = Strided{w2s, 1, e[0] * e[2]};
-                    sk.K = e[0] * e[2];
-                    c.tasks.push_back(sk);
-                }
-                flops += 2.0 * (double)e[0] * e[1] * e[2] * d[nl];
-            }
-            st.task_end = (int)c.tasks.size();
-            for (int a = 0; a < tiles_of(st.M); ++a)
-                for (int b = 0; b < tiles_of(st.N); ++b) {
-                    st.ti = a;
-                    st.tj = b;
-                    c.sums.push_back(st);
-                }
-        }
-        wmax = std::max(wmax, wtop);
-        c.ndot = partition_sums(c.sums);
-        chunks.push_back(std::move(c));
+    for (int r = 0; r < nranges; ++r) {
+        for (Chunk& c : rchunks[r]) chunks.push_back(std::move(c));
+        flops += rflops[r];
+        wmax = std::max(wmax, rwmax[r]);
     }
 
     DevBuf<double> arena(std::max<int64_t>(1, w_base + wmax));

@@ -2,9 +2,12 @@
 #include "renorm.hpp"
 
 #include <algorithm>
-#include <string>
+#include <atomic>
 #include <cmath>
 #include <stdexcept>
+#include <string>
+
+#include "host_threads.hpp"
 
 namespace t3 {
 
@@ -336,6 +339,8 @@ namespace t3 {
 // error instead.
 constexpr int64_t kMaxBUpdateTasks = 100000000;
 
+
+
 BUpdatePlan plan_bupdate(const BUpdateInput& in) {
     const int uc = in.update_case;
     if (uc < 0 || uc > 2) throw std::invalid_argument("bUpdate: bad update case");
@@ -422,9 +427,17 @@ BUpdatePlan plan_bupdate(const BUpdateInput& in) {
     };
     const int first = looptype ? 0 : 1, second = 1 - first;
     const int N = in.begin_blocks_of_hss[in.nrhss];
-    std::vector<std::vector<BUpdateTask>> per_dest((size_t)nu * N);
-    int64_t ntasks = 0;
-    for (int new_sb = 0; new_sb < N; ++new_sb) {
+    // Contributions of every new block, grouped by unique operator u (in
+    // loop order within a group): each new block fills only its own entry,
+    // so host threads take new blocks from a shared counter; the
+    // destinations (u, block) are then laid out in (u, block) order, which
+    // makes the plan independent of the threads.
+    std::vector<std::vector<BUpdateTask>> blk_tasks(N);
+    std::vector<std::vector<int>> blk_uoff(N);  // [nu + 1] offsets per new block
+    std::atomic<int64_t> ntasks{0};
+    auto plan_block = [&](int new_sb) {
+        std::vector<BUpdateTask> loc;
+        std::vector<int> loc_u;
         int bra[3], ket[3], hss[3];
         ids_of(in.qnumbers[new_sb], uc, &bra[uc], &ket[uc], &hss[2]);
         const int hn = hss[2];
@@ -508,33 +521,61 @@ BUpdatePlan plan_bupdate(const BUpdateInput& in) {
                                 t.e[i] = in.site_ss[i].dims[bra[i]];
                             }
                             t.pref = pref;
-                            per_dest[(size_t)u * N + new_sb].push_back(t);
-                            if (++ntasks > kMaxBUpdateTasks)
-                                throw std::runtime_error(
-                                    "bUpdate: more than " + std::to_string(kMaxBUpdateTasks) +
-                                    " block products (tiny symmetry sectors); the plan would not "
-                                    "fit in host memory");
+                            loc.push_back(t);
+                            loc_u.push_back(u);
                         }
                     }
                 }
             }
         }
-    }
+        const int64_t added = (int64_t)loc.size();
+        if (ntasks.fetch_add(added) + added > kMaxBUpdateTasks)
+            throw std::runtime_error(
+                "bUpdate: more than " + std::to_string(kMaxBUpdateTasks) +
+                " block products (tiny symmetry sectors); the plan would not fit in host memory");
+        // counting sort by u (stable: loop order kept within each u)
+        std::vector<int>& off = blk_uoff[new_sb];
+        off.assign(nu + 1, 0);
+        for (int u : loc_u) ++off[u + 1];
+        for (int u = 0; u < nu; ++u) off[u + 1] += off[u];
+        std::vector<int> pos(off.begin(), off.end() - 1);
+        std::vector<BUpdateTask>& out = blk_tasks[new_sb];
+        out.resize(loc.size());
+        for (size_t i = 0; i < loc.size(); ++i) out[pos[loc_u[i]]++] = loc[i];
+    };
+    run_blocks_threaded(N, plan_block);
+    // destinations in (u, block) order, then the tasks copied into place
+    struct Seg {
+        int sb;
+        int64_t from, to;
+    };
+    std::vector<Seg> segs;
+    int64_t total = 0;
     for (int u = 0; u < nu; ++u) {
         const int h = U.hss_of_ops[u];
         for (int b = 0; b < U.nblocks_hss(h); ++b) {
-            auto& v = per_dest[(size_t)u * N + in.begin_blocks_of_hss[h] + b];
-            if (v.empty()) continue;
+            const int sb = in.begin_blocks_of_hss[h] + b;
+            const std::vector<int>& off = blk_uoff[sb];
+            if (off.empty() || off[u + 1] == off[u]) continue;
+            const BUpdateTask& t0 = blk_tasks[sb][off[u]];
             BUpdateDest dd;
             dd.dst = U.op_offset[u] + U.beginblock[u][b];
-            dd.M = v[0].e[uc];
-            dd.N = v[0].d[uc];
-            dd.task_begin = (int)P.tasks.size();
-            P.tasks.insert(P.tasks.end(), v.begin(), v.end());
-            dd.task_end = (int)P.tasks.size();
+            dd.M = t0.e[uc];
+            dd.N = t0.d[uc];
+            dd.task_begin = (int)total;
+            total += off[u + 1] - off[u];
+            dd.task_end = (int)total;
             P.dests.push_back(dd);
+            segs.push_back(Seg{sb, off[u], off[u + 1]});
         }
     }
+    if (total > INT32_MAX) throw std::runtime_error("bUpdate: more than 2^31 block products");
+    P.tasks.resize(total);
+    run_blocks_threaded((int)segs.size(), [&](int i) {
+        const Seg& g = segs[i];
+        std::copy(blk_tasks[g.sb].begin() + g.from, blk_tasks[g.sb].begin() + g.to,
+                  P.tasks.begin() + P.dests[i].task_begin);
+    });
 
     // sum_unique_rOperators
     ROperators& nr = P.nr;

@@ -104,13 +104,15 @@ class DeviceSolver:
     env = {}
 
     def __init__(self, precision="f64", max_vecs=30, keep_deflate=2, max_its=300,
-                 mixed_min_size=200000, autotune_min_size=500000):
+                 mixed_min_size=200000, autotune_min_size=None):
         """mixed_min_size: below this many elements a step's Heff is
         launch-bound, FP32 matvecs save nothing and the refinement adds
         matvecs, so "mixed" solves such steps in FP64.  autotune_min_size:
         steps of at least this many elements have their work lists
         autotuned (Heff.autotune; shapes seen before reuse their setting);
-        None turns it off."""
+        None (default) keeps the default setting: a step's shape rarely
+        repeats exactly within a sweep, and six timed settings cost about
+        as much as a step's Davidson at QC-24 D = 128."""
         if precision not in ("f64", "mixed"):
             raise ValueError("precision is f64 or mixed")
         self.precision = precision
