@@ -7,6 +7,7 @@
 #include <stdexcept>
 
 #include "heff_dev.hpp"
+#include "../host/host_threads.hpp"
 
 namespace t3 {
 
@@ -1652,18 +1653,27 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
                 gcap = std::max<int64_t>(gcap, (int64_t)w[0] * w[1] * w[2]);
             }
         }
-    std::vector<DevPair> hp;
-    std::vector<DevChunk> hc;
-    std::vector<int> hr;
-    std::vector<DevTerm> ht;
-    std::vector<DevBlock> hb;
-    std::vector<double> cost(plan_.nrblocks, 0.0);
-    std::vector<double> pcost;
-    for (int sb = 0; sb < plan_.nrblocks; ++sb) {
+    // Work lists per output block, built on host threads: every output
+    // block's pairs, chunks, run leads and terms go to its own lists with
+    // indices local to them, then the lists are joined in block order with
+    // their indices rebased, so the result does not depend on the threads.
+    struct Lists {
+        std::vector<DevPair> hp;
+        std::vector<DevChunk> hc;
+        std::vector<int> hr;
+        std::vector<DevTerm> ht;
+        std::vector<double> pcost;
+        double cost = 0;
+        HeffStats st;
+        DevBlock B{};
+    };
+    std::vector<Lists> lists(plan_.nrblocks);
+    auto build_block = [&](int sb) {
+        Lists& L = lists[sb];
         DevBlock B{};
         B.y_off = plan_.beginblock[sb];
         for (int k = 0; k < 3; ++k) B.d[k] = plan_.dims[sb][k];
-        B.pair_begin = (int)hp.size();
+        B.pair_begin = (int)L.hp.size();
         for (int pi = plan_.pair_begin[sb]; pi < plan_.pair_begin[sb + 1]; ++pi) {
             const PlanPair& pp = plan_.pairs[pi];
             DevPair P{};
@@ -1676,7 +1686,7 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
             } else {
                 for (int k = 0; k < 3; ++k) P.ord[k] = hOrd[pp.bestorder][k];
             }
-            const size_t tb = ht.size();
+            const size_t tb = L.ht.size();
             for (const PlanEntry& e : pp.entries) {
                 for (int i = is.MPOc_beg[e.mpo_group]; i < is.MPOc_beg[e.mpo_group + 1]; ++i) {
                     const Instruction& ins = is.instr[i];
@@ -1696,19 +1706,19 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
                     if (bf16)
                         for (int k = 0; k < nops; ++k)
                             t.alpha *= bscale_[k][ins.instr[k]][e.sbops[k]];
-                    ht.push_back(t);
+                    L.ht.push_back(t);
                 }
             }
-            if ((int64_t)ht.size() > INT32_MAX)
+            if ((int64_t)L.ht.size() > INT32_MAX)
                 throw std::runtime_error("HeffDevice: more than 2^31 terms");
-            if (ht.size() == tb) continue;
+            if (L.ht.size() == tb) continue;
             if (order_by_model)
-                choose_order(P, B.d, ht.data() + tb, (int)(ht.size() - tb), isdmrg_, pool, f32,
+                choose_order(P, B.d, L.ht.data() + tb, (int)(L.ht.size() - tb), isdmrg_, pool, f32,
                              chunk_steps, fuse, fuse_over);
             // Terms sorted by their first-stage operator block: runs of equal
             // first operators share one first-stage output.
             const int first = P.ord[0];
-            std::stable_sort(ht.begin() + tb, ht.end(), [first](const DevTerm& a, const DevTerm& b) {
+            std::stable_sort(L.ht.begin() + tb, L.ht.end(), [first](const DevTerm& a, const DevTerm& b) {
                 return a.op[first] < b.op[first];
             });
             // stage output sizes
@@ -1729,7 +1739,7 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
             P.w1_off = P.xlds ? (int)xsz : 0;
             const int64_t avail = pool - P.w1_off;
             // chunks
-            P.chunk_begin = (int)hc.size();
+            P.chunk_begin = (int)L.hc.size();
             DevChunk C{};
             int nruns = 0, nterms = 0;
             int64_t prev = -1;
@@ -1748,27 +1758,27 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
             };
             auto close = [&](int end) {
                 C.term_end = end;
-                C.run_end = (int)hr.size();
+                C.run_end = (int)L.hr.size();
                 const int nt = end - C.term_begin;
                 C.ncs = group ? (int)cops.size() : nt;
                 // terms in group order (stable: each group keeps the order
                 // of its terms); run leads re-pointed at a term of the run
-                std::stable_sort(ht.begin() + C.term_begin, ht.begin() + end,
+                std::stable_sort(L.ht.begin() + C.term_begin, L.ht.begin() + end,
                                  [](const DevTerm& x, const DevTerm& y) { return x.cslot < y.cslot; });
-                for (int r = 0; r < nruns; ++r) hr[C.run_begin + r] = -1;
+                for (int r = 0; r < nruns; ++r) L.hr[C.run_begin + r] = -1;
                 for (int t = C.term_begin; t < end; ++t)
-                    if (hr[C.run_begin + ht[t].slot] < 0) hr[C.run_begin + ht[t].slot] = t;
+                    if (L.hr[C.run_begin + L.ht[t].slot] < 0) L.hr[C.run_begin + L.ht[t].slot] = t;
                 for (int g = 0, t = 0; g <= C.ncs; ++g) {
-                    while (t < nt && ht[C.term_begin + t].cslot < g) ++t;
-                    hr.push_back(t);
+                    while (t < nt && L.ht[C.term_begin + t].cslot < g) ++t;
+                    L.hr.push_back(t);
                 }
                 const int64_t r1 = (int64_t)nruns * P.w1;
                 C.w2_off = P.wlds ? (int)r1 : (int)gcap;
-                hc.push_back(C);
+                L.hc.push_back(C);
             };
-            for (size_t t = tb; t < ht.size(); ++t) {
-                const bool newrun = nterms == 0 || ht[t].op[first] != prev;
-                const bool newc = !group || nterms == 0 || group_of(ht[t]) < 0;
+            for (size_t t = tb; t < L.ht.size(); ++t) {
+                const bool newrun = nterms == 0 || L.ht[t].op[first] != prev;
+                const bool newc = !group || nterms == 0 || group_of(L.ht[t]) < 0;
                 const int64_t r1 = (int64_t)(nruns + newrun) * P.w1;
                 const int64_t r2 =
                     isdmrg_ ? 0
@@ -1784,29 +1794,29 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
                 if (nterms == 0) {
                     C = DevChunk{};
                     C.term_begin = (int)t;
-                    C.run_begin = (int)hr.size();
+                    C.run_begin = (int)L.hr.size();
                     cops.clear();
                 }
-                if (nterms == 0 || ht[t].op[first] != prev) {
-                    hr.push_back((int)t);
+                if (nterms == 0 || L.ht[t].op[first] != prev) {
+                    L.hr.push_back((int)t);
                     ++nruns;
                 }
-                ht[t].slot = nruns - 1;
+                L.ht[t].slot = nruns - 1;
                 if (!group) {
-                    ht[t].cslot = nterms;
+                    L.ht[t].cslot = nterms;
                 } else {
-                    int g = group_of(ht[t]);
+                    int g = group_of(L.ht[t]);
                     if (g < 0) {
                         g = (int)cops.size();
-                        cops.push_back(ht[t].op[last]);
+                        cops.push_back(L.ht[t].op[last]);
                     }
-                    ht[t].cslot = g;
+                    L.ht[t].cslot = g;
                 }
-                prev = ht[t].op[first];
+                prev = L.ht[t].op[first];
                 ++nterms;
             }
-            close((int)ht.size());
-            P.chunk_end = (int)hc.size();
+            close((int)L.ht.size());
+            P.chunk_end = (int)L.hc.size();
             {
                 // matrix-core steps: tiles x ceil(K/4) per stage output
                 int cd[3] = {P.d[0], P.d[1], P.d[2]};
@@ -1821,33 +1831,94 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
                 }
                 double nr = 0, ntt = 0, ncs = 0;
                 for (int ci = P.chunk_begin; ci < P.chunk_end; ++ci) {
-                    nr += hc[ci].run_end - hc[ci].run_begin;
-                    ntt += hc[ci].term_end - hc[ci].term_begin;
-                    ncs += hc[ci].ncs;
+                    nr += L.hc[ci].run_end - L.hc[ci].run_begin;
+                    ntt += L.hc[ci].term_end - L.hc[ci].term_begin;
+                    ncs += L.hc[ci].ncs;
                 }
-                stats_.stage_steps[0] += nr * st[0];
-                stats_.stage_steps[1] += P.fuse ? 0.0 : ntt * st[1];
-                stats_.pairs_fused += P.fuse;
-                if (!isdmrg_) stats_.stage_steps[2] += ncs * st[2];
-                stats_.nruns += (int64_t)nr;
-                stats_.pairs_global_x += !P.xlds;
-                stats_.pairs_global_w += !P.wlds;
+                L.st.stage_steps[0] += nr * st[0];
+                L.st.stage_steps[1] += P.fuse ? 0.0 : ntt * st[1];
+                L.st.pairs_fused += P.fuse;
+                if (!isdmrg_) L.st.stage_steps[2] += ncs * st[2];
+                L.st.nruns += (int64_t)nr;
+                L.st.pairs_global_x += !P.xlds;
+                L.st.pairs_global_w += !P.wlds;
             }
             {
                 double c = 0;
                 for (int ci = P.chunk_begin; ci < P.chunk_end; ++ci) {
-                    const double nr = hc[ci].run_end - hc[ci].run_begin;
-                    const double nt = hc[ci].term_end - hc[ci].term_begin;
+                    const double nr = L.hc[ci].run_end - L.hc[ci].run_begin;
+                    const double nt = L.hc[ci].term_end - L.hc[ci].term_begin;
                     c += nr * xsz * B.d[P.ord[0]] + nt * (double)P.w1 * B.d[P.ord[1]] +
-                         (isdmrg_ ? 0.0 : hc[ci].ncs * (double)P.w2 * B.d[P.ord[2]]);
+                         (isdmrg_ ? 0.0 : L.hc[ci].ncs * (double)P.w2 * B.d[P.ord[2]]);
                 }
-                cost[sb] += c;
-                pcost.push_back(c);
+                L.cost += c;
+                L.pcost.push_back(c);
             }
-            hp.push_back(P);
+            L.hp.push_back(P);
         }
-        B.pair_end = (int)hp.size();
-        hb.push_back(B);
+        B.pair_end = (int)L.hp.size();
+        L.B = B;
+    };
+    run_blocks_threaded(plan_.nrblocks, build_block, (double)plan_.pairs.size(), 2e4);
+    std::vector<DevPair> hp;
+    std::vector<DevChunk> hc;
+    std::vector<int> hr;
+    std::vector<DevTerm> ht;
+    std::vector<DevBlock> hb(plan_.nrblocks);
+    std::vector<double> cost(plan_.nrblocks, 0.0);
+    std::vector<double> pcost;
+    {
+        std::vector<int64_t> pb(plan_.nrblocks + 1, 0), cb(pb), rb(pb), tb(pb);
+        for (int sb = 0; sb < plan_.nrblocks; ++sb) {
+            const Lists& L = lists[sb];
+            pb[sb + 1] = pb[sb] + (int64_t)L.hp.size();
+            cb[sb + 1] = cb[sb] + (int64_t)L.hc.size();
+            rb[sb + 1] = rb[sb] + (int64_t)L.hr.size();
+            tb[sb + 1] = tb[sb] + (int64_t)L.ht.size();
+            for (int k = 0; k < 3; ++k) stats_.stage_steps[k] += L.st.stage_steps[k];
+            stats_.nruns += L.st.nruns;
+            stats_.pairs_global_x += L.st.pairs_global_x;
+            stats_.pairs_global_w += L.st.pairs_global_w;
+            stats_.pairs_fused += L.st.pairs_fused;
+            cost[sb] = L.cost;
+        }
+        const int64_t nb = plan_.nrblocks;
+        if (tb[nb] > INT32_MAX || rb[nb] > INT32_MAX || cb[nb] > INT32_MAX)
+            throw std::runtime_error("HeffDevice: more than 2^31 terms");
+        hp.resize(pb[nb]);
+        hc.resize(cb[nb]);
+        hr.resize(rb[nb]);
+        ht.resize(tb[nb]);
+        pcost.resize(pb[nb]);
+        run_blocks_threaded(plan_.nrblocks, [&](int sb) {
+            Lists& L = lists[sb];
+            DevBlock B = L.B;
+            B.pair_begin += (int)pb[sb];
+            B.pair_end += (int)pb[sb];
+            hb[sb] = B;
+            for (size_t i = 0; i < L.hp.size(); ++i) {
+                DevPair P = L.hp[i];
+                P.chunk_begin += (int)cb[sb];
+                P.chunk_end += (int)cb[sb];
+                hp[pb[sb] + i] = P;
+                pcost[pb[sb] + i] = L.pcost[i];
+            }
+            for (size_t i = 0; i < L.hc.size(); ++i) {
+                DevChunk C = L.hc[i];
+                // run leads are term indices; the group offsets after them
+                // are chunk-relative
+                for (int r = C.run_begin; r < C.run_end; ++r) L.hr[r] += (int)tb[sb];
+                C.term_begin += (int)tb[sb];
+                C.term_end += (int)tb[sb];
+                C.run_begin += (int)rb[sb];
+                C.run_end += (int)rb[sb];
+                hc[cb[sb] + i] = C;
+            }
+            std::copy(L.hr.begin(), L.hr.end(), hr.begin() + rb[sb]);
+            std::copy(L.ht.begin(), L.ht.end(), ht.begin() + tb[sb]);
+            std::vector<DevPair>().swap(L.hp);  // release as we go
+            std::vector<DevTerm>().swap(L.ht);
+        }, (double)tb[nb], 1e6);
     }
     const int64_t maxscratch = 2 * gcap;
     // Work units: output blocks whose cost exceeds a share of the machine

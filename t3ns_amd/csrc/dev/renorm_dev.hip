@@ -768,7 +768,7 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
             chunks.push_back(std::move(c));
         }
     };
-    run_blocks_threaded(nranges, build_range);
+    run_blocks_threaded(nranges, build_range, (double)plan.tasks.size(), 1e5);
     std::vector<Chunk> chunks;
     double flops = 0;
     int64_t wmax = 0;

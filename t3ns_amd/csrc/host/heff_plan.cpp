@@ -1,5 +1,6 @@
 // Effective-Hamiltonian plan builder; see heff_plan.hpp.
 #include "heff_plan.hpp"
+#include "host_threads.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -289,9 +290,11 @@ HeffPlan build_heff_plan(const HeffInput& in) {
         for (int k = 0; k < 3; ++k) P.dims[sb][k] = bidx[sb].dim[k];
     }
 
-    // First-run table (src/Heff.c:815-914, 1002-1047).
+    // First-run table (src/Heff.c:815-914, 1002-1047).  Every qnB owns its
+    // new blocks' entries of bynew, so host threads take qnBs from a shared
+    // counter; the pairs are joined in new-block order afterwards.
     std::vector<std::vector<PlanPair>> bynew(t.nrblocks);
-    for (int nq = 0; nq < nqnB; ++nq) {
+    run_blocks_threaded(nqnB, [&](int nq) {
         for (int newsb : sb_with[nq]) {
             const BlockIdx& nb = bidx[newsb];
             for (size_t oq = 0; oq < toqnB[nq].size(); ++oq) {
@@ -333,7 +336,7 @@ HeffPlan build_heff_plan(const HeffInput& in) {
                 }
             }
         }
-    }
+    }, (double)t.nrblocks * t.nrblocks, 1e6);  // pairs considered, at most
     P.pair_begin.assign(t.nrblocks + 1, 0);
     for (int sb = 0; sb < t.nrblocks; ++sb) {
         P.pair_begin[sb + 1] = P.pair_begin[sb] + (int)bynew[sb].size();

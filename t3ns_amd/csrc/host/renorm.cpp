@@ -543,7 +543,7 @@ BUpdatePlan plan_bupdate(const BUpdateInput& in) {
         out.resize(loc.size());
         for (size_t i = 0; i < loc.size(); ++i) out[pos[loc_u[i]]++] = loc[i];
     };
-    run_blocks_threaded(N, plan_block);
+    run_blocks_threaded(N, plan_block, (double)N * T.nrblocks, 1e5);  // block combinations, roughly
     // destinations in (u, block) order, then the tasks copied into place
     struct Seg {
         int sb;
@@ -575,7 +575,7 @@ BUpdatePlan plan_bupdate(const BUpdateInput& in) {
         const Seg& g = segs[i];
         std::copy(blk_tasks[g.sb].begin() + g.from, blk_tasks[g.sb].begin() + g.to,
                   P.tasks.begin() + P.dests[i].task_begin);
-    });
+    }, (double)total, 1e6);
 
     // sum_unique_rOperators
     ROperators& nr = P.nr;
