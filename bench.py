@@ -88,6 +88,8 @@ def parse():
                     help="skip the DMRG sweep of the 10-site Hubbard chain")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the served-solve stream")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-matvec leg (15 small objects)")
     ap.add_argument("--qc24-sweep", type=int, default=0, metavar="D",
                     help="also run one two-site sweep of the QC-24 system at bond dimension D")
     ap.add_argument("--serve-requests", type=int, default=12,
@@ -318,6 +320,70 @@ def _time_matvec(h, steps=3):
     return e0.elapsed_time(e1) / steps
 
 
+def measure_batch(reps=20):
+    """Cross-solve batching: the matvecs of the 15 two-site steps of a
+    16-site NN_HUBBARD chain (t = 1, U = 4, U1 x U1 (8, 8), D = 128, seeded
+    operators), as 15 small concurrent solves would issue them: one after
+    another on one stream, each on its own stream, and in one batched
+    launch per kernel.  ms per round of 15 matvecs (HIP events), and
+    whether the batched results equal the separate ones bitwise."""
+    import torch
+    import t3ns_amd as T
+    with open(os.path.join(ROOT, "tests", "golden", "networks", "16_DMRG.netw")) as f:
+        netw = f.read()
+    ctx = T.Context(netw, "NN_HUBBARD", [1.0, 4.0], ["U1", "U1"], [8, 8], 128)
+    steps = [T.Step(ctx, [i, i + 1], seed=2) for i in range(15)]
+    hs = [T.Heff.from_step(st, T.FP64) for st in steps]
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(5)
+    xs = [torch.from_numpy(rng.standard_normal(h.size)).to(dev) for h in hs]
+    ys = [torch.empty_like(x) for x in xs]
+    streams = [torch.cuda.Stream(device=dev) for _ in hs]
+    main = torch.cuda.Stream(device=dev)
+    b = T.HeffBatch()
+
+    def seq():
+        for h, x, y in zip(hs, xs, ys):
+            h.matvec_device(x.data_ptr(), y.data_ptr(), main.cuda_stream)
+
+    def multi():
+        ev = torch.cuda.Event()
+        ev.record(main)
+        for h, x, y, st in zip(hs, xs, ys, streams):
+            st.wait_event(ev)
+            h.matvec_device(x.data_ptr(), y.data_ptr(), st.cuda_stream)
+        for st in streams:
+            main.wait_stream(st)
+
+    def batched():
+        b.matvec_device(hs, [x.data_ptr() for x in xs], [y.data_ptr() for y in ys], main.cuda_stream)
+
+    out = {"workload": "nn_hubbard_16_chain_D128_15_steps", "objects": len(hs),
+           "sizes": [h.size for h in hs]}
+    ref = None
+    for name, fn in (("one_stream", seq), ("own_streams", multi), ("batched", batched)):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(main)
+        for _ in range(reps):
+            fn()
+        e1.record(main)
+        torch.cuda.synchronize()
+        out[name + "_ms"] = e0.elapsed_time(e1) / reps
+        res = [y.cpu().numpy().copy() for y in ys]
+        if ref is None:
+            ref = res
+        else:
+            out[name + "_equal"] = all(np.array_equal(a, r) for a, r in zip(res, ref))
+    out["speedup_vs_one_stream"] = out["one_stream_ms"] / out["batched_ms"]
+    out["speedup_vs_own_streams"] = out["own_streams_ms"] / out["batched_ms"]
+    b.close()
+    print("# batch: %s" % out, file=sys.stderr, flush=True)
+    return out
+
+
 def measure_solve(ctx, step, tol):
     """Lowest eigenpair of the step's effective Hamiltonian built from the
     Hamiltonian's own operators: every bond's operators renormalised on the
@@ -539,7 +605,7 @@ def main():
     if args.bond_dim is None:
         args.bond_dim = wl["bond_dim"]
     if args.workload != "qc24":  # the other legs belong to the default workload
-        args.no_solve = args.no_sweep = args.no_rows = args.no_serve = True
+        args.no_solve = args.no_sweep = args.no_rows = args.no_serve = args.no_batch = True
     t0 = time.perf_counter()
     ctx, step = build_step(sites, args.bond_dim, args.workload)
     t_setup = time.perf_counter() - t0
@@ -645,6 +711,8 @@ def main():
     out["autotune"] = {p: results[p]["tuning"] for p in results}
     if not args.no_solve:
         out["solve"] = measure_solve(ctx, step, args.solve_tol)
+    if not args.no_batch:
+        out["batch"] = measure_batch()
     if not args.no_sweep:
         out["sweep"] = measure_sweep()
         if args.qc24_sweep:

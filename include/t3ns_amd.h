@@ -135,6 +135,23 @@ int64_t t3ns_heff_size(const t3ns_heff* h);
 int t3ns_heff_stats(const t3ns_heff* h, int64_t* nblocks, int64_t* npairs, int64_t* nterms,
                     double* flops);
 
+/* Batched matvecs (no counterpart in the reference: its solves run one at
+ * a time): the matvecs of n objects of one precision go through one launch
+ * per kernel, their work units sharing the persistent grid, so that the
+ * matvecs of concurrent small solves fill the GPU together.  Each result is
+ * bitwise that of the object's own matvec.  A batch object keeps its
+ * buffers between calls; calls on one batch must use one stream.
+ * batch_matvec takes device arrays (dx[i], dy[i] of t3ns_heff_size(hs[i])
+ * doubles, on `stream`); batch_matvec_host host arrays (each object at
+ * most once), synchronously. */
+typedef struct t3ns_heff_batch t3ns_heff_batch;
+int t3ns_heff_batch_create(t3ns_heff_batch** b);
+int t3ns_heff_batch_destroy(t3ns_heff_batch* b);
+int t3ns_heff_batch_matvec(t3ns_heff_batch* b, t3ns_heff* const* hs, const double* const* dx,
+                           double* const* dy, int n, void* stream);
+int t3ns_heff_batch_matvec_host(t3ns_heff_batch* b, t3ns_heff* const* hs, const double* const* x,
+                                double* const* y, int n);
+
 /* Autotuner (no counterpart in the reference: its matvec has no tunable
  * work decomposition).  Times matvecs under at most max_trials settings of
  * the work-list cost model (chunk price, fused-operand price, fused middle

@@ -198,6 +198,12 @@ def lib():
         "t3ns_heff_plan_table": (C.c_int64, [P, c_int_p, c_dbl_p]),
         "t3ns_heff_kernel_stats": (C.c_int, [P, c_dbl_p, C.c_int]),
         "t3ns_heff_autotune": (C.c_int, [P, C.c_int, c_dbl_p, C.c_int]),
+        "t3ns_heff_batch_create": (C.c_int, [C.POINTER(P)]),
+        "t3ns_heff_batch_destroy": (C.c_int, [P]),
+        "t3ns_heff_batch_matvec": (C.c_int, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.c_int,
+                                             P]),
+        "t3ns_heff_batch_matvec_host": (C.c_int, [P, C.POINTER(P), C.POINTER(c_dbl_p),
+                                                  C.POINTER(c_dbl_p), C.c_int]),
         "t3ns_bupdate_plan_stats": (C.c_int, [C.POINTER(BUpdateDesc), C.POINTER(RopsDesc), c_dbl_p,
                                               C.c_int]),
         "t3ns_ckpt_write": (C.c_int, [P, C.c_char_p, C.c_int, c_int_p, C.POINTER(c_i64_p),

@@ -252,6 +252,70 @@ int t3ns_heff_use_graphs(t3ns_heff* h, int on) {
     return guarded([&] { h->dev->set_graphs(on != 0); });
 }
 
+struct t3ns_heff_batch {
+    HeffBatch b;
+    hipStream_t stream = nullptr;
+};
+
+int t3ns_heff_batch_create(t3ns_heff_batch** out) {
+    if (!out) return fail(T3NS_ERR_INVALID, "null argument");
+    return guarded([&] {
+        auto b = std::make_unique<t3ns_heff_batch>();
+        T3_HIP_CHECK(hipStreamCreate(&b->stream));
+        *out = b.release();
+    });
+}
+
+int t3ns_heff_batch_destroy(t3ns_heff_batch* b) {
+    if (!b) return T3NS_OK;
+    return guarded([&] {
+        if (b->stream) {
+            (void)hipStreamSynchronize(b->stream);
+            (void)hipStreamDestroy(b->stream);
+        }
+        delete b;
+    });
+}
+
+int t3ns_heff_batch_matvec(t3ns_heff_batch* b, t3ns_heff* const* hs, const double* const* dx,
+                           double* const* dy, int n, void* stream) {
+    if (!b || n < 0 || (n > 0 && (!hs || !dx || !dy))) return fail(T3NS_ERR_INVALID, "null argument");
+    return guarded([&] {
+        std::vector<HeffDevice*> d(n);
+        for (int i = 0; i < n; ++i) {
+            if (!hs[i] || !dx[i] || !dy[i]) throw std::invalid_argument("batched matvec: null entry");
+            d[i] = hs[i]->dev.get();
+        }
+        b->b.matvec(d.data(), dx, dy, n, static_cast<hipStream_t>(stream));
+    });
+}
+
+int t3ns_heff_batch_matvec_host(t3ns_heff_batch* b, t3ns_heff* const* hs, const double* const* x,
+                                double* const* y, int n) {
+    if (!b || n < 0 || (n > 0 && (!hs || !x || !y))) return fail(T3NS_ERR_INVALID, "null argument");
+    return guarded([&] {
+        std::vector<HeffDevice*> d(n);
+        std::vector<const double*> dx(n);
+        std::vector<double*> dy(n);
+        for (int i = 0; i < n; ++i) {
+            if (!hs[i] || !x[i] || !y[i]) throw std::invalid_argument("batched matvec: null entry");
+            for (int j = 0; j < i; ++j)
+                if (hs[j] == hs[i]) throw std::invalid_argument("batched matvec: an object twice");
+            d[i] = hs[i]->dev.get();
+            const size_t sz = d[i]->size();
+            T3_HIP_CHECK(hipMemcpyAsync(hs[i]->dvec.get(), x[i], sz * sizeof(double),
+                                        hipMemcpyHostToDevice, b->stream));
+            dx[i] = hs[i]->dvec.get();
+            dy[i] = hs[i]->dres.get();
+        }
+        b->b.matvec(d.data(), dx.data(), dy.data(), n, b->stream);
+        for (int i = 0; i < n; ++i)
+            T3_HIP_CHECK(hipMemcpyAsync(y[i], dy[i], d[i]->size() * sizeof(double),
+                                        hipMemcpyDeviceToHost, b->stream));
+        T3_HIP_CHECK(hipStreamSynchronize(b->stream));
+    });
+}
+
 int t3ns_heff_autotune(t3ns_heff* h, int max_trials, double* out, int n) {
     if (!h || (n > 0 && !out)) return fail(T3NS_ERR_INVALID, "null argument");
     if (max_trials < 1) return fail(T3NS_ERR_INVALID, "max_trials must be at least 1");

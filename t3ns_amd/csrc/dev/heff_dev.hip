@@ -831,19 +831,16 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
     const int ntnC = (gC.dn + 15) >> 4;
     kC.G = gC.tf * ntnC >= NW || kC.tng > 1 ? 1 : NW / (gC.tf * ntnC);
     kC.total = kC.tf * kC.tng * kC.G;
-    // When every wave has at most one stage C task, its accumulators stay in
-    // registers across the pair's chunks and Y is read and written once per
-    // pair instead of once per chunk.
-    const bool pC = !fz && !vC && !rbC && kC.total <= NW;
-    const bool pCv = vC && tC.total <= NW;  // the same on the vector-unit path
+    // On the vector-unit path, when every wave has at most one stage C task,
+    // its accumulators stay in registers across the pair's chunks and Y is
+    // read and written once per pair instead of once per chunk.  (Kept
+    // across chunks on the matrix-core path too, they took 46 registers of
+    // the kernel's 256 and saved nothing measurable.)
+    const bool pCv = vC && tC.total <= NW;
     const VLane LV = vlane(gC, wave % tC.nft);
     const int grpV = wave / tC.nft;
     const int baseC = kC.tf * kC.tng;
-    const int rC = wave % baseC, grpC = wave / baseC;
-    const RowLane LC = row_lane<T>(gC, rC % kC.tf, (rC / kC.tf) * kTN);
-    V accC[kTN];
-#pragma unroll
-    for (int q = 0; q < kTN; ++q) accC[q] = V{0, 0, 0, 0};
+    V accV = V{0, 0, 0, 0};
     // stage C partial tiles of item groups, summed in group order into Y
     auto reduce_parts = [&]() {
         __syncthreads();
@@ -1097,7 +1094,7 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
             };
             if (pCv) {
                 if (wave < tC.total)
-                    for (int t = grpV; t < ncs; t += tC.G) cmac(t, LV, accC[0]);
+                    for (int t = grpV; t < ncs; t += tC.G) cmac(t, LV, accV);
             } else {
                 for (int task = wave; task < tC.total; task += NW) {
                     const int ft = task % tC.nft, grp = task / tC.nft;
@@ -1122,14 +1119,6 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
                     else blk_mac_terms<T, OT, false>(R, gC, L, acc);
                 }
                 blk_put<T>(Y, gC, L, acc, true);
-            }
-        } else if (pC) {  // stage C into the pair's persistent accumulators
-            if (wave < kC.total) {
-                for (int t0 = grpC; t0 < ncs; t0 += kC.G * kTU) {
-                    const Terms<T, OT> R = c_terms<T, OT>(rec, t0, kC.G, ncs, isdmrg, W1, w1, W2, w2, opc);
-                    if (isdmrg) row_mac_terms<T, OT, true>(R, gC, LC, accC);
-                    else row_mac_terms<T, OT, false>(R, gC, LC, accC);
-                }
             }
         } else {  // stage C: Y += sum_t O_c[t] x_c W2[t]  (DMRG: alpha_t O_c[t] x_c W1[slot t])
             for (int task = wave; task < kC.total; task += NW) {
@@ -1165,19 +1154,10 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
     }
     if (pCv) {  // the pair's vector-unit stage C sums into Y
         if (tC.G == 1) {
-            if (wave < tC.total) vput<T>(Y, gC, LV, accC[0], true);
+            if (wave < tC.total) vput<T>(Y, gC, LV, accV, true);
         } else {
-            if (wave < tC.total) vstore_part(accC[0], grpV, wave % tC.nft);
+            if (wave < tC.total) vstore_part(accV, grpV, wave % tC.nft);
             vreduce_parts();
-            __syncthreads();
-        }
-    }
-    if (pC) {  // the pair's stage C sums into Y
-        if (kC.G == 1) {
-            if (wave < kC.total) row_put<T>(Y, gC, LC, accC, true);
-        } else {
-            if (wave < kC.total) store_part(accC, LC, grpC, rC);
-            reduce_parts();
             __syncthreads();
         }
     }
@@ -1191,16 +1171,18 @@ __device__ __forceinline__ void pair_chunks(const DevPair& P, const int* bd,
 //   kModeLdsW    input block read from global memory, stage outputs in LDS;
 //   kModeGlobal  stage outputs in the workgroup's global scratch region.
 enum { kModeLds = 0, kModeLdsW = 1, kModeGlobal = 2 };
-template <typename T, typename OT, int POOL, int NT, int MODE>
+// Units of several objects can share one launch (MULTI, batched matvec):
+// every unit names the object it belongs to (DevBlock::ref), whose lists,
+// operator arenas and buffers come from refs[ref].  A single object's launch
+// takes its view by value (`one`, in kernel-argument registers for the whole
+// launch).  An object's input and output vectors come from its view, or
+// from x and y where the view leaves them null.
+template <typename T, typename OT, int POOL, int NT, int MODE, bool MULTI>
 __global__ void __launch_bounds__(NT)
-    heff_matvec_kernel(const DevBlock* __restrict__ blocks, int nblocks,
-                       const DevPair* __restrict__ pairs, const DevChunk* __restrict__ chunks,
-                       const int* __restrict__ runs, const DevTerm* __restrict__ terms,
-                       const OT* __restrict__ op0, const OT* __restrict__ op1,
-                       const OT* __restrict__ op2, const T* __restrict__ x, T* __restrict__ y,
-                       T* __restrict__ scratch, int64_t scratch_stride, int* __restrict__ next,
-                       const int* __restrict__ seg, int nseg, T* __restrict__ partial, int isdmrg,
-                       int m0, int m1, int m2) {
+    heff_matvec_kernel(const DevBlock* __restrict__ blocks, int nblocks, const HeffRef one,
+                       const HeffRef* __restrict__ refs, const T* __restrict__ x,
+                       T* __restrict__ y, int* __restrict__ next, const int* __restrict__ seg,
+                       int nseg) {
     constexpr int kStage = MODE == kModeGlobal ? 0 : POOL;
     __shared__ T lds[kStage + kPartial];
     __shared__ ChunkRec rec[2];
@@ -1227,27 +1209,40 @@ __global__ void __launch_bounds__(NT)
             cur_block = u;
         }
         __syncthreads();
-        const int bi = cur_block;
+        const int bi = uni(cur_block);  // uniform: the unit and its object load through scalar loads
         __syncthreads();
         if (bi >= nblocks) break;
         const DevBlock B = blocks[bi];
-        T* Y = (B.to_part ? partial : y) + B.y_off;
+        const HeffRef& R = MULTI ? refs[uni(B.ref)] : one;
+        const DevPair* __restrict__ pairs = R.pairs;
+        const DevChunk* __restrict__ chunks = R.chunks;
+        const int* __restrict__ runs = R.runs;
+        const DevTerm* __restrict__ terms = R.terms;
+        const OT* __restrict__ op0 = static_cast<const OT*>(R.ops[0]);
+        const OT* __restrict__ op1 = static_cast<const OT*>(R.ops[1]);
+        const OT* __restrict__ op2 = static_cast<const OT*>(R.ops[2]);
+        const T* __restrict__ xb = R.x ? static_cast<const T*>(R.x) : x;
+        T* __restrict__ Y =
+            (B.to_part ? static_cast<T*>(R.partial) : (R.y ? static_cast<T*>(R.y) : y)) + B.y_off;
+        const int isdmrg = uni(R.isdmrg), m0 = uni(R.map[0]), m1 = uni(R.map[1]),
+                  m2 = uni(R.map[2]);
+        constexpr bool FZ = kFused<T> && MODE != kModeGlobal;
         for (int pi = B.pair_begin; pi < B.pair_end; ++pi) {
             const DevPair P = pairs[pi];
             if (MODE == kModeLdsW) {
-                pair_chunks<T, OT, NT, kFused<T> && MODE != kModeGlobal>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, lds, Y,
-                                   part, rec, isdmrg, m0, m1, m2);
+                pair_chunks<T, OT, NT, FZ>(P, B.d, chunks, runs, terms, op0, op1, op2, xb + P.x_off,
+                                           lds, Y, part, rec, isdmrg, m0, m1, m2);
             } else if (MODE == kModeGlobal) {
-                T* gw1 = scratch + (int64_t)blockIdx.x * scratch_stride;
-                pair_chunks<T, OT, NT, kFused<T> && MODE != kModeGlobal>(P, B.d, chunks, runs, terms, op0, op1, op2, x + P.x_off, gw1, Y,
-                                   part, rec, isdmrg, m0, m1, m2);
+                T* gw1 = static_cast<T*>(R.scratch) + (int64_t)blockIdx.x * R.scratch_stride;
+                pair_chunks<T, OT, NT, FZ>(P, B.d, chunks, runs, terms, op0, op1, op2, xb + P.x_off,
+                                           gw1, Y, part, rec, isdmrg, m0, m1, m2);
             } else {
-                const T* X = x + P.x_off;
+                const T* X = xb + P.x_off;
                 const int xsz = P.d[0] * P.d[1] * P.d[2];
                 for (int i = threadIdx.x; i < xsz; i += NT) lds[i] = X[i];
                 __syncthreads();
-                pair_chunks<T, OT, NT, kFused<T> && MODE != kModeGlobal>(P, B.d, chunks, runs, terms, op0, op1, op2, lds,
-                                   lds + P.w1_off, Y, part, rec, isdmrg, m0, m1, m2);
+                pair_chunks<T, OT, NT, FZ>(P, B.d, chunks, runs, terms, op0, op1, op2, lds,
+                                           lds + P.w1_off, Y, part, rec, isdmrg, m0, m1, m2);
             }
         }
     }
@@ -1596,6 +1591,7 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream,
     }
     T3_HIP_CHECK(hipGetLastError());
     T3_HIP_CHECK(hipStreamSynchronize(stream));
+    build_ref(stream);
 }
 
 // Work lists of the current tuning (everything the matvec kernels read
@@ -2022,6 +2018,7 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
         v.insert(v.end(), rest.begin(), rest.end());
     };
     std::vector<DevBlock> hbs;
+    std::vector<double> hbcost;
     std::vector<int> hseg;
     for (int m = 0; m < 3; ++m) {
         std::vector<int> v;
@@ -2045,7 +2042,10 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
             if (sched == 0 || sched == 2) by_cost(segs[sg]);
             else by_block(segs[sg]);
             sb[sg] = (int)hbs.size() - base;
-            for (int i : segs[sg]) hbs.push_back(units[i]);
+            for (int i : segs[sg]) {
+                hbs.push_back(units[i]);
+                hbcost.push_back(ucost[i]);
+            }
         }
         sb[nseg] = (int)hbs.size() - base;
         (m == 0 ? nfast_ : m == 1 ? nxg_ : nglob_) = (int)hbs.size() - base;
@@ -2065,6 +2065,8 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
     stats_.mfma_steps = stats_.stage_steps[0] + stats_.stage_steps[1] + stats_.stage_steps[2];
 
     blocks_.upload(hbs, stream);
+    hunits_ = hbs;
+    hucost_ = hbcost;
     pairs_.upload(hp, stream);
     chunks_.upload(hc, stream);
     runs_.upload(hr, stream);
@@ -2087,14 +2089,14 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
 void HeffDevice::retune(const HeffInput& in, const HeffTuning& t, hipStream_t stream) {
     tune_ = t;
     build_lists(in, stream);
+    build_ref(stream);
 }
 
 
 HeffDevice::~HeffDevice() { drop_graphs(); }
 
 template <typename T, typename OT, int POOL>
-void HeffDevice::launch(const T* x, T* y, const OT* const* ops, T* scratch, T* part,
-                        hipStream_t s) {
+void HeffDevice::launch(const T* x, T* y, T* part, hipStream_t s) {
     hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(size_)), dim3(256), 0, s, y, size_, (T)0);
     hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s, counter_.get(),
                        (int64_t)(3 * kSegments), 0);
@@ -2102,29 +2104,48 @@ void HeffDevice::launch(const T* x, T* y, const OT* const* ops, T* scratch, T* p
         hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(part_elems_)), dim3(256), 0, s, part,
                            part_elems_, (T)0);
     if (nfast_)
-        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeLds>),
+        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeLds, false>),
                            dim3(std::min(nfast_, ncu_)), dim3(kThreads), 0, s, blocks_.get(),
-                           nfast_, pairs_.get(), chunks_.get(), runs_.get(), terms_.get(), ops[0],
-                           ops[1], ops[2], x, y, scratch, stats_.scratch_per_wg, counter_.get(),
-                           seg_.get(), nseg_[0], part, isdmrg_, map_[0], map_[1], map_[2]);
+                           nfast_, href_, (const HeffRef*)nullptr, x, y, counter_.get(),
+                           seg_.get(), nseg_[0]);
     if (nxg_)
-        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeLdsW>),
+        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeLdsW, false>),
                            dim3(std::min(nxg_, ncu_)), dim3(kThreads), 0, s,
-                           blocks_.get() + nfast_, nxg_, pairs_.get(), chunks_.get(), runs_.get(),
-                           terms_.get(), ops[0], ops[1], ops[2], x, y, scratch,
-                           stats_.scratch_per_wg, counter_.get() + kSegments,
-                           seg_.get() + (kSegments + 1), nseg_[1], part, isdmrg_, map_[0], map_[1],
-                           map_[2]);
+                           blocks_.get() + nfast_, nxg_, href_, (const HeffRef*)nullptr, x, y,
+                           counter_.get() + kSegments, seg_.get() + (kSegments + 1), nseg_[1]);
     if (nglob_)
-        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeGlobal>),
+        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeGlobal, false>),
                            dim3(nwg_glob_), dim3(kThreads), 0, s, blocks_.get() + nfast_ + nxg_,
-                           nglob_, pairs_.get(), chunks_.get(), runs_.get(), terms_.get(), ops[0],
-                           ops[1], ops[2], x, y, scratch, stats_.scratch_per_wg,
+                           nglob_, href_, (const HeffRef*)nullptr, x, y,
                            counter_.get() + 2 * kSegments, seg_.get() + 2 * (kSegments + 1),
-                           nseg_[2], part, isdmrg_, map_[0], map_[1], map_[2]);
+                           nseg_[2]);
     if (nsplits_)
         hipLaunchKernelGGL((split_reduce_kernel<T>), dim3(std::min(nsplits_, 4096)), dim3(256), 0,
                            s, splits_.get(), nsplits_, part, y);
+}
+
+// The object's device view for the kernels (input and output left to the
+// launch).  Rebuilt whenever lists or arenas are (re)allocated.
+void HeffDevice::build_ref(hipStream_t s) {
+    HeffRef r{};
+    r.pairs = pairs_.get();
+    r.chunks = chunks_.get();
+    r.runs = runs_.get();
+    r.terms = terms_.get();
+    for (int k = 0; k < 3; ++k)
+        r.ops[k] = prec_ == PREC_FP64 ? (const void*)ops64_[k].get()
+                   : prec_ == PREC_FP32 ? (const void*)ops32_[k].get()
+                                        : (const void*)ops16_[k].get();
+    r.x = nullptr;
+    r.y = nullptr;
+    r.partial = prec_ == PREC_FP64 ? (void*)part64_.get() : (void*)part32_.get();
+    r.scratch = prec_ == PREC_FP64 ? (void*)scratch64_.get() : (void*)scratch32_.get();
+    r.scratch_stride = stats_.scratch_per_wg;
+    r.isdmrg = isdmrg_;
+    for (int k = 0; k < 3; ++k) r.map[k] = map_[k];
+    href_ = r;
+    ref_.upload(&href_, 1, s);
+    T3_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 void HeffDevice::matvec(const double* dvec, double* dres, hipStream_t s) {
@@ -2168,23 +2189,124 @@ void HeffDevice::drop_graphs() {
 
 void HeffDevice::enqueue(const double* dvec, double* dres, hipStream_t s) {
     if (prec_ == PREC_FP64) {
-        const double* ops[3] = {ops64_[0].get(), ops64_[1].get(), ops64_[2].get()};
-        launch<double, double, kPool64>(dvec, dres, ops, scratch64_.get(), part64_.get(), s);
+        launch<double, double, kPool64>(dvec, dres, part64_.get(), s);
     } else {
         hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(size_)), dim3(256), 0, s,
                            dvec, x32_.get(), size_);
-        if (prec_ == PREC_BF16) {
-            const uint16_t* ops[3] = {ops16_[0].get(), ops16_[1].get(), ops16_[2].get()};
-            launch<float, uint16_t, kPool32>(x32_.get(), y32_.get(), ops, scratch32_.get(),
-                                             part32_.get(), s);
-        } else {
-            const float* ops[3] = {ops32_[0].get(), ops32_[1].get(), ops32_[2].get()};
-            launch<float, float, kPool32>(x32_.get(), y32_.get(), ops, scratch32_.get(),
-                                          part32_.get(), s);
-        }
+        if (prec_ == PREC_BF16)
+            launch<float, uint16_t, kPool32>(x32_.get(), y32_.get(), part32_.get(), s);
+        else
+            launch<float, float, kPool32>(x32_.get(), y32_.get(), part32_.get(), s);
         hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(size_)), dim3(256), 0, s,
                            y32_.get(), dres, size_);
     }
+    T3_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T, typename OT, int POOL>
+void HeffBatch::launch(HeffDevice* const* hs, const T* const* x, T* const* y, int n,
+                       hipStream_t s) {
+    // Units of the kernels with no global scratch from every object, one
+    // longest-first list per kernel; each unit names its object.
+    std::vector<DevBlock> units[2];
+    std::vector<double> cost[2];
+    std::vector<HeffRef> refs(n);
+    for (int i = 0; i < n; ++i) {
+        HeffDevice& h = *hs[i];
+        refs[i] = h.href_;
+        refs[i].x = x[i];
+        refs[i].y = y[i];
+        const int first[2] = {0, h.nfast_}, count[2] = {h.nfast_, h.nxg_};
+        for (int m = 0; m < 2; ++m)
+            for (int u = first[m]; u < first[m] + count[m]; ++u) {
+                DevBlock B = h.hunits_[u];
+                B.ref = i;
+                units[m].push_back(B);
+                cost[m].push_back(h.hucost_[u]);
+            }
+    }
+    std::vector<DevBlock> all;
+    int nmode[2];
+    for (int m = 0; m < 2; ++m) {
+        std::vector<int> idx(units[m].size());
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[m][a] > cost[m][b]; });
+        for (int k : idx) all.push_back(units[m][k]);
+        nmode[m] = (int)idx.size();
+    }
+    const std::vector<int> seg = {0, nmode[0], 0, nmode[1]};  // one segment per kernel
+    units_.upload(all, s);
+    refs_.upload(refs, s);
+    seg_.upload(seg, s);
+    counter_.reserve(2);
+    hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s, counter_.get(), (int64_t)2, 0);
+    for (int i = 0; i < n; ++i) {
+        HeffDevice& h = *hs[i];
+        hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(h.size_)), dim3(256), 0, s, y[i],
+                           h.size_, (T)0);
+        if (h.part_elems_)
+            hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(h.part_elems_)), dim3(256), 0, s,
+                               static_cast<T*>(h.href_.partial), h.part_elems_, (T)0);
+    }
+    const int ncu = hs[0]->ncu_;
+    if (nmode[0])
+        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeLds, true>),
+                           dim3(std::min(nmode[0], ncu)), dim3(kThreads), 0, s, units_.get(),
+                           nmode[0], HeffRef{}, refs_.get(), (const T*)nullptr, (T*)nullptr,
+                           counter_.get(), seg_.get(), 1);
+    if (nmode[1])
+        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeLdsW, true>),
+                           dim3(std::min(nmode[1], ncu)), dim3(kThreads), 0, s,
+                           units_.get() + nmode[0], nmode[1], HeffRef{}, refs_.get(),
+                           (const T*)nullptr, (T*)nullptr, counter_.get() + 1, seg_.get() + 2, 1);
+    // units that need global scratch run in their objects' own launches
+    // (the scratch is sized for the object's grid); then the split sums
+    for (int i = 0; i < n; ++i) {
+        HeffDevice& h = *hs[i];
+        if (h.nglob_) {
+            hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s,
+                               h.counter_.get() + 2 * HeffDevice::kSegments, (int64_t)1, 0);
+            hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeGlobal, false>),
+                               dim3(h.nwg_glob_), dim3(kThreads), 0, s,
+                               h.blocks_.get() + h.nfast_ + h.nxg_, h.nglob_, h.href_,
+                               (const HeffRef*)nullptr, x[i], y[i],
+                               h.counter_.get() + 2 * HeffDevice::kSegments,
+                               h.seg_.get() + 2 * (HeffDevice::kSegments + 1), h.nseg_[2]);
+        }
+        if (h.nsplits_)
+            hipLaunchKernelGGL((split_reduce_kernel<T>), dim3(std::min(h.nsplits_, 4096)), dim3(256),
+                               0, s, h.splits_.get(), h.nsplits_, static_cast<T*>(h.href_.partial),
+                               y[i]);
+    }
+    T3_HIP_CHECK(hipGetLastError());
+}
+
+void HeffBatch::matvec(HeffDevice* const* hs, const double* const* x, double* const* y, int n,
+                       hipStream_t s) {
+    if (n <= 0) return;
+    const int prec = hs[0]->prec_;
+    for (int i = 1; i < n; ++i)
+        if (hs[i]->prec_ != prec) throw std::invalid_argument("batched matvec: mixed precisions");
+    if (prec == PREC_FP64) {
+        launch<double, double, kPool64>(hs, x, y, n, s);
+        return;
+    }
+    std::vector<const float*> x32(n);
+    std::vector<float*> y32(n);
+    for (int i = 0; i < n; ++i) {
+        HeffDevice& h = *hs[i];
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(h.size_)), dim3(256), 0,
+                           s, x[i], h.x32_.get(), h.size_);
+        x32[i] = h.x32_.get();
+        y32[i] = h.y32_.get();
+    }
+    if (prec == PREC_BF16)
+        launch<float, uint16_t, kPool32>(hs, x32.data(), y32.data(), n, s);
+    else
+        launch<float, float, kPool32>(hs, x32.data(), y32.data(), n, s);
+    for (int i = 0; i < n; ++i)
+        hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(hs[i]->size_)), dim3(256),
+                           0, s, y32[i], y[i], hs[i]->size_);
     T3_HIP_CHECK(hipGetLastError());
 }
 

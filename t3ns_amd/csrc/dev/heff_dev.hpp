@@ -21,6 +21,26 @@ struct DevBlock {      // one work unit: an output block or a slice of its pairs
     int pair_begin;
     int pair_end;
     int to_part;       // 1: accumulate into the partial buffer
+    int ref;           // the object it belongs to (index into the launch's refs)
+};
+struct DevPair;
+struct DevChunk;
+struct DevTerm;
+// Device addresses of one object's work lists, operator arenas and buffers
+// (the kernels' view of a HeffDevice; a batched launch holds several).
+struct HeffRef {
+    const DevPair* pairs;
+    const DevChunk* chunks;
+    const int* runs;
+    const DevTerm* terms;
+    const void* ops[3];  // arenas in the storage type
+    const void* x;       // input vector in the arithmetic type (null: the launch's)
+    void* y;             // output vector (null: the launch's)
+    void* partial;       // partial copies of split output blocks
+    void* scratch;       // global scratch of the kModeGlobal kernel
+    int64_t scratch_stride;
+    int isdmrg;
+    int map[3];
 };
 struct DevSplit {      // an output block computed by several units
     int64_t y_off;     // block offset in the vector
@@ -99,7 +119,31 @@ struct HeffTuning {
     static HeffTuning from_env();
 };
 
+class HeffDevice;
+
+// Matvecs of several objects of one precision in one launch per kernel
+// (cross-solve batching): their work units share the persistent grid, so
+// small objects fill the GPU together.  Each object's result is bitwise the
+// one of its own matvec (a unit's work does not depend on its neighbours).
+// The buffers are reused by the next call; calls on one batch must share a
+// stream.
+class HeffBatch {
+public:
+    void matvec(HeffDevice* const* hs, const double* const* x, double* const* y, int n,
+                hipStream_t s);
+
+private:
+    template <typename T, typename OT, int POOL>
+    void launch(HeffDevice* const* hs, const T* const* x, T* const* y, int n, hipStream_t s);
+    DevBuf<DevBlock> units_;
+    DevBuf<HeffRef> refs_;
+    DevBuf<int> counter_;
+    DevBuf<int> seg_;
+};
+
 class HeffDevice {
+    friend class HeffBatch;
+
 public:
     // Builds the plan tables and uploads the operator arenas (host data in
     // in.ops[k]->tel).  `stream` may be null (default stream); `tuning`
@@ -135,11 +179,16 @@ private:
     void enqueue(const double* dvec, double* dres, hipStream_t s);
     void drop_graphs();
     template <typename T, typename OT, int POOL>
-    void launch(const T* x, T* y, const OT* const* ops, T* scratch, T* part, hipStream_t s);
+    void launch(const T* x, T* y, T* part, hipStream_t s);
+    void build_ref(hipStream_t s);
 
     HeffPlan plan_;
     HeffStats stats_;
     HeffTuning tune_;
+    HeffRef href_{};        // the kernels' view of this object (host copy)
+    std::vector<DevBlock> hunits_;  // the unit list (host copy) and unit costs
+    std::vector<double> hucost_;
+    DevBuf<HeffRef> ref_;   // ... and on the device
     std::vector<std::vector<std::vector<float>>> bscale_;  // BF16 block scales [leg][op][block]
     int prec_;
     int isdmrg_;

@@ -489,6 +489,51 @@ class Heff:
             pass
 
 
+class HeffBatch:
+    """Batched matvecs of several Heff objects of one precision in one
+    launch per kernel (t3ns_heff_batch_*): each result equals the object's
+    own matvec bitwise."""
+
+    def __init__(self):
+        b = C.c_void_p()
+        check(lib().t3ns_heff_batch_create(C.byref(b)), "t3ns_heff_batch_create")
+        self._b = b
+
+    def matvec(self, heffs, vecs):
+        """Host arrays in, host arrays out (one per object)."""
+        n = len(heffs)
+        if len(vecs) != n:
+            raise ValueError("one vector per object")
+        xs = [_vec(v, h.size, "vector") for h, v in zip(heffs, vecs)]
+        ys = [np.empty(h.size, dtype=np.float64) for h in heffs]
+        hs = (C.c_void_p * n)(*[h.handle for h in heffs])
+        xp = (_lib.c_dbl_p * n)(*[x.ctypes.data_as(_lib.c_dbl_p) for x in xs])
+        yp = (_lib.c_dbl_p * n)(*[y.ctypes.data_as(_lib.c_dbl_p) for y in ys])
+        check(lib().t3ns_heff_batch_matvec_host(self._b, hs, xp, yp, n),
+              "t3ns_heff_batch_matvec_host")
+        return ys
+
+    def matvec_device(self, heffs, dx_ptrs, dy_ptrs, stream_ptr=0):
+        """Device arrays (pointers) on `stream_ptr`, asynchronously."""
+        n = len(heffs)
+        hs = (C.c_void_p * n)(*[h.handle for h in heffs])
+        xp = (C.c_void_p * n)(*dx_ptrs)
+        yp = (C.c_void_p * n)(*dy_ptrs)
+        check(lib().t3ns_heff_batch_matvec(self._b, hs, xp, yp, n, C.c_void_p(stream_ptr)),
+              "t3ns_heff_batch_matvec")
+
+    def close(self):
+        if self._b:
+            lib().t3ns_heff_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def clear_tune_cache():
     """Forgets every setting the autotuner recorded in this process."""
     lib().t3ns_tune_cache_clear()
