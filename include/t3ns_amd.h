@@ -154,16 +154,19 @@ int t3ns_heff_batch_matvec_host(t3ns_heff_batch* b, t3ns_heff* const* hs, const 
 
 /* Autotuner (no counterpart in the reference: its matvec has no tunable
  * work decomposition).  Times matvecs under at most max_trials settings of
- * the work-list cost model (chunk price, fused-operand price, fused middle
- * stage on/off, unit schedule), keeps the fastest and records it for the
+ * the work-list cost model (grid shape, fused-operand price, chunk price,
+ * fused middle stage on/off), keeps the fastest and records it for the
  * object's block-size histogram; an object whose histogram was tuned before
  * in this process takes the recorded setting without timing.  out[0..n):
  * chunk price, fused-operand price, fused (0/1), schedule, ms per matvec
  * under the result, ms under the setting before tuning (0 when taken from
- * the record), settings timed, taken from the record (0/1).  Results change
- * by rounding only. */
+ * the record), settings timed, taken from the record (0/1), workgroups per
+ * CU.  Results change by rounding only. */
 int t3ns_heff_autotune(t3ns_heff* h, int max_trials, double* out, int n);
-/* The work-list setting t[0..4) (as out[0..4) above) read or imposed. */
+/* The work-list setting t[0..5) read or imposed: chunk price, fused-operand
+ * price, fused (0/1), schedule (0 cost, 1 block, 2 xcd, 3 xcdblock),
+ * workgroups per CU (1, 2 or 4: 8, 4 or 2 waves each with the whole, half
+ * or quarter LDS pool; 0 picks by the object's structure). */
 int t3ns_heff_get_tuning(const t3ns_heff* h, double* t);
 int t3ns_heff_set_tuning(t3ns_heff* h, const double* t);
 /* Forgets every recorded tuning of this process. */
@@ -174,7 +177,8 @@ void t3ns_tune_cache_clear(void);
  * outputs use global scratch, matrix-core steps per matvec (total and per
  * stage A/B/C), global scratch elements per workgroup, work units, output
  * blocks split over several units, pairs whose middle stage is fused into
- * the last.  Returns the number of statistics available. */
+ * the last, workgroups per CU of the grid.  Returns the number of
+ * statistics available. */
 int t3ns_heff_kernel_stats(const t3ns_heff* h, double* out, int n);
 
 /* The dispatch table the reference builds in its first matvec

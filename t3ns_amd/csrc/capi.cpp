@@ -329,14 +329,15 @@ int t3ns_heff_autotune(t3ns_heff* h, int max_trials, double* out, int n) {
                                            h->dres.get(), max_trials);
         const double v[] = {R.best.chunk_steps, R.best.fuse_over, (double)R.best.fuse,
                             (double)R.best.sched,  R.ms, R.baseline_ms, (double)R.trials,
-                            R.cached ? 1.0 : 0.0};
-        for (int i = 0; i < n && i < 8; ++i) out[i] = v[i];
+                            R.cached ? 1.0 : 0.0, (double)R.best.wpc};
+        for (int i = 0; i < n && i < 9; ++i) out[i] = v[i];
     });
 }
 
 int t3ns_heff_set_tuning(t3ns_heff* h, const double* t) {
     if (!h || !t) return fail(T3NS_ERR_INVALID, "null argument");
-    if (!(t[0] >= 0) || !(t[1] >= 0) || t[3] < 0 || t[3] > 3)
+    if (!(t[0] >= 0) || !(t[1] >= 0) || t[3] < 0 || t[3] > 3 ||
+        (t[4] != 0 && t[4] != 1 && t[4] != 2 && t[4] != 4))
         return fail(T3NS_ERR_INVALID, "tuning out of range");
     return guarded([&] {
         HeffTuning q;
@@ -344,6 +345,7 @@ int t3ns_heff_set_tuning(t3ns_heff* h, const double* t) {
         q.fuse_over = t[1];
         q.fuse = t[2] != 0;
         q.sched = (int)t[3];
+        q.wpc = (int)t[4];
         h->dev->retune(h->host.in, q, h->stream);
     });
 }
@@ -355,6 +357,7 @@ int t3ns_heff_get_tuning(const t3ns_heff* h, double* t) {
     t[1] = q.fuse_over;
     t[2] = q.fuse;
     t[3] = q.sched;
+    t[4] = q.wpc;
     return T3NS_OK;
 }
 
@@ -1055,7 +1058,8 @@ int t3ns_heff_kernel_stats(const t3ns_heff* h, double* out, int n) {
     const double v[] = {(double)s.nchunks, (double)s.nruns, (double)s.pairs_global_x,
                         (double)s.pairs_global_w, s.mfma_steps, s.stage_steps[0],
                         s.stage_steps[1], s.stage_steps[2], (double)s.scratch_per_wg,
-                        (double)s.nunits, (double)s.nsplit_blocks, (double)s.pairs_fused};
+                        (double)s.nunits, (double)s.nsplit_blocks, (double)s.pairs_fused,
+                        (double)h->dev->wpc()};
     const int m = (int)(sizeof(v) / sizeof(v[0]));
     for (int i = 0; i < n && i < m; ++i) out[i] = v[i];
     return m;

@@ -13,13 +13,26 @@ namespace t3 {
 
 namespace {
 constexpr int kThreads = 512;  // 8 waves
-constexpr int kWaves = kThreads / 64;
 constexpr int kDiagThreads = 256;
 // LDS pool of a workgroup (elements), laid out per pair: the input block,
 // then the first-stage outputs of a chunk, then its second-stage outputs
-// (chunks are cut to fit); the stage-C partial tiles have their own region.
-constexpr int kPool64 = 16384, kPool32 = 32768;
-constexpr int kPartial = kWaves * 256;
+// (chunks are cut to fit); the stage-C partial tiles (256 values per wave)
+// have their own region.  Three shapes of the persistent grid: WPC
+// workgroups of 8 / WPC waves per CU (WPC = 1, 2, 4), each with the pool
+// that lets WPC of them share a CU's 160 KiB of LDS (with the chunk
+// records and partial tiles); smaller workgroups wait on memory and
+// barriers independently of each other.
+template <typename T, int WPC>
+struct Grid {
+    static constexpr int NT = kThreads / WPC;
+    static constexpr int POOL = sizeof(T) == 8 ? (WPC == 4 ? 3072 : WPC == 2 ? 7680 : 16384)
+                                               : (WPC == 4 ? 6912 : WPC == 2 ? 15360 : 32768);
+};
+constexpr int kShapes = 3;  // grid shapes: WPC = 1 << shape
+// Terms of a chunk: one chunk record per thread, and one more group bound
+// than groups.
+constexpr int max_chunk_terms(int nt) { return nt - 1 < 128 ? nt - 1 : 128; }
+constexpr int kPool64 = Grid<double, 1>::POOL, kPool32 = Grid<float, 1>::POOL;
 constexpr int kMaxChunkTerms = 128;  // chunk records staged by one thread each
 // Global scratch per workgroup for pairs whose stage outputs exceed LDS.
 constexpr int64_t kGlobalChunkElems = 1 << 18;
@@ -1177,14 +1190,15 @@ enum { kModeLds = 0, kModeLdsW = 1, kModeGlobal = 2 };
 // takes its view by value (`one`, in kernel-argument registers for the whole
 // launch).  An object's input and output vectors come from its view, or
 // from x and y where the view leaves them null.
+// Two waves per SIMD in either grid shape: at most 256 registers per lane.
 template <typename T, typename OT, int POOL, int NT, int MODE, bool MULTI>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2)))
     heff_matvec_kernel(const DevBlock* __restrict__ blocks, int nblocks, const HeffRef one,
                        const HeffRef* __restrict__ refs, const T* __restrict__ x,
                        T* __restrict__ y, int* __restrict__ next, const int* __restrict__ seg,
                        int nseg) {
     constexpr int kStage = MODE == kModeGlobal ? 0 : POOL;
-    __shared__ T lds[kStage + kPartial];
+    __shared__ T lds[kStage + NT / 64 * 256];
     __shared__ ChunkRec rec[2];
     __shared__ int cur_block;
     T* part = lds + kStage;
@@ -1452,6 +1466,7 @@ HeffTuning HeffTuning::from_env() {
     if (const char* cs = std::getenv("T3NS_HEFF_CHUNK_STEPS")) t.chunk_steps = std::atof(cs);
     if (const char* fv = std::getenv("T3NS_HEFF_FUSE")) t.fuse = std::atoi(fv) != 0;
     if (const char* fo = std::getenv("T3NS_HEFF_FUSE_OVER")) t.fuse_over = std::atof(fo);
+    if (const char* wv = std::getenv("T3NS_HEFF_WPC")) t.wpc = std::atoi(wv);
     if (const char* sv = std::getenv("T3NS_HEFF_SCHED")) {
         const char* names[4] = {"cost", "block", "xcd", "xcdblock"};
         t.sched = -1;
@@ -1591,12 +1606,29 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream,
     }
     T3_HIP_CHECK(hipGetLastError());
     T3_HIP_CHECK(hipStreamSynchronize(stream));
-    build_ref(stream);
+    build_ref();
 }
 
 // Work lists of the current tuning (everything the matvec kernels read
-// besides the vectors and operator arenas).  Rebuilt by retune().
+// besides the vectors and operator arenas).  Rebuilt by retune().  A
+// tuning that leaves the grid shape open (wpc 0) builds the lists for one
+// workgroup per CU and keeps them unless a smaller pool would cut its pairs
+// into hardly more chunks (the chunks' LDS demand, re-cut for the pools of
+// 2 and 4 workgroups per CU, grows by at most a tenth and no pair loses its
+// LDS placement): then the lists are built again for the most workgroups
+// that pass.  Smaller workgroups wait on memory and barriers independently,
+// which pays where chunks are short; where they are long the whole pool
+// keeps them fewer.
 void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
+    const int w0 = tune_.wpc ? tune_.wpc : 1;
+    const int w = build_lists_at(in, stream, w0, tune_.wpc == 0);
+    if (w != w0) build_lists_at(in, stream, w, false);
+}
+
+// Builds and uploads the lists for `wpc` workgroups per CU and returns
+// wpc; with `pick`, returns the shape to build instead (uploading nothing)
+// when the estimate above prefers another.
+int HeffDevice::build_lists_at(const HeffInput& in, hipStream_t stream, int wpc, bool pick) {
     drop_graphs();  // captured launches point at the old lists
     {
         const HeffStats old = stats_;
@@ -1609,7 +1641,13 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
     // grouping).
     const bool f32 = prec_ != PREC_FP64;
     const bool bf16 = prec_ == PREC_BF16;
-    int pool = f32 ? kPool32 : kPool64;
+    if (wpc != 1 && wpc != 2 && wpc != 4)
+        throw std::invalid_argument("workgroups per CU must be 0 (by structure), 1, 2 or 4");
+    wpc_ = wpc;
+    int pool = f32 ? (wpc_ == 4 ? Grid<float, 4>::POOL : wpc_ == 2 ? Grid<float, 2>::POOL : kPool32)
+                   : (wpc_ == 4 ? Grid<double, 4>::POOL
+                                : wpc_ == 2 ? Grid<double, 2>::POOL : kPool64);
+    const int chunk_terms = max_chunk_terms(kThreads / wpc_);
     bool x_in_lds = true;
     // T3NS_HEFF_PLACEMENT=ldsw reads every input block from global memory,
     // =global keeps every stage output in global scratch as well: the paths
@@ -1662,7 +1700,13 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
         double cost = 0;
         HeffStats st;
         DevBlock B{};
+        // chunks re-cut for the pools of 2 and 4 workgroups per CU (pick)
+        double est[2] = {0, 0};
+        int64_t lost[2] = {0, 0};  // pairs that would lose their LDS placement
     };
+    const int qpool[2] = {f32 ? Grid<float, 2>::POOL : Grid<double, 2>::POOL,
+                          f32 ? Grid<float, 4>::POOL : Grid<double, 4>::POOL};
+    const int qterms[2] = {max_chunk_terms(Grid<double, 2>::NT), max_chunk_terms(Grid<double, 4>::NT)};
     std::vector<Lists> lists(plan_.nrblocks);
     auto build_block = [&](int sb) {
         Lists& L = lists[sb];
@@ -1781,7 +1825,7 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
                     : P.fuse ? (int64_t)(nterms + 1) * kFuseStage
                              : (int64_t)((nterms == 0 ? 0 : (group ? (int)cops.size() : nterms)) + newc) *
                                    P.w2;
-                const bool fits = nterms < kMaxChunkTerms &&
+                const bool fits = nterms < chunk_terms &&
                                   (P.wlds ? r1 + r2 <= avail : r1 <= gcap && r2 <= gcap);
                 if (nterms > 0 && !fits) {
                     close((int)t);
@@ -1850,12 +1894,43 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
                 L.cost += c;
                 L.pcost.push_back(c);
             }
+            if (pick) {
+                // LDS demand of the pair's chunks (first- and second-stage
+                // outputs), spread over the smaller pools
+                double dem = 0, ntt = 0;
+                for (int ci = P.chunk_begin; ci < P.chunk_end; ++ci) {
+                    const DevChunk& Q = L.hc[ci];
+                    const double nt = Q.term_end - Q.term_begin;
+                    dem += (double)(Q.run_end - Q.run_begin) * P.w1 +
+                           (isdmrg_ ? 0.0 : P.fuse ? nt * kFuseStage : (double)Q.ncs * P.w2);
+                    ntt += nt;
+                }
+                for (int q = 0; q < 2; ++q) {
+                    const bool xq = x_in_lds && xsz + wmin <= qpool[q];
+                    if (wmin > qpool[q] || (P.xlds && !xq)) ++L.lost[q];
+                    const double avq = std::max(1.0, (double)qpool[q] - (xq ? (double)xsz : 0.0));
+                    L.est[q] += std::max(std::ceil(dem / avq), std::ceil(ntt / qterms[q]));
+                }
+            }
             L.hp.push_back(P);
         }
         B.pair_end = (int)L.hp.size();
         L.B = B;
     };
     run_blocks_threaded(plan_.nrblocks, build_block, (double)plan_.pairs.size(), 2e4);
+    if (pick && wpc == 1 && pool > 0) {
+        double nch = 0, est[2] = {0, 0};
+        int64_t lost[2] = {0, 0};
+        for (const Lists& L : lists) {
+            nch += (double)L.hc.size();
+            for (int q = 0; q < 2; ++q) {
+                est[q] += L.est[q];
+                lost[q] += L.lost[q];
+            }
+        }
+        for (int q = 1; q >= 0; --q)
+            if (nch > 0 && !lost[q] && est[q] <= 1.1 * nch) return q == 1 ? 4 : 2;
+    }
     std::vector<DevPair> hp;
     std::vector<DevChunk> hc;
     std::vector<int> hr;
@@ -1925,7 +2000,7 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
     T3_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     double total = 0;
     for (double c : cost) total += c;
-    const double unit_cap = std::max(1.0, total / (4.0 * ncu));
+    const double unit_cap = std::max(1.0, total / (4.0 * ncu * wpc_));  // 4 units per workgroup
     std::vector<DevBlock> units;
     std::vector<double> ucost;
     std::vector<int> uglob;
@@ -2074,7 +2149,7 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
     // Persistent grids: one workgroup per CU; global scratch only for the
     // workgroups of the global-scratch kernel.
     ncu_ = ncu;
-    nwg_glob_ = std::max(1, std::min(nglob_, ncu));
+    nwg_glob_ = std::max(1, std::min(nglob_, ncu * wpc_));
     if (f32) scratch32_.alloc(nglob_ ? (size_t)nwg_glob_ * maxscratch : 1);
     else scratch64_.alloc(nglob_ ? (size_t)nwg_glob_ * maxscratch : 1);
     counter_.alloc(3 * kSegments);
@@ -2084,19 +2159,22 @@ void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
     else part64_.alloc(std::max<int64_t>(1, part_elems_));
     T3_HIP_CHECK(hipGetLastError());
     T3_HIP_CHECK(hipStreamSynchronize(stream));
+    return wpc;
 }
 
 void HeffDevice::retune(const HeffInput& in, const HeffTuning& t, hipStream_t stream) {
     tune_ = t;
     build_lists(in, stream);
-    build_ref(stream);
+    build_ref();
 }
 
 
 HeffDevice::~HeffDevice() { drop_graphs(); }
 
-template <typename T, typename OT, int POOL>
+template <typename T, typename OT, int WPC>
 void HeffDevice::launch(const T* x, T* y, T* part, hipStream_t s) {
+    constexpr int NT = Grid<T, WPC>::NT, POOL = Grid<T, WPC>::POOL;
+    const int nwg = ncu_ * WPC;
     hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(size_)), dim3(256), 0, s, y, size_, (T)0);
     hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s, counter_.get(),
                        (int64_t)(3 * kSegments), 0);
@@ -2104,29 +2182,35 @@ void HeffDevice::launch(const T* x, T* y, T* part, hipStream_t s) {
         hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(part_elems_)), dim3(256), 0, s, part,
                            part_elems_, (T)0);
     if (nfast_)
-        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeLds, false>),
-                           dim3(std::min(nfast_, ncu_)), dim3(kThreads), 0, s, blocks_.get(),
-                           nfast_, href_, (const HeffRef*)nullptr, x, y, counter_.get(),
-                           seg_.get(), nseg_[0]);
+        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, NT, kModeLds, false>),
+                           dim3(std::min(nfast_, nwg)), dim3(NT), 0, s, blocks_.get(), nfast_,
+                           href_, (const HeffRef*)nullptr, x, y, counter_.get(), seg_.get(),
+                           nseg_[0]);
     if (nxg_)
-        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeLdsW, false>),
-                           dim3(std::min(nxg_, ncu_)), dim3(kThreads), 0, s,
-                           blocks_.get() + nfast_, nxg_, href_, (const HeffRef*)nullptr, x, y,
+        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, NT, kModeLdsW, false>),
+                           dim3(std::min(nxg_, nwg)), dim3(NT), 0, s, blocks_.get() + nfast_,
+                           nxg_, href_, (const HeffRef*)nullptr, x, y,
                            counter_.get() + kSegments, seg_.get() + (kSegments + 1), nseg_[1]);
     if (nglob_)
-        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeGlobal, false>),
-                           dim3(nwg_glob_), dim3(kThreads), 0, s, blocks_.get() + nfast_ + nxg_,
-                           nglob_, href_, (const HeffRef*)nullptr, x, y,
-                           counter_.get() + 2 * kSegments, seg_.get() + 2 * (kSegments + 1),
-                           nseg_[2]);
+        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, NT, kModeGlobal, false>),
+                           dim3(nwg_glob_), dim3(NT), 0, s, blocks_.get() + nfast_ + nxg_, nglob_,
+                           href_, (const HeffRef*)nullptr, x, y, counter_.get() + 2 * kSegments,
+                           seg_.get() + 2 * (kSegments + 1), nseg_[2]);
     if (nsplits_)
         hipLaunchKernelGGL((split_reduce_kernel<T>), dim3(std::min(nsplits_, 4096)), dim3(256), 0,
                            s, splits_.get(), nsplits_, part, y);
 }
 
+template <typename T, typename OT>
+void HeffDevice::launch_any(const T* x, T* y, T* part, hipStream_t s) {
+    if (wpc_ == 4) launch<T, OT, 4>(x, y, part, s);
+    else if (wpc_ == 2) launch<T, OT, 2>(x, y, part, s);
+    else launch<T, OT, 1>(x, y, part, s);
+}
+
 // The object's device view for the kernels (input and output left to the
 // launch).  Rebuilt whenever lists or arenas are (re)allocated.
-void HeffDevice::build_ref(hipStream_t s) {
+void HeffDevice::build_ref() {
     HeffRef r{};
     r.pairs = pairs_.get();
     r.chunks = chunks_.get();
@@ -2144,8 +2228,6 @@ void HeffDevice::build_ref(hipStream_t s) {
     r.isdmrg = isdmrg_;
     for (int k = 0; k < 3; ++k) r.map[k] = map_[k];
     href_ = r;
-    ref_.upload(&href_, 1, s);
-    T3_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 void HeffDevice::matvec(const double* dvec, double* dres, hipStream_t s) {
@@ -2189,57 +2271,68 @@ void HeffDevice::drop_graphs() {
 
 void HeffDevice::enqueue(const double* dvec, double* dres, hipStream_t s) {
     if (prec_ == PREC_FP64) {
-        launch<double, double, kPool64>(dvec, dres, part64_.get(), s);
+        launch_any<double, double>(dvec, dres, part64_.get(), s);
     } else {
         hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(size_)), dim3(256), 0, s,
                            dvec, x32_.get(), size_);
         if (prec_ == PREC_BF16)
-            launch<float, uint16_t, kPool32>(x32_.get(), y32_.get(), part32_.get(), s);
+            launch_any<float, uint16_t>(x32_.get(), y32_.get(), part32_.get(), s);
         else
-            launch<float, float, kPool32>(x32_.get(), y32_.get(), part32_.get(), s);
+            launch_any<float, float>(x32_.get(), y32_.get(), part32_.get(), s);
         hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(size_)), dim3(256), 0, s,
                            y32_.get(), dres, size_);
     }
     T3_HIP_CHECK(hipGetLastError());
 }
 
-template <typename T, typename OT, int POOL>
+template <typename T, typename OT>
 void HeffBatch::launch(HeffDevice* const* hs, const T* const* x, T* const* y, int n,
                        hipStream_t s) {
     // Units of the kernels with no global scratch from every object, one
-    // longest-first list per kernel; each unit names its object.
-    std::vector<DevBlock> units[2];
-    std::vector<double> cost[2];
+    // longest-first list per kernel and grid shape; each unit names its
+    // object.
+    std::vector<DevBlock> units[kShapes][2];  // [grid shape][kernel]
+    std::vector<double> cost[kShapes][2];
     std::vector<HeffRef> refs(n);
     for (int i = 0; i < n; ++i) {
         HeffDevice& h = *hs[i];
         refs[i] = h.href_;
         refs[i].x = x[i];
         refs[i].y = y[i];
+        const int g = h.wpc_ == 4 ? 2 : h.wpc_ - 1;
         const int first[2] = {0, h.nfast_}, count[2] = {h.nfast_, h.nxg_};
         for (int m = 0; m < 2; ++m)
             for (int u = first[m]; u < first[m] + count[m]; ++u) {
                 DevBlock B = h.hunits_[u];
                 B.ref = i;
-                units[m].push_back(B);
-                cost[m].push_back(h.hucost_[u]);
+                units[g][m].push_back(B);
+                cost[g][m].push_back(h.hucost_[u]);
             }
     }
+    for (int g = 0; g < kShapes; ++g)
+        for (int m = 0; m < 2; ++m) {
+            std::vector<int> idx(units[g][m].size());
+            std::iota(idx.begin(), idx.end(), 0);
+            std::stable_sort(idx.begin(), idx.end(),
+                             [&](int a, int b) { return cost[g][m][a] > cost[g][m][b]; });
+            std::vector<DevBlock> v;
+            for (int k : idx) v.push_back(units[g][m][k]);
+            units[g][m].swap(v);
+        }
     std::vector<DevBlock> all;
-    int nmode[2];
-    for (int m = 0; m < 2; ++m) {
-        std::vector<int> idx(units[m].size());
-        std::iota(idx.begin(), idx.end(), 0);
-        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[m][a] > cost[m][b]; });
-        for (int k : idx) all.push_back(units[m][k]);
-        nmode[m] = (int)idx.size();
-    }
-    const std::vector<int> seg = {0, nmode[0], 0, nmode[1]};  // one segment per kernel
+    std::vector<int> seg;  // one segment per kernel and grid shape
+    for (int g = 0; g < kShapes; ++g)
+        for (int m = 0; m < 2; ++m) {
+            seg.push_back(0);
+            seg.push_back((int)units[g][m].size());
+            all.insert(all.end(), units[g][m].begin(), units[g][m].end());
+        }
     units_.upload(all, s);
     refs_.upload(refs, s);
     seg_.upload(seg, s);
-    counter_.reserve(2);
-    hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s, counter_.get(), (int64_t)2, 0);
+    counter_.reserve(2 * kShapes);
+    hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s, counter_.get(),
+                       (int64_t)(2 * kShapes), 0);
     for (int i = 0; i < n; ++i) {
         HeffDevice& h = *hs[i];
         hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(h.size_)), dim3(256), 0, s, y[i],
@@ -2248,17 +2341,13 @@ void HeffBatch::launch(HeffDevice* const* hs, const T* const* x, T* const* y, in
             hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(h.part_elems_)), dim3(256), 0, s,
                                static_cast<T*>(h.href_.partial), h.part_elems_, (T)0);
     }
-    const int ncu = hs[0]->ncu_;
-    if (nmode[0])
-        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeLds, true>),
-                           dim3(std::min(nmode[0], ncu)), dim3(kThreads), 0, s, units_.get(),
-                           nmode[0], HeffRef{}, refs_.get(), (const T*)nullptr, (T*)nullptr,
-                           counter_.get(), seg_.get(), 1);
-    if (nmode[1])
-        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeLdsW, true>),
-                           dim3(std::min(nmode[1], ncu)), dim3(kThreads), 0, s,
-                           units_.get() + nmode[0], nmode[1], HeffRef{}, refs_.get(),
-                           (const T*)nullptr, (T*)nullptr, counter_.get() + 1, seg_.get() + 2, 1);
+    ncu_ = hs[0]->ncu_;
+    int base = 0;
+    launch_grid<T, OT, 1>(units[0], base, 0, s);
+    base += (int)(units[0][0].size() + units[0][1].size());
+    launch_grid<T, OT, 2>(units[1], base, 1, s);
+    base += (int)(units[1][0].size() + units[1][1].size());
+    launch_grid<T, OT, 4>(units[2], base, 2, s);
     // units that need global scratch run in their objects' own launches
     // (the scratch is sized for the object's grid); then the split sums
     for (int i = 0; i < n; ++i) {
@@ -2266,12 +2355,12 @@ void HeffBatch::launch(HeffDevice* const* hs, const T* const* x, T* const* y, in
         if (h.nglob_) {
             hipLaunchKernelGGL((fill_kernel<int>), dim3(1), dim3(64), 0, s,
                                h.counter_.get() + 2 * HeffDevice::kSegments, (int64_t)1, 0);
-            hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, kThreads, kModeGlobal, false>),
-                               dim3(h.nwg_glob_), dim3(kThreads), 0, s,
-                               h.blocks_.get() + h.nfast_ + h.nxg_, h.nglob_, h.href_,
-                               (const HeffRef*)nullptr, x[i], y[i],
-                               h.counter_.get() + 2 * HeffDevice::kSegments,
-                               h.seg_.get() + 2 * (HeffDevice::kSegments + 1), h.nseg_[2]);
+            if (h.wpc_ == 4)
+                launch_glob<T, OT, 4>(h, x[i], y[i], s);
+            else if (h.wpc_ == 2)
+                launch_glob<T, OT, 2>(h, x[i], y[i], s);
+            else
+                launch_glob<T, OT, 1>(h, x[i], y[i], s);
         }
         if (h.nsplits_)
             hipLaunchKernelGGL((split_reduce_kernel<T>), dim3(std::min(h.nsplits_, 4096)), dim3(256),
@@ -2281,6 +2370,38 @@ void HeffBatch::launch(HeffDevice* const* hs, const T* const* x, T* const* y, in
     T3_HIP_CHECK(hipGetLastError());
 }
 
+// The batch's kernels of one grid shape: units[m] (kernel m) start at
+// `base` in the uploaded unit list, counters and segments at 2 shape.
+template <typename T, typename OT, int WPC>
+void HeffBatch::launch_grid(const std::vector<DevBlock> (&units)[2], int base, int shape,
+                            hipStream_t s) {
+    constexpr int NT = Grid<T, WPC>::NT, POOL = Grid<T, WPC>::POOL;
+    const int nwg = ncu_ * WPC;
+    const int n0 = (int)units[0].size(), n1 = (int)units[1].size();
+    const int c = 2 * shape;
+    if (n0)
+        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, NT, kModeLds, true>),
+                           dim3(std::min(n0, nwg)), dim3(NT), 0, s, units_.get() + base, n0,
+                           HeffRef{}, refs_.get(), (const T*)nullptr, (T*)nullptr,
+                           counter_.get() + c, seg_.get() + 2 * c, 1);
+    if (n1)
+        hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, NT, kModeLdsW, true>),
+                           dim3(std::min(n1, nwg)), dim3(NT), 0, s, units_.get() + base + n0, n1,
+                           HeffRef{}, refs_.get(), (const T*)nullptr, (T*)nullptr,
+                           counter_.get() + c + 1, seg_.get() + 2 * c + 2, 1);
+}
+
+// An object's global-scratch kernel inside a batch (its own grid).
+template <typename T, typename OT, int WPC>
+void HeffBatch::launch_glob(HeffDevice& h, const T* x, T* y, hipStream_t s) {
+    constexpr int NT = Grid<T, WPC>::NT, POOL = Grid<T, WPC>::POOL;
+    hipLaunchKernelGGL((heff_matvec_kernel<T, OT, POOL, NT, kModeGlobal, false>),
+                       dim3(h.nwg_glob_), dim3(NT), 0, s, h.blocks_.get() + h.nfast_ + h.nxg_,
+                       h.nglob_, h.href_, (const HeffRef*)nullptr, x, y,
+                       h.counter_.get() + 2 * HeffDevice::kSegments,
+                       h.seg_.get() + 2 * (HeffDevice::kSegments + 1), h.nseg_[2]);
+}
+
 void HeffBatch::matvec(HeffDevice* const* hs, const double* const* x, double* const* y, int n,
                        hipStream_t s) {
     if (n <= 0) return;
@@ -2288,7 +2409,7 @@ void HeffBatch::matvec(HeffDevice* const* hs, const double* const* x, double* co
     for (int i = 1; i < n; ++i)
         if (hs[i]->prec_ != prec) throw std::invalid_argument("batched matvec: mixed precisions");
     if (prec == PREC_FP64) {
-        launch<double, double, kPool64>(hs, x, y, n, s);
+        launch<double, double>(hs, x, y, n, s);
         return;
     }
     std::vector<const float*> x32(n);
@@ -2301,9 +2422,9 @@ void HeffBatch::matvec(HeffDevice* const* hs, const double* const* x, double* co
         y32[i] = h.y32_.get();
     }
     if (prec == PREC_BF16)
-        launch<float, uint16_t, kPool32>(hs, x32.data(), y32.data(), n, s);
+        launch<float, uint16_t>(hs, x32.data(), y32.data(), n, s);
     else
-        launch<float, float, kPool32>(hs, x32.data(), y32.data(), n, s);
+        launch<float, float>(hs, x32.data(), y32.data(), n, s);
     for (int i = 0; i < n; ++i)
         hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(hs[i]->size_)), dim3(256),
                            0, s, y32[i], y[i], hs[i]->size_);

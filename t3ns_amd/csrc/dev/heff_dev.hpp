@@ -114,8 +114,10 @@ struct HeffTuning {
     double fuse_over = 2.0;     // fused operand work per fiber tile and k-step, in steps
     int fuse = 1;               // fused middle stage allowed
     int sched = 0;              // unit schedule: 0 cost, 1 block, 2 xcd, 3 xcdblock
+    int wpc = 0;                // workgroups per CU (8 / wpc waves each, 1 / wpc of the LDS): 1, 2
+                                // or 4; 0 picks by the object's structure
     // The defaults with T3NS_HEFF_CHUNK_STEPS, T3NS_HEFF_FUSE,
-    // T3NS_HEFF_FUSE_OVER and T3NS_HEFF_SCHED applied.
+    // T3NS_HEFF_FUSE_OVER, T3NS_HEFF_SCHED and T3NS_HEFF_WPC applied.
     static HeffTuning from_env();
 };
 
@@ -133,8 +135,13 @@ public:
                 hipStream_t s);
 
 private:
-    template <typename T, typename OT, int POOL>
+    template <typename T, typename OT>
     void launch(HeffDevice* const* hs, const T* const* x, T* const* y, int n, hipStream_t s);
+    template <typename T, typename OT, int WPC>
+    void launch_grid(const std::vector<DevBlock> (&units)[2], int base, int shape, hipStream_t s);
+    template <typename T, typename OT, int WPC>
+    void launch_glob(HeffDevice& h, const T* x, T* y, hipStream_t s);
+    int ncu_ = 256;
     DevBuf<DevBlock> units_;
     DevBuf<HeffRef> refs_;
     DevBuf<int> counter_;
@@ -173,22 +180,25 @@ public:
     // only (contraction orders may differ).
     void retune(const HeffInput& in, const HeffTuning& t, hipStream_t s);
     const HeffTuning& tuning() const { return tune_; }
+    int wpc() const { return wpc_; }  // grid shape in use (tuning().wpc, or the one picked)
 
 private:
     void build_lists(const HeffInput& in, hipStream_t stream);
+    int build_lists_at(const HeffInput& in, hipStream_t stream, int wpc, bool pick);
     void enqueue(const double* dvec, double* dres, hipStream_t s);
     void drop_graphs();
-    template <typename T, typename OT, int POOL>
+    template <typename T, typename OT, int WPC>
     void launch(const T* x, T* y, T* part, hipStream_t s);
-    void build_ref(hipStream_t s);
+    template <typename T, typename OT>
+    void launch_any(const T* x, T* y, T* part, hipStream_t s);
+    void build_ref();
 
     HeffPlan plan_;
     HeffStats stats_;
     HeffTuning tune_;
-    HeffRef href_{};        // the kernels' view of this object (host copy)
+    HeffRef href_{};        // the kernels' view of this object
     std::vector<DevBlock> hunits_;  // the unit list (host copy) and unit costs
     std::vector<double> hucost_;
-    DevBuf<HeffRef> ref_;   // ... and on the device
     std::vector<std::vector<std::vector<float>>> bscale_;  // BF16 block scales [leg][op][block]
     int prec_;
     int isdmrg_;
@@ -196,6 +206,7 @@ private:
     int64_t size_;
     int nwg_glob_ = 1;
     int ncu_ = 256;
+    int wpc_ = 1;     // workgroups per CU of the lists built (HeffTuning::wpc)
     int nfast_ = 0;   // units whose pairs all fit in LDS (first in the unit list)
     int nxg_ = 0;     // units whose stage outputs fit in LDS but input blocks do not
     int nglob_ = 0;   // units that use global scratch

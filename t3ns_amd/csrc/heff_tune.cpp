@@ -27,7 +27,7 @@ std::map<std::string, HeffTuning>& tune_cache() {
 
 bool same(const HeffTuning& a, const HeffTuning& b) {
     return a.chunk_steps == b.chunk_steps && a.fuse_over == b.fuse_over && a.fuse == b.fuse &&
-           a.sched == b.sched;
+           a.sched == b.sched && a.wpc == b.wpc;
 }
 
 // Milliseconds of one matvec: one untimed, then the best of `reps`.
@@ -87,9 +87,10 @@ TuneReport autotune_heff(HeffDevice& dev, const HeffInput& in, hipStream_t s, co
         R.trials = 0;
         return R;
     }
-    // Coordinate search from the current tuning: the fused operand price,
-    // the chunk price, then the fused form off.
+    // Coordinate search from the current tuning: the grid shape, the fused
+    // operand price, the chunk price, then the fused form off.
     HeffTuning best = dev.tuning();
+    best.wpc = dev.wpc();
     const int reps = 2;
     double best_ms = time_matvec(dev, x, y, s, reps);
     R.baseline_ms = best_ms;
@@ -107,6 +108,11 @@ TuneReport autotune_heff(HeffDevice& dev, const HeffInput& in, hipStream_t s, co
         }
     };
     const HeffTuning base = best;
+    for (int w : {1, 2, 4}) {
+        HeffTuning t = best;
+        t.wpc = w;
+        trial(t);
+    }
     for (double fo : {base.fuse_over * 0.5, base.fuse_over * 2}) {
         HeffTuning t = best;
         t.fuse_over = fo;

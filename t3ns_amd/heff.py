@@ -438,33 +438,37 @@ class Heff:
             check(rc, "t3ns_davidson")
         return e.value, x, its.value, rc
 
-    TUNING = ("chunk_steps", "fuse_over", "fuse", "sched")
+    TUNING = ("chunk_steps", "fuse_over", "fuse", "sched", "wpc")
 
     def autotune(self, max_trials=6):
         """Times matvecs under up to max_trials work-list settings and keeps
         the fastest (t3ns_heff_autotune); a block-size histogram tuned
         before in this process reuses its setting untimed.  Returns the
         setting with the timings."""
-        buf = (C.c_double * 8)()
-        check(lib().t3ns_heff_autotune(self._h, int(max_trials), buf, 8), "t3ns_heff_autotune")
+        buf = (C.c_double * 9)()
+        check(lib().t3ns_heff_autotune(self._h, int(max_trials), buf, 9), "t3ns_heff_autotune")
         v = list(buf)
-        out = dict(zip(self.TUNING, v[:4]))
+        out = dict(zip(self.TUNING[:4], v[:4]))
+        out["wpc"] = v[8]
         out.update({"ms": v[4], "baseline_ms": v[5], "trials": int(v[6]), "cached": bool(v[7])})
         return out
 
     def tuning(self):
-        buf = (C.c_double * 4)()
+        buf = (C.c_double * 5)()
         check(lib().t3ns_heff_get_tuning(self._h, buf), "t3ns_heff_get_tuning")
         return dict(zip(self.TUNING, list(buf)))
 
-    def set_tuning(self, chunk_steps=1000.0, fuse_over=2.0, fuse=1, sched=0):
-        buf = (C.c_double * 4)(float(chunk_steps), float(fuse_over), float(fuse), float(sched))
+    def set_tuning(self, chunk_steps=1000.0, fuse_over=2.0, fuse=1, sched=0, wpc=0):
+        """wpc: workgroups per CU (1, 2 or 4; 0 picks by the object's
+        structure)."""
+        buf = (C.c_double * 5)(float(chunk_steps), float(fuse_over), float(fuse), float(sched),
+                               float(wpc))
         check(lib().t3ns_heff_set_tuning(self._h, buf), "t3ns_heff_set_tuning")
 
     def kernel_stats(self):
         names = ["chunks", "runs", "pairs_global_x", "pairs_global_w", "mfma_steps",
                  "steps_A", "steps_B", "steps_C", "scratch_per_wg", "units", "split_blocks",
-                 "pairs_fused"]
+                 "pairs_fused", "wpc"]
         buf = (C.c_double * len(names))()
         check(lib().t3ns_heff_kernel_stats(self._h, buf, len(names)) < 0 and -1 or 0)
         return dict(zip(names, list(buf)))
