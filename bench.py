@@ -79,7 +79,10 @@ def parse():
     ap.add_argument("--precisions", default="f64,f32,bf16")
     ap.add_argument("--autotune-trials", type=int, default=6,
                     help="work-list settings the autotuner times before the timed matvecs of "
-                         "the first precision (0: the defaults, untuned)")
+                         "each precision (0: the defaults, untuned)")
+    ap.add_argument("--wpc", type=int, default=0, choices=(0, 1, 2, 4),
+                    help="with --autotune-trials 0: workgroups per CU of the matvec kernels "
+                         "(0: picked from the object's structure)")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the renormalisation and site-tensor kernels")
     ap.add_argument("--no-solve", action="store_true",
@@ -627,11 +630,13 @@ def main():
         heffs[prec] = h
         print("# %s: Heff init %.1f s" % (prec, t_init), file=sys.stderr, flush=True)
         tune = None
-        if args.autotune_trials > 0 and prec == args.precisions.split(",")[0]:
+        if args.autotune_trials > 0:
             t0 = time.perf_counter()
             tune = h.autotune(max_trials=args.autotune_trials)
             tune["seconds"] = time.perf_counter() - t0
             print("# %s: autotune %s" % (prec, tune), file=sys.stderr, flush=True)
+        elif args.wpc:
+            h.set_tuning(wpc=args.wpc)
         dy = torch.empty_like(dx)
         torch.cuda.synchronize()
         for _ in range(args.warmup):
