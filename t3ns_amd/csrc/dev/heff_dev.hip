@@ -1611,16 +1611,18 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream,
 
 // Work lists of the current tuning (everything the matvec kernels read
 // besides the vectors and operator arenas).  Rebuilt by retune().  A
-// tuning that leaves the grid shape open (wpc 0) builds the lists for one
-// workgroup per CU and keeps them unless a smaller pool would cut its pairs
-// into hardly more chunks (the chunks' LDS demand, re-cut for the pools of
-// 2 and 4 workgroups per CU, grows by at most a tenth and no pair loses its
-// LDS placement): then the lists are built again for the most workgroups
-// that pass.  Smaller workgroups wait on memory and barriers independently,
-// which pays where chunks are short; where they are long the whole pool
-// keeps them fewer.
+// tuning that leaves the grid shape open (wpc 0) takes the most workgroups
+// per CU whose pool cuts the pairs into hardly more chunks than the whole
+// pool would (the chunks' LDS demand, re-cut for each pool, grows by at
+// most a tenth) with no pair losing its LDS placement.  The lists are
+// built first for 4 workgroups per CU on small objects (where that is the
+// usual answer) and for 1 on the others, and built again only when the
+// estimate picks another shape.  Smaller workgroups wait on memory and
+// barriers independently, which pays where chunks are short; where they
+// are long the whole pool keeps them fewer.
 void HeffDevice::build_lists(const HeffInput& in, hipStream_t stream) {
-    const int w0 = tune_.wpc ? tune_.wpc : 1;
+    constexpr int64_t kSmallObject = 20000;  // vector elements
+    const int w0 = tune_.wpc ? tune_.wpc : (size_ < kSmallObject ? 4 : 1);
     const int w = build_lists_at(in, stream, w0, tune_.wpc == 0);
     if (w != w0) build_lists_at(in, stream, w, false);
 }
@@ -1700,13 +1702,14 @@ int HeffDevice::build_lists_at(const HeffInput& in, hipStream_t stream, int wpc,
         double cost = 0;
         HeffStats st;
         DevBlock B{};
-        // chunks re-cut for the pools of 2 and 4 workgroups per CU (pick)
-        double est[2] = {0, 0};
-        int64_t lost[2] = {0, 0};  // pairs that would lose their LDS placement
+        // chunks re-cut for the pools of 1, 2 and 4 workgroups per CU (pick)
+        double est[3] = {0, 0, 0};
+        int64_t lost[3] = {0, 0, 0};  // pairs that lose the placement the whole pool gives
     };
-    const int qpool[2] = {f32 ? Grid<float, 2>::POOL : Grid<double, 2>::POOL,
+    const int qpool[3] = {f32 ? kPool32 : kPool64, f32 ? Grid<float, 2>::POOL : Grid<double, 2>::POOL,
                           f32 ? Grid<float, 4>::POOL : Grid<double, 4>::POOL};
-    const int qterms[2] = {max_chunk_terms(Grid<double, 2>::NT), max_chunk_terms(Grid<double, 4>::NT)};
+    const int qterms[3] = {max_chunk_terms(Grid<double, 1>::NT), max_chunk_terms(Grid<double, 2>::NT),
+                           max_chunk_terms(Grid<double, 4>::NT)};
     std::vector<Lists> lists(plan_.nrblocks);
     auto build_block = [&](int sb) {
         Lists& L = lists[sb];
@@ -1905,9 +1908,10 @@ int HeffDevice::build_lists_at(const HeffInput& in, hipStream_t stream, int wpc,
                            (isdmrg_ ? 0.0 : P.fuse ? nt * kFuseStage : (double)Q.ncs * P.w2);
                     ntt += nt;
                 }
-                for (int q = 0; q < 2; ++q) {
+                const bool w1fit = wmin <= qpool[0], x1fit = x_in_lds && xsz + wmin <= qpool[0];
+                for (int q = 0; q < 3; ++q) {
                     const bool xq = x_in_lds && xsz + wmin <= qpool[q];
-                    if (wmin > qpool[q] || (P.xlds && !xq)) ++L.lost[q];
+                    if ((w1fit && wmin > qpool[q]) || (x1fit && !xq)) ++L.lost[q];
                     const double avq = std::max(1.0, (double)qpool[q] - (xq ? (double)xsz : 0.0));
                     L.est[q] += std::max(std::ceil(dem / avq), std::ceil(ntt / qterms[q]));
                 }
@@ -1918,18 +1922,24 @@ int HeffDevice::build_lists_at(const HeffInput& in, hipStream_t stream, int wpc,
         L.B = B;
     };
     run_blocks_threaded(plan_.nrblocks, build_block, (double)plan_.pairs.size(), 2e4);
-    if (pick && wpc == 1 && pool > 0) {
-        double nch = 0, est[2] = {0, 0};
-        int64_t lost[2] = {0, 0};
+    if (pick && pool > 0) {
+        // the whole pool's chunk count: counted when these lists are its,
+        // else estimated like the others
+        double est[3] = {0, 0, 0};
+        int64_t lost[3] = {0, 0, 0};
+        double nch = 0;
         for (const Lists& L : lists) {
             nch += (double)L.hc.size();
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < 3; ++q) {
                 est[q] += L.est[q];
                 lost[q] += L.lost[q];
             }
         }
-        for (int q = 1; q >= 0; --q)
-            if (nch > 0 && !lost[q] && est[q] <= 1.1 * nch) return q == 1 ? 4 : 2;
+        const double base = wpc == 1 ? nch : est[0];
+        int w = 1;
+        for (int q = 2; q >= 1 && w == 1; --q)
+            if (base > 0 && !lost[q] && est[q] <= 1.1 * base) w = q == 2 ? 4 : 2;
+        if (w != wpc) return w;
     }
     std::vector<DevPair> hp;
     std::vector<DevChunk> hc;
