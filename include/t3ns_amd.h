@@ -108,6 +108,15 @@ int t3ns_device_count(void);
 int t3ns_heff_init(t3ns_heff** h, const t3ns_heff_desc* desc, const t3ns_rops_desc* ops,
                    const t3ns_site_desc* site, int precision);
 
+/* An FP32 object on an FP64 one's plan and operator arenas (the arenas
+ * copied and converted on the device, the plan copied; only its FP32 work
+ * lists are built on the host), for the low-precision operator of
+ * t3ns_davidson_refined: no operator data crosses the host link.  No
+ * counterpart in the reference (its Heff is FP64 only).  precision must be
+ * T3NS_FP32; the result is independent of `base` afterwards (either may be
+ * destroyed first); autotune and set_tuning are refused on it. */
+int t3ns_heff_derive(t3ns_heff** h, const t3ns_heff* base, int precision);
+
 /* matvecT3NS (include/Heff.h:177): result = Heff vec, host FP64 arrays of
  * t3ns_heff_size() elements; result is overwritten. */
 int t3ns_heff_matvec(t3ns_heff* h, const double* vec, double* result);

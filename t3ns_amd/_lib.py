@@ -155,6 +155,7 @@ def lib():
         "t3ns_device_count": (C.c_int, []),
         "t3ns_heff_init": (C.c_int, [C.POINTER(P), C.POINTER(HeffDesc), C.POINTER(RopsDesc),
                                      C.POINTER(SiteDesc), C.c_int]),
+        "t3ns_heff_derive": (C.c_int, [C.POINTER(P), P, C.c_int]),
         "t3ns_heff_matvec": (C.c_int, [P, c_dbl_p, c_dbl_p]),
         "t3ns_heff_matvec_device": (C.c_int, [P, P, P, P]),
         "t3ns_heff_diagonal": (C.c_int, [P, c_dbl_p]),

@@ -195,10 +195,11 @@ int64_t write_plan_table(const HeffPlan& P, int* ints, double* prefs) {
 }  // namespace
 
 struct t3ns_heff {
-    HeffHost host;
+    HeffHost host;  // empty for a derived object (t3ns_heff_derive)
     std::unique_ptr<HeffDevice> dev;
     DevBuf<double> dvec, dres;
     hipStream_t stream = nullptr;
+    bool derived = false;
 };
 
 extern "C" {
@@ -223,6 +224,22 @@ int t3ns_heff_init(t3ns_heff** out, const t3ns_heff_desc* d, const t3ns_rops_des
         HeffInput& in = h->host.in;
         T3_HIP_CHECK(hipStreamCreate(&h->stream));
         h->dev = std::make_unique<HeffDevice>(in, precision, h->stream);
+        h->dvec.alloc(h->dev->size());
+        h->dres.alloc(h->dev->size());
+        *out = h.release();
+    });
+}
+
+int t3ns_heff_derive(t3ns_heff** out, const t3ns_heff* base, int precision) {
+    if (!out || !base) return fail(T3NS_ERR_INVALID, "null argument");
+    if (precision != T3NS_FP32 || base->dev->precision() != T3NS_FP64)
+        return fail(T3NS_ERR_INVALID, "a derived Heff is FP32 on an FP64 one");
+    return guarded([&] {
+        auto h = std::make_unique<t3ns_heff>();
+        h->derived = true;
+        T3_HIP_CHECK(hipStreamCreate(&h->stream));
+        T3_HIP_CHECK(hipStreamSynchronize(base->stream));
+        h->dev = std::make_unique<HeffDevice>(*base->dev, base->host.in, precision, h->stream);
         h->dvec.alloc(h->dev->size());
         h->dres.alloc(h->dev->size());
         *out = h.release();
@@ -318,6 +335,7 @@ int t3ns_heff_batch_matvec_host(t3ns_heff_batch* b, t3ns_heff* const* hs, const 
 
 int t3ns_heff_autotune(t3ns_heff* h, int max_trials, double* out, int n) {
     if (!h || (n > 0 && !out)) return fail(T3NS_ERR_INVALID, "null argument");
+    if (h->derived) return fail(T3NS_ERR_INVALID, "a derived Heff keeps its base's work lists");
     if (max_trials < 1) return fail(T3NS_ERR_INVALID, "max_trials must be at least 1");
     return guarded([&] {
         const size_t sz = h->dev->size();
@@ -336,6 +354,7 @@ int t3ns_heff_autotune(t3ns_heff* h, int max_trials, double* out, int n) {
 
 int t3ns_heff_set_tuning(t3ns_heff* h, const double* t) {
     if (!h || !t) return fail(T3NS_ERR_INVALID, "null argument");
+    if (h->derived) return fail(T3NS_ERR_INVALID, "a derived Heff keeps its base's work lists");
     if (!(t[0] >= 0) || !(t[1] >= 0) || t[3] < 0 || t[3] > 3 ||
         (t[4] != 0 && t[4] != 1 && t[4] != 2 && t[4] != 4))
         return fail(T3NS_ERR_INVALID, "tuning out of range");

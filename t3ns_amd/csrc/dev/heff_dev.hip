@@ -1609,6 +1609,37 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream,
     build_ref();
 }
 
+HeffDevice::HeffDevice(const HeffDevice& base, const HeffInput& in, int precision,
+                       hipStream_t stream)
+    : prec_(precision) {
+    if (base.prec_ != PREC_FP64 || precision != PREC_FP32)
+        throw std::invalid_argument("a derived Heff is FP32 on an FP64 one");
+    plan_ = base.plan_;
+    tune_ = base.tune_;
+    bscale_.assign(3, {});
+    isdmrg_ = base.isdmrg_;
+    for (int k = 0; k < 3; ++k) map_[k] = base.map_[k];
+    size_ = base.size_;
+    ndblocks_ = base.ndblocks_;
+    dblocks_.copy_from(base.dblocks_, stream);
+    dterms_.copy_from(base.dterms_, stream);
+    build_lists(in, stream);
+    for (int k = 0; k < 3; ++k) {
+        stats_.op_elems[k] = base.stats_.op_elems[k];
+        ops64_[k].copy_from(base.ops64_[k], stream);
+        const int64_t n = stats_.op_elems[k];
+        ops32_[k].alloc(n);
+        if (n)
+            hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0,
+                               stream, ops64_[k].get(), ops32_[k].get(), n);
+    }
+    x32_.alloc(size_);
+    y32_.alloc(size_);
+    T3_HIP_CHECK(hipGetLastError());
+    T3_HIP_CHECK(hipStreamSynchronize(stream));
+    build_ref();
+}
+
 // Work lists of the current tuning (everything the matvec kernels read
 // besides the vectors and operator arenas).  Rebuilt by retune().  A
 // tuning that leaves the grid shape open (wpc 0) takes the most workgroups

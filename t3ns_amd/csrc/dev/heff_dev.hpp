@@ -157,6 +157,11 @@ public:
     // null: HeffTuning::from_env().
     HeffDevice(const HeffInput& in, int precision, hipStream_t stream,
                const HeffTuning* tuning = nullptr);
+    // An FP32 object on the plan and FP64 operator arenas of an FP64 one
+    // built from the same `in` (the arenas copied and converted on the
+    // device, the plan copied): only its own work lists are built on the
+    // host, and no operator data crosses the host link.
+    HeffDevice(const HeffDevice& base, const HeffInput& in, int precision, hipStream_t stream);
     ~HeffDevice();
     HeffDevice(const HeffDevice&) = delete;
     HeffDevice& operator=(const HeffDevice&) = delete;

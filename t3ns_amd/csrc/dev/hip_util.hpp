@@ -58,6 +58,11 @@ public:
         if (n) T3_HIP_CHECK(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, s));
     }
     void upload(const std::vector<T>& v, hipStream_t s = nullptr) { upload(v.data(), v.size(), s); }
+    // A device copy of `o` (same size), enqueued on s.
+    void copy_from(const DevBuf& o, hipStream_t s = nullptr) {
+        alloc(o.n_);
+        if (o.n_) T3_HIP_CHECK(hipMemcpyAsync(p_, o.p_, o.n_ * sizeof(T), hipMemcpyDeviceToDevice, s));
+    }
     T* get() const { return p_; }
     size_t size() const { return n_; }
 

@@ -388,6 +388,19 @@ class Heff:
         d, o, s = step.descriptors()
         return cls(d, o, s, precision, keepalive=step)
 
+    def derive_fp32(self):
+        """An FP32 Heff on this FP64 object's plan and operator arenas,
+        copied and converted on the device (t3ns_heff_derive): only its own
+        work lists are built on the host.  Independent of this object
+        afterwards."""
+        h = C.c_void_p()
+        check(lib().t3ns_heff_derive(C.byref(h), self._h, int(FP32)), "t3ns_heff_derive")
+        out = Heff.__new__(Heff)
+        out._h = h
+        out._keep = self._keep
+        out.size = int(lib().t3ns_heff_size(h))
+        return out
+
     @property
     def handle(self):
         return self._h

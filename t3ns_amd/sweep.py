@@ -28,7 +28,7 @@ import numpy as np
 
 from . import _lib
 from .environment import Environment
-from .heff import FP32, FP64, Heff, Step, davidson_refined, make_site_tensor, split_site
+from .heff import FP64, Heff, Step, davidson_refined, make_site_tensor, split_site
 
 
 # One regime of an optimisation scheme, fields in the order of struct regime
@@ -145,13 +145,11 @@ class DeviceSolver:
                 self.counts["its_f64"] += its
                 self.timers["Heff: davidson"] += time.perf_counter() - t0
                 return e, x, (0, its)
+            # the FP32 operator on the FP64 object's plan and lists, its
+            # arenas converted on the device (no second host set-up)
             t1 = time.perf_counter()
-            h32 = Heff(d, o, s, FP32, keepalive=keep)
+            h32 = h64.derive_fp32()
             self.timers["Heff: prepare data"] += time.perf_counter() - t1
-            if self.autotune_min_size is not None and h32.size >= self.autotune_min_size:
-                t1 = time.perf_counter()
-                h32.autotune()
-                self.timers["Heff: autotune"] += time.perf_counter() - t1
             try:
                 t2 = time.perf_counter()
                 e, x, i1, i2, rc = davidson_refined(h32, h64, guess, self.max_vecs, self.keep, tol,
