@@ -57,19 +57,23 @@ __global__ void __launch_bounds__(kRedThreads)
     }
 }
 
-// out[i] = beta * out[i] + sum_j coef[j] A[j*size + i]
+// out[i] = beta * out[i] + sum_j (cscale coef[j]) A[j*size + i]
 __global__ void combine_kernel(const double* __restrict__ A, int64_t size, int n,
                                const double* __restrict__ coef, double* __restrict__ out,
-                               double beta) {
+                               double beta, double cscale) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < size;
          i += (int64_t)gridDim.x * blockDim.x) {
         double s = beta == 0.0 ? 0.0 : beta * out[i];
-        for (int j = 0; j < n; ++j) s += coef[j] * A[(int64_t)j * size + i];
+        for (int j = 0; j < n; ++j) s += (cscale * coef[j]) * A[(int64_t)j * size + i];
         out[i] = s;
     }
 }
 
-__global__ void scale_kernel(double* __restrict__ v, int64_t n, double a) {
+// v *= 1 / sqrt(*sq) (v /= its norm, the squared norm on the device); a
+// vanished vector stays zero (the caller checks *sq)
+__global__ void unit_kernel(double* __restrict__ v, int64_t n, const double* __restrict__ sq) {
+    const double q = *sq;
+    const double a = q > 0 ? 1.0 / sqrt(q) : 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         v[i] *= a;
@@ -123,10 +127,12 @@ __global__ void __launch_bounds__(kRedThreads)
     }
 }
 
-// t = -(t + alpha r) / (D - theta)  (src/davidson.c:243-256)
+// t = -(t + alpha r) / (D - theta), alpha = -uk[0] / uk[1]
+// (src/davidson.c:243-256)
 __global__ void precond_apply_kernel(double* __restrict__ t, const double* __restrict__ r,
-                                     const double* __restrict__ diag, double theta, double alpha,
-                                     int64_t size) {
+                                     const double* __restrict__ diag, double theta,
+                                     const double* __restrict__ uk, int64_t size) {
+    const double alpha = -uk[0] / uk[1];
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < size;
          i += (int64_t)gridDim.x * blockDim.x)
         t[i] = -(t[i] + alpha * r[i]) * precond_inv(diag[i] - theta);
@@ -224,7 +230,7 @@ size_t DeviceDavidson::workspace_elems(int64_t size, int max_vecs, int keep_defl
     if (keep >= mv) keep = mv - 1;
     const size_t nb = (size_t)std::max<int64_t>(1, std::min<int64_t>(kRedBlocks, (size + kRedThreads - 1) / kRedThreads));
     return 2 * rounded((size_t)size * mv) + rounded(size) + rounded((size_t)size * keep) +
-           rounded(nb * (mv + 2)) + rounded(mv) + rounded(mv + 2);
+           rounded(nb * (mv + 2)) + rounded(mv) + rounded(mv + 2) + rounded(kScalars);
 }
 
 void DeviceDavidson::carve(double* ws) {
@@ -241,6 +247,7 @@ void DeviceDavidson::carve(double* ws) {
     partial_ = take((size_t)nblk_ * (mv_ + 2));
     coef_ = take(mv_);
     out_ = take(mv_ + 2);
+    scal_ = take(kScalars);
 }
 
 DeviceDavidson::DeviceDavidson(int64_t size, int max_vecs, int keep_deflate, hipStream_t s)
@@ -273,7 +280,15 @@ void DeviceDavidson::combine(const double* A, int n, const double* coef_host, do
                              double beta) {
     T3_HIP_CHECK(hipMemcpyAsync(coef_, coef_host, n * sizeof(double), hipMemcpyHostToDevice, s_));
     hipLaunchKernelGGL(combine_kernel, dim3(grid(size_)), dim3(256), 0, s_, A, size_, n,
-                       coef_, out, beta);
+                       coef_, out, beta, 1.0);
+}
+
+// dst[j] = A[:, j] . b (j < n) into device memory: no host round trip.
+void DeviceDavidson::dots_dev(const double* A, int64_t lda, int n, const double* b, double* dst) {
+    hipLaunchKernelGGL(dots_kernel, dim3(nblk_), dim3(kRedThreads), 0, s_, A, lda, n, b, size_,
+                       partial_);
+    hipLaunchKernelGGL(finish_kernel, dim3(std::min(n, 64)), dim3(kRedThreads), 0, s_, partial_,
+                       nblk_, n, dst);
 }
 
 double DeviceDavidson::norm(const double* v) {
@@ -282,24 +297,29 @@ double DeviceDavidson::norm(const double* v) {
     return std::sqrt(h);
 }
 
-// t <- orthonormal complement of t against V[:, :m] (two CGS passes).
+// t <- orthonormal complement of t against V[:, :m] (two CGS passes), on
+// the device without a host round trip: the coefficients stay in device
+// memory, and t's squared norm before scaling is left in scal_[0] for the
+// caller to check (a vanished vector stays zero).
 void DeviceDavidson::orthonormalise(double* t) {
-    double* c = hc_;
     for (int pass = 0; pass < 2 && m_ > 0; ++pass) {
-        dots(V_, size_, m_, t, c);
-        for (int j = 0; j < m_; ++j) c[j] = -c[j];
-        combine(V_, m_, c, t, 1.0);
+        dots_dev(V_, size_, m_, t, coef_);
+        hipLaunchKernelGGL(combine_kernel, dim3(grid(size_)), dim3(256), 0, s_, V_, size_, m_,
+                           coef_, t, 1.0, -1.0);
     }
-    const double nrm = norm(t);
-    if (!(nrm > 0)) throw std::runtime_error("davidson: search vector vanished");
-    hipLaunchKernelGGL(scale_kernel, dim3(grid(size_)), dim3(256), 0, s_, t, size_, 1.0 / nrm);
+    dots_dev(t, size_, 1, t, scal_);
+    hipLaunchKernelGGL(unit_kernel, dim3(grid(size_)), dim3(256), 0, s_, t, size_, scal_);
 }
 
 // Column m of the projected matrix: V[:, i] . VA[:, m], i <= m
-// (src/davidson.c:132-145).
-void DeviceDavidson::expand(int m) {
+// (src/davidson.c:132-145).  With `sq`, the squared norm orthonormalise()
+// left on the device comes back in the same host round trip.
+void DeviceDavidson::expand(int m, double* sq) {
     double* c = hc_;
-    dots(V_, size_, m + 1, VA_ + (int64_t)m * size_, c);
+    dots_dev(V_, size_, m + 1, VA_ + (int64_t)m * size_, out_);
+    T3_HIP_CHECK(hipMemcpyAsync(c, out_, (m + 1) * sizeof(double), hipMemcpyDeviceToHost, s_));
+    if (sq) T3_HIP_CHECK(hipMemcpyAsync(sq, scal_, sizeof(double), hipMemcpyDeviceToHost, s_));
+    T3_HIP_CHECK(hipStreamSynchronize(s_));
     for (int i = 0; i <= m; ++i) sub_[i + (size_t)m * mv_] = c[i];
 }
 
@@ -351,25 +371,26 @@ double DeviceDavidson::relative_difference(const double* x, const DevMatvec& a, 
 // Ritz vector x = V y, residual t = VA y - theta x (src/davidson.c:187-202),
 // then the preconditioned correction of t against x (src/davidson.c:
 // 204-257).  Returns the residual norm; theta is eigval_[0].
+//  One host round trip: the Ritz coefficients go down once, the residual's
+// squared norm comes back; the preconditioner's two dot products stay on
+// the device.
 double DeviceDavidson::ritz_and_correct(double* x, const double* diag) {
-    double* c = hc_;
     const double theta = eigval_[0];
-    combine(V_, m_, eigv_, x, 0.0);
-    combine(VA_, m_, eigv_, t_, 0.0);
-    for (int j = 0; j < m_; ++j) c[j] = -theta * eigv_[j];
-    combine(V_, m_, c, t_, 1.0);
-    const double residual = norm(t_);
+    const dim3 g(grid(size_)), b(256);
+    T3_HIP_CHECK(hipMemcpyAsync(coef_, eigv_, m_ * sizeof(double), hipMemcpyHostToDevice, s_));
+    hipLaunchKernelGGL(combine_kernel, g, b, 0, s_, V_, size_, m_, coef_, x, 0.0, 1.0);
+    hipLaunchKernelGGL(combine_kernel, g, b, 0, s_, VA_, size_, m_, coef_, t_, 0.0, 1.0);
+    hipLaunchKernelGGL(combine_kernel, g, b, 0, s_, V_, size_, m_, coef_, t_, 1.0, -theta);
+    dots_dev(t_, size_, 1, t_, scal_ + 1);
     hipLaunchKernelGGL(precond_dots_kernel, dim3(nblk_), dim3(kRedThreads), 0, s_, x, t_,
                        diag, theta, size_, partial_);
     hipLaunchKernelGGL(finish_kernel, dim3(2), dim3(kRedThreads), 0, s_, partial_, nblk_,
-                       2, out_);
-    double uk[2];
-    T3_HIP_CHECK(hipMemcpyAsync(uk, out_, 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
+                       2, scal_ + 2);
+    double r2;
+    T3_HIP_CHECK(hipMemcpyAsync(&r2, scal_ + 1, sizeof(double), hipMemcpyDeviceToHost, s_));
     T3_HIP_CHECK(hipStreamSynchronize(s_));
-    const double alpha = -uk[0] / uk[1];
-    hipLaunchKernelGGL(precond_apply_kernel, dim3(grid(size_)), dim3(256), 0, s_, t_, x,
-                       diag, theta, alpha, size_);
-    return residual;
+    hipLaunchKernelGGL(precond_apply_kernel, g, b, 0, s_, t_, x, diag, theta, scal_ + 2, size_);
+    return std::sqrt(r2);
 }
 
 DavidsonResult DeviceDavidson::iterate(double* x, const double* diag, const DevMatvec& mv,
@@ -387,7 +408,9 @@ DavidsonResult DeviceDavidson::iterate(double* x, const double* diag, const DevM
         double* VAm = VA_ + (int64_t)m_ * size_;
         hipLaunchKernelGGL(copy_kernel, dim3(grid(size_)), dim3(256), 0, s_, t_, Vm, size_);
         mv(Vm, VAm, s_);
-        expand(m_);
+        double sq = 0;
+        expand(m_, &sq);
+        if (!(sq > 0)) throw std::runtime_error("davidson: search vector vanished");
         ++m_;
         if (eigsolve() != 0) {
             R.status = -1;

@@ -66,10 +66,11 @@ public:
 
 private:
     void dots(const double* A, int64_t lda, int n, const double* b, double* host_out);
+    void dots_dev(const double* A, int64_t lda, int n, const double* b, double* dst);
     void combine(const double* A, int n, const double* coef_host, double* out, double beta);
     double norm(const double* v);
     void orthonormalise(double* t);
-    void expand(int m);
+    void expand(int m, double* sq = nullptr);
     int eigsolve();
     void deflate();
     double ritz_and_correct(double* x, const double* diag);
@@ -85,6 +86,10 @@ private:
     DevBuf<double> own_;  // the device buffers when the solver allocates them
     double *V_ = nullptr, *VA_ = nullptr, *t_ = nullptr, *tmp_ = nullptr;
     double *coef_ = nullptr, *partial_ = nullptr, *out_ = nullptr;
+    // device scalars: |t|^2 before normalising, |r|^2, the preconditioner's
+    // two dot products
+    static constexpr int kScalars = 8;
+    double* scal_ = nullptr;
     // projected matrix, its eigenvectors (both mv_ x mv_, leading dimension
     // mv_), eigenvalues, host coefficients and the eigensolver's work
     double sub_[kMaxDavidsonVecs * kMaxDavidsonVecs];
