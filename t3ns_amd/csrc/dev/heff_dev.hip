@@ -1681,17 +1681,22 @@ int HeffDevice::build_lists_at(const HeffInput& in, hipStream_t stream, int wpc,
                    : (wpc_ == 4 ? Grid<double, 4>::POOL
                                 : wpc_ == 2 ? Grid<double, 2>::POOL : kPool64);
     const int chunk_terms = max_chunk_terms(kThreads / wpc_);
-    bool x_in_lds = true;
-    // T3NS_HEFF_PLACEMENT=ldsw reads every input block from global memory,
-    // =global keeps every stage output in global scratch as well: the paths
-    // that large blocks take, forced on any problem (for tests).
+    // Input blocks are read from global memory (through L2) by default,
+    // which leaves the whole pool to the stage outputs: fewer, larger chunks
+    // and no staging barrier, faster on every workload measured (QC-24
+    // D = 512 step 370 -> 356 ms at two workgroups per CU; the QC-24
+    // D = 128 sweep's matvecs 285 -> 263 ms).  T3NS_HEFF_PLACEMENT=lds
+    // copies each input block into the pool when it fits beside one run
+    // and one term; =global keeps every stage output in global scratch as
+    // well (the path of pairs too large for the pool, forced on any
+    // problem for tests); =ldsw is the default.
+    bool x_in_lds = false;
     if (const char* pl = std::getenv("T3NS_HEFF_PLACEMENT")) {
-        if (!std::strcmp(pl, "ldsw")) {
-            x_in_lds = false;
+        if (!std::strcmp(pl, "lds")) {
+            x_in_lds = true;
         } else if (!std::strcmp(pl, "global")) {
-            x_in_lds = false;
             pool = 0;
-        } else if (std::strcmp(pl, "lds") != 0) {
+        } else if (std::strcmp(pl, "ldsw") != 0) {
             throw std::runtime_error("T3NS_HEFF_PLACEMENT must be lds, ldsw or global");
         }
     }

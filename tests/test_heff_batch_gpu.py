@@ -56,10 +56,11 @@ def test_batch_equals_own_matvecs(gpu, prec):
 
 @pytest.mark.gpu
 def test_batch_mixes_kernels(gpu, monkeypatch):
-    """Objects whose units use the global-memory paths (forced), batched
-    with ordinary ones."""
+    """Objects whose units take other kernel paths (input blocks staged in
+    LDS, stage outputs in global scratch; forced), batched with ordinary
+    ones."""
     print("batch placements", flush=True)
-    hs, keep = _objects(T.FP64, monkeypatch, placements=(None, "ldsw", "global"))
+    hs, keep = _objects(T.FP64, monkeypatch, placements=(None, "lds", "global"))
     rng = np.random.default_rng(9)
     xs = [rng.standard_normal(h.size) for h in hs]
     own = [h.matvec(x) for h, x in zip(hs, xs)]
