@@ -1561,12 +1561,36 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream,
             ops64_[k].upload(op.tel, stream);
         } else if ((int)op.tel_src.size() == op.nrops) {  // straight from the caller's arrays
             ops64_[k].alloc(op.total_size());
+            // Small operators are gathered into host chunks (one copy per
+            // chunk: a copy call per operator cost more than the data on
+            // objects with hundreds of small operators); large ones are
+            // copied as they are.  The arena holds the operators in order.
+            constexpr int64_t kDirect = 1 << 15, kChunk = 1 << 22;  // elements
+            std::vector<double> stage;
+            int64_t base = 0;  // arena offset of stage[0]
+            auto flush = [&]() {
+                if (stage.empty()) return;
+                T3_HIP_CHECK(hipMemcpyAsync(ops64_[k].get() + base, stage.data(),
+                                            stage.size() * sizeof(double), hipMemcpyHostToDevice,
+                                            stream));
+                T3_HIP_CHECK(hipStreamSynchronize(stream));  // stage is reused
+                stage.clear();
+            };
             for (int o = 0; o < op.nrops; ++o) {
-                const int64_t n = op.op_offset[o + 1] - op.op_offset[o];
-                if (n)
-                    T3_HIP_CHECK(hipMemcpyAsync(ops64_[k].get() + op.op_offset[o], op.tel_src[o],
+                const int64_t off = op.op_offset[o], n = op.op_offset[o + 1] - off;
+                if (!n) continue;
+                if (n >= kDirect) {
+                    T3_HIP_CHECK(hipMemcpyAsync(ops64_[k].get() + off, op.tel_src[o],
                                                 n * sizeof(double), hipMemcpyHostToDevice, stream));
+                    continue;
+                }
+                if (!stage.empty() &&
+                    (off != base + (int64_t)stage.size() || (int64_t)stage.size() + n > kChunk))
+                    flush();
+                if (stage.empty()) base = off;
+                stage.insert(stage.end(), op.tel_src[o], op.tel_src[o] + n);
             }
+            flush();
         } else {
             throw std::runtime_error("HeffDevice: operator host data missing");
         }
