@@ -18,6 +18,7 @@
 #include "dev/heff_dev.hpp"
 #include "dev/sitetensor_dev.hpp"
 #include "dev/renorm_dev.hpp"
+#include "dev/resident_ops.hpp"
 #include "heff_tune.hpp"
 #include "host/context.hpp"
 #include "host/heff_plan.hpp"
@@ -626,6 +627,7 @@ int t3ns_sparse_eigensolve(double* result, double* energy, int size, int max_vec
 struct t3ns_rops {
     ROperators r;
     RenormWork work;
+    DevArena dev;  // the device copy of r.tel, registered as resident
     std::vector<int64_t> bb_flat;
     std::vector<const int64_t*> bb_ptr;
     std::vector<const double*> tel_ptr;
@@ -636,8 +638,23 @@ struct t3ns_rops {
             bb_ptr[k] = r.beginblock[k].data();
             tel_ptr[k] = r.tel.data() + r.op_offset[k];
         }
+        // later updates and Heff set-ups on these operators copy them on
+        // the device
+        if (dev) resident_put(r.tel.data(), (int64_t)r.tel.size(), dev);
+        dev.reset();
     }
+    ~t3ns_rops() { resident_drop(r.tel.data()); }
 };
+
+// The operators of a descriptor, with the device copy they were made in
+// when it is still resident (then without a host copy), else with a host
+// copy for upload.
+ROperators rops_resident(const t3ns_rops_desc& d, ResidentRef& ref) {
+    ROperators r = rops_from(d, /*copy=*/false);
+    ref = resident_find(r);
+    if (ref) return r;
+    return rops_from(d, /*copy=*/true);
+}
 
 namespace {
 AppendInput append_input_from(const t3ns_append_desc* d, const ROperators* old) {
@@ -691,12 +708,13 @@ extern "C" {
 int t3ns_append_phys(const t3ns_append_desc* d, const t3ns_rops_desc* old, t3ns_rops** out) {
     if (!d || !old || !out) return fail(T3NS_ERR_INVALID, "null argument");
     return guarded([&] {
-        const ROperators o = rops_from(*old);
+        ResidentRef ref;
+        const ROperators o = rops_resident(*old, ref);
         const AppendInput in = append_input_from(d, &o);
         const AppendPlan plan = plan_append(in);
         auto res = std::make_unique<t3ns_rops>();
         hipStream_t s = hipStreamPerThread;
-        res->r = run_append(plan, o, s, &res->work);
+        res->r = run_append(plan, o, s, &res->work, ref.ptr(), &res->dev);
         res->views();
         *out = res.release();
     });
@@ -705,7 +723,8 @@ int t3ns_append_phys(const t3ns_append_desc* d, const t3ns_rops_desc* old, t3ns_
 int t3ns_update_physical(const t3ns_pupdate_desc* d, const t3ns_rops_desc* pops, t3ns_rops** out) {
     if (!d || !pops || !out) return fail(T3NS_ERR_INVALID, "null argument");
     return guarded([&] {
-        const ROperators po = rops_from(*pops);
+        ResidentRef ref;
+        const ROperators po = rops_resident(*pops, ref);
         PUpdateInput in;
         in.sym.nrSyms = d->nrSyms;
         for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) in.sym.sgs[i] = d->sgs[i];
@@ -739,7 +758,7 @@ int t3ns_update_physical(const t3ns_pupdate_desc* d, const t3ns_rops_desc* pops,
         const PUpdatePlan plan = plan_pupdate(in);
         auto res = std::make_unique<t3ns_rops>();
         hipStream_t s = hipStreamPerThread;
-        res->r = run_pupdate(plan, po, T, s, &res->work);
+        res->r = run_pupdate(plan, po, T, s, &res->work, ref.ptr(), &res->dev);
         res->views();
         *out = res.release();
     });
@@ -749,9 +768,11 @@ namespace {
 // The branching-update input of a descriptor (the operators and tensor
 // copied into o0, o1, T, which `in` points at).
 void bupdate_input(const t3ns_bupdate_desc* d, const t3ns_rops_desc* ops, ROperators& o0,
-                   ROperators& o1, SiteTensor& T, BUpdateInput& in) {
-        o0 = rops_from(ops[0]);
-        o1 = rops_from(ops[1]);
+                   ROperators& o1, SiteTensor& T, BUpdateInput& in, ResidentRef* ref0 = nullptr,
+                   ResidentRef* ref1 = nullptr) {
+        ResidentRef r0, r1;
+        o0 = rops_resident(ops[0], ref0 ? *ref0 : r0);
+        o1 = rops_resident(ops[1], ref1 ? *ref1 : r1);
         in.sym.nrSyms = d->nrSyms;
         for (int i = 0; i < T3NS_MAX_SYMMETRIES; ++i) in.sym.sgs[i] = d->sgs[i];
         in.update_case = d->update_case;
@@ -803,10 +824,12 @@ int t3ns_update_branching(const t3ns_bupdate_desc* d, const t3ns_rops_desc* ops,
         ROperators o0, o1;
         SiteTensor T;
         BUpdateInput in;
-        bupdate_input(d, ops, o0, o1, T, in);
+        ResidentRef ref0, ref1;
+        bupdate_input(d, ops, o0, o1, T, in, &ref0, &ref1);
         const BUpdatePlan plan = plan_bupdate(in);
         auto res = std::make_unique<t3ns_rops>();
-        res->r = run_bupdate(plan, T, o0, o1, hipStreamPerThread, &res->work);
+        res->r = run_bupdate(plan, T, o0, o1, hipStreamPerThread, &res->work, ref0.ptr(), ref1.ptr(),
+                             &res->dev);
         res->views();
         *out = res.release();
     });

@@ -7,6 +7,7 @@
 #include <stdexcept>
 
 #include "heff_dev.hpp"
+#include "resident_ops.hpp"
 #include "../host/host_threads.hpp"
 
 namespace t3 {
@@ -1559,6 +1560,11 @@ HeffDevice::HeffDevice(const HeffInput& in, int precision, hipStream_t stream,
         stats_.op_elems[k] = op.total_size();
         if ((int64_t)op.tel.size() == op.total_size()) {
             ops64_[k].upload(op.tel, stream);
+        } else if (const ResidentRef ref = resident_find(op)) {  // made on the device
+            ops64_[k].alloc(op.total_size());
+            T3_HIP_CHECK(hipMemcpyAsync(ops64_[k].get(), ref.ptr(), op.total_size() * sizeof(double),
+                                        hipMemcpyDeviceToDevice, stream));
+            T3_HIP_CHECK(hipStreamSynchronize(stream));  // before `ref` lets the copy go
         } else if ((int)op.tel_src.size() == op.nrops) {  // straight from the caller's arrays
             ops64_[k].alloc(op.total_size());
             // Small operators are gathered into host chunks (one copy per

@@ -66,7 +66,7 @@ int grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(8192, n
 // the result copied to the host.
 static ROperators finish_sum(const ROperators& nr, const std::vector<AppendGroup>& groups,
                              const std::vector<AppendTask>& tasks, const double* d_src, hipStream_t s,
-                             float* kernel_ms = nullptr) {
+                             float* kernel_ms = nullptr, DevArena* keep_dev = nullptr) {
     const int64_t nnew = nr.total_size();
     // flat block index of every (op, block)
     std::vector<int> flat_of_op(nr.nrops + 1, 0);
@@ -122,26 +122,32 @@ static ROperators finish_sum(const ROperators& nr, const std::vector<AppendGroup
     out.tel.assign(ncomp, 0.0);
     if (ncomp) {
         DevBuf<int64_t> cs, cd, cn;
-        DevBuf<double> dcomp(ncomp);
+        auto dcomp = std::make_shared<DevBuf<double>>(ncomp);
         cs.upload(cpt.src, s);
         cd.upload(cpt.dst, s);
         cn.upload(cpt.n, s);
         hipLaunchKernelGGL(gather_blocks_kernel, dim3(grid((int64_t)cpt.n.size())), dim3(kThreads), 0,
-                           s, cs.get(), cd.get(), cn.get(), (int)cpt.n.size(), dnew.get(), dcomp.get());
-        T3_HIP_CHECK(hipMemcpyAsync(out.tel.data(), dcomp.get(), ncomp * sizeof(double),
+                           s, cs.get(), cd.get(), cn.get(), (int)cpt.n.size(), dnew.get(),
+                           dcomp->get());
+        T3_HIP_CHECK(hipMemcpyAsync(out.tel.data(), dcomp->get(), ncomp * sizeof(double),
                                     hipMemcpyDeviceToHost, s));
         T3_HIP_CHECK(hipStreamSynchronize(s));
+        if (keep_dev) *keep_dev = std::move(dcomp);
     }
     T3_HIP_CHECK(hipGetLastError());
     return out;
 }
 
-ROperators run_append(const AppendPlan& plan, const ROperators& old, hipStream_t s, RenormWork* work) {
+ROperators run_append(const AppendPlan& plan, const ROperators& old, hipStream_t s, RenormWork* work,
+                      const double* dev_old, DevArena* keep) {
     DevBuf<double> dold;
-    dold.upload(old.tel, s);
-    if (old.tel.empty()) dold.alloc(1);
+    if (!dev_old) {
+        dold.upload(old.tel, s);
+        if (old.tel.empty()) dold.alloc(1);
+    }
     float ms = 0;
-    ROperators out = finish_sum(plan.nr, plan.groups, plan.tasks, dold.get(), s, &ms);
+    ROperators out = finish_sum(plan.nr, plan.groups, plan.tasks, dev_old ? dev_old : dold.get(), s,
+                                &ms, keep);
     if (work) {
         // each task reads its source block once; every new block is written once
         double flops = 0, rd = 0;
@@ -367,7 +373,7 @@ void launch_sums(const SumTile* d, int ndot, int n, const SumTask* tasks, const 
 }  // namespace
 
 ROperators run_pupdate(const PUpdatePlan& plan, const ROperators& pops, const SiteTensor& T,
-                       hipStream_t s, RenormWork* work) {
+                       hipStream_t s, RenormWork* work, const double* dev_pops, DevArena* keep_dev) {
     const ROperators& nr = plan.nr;
     const int64_t nT = T.size(), nold = pops.total_size(), nnew = nr.total_size();
     std::vector<int> flat_of_op(nr.nrops + 1, 0);
@@ -457,7 +463,9 @@ ROperators run_pupdate(const PUpdatePlan& plan, const ROperators& pops, const Si
     if (nT) T3_HIP_CHECK(hipMemcpyAsync(arena.get(), T.tel.data(), nT * sizeof(double),
                                         hipMemcpyHostToDevice, s));
     if (nold)
-        T3_HIP_CHECK(hipMemcpyAsync(arena.get() + oldoff, pops.tel.data(), nold * sizeof(double),
+        T3_HIP_CHECK(hipMemcpyAsync(arena.get() + oldoff, dev_pops ? dev_pops : pops.tel.data(),
+                                    nold * sizeof(double),
+                                    dev_pops ? hipMemcpyDeviceToDevice :
                                     hipMemcpyHostToDevice, s));
     DevBuf<GemmTile> dfirst;
     DevBuf<SumTile> dsums;
@@ -506,15 +514,17 @@ ROperators run_pupdate(const PUpdatePlan& plan, const ROperators& pops, const Si
     out.tel.assign(ncomp, 0.0);
     if (ncomp) {
         DevBuf<int64_t> cs, cd, cn;
-        DevBuf<double> dcomp(ncomp);
+        auto dcomp = std::make_shared<DevBuf<double>>(ncomp);
         cs.upload(cpt.src, s);
         cd.upload(cpt.dst, s);
         cn.upload(cpt.n, s);
         hipLaunchKernelGGL(gather_blocks_kernel, dim3(grid((int64_t)cpt.n.size())), dim3(kThreads), 0,
-                           s, cs.get(), cd.get(), cn.get(), (int)cpt.n.size(), dnew.get(), dcomp.get());
-        T3_HIP_CHECK(hipMemcpyAsync(out.tel.data(), dcomp.get(), ncomp * sizeof(double),
+                           s, cs.get(), cd.get(), cn.get(), (int)cpt.n.size(), dnew.get(),
+                           dcomp->get());
+        T3_HIP_CHECK(hipMemcpyAsync(out.tel.data(), dcomp->get(), ncomp * sizeof(double),
                                     hipMemcpyDeviceToHost, s));
         T3_HIP_CHECK(hipStreamSynchronize(s));
+        if (keep_dev) *keep_dev = std::move(dcomp);
     }
     T3_HIP_CHECK(hipGetLastError());
     return out;
@@ -613,7 +623,8 @@ __global__ void __launch_bounds__(256)
 }  // namespace
 
 ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROperators& ops0,
-                       const ROperators& ops1, hipStream_t s, RenormWork* work) {
+                       const ROperators& ops1, hipStream_t s, RenormWork* work, const double* dev0,
+                       const double* dev1, DevArena* keep) {
     const int nl = plan.n_leg, xl = plan.x_leg, yl = plan.y_leg;
     const int64_t nT = T.size(), n0 = ops0.total_size(), n1 = ops1.total_size();
     const int64_t nU = plan.uniq.total_size();
@@ -781,10 +792,14 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
     DevBuf<double> arena(std::max<int64_t>(1, w_base + wmax));
     if (nT) T3_HIP_CHECK(hipMemcpyAsync(arena.get(), T.tel.data(), nT * sizeof(double),
                                         hipMemcpyHostToDevice, s));
-    if (n0) T3_HIP_CHECK(hipMemcpyAsync(arena.get() + o0_base, ops0.tel.data(), n0 * sizeof(double),
-                                        hipMemcpyHostToDevice, s));
-    if (n1) T3_HIP_CHECK(hipMemcpyAsync(arena.get() + o1_base, ops1.tel.data(), n1 * sizeof(double),
-                                        hipMemcpyHostToDevice, s));
+    if (n0)
+        T3_HIP_CHECK(hipMemcpyAsync(arena.get() + o0_base, dev0 ? dev0 : ops0.tel.data(),
+                                    n0 * sizeof(double),
+                                    dev0 ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    if (n1)
+        T3_HIP_CHECK(hipMemcpyAsync(arena.get() + o1_base, dev1 ? dev1 : ops1.tel.data(),
+                                    n1 * sizeof(double),
+                                    dev1 ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(fill_zero<double>, dim3(grid((nU + 255) / 256)), dim3(256), 0, s,
                        arena.get() + u_base, nU);
     DevBuf<unsigned int> dkeep(1);
@@ -831,7 +846,7 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
         work->bytes = 8.0 * (double)(nT + n0 + n1 + plan.nr.total_size());
         work->kernel_ms = ms;
     }
-    return finish_sum(plan.nr, plan.groups, plan.stasks, arena.get() + u_base, s);
+    return finish_sum(plan.nr, plan.groups, plan.stasks, arena.get() + u_base, s, nullptr, keep);
 }
 
 }  // namespace t3
