@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -109,6 +110,45 @@ ROperators rops_from(const t3ns_rops_desc& d, bool copy = true) {
         if (n) std::memcpy(r.tel.data() + r.op_offset[o], d.tel[o] + d.beginblock[o][0], n * sizeof(double));
     }
     return r;
+}
+
+// Streams of destroyed Heff objects and finished solves, kept per device
+// for the next owner: a sweep makes one Heff per step, and creating a
+// stream costs about as much host time as a step's small kernel launches.
+// A stream enters the pool idle (synchronised), so its next owner's work
+// is ordered after nothing.
+std::mutex g_stream_mu;
+std::vector<std::pair<int, hipStream_t>> g_streams;
+constexpr size_t kStreamPool = 32;
+
+hipStream_t stream_acquire() {
+    int dev = 0;
+    T3_HIP_CHECK(hipGetDevice(&dev));
+    {
+        std::lock_guard<std::mutex> lk(g_stream_mu);
+        for (size_t i = g_streams.size(); i-- > 0;)
+            if (g_streams[i].first == dev) {
+                hipStream_t s = g_streams[i].second;
+                g_streams.erase(g_streams.begin() + i);
+                return s;
+            }
+    }
+    hipStream_t s = nullptr;
+    T3_HIP_CHECK(hipStreamCreate(&s));
+    return s;
+}
+
+void stream_release(hipStream_t s) {
+    if (!s) return;
+    hipDevice_t dev = 0;
+    if (hipStreamSynchronize(s) == hipSuccess && hipStreamGetDevice(s, &dev) == hipSuccess) {
+        std::lock_guard<std::mutex> lk(g_stream_mu);
+        if (g_streams.size() < kStreamPool) {
+            g_streams.emplace_back((int)dev, s);
+            return;
+        }
+    }
+    (void)hipStreamDestroy(s);
 }
 }  // namespace
 
@@ -223,7 +263,7 @@ int t3ns_heff_init(t3ns_heff** out, const t3ns_heff_desc* d, const t3ns_rops_des
         auto h = std::make_unique<t3ns_heff>();
         convert_heff_input(&h->host, d, ops, site);
         HeffInput& in = h->host.in;
-        T3_HIP_CHECK(hipStreamCreate(&h->stream));
+        h->stream = stream_acquire();
         h->dev = std::make_unique<HeffDevice>(in, precision, h->stream);
         h->dvec.alloc(h->dev->size());
         h->dres.alloc(h->dev->size());
@@ -238,7 +278,7 @@ int t3ns_heff_derive(t3ns_heff** out, const t3ns_heff* base, int precision) {
     return guarded([&] {
         auto h = std::make_unique<t3ns_heff>();
         h->derived = true;
-        T3_HIP_CHECK(hipStreamCreate(&h->stream));
+        h->stream = stream_acquire();
         T3_HIP_CHECK(hipStreamSynchronize(base->stream));
         h->dev = std::make_unique<HeffDevice>(*base->dev, base->host.in, precision, h->stream);
         h->dvec.alloc(h->dev->size());
@@ -580,11 +620,10 @@ int t3ns_davidson_cb(double* result, double* energy, int size, int max_vecs, int
     int status = 0;
     const int rc = guarded([&] {
         const int64_t n = size;
-        hipStream_t s = nullptr;
-        T3_HIP_CHECK(hipStreamCreate(&s));
+        hipStream_t s = stream_acquire();
         struct StreamGuard {
             hipStream_t s;
-            ~StreamGuard() { (void)hipStreamDestroy(s); }
+            ~StreamGuard() { stream_release(s); }
         } guard{s};
         DevBuf<double> x(n), d(n);
         T3_HIP_CHECK(hipMemcpyAsync(x.get(), result, n * sizeof(double), hipMemcpyHostToDevice, s));
@@ -1077,7 +1116,7 @@ int t3ns_heff_destroy(t3ns_heff* h) {
     return guarded([&] {
         hipStream_t s = h->stream;
         delete h;
-        if (s) (void)hipStreamDestroy(s);
+        stream_release(s);
     });
 }
 

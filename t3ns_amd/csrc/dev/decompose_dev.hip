@@ -91,6 +91,100 @@ __global__ void __launch_bounds__(kJThreads)
     }
 }
 
+// Small matrices (at most kJSmallCols columns and kJSmallElems elements)
+// run every round of every sweep in one workgroup: the pairs of a round
+// touch disjoint columns, so each wave rotates one pair at a time (its dot
+// products reduced across the wave), a barrier separates the rounds, and
+// the workgroup stops after a sweep that rotated nothing (the rotations are
+// those of the one-launch-per-round schedule: a matrix that has converged
+// rotates nothing in later sweeps).  Larger matrices keep one launch per
+// round, where every pair of a round has a workgroup of its own.
+constexpr int kJSmallThreads = 512;
+constexpr int kJSmallCols = 256;
+constexpr int64_t kJSmallElems = 1 << 16;
+
+__global__ void __launch_bounds__(kJSmallThreads)
+    jacobi_small_kernel(const JacobiMat* __restrict__ mats, const int* __restrict__ list, int nlist,
+                        int max_sweeps, double tol, double* arena, int* sweeps_out) {
+    __shared__ int rotated[2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
+        const JacobiMat M = mats[list[li]];
+        const int np = M.n + (M.n & 1);
+        int sweep = 0;
+        for (; sweep < max_sweeps; ++sweep) {
+            // two flags: the next sweep clears the other one while every
+            // thread may still be reading this one
+            if (threadIdx.x == 0) rotated[sweep & 1] = 0;
+            __syncthreads();
+            for (int r = 0; r < np - 1; ++r) {
+                for (int k = wave; k < np / 2; k += nw) {
+                    int p, q;
+                    round_pair(np, r, k, &p, &q);
+                    if (p >= M.n || q >= M.n) continue;
+                    if (p > q) {
+                        const int t = p;
+                        p = q;
+                        q = t;
+                    }
+                    double* bp = arena + M.b + (int64_t)p * M.m;
+                    double* bq = arena + M.b + (int64_t)q * M.m;
+                    // the sums of jacobi_round_kernel in its order: lane l
+                    // runs the chains of its threads l + 64 j (rows l + 64 j
+                    // + kJThreads s), then the same pairwise tree
+                    double a4[4] = {0, 0, 0, 0}, b4[4] = {0, 0, 0, 0}, g4[4] = {0, 0, 0, 0};
+                    for (int i0 = 0; i0 < M.m; i0 += kJThreads)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int i = i0 + lane + 64 * j;
+                            if (i < M.m) {
+                                const double x = bp[i], y = bq[i];
+                                a4[j] = fma(x, x, a4[j]);
+                                b4[j] = fma(y, y, b4[j]);
+                                g4[j] = fma(x, y, g4[j]);
+                            }
+                        }
+                    double a = (a4[0] + a4[2]) + (a4[1] + a4[3]);
+                    double b = (b4[0] + b4[2]) + (b4[1] + b4[3]);
+                    double g = (g4[0] + g4[2]) + (g4[1] + g4[3]);
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) {
+                        a += __shfl_down(a, o);
+                        b += __shfl_down(b, o);
+                        g += __shfl_down(g, o);
+                    }
+                    a = __shfl(a, 0);
+                    b = __shfl(b, 0);
+                    g = __shfl(g, 0);
+                    // the rotation test and angle of jacobi_round_kernel
+                    if (g != 0 && fabs(g) > tol * sqrt((double)M.m) * sqrt(a * b)) {
+                        const double zeta = (b - a) / (2 * g);
+                        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+                        const double c = 1 / sqrt(1 + t * t), s = c * t;
+                        for (int i = lane; i < M.m; i += 64) {
+                            const double x = bp[i], y = bq[i];
+                            bp[i] = c * x - s * y;
+                            bq[i] = s * x + c * y;
+                        }
+                        double* vp = arena + M.v + (int64_t)p * M.n;
+                        double* vq = arena + M.v + (int64_t)q * M.n;
+                        for (int i = lane; i < M.n; i += 64) {
+                            const double x = vp[i], y = vq[i];
+                            vp[i] = c * x - s * y;
+                            vq[i] = s * x + c * y;
+                        }
+                        if (lane == 0) rotated[sweep & 1] = 1;
+                    }
+                }
+                __syncthreads();
+            }
+            if (!rotated[sweep & 1]) break;
+        }
+        if (threadIdx.x == 0) atomicMax(sweeps_out, sweep + 1);
+        __syncthreads();
+    }
+}
+
 __global__ void identity_kernel(const JacobiMat* __restrict__ mats, int nm, double* arena) {
     for (int m = blockIdx.x; m < nm; m += gridDim.x) {
         const JacobiMat M = mats[m];
@@ -213,31 +307,52 @@ int jacobi_svd(double* arena, const std::vector<JacobiMat>& mats, hipStream_t s,
     DevBuf<JacobiMat> dm;
     dm.upload(mats, s);
     std::vector<JItem> items;
+    std::vector<int> small;
     int maxround = 0;
     for (size_t m = 0; m < mats.size(); ++m) {
         if (mats[m].m < mats[m].n) throw std::invalid_argument("jacobi: rows < columns");
+        if (mats[m].n <= kJSmallCols && (int64_t)mats[m].m * mats[m].n <= kJSmallElems) {
+            small.push_back((int)m);
+            continue;
+        }
         const int np = mats[m].n + (mats[m].n & 1);
         for (int k = 0; k < np / 2; ++k) items.push_back(JItem{(int)m, k});
         maxround = std::max(maxround, np - 1);
     }
+    DevBuf<int> dsmall;
+    dsmall.upload(small, s);
     DevBuf<JItem> di;
     di.upload(items, s);
-    DevBuf<int> flag(1);
+    DevBuf<int> flag(2);  // rotation flag of the per-round schedule, sweeps of the small matrices
+    hipLaunchKernelGGL(zero_kernel<int>, dim3(1), dim3(64), 0, s, flag.get(), (int64_t)2);
     hipLaunchKernelGGL(identity_kernel, dim3(gridn((int64_t)mats.size())), dim3(256), 0, s, dm.get(),
                        (int)mats.size(), arena);
+    if (!small.empty())
+        hipLaunchKernelGGL(jacobi_small_kernel, dim3(gridn((int64_t)small.size())), dim3(kJSmallThreads), 0,
+                           s, dm.get(), dsmall.get(), (int)small.size(), max_sweeps, tol, arena,
+                           flag.get() + 1);
     int sweep = 0;
-    for (; sweep < max_sweeps; ++sweep) {
-        hipLaunchKernelGGL(zero_kernel<int>, dim3(1), dim3(64), 0, s, flag.get(), (int64_t)1);
-        for (int r = 0; r < maxround; ++r)
-            hipLaunchKernelGGL(jacobi_round_kernel, dim3(gridn((int64_t)items.size())), dim3(kJThreads), 0,
-                               s, dm.get(), di.get(), (int)items.size(), r, tol, arena, flag.get());
+    if (!items.empty()) {
+        for (; sweep < max_sweeps; ++sweep) {
+            hipLaunchKernelGGL(zero_kernel<int>, dim3(1), dim3(64), 0, s, flag.get(), (int64_t)1);
+            for (int r = 0; r < maxround; ++r)
+                hipLaunchKernelGGL(jacobi_round_kernel, dim3(gridn((int64_t)items.size())), dim3(kJThreads),
+                                   0, s, dm.get(), di.get(), (int)items.size(), r, tol, arena, flag.get());
+            int h = 0;
+            T3_HIP_CHECK(hipMemcpyAsync(&h, flag.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+            T3_HIP_CHECK(hipStreamSynchronize(s));
+            if (!h) break;
+        }
+    }
+    int sweeps = items.empty() ? 0 : sweep + 1;
+    if (!small.empty()) {
         int h = 0;
-        T3_HIP_CHECK(hipMemcpyAsync(&h, flag.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        T3_HIP_CHECK(hipMemcpyAsync(&h, flag.get() + 1, sizeof(int), hipMemcpyDeviceToHost, s));
         T3_HIP_CHECK(hipStreamSynchronize(s));
-        if (!h) break;
+        sweeps = std::max(sweeps, h);
     }
     T3_HIP_CHECK(hipGetLastError());
-    return sweep + 1;
+    return sweeps;
 }
 
 SplitResult run_split(const SplitInput& in, const SplitPlan& P, const std::vector<double>& A,
