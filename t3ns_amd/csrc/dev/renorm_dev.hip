@@ -637,7 +637,8 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
         std::vector<Swap12> swaps;  // new operators on leg 1: K made contiguous
         std::vector<SumTile> sums;
         std::vector<SumTask> tasks;
-        int ndot = 0;  // small-output tiles, first in sums
+        int ndot = 0;      // small-output tiles, first in sums
+        int64_t wtop = 0;  // workspace used
     };
     // Tile lists are built on host threads over ranges of destinations of
     // about equal task counts, each range cut into its own chunks (a chunk
@@ -775,18 +776,70 @@ ROperators run_bupdate(const BUpdatePlan& plan, const SiteTensor& T, const ROper
                     }
             }
             wmax = std::max(wmax, wtop);
-            c.ndot = partition_sums(c.sums);
+            c.wtop = wtop;
             chunks.push_back(std::move(c));
         }
     };
     run_blocks_threaded(nranges, build_range, (double)plan.tasks.size(), 1e5);
-    std::vector<Chunk> chunks;
     double flops = 0;
     int64_t wmax = 0;
     for (int r = 0; r < nranges; ++r) {
-        for (Chunk& c : rchunks[r]) chunks.push_back(std::move(c));
         flops += rflops[r];
         wmax = std::max(wmax, rwmax[r]);
+    }
+    // Consecutive chunks run as one while their workspaces fit together in
+    // max(largest chunk's, 8 Mi values): the ranges exist for the host
+    // threads, and every device chunk costs its list uploads, launches and a
+    // synchronisation.  A chunk appended to another has its workspace
+    // offsets (those at or past w_base) moved past the other's workspace and
+    // its task indices past the other's tasks; every destination keeps its
+    // tasks in order, so the sums are unchanged.
+    const int64_t wcap = std::max<int64_t>(wmax, (int64_t)1 << 23);
+    std::vector<Chunk> chunks;
+    for (int r = 0; r < nranges; ++r)
+        for (Chunk& c : rchunks[r]) {
+            if (chunks.empty() || chunks.back().wtop + c.wtop > wcap) {
+                chunks.push_back(std::move(c));
+                continue;
+            }
+            Chunk& m = chunks.back();
+            const int64_t dw = m.wtop;
+            const int dt = (int)m.tasks.size();
+            auto rel = [&](int64_t& off) {
+                if (off >= w_base) off += dw;
+            };
+            for (GemmTile g : c.p1) {
+                rel(g.p.off);
+                rel(g.q.off);
+                rel(g.x_off);
+                m.p1.push_back(g);
+            }
+            for (GemmTile g : c.p2) {
+                rel(g.p.off);
+                rel(g.q.off);
+                rel(g.x_off);
+                m.p2.push_back(g);
+            }
+            for (Swap12 w : c.swaps) {
+                rel(w.src);
+                rel(w.dst);
+                m.swaps.push_back(w);
+            }
+            for (SumTask t : c.tasks) {
+                rel(t.r.off);
+                rel(t.s.off);
+                m.tasks.push_back(t);
+            }
+            for (SumTile st : c.sums) {
+                st.task_begin += dt;
+                st.task_end += dt;
+                m.sums.push_back(st);
+            }
+            m.wtop += c.wtop;
+        }
+    for (Chunk& c : chunks) {
+        wmax = std::max(wmax, c.wtop);
+        c.ndot = partition_sums(c.sums);
     }
 
     DevBuf<double> arena(std::max<int64_t>(1, w_base + wmax));
