@@ -172,9 +172,7 @@ struct t3ns_step {
     MultisiteInfo md;
     SiteTensor site;
     ROperators ops[3];
-    InstructionSet merge_raw;
-    std::vector<int> merge_instr;
-    std::vector<double> merge_pref;
+    const Context::RawMerge* merge = nullptr;  // owned by the context
     std::vector<int> hermitian;
     std::vector<unsigned char> singlet;
     SymsecsFlat mpo;
@@ -645,11 +643,7 @@ int t3ns_step_random(t3ns_step** out, t3ns_ctx* cx, const int* sites, int nsites
                     const int m[3] = {a, b2, d};
                     st->singlet[a + (size_t)b2 * H + (size_t)d * H * H] = c.ham->mpo_couples_to_singlet(m);
                 }
-        st->merge_raw = c.ham->raw_merge(c.netw, st->ops[0].bond, isdmrg);
-        for (const auto& ins : st->merge_raw.instr) {
-            for (int k = 0; k < 3; ++k) st->merge_instr.push_back(ins.instr[k]);
-            st->merge_pref.push_back(ins.pref);
-        }
+        st->merge = &c.fetch_raw_merge(st->ops[0].bond, isdmrg);
         if (c.fill_seeded)
             for (auto& v : T.tel) v = rng.normal();
         *out = st.release();
@@ -1001,10 +995,10 @@ int t3ns_step_heff_desc(t3ns_step* s, t3ns_heff_desc* d, t3ns_rops_desc* ops, t3
         for (int i = 0; i < T.nrsites; ++i)
             for (int k = 0; k < 3; ++k) d->symarr[i][k] = s->symarr[i][k].desc;
         for (int k = 0; k < 3; ++k) d->rops_on_site[k] = s->rops_on_site[k];
-        d->merge_step = s->merge_raw.step;
-        d->merge_nr = s->merge_raw.nr_instr();
-        d->merge_instr = s->merge_instr.data();
-        d->merge_pref = s->merge_pref.data();
+        d->merge_step = s->merge->set.step;
+        d->merge_nr = s->merge->set.nr_instr();
+        d->merge_instr = s->merge->instr.data();
+        d->merge_pref = s->merge->pref.data();
         for (int k = 0; k < 3; ++k) {
             const ROperators& r = s->ops[k];
             t3ns_rops_desc& o = ops[k];

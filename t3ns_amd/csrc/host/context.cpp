@@ -488,6 +488,21 @@ const InstructionSet& Context::fetch_merge(int bond, int isdmrg, const int* cons
     return merge_[key] = std::move(s);
 }
 
+const Context::RawMerge& Context::fetch_raw_merge(int bond, int isdmrg) {
+    auto key = std::make_pair(bond, isdmrg);
+    auto it = rawmerge_.find(key);
+    if (it != rawmerge_.end()) return it->second;
+    RawMerge m;
+    m.set = ham->raw_merge(netw, bond, isdmrg);
+    m.instr.reserve(3 * m.set.instr.size());
+    m.pref.reserve(m.set.instr.size());
+    for (const auto& ins : m.set.instr) {
+        for (int k = 0; k < 3; ++k) m.instr.push_back(ins.instr[k]);
+        m.pref.push_back(ins.pref);
+    }
+    return rawmerge_[key] = std::move(m);
+}
+
 std::unique_ptr<Context> make_context(Network netw, std::unique_ptr<Model> ham,
                                       const std::vector<int>& sgs_in,
                                       const std::vector<int>& ts_in) {

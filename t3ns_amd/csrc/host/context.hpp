@@ -71,12 +71,22 @@ public:
     const InstructionSet& fetch_pUpdate(int bond, int is_left);
     const InstructionSet& fetch_bUpdate(int bond, int is_left);
     const InstructionSet& fetch_merge(int bond, int isdmrg, const int* const* hss_ops);
+    // Unsorted merge instructions of a bond with their flat (operator
+    // triple, prefactor) arrays, cached per (bond, isdmrg): they depend on
+    // the network and the Hamiltonian only.
+    struct RawMerge {
+        InstructionSet set;
+        std::vector<int> instr;
+        std::vector<double> pref;
+    };
+    const RawMerge& fetch_raw_merge(int bond, int isdmrg);
 
     bool bond_is_internal_dims1(int bond) const;
     int full_dimension(const Symsecs& s) const;
 
 private:
     std::map<std::pair<int, int>, InstructionSet> pupd_, bupd_, merge_;
+    std::map<std::pair<int, int>, RawMerge> rawmerge_;
     void make_symsec(Symsecs* symsec, int bond, int is_left, char o);
     bool select_lowest(Symsecs& s1, const Symsecs& s2) const;
     void kick_impossibles(Symsecs& s) const;
