@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <numeric>
 #include <stdexcept>
 
@@ -306,12 +307,16 @@ int jacobi_svd(double* arena, const std::vector<JacobiMat>& mats, hipStream_t s,
     if (mats.empty()) return 0;
     DevBuf<JacobiMat> dm;
     dm.upload(mats, s);
+    // T3NS_JACOBI_SMALL_COLS lowers the small path's column limit (0: every
+    // matrix takes the per-round path; both give the same bits)
+    int small_cols = kJSmallCols;
+    if (const char* e = std::getenv("T3NS_JACOBI_SMALL_COLS")) small_cols = std::min(kJSmallCols, std::atoi(e));
     std::vector<JItem> items;
     std::vector<int> small;
     int maxround = 0;
     for (size_t m = 0; m < mats.size(); ++m) {
         if (mats[m].m < mats[m].n) throw std::invalid_argument("jacobi: rows < columns");
-        if (mats[m].n <= kJSmallCols && (int64_t)mats[m].m * mats[m].n <= kJSmallElems) {
+        if (mats[m].n <= small_cols && (int64_t)mats[m].m * mats[m].n <= kJSmallElems) {
             small.push_back((int)m);
             continue;
         }

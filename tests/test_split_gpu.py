@@ -116,3 +116,25 @@ def test_split_null_singular_values_match_oracle(gpu, case):
         d = st.split_problem(sv, maxD=100000)
         keep_first_block(d)
         _compare(d)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [CASES[0], CASES[-1]], ids=_cid)
+def test_split_jacobi_paths_agree(gpu, case, monkeypatch):
+    """The one-workgroup Jacobi path for small sector matrices sums in the
+    per-round path's order: both give the same bits (the QC-24 D = 512
+    split has sectors over the small path's limit, so both paths also run
+    together there)."""
+    ctx, st = _step(case)
+    d0, o, s = st.descriptors()
+    for sv in splittable(st, s.nrsites):
+        print("case", _cid(case), "site", sv, flush=True)
+        d = st.split_problem(sv, maxD=case[-2])
+        monkeypatch.delenv("T3NS_JACOBI_SMALL_COLS", raising=False)
+        a, _ = T.split_site(d)
+        monkeypatch.setenv("T3NS_JACOBI_SMALL_COLS", "0")
+        b, _ = T.split_site(d)
+        monkeypatch.delenv("T3NS_JACOBI_SMALL_COLS")
+        assert a.keys() == b.keys()
+        for key in a:
+            assert np.array_equal(np.asarray(a[key]), np.asarray(b[key])), key
