@@ -397,6 +397,11 @@ __device__ __forceinline__ void blk_put(T* __restrict__ Z, const Geo& g, const R
 // all terms of a group are in flight together: a per-term walk over a short
 // K (legs of a few states) waits out one memory latency per term.
 constexpr int kTU = 4;
+// FP32 stage C raises the wave's issue priority over its matrix-core steps
+// (242 against 247 ms per matvec on the QC-24 D = 512 step); FP64 does not
+// (358 against 356).
+template <typename T>
+constexpr bool kMfmaPrio = sizeof(T) == 4;
 __device__ __forceinline__ int64_t uni64(int64_t v) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
@@ -433,6 +438,7 @@ __device__ __forceinline__ void row_mac_terms_n(const Terms<T, OT>& R, const Geo
 #pragma unroll
             for (int j = 0; j < NTN; ++j) bv[u][j] = cvt(R.o[u][bo + coff[j]]);
         }
+        if constexpr (kMfmaPrio<T>) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int u = 0; u < kTU; ++u) {
             if (u < nu) {
@@ -442,6 +448,7 @@ __device__ __forceinline__ void row_mac_terms_n(const Terms<T, OT>& R, const Geo
                 for (int j = 0; j < NTN; ++j) acc[j] = Mfma<T>::run(a, bv[u][j], acc[j]);
             }
         }
+        if constexpr (kMfmaPrio<T>) __builtin_amdgcn_s_setprio(0);
     }
 }
 
@@ -552,6 +559,7 @@ __device__ __forceinline__ void row_mac_fused_n(const FTerms<T, OT>& R, const Ge
             }
             av[u] = a;
         }
+        if constexpr (kMfmaPrio<T>) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int u = 0; u < kFTU; ++u) {
             if (u < nu) {
@@ -560,6 +568,7 @@ __device__ __forceinline__ void row_mac_fused_n(const FTerms<T, OT>& R, const Ge
                 for (int j = 0; j < NTN; ++j) acc[j] = Mfma<T>::run(a, bv[u][j], acc[j]);
             }
         }
+        if constexpr (kMfmaPrio<T>) __builtin_amdgcn_s_setprio(0);
     }
 }
 
