@@ -61,6 +61,12 @@ struct Mfma<float> {
     static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
+// FP32 raises the wave's issue priority over its runs of matrix-core steps
+// (stage C: 242 against 247 ms per matvec on the QC-24 D = 512 step);
+// FP64 does not (358 against 356).
+template <typename T>
+constexpr bool kMfmaPrio = sizeof(T) == 4;
+
 // Operator values as the arithmetic type: FP64 and FP32 arenas as they
 // are, BF16 arenas (the upper 16 bits of an FP32 value, per-block scaled)
 // widened to FP32.
@@ -165,6 +171,7 @@ __device__ __forceinline__ void row_slab(const T* sp, const OT* op, const int (&
 #pragma unroll
         for (int j = 0; j < NTN; ++j) bv[u][j] = cvt(op[uu * ost + coff[j]]);
     }
+    if constexpr (kMfmaPrio<T>) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
         if (u < ns) {
@@ -174,6 +181,7 @@ __device__ __forceinline__ void row_slab(const T* sp, const OT* op, const int (&
             for (int j = 0; j < NTN; ++j) acc[j] = Mfma<T>::run(a, bv[u][j], acc[j]);
         }
     }
+    if constexpr (kMfmaPrio<T>) __builtin_amdgcn_s_setprio(0);
 }
 
 // acc[j] += (scale S)[row tile, :] O[column tile tn0 + j, :]^T over the whole K,
@@ -306,6 +314,7 @@ __device__ __forceinline__ void blk_slab(const T* S, const OT* op, const RowBlk&
 #pragma unroll
         for (int r = 0; r < NTF; ++r) av[u][r] = S[B.soff[r] + so + uu * sst];
     }
+    if constexpr (kMfmaPrio<T>) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
         if (u < ns) {
@@ -317,6 +326,7 @@ __device__ __forceinline__ void blk_slab(const T* S, const OT* op, const RowBlk&
             }
         }
     }
+    if constexpr (kMfmaPrio<T>) __builtin_amdgcn_s_setprio(0);
 }
 
 // acc[r] += (scale S)[row tile tf0 + r, :] O[column tile, :]^T, r < NTF.
@@ -397,11 +407,6 @@ __device__ __forceinline__ void blk_put(T* __restrict__ Z, const Geo& g, const R
 // all terms of a group are in flight together: a per-term walk over a short
 // K (legs of a few states) waits out one memory latency per term.
 constexpr int kTU = 4;
-// FP32 stage C raises the wave's issue priority over its matrix-core steps
-// (242 against 247 ms per matvec on the QC-24 D = 512 step); FP64 does not
-// (358 against 356).
-template <typename T>
-constexpr bool kMfmaPrio = sizeof(T) == 4;
 __device__ __forceinline__ int64_t uni64(int64_t v) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
@@ -602,6 +607,7 @@ __device__ __forceinline__ void blk_mac_terms_n(const Terms<T, OT>& R, const Geo
 #pragma unroll
             for (int r = 0; r < NTF; ++r) av[u][r] = R.s[u][B.soff[r] + so];
         }
+        if constexpr (kMfmaPrio<T>) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int u = 0; u < kTU; ++u) {
             if (u < nu) {
@@ -613,6 +619,7 @@ __device__ __forceinline__ void blk_mac_terms_n(const Terms<T, OT>& R, const Geo
                 }
             }
         }
+        if constexpr (kMfmaPrio<T>) __builtin_amdgcn_s_setprio(0);
     }
 }
 
